@@ -1,0 +1,129 @@
+// h8_device.h -- device-side building blocks shared by the h8pcg kernels (gfx950).
+//
+// Structured H8 grid (box mesh): node(i,j,k) = k*NI*NJ + i*NJ + j with
+// NJ = ny+1 (y fastest), NI = nx+1, NK = nz+1; element e(i,j,k) = k*nx*ny +
+// i*ny + j; element corner a sits at node offset (OX[a], OY[a], OZ[a]) from the
+// element origin, DOF = 3*node + c.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace h8 {
+
+// Corner offsets of the Hex8 natural ordering (bottom face CCW, then top).
+__host__ __device__ constexpr int OX(int a) { return (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : 0; }
+__host__ __device__ constexpr int OY(int a) { return (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : 0; }
+__host__ __device__ constexpr int OZ(int a) { return a >= 4 ? 1 : 0; }
+
+struct GridDev {
+    int nx, ny, nz;        // elements
+    int NJ, NI, NK;        // nodes per axis
+    long long nnodes;      // NJ*NI*NK
+    long long ndof;        // 3*nnodes
+    int tj, ti, tk;        // tiles per axis (node tiles)
+    int ntiles;            // tj*ti*tk
+};
+
+// Solver status held in device memory.
+enum : int { ST_RUNNING = 0, ST_DONE = 1, ST_NOT_SPD = 2, ST_MAXITER = 3, ST_REPLACE = 4 };
+
+struct PcgState {
+    double rz_old;     // r.z of the previous iteration
+    double alpha, beta;
+    double pq;         // p.Ap of the current iteration
+    double bnorm;      // ||b|| (1 if 0) -- for the refined solve, ||f||
+    double tol;
+    double rel;        // recurrence ||r||/bnorm
+    double rmax;       // refined: ||r64|| at the last replacement
+    double delta;      // refined: replacement threshold
+    double true_rel;   // refined: FP64 ||f-Ku||/||f|| at the last replacement
+    double bad_pq;     // p.Ap when status = NOT_SPD
+    long long k;       // current iteration (1-based)
+    long long maxiter;
+    long long iters;   // iteration count reported at exit
+    long long outer;   // refined: replacements done
+    int status;
+    int first;         // next direction kernel uses p = z
+    unsigned cnt[4];   // last-block counters (reset by the last block)
+};
+
+// ---------------------------------------------------------------------------
+// Reductions: wave shuffle tree + ordered LDS sum, then "last block" finish.
+// Both stages use a fixed order, so results are bitwise reproducible.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* sh /* >= 16*NV */) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[wid * NV + i] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += sh[w * NV + i];
+        v[i] = s;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void st_relaxed(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_relaxed(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every block contributes v[NV]; returns true in the last block to finish, in
+// which `tot` holds the sum over blocks (ordered by block index).
+template <int NV>
+__device__ bool reduce_last_block(double (&v)[NV], double* partials, unsigned* counter,
+                                  double (&tot)[NV], double* sh) {
+    __shared__ int s_last;
+    block_sum<NV>(v, sh);
+    const unsigned nb = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) st_relaxed(&partials[(size_t)blockIdx.x * NV + i], v[i]);
+        __threadfence();
+        unsigned t = atomicAdd(counter, 1u);
+        s_last = (t == nb - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += ld_relaxed(&partials[(size_t)b * NV + i]);
+    }
+    block_sum<NV>(acc, sh);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) tot[i] = acc[i];
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+__device__ __forceinline__ void tile_origin(const GridDev& g, int tile, int TJ, int TI, int TK,
+                                            int& j0, int& i0, int& k0) {
+    const int bj = tile % g.tj;
+    const int rest = tile / g.tj;
+    const int bi = rest % g.ti;
+    const int bk = rest / g.ti;
+    j0 = bj * TJ; i0 = bi * TI; k0 = bk * TK;
+}
+
+}  // namespace h8

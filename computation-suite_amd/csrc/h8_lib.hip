@@ -1,0 +1,666 @@
+// h8_lib.hip -- C ABI of libh8pcg.so: launches, workspace carving, the PCG
+// driver (hipGraph-captured iteration chunks, device-side convergence flag,
+// one host poll per chunk) and the mixed-precision refined solve.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/h8pcg.h"
+#include "h8_device.h"
+#include "h8_node.h"
+#include "h8_stream.h"
+
+using namespace h8;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(H8_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+#define LAUNCHCHK()                                                                    \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess)                                                          \
+            return fail(H8_ERR_HIP, "kernel launch failed: %s (%s:%d)",                \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                    \
+    } while (0)
+
+extern "C" const char* h8_last_error(void) { return g_err.c_str(); }
+extern "C" int h8_version(void) { return 1; }
+
+extern "C" int h8_device_info(char* buf, int buflen) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    snprintf(buf, buflen, "%s|%s|cu=%d|l2=%d|clock_khz=%d", p.name, p.gcnArchName,
+             p.multiProcessorCount, p.l2CacheSize, p.clockRate);
+    return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tiling
+// ---------------------------------------------------------------------------
+// Node brick owned by one block of the node-centric kernel: TJ (y, unit
+// stride) x TI (x) x TK (z) = 256 threads.
+constexpr int NTJ = 8, NTI = 8, NTK = 4;
+
+static int make_grid(const h8_grid* g, GridDev& d) {
+    if (!g) return fail(H8_ERR_ARG, "grid is NULL");
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1)
+        return fail(H8_ERR_ARG, "grid needs nx, ny, nz >= 1 (got %d, %d, %d)", g->nx, g->ny, g->nz);
+    d.nx = g->nx; d.ny = g->ny; d.nz = g->nz;
+    d.NJ = g->ny + 1; d.NI = g->nx + 1; d.NK = g->nz + 1;
+    d.nnodes = (long long)d.NJ * d.NI * d.NK;
+    d.ndof = 3 * d.nnodes;
+    d.tj = (d.NJ + NTJ - 1) / NTJ;
+    d.ti = (d.NI + NTI - 1) / NTI;
+    d.tk = (d.NK + NTK - 1) / NTK;
+    long long nt = (long long)d.tj * d.ti * d.tk;
+    if (nt > (1LL << 30)) return fail(H8_ERR_ARG, "grid too large");
+    d.ntiles = int(nt);
+    return H8_OK;
+}
+
+static int stream_blocks(long long n) {
+    long long b = (n + STREAM_THREADS * 4 - 1) / (STREAM_THREADS * 4);
+    return int(std::max(1LL, std::min(b, 2048LL)));
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// ---------------------------------------------------------------------------
+// element stiffness
+// ---------------------------------------------------------------------------
+extern "C" int h8_ke_batched_f64(const double* X, const double* D, int64_t d_stride, int64_t m,
+                                 double* ke, void* stream) {
+    if (m < 0) return fail(H8_ERR_ARG, "m < 0");
+    if (m == 0) return H8_OK;
+    if (!X || !D || !ke) return fail(H8_ERR_ARG, "NULL pointer");
+    if (d_stride != 0 && d_stride != 36) return fail(H8_ERR_ARG, "d_stride must be 0 or 36");
+    hipStream_t s = (hipStream_t)stream;
+    const long long blocks = (m + 3) / 4;
+    hipLaunchKernelGGL(k_ke, dim3((unsigned)blocks), dim3(256), 0, s, (long long)m, X, D,
+                       (long long)d_stride, ke);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// operator
+// ---------------------------------------------------------------------------
+template <typename T>
+static int launch_node_plain(const GridDev& d, const T* s_e, const T* K0, const T* u, T* y,
+                             hipStream_t s) {
+    NodeArgs<T> A{};
+    A.g = d; A.s_e = s_e; A.K0 = K0; A.u = u; A.y = y;
+    hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE_PLAIN>), dim3(d.ntiles), dim3(NTJ * NTI * NTK),
+                       0, s, A);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_spmv_f64(const h8_grid* g, const double* s_e, const double* K0,
+                           const double* u, double* y, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e || !K0 || !u || !y) return fail(H8_ERR_ARG, "NULL pointer");
+    return launch_node_plain<double>(d, s_e, K0, u, y, (hipStream_t)stream);
+}
+
+extern "C" int h8_spmv_f32(const h8_grid* g, const float* s_e, const float* K0,
+                           const float* u, float* y, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e || !K0 || !u || !y) return fail(H8_ERR_ARG, "NULL pointer");
+    return launch_node_plain<float>(d, s_e, K0, u, y, (hipStream_t)stream);
+}
+
+extern "C" int h8_diag_f64(const h8_grid* g, const double* s_e, const double* K0,
+                           const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e || !K0 || !free_mask || !dinv || !d_flag) return fail(H8_ERR_ARG, "NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, (int*)d_flag, 0);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_diag, dim3(stream_blocks(d.nnodes)), dim3(STREAM_THREADS), 0, s, d, s_e, K0,
+                       free_mask, dinv, (int*)d_flag);
+    LAUNCHCHK();
+    int32_t flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, d_flag, sizeof flag, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (flag) return fail(H8_ZERO_DIAG, "jacobi_preconditioner: A has a zero diagonal entry.");
+    return H8_OK;
+}
+
+extern "C" int h8_simp_scale_f64(const double* rho, double p, double rho_min, int64_t m,
+                                 double* s, double* ds, void* stream) {
+    if (m <= 0) return m == 0 ? H8_OK : fail(H8_ERR_ARG, "m < 0");
+    if (!rho || !s) return fail(H8_ERR_ARG, "NULL pointer");
+    hipLaunchKernelGGL(k_simp, dim3(stream_blocks(m)), dim3(STREAM_THREADS), 0, (hipStream_t)stream,
+                       (long long)m, rho, p, rho_min, s, ds);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_elem_bilinear_f64(const h8_grid* g, const double* K0, const double* u,
+                                    const double* v, const double* w, double* out, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!K0 || !u || !v || !w || !out) return fail(H8_ERR_ARG, "NULL pointer");
+    const long long m = (long long)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_elem_bilinear, dim3(stream_blocks(m)), dim3(STREAM_THREADS), 0,
+                       (hipStream_t)stream, d, K0, u, v, w, out);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_cast_f64_f32(const double* x, float* y, int64_t n, void* stream) {
+    if (n <= 0) return n == 0 ? H8_OK : fail(H8_ERR_ARG, "n < 0");
+    if (!x || !y) return fail(H8_ERR_ARG, "NULL pointer");
+    hipLaunchKernelGGL(k_cast_f64_f32, dim3(stream_blocks(n)), dim3(STREAM_THREADS), 0,
+                       (hipStream_t)stream, (long long)n, x, y);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_masked_sub_f64(const double* a, const double* b, const double* dinv, double* out,
+                                 int64_t n, void* stream) {
+    if (n <= 0) return n == 0 ? H8_OK : fail(H8_ERR_ARG, "n < 0");
+    if (!a || !dinv || !out) return fail(H8_ERR_ARG, "NULL pointer");
+    hipLaunchKernelGGL(k_masked_sub, dim3(stream_blocks(n)), dim3(STREAM_THREADS), 0,
+                       (hipStream_t)stream, (long long)n, a, b, dinv, out);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// workspace layout
+// ---------------------------------------------------------------------------
+struct Ws64 {
+    double *r, *p0, *p1, *q, *partials;
+    PcgState* st;
+};
+struct WsRef {
+    float *r, *x, *p0, *p1, *q, *dinv32;
+    double* partials;
+    PcgState* st;
+};
+
+static size_t partial_slots(const GridDev& d) {
+    return size_t(std::max<long long>(d.ntiles, 2048)) * 4;
+}
+
+static size_t ws_bytes_64(const GridDev& d) {
+    // r, p0, p1, q, partials, state, and one spare vector (the timing helper's x)
+    const size_t nv = align_up(sizeof(double) * d.ndof);
+    return 5 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
+}
+static size_t ws_bytes_ref(const GridDev& d) {
+    const size_t nv = align_up(sizeof(float) * d.ndof);
+    return 6 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
+}
+
+extern "C" size_t h8_pcg_workspace_bytes(const h8_grid* g, int precision) {
+    GridDev d;
+    if (make_grid(g, d)) return 0;
+    return precision == 32 ? ws_bytes_ref(d) : ws_bytes_64(d);
+}
+
+static char* carve(char*& p, size_t bytes) {
+    char* q = p;
+    p += align_up(bytes);
+    return q;
+}
+
+// ---------------------------------------------------------------------------
+// graph cache: one captured chunk of iterations per (kind, pointers, grid)
+// ---------------------------------------------------------------------------
+struct GraphKey {
+    int kind;
+    int dev;
+    long long nx, ny, nz;
+    const void* ptr[12];
+    int chunk;
+    bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
+};
+struct GraphEntry {
+    GraphKey key;
+    hipGraphExec_t exec;
+    unsigned long long last_use;
+};
+
+static std::mutex g_graph_mu;
+static std::vector<GraphEntry> g_graphs;
+static unsigned long long g_clock = 0;
+static hipStream_t g_capture_stream[64] = {};
+
+static int capture_stream(hipStream_t* out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(H8_ERR_HIP, "device index out of range");
+    if (!g_capture_stream[dev]) HIPCHK(hipStreamCreateWithFlags(&g_capture_stream[dev], hipStreamNonBlocking));
+    *out = g_capture_stream[dev];
+    return H8_OK;
+}
+
+template <typename Body>
+static int get_graph(const GraphKey& key, Body body, hipGraphExec_t* out) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    for (auto& e : g_graphs)
+        if (e.key == key) {
+            e.last_use = ++g_clock;
+            *out = e.exec;
+            return H8_OK;
+        }
+    hipStream_t cs;
+    int rc = capture_stream(&cs);
+    if (rc) return rc;
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    rc = body(cs);
+    hipGraph_t graph;
+    hipError_t ee = hipStreamEndCapture(cs, &graph);
+    if (rc) return rc;
+    if (ee != hipSuccess) return fail(H8_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+    hipGraphExec_t exec;
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(H8_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    if (g_graphs.size() >= 16) {
+        auto it = std::min_element(g_graphs.begin(), g_graphs.end(),
+                                   [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+        (void)hipGraphExecDestroy(it->exec);
+        g_graphs.erase(it);
+    }
+    g_graphs.push_back(GraphEntry{key, exec, ++g_clock});
+    *out = exec;
+    return H8_OK;
+}
+
+// One PCG iteration = direction+product kernel (node-centric) + update kernel.
+template <typename T, bool REFINED>
+static int launch_iteration(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r,
+                            T* p0, T* p1, T* q, PcgState* st, double* partials, hipStream_t s) {
+    NodeArgs<T> A{};
+    A.g = d; A.s_e = s_e; A.K0 = K0; A.r = r; A.dinv = dinv; A.p0 = p0; A.p1 = p1; A.q = q;
+    A.st = st; A.partials = partials;
+    hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE_DIR>), dim3(d.ntiles), dim3(NTJ * NTI * NTK), 0, s, A);
+    LAUNCHCHK();
+    hipLaunchKernelGGL((k_pcg_update<T, REFINED>), dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s,
+                       d.ndof, x, r, (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+// Iterations captured per graph: once the device flag leaves RUNNING the rest
+// of the chunk returns at kernel entry, so a chunk bounds the wasted launches.
+static int chunk_for(long long maxiter) {
+    return int(std::max(1LL, std::min<long long>(maxiter, 32)));
+}
+
+// Runs captured chunks until the device-side status leaves RUNNING.
+template <typename T, bool REFINED>
+static int run_chunks(const GraphKey& key, const GridDev& d, const T* s_e, const T* K0, const T* dinv,
+                      T* x, T* r, T* p0, T* p1, T* q, PcgState* st, double* partials, hipStream_t s,
+                      PcgState& hst, long long& launches) {
+    const int chunk = key.chunk;
+    hipGraphExec_t exec;
+    int rc = get_graph(key, [&](hipStream_t cs) -> int {
+        for (int it = 0; it < chunk; ++it) {
+            int r1 = launch_iteration<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, q, st, partials, cs);
+            if (r1) return r1;
+        }
+        return H8_OK;
+    }, &exec);
+    if (rc) return rc;
+    for (;;) {
+        HIPCHK(hipGraphLaunch(exec, s));
+        ++launches;
+        HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hst.status != ST_RUNNING) break;
+    }
+    return H8_OK;
+}
+
+static void fill_stats(h8_pcg_stats* stats, const PcgState& h, float ms, long long launches) {
+    if (!stats) return;
+    stats->iterations = h.iters;
+    stats->status = h.status == ST_DONE ? H8_OK : (h.status == ST_NOT_SPD ? H8_NOT_SPD : H8_MAXITER);
+    stats->outer = 0;
+    stats->rel_residual = h.rel;
+    stats->true_rel_residual = -1.0;
+    stats->bnorm = h.bnorm;
+    stats->pAp = h.bad_pq;
+    stats->elapsed_ms = ms;
+    stats->inner_iterations = 0;
+    stats->launches = launches;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Jacobi-PCG
+// ---------------------------------------------------------------------------
+extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
+                          const double* b, double* x, int use_x0, double tol, int64_t maxiter,
+                          void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e || !K0 || !dinv || !b || !x || !ws) return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < ws_bytes_64(d))
+        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, ws_bytes_64(d));
+    if (!(tol >= 0.0)) return fail(H8_ERR_ARG, "tol must be >= 0");
+    if (maxiter < 0) return fail(H8_ERR_ARG, "maxiter must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    Ws64 w;
+    w.r = (double*)carve(p, sizeof(double) * d.ndof);
+    w.p0 = (double*)carve(p, sizeof(double) * d.ndof);
+    w.p1 = (double*)carve(p, sizeof(double) * d.ndof);
+    w.q = (double*)carve(p, sizeof(double) * d.ndof);
+    w.partials = (double*)carve(p, sizeof(double) * partial_slots(d));
+    w.st = (PcgState*)carve(p, sizeof(PcgState));
+
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+
+    PcgState hst{};
+    hst.tol = tol;
+    hst.maxiter = maxiter;
+    hst.status = ST_RUNNING;
+    HIPCHK(hipMemcpyAsync(w.st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    const double* Ax0 = nullptr;
+    if (use_x0) {
+        rc = launch_node_plain<double>(d, s_e, K0, x, w.q, s);
+        if (rc) return rc;
+        Ax0 = w.q;
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s,
+                           d.ndof, x, 0.0);
+        LAUNCHCHK();
+    }
+    hipLaunchKernelGGL(k_pcg_init<double>, dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s,
+                       d.ndof, b, Ax0, dinv, w.r, w.st, w.partials);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    long long launches = 0;
+    if (hst.status == ST_RUNNING && maxiter > 0) {
+        GraphKey key{};
+        key.kind = 1;
+        HIPCHK(hipGetDevice(&key.dev));
+        key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
+        const void* ptrs[] = {s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
+        std::memcpy(key.ptr, ptrs, sizeof ptrs);
+        key.chunk = chunk_for(maxiter);
+        rc = run_chunks<double, false>(key, d, s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials, s,
+                                       hst, launches);
+        if (rc) return rc;
+    } else if (hst.status == ST_RUNNING) {
+        hst.status = ST_MAXITER;
+        hst.iters = 0;
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    fill_stats(stats, hst, ms, launches);
+    if (hst.status == ST_NOT_SPD)
+        return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
+                    hst.bad_pq, hst.iters);
+    if (hst.status == ST_MAXITER)
+        return fail(H8_MAXITER,
+                    "preconditioned_cg: did not converge in %lld iterations (final relative residual "
+                    "%.3e >= tol=%.1e).",
+                    (long long)maxiter, hst.rel, tol);
+    return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 PCG with FP64 reliable residual replacement
+// ---------------------------------------------------------------------------
+extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const double* K064,
+                               const float* s_e32, const float* K032, const double* dinv64,
+                               const double* f, double* u, double tol, int64_t maxiter, double delta,
+                               void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e64 || !K064 || !s_e32 || !K032 || !dinv64 || !f || !u || !ws)
+        return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < ws_bytes_ref(d))
+        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, ws_bytes_ref(d));
+    if (!(tol > 0.0)) return fail(H8_ERR_ARG, "tol must be > 0");
+    if (!(delta > 0.0 && delta < 1.0)) return fail(H8_ERR_ARG, "delta must be in (0, 1)");
+    if (maxiter < 0) return fail(H8_ERR_ARG, "maxiter must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    WsRef w;
+    w.r = (float*)carve(p, sizeof(float) * d.ndof);
+    w.x = (float*)carve(p, sizeof(float) * d.ndof);
+    w.p0 = (float*)carve(p, sizeof(float) * d.ndof);
+    w.p1 = (float*)carve(p, sizeof(float) * d.ndof);
+    w.q = (float*)carve(p, sizeof(float) * d.ndof);
+    w.dinv32 = (float*)carve(p, sizeof(float) * d.ndof);
+    w.partials = (double*)carve(p, sizeof(double) * partial_slots(d));
+    w.st = (PcgState*)carve(p, sizeof(PcgState));
+
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    const int sb = stream_blocks(d.ndof);
+    hipLaunchKernelGGL(k_cast_f64_f32, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, dinv64, w.dinv32);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, 0.0);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_fill<float>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, w.x, 0.0f);
+    LAUNCHCHK();
+
+    // ||f|| over the free DOFs and the early-out test: the init kernel on a
+    // zero guess, with p0|p1 (two adjacent float vectors) as FP64 scratch.
+    PcgState hst{};
+    hst.tol = tol;
+    hst.maxiter = maxiter;
+    hst.delta = delta;
+    hst.status = ST_RUNNING;
+    HIPCHK(hipMemcpyAsync(w.st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pcg_init<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, f,
+                       (const double*)nullptr, dinv64, (double*)w.p0, w.st, w.partials);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hst.status == ST_RUNNING && maxiter == 0) {
+        hst.status = ST_MAXITER;
+        hst.iters = 0;
+    }
+    NodeArgs<double> R{};
+    R.g = d; R.s_e = s_e64; R.K0 = K064; R.u = u; R.dinv = dinv64; R.f = f; R.r32 = w.r;
+    R.dinv32 = w.dinv32; R.st = w.st; R.partials = w.partials;
+    long long launches = 0;
+    long long resid_calls = 0;
+    hst.true_rel = hst.rel;
+    if (hst.status == ST_RUNNING) {
+        // first residual (u = 0): r32 = float(mask f), rz_old, rmax
+        hipLaunchKernelGGL((k_node<double, NTJ, NTI, NTK, MODE_RESID>), dim3(d.ntiles),
+                           dim3(NTJ * NTI * NTK), 0, s, R);
+        LAUNCHCHK();
+        ++resid_calls;
+        HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    GraphKey key{};
+    key.kind = 2;
+    HIPCHK(hipGetDevice(&key.dev));
+    key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
+    const void* ptrs[] = {s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
+    std::memcpy(key.ptr, ptrs, sizeof ptrs);
+    key.chunk = chunk_for(maxiter);
+    while (hst.status == ST_RUNNING) {
+        rc = run_chunks<float, true>(key, d, s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, w.st,
+                                     w.partials, s, hst, launches);
+        if (rc) return rc;
+        if (hst.status != ST_REPLACE) break;
+        // fold the FP32 iterate into u, then replace the residual in FP64
+        hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, w.x);
+        LAUNCHCHK();
+        hipLaunchKernelGGL((k_node<double, NTJ, NTI, NTK, MODE_RESID>), dim3(d.ntiles),
+                           dim3(NTJ * NTI * NTK), 0, s, R);
+        LAUNCHCHK();
+        ++resid_calls;
+        HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    fill_stats(stats, hst, ms, launches);
+    if (stats) {
+        stats->outer = int(std::max(0LL, resid_calls - 1));
+        stats->true_rel_residual = hst.true_rel;
+        stats->inner_iterations = hst.iters;
+        stats->bnorm = hst.bnorm;
+    }
+    if (hst.status == ST_NOT_SPD)
+        return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
+                    hst.bad_pq, hst.iters);
+    if (hst.status == ST_MAXITER)
+        return fail(H8_MAXITER,
+                    "refined solve: did not converge in %lld iterations (true relative residual %.3e >= "
+                    "tol=%.1e).",
+                    (long long)maxiter, hst.true_rel, tol);
+    return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Measurement helper: average device time of the two kernels of one PCG
+// iteration (direction+product, update), each launched `reps` times alone.
+// ---------------------------------------------------------------------------
+template <typename T, bool REFINED>
+static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
+                        T* q, PcgState* st, double* partials, int reps, double* avg_ms, hipStream_t s) {
+    const int sb = stream_blocks(d.ndof);
+    hipLaunchKernelGGL(k_fill<T>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, r, T(1));
+    hipLaunchKernelGGL(k_fill<T>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, p0, T(0.5));
+    hipLaunchKernelGGL(k_fill<T>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, p1, T(0.25));
+    hipLaunchKernelGGL(k_fill<T>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, x, T(0));
+    LAUNCHCHK();
+    PcgState h{};
+    h.status = ST_RUNNING;
+    h.k = 2;
+    h.first = 0;
+    h.beta = 0.5;
+    h.alpha = 1e-6;
+    h.rz_old = 1.0;
+    h.bnorm = 1.0;
+    h.tol = 0.0;
+    h.rmax = 0.0;
+    h.delta = 0.0;
+    h.maxiter = (1LL << 62);
+    HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
+    NodeArgs<T> A{};
+    A.g = d; A.s_e = s_e; A.K0 = K0; A.r = r; A.dinv = dinv; A.p0 = p0; A.p1 = p1; A.q = q;
+    A.st = st; A.partials = partials;
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(ev[0], s));
+    for (int i = 0; i < reps; ++i)
+        hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE_DIR>), dim3(d.ntiles), dim3(NTJ * NTI * NTK), 0, s, A);
+    LAUNCHCHK();
+    HIPCHK(hipEventRecord(ev[1], s));
+    HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(ev[2], s));
+    for (int i = 0; i < reps; ++i)
+        hipLaunchKernelGGL((k_pcg_update<T, REFINED>), dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, x, r,
+                           (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials);
+    LAUNCHCHK();
+    HIPCHK(hipEventRecord(ev[3], s));
+    HIPCHK(hipEventSynchronize(ev[3]));
+    float m1 = 0.f, m2 = 0.f;
+    HIPCHK(hipEventElapsedTime(&m1, ev[0], ev[1]));
+    HIPCHK(hipEventElapsedTime(&m2, ev[2], ev[3]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    avg_ms[0] = double(m1) / reps;
+    avg_ms[1] = double(m2) / reps;
+    PcgState hh{};
+    HIPCHK(hipMemcpy(&hh, st, sizeof hh, hipMemcpyDeviceToHost));
+    if (hh.status != ST_RUNNING) return fail(H8_ERR_ARG, "timing loop left the running state (%d)", hh.status);
+    return H8_OK;
+}
+
+extern "C" int h8_time_kernels(const h8_grid* g, int precision, const void* s_e, const void* K0,
+                               const double* dinv64, void* ws, size_t ws_bytes, int reps, double* avg_ms,
+                               void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e || !K0 || !dinv64 || !ws || !avg_ms || reps < 1) return fail(H8_ERR_ARG, "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    if (precision == 64) {
+        if (ws_bytes < ws_bytes_64(d)) return fail(H8_ERR_WORKSPACE, "workspace too small");
+        double* r = (double*)carve(p, sizeof(double) * d.ndof);
+        double* p0 = (double*)carve(p, sizeof(double) * d.ndof);
+        double* p1 = (double*)carve(p, sizeof(double) * d.ndof);
+        double* q = (double*)carve(p, sizeof(double) * d.ndof);
+        double* partials = (double*)carve(p, sizeof(double) * partial_slots(d));
+        PcgState* st = (PcgState*)carve(p, sizeof(PcgState));
+        double* x = (double*)carve(p, sizeof(double) * d.ndof);
+        return time_kernels<double, false>(d, (const double*)s_e, (const double*)K0, dinv64, x, r, p0, p1, q,
+                                           st, partials, reps, avg_ms, s);
+    }
+    if (precision == 32) {
+        if (ws_bytes < ws_bytes_ref(d)) return fail(H8_ERR_WORKSPACE, "workspace too small");
+        float* r = (float*)carve(p, sizeof(float) * d.ndof);
+        float* x = (float*)carve(p, sizeof(float) * d.ndof);
+        float* p0 = (float*)carve(p, sizeof(float) * d.ndof);
+        float* p1 = (float*)carve(p, sizeof(float) * d.ndof);
+        float* q = (float*)carve(p, sizeof(float) * d.ndof);
+        float* dinv32 = (float*)carve(p, sizeof(float) * d.ndof);
+        double* partials = (double*)carve(p, sizeof(double) * partial_slots(d));
+        PcgState* st = (PcgState*)carve(p, sizeof(PcgState));
+        hipLaunchKernelGGL(k_cast_f64_f32, dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s, d.ndof,
+                           dinv64, dinv32);
+        LAUNCHCHK();
+        return time_kernels<float, true>(d, (const float*)s_e, (const float*)K0, dinv32, x, r, p0, p1, q, st,
+                                         partials, reps, avg_ms, s);
+    }
+    return fail(H8_ERR_ARG, "precision must be 64 or 32");
+}

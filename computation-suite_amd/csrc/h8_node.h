@@ -1,0 +1,222 @@
+// h8_node.h -- node-centric matrix-free K*u for the structured H8 grid.
+//
+// One thread owns one node and computes its three rows of K*u by gathering
+// over the (up to) eight elements that touch it:
+//     q_n[c] = sum_a s_e(a) sum_b sum_d K0[3a+c][3b+d] * u[nbr(n,a,b)][d]
+// where a is the corner the node occupies in element e(a) and b runs over that
+// element's corners.  Each element's 24x24 product is therefore split across
+// its 8 nodes with no redundant arithmetic (576 FMA per node) and no atomics:
+// the result is deterministic.  A block owns a TJ x TI x TK brick of nodes,
+// stages the brick plus a one-node halo (component-planar, so lane-adjacent
+// reads are unit stride) and the (TJ+1)(TI+1)(TK+1) element scales in LDS,
+// then each thread keeps its 27-node neighbourhood in registers.  K0 is read
+// with uniform (scalar) loads.
+//
+// MODE_PLAIN : y = K u (every DOF).
+// MODE_DIR   : PCG direction + product, fused: p = dinv*r (+ beta*p_old) is
+//              formed while staging the halo (so no separate direction pass),
+//              written for owned nodes, q = K p, and p.q is reduced; the last
+//              block forms alpha = rz_old / p.q or flags p.q <= 0.
+// MODE_RESID : FP64 true residual of the refined solve: r64 = mask(f - K u),
+//              r32 = float(r64), reduces ||r64||^2 and r32.(dinv32*r32); the
+//              last block decides between "converged" and "continue with the
+//              replaced residual".
+#pragma once
+#include "h8_device.h"
+
+namespace h8 {
+
+enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2 };
+
+template <typename T>
+struct NodeArgs {
+    GridDev g;
+    const T* s_e;
+    const T* K0;      // 24x24 row-major
+    const T* u;       // PLAIN / RESID input
+    T* y;             // PLAIN output
+    const T* r;       // DIR
+    const T* dinv;    // DIR (T); RESID mask (double)
+    T* p0;            // DIR: direction buffers, iteration k writes p[k&1]
+    T* p1;            //      and reads the previous direction from p[(k-1)&1]
+    T* q;             // DIR
+    const double* f;  // RESID
+    float* r32;       // RESID
+    const float* dinv32;  // RESID
+    PcgState* st;
+    double* partials;
+};
+
+template <typename T, int TJ, int TI, int TK, int MODE>
+__global__ void __launch_bounds__(TJ * TI * TK)
+k_node(NodeArgs<T> A) {
+    constexpr int NT = TJ * TI * TK;
+    constexpr int HJ = TJ + 2, HI = TI + 2, HK = TK + 2;
+    constexpr int HPL = HJ * HI * HK;             // halo nodes
+    constexpr int EJ = TJ + 1, EI = TI + 1, EK = TK + 1;
+    constexpr int NE = EJ * EI * EK;
+
+    __shared__ T sP[3 * HPL];
+    __shared__ T sS[NE];
+    __shared__ double sRed[16 * 4];
+
+    const GridDev& g = A.g;
+    if constexpr (MODE == MODE_DIR) {
+        if (A.st->status != ST_RUNNING) return;
+    }
+    int j0, i0, k0;
+    tile_origin(g, blockIdx.x, TJ, TI, TK, j0, i0, k0);
+
+    T beta = T(0);
+    int first = 1;
+    T* p_new = nullptr;
+    const T* p_old = nullptr;
+    if constexpr (MODE == MODE_DIR) {
+        beta = T(A.st->beta);
+        first = A.st->first;
+        const bool odd = (A.st->k & 1) != 0;
+        p_new = odd ? A.p1 : A.p0;
+        p_old = odd ? A.p0 : A.p1;
+    }
+
+    // ---- stage the halo of the input vector (rows of 3*HJ contiguous words)
+    constexpr int ROW = 3 * HJ;
+    for (int idx = threadIdx.x; idx < ROW * HI * HK; idx += NT) {
+        const int w = idx % ROW;
+        const int row = idx / ROW;
+        const int ii = row % HI, kk = row / HI;
+        const int jj = w / 3, c = w - 3 * (w / 3);
+        const int j = j0 - 1 + jj, i = i0 - 1 + ii, k = k0 - 1 + kk;
+        T val = T(0);
+        if (j >= 0 && j < g.NJ && i >= 0 && i < g.NI && k >= 0 && k < g.NK) {
+            const long long gi = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j) + c;
+            if constexpr (MODE == MODE_DIR) {
+                T z = A.dinv[gi] * A.r[gi];
+                val = first ? z : z + beta * p_old[gi];
+                if (jj >= 1 && jj <= TJ && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK)
+                    p_new[gi] = val;
+            } else {
+                val = A.u[gi];
+            }
+        }
+        sP[(c * HK + kk) * HI * HJ + ii * HJ + jj] = val;
+    }
+    // ---- stage element scales (0 outside the mesh)
+    for (int idx = threadIdx.x; idx < NE; idx += NT) {
+        const int ej = idx % EJ, ei = (idx / EJ) % EI, ek = idx / (EJ * EI);
+        const int je = j0 - 1 + ej, ie = i0 - 1 + ei, ke = k0 - 1 + ek;
+        T s = T(0);
+        if (je >= 0 && je < g.ny && ie >= 0 && ie < g.nx && ke >= 0 && ke < g.nz)
+            s = A.s_e[(long long)ke * g.nx * g.ny + (long long)ie * g.ny + je];
+        sS[idx] = s;
+    }
+    __syncthreads();
+
+    const int tj = threadIdx.x % TJ;
+    const int ti = (threadIdx.x / TJ) % TI;
+    const int tk = threadIdx.x / (TJ * TI);
+    const int j = j0 + tj, i = i0 + ti, k = k0 + tk;
+    const bool owned = (j < g.NJ && i < g.NI && k < g.NK);
+
+    double part[2] = {0.0, 0.0};
+    if (owned) {
+        // 27-node neighbourhood, 3 components each
+        T nb[3][3][3][3];
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        nb[dz][dx][dy][d] = sP[(d * HK + tk + dz) * HI * HJ + (ti + dx) * HJ + (tj + dy)];
+        T qv[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
+            T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        acc[c] += A.K0[(3 * a + c) * 24 + 3 * b + d] * nb[dz][dx][dy][d];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) qv[c] += s * acc[c];
+        }
+        const long long gn = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j);
+        if constexpr (MODE == MODE_PLAIN) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) A.y[gn + c] = qv[c];
+        } else if constexpr (MODE == MODE_DIR) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                A.q[gn + c] = qv[c];
+                part[0] += double(nb[1][1][1][c]) * double(qv[c]);
+            }
+        } else {  // MODE_RESID (T = double)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double msk = A.dinv[gn + c];
+                const double r64 = (msk != 0.0) ? (A.f[gn + c] - double(qv[c])) : 0.0;
+                const float r32 = float(r64);
+                A.r32[gn + c] = r32;
+                part[0] += r64 * r64;
+                part[1] += double(r32) * double(A.dinv32[gn + c] * r32);
+            }
+        }
+    }
+
+    if constexpr (MODE == MODE_DIR) {
+        double v[1] = {part[0]}, tot[1];
+        if (reduce_last_block<1>(v, A.partials, &A.st->cnt[0], tot, sRed)) {
+            if (threadIdx.x == 0) {
+                PcgState* st = A.st;
+                const double pq = tot[0];
+                st->pq = pq;
+                if (!(pq > 0.0)) {
+                    st->bad_pq = pq;
+                    st->iters = st->k;
+                    st->status = ST_NOT_SPD;
+                } else {
+                    st->alpha = st->rz_old / pq;
+                }
+            }
+        }
+    } else if constexpr (MODE == MODE_RESID) {
+        double v[2] = {part[0], part[1]}, tot[2];
+        if (reduce_last_block<2>(v, A.partials, &A.st->cnt[2], tot, sRed)) {
+            if (threadIdx.x == 0) {
+                PcgState* st = A.st;
+                const double rn = sqrt(tot[0]);
+                st->true_rel = rn / st->bnorm;
+                st->outer += 1;
+                if (st->true_rel < st->tol) {
+                    st->status = ST_DONE;
+                    st->iters = st->k - 1;
+                    st->rel = st->true_rel;
+                } else if (st->k - 1 >= st->maxiter) {
+                    st->status = ST_MAXITER;
+                    st->iters = st->k - 1;
+                } else {
+                    st->rmax = rn;
+                    // keep the search direction: beta from the replaced residual
+                    if (st->first) {
+                        st->rz_old = tot[1];
+                    } else {
+                        st->beta = tot[1] / st->rz_old;
+                        st->rz_old = tot[1];
+                    }
+                    st->rel = st->true_rel;
+                    st->status = ST_RUNNING;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace h8

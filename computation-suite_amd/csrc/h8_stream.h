@@ -1,0 +1,302 @@
+// h8_stream.h -- streaming (one-pass, coalesced) kernels of the PCG recurrence
+// and the per-element kernels (Jacobi diagonal, SIMP scale, bilinear forms).
+#pragma once
+#include "h8_device.h"
+
+namespace h8 {
+
+constexpr int STREAM_THREADS = 256;
+
+// r = mask(b - Ax0) (Ax0 may be null), reduces b.b, r.r, r.(dinv r); the last
+// block applies the early-out test of fe/iterative_solvers.py:434-437.
+template <typename T>
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_pcg_init(long long n, const T* __restrict__ b, const T* __restrict__ Ax0, const T* __restrict__ dinv,
+           T* __restrict__ r, PcgState* st, double* partials) {
+    __shared__ double sRed[16 * 3];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const T d = dinv[i];
+        const T bi = (d != T(0)) ? b[i] : T(0);
+        const T ri = (d != T(0)) ? (Ax0 ? T(b[i] - Ax0[i]) : b[i]) : T(0);
+        r[i] = ri;
+        v[0] += double(bi) * double(bi);
+        v[1] += double(ri) * double(ri);
+        v[2] += double(ri) * double(d * ri);
+    }
+    double tot[3];
+    if (reduce_last_block<3>(v, partials, &st->cnt[1], tot, sRed) && threadIdx.x == 0) {
+        double bn = sqrt(tot[0]);
+        if (bn == 0.0) bn = 1.0;
+        st->bnorm = bn;
+        st->rel = sqrt(tot[1]) / bn;
+        st->k = 1;
+        st->first = 1;
+        st->beta = 0.0;
+        st->alpha = 0.0;
+        st->outer = 0;
+        if (st->rel < st->tol) {
+            st->status = ST_DONE;
+            st->iters = 0;
+        } else {
+            st->rz_old = tot[2];
+            st->status = ST_RUNNING;
+        }
+    }
+}
+
+// x += alpha p ; r = mask(r - alpha q) ; reduces r.z and r.r with z = dinv r.
+// REFINED = false: the stopping rule / beta of fe/iterative_solvers.py:455-466.
+// REFINED = true : additionally stops the FP32 loop for an FP64 residual
+//                  replacement when ||r|| < delta * rmax (or the recurrence
+//                  claims convergence, or maxiter is reached).
+template <typename T, bool REFINED>
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p0,
+             const T* __restrict__ p1, const T* __restrict__ q, const T* __restrict__ dinv, PcgState* st,
+             double* partials) {
+    __shared__ double sRed[16 * 2];
+    if (st->status != ST_RUNNING) return;
+    const T* __restrict__ p = (st->k & 1) ? p1 : p0;
+    const T alpha = T(st->alpha);
+    double v[2] = {0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const T d = dinv[i];
+        T xi = x[i];
+        T ri = r[i];
+        xi += alpha * p[i];
+        ri = (d != T(0)) ? T(ri - alpha * q[i]) : T(0);
+        x[i] = xi;
+        r[i] = ri;
+        v[0] += double(ri) * double(d * ri);
+        v[1] += double(ri) * double(ri);
+    }
+    double tot[2];
+    if (reduce_last_block<2>(v, partials, &st->cnt[1], tot, sRed) && threadIdx.x == 0) {
+        const double rn = sqrt(tot[1]);
+        const double rel = rn / st->bnorm;
+        st->rel = rel;
+        const long long k = st->k;
+        if (!REFINED) {
+            if (rel < st->tol) {
+                st->status = ST_DONE;
+                st->iters = k;
+            } else if (k >= st->maxiter) {
+                st->status = ST_MAXITER;
+                st->iters = k;
+            } else {
+                st->beta = tot[0] / st->rz_old;
+                st->rz_old = tot[0];
+                st->first = 0;
+                st->k = k + 1;
+            }
+        } else {
+            st->first = 0;
+            st->k = k + 1;
+            if (rel < st->tol || rn < st->delta * st->rmax || k >= st->maxiter) {
+                st->status = ST_REPLACE;
+            } else {
+                st->beta = tot[0] / st->rz_old;
+                st->rz_old = tot[0];
+            }
+        }
+    }
+}
+
+// u64 += double(x32); x32 = 0 (fold the FP32 iterate into the FP64 solution).
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_fold(long long n, double* __restrict__ u, float* __restrict__ x32) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        u[i] += double(x32[i]);
+        x32[i] = 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_cast_f64_f32(long long n, const double* __restrict__ x, float* __restrict__ y) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = float(x[i]);
+}
+
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_masked_sub(long long n, const double* __restrict__ a, const double* __restrict__ b,
+             const double* __restrict__ dinv, double* __restrict__ out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = (dinv[i] != 0.0) ? (b ? a[i] - b[i] : a[i]) : 0.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_fill(long long n, T* __restrict__ x, T v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+// Jacobi diagonal: K_ii = sum over the node's elements of s_e K0[3a+c][3a+c].
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_diag(GridDev g, const double* __restrict__ s_e, const double* __restrict__ K0,
+       const uint8_t* __restrict__ free_mask, double* __restrict__ dinv, int* __restrict__ flag) {
+    for (long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x; nd < g.nnodes;
+         nd += (long long)gridDim.x * blockDim.x) {
+        const int j = int(nd % g.NJ);
+        const int i = int((nd / g.NJ) % g.NI);
+        const int k = int(nd / ((long long)g.NJ * g.NI));
+        double dg[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 8; ++a) {
+            const int je = j - OY(a), ie = i - OX(a), ke = k - OZ(a);
+            if (je < 0 || je >= g.ny || ie < 0 || ie >= g.nx || ke < 0 || ke >= g.nz) continue;
+            const double s = s_e[(long long)ke * g.nx * g.ny + (long long)ie * g.ny + je];
+            for (int c = 0; c < 3; ++c) dg[c] += s * K0[(3 * a + c) * 25];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const long long di = 3 * nd + c;
+            const bool fr = free_mask[di] != 0;
+            if (fr && dg[c] == 0.0) atomicOr(flag, 1);
+            dinv[di] = (fr && dg[c] != 0.0) ? 1.0 / dg[c] : 0.0;
+        }
+    }
+}
+
+// s = rho_min + (1-rho_min) rho^p ; ds = p (1-rho_min) rho^(p-1)
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_simp(long long m, const double* __restrict__ rho, double p, double rho_min,
+       double* __restrict__ s, double* __restrict__ ds) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (long long)gridDim.x * blockDim.x) {
+        const double r = rho[e];
+        s[e] = rho_min + (1.0 - rho_min) * pow(r, p);
+        if (ds) ds[e] = p * (1.0 - rho_min) * pow(r, p - 1.0);
+    }
+}
+
+// out_e = w_e * u_e^T K0 v_e, one thread per element, element order e.
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_elem_bilinear(GridDev g, const double* __restrict__ K0, const double* __restrict__ u,
+                const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ out) {
+    const long long m = (long long)g.nx * g.ny * g.nz;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int je = int(e % g.ny);
+        const int ie = int((e / g.ny) % g.nx);
+        const int ke = int(e / ((long long)g.ny * g.nx));
+        double ue[24], ve[24];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const long long nd = (long long)(ke + OZ(a)) * g.NI * g.NJ + (long long)(ie + OX(a)) * g.NJ + (je + OY(a));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ue[3 * a + c] = u[3 * nd + c];
+                ve[3 * a + c] = v[3 * nd + c];
+            }
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int r = 0; r < 24; ++r) {
+            double y = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < 24; ++cc) y += K0[r * 24 + cc] * ve[cc];
+            acc += ue[r] * y;
+        }
+        out[e] = w[e] * acc;
+    }
+}
+
+// General-geometry element stiffness, one wavefront per element.  Lanes 0..7
+// each take one Gauss point (xi outer, eta middle, zeta inner, weight 1): they
+// form J = dN X, detJ and the physical gradients J^-1 dN into LDS; then every
+// lane owns 9 of the 576 (i,j) entries of sum_gp B^T D B detJ w.
+__global__ void __launch_bounds__(256)
+k_ke(long long m, const double* __restrict__ X, const double* __restrict__ D, long long d_stride,
+     double* __restrict__ ke) {
+    __shared__ double sG[4][8][3][8];
+    __shared__ double sDet[4][8];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long e = (long long)blockIdx.x * 4 + w;
+    const bool live = e < m;
+    if (live && lane < 8) {
+        const int q = lane;
+        const double gp = 0.57735026918962576451;  // 1/sqrt(3)
+        const double xi = (q & 4) ? gp : -gp;
+        const double et = (q & 2) ? gp : -gp;
+        const double ze = (q & 1) ? gp : -gp;
+        double dN[3][8];
+        for (int a = 0; a < 8; ++a) {
+            const double xa = OX(a) ? 1.0 : -1.0, ya = OY(a) ? 1.0 : -1.0, za = OZ(a) ? 1.0 : -1.0;
+            dN[0][a] = 0.125 * xa * (1 + et * ya) * (1 + ze * za);
+            dN[1][a] = 0.125 * ya * (1 + xi * xa) * (1 + ze * za);
+            dN[2][a] = 0.125 * za * (1 + xi * xa) * (1 + et * ya);
+        }
+        double J[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double acc = 0.0;
+                for (int a = 0; a < 8; ++a) acc += dN[r][a] * X[e * 24 + 3 * a + c];
+                J[r][c] = acc;
+            }
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double id = 1.0 / det;
+        double Ji[3][3];
+        Ji[0][0] = c00 * id;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+        Ji[1][0] = c01 * id;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        Ji[2][0] = c02 * id;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        for (int r = 0; r < 3; ++r)
+            for (int a = 0; a < 8; ++a)
+                sG[w][q][r][a] = Ji[r][0] * dN[0][a] + Ji[r][1] * dN[1][a] + Ji[r][2] * dN[2][a];
+        sDet[w][q] = det;
+    }
+    __syncthreads();
+    if (!live) return;
+    const double* De = D + e * d_stride;
+    double Dm[6][6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+        for (int t = 0; t < 6; ++t) Dm[s][t] = De[6 * s + t];
+#pragma unroll 1
+    for (int t = 0; t < 9; ++t) {
+        const int idx = lane + 64 * t;
+        const int I = idx / 24, Jc = idx % 24;
+        const int a = I / 3, c = I % 3, b = Jc / 3, d = Jc % 3;
+        double acc = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < 8; ++q) {
+            const double ga0 = sG[w][q][0][a], ga1 = sG[w][q][1][a], ga2 = sG[w][q][2][a];
+            const double gb0 = sG[w][q][0][b], gb1 = sG[w][q][1][b], gb2 = sG[w][q][2][b];
+            // column 3a+c of B over rows (xx, yy, zz, xy, yz, zx)
+            double Bi[6] = {0, 0, 0, 0, 0, 0}, Bj[6] = {0, 0, 0, 0, 0, 0};
+            if (c == 0) { Bi[0] = ga0; Bi[3] = ga1; Bi[5] = ga2; }
+            else if (c == 1) { Bi[1] = ga1; Bi[3] = ga0; Bi[4] = ga2; }
+            else { Bi[2] = ga2; Bi[4] = ga1; Bi[5] = ga0; }
+            if (d == 0) { Bj[0] = gb0; Bj[3] = gb1; Bj[5] = gb2; }
+            else if (d == 1) { Bj[1] = gb1; Bj[3] = gb0; Bj[4] = gb2; }
+            else { Bj[2] = gb2; Bj[4] = gb1; Bj[5] = gb0; }
+            double sum = 0.0;
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                double dbj = 0.0;
+#pragma unroll
+                for (int tt = 0; tt < 6; ++tt) dbj += Dm[s][tt] * Bj[tt];
+                sum += Bi[s] * dbj;
+            }
+            acc += sum * sDet[w][q];
+        }
+        ke[e * 576 + idx] = acc;
+    }
+}
+
+}  // namespace h8

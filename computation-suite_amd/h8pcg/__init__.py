@@ -1,0 +1,20 @@
+"""h8pcg: MI355X (gfx950) matrix-free H8 linear-elastic solver.
+
+A drop-in for fea_engine's Hex8 assemble / Jacobi-PCG solve path: the host API
+mirrors fea_engine (FESystem, box_mesh, Hex8Solid3D, Material, D_solid3d,
+coefficients.by_element), and every arithmetic step runs in libh8pcg.so
+(hand-written HIP kernels) on the GPU.
+"""
+from .material import Material, D_solid3d
+from .mesh import Mesh, box_mesh, structured_grid
+from .elements import Hex8Solid3D
+from . import coefficients
+from .fields import FEField
+from .facet_loads import list_facets, boundary_faces, face_load_shares
+from .solver import FESystem
+from ._lib import NotSPDError, H8LibraryMissing
+
+__all__ = ["Material", "D_solid3d", "Mesh", "box_mesh", "structured_grid", "Hex8Solid3D", "coefficients",
+           "FEField", "list_facets", "boundary_faces", "face_load_shares", "FESystem", "NotSPDError",
+           "H8LibraryMissing"]
+__version__ = "0.1.0"

@@ -1,0 +1,152 @@
+"""ctypes binding of libh8pcg.so (the C ABI declared in include/h8pcg.h).
+
+The library is loaded from this package directory.  There is no CPU fallback:
+if the shared library is missing or a device call is attempted without a GPU,
+the error is raised here, loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libh8pcg.so")
+
+H8_OK = 0
+H8_ERR_ARG = 1
+H8_ERR_HIP = 2
+H8_NOT_SPD = 3
+H8_MAXITER = 4
+H8_ZERO_DIAG = 5
+H8_ERR_WORKSPACE = 6
+
+VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA = 0, 1, 2
+
+
+class H8Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("variant", C.c_int32)]
+
+
+class H8PcgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("status", C.c_int32), ("outer", C.c_int32),
+                ("rel_residual", C.c_double), ("true_rel_residual", C.c_double),
+                ("bnorm", C.c_double), ("pAp", C.c_double), ("elapsed_ms", C.c_double),
+                ("inner_iterations", C.c_int64), ("launches", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class H8Instance(C.Structure):
+    """One structured instance of a batched solve (see include/h8pcg.h)."""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nrhs", C.c_int32),
+                ("node_off", C.c_int64), ("elem_off", C.c_int64), ("vec_off", C.c_int64),
+                ("tile_off", C.c_int32), ("ntiles", C.c_int32), ("sys_off", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_D = C.c_double
+_GRID = C.POINTER(H8Grid)
+_STATS = C.POINTER(H8PcgStats)
+
+# name -> (restype, argtypes); every symbol include/h8pcg.h declares.
+SIGNATURES = {
+    "h8_last_error": (C.c_char_p, []),
+    "h8_version": (C.c_int, []),
+    "h8_device_info": (C.c_int, [C.c_char_p, C.c_int]),
+    "h8_ke_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
+    "h8_diag_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
+    "h8_spmv_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),
+    "h8_spmv_f32": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),
+    "h8_pcg_workspace_bytes": (C.c_size_t, [_GRID, C.c_int]),
+    "h8_pcg_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, C.c_int, _D, _I64, _P, C.c_size_t, _STATS, _P]),
+    "h8_refine_solve": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, _P, C.c_size_t,
+                                  _STATS, _P]),
+    "h8_elem_bilinear_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
+    "h8_simp_scale_f64": (C.c_int, [_P, _D, _D, _I64, _P, _P, _P]),
+    "h8_cast_f64_f32": (C.c_int, [_P, _P, _I64, _P]),
+    "h8_masked_sub_f64": (C.c_int, [_P, _P, _P, _P, _I64, _P]),
+    "h8_time_kernels": (C.c_int, [_GRID, C.c_int, _P, _P, _P, _P, C.c_size_t, C.c_int, _P, _P]),
+    "h8_batched_workspace_bytes": (C.c_size_t, [C.POINTER(H8Instance), C.c_int32, C.c_int64, C.c_int64,
+                                                C.c_int32]),
+    "h8_pcg_batched_f64": (C.c_int, [C.POINTER(H8Instance), C.c_int32, _P, _P, _P, _P, _P, _D, _I64,
+                                     _P, C.c_size_t, _P, _P, _P, _P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class H8LibraryMissing(ImportError):
+    pass
+
+
+def lib():
+    """Load libh8pcg.so once; raise H8LibraryMissing if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise H8LibraryMissing(
+                f"libh8pcg.so not found at {LIB_PATH}: build it with "
+                f"`python computation-suite_amd/build.py` (or __graft_entry__.build()).")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    L = lib()
+    return [n for n in SIGNATURES if getattr(L, n, None) is not None]
+
+
+def last_error() -> str:
+    msg = lib().h8_last_error()
+    return msg.decode() if msg else ""
+
+
+class NotSPDError(np.linalg.LinAlgError):
+    """p^T A p <= 0 during PCG (the reference raises numpy.linalg.LinAlgError)."""
+
+
+def check(rc: int, who: str = "") -> None:
+    """Map a C status code to the reference's exception type."""
+    if rc == H8_OK:
+        return
+    msg = last_error()
+    if who and not msg.startswith(who):
+        msg = f"{who}: {msg}" if msg else who
+    if rc in (H8_ERR_ARG, H8_ZERO_DIAG, H8_ERR_WORKSPACE):
+        raise ValueError(msg)
+    if rc == H8_NOT_SPD:
+        raise NotSPDError(msg)
+    if rc == H8_MAXITER:
+        raise RuntimeError(msg)
+    raise RuntimeError(msg or f"h8pcg error {rc}")
+
+
+def ptr(t) -> int:
+    """Device pointer of a contiguous torch tensor (0 for None)."""
+    if t is None:
+        return 0
+    if not t.is_contiguous():
+        raise ValueError("h8pcg: tensors passed to the device library must be contiguous")
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

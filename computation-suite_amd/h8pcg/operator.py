@@ -1,0 +1,222 @@
+"""Device-resident structured H8 operator and its solves.
+
+A ``GridOperator`` owns, on one GPU: the grid (nx, ny, nz), the reference
+element matrix K0 (24x24, computed on the device by the batched element kernel),
+the per-element scale s_e, FP32 copies of both for the mixed-precision solve,
+and the inverse Jacobi diagonal (zero at fixed DOFs).  Every arithmetic step
+runs in libh8pcg.so; torch only allocates device memory and supplies the stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import H8Grid, H8PcgStats, check, ptr
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("h8pcg: no GPU is visible; the HIP backend has no CPU fallback")
+    _lib.lib()
+    return torch
+
+
+def device_of(device):
+    torch = require_gpu()
+    dev = torch.device(device if device is not None else "cuda")
+    if dev.type != "cuda":
+        raise ValueError(f"h8pcg: device must be a GPU device ('cuda' / 'cuda:N'), got {device!r}")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
+def element_stiffness_batched(coords, D, device="cuda"):
+    """ke (m,24,24) for element coordinates (m,8,3) and D (6,6) or (m,6,6),
+    computed by the device kernel; returns a device tensor."""
+    torch = require_gpu()
+    dev = device_of(device)
+    X = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64), device=dev).reshape(-1, 8, 3)
+    Dt = torch.as_tensor(np.ascontiguousarray(D, dtype=np.float64), device=dev)
+    m = X.shape[0]
+    if Dt.dim() == 2:
+        if Dt.shape != (6, 6):
+            raise ValueError(f"D must be 6x6, got {tuple(Dt.shape)}")
+        stride = 0
+    else:
+        if Dt.shape != (m, 6, 6):
+            raise ValueError(f"per-element D must be ({m},6,6), got {tuple(Dt.shape)}")
+        stride = 36
+    ke = torch.empty((m, 24, 24), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        check(_lib.lib().h8_ke_batched_f64(ptr(X), ptr(Dt.contiguous()), stride, m, ptr(ke),
+                                           _lib.stream_handle(dev)), "h8_ke_batched_f64")
+    return ke
+
+
+class GridOperator:
+    """K = sum_e s_e K0 on an nx x ny x nz box mesh, matrix-free on one GPU."""
+
+    def __init__(self, nx, ny, nz, K0, s_e, device="cuda", variant=_lib.VARIANT_AUTO):
+        torch = require_gpu()
+        self.device = device_of(device)
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.grid = H8Grid(self.nx, self.ny, self.nz, int(variant))
+        self.n_nodes = (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
+        self.n_dof = 3 * self.n_nodes
+        self.m = self.nx * self.ny * self.nz
+        self.K0 = torch.as_tensor(K0, dtype=torch.float64, device=self.device).contiguous()
+        if self.K0.shape != (24, 24):
+            raise ValueError("K0 must be 24x24")
+        self.s_e = torch.as_tensor(s_e, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
+        if self.s_e.numel() != self.m:
+            raise ValueError(f"s_e has {self.s_e.numel()} entries, the grid has {self.m} elements")
+        self.K0_32 = self.K0.float().contiguous()
+        self.s_e32 = self.s_e.float().contiguous()
+        self.dinv = None
+        self._ws = {}
+
+    # ---- helpers -----------------------------------------------------------
+    def _stream(self):
+        return _lib.stream_handle(self.device)
+
+    def _vec(self, v, dtype=None):
+        import torch
+        dtype = dtype or torch.float64
+        if isinstance(v, torch.Tensor):
+            t = v.to(device=self.device, dtype=dtype)
+        else:
+            t = torch.as_tensor(np.asarray(v), dtype=dtype, device=self.device)
+        t = t.reshape(-1).contiguous()
+        if t.numel() != self.n_dof:
+            raise ValueError(f"vector has {t.numel()} entries, expected {self.n_dof}")
+        return t
+
+    def workspace(self, precision):
+        import torch
+        ws = self._ws.get(precision)
+        if ws is None:
+            nbytes = _lib.lib().h8_pcg_workspace_bytes(C.byref(self.grid), precision)
+            ws = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+            self._ws[precision] = ws
+        return ws
+
+    # ---- operator ------------------------------------------------------------
+    def matvec(self, u):
+        """K u (every DOF), FP64."""
+        import torch
+        u = self._vec(u)
+        y = torch.empty_like(u)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().h8_spmv_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(u), ptr(y),
+                                         self._stream()), "h8_spmv_f64")
+        return y
+
+    def matvec32(self, u):
+        """K u in FP32 (FP32 s_e, K0 and vectors)."""
+        import torch
+        u = self._vec(u, torch.float32)
+        y = torch.empty_like(u)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().h8_spmv_f32(C.byref(self.grid), ptr(self.s_e32), ptr(self.K0_32), ptr(u), ptr(y),
+                                         self._stream()), "h8_spmv_f32")
+        return y
+
+    def set_free_mask(self, free_mask, preconditioner="jacobi"):
+        """Builds dinv: 1/K_ii at free DOFs (jacobi) or 1 (no preconditioner), 0 at fixed."""
+        import torch
+        fm = torch.as_tensor(np.asarray(free_mask, dtype=np.uint8), device=self.device).reshape(-1).contiguous()
+        if fm.numel() != self.n_dof:
+            raise ValueError("free mask length mismatch")
+        self.free_mask = fm
+        dinv = torch.empty(self.n_dof, dtype=torch.float64, device=self.device)
+        if preconditioner == "jacobi":
+            flag = torch.zeros(1, dtype=torch.int32, device=self.device)
+            with torch.cuda.device(self.device):
+                check(_lib.lib().h8_diag_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(fm), ptr(dinv),
+                                             ptr(flag), self._stream()), "")
+        elif preconditioner is None or preconditioner == "none":
+            dinv.copy_(fm.to(torch.float64))
+        else:
+            raise ValueError(f"unknown preconditioner {preconditioner!r}")
+        self.dinv = dinv
+        return dinv
+
+    def masked(self, a, b=None):
+        """mask(a - b) with mask = free DOFs."""
+        import torch
+        out = torch.empty(self.n_dof, dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().h8_masked_sub_f64(ptr(a), ptr(b), ptr(self.dinv), ptr(out), self.n_dof,
+                                               self._stream()), "h8_masked_sub_f64")
+        return out
+
+    # ---- solves ----------------------------------------------------------------
+    def pcg(self, b, tol=1e-8, maxiter=None, x0=None):
+        """Jacobi-PCG (FP64) on the free DOFs; b must already be masked.
+        Returns (x tensor, stats dict)."""
+        import torch
+        if self.dinv is None:
+            raise RuntimeError("set_free_mask() must be called before solving")
+        b = self._vec(b)
+        x = torch.zeros_like(b) if x0 is None else self._vec(x0).clone()
+        if maxiter is None:
+            maxiter = 10 * int(self.free_mask.sum().item())
+        ws = self.workspace(64)
+        st = H8PcgStats()
+        with torch.cuda.device(self.device):
+            rc = _lib.lib().h8_pcg_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(self.dinv), ptr(b),
+                                       ptr(x), 1 if x0 is not None else 0, float(tol), int(maxiter), ptr(ws),
+                                       ws.numel(), C.byref(st), self._stream())
+        stats = st.as_dict()
+        stats["rc"] = rc
+        check(rc)
+        return x, stats
+
+    def refined(self, f, tol=1e-8, maxiter=None, delta=0.1):
+        """FP32 PCG iterations with FP64 residual replacement; returns (u, stats)."""
+        import torch
+        if self.dinv is None:
+            raise RuntimeError("set_free_mask() must be called before solving")
+        f = self._vec(f)
+        u = torch.zeros_like(f)
+        if maxiter is None:
+            maxiter = 10 * int(self.free_mask.sum().item())
+        ws = self.workspace(32)
+        st = H8PcgStats()
+        with torch.cuda.device(self.device):
+            rc = _lib.lib().h8_refine_solve(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(self.s_e32),
+                                            ptr(self.K0_32), ptr(self.dinv), ptr(f), ptr(u), float(tol),
+                                            int(maxiter), float(delta), ptr(ws), ws.numel(), C.byref(st),
+                                            self._stream())
+        stats = st.as_dict()
+        stats["rc"] = rc
+        check(rc)
+        return u, stats
+
+    def element_bilinear(self, u, v, w):
+        """out_e = w_e u_e^T K0 v_e (device)."""
+        import torch
+        u = self._vec(u)
+        v = self._vec(v)
+        w = torch.as_tensor(w, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
+        if w.numel() != self.m:
+            raise ValueError("w must have one entry per element")
+        out = torch.empty(self.m, dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().h8_elem_bilinear_f64(C.byref(self.grid), ptr(self.K0), ptr(u), ptr(v), ptr(w),
+                                                  ptr(out), self._stream()), "h8_elem_bilinear_f64")
+        return out
+
+    def set_scale(self, s_e):
+        import torch
+        s = torch.as_tensor(s_e, dtype=torch.float64, device=self.device).reshape(-1)
+        if s.numel() != self.m:
+            raise ValueError("s_e length mismatch")
+        self.s_e = s.contiguous()
+        self.s_e32 = self.s_e.float().contiguous()
+        if getattr(self, "free_mask", None) is not None and self.dinv is not None:
+            self.set_free_mask(self.free_mask.cpu().numpy())

@@ -1,0 +1,399 @@
+"""FESystem on the HIP backend: the drop-in for fea_engine's linear-elastic H8
+assemble/solve path (fe/solver.py FESystem).
+
+Same constructor, same ``assemble_stiffness`` / ``fix_dofs`` / ``add_nodal_force``
+/ ``add_consistent_facet_load`` / ``free_dofs`` / ``solve_static`` methods,
+argument meanings and exception types as the reference (fe/solver.py:58-146,
+447-518, 569-697, 1109-1270, 1399-1509, 1726-2045).  What changes is where
+the work happens: assembly stores one 24x24 element matrix K0 (computed on the
+GPU) and a per-element scale s_e instead of a global K, and solve_static runs
+the matrix-free Jacobi-PCG in libh8pcg.so.  ``backend="hip"`` is the only
+backend of this package.
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+
+from . import _lib
+from .coefficients import Coefficient, factor_scales, has_coefficient
+from .elements import Hex8Solid3D
+from .facet_loads import face_load_shares_batched
+from .fields import FEField
+from .mesh import structured_grid
+
+
+class FESystem:
+    def __init__(self, mesh, elem_formulation, thickness=1.0, sparse=False, backend="hip", device="cuda"):
+        if backend != "hip":
+            raise ValueError(
+                f"FESystem: unknown backend={backend!r} -- this package provides backend='hip' "
+                f"(matrix-free Jacobi-PCG on an AMD GPU); the reference's 'scipy', 'torch' and "
+                f"'auto' backends live in fea_engine itself.")
+        if not isinstance(elem_formulation, Hex8Solid3D):
+            raise NotImplementedError(
+                f"FESystem(backend='hip') supports Hex8Solid3D only, got {type(elem_formulation).__name__}")
+        _lib.lib()   # fail fast when the device library is missing (as backend='torch' does for torch)
+        self.backend = backend
+        self.device = device
+        self.mesh = mesh
+        self.elem = elem_formulation
+        self.thickness = thickness
+        self.sparse = sparse
+        self.npn = 3
+        self.n_dof = 3 * len(mesh.nodes)
+        self.F = np.zeros(self.n_dof)
+        self.fixed_dofs = set()
+        self.fixed_dof_values = {}
+        self._assembled = set()
+        self._warned_reassembly = set()
+        self._op = None
+        self._D_ref = None
+        self._K_cache = None
+        self._structure = None
+        self.last_solve_stats = None
+
+    # ---- validation helpers (fe/solver.py:447-496) ------------------------------------------
+    @staticmethod
+    def _as_int_array(values, what, who):
+        arr = np.asarray(list(values) if isinstance(values, (set, frozenset)) else values)
+        arr = arr.reshape(-1)
+        if arr.size and arr.dtype.kind not in "iu":
+            if arr.dtype.kind == "f" and np.all(arr == np.round(arr)):
+                arr = arr.astype(int)
+            else:
+                raise ValueError(f"{who}: {what} must be integers, got {arr.dtype}.")
+        return arr.astype(int)
+
+    @property
+    def dof_names(self):
+        return self.elem.local_dof_names()
+
+    def dof_index(self, name):
+        return self._dof_index_by_name(name, "dof_index")
+
+    def _dof_index_by_name(self, name, who):
+        try:
+            return self.elem.dof_index(name)
+        except ValueError as err:
+            raise ValueError(f"{who}: {err}") from None
+
+    def _check_nodes(self, node_ids, who):
+        if isinstance(node_ids, str) or (isinstance(node_ids, (list, tuple)) and len(node_ids) > 0
+                                         and all(isinstance(n, str) for n in node_ids)):
+            names = [node_ids] if isinstance(node_ids, str) else list(node_ids)
+            sets = getattr(self.mesh, "node_sets", {})
+            missing = [n for n in names if n not in sets]
+            if missing:
+                raise ValueError(f"{who}: unknown node set(s) {missing}; available: {sorted(sets)}. "
+                                 f"Register one with mesh.add_node_set(name, ids) or "
+                                 f"mesh.select_nodes(..., name=name).")
+            node_ids = np.unique(np.concatenate([np.asarray(sets[n], dtype=int) for n in names]))
+        nodes = self._as_int_array(node_ids, "node ids", who)
+        if nodes.size == 0:
+            raise ValueError(
+                f"{who}: no nodes selected (node_ids is empty). A selector such as "
+                f"mesh.nodes_on_line(axis, value) matched nothing -- check the coordinate "
+                f"value and its tolerance (tol=).")
+        n_nodes = len(self.mesh.nodes)
+        bad = nodes[(nodes < 0) | (nodes >= n_nodes)]
+        if bad.size:
+            raise ValueError(
+                f"{who}: node id(s) {sorted(set(bad.tolist()))[:5]} outside the mesh "
+                f"(valid range 0..{n_nodes - 1}).")
+        return nodes
+
+    def _check_local_dofs(self, dof_indices, who):
+        if isinstance(dof_indices, (set, frozenset)):
+            items = list(dof_indices)
+        elif isinstance(dof_indices, str) or np.ndim(dof_indices) == 0:
+            items = [dof_indices]
+        else:
+            items = list(dof_indices)
+        resolved = [self._dof_index_by_name(it, who) if isinstance(it, str) else it for it in items]
+        dofs = self._as_int_array(np.array(resolved), "local DOF indices", who)
+        if dofs.size == 0:
+            raise ValueError(f"{who}: no DOF indices given.")
+        bad = dofs[(dofs < 0) | (dofs >= self.npn)]
+        if bad.size:
+            raise ValueError(
+                f"{who}: local DOF index {sorted(set(bad.tolist()))} is out of range -- "
+                f"Hex8Solid3D has {self.npn} DOF(s) per node, so valid indices "
+                f"are 0..{self.npn - 1}.")
+        return dofs
+
+    # ---- boundary conditions and loads (fe/solver.py:1109-1270) ----------------------------
+    def fix_dofs(self, node_ids, dof_indices, value=0.0):
+        nodes = self._check_nodes(node_ids, "fix_dofs")
+        dofs = self._check_local_dofs(dof_indices, "fix_dofs")
+        for n in nodes:
+            for k in dofs:
+                dof = self.npn * int(n) + int(k)
+                self.fixed_dofs.add(dof)
+                self.fixed_dof_values[dof] = float(value)
+
+    def add_nodal_force(self, node_ids, dof_index, total_force):
+        nodes = self._check_nodes(node_ids, "add_nodal_force")
+        dof = int(self._check_local_dofs(dof_index, "add_nodal_force")[0])
+        share = total_force / len(nodes)
+        np.add.at(self.F, self.npn * nodes + dof, share)
+
+    def add_consistent_facet_load(self, formulation, facets, dof_index, traction, quad_order=2, thickness=1.0):
+        facets = [tuple(f) for f in facets]
+        self._check_nodes([n for f in facets for n in f], "add_consistent_facet_load")
+        dof_index = int(self._check_local_dofs(dof_index, "add_consistent_facet_load")[0])
+        if not facets:
+            return
+        if any(len(f) != 4 for f in facets):
+            raise ValueError("add_consistent_facet_load: Hex8 faces have 4 nodes")
+        ids = np.asarray(facets, dtype=int)
+        shares = face_load_shares_batched(np.asarray(self.mesh.nodes)[ids], traction * thickness,
+                                          quad_order=quad_order)
+        np.add.at(self.F, self.npn * ids.ravel() + dof_index, shares.ravel())
+
+    # ---- DOF partition (fe/solver.py:1399-1401,1481-1494) ----------------------------------
+    @property
+    def free_dofs(self):
+        mask = np.ones(self.n_dof, dtype=bool)
+        if self.fixed_dofs:
+            mask[np.fromiter(self.fixed_dofs, dtype=np.int64)] = False
+        return np.nonzero(mask)[0]
+
+    @property
+    def fixed_dofs_array(self):
+        return np.array(sorted(self.fixed_dofs), dtype=int)
+
+    def _fixed_dof_values_array(self, fixed):
+        return np.array([self.fixed_dof_values.get(int(d), 0.0) for d in fixed])
+
+    def _free_mask(self):
+        mask = np.ones(self.n_dof, dtype=np.uint8)
+        if self.fixed_dofs:
+            mask[np.fromiter(self.fixed_dofs, dtype=np.int64)] = 0
+        return mask
+
+    # ---- assembly (fe/solver.py:498-518,569-697) -------------------------------------------
+    def _begin_assembly(self, accumulate, who):
+        if "K" in self._assembled and not accumulate and "K" not in self._warned_reassembly:
+            warnings.warn(
+                f"{who}: K was already assembled, so it is being REPLACED (earlier "
+                f"versions silently added a second copy, doubling the matrix). Pass "
+                f"accumulate=True to add to the existing matrix instead.",
+                UserWarning, stacklevel=3)
+            self._warned_reassembly.add("K")
+
+    def assemble_stiffness(self, D, method="full", vectorized=False, accumulate=False, **kwargs):
+        """Stores K0 (24x24, device) and s_e (per element, device).
+
+        D: a 6x6 matrix, or coefficients.by_element([...6x6...]) for per-element
+        material (the matrices must be scalar multiples of one another).
+        kwargs: thickness= (scales every element), gauss_order= (2 only),
+        element_scale= (an (m,) array multiplying each element, e.g. a SIMP scale)."""
+        if method not in ("full", "reduced", "hourglass_stabilized"):
+            raise ValueError(
+                f"assemble_stiffness: unknown method={method!r} -- expected "
+                f"'full' (default), 'reduced', or 'hourglass_stabilized'.")
+        if method != "full":
+            raise NotImplementedError(f"assemble_stiffness(method={method!r}) is not available on backend='hip'")
+        kw = dict(kwargs)
+        thickness = float(kw.pop("thickness", 1.0))
+        gauss_order = kw.pop("gauss_order", None)
+        elem_scale = kw.pop("element_scale", None)
+        kw.pop("chunk_size", None)
+        kw.pop("backend", None)
+        kw.pop("device", None)
+        if kw:
+            raise TypeError(f"assemble_stiffness: unsupported keyword(s) {sorted(kw)}")
+        if gauss_order not in (None, 2):
+            raise NotImplementedError("backend='hip' integrates Hex8 with the 2x2x2 rule only")
+        self._begin_assembly(accumulate, "assemble_stiffness")
+        m = len(self.mesh.elements)
+        if has_coefficient(D):
+            D_ref, s = factor_scales(D)
+            if len(s) != m:
+                raise ValueError(f"Coefficient.by_element: {len(s)} values but the mesh has {m} elements.")
+        else:
+            D_ref = np.asarray(D, dtype=float)
+            if D_ref.shape != (6, 6):
+                raise ValueError(f"assemble_stiffness: D must be 6x6 for Hex8Solid3D, got {D_ref.shape}")
+            s = np.ones(m)
+        s = s * thickness
+        if elem_scale is not None:
+            es = np.asarray(elem_scale, dtype=float).reshape(-1)
+            if es.shape != (m,):
+                raise ValueError(f"element_scale must have one entry per element ({m})")
+            s = s * es
+        st = self._structure if self._structure is not None else structured_grid(self.mesh)
+        if st is None:
+            raise NotImplementedError(
+                "FESystem(backend='hip'): the matrix-free operator needs a structured box mesh "
+                "(box_mesh connectivity with congruent bricks)")
+        self._structure = st
+        nx, ny, nz, coords0 = st
+        from .operator import GridOperator
+        if accumulate and self._op is not None:
+            ratio = _proportional(self._D_ref, D_ref)
+            s = self._op.s_e.cpu().numpy() + ratio * s
+            self._op.set_scale(s)
+        else:
+            ke0 = self.elem.stiffness_batched(coords0[None], D_ref, device=self.device)[0]
+            self._op = GridOperator(nx, ny, nz, ke0, s, device=self.device)
+            self._D_ref = D_ref
+        self._assembled.add("K")
+        self._K_cache = None
+        self._stiffness_args = (D, dict(kwargs))
+
+    def _require_stiffness(self, who):
+        if self._op is None:
+            raise RuntimeError(
+                f"{who}: the global stiffness matrix K is all zeros -- call "
+                f"assemble_stiffness(...) before solving.")
+
+    @property
+    def operator(self):
+        self._require_stiffness("operator")
+        return self._op
+
+    @property
+    def K(self):
+        """The assembled global stiffness as scipy CSR, built on demand on the GPU
+        (COO values s_e*K0 summed by torch's coalesce) and copied to the host."""
+        self._require_stiffness("K")
+        if self._K_cache is None:
+            self._K_cache = _materialize_csr(self._op, np.asarray(self.mesh.elements))
+        return self._K_cache
+
+    # ---- solve (fe/solver.py:1726-2045) ----------------------------------------------------
+    def _dirichlet_rhs(self, op, F_host):
+        """b = mask(F - K u_D) on the device and the full u_D vector (None if all zero)."""
+        import torch
+        fixed = self.fixed_dofs_array
+        u_fixed = self._fixed_dof_values_array(fixed)
+        F_t = torch.as_tensor(np.asarray(F_host, dtype=float), device=op.device)
+        if len(fixed) and np.any(u_fixed):
+            uD = np.zeros(self.n_dof)
+            uD[fixed] = u_fixed
+            uD_t = torch.as_tensor(uD, device=op.device)
+            return op.masked(F_t, op.matvec(uD_t)), uD_t
+        return op.masked(F_t), None
+
+    def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner="jacobi",
+                            precision="fp64", delta=0.1, F=None):
+        """solve_static, returning the full displacement as a device tensor (no host copy)."""
+        self._require_stiffness("solve_static")
+        if method is None:
+            method = "pcg"
+        if method not in ("cg", "pcg"):
+            raise ValueError(
+                f"solve_static: unknown method={method!r} -- choose from 'cg', 'pcg' (the HIP backend "
+                f"solves iteratively; the direct factorisations are CPU-only in the reference).")
+        if method == "cg":
+            preconditioner = None
+        elif preconditioner in ("ssor", "ic0"):
+            raise NotImplementedError(
+                f"solve_static(method='pcg', preconditioner={preconditioner!r}) is not supported with "
+                f"backend='hip'; use preconditioner='jacobi'.")
+        elif preconditioner != "jacobi":
+            raise ValueError(
+                f"solve_static(method='pcg'): unknown preconditioner="
+                f"{preconditioner!r} -- choose 'jacobi', 'ssor', or 'ic0'.")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        op = self._op
+        op.set_free_mask(self._free_mask(), preconditioner="jacobi" if preconditioner else None)
+        b, uD = self._dirichlet_rhs(op, self.F if F is None else F)
+        n_free = self.n_dof - len(self.fixed_dofs)
+        if maxiter is None:
+            maxiter = 10 * n_free
+        if precision == "fp64":
+            x0_t = None
+            if x0 is not None:
+                x0 = np.asarray(x0, dtype=float)
+                full = np.zeros(self.n_dof)
+                if x0.shape == (n_free,):
+                    full[self.free_dofs] = x0
+                elif x0.shape == (self.n_dof,):
+                    full[self.free_dofs] = x0[self.free_dofs]
+                else:
+                    raise ValueError(f"x0 must have {n_free} (free) or {self.n_dof} entries")
+                x0_t = full
+            x, stats = op.pcg(b, tol=tol, maxiter=maxiter, x0=x0_t)
+        else:
+            if x0 is not None:
+                raise ValueError("x0 is not supported with precision='mixed'")
+            x, stats = op.refined(b, tol=tol, maxiter=maxiter, delta=delta)
+        if uD is not None:
+            x = x + uD
+        stats["method"] = method
+        stats["precision"] = precision
+        self.last_solve_stats = stats
+        return x
+
+    def solve_static(self, verbose=False, method=None, **method_kwargs):
+        """Jacobi-PCG on the GPU (method None, 'pcg' or 'cg'); returns the full-length
+        displacement as an FEField on the host.  method_kwargs: tol (1e-8),
+        maxiter (10 * free DOFs), x0, preconditioner ('jacobi'), precision
+        ('fp64' or 'mixed': FP32 iterations with FP64 residual replacement),
+        delta (replacement threshold of the mixed solve)."""
+        self._require_stiffness("solve_static")
+        fixed = self.fixed_dofs_array
+        u_fixed = self._fixed_dof_values_array(fixed)
+        if not np.any(self.F) and not np.any(u_fixed):
+            warnings.warn(
+                "solve_static: the load vector F is all zeros and no non-zero displacement is "
+                "prescribed, so the solution is trivially zero. Add loads first "
+                "(add_nodal_force, add_consistent_edge_load, ...).", UserWarning, stacklevel=2)
+        x = self.solve_static_device(method=method, **method_kwargs)
+        if verbose:
+            st = self.last_solve_stats
+            print(f"[fea_engine backend=hip] solve: n={self.n_dof - len(self.fixed_dofs)}, nnz=matrix-free, "
+                  f"dtype=float64, device={self._op.device}, symmetric=True, spd=n/a, backend=hip, "
+                  f"method={st['method']}/{st['precision']}, iterations={st['iterations']}, "
+                  f"rel_residual={st['rel_residual']:.3e}, ms={st['elapsed_ms']:.2f}")
+        return self.field(x.cpu().numpy(), label="displacement")
+
+    def field(self, data, label=None):
+        if not isinstance(data, np.ndarray) or data.ndim not in (1, 2) or data.shape[0] != self.n_dof:
+            return data
+        return FEField(data, n_nodes=len(self.mesh.nodes), dofs_per_node=3, dof_names=self.dof_names,
+                       mesh=self.mesh, label=label)
+
+    def reactions(self, U=None):
+        """Support reactions R = K U - F at the constrained DOFs, zero elsewhere
+        (fe/recovery.py:422-448); K U is the device matrix-free product."""
+        import torch
+        if U is None:
+            U = self.solve_static()
+        self._require_stiffness("reactions")
+        U = np.asarray(U, dtype=float).reshape(-1)
+        if U.shape != (self.n_dof,):
+            raise ValueError(f"U must have shape ({self.n_dof},), got {U.shape}")
+        KU = self._op.matvec(torch.as_tensor(U, device=self._op.device)).cpu().numpy()
+        out = np.zeros(self.n_dof)
+        fixed = self.fixed_dofs_array
+        out[fixed] = KU[fixed] - self.F[fixed]
+        return self.field(out, label="reaction")
+
+
+def _proportional(A, B):
+    """c with B = c A (ValueError if not proportional)."""
+    piv = np.unravel_index(np.argmax(np.abs(A)), A.shape)
+    c = B[piv] / A[piv]
+    if np.abs(B - c * A).max() > 1e-12 * max(np.abs(B).max(), 1e-300):
+        raise ValueError("accumulate=True needs D proportional to the one already assembled")
+    return c
+
+
+def _materialize_csr(op, elements):
+    import scipy.sparse as sp
+    import torch
+    dev = op.device
+    conn = torch.as_tensor(elements, dtype=torch.int64, device=dev)
+    g = (3 * conn[:, :, None] + torch.arange(3, device=dev)[None, None, :]).reshape(-1, 24)
+    rows = g[:, :, None].expand(-1, 24, 24).reshape(-1)
+    cols = g[:, None, :].expand(-1, 24, 24).reshape(-1)
+    vals = (op.s_e[:, None, None] * op.K0[None]).reshape(-1)
+    K = torch.sparse_coo_tensor(torch.stack([rows, cols]), vals, (op.n_dof, op.n_dof)).coalesce()
+    idx = K.indices().cpu().numpy()
+    return sp.csr_matrix((K.values().cpu().numpy(), (idx[0], idx[1])), shape=(op.n_dof, op.n_dof))

@@ -1,0 +1,158 @@
+/* h8pcg.h -- C ABI of the MI355X (gfx950) H8 linear-elastic solver library.
+ *
+ * One shared library, libh8pcg.so, built from computation-suite_amd/csrc/.
+ * Every buffer argument is a DEVICE pointer (for example torch.Tensor.data_ptr()
+ * of a tensor on the GPU); `stream` is a hipStream_t passed as void* (0 = the
+ * null stream).  Functions return an int status (H8_OK on success) and never
+ * throw.  h8_last_error() returns a thread-local message for the last failure.
+ *
+ * Reference citations are file:line in abhi2k16/computation-suite, with
+ * fe/ = fea_engine/src/fea_engine/.
+ *
+ * Structured grids.  A `h8_grid` is the box mesh of fe/mesh.py:1288-1289:
+ * node(i,j,k) = k(nx+1)(ny+1) + i(ny+1) + j (y fastest), element
+ * e(i,j,k) = k*nx*ny + i*ny + j, corner order [n(i,j,k), n(i+1,j,k),
+ * n(i+1,j+1,k), n(i,j+1,k)] then the same at k+1, DOF 3*node + c.  Every
+ * element shares one 24x24 matrix K0 (uniform spacing) scaled by a per-element
+ * factor s_e (per-element Young's modulus, or the SIMP scale).  Full-length
+ * vectors have 3*(nx+1)(ny+1)(nz+1) entries; fixed DOFs are marked by a zero in
+ * `dinv` (the inverse Jacobi diagonal), which keeps every Krylov vector zero
+ * there -- the same iterates as the reduced system K_ff u_f = F_f.
+ */
+#ifndef H8PCG_H
+#define H8PCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped to the reference's exception types by the host) */
+#define H8_OK 0
+#define H8_ERR_ARG 1        /* ValueError: bad shape/size/pointer            */
+#define H8_ERR_HIP 2        /* RuntimeError: a HIP call failed               */
+#define H8_NOT_SPD 3        /* LinAlgError: p^T A p <= 0                     */
+#define H8_MAXITER 4        /* RuntimeError: not converged in maxiter        */
+#define H8_ZERO_DIAG 5      /* ValueError: zero Jacobi diagonal at a free DOF */
+#define H8_ERR_WORKSPACE 6  /* ValueError: workspace too small               */
+
+#define H8_VARIANT_AUTO 0   /* library default for the precision             */
+#define H8_VARIANT_NODE 1   /* node-centric gather (VALU), deterministic     */
+#define H8_VARIANT_MFMA 2   /* element-batched 24x24 contraction on MFMA     */
+
+typedef struct h8_grid {
+    int32_t nx, ny, nz;     /* elements per axis (>= 1)                      */
+    int32_t variant;        /* H8_VARIANT_* for the K*u kernels              */
+} h8_grid;
+
+typedef struct h8_pcg_stats {
+    int64_t iterations;     /* k at which ||r_k||/||b|| < tol (0: early out) */
+    int32_t status;         /* H8_OK, H8_NOT_SPD or H8_MAXITER               */
+    int32_t outer;          /* FP64 residual replacements (refined solve)    */
+    double rel_residual;    /* recurrence ||r_k||/||b|| at exit              */
+    double true_rel_residual; /* FP64 ||f - K u||/||f|| (refined), else -1   */
+    double bnorm;           /* ||b|| (1 when b = 0)                          */
+    double pAp;             /* the offending p^T A p when status = NOT_SPD   */
+    double elapsed_ms;      /* device time of the solve                      */
+    int64_t inner_iterations; /* FP32 iterations (refined solve)             */
+    int64_t launches;       /* graph launches (host polls)                   */
+} h8_pcg_stats;
+
+/* ---- diagnostics ------------------------------------------------------- */
+const char* h8_last_error(void);
+int h8_version(void);
+/* Writes the device name / CU count / arch of the current device. */
+int h8_device_info(char* buf, int buflen);
+
+/* ---- element stiffness ------------------------------------------------- */
+/* ke[e] = sum over the 2x2x2 Gauss rule of B^T D B detJ w, general geometry.
+ * X: (m,8,3) nodal coordinates; D: (6,6) when d_stride == 0, else (m,36) with
+ * d_stride = 36; ke: (m,24,24).  Replaces Element.stiffness +
+ * Hex8Solid3D.B_matrix (fe/elements/base.py:494-514, fe/elements/solids.py:
+ * 144-158) and tensorized_element_stiffness (fe/vectorized_assembly.py:165-214). */
+int h8_ke_batched_f64(const double* X, const double* D, int64_t d_stride, int64_t m,
+                      double* ke, void* stream);
+
+/* ---- structured matrix-free operator ------------------------------------ */
+/* dinv[i] = 1/K_ii at free DOFs (free_mask[i] != 0), 0 at fixed DOFs, with
+ * K_ii = sum_e s_e K0_aa.  Returns H8_ZERO_DIAG if a free diagonal is 0.
+ * Replaces jacobi_preconditioner / jacobi_preconditioner_torch
+ * (fe/iterative_solvers.py:193-202,305-321). */
+int h8_diag_f64(const h8_grid* g, const double* s_e, const double* K0,
+                const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream);
+
+/* y = K u over the whole grid (every DOF), K = assembled sum_e s_e K0.
+ * Replaces the CSR product A @ p of preconditioned_cg and _cg_torch
+ * (fe/iterative_solvers.py:439-441, fe/torch_sparse_solver.py:183) and the
+ * matrix-free _Ku_free of fe/topopt.py:214-220. */
+int h8_spmv_f64(const h8_grid* g, const double* s_e, const double* K0,
+                const double* u, double* y, void* stream);
+int h8_spmv_f32(const h8_grid* g, const float* s_e, const float* K0,
+                const float* u, float* y, void* stream);
+
+/* Workspace for h8_pcg_* / h8_refine_solve (bytes), precision 64 or 32 (32 =
+ * the refined solve, which needs both FP32 and FP64 vectors). */
+size_t h8_pcg_workspace_bytes(const h8_grid* g, int precision);
+
+/* Jacobi-PCG, Hestenes-Stiefel recurrence of preconditioned_cg
+ * (fe/iterative_solvers.py:434-469) on the masked full-length system:
+ * x0 = x if use_x0 else 0, bnorm = ||b|| (1 if 0), early return when
+ * ||r0||/bnorm < tol, H8_NOT_SPD when p^T A p <= 0, stop when
+ * ||r_k||/bnorm < tol, H8_MAXITER after maxiter iterations.  b must be zero at
+ * fixed DOFs.  Replaces FESystem._solve_static_iterative's "pcg" branch
+ * (fe/solver.py:1922-2029) and fesystem_solve_static_torch's CG
+ * (fe/torch_sparse_solver.py:268-314). */
+int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
+               const double* b, double* x, int use_x0, double tol, int64_t maxiter,
+               void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
+
+/* Mixed precision: FP32 Jacobi-PCG iterations (s_e32, K032, FP32 vectors,
+ * FP64 dot products) whose residual is replaced by the FP64 true residual
+ * r = f - K u (s_e64, K064) and whose iterate is folded into the FP64 u
+ * whenever the FP32 residual has fallen by `delta` (reliable updates; the
+ * search direction is kept).  Stops when the FP64 ||f - K u||/||f|| <= tol.
+ * No counterpart in the reference (it has one precision); it returns the same
+ * u as h8_pcg_f64 to within tol.  f zero at fixed DOFs; dinv64 as above. */
+int h8_refine_solve(const h8_grid* g, const double* s_e64, const double* K064,
+                    const float* s_e32, const float* K032, const double* dinv64,
+                    const double* f, double* u, double tol, int64_t maxiter, double delta,
+                    void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
+
+/* ---- element bilinear forms (SIMP sensitivities) ----------------------- */
+/* out[e] = w[e] * u_e^T K0 v_e for every element (u_e = u at the element's 24
+ * DOFs).  With w = -s'(rho), u = v this is compliance_sensitivity_closed_form
+ * (fe/topopt.py:161-177); with w = s'(rho), u = lambda, v = displacement it is
+ * the vector-Jacobian product of compliance_sensitivity_via_adjoint
+ * (fe/topopt.py:180-229, fe/differentiable.py:478-512). */
+int h8_elem_bilinear_f64(const h8_grid* g, const double* K0, const double* u, const double* v,
+                         const double* w, double* out, void* stream);
+
+/* s[e] = rho_min + (1-rho_min) rho^p and ds[e] = p (1-rho_min) rho^(p-1)
+ * (simp_scale / simp_scale_grad, fe/topopt.py:74-90).  ds may be NULL. */
+int h8_simp_scale_f64(const double* rho, double p, double rho_min, int64_t m,
+                      double* s, double* ds, void* stream);
+
+/* ---- measurement ------------------------------------------------------- */
+/* Average device time (ms) of the two kernels of one PCG iteration, each
+ * launched `reps` times alone between HIP events on `stream`: avg_ms[0] = the
+ * fused direction + K*p kernel, avg_ms[1] = the update kernel.  precision 64
+ * (workspace of h8_pcg_f64, s_e/K0 double) or 32 (workspace of
+ * h8_refine_solve, s_e/K0 float); dinv64 as for the solves.  Measurement
+ * helper with no counterpart in the reference; scribbles on the workspace. */
+int h8_time_kernels(const h8_grid* g, int precision, const void* s_e, const void* K0,
+                    const double* dinv64, void* ws, size_t ws_bytes, int reps, double* avg_ms,
+                    void* stream);
+
+/* ---- small vector helpers used by the host around the solves ----------- */
+/* y = float(x) */
+int h8_cast_f64_f32(const double* x, float* y, int64_t n, void* stream);
+/* out = mask ? (a - b) : 0 with mask = (dinv != 0)  (b may be NULL -> a) */
+int h8_masked_sub_f64(const double* a, const double* b, const double* dinv, double* out,
+                      int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H8PCG_H */

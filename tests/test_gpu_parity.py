@@ -1,0 +1,193 @@
+"""GPU parity: libh8pcg.so (through the C ABI, via h8pcg) against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import h8_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def h8(gpu):
+    import h8pcg
+    from h8pcg import _lib
+    buf = __import__("ctypes").create_string_buffer(256)
+    _lib.check(_lib.lib().h8_device_info(buf, 256))
+    say("device", buf.value.decode())
+    return h8pcg
+
+
+def test_ke_kernel_matches_oracle(h8, gpu):
+    say("ke kernel")
+    rng = np.random.default_rng(0)
+    base = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    X = base[None] * np.array([1.5, 0.7, 1.1]) + 0.12 * rng.uniform(-1, 1, (37, 8, 3))
+    Ds = np.stack([o.d_solid3d(E, nu) for E, nu in zip(rng.uniform(1, 5, 37), rng.uniform(0.1, 0.45, 37))])
+    ke = h8.Hex8Solid3D().stiffness_batched(X, Ds, device=gpu).cpu().numpy()
+    ref = o.hex8_ke_batched(X, Ds)
+    assert np.abs(ke - ref).max() < 1e-13 * np.abs(ref).max()
+    ke1 = h8.Hex8Solid3D().stiffness_batched(X, Ds[0], device=gpu).cpu().numpy()
+    assert np.abs(ke1 - o.hex8_ke_batched(X, Ds[0])).max() < 1e-13 * np.abs(ref).max()
+
+
+def _op(h8, gpu, nx, ny, nz, s, h=(1.0, 1.0, 1.0), nu=0.3):
+    from h8pcg.operator import GridOperator
+    X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float) * h
+    K0 = h8.Hex8Solid3D().stiffness_batched(X[None], o.d_solid3d(1.0, nu), device=gpu)[0]
+    return GridOperator(nx, ny, nz, K0, s, device=gpu)
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (5, 3, 2), (13, 9, 7), (20, 4, 4), (33, 17, 9)])
+def test_spmv_f64_f32_match_oracle_csr(h8, gpu, dims):
+    import torch
+    nx, ny, nz = dims
+    say("spmv", dims)
+    rng = np.random.default_rng(sum(dims))
+    m = nx * ny * nz
+    s = 10 ** rng.uniform(0, 2, m)
+    op = _op(h8, gpu, nx, ny, nz, s, h=(1.0, 0.5, 0.75))
+    nodes, elems = o.box_mesh(nx * 1.0, ny * 0.5, nz * 0.75, nx, ny, nz)
+    K = o.assemble_csr(nodes, elems, o.d_solid3d(1.0, 0.3), scale=s)
+    u = rng.normal(size=K.shape[0])
+    ref = K @ u
+    y = op.matvec(torch.as_tensor(u, device=gpu)).cpu().numpy()
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+    y32 = op.matvec32(torch.as_tensor(u, device=gpu)).cpu().numpy()
+    assert np.abs(y32 - ref).max() <= 2e-5 * np.abs(ref).max()
+
+
+def _fesys(h8, nx, ny, nz, E=210e9, E_elem=None, load="facet"):
+    mesh = h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    fs = h8.FESystem(mesh, h8.Hex8Solid3D(), backend="hip", device="cuda")
+    if E_elem is None:
+        fs.assemble_stiffness(h8.D_solid3d(h8.Material(E=E, nu=0.3)))
+    else:
+        Ds = [h8.D_solid3d(h8.Material(E=float(e), nu=0.3)) for e in E_elem]
+        fs.assemble_stiffness(h8.coefficients.by_element(Ds))
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    if load == "facet":
+        fs.add_consistent_facet_load(h8.Hex8Solid3D(), h8.boundary_faces(mesh, 0, float(nx)), 2, -1e6 / (ny * nz))
+    else:
+        fs.add_nodal_force(mesh.nodes_on_plane(0, float(nx)), 2, -1e6)
+    return fs
+
+
+def test_config1_pcg_matches_reference_and_spsolve(h8, gpu, golden):
+    say("config1 pcg")
+    fs = _fesys(h8, 20, 4, 4)
+    g = "c1_facet_1e-10"
+    assert np.allclose(fs.F, golden[g + "/F"], rtol=0, atol=1e-9)
+    U = fs.solve_static(method="pcg", tol=1e-10)
+    st = fs.last_solve_stats
+    c = o.cantilever(20, 4, 4)
+    U_direct = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
+    err = np.linalg.norm(np.asarray(U) - U_direct) / np.linalg.norm(U_direct)
+    say("iters", st["iterations"], "golden", int(golden[g + "/iters"]), "err vs spsolve", err)
+    assert err < 1e-8
+    assert abs(st["iterations"] - int(golden[g + "/iters"])) <= 2
+    assert np.linalg.norm(np.asarray(U) - golden[g + "/U"]) / np.linalg.norm(U_direct) < 1e-8
+    # fixed DOFs exactly zero
+    assert np.all(np.asarray(U)[fs.fixed_dofs_array] == 0.0)
+
+
+def test_nodal_and_heterogeneous_golden(h8, gpu, golden):
+    fs = _fesys(h8, 20, 4, 4, load="nodal")
+    U = fs.solve_static(method="pcg", tol=1e-8)
+    assert abs(fs.last_solve_stats["iterations"] - int(golden["c1_nodal_1e-8/iters"])) <= 2
+    E = golden["small_hetE_1e-8/E_elem"]
+    fs = _fesys(h8, 8, 2, 2, E_elem=E)
+    U = fs.solve_static(tol=1e-8)
+    say("het iters", fs.last_solve_stats["iterations"], int(golden["small_hetE_1e-8/iters"]))
+    assert abs(fs.last_solve_stats["iterations"] - int(golden["small_hetE_1e-8/iters"])) <= 3
+    ref = golden["small_hetE_1e-8/U"]
+    assert np.linalg.norm(np.asarray(U) - ref) / np.linalg.norm(ref) < 1e-7
+
+
+def test_mixed_precision_refined_reaches_fp64_accuracy(h8, gpu):
+    say("refined")
+    rng = np.random.default_rng(2)
+    nx, ny, nz = 24, 6, 6
+    E = 10 ** rng.uniform(9, 12, nx * ny * nz)
+    fs = _fesys(h8, nx, ny, nz, E_elem=E)
+    U = np.asarray(fs.solve_static(tol=1e-8, precision="mixed"))
+    st = fs.last_solve_stats
+    c = o.cantilever(nx, ny, nz, E_elem=E)
+    Ud = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
+    free, _ = o.partition(c["K"].shape[0], c["fixed_dofs"])
+    true_res = np.linalg.norm((c["F"] - c["K"] @ U)[free]) / np.linalg.norm(c["F"][free])
+    err = np.linalg.norm(U - Ud) / np.linalg.norm(Ud)
+    say("refined stats", st, "true_res", true_res, "err", err)
+    assert true_res < 1e-8
+    assert abs(true_res - st["true_rel_residual"]) < 1e-3 * max(true_res, 1e-12) + 1e-13
+    U64 = np.asarray(fs.solve_static(tol=1e-8))
+    say("fp64 iters", fs.last_solve_stats["iterations"])
+    assert np.linalg.norm(U - U64) / np.linalg.norm(Ud) < 1e-6
+
+
+def test_nonhomogeneous_dirichlet_and_reactions(h8, gpu):
+    say("dirichlet")
+    nx, ny, nz = 6, 3, 3
+    mesh = h8.box_mesh(6.0, 3.0, 3.0, nx, ny, nz)
+    fs = h8.FESystem(mesh, h8.Hex8Solid3D())
+    fs.assemble_stiffness(h8.D_solid3d(h8.Material(E=1000.0, nu=0.3)))
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.fix_dofs(mesh.nodes_on_plane(0, 6.0), [0], value=0.01)
+    U = np.asarray(fs.solve_static(method="pcg", tol=1e-12))
+    c = o.cantilever(nx, ny, nz, E=1000.0)
+    fixed = sorted(fs.fixed_dofs)
+    vals = {d: fs.fixed_dof_values[d] for d in fixed}
+    Ud = o.solve_direct(c["K"], np.zeros(c["K"].shape[0]), fixed, vals)
+    assert np.linalg.norm(U - Ud) / np.linalg.norm(Ud) < 1e-9
+    R = np.asarray(fs.reactions(U))
+    Rref = c["K"] @ Ud
+    assert np.abs(R[fixed] - Rref[fixed]).max() < 1e-8 * np.abs(Rref).max()
+    assert abs(R[0::3].sum()) < 1e-8 * np.abs(R).max()       # equilibrium: x reactions balance
+
+
+def test_error_paths(h8, gpu):
+    say("errors")
+    mesh = h8.box_mesh(4.0, 2.0, 2.0, 4, 2, 2)
+    fs = h8.FESystem(mesh, h8.Hex8Solid3D())
+    with pytest.raises(RuntimeError):
+        fs.solve_static()
+    fs.assemble_stiffness(h8.D_solid3d(h8.Material(E=1.0, nu=0.3)))
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.add_nodal_force(mesh.nodes_on_plane(0, 4.0), 2, -1.0)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        fs.solve_static(tol=1e-14, maxiter=3)
+    with pytest.raises(ValueError, match="preconditioner"):
+        fs.solve_static(method="pcg", preconditioner="bogus")
+    # an indefinite operator (negative element scales) trips p^T A p <= 0
+    fs2 = h8.FESystem(mesh, h8.Hex8Solid3D())
+    fs2.assemble_stiffness(h8.D_solid3d(h8.Material(E=1.0, nu=0.3)), element_scale=-np.ones(16))
+    fs2.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs2.add_nodal_force(mesh.nodes_on_plane(0, 4.0), 2, -1.0)
+    with pytest.raises(np.linalg.LinAlgError):
+        fs2.solve_static(method="cg")
+    # zero Jacobi diagonal at a free DOF (all elements around it have zero scale)
+    fs3 = h8.FESystem(mesh, h8.Hex8Solid3D())
+    fs3.assemble_stiffness(h8.D_solid3d(h8.Material(E=1.0, nu=0.3)), element_scale=np.zeros(16))
+    fs3.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs3.add_nodal_force(mesh.nodes_on_plane(0, 4.0), 2, -1.0)
+    with pytest.raises(ValueError, match="zero diagonal"):
+        fs3.solve_static()
+    # zero load: warning and the trivial solution in 0 iterations
+    fs4 = h8.FESystem(mesh, h8.Hex8Solid3D())
+    fs4.assemble_stiffness(h8.D_solid3d(h8.Material(E=1.0, nu=0.3)))
+    fs4.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    with pytest.warns(UserWarning):
+        U = fs4.solve_static()
+    assert np.all(np.asarray(U) == 0) and fs4.last_solve_stats["iterations"] == 0
+
+
+def test_lazy_K_matches_oracle(h8, gpu):
+    fs = _fesys(h8, 5, 3, 2)
+    c = o.cantilever(5, 3, 2)
+    assert abs(fs.K - c["K"]).max() < 1e-12 * abs(c["K"]).max()

@@ -1,0 +1,113 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every symbol
+the header declares, and the FESystem mirror's host logic (meshes, node
+selection, loads, BC bookkeeping, validation errors) matches the oracle."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import h8_oracle as o
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "h8pcg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(h8_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    from h8pcg import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 14
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the binding knows a signature for each of them
+    assert not [n for n in names if n not in _lib.SIGNATURES]
+
+
+def test_library_non_device_calls():
+    from h8pcg import _lib
+    L = _lib.lib()
+    assert L.h8_version() >= 1
+    g = _lib.H8Grid(128, 32, 32, 0)
+    n = 3 * 129 * 33 * 33
+    b64 = L.h8_pcg_workspace_bytes(g, 64)
+    b32 = L.h8_pcg_workspace_bytes(g, 32)
+    assert b64 >= 4 * 8 * n and b32 >= 6 * 4 * n
+    bad = _lib.H8Grid(0, 1, 1, 0)
+    assert L.h8_pcg_workspace_bytes(bad, 64) == 0
+    # argument validation happens before any device work
+    rc = L.h8_spmv_f64(bad, None, None, None, None, None)
+    assert rc == _lib.H8_ERR_ARG and "nx, ny, nz" in _lib.last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc)
+
+
+def test_box_mesh_matches_oracle_and_structure():
+    import h8pcg
+    m = h8pcg.box_mesh(7.0, 3.0, 2.0, 7, 3, 4)
+    nodes, elems = o.box_mesh(7.0, 3.0, 2.0, 7, 3, 4)
+    assert np.array_equal(m.elements, elems)
+    assert np.abs(m.nodes - nodes).max() < 1e-14
+    nx, ny, nz, X0 = h8pcg.structured_grid(m)
+    assert (nx, ny, nz) == (7, 3, 4)
+    # a plain Mesh with the same arrays (e.g. built by the reference) is recognised too
+    plain = h8pcg.Mesh(nodes.copy(), elems.copy(), 3)
+    assert h8pcg.structured_grid(plain)[:3] == (7, 3, 4)
+    # a distorted interior node breaks the shared-K0 structure
+    plain.nodes[len(nodes) // 2] += 0.1
+    assert h8pcg.structured_grid(plain) is None
+
+
+def test_loads_and_dofs_match_oracle():
+    import h8pcg
+    mesh = h8pcg.box_mesh(20.0, 4.0, 4.0, 20, 4, 4)
+    fs = h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D())
+    fs.add_consistent_facet_load(h8pcg.Hex8Solid3D(), h8pcg.boundary_faces(mesh, 0, 20.0), 2, -1e6 / 16)
+    c = o.cantilever(20, 4, 4)
+    assert np.abs(fs.F - c["F"]).max() < 1e-9
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), ["ux", "uy", "uz"])
+    assert sorted(fs.fixed_dofs) == sorted(c["fixed_dofs"])
+    free, fixed = o.partition(fs.n_dof, c["fixed_dofs"])
+    assert np.array_equal(fs.free_dofs, free) and np.array_equal(fs.fixed_dofs_array, fixed)
+    fs2 = h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D())
+    fs2.add_nodal_force(mesh.nodes_on_plane(0, 20.0), "uz", -1e6)
+    c2 = o.cantilever(20, 4, 4, load="nodal")
+    assert np.abs(fs2.F - c2["F"]).max() < 1e-9
+
+
+def test_validation_errors():
+    import h8pcg
+    mesh = h8pcg.box_mesh(2.0, 1.0, 1.0, 2, 1, 1)
+    with pytest.raises(ValueError, match="backend"):
+        h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D(), backend="scipy")
+    fs = h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D())
+    with pytest.raises(ValueError, match="no nodes selected"):
+        fs.fix_dofs([], [0])
+    with pytest.raises(ValueError, match="out of range"):
+        fs.fix_dofs([0], [3])
+    with pytest.raises(ValueError, match="outside the mesh"):
+        fs.add_nodal_force([99], 0, 1.0)
+    with pytest.raises(ValueError, match="unknown node set"):
+        fs.fix_dofs("tip", [0])
+    mesh.select_nodes(x=2.0, name="tip")
+    fs.fix_dofs("tip", ["ux"])
+    assert len(fs.fixed_dofs) == 4
+    with pytest.raises(RuntimeError, match="assemble_stiffness"):
+        fs.solve_static()
+    with pytest.raises(ValueError, match="unknown method"):
+        fs.assemble_stiffness(np.eye(6), method="bogus")
+
+
+def test_coefficient_factoring():
+    from h8pcg import coefficients, D_solid3d, Material
+    E = np.array([1.0, 3.0, 7.0])
+    D_ref, s = coefficients.factor_scales(coefficients.by_element([D_solid3d(Material(e, 0.3)) for e in E]))
+    assert np.allclose(s * D_ref[0, 0], [D_solid3d(Material(e, 0.3))[0, 0] for e in E])
+    with pytest.raises(ValueError, match="scalar multiples"):
+        coefficients.factor_scales(coefficients.by_element([D_solid3d(Material(1, 0.3)), D_solid3d(Material(1, 0.2))]))
