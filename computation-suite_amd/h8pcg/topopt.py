@@ -1,0 +1,190 @@
+"""SIMP compliance and its adjoint gradient on the GPU (fe/topopt.py:74-229,
+fe/differentiable.py:478-612).
+
+``SIMPProblem`` holds one structured H8 grid with a reference element K0 (E0,
+nu), a load F and clamped DOFs.  K(rho) = sum_e s(rho_e) K0 with
+s = rho_min + (1-rho_min) rho^p (simp_scale); every solve, scale and
+element contraction runs in libh8pcg.so.
+
+Two torch.autograd.Functions sit on top:
+  * ``compliance(rho, problem)``  -> C = F.u.  Backward uses the self-adjoint
+    closed form dC/drho_e = -s'(rho_e) u_e^T K0 u_e (no second solve), with u
+    stored by the forward pass (or recomputed when store_forward=False).
+  * ``displacement(rho, problem)`` -> u (full length).  Backward solves the
+    adjoint K lambda = dL/du (masked) with the same PCG and returns
+    dL/drho_e = -s'(rho_e) lambda_e^T K0 u_e.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+from .elements import Hex8Solid3D
+from .material import D_solid3d, Material
+from .operator import GridOperator, require_gpu
+
+
+def simp_scale(rho, p=3.0, rho_min=1e-3, with_grad=False):
+    """s(rho) (and s'(rho)) as device tensors (fe/topopt.py:74-90)."""
+    import torch
+    r = rho.detach().to(torch.float64).contiguous()
+    s = torch.empty_like(r)
+    ds = torch.empty_like(r) if with_grad else None
+    with torch.cuda.device(r.device):
+        check(_lib.lib().h8_simp_scale_f64(ptr(r), float(p), float(rho_min), r.numel(), ptr(s), ptr(ds),
+                                           _lib.stream_handle(r.device)), "h8_simp_scale_f64")
+    return (s, ds) if with_grad else s
+
+
+class SIMPProblem:
+    """Clamped structured H8 body under a fixed load, stiffness scaled per element."""
+
+    def __init__(self, nx, ny, nz, F, fixed_dofs, E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0), p=3.0, rho_min=1e-9,
+                 device="cuda", tol=1e-10, precision="fp64", maxiter=None, store_forward=True):
+        torch = require_gpu()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.m = self.nx * self.ny * self.nz
+        self.p, self.rho_min = float(p), float(rho_min)
+        self.tol, self.precision, self.maxiter = tol, precision, maxiter
+        self.store_forward = store_forward
+        X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                      [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float) * np.asarray(h, dtype=float)
+        K0 = Hex8Solid3D().stiffness_batched(X[None], D_solid3d(Material(E=E0, nu=nu)), device=device)[0]
+        self.op = GridOperator(nx, ny, nz, K0, np.ones(self.m), device=device)
+        n = self.op.n_dof
+        mask = np.ones(n, dtype=np.uint8)
+        mask[np.asarray(list(fixed_dofs), dtype=np.int64)] = 0
+        self.free_mask = mask
+        self.F = torch.as_tensor(np.asarray(F, dtype=float), device=self.op.device)
+        self.last_stats = None
+        self.solves = 0
+
+    @property
+    def device(self):
+        return self.op.device
+
+    def _set_rho(self, rho):
+        s, ds = simp_scale(rho, self.p, self.rho_min, with_grad=True)
+        self.op.s_e = s.contiguous()
+        self.op.s_e32 = s.float().contiguous()
+        self.op.set_free_mask(self.free_mask)
+        return s, ds
+
+    def solve(self, rho, rhs=None, x0=None, tol=None):
+        """u with K(rho) u = rhs (default F) on the free DOFs."""
+        self._set_rho(rho)
+        b = self.op.masked(self.F if rhs is None else rhs.to(torch_f64()).contiguous())
+        tol = self.tol if tol is None else tol
+        if self.precision == "mixed" and x0 is None:
+            u, st = self.op.refined(b, tol=tol, maxiter=self.maxiter)
+        else:
+            u, st = self.op.pcg(b, tol=tol, maxiter=self.maxiter, x0=x0)
+        self.last_stats = st
+        self.solves += 1
+        return u
+
+    def compliance_value(self, u):
+        return float((self.F * u).sum())
+
+    def sensitivity_closed_form(self, rho, u):
+        """dC/drho_e = -s'(rho_e) u_e^T K0 u_e (fe/topopt.py:161-177)."""
+        _, ds = simp_scale(rho, self.p, self.rho_min, with_grad=True)
+        return self.op.element_bilinear(u, u, -ds)
+
+    def adjoint_vjp(self, rho, u, lam):
+        """dL/drho_e = -s'(rho_e) lambda_e^T K0 u_e for K lambda = dL/du."""
+        _, ds = simp_scale(rho, self.p, self.rho_min, with_grad=True)
+        return self.op.element_bilinear(lam, u, -ds)
+
+
+def torch_f64():
+    import torch
+    return torch.float64
+
+
+def _functions():
+    import torch
+
+    class _Compliance(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, rho, problem):
+            u = problem.solve(rho)
+            ctx.problem = problem
+            if problem.store_forward:
+                ctx.save_for_backward(rho, u)
+            else:
+                ctx.save_for_backward(rho)
+            return (problem.F * u).sum().reshape(())
+
+        @staticmethod
+        def backward(ctx, gC):
+            problem = ctx.problem
+            if problem.store_forward:
+                rho, u = ctx.saved_tensors
+            else:
+                (rho,) = ctx.saved_tensors
+                u = problem.solve(rho)
+            g = problem.sensitivity_closed_form(rho, u)
+            return (gC * g).to(rho.dtype), None
+
+    class _Displacement(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, rho, problem):
+            u = problem.solve(rho)
+            ctx.problem = problem
+            ctx.save_for_backward(rho, u)
+            return u
+
+        @staticmethod
+        def backward(ctx, gu):
+            rho, u = ctx.saved_tensors
+            problem = ctx.problem
+            lam = problem.solve(rho, rhs=gu.contiguous())
+            return problem.adjoint_vjp(rho, u, lam).to(rho.dtype), None
+
+    return _Compliance, _Displacement
+
+
+_FNS = None
+
+
+def compliance(rho, problem):
+    """C(rho) = F.u(rho) as a differentiable torch scalar."""
+    global _FNS
+    if _FNS is None:
+        _FNS = _functions()
+    return _FNS[0].apply(rho, problem)
+
+
+def displacement(rho, problem):
+    """u(rho) (full length) as a differentiable torch tensor (general adjoint backward)."""
+    global _FNS
+    if _FNS is None:
+        _FNS = _functions()
+    return _FNS[1].apply(rho, problem)
+
+
+def fd_check(problem, rho, elements, h=1e-6, tol=1e-12):
+    """Cancellation-free central differences of C at the given elements:
+    C(rho+h e) - C(rho-h e) = -(s+ - s-) u-_e^T K0 u+_e (K symmetric), with the
+    perturbed solves warm-started from u(rho).  Returns (fd, u) as numpy."""
+    import torch
+    u = problem.solve(rho, tol=tol)
+    out = []
+    s_all = simp_scale(rho, problem.p, problem.rho_min)
+    for e in elements:
+        rp = rho.clone(); rm = rho.clone()
+        rp[e] += h
+        rm[e] -= h
+        up = problem.solve(rp, x0=u, tol=tol)
+        um = problem.solve(rm, x0=u, tol=tol)
+        sp_ = simp_scale(rp[e:e + 1], problem.p, problem.rho_min)
+        sm_ = simp_scale(rm[e:e + 1], problem.p, problem.rho_min)
+        w = torch.zeros(problem.m, dtype=torch.float64, device=problem.device)
+        w[e] = -(sp_ - sm_)[0]
+        dC = problem.op.element_bilinear(um, up, w)[e]
+        out.append(float(dC) / (2 * h))
+    problem._set_rho(rho)
+    del s_all
+    return np.array(out), u
