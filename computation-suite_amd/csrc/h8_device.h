@@ -22,7 +22,15 @@ struct GridDev {
     long long ndof;        // 3*nnodes
     int tj, ti, tk;        // tiles per axis (node tiles)
     int ntiles;            // tj*ti*tk
+    int variant;           // 1 = node-centric VALU, 2 = element-batched MFMA
 };
+
+inline void set_tiles(GridDev& d, int TJ, int TI, int TK) {
+    d.tj = (d.NJ + TJ - 1) / TJ;
+    d.ti = (d.NI + TI - 1) / TI;
+    d.tk = (d.NK + TK - 1) / TK;
+    d.ntiles = d.tj * d.ti * d.tk;
+}
 
 // Solver status held in device memory.
 enum : int { ST_RUNNING = 0, ST_DONE = 1, ST_NOT_SPD = 2, ST_MAXITER = 3, ST_REPLACE = 4 };

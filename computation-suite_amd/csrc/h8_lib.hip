@@ -16,6 +16,7 @@
 #include "h8_device.h"
 #include "h8_node.h"
 #include "h8_stream.h"
+#include "h8_mfma.h"
 
 using namespace h8;
 
@@ -69,6 +70,15 @@ extern "C" int h8_device_info(char* buf, int buflen) {
 // Node brick owned by one block of the node-centric kernel: TJ (y, unit
 // stride) x TI (x) x TK (z) = 256 threads.
 constexpr int NTJ = 8, NTI = 8, NTK = 4;
+// Node brick of the MFMA kernel (256 threads, larger bricks amortise the
+// one-element halo the block recomputes): FP32 16x8x8, FP64 8x8x8 (LDS).
+template <typename T> struct MfmaTile;
+template <> struct MfmaTile<float> { static constexpr int J = 16, I = 8, K = 8; };
+template <> struct MfmaTile<double> { static constexpr int J = 8, I = 8, K = 8; };
+
+template <typename T> static int default_variant();
+template <> int default_variant<float>() { return H8_VARIANT_MFMA; }
+template <> int default_variant<double>() { return H8_VARIANT_NODE; }
 
 static int make_grid(const h8_grid* g, GridDev& d) {
     if (!g) return fail(H8_ERR_ARG, "grid is NULL");
@@ -84,6 +94,29 @@ static int make_grid(const h8_grid* g, GridDev& d) {
     long long nt = (long long)d.tj * d.ti * d.tk;
     if (nt > (1LL << 30)) return fail(H8_ERR_ARG, "grid too large");
     d.ntiles = int(nt);
+    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_MFMA)
+        return fail(H8_ERR_ARG, "unknown kernel variant %d", g->variant);
+    d.variant = g->variant;
+    return H8_OK;
+}
+
+// Launch one structured operator kernel (mode MODE) with the variant recorded
+// in d (AUTO -> the precision's default).
+template <typename T, int MODE>
+static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
+    GridDev d = d0;
+    const int v = d.variant == H8_VARIANT_AUTO ? default_variant<T>() : d.variant;
+    if (v == H8_VARIANT_MFMA) {
+        constexpr int TJ = MfmaTile<T>::J, TI = MfmaTile<T>::I, TK = MfmaTile<T>::K;
+        set_tiles(d, TJ, TI, TK);
+        A.g = d;
+        hipLaunchKernelGGL((k_mfma<T, TJ, TI, TK, MODE>), dim3(d.ntiles), dim3(256), 0, s, A);
+    } else {
+        set_tiles(d, NTJ, NTI, NTK);
+        A.g = d;
+        hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE>), dim3(d.ntiles), dim3(NTJ * NTI * NTK), 0, s, A);
+    }
+    LAUNCHCHK();
     return H8_OK;
 }
 
@@ -119,10 +152,7 @@ static int launch_node_plain(const GridDev& d, const T* s_e, const T* K0, const 
                              hipStream_t s) {
     NodeArgs<T> A{};
     A.g = d; A.s_e = s_e; A.K0 = K0; A.u = u; A.y = y;
-    hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE_PLAIN>), dim3(d.ntiles), dim3(NTJ * NTI * NTK),
-                       0, s, A);
-    LAUNCHCHK();
-    return H8_OK;
+    return launch_op<T, MODE_PLAIN>(d, A, s);
 }
 
 extern "C" int h8_spmv_f64(const h8_grid* g, const double* s_e, const double* K0,
@@ -248,6 +278,7 @@ static char* carve(char*& p, size_t bytes) {
 // ---------------------------------------------------------------------------
 struct GraphKey {
     int kind;
+    int variant;
     int dev;
     long long nx, ny, nz;
     const void* ptr[12];
@@ -314,8 +345,8 @@ static int launch_iteration(const GridDev& d, const T* s_e, const T* K0, const T
     NodeArgs<T> A{};
     A.g = d; A.s_e = s_e; A.K0 = K0; A.r = r; A.dinv = dinv; A.p0 = p0; A.p1 = p1; A.q = q;
     A.st = st; A.partials = partials;
-    hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE_DIR>), dim3(d.ntiles), dim3(NTJ * NTI * NTK), 0, s, A);
-    LAUNCHCHK();
+    int rc = launch_op<T, MODE_DIR>(d, A, s);
+    if (rc) return rc;
     hipLaunchKernelGGL((k_pcg_update<T, REFINED>), dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s,
                        d.ndof, x, r, (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials);
     LAUNCHCHK();
@@ -420,6 +451,7 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
     if (hst.status == ST_RUNNING && maxiter > 0) {
         GraphKey key{};
         key.kind = 1;
+        key.variant = d.variant;
         HIPCHK(hipGetDevice(&key.dev));
         key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
         const void* ptrs[] = {s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
@@ -516,15 +548,15 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     hst.true_rel = hst.rel;
     if (hst.status == ST_RUNNING) {
         // first residual (u = 0): r32 = float(mask f), rz_old, rmax
-        hipLaunchKernelGGL((k_node<double, NTJ, NTI, NTK, MODE_RESID>), dim3(d.ntiles),
-                           dim3(NTJ * NTI * NTK), 0, s, R);
-        LAUNCHCHK();
+        rc = launch_op<double, MODE_RESID>(d, R, s);
+        if (rc) return rc;
         ++resid_calls;
         HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
     GraphKey key{};
     key.kind = 2;
+    key.variant = d.variant;
     HIPCHK(hipGetDevice(&key.dev));
     key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
     const void* ptrs[] = {s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
@@ -538,9 +570,8 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
         // fold the FP32 iterate into u, then replace the residual in FP64
         hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, w.x);
         LAUNCHCHK();
-        hipLaunchKernelGGL((k_node<double, NTJ, NTI, NTK, MODE_RESID>), dim3(d.ntiles),
-                           dim3(NTJ * NTI * NTK), 0, s, R);
-        LAUNCHCHK();
+        rc = launch_op<double, MODE_RESID>(d, R, s);
+        if (rc) return rc;
         ++resid_calls;
         HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -601,9 +632,10 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
     hipEvent_t ev[4];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(ev[0], s));
-    for (int i = 0; i < reps; ++i)
-        hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE_DIR>), dim3(d.ntiles), dim3(NTJ * NTI * NTK), 0, s, A);
-    LAUNCHCHK();
+    for (int i = 0; i < reps; ++i) {
+        int rc = launch_op<T, MODE_DIR>(d, A, s);
+        if (rc) return rc;
+    }
     HIPCHK(hipEventRecord(ev[1], s));
     HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(ev[2], s));

@@ -47,6 +47,46 @@ struct NodeArgs {
     double* partials;
 };
 
+// Last block of a direction kernel: alpha = rz_old / p.Ap, or stop on p.Ap <= 0
+// (fe/iterative_solvers.py:449-454).
+__device__ inline void dir_finish(PcgState* st, double pq) {
+    st->pq = pq;
+    if (!(pq > 0.0)) {
+        st->bad_pq = pq;
+        st->iters = st->k;
+        st->status = ST_NOT_SPD;
+    } else {
+        st->alpha = st->rz_old / pq;
+    }
+}
+
+// Last block of an FP64 residual replacement: converged on the true residual,
+// out of iterations, or continue with the replaced residual keeping the search
+// direction (beta from the new r.z).
+__device__ inline void resid_finish(PcgState* st, double rr, double rz) {
+    const double rn = sqrt(rr);
+    st->true_rel = rn / st->bnorm;
+    st->outer += 1;
+    if (st->true_rel < st->tol) {
+        st->status = ST_DONE;
+        st->iters = st->k - 1;
+        st->rel = st->true_rel;
+    } else if (st->k - 1 >= st->maxiter) {
+        st->status = ST_MAXITER;
+        st->iters = st->k - 1;
+    } else {
+        st->rmax = rn;
+        if (st->first) {
+            st->rz_old = rz;
+        } else {
+            st->beta = rz / st->rz_old;
+            st->rz_old = rz;
+        }
+        st->rel = st->true_rel;
+        st->status = ST_RUNNING;
+    }
+}
+
 template <typename T, int TJ, int TI, int TK, int MODE>
 __global__ void __launch_bounds__(TJ * TI * TK)
 k_node(NodeArgs<T> A) {
@@ -174,47 +214,12 @@ k_node(NodeArgs<T> A) {
     if constexpr (MODE == MODE_DIR) {
         double v[1] = {part[0]}, tot[1];
         if (reduce_last_block<1>(v, A.partials, &A.st->cnt[0], tot, sRed)) {
-            if (threadIdx.x == 0) {
-                PcgState* st = A.st;
-                const double pq = tot[0];
-                st->pq = pq;
-                if (!(pq > 0.0)) {
-                    st->bad_pq = pq;
-                    st->iters = st->k;
-                    st->status = ST_NOT_SPD;
-                } else {
-                    st->alpha = st->rz_old / pq;
-                }
-            }
+            if (threadIdx.x == 0) dir_finish(A.st, tot[0]);
         }
     } else if constexpr (MODE == MODE_RESID) {
         double v[2] = {part[0], part[1]}, tot[2];
         if (reduce_last_block<2>(v, A.partials, &A.st->cnt[2], tot, sRed)) {
-            if (threadIdx.x == 0) {
-                PcgState* st = A.st;
-                const double rn = sqrt(tot[0]);
-                st->true_rel = rn / st->bnorm;
-                st->outer += 1;
-                if (st->true_rel < st->tol) {
-                    st->status = ST_DONE;
-                    st->iters = st->k - 1;
-                    st->rel = st->true_rel;
-                } else if (st->k - 1 >= st->maxiter) {
-                    st->status = ST_MAXITER;
-                    st->iters = st->k - 1;
-                } else {
-                    st->rmax = rn;
-                    // keep the search direction: beta from the replaced residual
-                    if (st->first) {
-                        st->rz_old = tot[1];
-                    } else {
-                        st->beta = tot[1] / st->rz_old;
-                        st->rz_old = tot[1];
-                    }
-                    st->rel = st->true_rel;
-                    st->status = ST_RUNNING;
-                }
-            }
+            if (threadIdx.x == 0) resid_finish(A.st, tot[0], tot[1]);
         }
     }
 }

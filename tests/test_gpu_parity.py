@@ -57,10 +57,15 @@ def test_spmv_f64_f32_match_oracle_csr(h8, gpu, dims):
     K = o.assemble_csr(nodes, elems, o.d_solid3d(1.0, 0.3), scale=s)
     u = rng.normal(size=K.shape[0])
     ref = K @ u
-    y = op.matvec(torch.as_tensor(u, device=gpu)).cpu().numpy()
-    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
-    y32 = op.matvec32(torch.as_tensor(u, device=gpu)).cpu().numpy()
-    assert np.abs(y32 - ref).max() <= 2e-5 * np.abs(ref).max()
+    for variant in (1, 2):          # node-centric VALU, element-batched MFMA
+        op.grid.variant = variant
+        y = op.matvec(torch.as_tensor(u, device=gpu)).cpu().numpy()
+        e64 = np.abs(y - ref).max() / np.abs(ref).max()
+        y32 = op.matvec32(torch.as_tensor(u, device=gpu)).cpu().numpy()
+        e32 = np.abs(y32 - ref).max() / np.abs(ref).max()
+        say("variant", variant, "err64", e64, "err32", e32)
+        assert e64 <= 1e-12
+        assert e32 <= 2e-5
 
 
 def _fesys(h8, nx, ny, nz, E=210e9, E_elem=None, load="facet"):
@@ -95,6 +100,23 @@ def test_config1_pcg_matches_reference_and_spsolve(h8, gpu, golden):
     assert np.linalg.norm(np.asarray(U) - golden[g + "/U"]) / np.linalg.norm(U_direct) < 1e-8
     # fixed DOFs exactly zero
     assert np.all(np.asarray(U)[fs.fixed_dofs_array] == 0.0)
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_pcg_variants_agree(h8, gpu, variant):
+    say("pcg variant", variant)
+    rng = np.random.default_rng(4)
+    nx, ny, nz = 16, 5, 4
+    E = 10 ** rng.uniform(9, 11, nx * ny * nz)
+    fs = _fesys(h8, nx, ny, nz, E_elem=E)
+    fs.operator.grid.variant = variant
+    c = o.cantilever(nx, ny, nz, E_elem=E)
+    Ud = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
+    for prec in ("fp64", "mixed"):
+        U = np.asarray(fs.solve_static(tol=1e-9, precision=prec))
+        err = np.linalg.norm(U - Ud) / np.linalg.norm(Ud)
+        say(prec, fs.last_solve_stats["iterations"], err)
+        assert err < 1e-6
 
 
 def test_nodal_and_heterogeneous_golden(h8, gpu, golden):
