@@ -95,7 +95,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--dims", default="128,32,32")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--delta", type=float, default=0.5)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); the multi-GPU "
@@ -187,7 +187,7 @@ def main():
                    "n_free": n_free, "elements": m, "delta": a.delta},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_node<float,8,8,4,MODE_DIR> (fused direction + K*p + p.q)",
+                     "kernel": "fused direction + K*p + p.q (FP32, default variant)",
                      "kernel_avg_us": dir_ms * 1e3, "bytes_per_launch": dir_bytes,
                      "update_kernel_avg_us": upd_ms * 1e3, "update_bytes_per_launch": upd_bytes,
                      "update_frac": upd_bytes / (upd_ms * 1e-3) / HBM_PEAK,

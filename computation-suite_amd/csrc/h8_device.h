@@ -95,6 +95,13 @@ __device__ __forceinline__ double ld_relaxed(const double* p) {
 
 // Every block contributes v[NV]; returns true in the last block to finish, in
 // which `tot` holds the sum over blocks (ordered by block index).
+//
+// Cross-block visibility without L2 maintenance: the partials are written and
+// read with device-scope (sc1) accesses, which bypass the per-XCD L2, and the
+// writer waits for its stores to complete (workgroup-scope release = wait for
+// the outstanding stores, no cache write-back) before taking a ticket with a
+// device-scope atomic.  A device-scope __threadfence() here would write back
+// and invalidate the whole L2 of the XCD in every block.
 template <int NV>
 __device__ bool reduce_last_block(double (&v)[NV], double* partials, unsigned* counter,
                                   double (&tot)[NV], double* sh) {
@@ -104,25 +111,45 @@ __device__ bool reduce_last_block(double (&v)[NV], double* partials, unsigned* c
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) st_relaxed(&partials[(size_t)blockIdx.x * NV + i], v[i]);
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (t == nb - 1);
     }
     __syncthreads();
     if (!s_last) return false;
-    __threadfence();
+    // issue the device-scope loads in groups of 8 per thread before summing,
+    // so their latencies overlap (fixed order: b = t, t+nt, ...)
     double acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) {
+    const unsigned nt = blockDim.x;
+    for (unsigned b0 = threadIdx.x; b0 < nb; b0 += 8 * nt) {
+        double buf[8][NV];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] += ld_relaxed(&partials[(size_t)b * NV + i]);
+        for (int u = 0; u < 8; ++u) {
+            const unsigned b = b0 + u * nt;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) buf[u][i] = b < nb ? ld_relaxed(&partials[(size_t)b * NV + i]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[i] += buf[u][i];
     }
     block_sum<NV>(acc, sh);
 #pragma unroll
     for (int i = 0; i < NV; ++i) tot[i] = acc[i];
     if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
+}
+
+// Read-only, wave-uniform tables (the 24x24 K0): accessed through the constant
+// address space so the compiler emits scalar (s_load) reads shared by the wave
+// instead of one vector load per lane.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
 }
 
 __device__ __forceinline__ void tile_origin(const GridDev& g, int tile, int TJ, int TI, int TK,

@@ -14,7 +14,8 @@
 // as the node-centric kernel (h8_node.h) are provided.
 //
 // MFMA operand layout used (16x16x4, wave64): lane l supplies A[l%16][l/16]
-// and B[l/16][l%16] and receives D[4*(l/16)+r][l%16], r = 0..3.  Every LDS and
+// and B[l/16][l%16]; it receives D[4*(l/16)+r][l%16] (f32) or
+// D[4*r+l/16][l%16] (f64), r = 0..3 (checked with a lane-layout probe).  Every LDS and
 // global index below is formed from in-range loop counters, so any layout
 // mismatch would show up as wrong values in the parity tests, never as an
 // out-of-bounds access.
@@ -28,14 +29,17 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Mfma;
+// Output row held by (lane, r) of a 16x16x4 MFMA (column = lane % 16).
 template <> struct Mfma<float> {
     typedef f32x4 acc_t;
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
     static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
 };
 template <> struct Mfma<double> {
     typedef f64x4 acc_t;
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
     static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -142,14 +146,16 @@ k_mfma(NodeArgs<T> A) {
             for (int s = 0; s < 6; ++s) {
                 const int kdof = 4 * s + (lane >> 4);   // element DOF 0..23
                 const int a = kdof / 3, d = kdof - 3 * (kdof / 3);
-                const T bv = sP[(d * HK + ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a)];
+                // offset by the element's first corner (K0 annihilates translations)
+                const T bv = sP[(d * HK + ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a)] -
+                             sP[(d * HK + ek) * HI * HJ + ei * HJ + ej];
                 acc0 = Mfma<T>::run(a_frag[0][s], bv, acc0);
                 acc1 = Mfma<T>::run(a_frag[1][s], bv, acc1);
             }
             if (valid) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row0 = 4 * (lane >> 4) + r;         // rows 0..15
+                    const int row0 = Mfma<T>::row(lane, r);        // rows 0..15
                     {
                         const int a = row0 / 3, c = row0 - 3 * (row0 / 3);
                         const int li = (c * HK + ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a);

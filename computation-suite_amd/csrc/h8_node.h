@@ -10,7 +10,9 @@
 // stages the brick plus a one-node halo (component-planar, so lane-adjacent
 // reads are unit stride) and the (TJ+1)(TI+1)(TK+1) element scales in LDS,
 // then each thread keeps its 27-node neighbourhood in registers.  K0 is read
-// with uniform (scalar) loads.
+// with uniform (scalar) loads; only its first three rows are used (the brick
+// reflection structure gives every other block), so K0 must have that
+// structure exactly (GridOperator imposes it).
 //
 // MODE_PLAIN : y = K u (every DOF).
 // MODE_DIR   : PCG direction + product, fused: p = dinv*r (+ beta*p_old) is
@@ -46,6 +48,18 @@ struct NodeArgs {
     PcgState* st;
     double* partials;
 };
+
+// Corner whose offsets are the XOR of corners a and b, and the sign of
+// component c under the reflection that takes corner a to corner 0.
+__host__ __device__ constexpr int corner_of(int ox, int oy, int oz) {
+    return oz * 4 + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
+}
+__host__ __device__ constexpr int corner_xor(int a, int b) {
+    return corner_of(OX(a) ^ OX(b), OY(a) ^ OY(b), OZ(a) ^ OZ(b));
+}
+__host__ __device__ constexpr int refl_sign(int a, int c) {
+    return ((c == 0 ? OX(a) : (c == 1 ? OY(a) : OZ(a))) != 0) ? -1 : 1;
+}
 
 // Last block of a direction kernel: alpha = rz_old / p.Ap, or stop on p.Ap <= 0
 // (fe/iterative_solvers.py:449-454).
@@ -159,6 +173,7 @@ k_node(NodeArgs<T> A) {
     const bool owned = (j < g.NJ && i < g.NI && k < g.NK);
 
     double part[2] = {0.0, 0.0};
+    const auto K0c = as_const(A.K0);
     if (owned) {
         // 27-node neighbourhood, 3 components each
         T nb[3][3][3][3];
@@ -171,22 +186,49 @@ k_node(NodeArgs<T> A) {
 #pragma unroll
                     for (int d = 0; d < 3; ++d)
                         nb[dz][dx][dy][d] = sP[(d * HK + tk + dz) * HI * HJ + (ti + dx) * HJ + (tj + dy)];
-        T qv[3] = {T(0), T(0), T(0)};
+        // Contract (u_b - u_n) instead of u_b: K0 annihilates rigid translations,
+        // so the product is unchanged, but rounding now scales with the local
+        // variation of u instead of its magnitude (matters in FP32).
+        T ctr[3];
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
-            T acc[3] = {T(0), T(0), T(0)};
+        for (int d = 0; d < 3; ++d) ctr[d] = nb[1][1][1][d];
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
+        for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
+            for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        acc[c] += A.K0[(3 * a + c) * 24 + 3 * b + d] * nb[dz][dx][dy][d];
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) nb[dz][dx][dy][d] -= ctr[d];
+        // Brick symmetry: K0 block (a,b) = R_a K0 block (0, a^b) R_a with R_a the
+        // reflection taking corner a to corner 0, so the 64 blocks come from
+        // the first three rows of K0 (72 values) with compile-time signs.
+        // Output component c is the outer loop: only 24 table values are live
+        // at a time (scalar registers).
+        T qv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T q = T(0);
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
+                T acc = T(0);
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
+                    const int x = corner_xor(a, b);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const T kv = K0c[c * 24 + 3 * x + d];
+                        if (refl_sign(a, c) * refl_sign(a, d) > 0)
+                            acc += kv * nb[dz][dx][dy][d];
+                        else
+                            acc -= kv * nb[dz][dx][dy][d];
+                    }
+                }
+                q += s * acc;
             }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) qv[c] += s * acc[c];
+            qv[c] = q;
         }
         const long long gn = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j);
         if constexpr (MODE == MODE_PLAIN) {
@@ -196,7 +238,7 @@ k_node(NodeArgs<T> A) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 A.q[gn + c] = qv[c];
-                part[0] += double(nb[1][1][1][c]) * double(qv[c]);
+                part[0] += double(ctr[c]) * double(qv[c]);
             }
         } else {  // MODE_RESID (T = double)
 #pragma unroll

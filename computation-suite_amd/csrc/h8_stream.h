@@ -51,24 +51,43 @@ k_pcg_init(long long n, const T* __restrict__ b, const T* __restrict__ Ax0, cons
 // REFINED = true : additionally stops the FP32 loop for an FP64 residual
 //                  replacement when ||r|| < delta * rmax (or the recurrence
 //                  claims convergence, or maxiter is reached).
+// Vectors are read and written in 16-byte groups (2 doubles / 4 floats); the
+// workspace and torch allocations are 256-byte aligned, the tail is scalar.
 template <typename T, bool REFINED>
 __global__ void __launch_bounds__(STREAM_THREADS)
 k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p0,
              const T* __restrict__ p1, const T* __restrict__ q, const T* __restrict__ dinv, PcgState* st,
              double* partials) {
+    constexpr int V = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(V)));
     __shared__ double sRed[16 * 2];
     if (st->status != ST_RUNNING) return;
     const T* __restrict__ p = (st->k & 1) ? p1 : p0;
     const T alpha = T(st->alpha);
     double v[2] = {0.0, 0.0};
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
+    const long long nvec = n / V;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long iv = (long long)blockIdx.x * blockDim.x + threadIdx.x; iv < nvec; iv += gstride) {
+        const vec dv = reinterpret_cast<const vec*>(dinv)[iv];
+        vec xv = reinterpret_cast<const vec*>(x)[iv];
+        vec rv = reinterpret_cast<const vec*>(r)[iv];
+        const vec pv = reinterpret_cast<const vec*>(p)[iv];
+        const vec qv = reinterpret_cast<const vec*>(q)[iv];
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            xv[c] += alpha * pv[c];
+            const T ri = (dv[c] != T(0)) ? T(rv[c] - alpha * qv[c]) : T(0);
+            rv[c] = ri;
+            v[0] += double(ri) * double(dv[c] * ri);
+            v[1] += double(ri) * double(ri);
+        }
+        reinterpret_cast<vec*>(x)[iv] = xv;
+        reinterpret_cast<vec*>(r)[iv] = rv;
+    }
+    for (long long i = nvec * V + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gstride) {
         const T d = dinv[i];
-        T xi = x[i];
-        T ri = r[i];
-        xi += alpha * p[i];
-        ri = (d != T(0)) ? T(ri - alpha * q[i]) : T(0);
-        x[i] = xi;
+        x[i] += alpha * p[i];
+        const T ri = (d != T(0)) ? T(r[i] - alpha * q[i]) : T(0);
         r[i] = ri;
         v[0] += double(ri) * double(d * ri);
         v[1] += double(ri) * double(ri);
@@ -196,11 +215,12 @@ k_elem_bilinear(GridDev g, const double* __restrict__ K0, const double* __restri
             }
         }
         double acc = 0.0;
+        const auto K0c = as_const(K0);
 #pragma unroll
         for (int r = 0; r < 24; ++r) {
             double y = 0.0;
 #pragma unroll
-            for (int cc = 0; cc < 24; ++cc) y += K0[r * 24 + cc] * ve[cc];
+            for (int cc = 0; cc < 24; ++cc) y += K0c[r * 24 + cc] * ve[cc];
             acc += ue[r] * y;
         }
         out[e] = w[e] * acc;

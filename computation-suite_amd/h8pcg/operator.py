@@ -57,6 +57,31 @@ def element_stiffness_batched(coords, D, device="cuda"):
     return ke
 
 
+_OX = (0, 1, 1, 0, 0, 1, 1, 0)
+_OY = (0, 0, 1, 1, 0, 0, 1, 1)
+_OZ = (0, 0, 0, 0, 1, 1, 1, 1)
+
+
+def brick_structured(K0):
+    """K0 rebuilt from its first three rows with the brick element's exact
+    reflection symmetry, K0[3a+c, 3b+d] = s_c(a) s_d(a) T[a^b][c, d] with
+    T[x] = K0[0:3, 3x:3x+3] made to satisfy T[x] = R_x T[x]^T R_x (so the
+    result is exactly symmetric).  For an axis-aligned brick this changes K0 by
+    rounding only; the node-centric kernel reads just these 72 values."""
+    import torch
+    K0 = K0.to(torch.float64)
+    bits = torch.tensor([_OX, _OY, _OZ], dtype=torch.int64, device=K0.device).T      # (8,3)
+    sig = 1.0 - 2.0 * bits.to(torch.float64)                                          # (8,3) reflection signs
+    T = K0[0:3].reshape(3, 8, 3).permute(1, 0, 2)                                     # T[x][c][d]
+    T = 0.5 * (T + sig[:, :, None] * T.transpose(1, 2) * sig[:, None, :])
+    code = bits[:, 0] + 2 * bits[:, 1] + 4 * bits[:, 2]                              # corner -> bit code
+    inv = torch.empty(8, dtype=torch.int64, device=K0.device)
+    inv[code] = torch.arange(8, device=K0.device)                                     # bit code -> corner
+    x = inv[code[:, None] ^ code[None, :]]                                            # (8,8) corner a^b
+    blocks = sig[:, None, :, None] * T[x] * sig[:, None, None, :]                    # (a,b,c,d)
+    return blocks.permute(0, 2, 1, 3).reshape(24, 24).contiguous()
+
+
 class GridOperator:
     """K = sum_e s_e K0 on an nx x ny x nz box mesh, matrix-free on one GPU."""
 
@@ -68,9 +93,10 @@ class GridOperator:
         self.n_nodes = (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
         self.n_dof = 3 * self.n_nodes
         self.m = self.nx * self.ny * self.nz
-        self.K0 = torch.as_tensor(K0, dtype=torch.float64, device=self.device).contiguous()
-        if self.K0.shape != (24, 24):
+        K0 = torch.as_tensor(K0, dtype=torch.float64, device=self.device)
+        if K0.shape != (24, 24):
             raise ValueError("K0 must be 24x24")
+        self.K0 = brick_structured(K0)
         self.s_e = torch.as_tensor(s_e, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
         if self.s_e.numel() != self.m:
             raise ValueError(f"s_e has {self.s_e.numel()} entries, the grid has {self.m} elements")
@@ -176,7 +202,7 @@ class GridOperator:
         check(rc)
         return x, stats
 
-    def refined(self, f, tol=1e-8, maxiter=None, delta=0.1):
+    def refined(self, f, tol=1e-8, maxiter=None, delta=0.5):
         """FP32 PCG iterations with FP64 residual replacement; returns (u, stats)."""
         import torch
         if self.dinv is None:

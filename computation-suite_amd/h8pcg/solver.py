@@ -279,7 +279,7 @@ class FESystem:
         return op.masked(F_t), None
 
     def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner="jacobi",
-                            precision="fp64", delta=0.1, F=None):
+                            precision="fp64", delta=0.5, F=None):
         """solve_static, returning the full displacement as a device tensor (no host copy)."""
         self._require_stiffness("solve_static")
         if method is None:

@@ -15,6 +15,7 @@ def case(nx=12, ny=4, nz=4, seed=3, p=3.0, rmin=1e-9):
     rho = np.random.default_rng(seed).uniform(0.3, 1.0, len(elems))
     F = np.zeros(3 * len(nodes))
     tip = np.where((np.abs(nodes[:, 0] - nx) < 1e-9) & (np.abs(nodes[:, 1] - ny / 2) < 1e-9) & (nodes[:, 2] < 1e-9))[0]
+    assert len(tip) == 1
     F[3 * tip + 2] = -1.0
     clamp = o.nodes_on_plane(nodes, 0, 0.0)
     fixed = [3 * int(n) + c for n in clamp for c in range(3)]
@@ -62,7 +63,7 @@ def test_compliance_and_closed_form_gradient(gpu):
 def test_general_adjoint_displacement_objective(gpu):
     import torch
     from h8pcg import topopt
-    c = case(10, 3, 3, seed=5)
+    c = case(10, 4, 4, seed=5)
     pb = problem(c, gpu, tol=1e-12)
     rng = np.random.default_rng(1)
     w = rng.normal(size=len(c["F"]))
