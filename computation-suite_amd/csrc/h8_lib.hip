@@ -124,6 +124,8 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
 // FP64), at most 2048 blocks (the partials array holds one slot per block).
 static int stream_blocks(long long n) {
     long long b = (n + STREAM_THREADS * 8 - 1) / (STREAM_THREADS * 8);
+    // keep every CU busy: at least 2 blocks per CU when there is work for them
+    b = std::max(b, std::min(512LL, (n + STREAM_THREADS - 1) / STREAM_THREADS));
     return int(std::max(1LL, std::min(b, 2048LL)));
 }
 

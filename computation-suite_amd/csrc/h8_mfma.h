@@ -93,28 +93,7 @@ k_mfma(NodeArgs<T> A) {
         }
 
     // ---- stage the halo (same layout as the node kernel) and zero accumulators
-    constexpr int ROW = 3 * HJ;
-    for (int idx = threadIdx.x; idx < ROW * HI * HK; idx += NT) {
-        const int w = idx % ROW;
-        const int row = idx / ROW;
-        const int ii = row % HI, kk = row / HI;
-        const int jj = w / 3, c = w - 3 * (w / 3);
-        const int j = j0 - 1 + jj, i = i0 - 1 + ii, k = k0 - 1 + kk;
-        T val = T(0);
-        if (j >= 0 && j < g.NJ && i >= 0 && i < g.NI && k >= 0 && k < g.NK) {
-            const long long gi = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j) + c;
-            if constexpr (MODE == MODE_DIR) {
-                T z = A.dinv[gi] * A.r[gi];
-                val = first ? z : z + beta * p_old[gi];
-                if (jj >= 1 && jj <= TJ && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK) p_new[gi] = val;
-            } else {
-                val = A.u[gi];
-            }
-        }
-        const int li = (c * HK + kk) * HI * HJ + ii * HJ + jj;
-        sP[li] = val;
-        sA[li] = T(0);
-    }
+    stage_halo<T, TJ, TI, TK, MODE, NT>(A, j0, i0, k0, beta, first, p_old, p_new, sP, sA);
     for (int idx = threadIdx.x; idx < NE; idx += NT) {
         const int ej = idx % EJ, ei = (idx / EJ) % EI, ek = idx / (EJ * EI);
         const int je = j0 - 1 + ej, ie = i0 - 1 + ei, ke = k0 - 1 + ek;

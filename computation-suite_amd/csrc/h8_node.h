@@ -101,6 +101,63 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
     }
 }
 
+// Stage the input vector over the block's node brick plus a one-node halo into
+// LDS (component-planar [c][k][i][j]); rows of 3*HJ words are contiguous in
+// global memory.  All of a thread's global loads are issued before any is
+// consumed, so their latencies overlap.  MODE_DIR forms p = dinv*r + beta*p_old
+// on the fly and writes it for the owned nodes; sZero (if given) is cleared
+// over the same index range.
+template <typename T, int TJ, int TI, int TK, int MODE, int NT>
+__device__ __forceinline__ void stage_halo(const NodeArgs<T>& A, int j0, int i0, int k0, T beta, int first,
+                                           const T* p_old, T* p_new, T* sP, T* sZero) {
+    constexpr int HJ = TJ + 2, HI = TI + 2, HK = TK + 2;
+    const GridDev& g = A.g;
+    constexpr int ROW = 3 * HJ;
+    constexpr int NH = ROW * HI * HK;
+    constexpr int PER = (NH + NT - 1) / NT;
+    T v0[PER], v1[PER], v2[PER];
+    long long gidx[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int idx = threadIdx.x + u * NT;
+        const int w = idx % ROW;
+        const int row = idx / ROW;
+        const int ii = row % HI, kk = row / HI;
+        const int jj = w / 3, c = w - 3 * (w / 3);
+        const int j = j0 - 1 + jj, i = i0 - 1 + ii, k = k0 - 1 + kk;
+        const bool in = idx < NH && j >= 0 && j < g.NJ && i >= 0 && i < g.NI && k >= 0 && k < g.NK;
+        const long long gi = in ? 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j) + c : -1;
+        gidx[u] = gi;
+        if constexpr (MODE == MODE_DIR) {
+            v0[u] = in ? A.r[gi] : T(0);
+            v1[u] = in ? A.dinv[gi] : T(0);
+            v2[u] = (in && !first) ? p_old[gi] : T(0);
+        } else {
+            v0[u] = in ? A.u[gi] : T(0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int idx = threadIdx.x + u * NT;
+        if (idx >= NH) continue;
+        const int w = idx % ROW;
+        const int row = idx / ROW;
+        const int ii = row % HI, kk = row / HI;
+        const int jj = w / 3, c = w - 3 * (w / 3);
+        T val;
+        if constexpr (MODE == MODE_DIR) {
+            val = v1[u] * v0[u] + beta * v2[u];
+            if (gidx[u] >= 0 && jj >= 1 && jj <= TJ && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK)
+                p_new[gidx[u]] = val;
+        } else {
+            val = v0[u];
+        }
+        const int li = (c * HK + kk) * HI * HJ + ii * HJ + jj;
+        sP[li] = val;
+        if (sZero) sZero[li] = T(0);
+    }
+}
+
 template <typename T, int TJ, int TI, int TK, int MODE>
 __global__ void __launch_bounds__(TJ * TI * TK)
 k_node(NodeArgs<T> A) {
@@ -133,28 +190,7 @@ k_node(NodeArgs<T> A) {
         p_old = odd ? A.p0 : A.p1;
     }
 
-    // ---- stage the halo of the input vector (rows of 3*HJ contiguous words)
-    constexpr int ROW = 3 * HJ;
-    for (int idx = threadIdx.x; idx < ROW * HI * HK; idx += NT) {
-        const int w = idx % ROW;
-        const int row = idx / ROW;
-        const int ii = row % HI, kk = row / HI;
-        const int jj = w / 3, c = w - 3 * (w / 3);
-        const int j = j0 - 1 + jj, i = i0 - 1 + ii, k = k0 - 1 + kk;
-        T val = T(0);
-        if (j >= 0 && j < g.NJ && i >= 0 && i < g.NI && k >= 0 && k < g.NK) {
-            const long long gi = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j) + c;
-            if constexpr (MODE == MODE_DIR) {
-                T z = A.dinv[gi] * A.r[gi];
-                val = first ? z : z + beta * p_old[gi];
-                if (jj >= 1 && jj <= TJ && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK)
-                    p_new[gi] = val;
-            } else {
-                val = A.u[gi];
-            }
-        }
-        sP[(c * HK + kk) * HI * HJ + ii * HJ + jj] = val;
-    }
+    stage_halo<T, TJ, TI, TK, MODE, NT>(A, j0, i0, k0, beta, first, p_old, p_new, sP, (T*)nullptr);
     // ---- stage element scales (0 outside the mesh)
     for (int idx = threadIdx.x; idx < NE; idx += NT) {
         const int ej = idx % EJ, ei = (idx / EJ) % EI, ek = idx / (EJ * EI);
@@ -175,60 +211,55 @@ k_node(NodeArgs<T> A) {
     double part[2] = {0.0, 0.0};
     const auto K0c = as_const(A.K0);
     if (owned) {
-        // 27-node neighbourhood, 3 components each
-        T nb[3][3][3][3];
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        nb[dz][dx][dy][d] = sP[(d * HK + tk + dz) * HI * HJ + (ti + dx) * HJ + (tj + dy)];
         // Contract (u_b - u_n) instead of u_b: K0 annihilates rigid translations,
         // so the product is unchanged, but rounding now scales with the local
         // variation of u instead of its magnitude (matters in FP32).
-        T ctr[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) ctr[d] = nb[1][1][1][d];
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) nb[dz][dx][dy][d] -= ctr[d];
         // Brick symmetry: K0 block (a,b) = R_a K0 block (0, a^b) R_a with R_a the
         // reflection taking corner a to corner 0, so the 64 blocks come from
         // the first three rows of K0 (72 values) with compile-time signs.
-        // Output component c is the outer loop: only 24 table values are live
-        // at a time (scalar registers).
-        T qv[3];
+        // The input component d is the outer loop: 27 neighbour values and
+        // 24 table values are live at a time, accumulated per (corner, output).
+        const int hb = (tk + 1) * HI * HJ + (ti + 1) * HJ + (tj + 1);
+        T ctr[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            T q = T(0);
+        for (int d = 0; d < 3; ++d) ctr[d] = sP[d * HK * HI * HJ + hb];
+        T acc[8][3];
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
-                T acc = T(0);
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[a][c] = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T nb[3][3][3];
+#pragma unroll
+            for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy)
+                        nb[dz][dx][dy] = sP[d * HK * HI * HJ + hb + (dz - 1) * HI * HJ + (dx - 1) * HJ + (dy - 1)] - ctr[d];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
 #pragma unroll
                 for (int b = 0; b < 8; ++b) {
                     const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
                     const int x = corner_xor(a, b);
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
+                    for (int c = 0; c < 3; ++c) {
                         const T kv = K0c[c * 24 + 3 * x + d];
                         if (refl_sign(a, c) * refl_sign(a, d) > 0)
-                            acc += kv * nb[dz][dx][dy][d];
+                            acc[a][c] += kv * nb[dz][dx][dy];
                         else
-                            acc -= kv * nb[dz][dx][dy][d];
+                            acc[a][c] -= kv * nb[dz][dx][dy];
                     }
                 }
-                q += s * acc;
-            }
-            qv[c] = q;
+        }
+        T qv[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) qv[c] += s * acc[a][c];
         }
         const long long gn = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j);
         if constexpr (MODE == MODE_PLAIN) {

@@ -67,22 +67,35 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     double v[2] = {0.0, 0.0};
     const long long nvec = n / V;
     const long long gstride = (long long)gridDim.x * blockDim.x;
-    for (long long iv = (long long)blockIdx.x * blockDim.x + threadIdx.x; iv < nvec; iv += gstride) {
-        const vec dv = reinterpret_cast<const vec*>(dinv)[iv];
-        vec xv = reinterpret_cast<const vec*>(x)[iv];
-        vec rv = reinterpret_cast<const vec*>(r)[iv];
-        const vec pv = reinterpret_cast<const vec*>(p)[iv];
-        const vec qv = reinterpret_cast<const vec*>(q)[iv];
+    // two 16-byte groups per thread per pass, loads of both issued first
+    for (long long iv0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; iv0 < nvec; iv0 += 2 * gstride) {
+        vec dv[2], xv[2], rv[2], pv[2], qv[2];
+        bool ok[2];
 #pragma unroll
-        for (int c = 0; c < V; ++c) {
-            xv[c] += alpha * pv[c];
-            const T ri = (dv[c] != T(0)) ? T(rv[c] - alpha * qv[c]) : T(0);
-            rv[c] = ri;
-            v[0] += double(ri) * double(dv[c] * ri);
-            v[1] += double(ri) * double(ri);
+        for (int u = 0; u < 2; ++u) {
+            const long long iv = iv0 + u * gstride;
+            ok[u] = iv < nvec;
+            const long long j = ok[u] ? iv : iv0;
+            dv[u] = reinterpret_cast<const vec*>(dinv)[j];
+            xv[u] = reinterpret_cast<const vec*>(x)[j];
+            rv[u] = reinterpret_cast<const vec*>(r)[j];
+            pv[u] = reinterpret_cast<const vec*>(p)[j];
+            qv[u] = reinterpret_cast<const vec*>(q)[j];
         }
-        reinterpret_cast<vec*>(x)[iv] = xv;
-        reinterpret_cast<vec*>(r)[iv] = rv;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (!ok[u]) continue;
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                xv[u][c] += alpha * pv[u][c];
+                const T ri = (dv[u][c] != T(0)) ? T(rv[u][c] - alpha * qv[u][c]) : T(0);
+                rv[u][c] = ri;
+                v[0] += double(ri) * double(dv[u][c] * ri);
+                v[1] += double(ri) * double(ri);
+            }
+            reinterpret_cast<vec*>(x)[iv0 + u * gstride] = xv[u];
+            reinterpret_cast<vec*>(r)[iv0 + u * gstride] = rv[u];
+        }
     }
     for (long long i = nvec * V + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gstride) {
         const T d = dinv[i];
