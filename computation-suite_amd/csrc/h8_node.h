@@ -239,20 +239,23 @@ k_node(NodeArgs<T> A) {
                     for (int dy = 0; dy < 3; ++dy)
                         nb[dz][dx][dy] = sP[d * HK * HI * HJ + hb + (dz - 1) * HI * HJ + (dx - 1) * HJ + (dy - 1)] - ctr[d];
 #pragma unroll
-            for (int a = 0; a < 8; ++a)
+            for (int c = 0; c < 3; ++c) {
+                // the 8 table values of (c, d): K0[c][3x + d], x = 0..7
+                T kt[8];
 #pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
-                    const int x = corner_xor(a, b);
+                for (int x = 0; x < 8; ++x) kt[x] = K0c[c * 24 + 3 * x + d];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const T kv = K0c[c * 24 + 3 * x + d];
+                for (int a = 0; a < 8; ++a)
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
+                        const T kv = kt[corner_xor(a, b)];
                         if (refl_sign(a, c) * refl_sign(a, d) > 0)
                             acc[a][c] += kv * nb[dz][dx][dy];
                         else
                             acc[a][c] -= kv * nb[dz][dx][dy];
                     }
-                }
+            }
         }
         T qv[3] = {T(0), T(0), T(0)};
 #pragma unroll
