@@ -92,7 +92,8 @@ static int make_grid(const h8_grid* g, GridDev& d) {
     d.ti = (d.NI + NTI - 1) / NTI;
     d.tk = (d.NK + NTK - 1) / NTK;
     long long nt = (long long)d.tj * d.ti * d.tk;
-    if (nt > (1LL << 30)) return fail(H8_ERR_ARG, "grid too large");
+    if (nt > (1LL << 30) || d.ndof >= (1LL << 31))
+        return fail(H8_ERR_ARG, "grid too large: %lld DOFs (the kernels index with 32-bit integers)", d.ndof);
     d.ntiles = int(nt);
     if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_MFMA)
         return fail(H8_ERR_ARG, "unknown kernel variant %d", g->variant);

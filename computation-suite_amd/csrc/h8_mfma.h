@@ -126,8 +126,8 @@ k_mfma(NodeArgs<T> A) {
                 const int kdof = 4 * s + (lane >> 4);   // element DOF 0..23
                 const int a = kdof / 3, d = kdof - 3 * (kdof / 3);
                 // offset by the element's first corner (K0 annihilates translations)
-                const T bv = sP[(d * HK + ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a)] -
-                             sP[(d * HK + ek) * HI * HJ + ei * HJ + ej];
+                const T bv = sP[3 * ((ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a)) + d] -
+                             sP[3 * (ek * HI * HJ + ei * HJ + ej) + d];
                 acc0 = Mfma<T>::run(a_frag[0][s], bv, acc0);
                 acc1 = Mfma<T>::run(a_frag[1][s], bv, acc1);
             }
@@ -137,13 +137,13 @@ k_mfma(NodeArgs<T> A) {
                     const int row0 = Mfma<T>::row(lane, r);        // rows 0..15
                     {
                         const int a = row0 / 3, c = row0 - 3 * (row0 / 3);
-                        const int li = (c * HK + ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a);
+                        const int li = 3 * ((ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a)) + c;
                         sA[li] += se * acc0[r];
                     }
                     const int row1 = 16 + row0;                    // rows 16..31 (valid < 24)
                     if (row1 < 24) {
                         const int a = row1 / 3, c = row1 - 3 * (row1 / 3);
-                        const int li = (c * HK + ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a);
+                        const int li = 3 * ((ek + OZ(a)) * HI * HJ + (ei + OX(a)) * HJ + ej + OY(a)) + c;
                         sA[li] += se * acc1[r];
                     }
                 }
@@ -162,7 +162,7 @@ k_mfma(NodeArgs<T> A) {
         const long long gn = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const int li = (c * HK + tk + 1) * HI * HJ + (ti + 1) * HJ + tj + 1;
+            const int li = 3 * ((tk + 1) * HI * HJ + (ti + 1) * HJ + tj + 1) + c;
             const T qv = sA[li];
             if constexpr (MODE == MODE_PLAIN) {
                 A.y[gn + c] = qv;

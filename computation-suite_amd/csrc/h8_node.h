@@ -102,31 +102,41 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
 }
 
 // Stage the input vector over the block's node brick plus a one-node halo into
-// LDS (component-planar [c][k][i][j]); rows of 3*HJ words are contiguous in
-// global memory.  All of a thread's global loads are issued before any is
-// consumed, so their latencies overlap.  MODE_DIR forms p = dinv*r + beta*p_old
-// on the fly and writes it for the owned nodes; sZero (if given) is cleared
-// over the same index range.
+// LDS, node-interleaved ([k][i][j][c], the global DOF order, so a staged row is
+// one contiguous LDS row and lane-adjacent nodes sit 3 words apart).  A halo row (fixed i, k) is 3*HJ
+// contiguous words in global memory; each thread owns one word position w of
+// a 32- or 64-lane row slot (so j, c and the row-invariant tests are computed
+// once) and walks rows in passes of NT/LW.  All of a thread's global loads are
+// issued before any is consumed, so their latencies overlap.  MODE_DIR forms
+// p = dinv*r + beta*p_old on the fly and writes it for the owned nodes; sZero
+// (if given) is cleared over the same LDS range.  Indices are 32-bit (the
+// library rejects grids with 2^31 or more DOFs).
 template <typename T, int TJ, int TI, int TK, int MODE, int NT>
 __device__ __forceinline__ void stage_halo(const NodeArgs<T>& A, int j0, int i0, int k0, T beta, int first,
                                            const T* p_old, T* p_new, T* sP, T* sZero) {
     constexpr int HJ = TJ + 2, HI = TI + 2, HK = TK + 2;
-    const GridDev& g = A.g;
     constexpr int ROW = 3 * HJ;
-    constexpr int NH = ROW * HI * HK;
-    constexpr int PER = (NH + NT - 1) / NT;
+    static_assert(ROW <= 64, "halo row longer than 64 words");
+    constexpr int LW = ROW <= 32 ? 32 : 64;
+    constexpr int RPP = NT / LW;
+    constexpr int NROW = HI * HK;
+    constexpr int PER = (NROW + RPP - 1) / RPP;
+    const GridDev& g = A.g;
+    const int w = threadIdx.x % LW, r0 = threadIdx.x / LW;
+    const int jj = w / 3, c = w - 3 * (w / 3);
+    const int j = j0 - 1 + jj;
+    const bool jok = (w < ROW) && j >= 0 && j < g.NJ;
+    const bool jown = jj >= 1 && jj <= TJ;
+    const int plane = g.NI * g.NJ;
     T v0[PER], v1[PER], v2[PER];
-    long long gidx[PER];
+    int gidx[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int idx = threadIdx.x + u * NT;
-        const int w = idx % ROW;
-        const int row = idx / ROW;
+        const int row = r0 + u * RPP;
         const int ii = row % HI, kk = row / HI;
-        const int jj = w / 3, c = w - 3 * (w / 3);
-        const int j = j0 - 1 + jj, i = i0 - 1 + ii, k = k0 - 1 + kk;
-        const bool in = idx < NH && j >= 0 && j < g.NJ && i >= 0 && i < g.NI && k >= 0 && k < g.NK;
-        const long long gi = in ? 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j) + c : -1;
+        const int i = i0 - 1 + ii, k = k0 - 1 + kk;
+        const bool in = jok && row < NROW && i >= 0 && i < g.NI && k >= 0 && k < g.NK;
+        const int gi = in ? 3 * (k * plane + i * g.NJ + j) + c : -1;
         gidx[u] = gi;
         if constexpr (MODE == MODE_DIR) {
             v0[u] = in ? A.r[gi] : T(0);
@@ -138,21 +148,17 @@ __device__ __forceinline__ void stage_halo(const NodeArgs<T>& A, int j0, int i0,
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int idx = threadIdx.x + u * NT;
-        if (idx >= NH) continue;
-        const int w = idx % ROW;
-        const int row = idx / ROW;
+        const int row = r0 + u * RPP;
+        if (w >= ROW || row >= NROW) continue;
         const int ii = row % HI, kk = row / HI;
-        const int jj = w / 3, c = w - 3 * (w / 3);
         T val;
         if constexpr (MODE == MODE_DIR) {
             val = v1[u] * v0[u] + beta * v2[u];
-            if (gidx[u] >= 0 && jj >= 1 && jj <= TJ && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK)
-                p_new[gidx[u]] = val;
+            if (gidx[u] >= 0 && jown && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK) p_new[gidx[u]] = val;
         } else {
             val = v0[u];
         }
-        const int li = (c * HK + kk) * HI * HJ + ii * HJ + jj;
+        const int li = (kk * HI + ii) * ROW + w;
         sP[li] = val;
         if (sZero) sZero[li] = T(0);
     }
@@ -219,10 +225,10 @@ k_node(NodeArgs<T> A) {
         // the first three rows of K0 (72 values) with compile-time signs.
         // The input component d is the outer loop: 27 neighbour values and
         // 24 table values are live at a time, accumulated per (corner, output).
-        const int hb = (tk + 1) * HI * HJ + (ti + 1) * HJ + (tj + 1);
+        const int hb = 3 * ((tk + 1) * HI * HJ + (ti + 1) * HJ + (tj + 1));
         T ctr[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ctr[d] = sP[d * HK * HI * HJ + hb];
+        for (int d = 0; d < 3; ++d) ctr[d] = sP[hb + d];
         T acc[8][3];
 #pragma unroll
         for (int a = 0; a < 8; ++a)
@@ -237,7 +243,7 @@ k_node(NodeArgs<T> A) {
                 for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
                     for (int dy = 0; dy < 3; ++dy)
-                        nb[dz][dx][dy] = sP[d * HK * HI * HJ + hb + (dz - 1) * HI * HJ + (dx - 1) * HJ + (dy - 1)] - ctr[d];
+                        nb[dz][dx][dy] = sP[hb + 3 * ((dz - 1) * HI * HJ + (dx - 1) * HJ + (dy - 1)) + d] - ctr[d];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 // the 8 table values of (c, d): K0[c][3x + d], x = 0..7
@@ -248,6 +254,7 @@ k_node(NodeArgs<T> A) {
                 for (int a = 0; a < 8; ++a)
 #pragma unroll
                     for (int b = 0; b < 8; ++b) {
+                        if (b == a) continue;   // u_n - u_n = 0: the self block contributes nothing
                         const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
                         const T kv = kt[corner_xor(a, b)];
                         if (refl_sign(a, c) * refl_sign(a, d) > 0)
