@@ -77,7 +77,7 @@ template <> struct MfmaTile<float> { static constexpr int J = 16, I = 8, K = 8; 
 template <> struct MfmaTile<double> { static constexpr int J = 8, I = 8, K = 8; };
 
 template <typename T> static int default_variant();
-template <> int default_variant<float>() { return H8_VARIANT_NODE; }
+template <> int default_variant<float>() { return H8_VARIANT_NODE_PERSIST; }
 template <> int default_variant<double>() { return H8_VARIANT_NODE; }
 
 static int make_grid(const h8_grid* g, GridDev& d) {
@@ -95,10 +95,38 @@ static int make_grid(const h8_grid* g, GridDev& d) {
     if (nt > (1LL << 30) || d.ndof >= (1LL << 31))
         return fail(H8_ERR_ARG, "grid too large: %lld DOFs (the kernels index with 32-bit integers)", d.ndof);
     d.ntiles = int(nt);
-    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_MFMA)
+    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_NODE_PERSIST)
         return fail(H8_ERR_ARG, "unknown kernel variant %d", g->variant);
     d.variant = g->variant;
     return H8_OK;
+}
+
+// Persistent grid for a tile kernel: (resident blocks per CU) x CUs, capped at
+// the tile count and rounded down to a multiple of the XCD count so every XCD
+// gets the same number of blocks.
+static std::mutex g_occ_mu;
+static std::vector<std::pair<const void*, int>> g_occ;
+static int g_cus[64] = {};
+
+static int persistent_grid(const void* kernel, int threads, int ntiles) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return ntiles;
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    if (!g_cus[dev]) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) != hipSuccess) return ntiles;
+        g_cus[dev] = p.multiProcessorCount;
+    }
+    int per = 0;
+    for (auto& e : g_occ)
+        if (e.first == kernel) per = e.second;
+    if (!per) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+        g_occ.push_back({kernel, per});
+    }
+    long long gsz = std::min<long long>((long long)g_cus[dev] * per, ntiles);
+    if (gsz >= NXCD) gsz -= gsz % NXCD;
+    return int(std::max(1LL, gsz));
 }
 
 // Launch one structured operator kernel (mode MODE) with the variant recorded
@@ -115,7 +143,14 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     } else {
         set_tiles(d, NTJ, NTI, NTK);
         A.g = d;
-        hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE>), dim3(d.ntiles), dim3(NTJ * NTI * NTK), 0, s, A);
+        if (v == H8_VARIANT_NODE_PERSIST) {
+            const void* kfn = (const void*)k_node_persist<T, NTJ, NTI, NTK, MODE>;
+            const int grid = persistent_grid(kfn, NTJ * NTI * NTK, d.ntiles);
+            hipLaunchKernelGGL((k_node_persist<T, NTJ, NTI, NTK, MODE>), dim3(grid), dim3(NTJ * NTI * NTK), 0, s,
+                               A);
+        } else {
+            hipLaunchKernelGGL((k_node<T, NTJ, NTI, NTK, MODE>), dim3(d.ntiles), dim3(NTJ * NTI * NTK), 0, s, A);
+        }
     }
     LAUNCHCHK();
     return H8_OK;

@@ -92,16 +92,8 @@ k_mfma(NodeArgs<T> A) {
             a_frag[rb][s] = row < 24 ? A.K0[row * 24 + col] : T(0);
         }
 
-    // ---- stage the halo (same layout as the node kernel) and zero accumulators
-    stage_halo<T, TJ, TI, TK, MODE, NT>(A, j0, i0, k0, beta, first, p_old, p_new, sP, sA);
-    for (int idx = threadIdx.x; idx < NE; idx += NT) {
-        const int ej = idx % EJ, ei = (idx / EJ) % EI, ek = idx / (EJ * EI);
-        const int je = j0 - 1 + ej, ie = i0 - 1 + ei, ke = k0 - 1 + ek;
-        T s = T(0);
-        if (je >= 0 && je < g.ny && ie >= 0 && ie < g.nx && ke >= 0 && ke < g.nz)
-            s = A.s_e[(long long)ke * g.nx * g.ny + (long long)ie * g.ny + je];
-        sS[idx] = s;
-    }
+    // ---- stage the halo and element scales (same layout as the node kernel), zero accumulators
+    stage_halo<T, TJ, TI, TK, MODE, NT>(A, j0, i0, k0, beta, first, p_old, p_new, sP, sS, sA);
     __syncthreads();
 
     // ---- 8 parity colours; batches of 16 same-colour elements per MFMA pass

@@ -101,199 +101,211 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
     }
 }
 
-// Stage the input vector over the block's node brick plus a one-node halo into
-// LDS, node-interleaved ([k][i][j][c], the global DOF order, so a staged row is
-// one contiguous LDS row and lane-adjacent nodes sit 3 words apart).  A halo row (fixed i, k) is 3*HJ
-// contiguous words in global memory; each thread owns one word position w of
-// a 32- or 64-lane row slot (so j, c and the row-invariant tests are computed
-// once) and walks rows in passes of NT/LW.  All of a thread's global loads are
-// issued before any is consumed, so their latencies overlap.  MODE_DIR forms
-// p = dinv*r + beta*p_old on the fly and writes it for the owned nodes; sZero
-// (if given) is cleared over the same LDS range.  Indices are 32-bit (the
-// library rejects grids with 2^31 or more DOFs).
-template <typename T, int TJ, int TI, int TK, int MODE, int NT>
-__device__ __forceinline__ void stage_halo(const NodeArgs<T>& A, int j0, int i0, int k0, T beta, int first,
-                                           const T* p_old, T* p_new, T* sP, T* sZero) {
-    constexpr int HJ = TJ + 2, HI = TI + 2, HK = TK + 2;
-    constexpr int ROW = 3 * HJ;
+// Halo staging, split in two so a persistent block can issue the global loads
+// of its next tile before computing the current one.
+//
+// The input vector over the block's node brick plus a one-node halo goes to
+// LDS node-interleaved ([k][i][j][c], the global DOF order: a staged row is one
+// contiguous LDS row and lane-adjacent nodes sit 3 words apart).  A halo row
+// (fixed i, k) is 3*HJ contiguous words in global memory; each thread owns one
+// word position w of a 32- or 64-lane row slot (j, c and the row-invariant
+// tests are computed once) and walks the rows in passes of NT/LW.  MODE_DIR
+// forms p = dinv*r + beta*p_old and writes it for the owned nodes.  The
+// (TJ+1)(TI+1)(TK+1) element scales (0 outside the mesh) are staged the same
+// way.  Indices are 32-bit (the library rejects grids with 2^31 or more DOFs).
+template <typename T, int TJ, int TI, int TK, int NT>
+struct Halo {
+    static constexpr int HJ = TJ + 2, HI = TI + 2, HK = TK + 2;
+    static constexpr int ROW = 3 * HJ;
+    static constexpr int LW = ROW <= 32 ? 32 : 64;
+    static constexpr int RPP = NT / LW;
+    static constexpr int NROW = HI * HK;
+    static constexpr int PER = (NROW + RPP - 1) / RPP;
+    static constexpr int EJ = TJ + 1, EI = TI + 1, EK = TK + 1;
+    static constexpr int NE = EJ * EI * EK;
+    static constexpr int PES = (NE + NT - 1) / NT;
     static_assert(ROW <= 64, "halo row longer than 64 words");
-    constexpr int LW = ROW <= 32 ? 32 : 64;
-    constexpr int RPP = NT / LW;
-    constexpr int NROW = HI * HK;
-    constexpr int PER = (NROW + RPP - 1) / RPP;
-    const GridDev& g = A.g;
-    const int w = threadIdx.x % LW, r0 = threadIdx.x / LW;
-    const int jj = w / 3, c = w - 3 * (w / 3);
-    const int j = j0 - 1 + jj;
-    const bool jok = (w < ROW) && j >= 0 && j < g.NJ;
-    const bool jown = jj >= 1 && jj <= TJ;
-    const int plane = g.NI * g.NJ;
     T v0[PER], v1[PER], v2[PER];
     int gidx[PER];
+    T sv[PES];
+};
+
+template <typename T, int TJ, int TI, int TK, int NT, int MODE>
+__device__ __forceinline__ void halo_load(const NodeArgs<T>& A, int j0, int i0, int k0, int first,
+                                          const T* p_old, Halo<T, TJ, TI, TK, NT>& H) {
+    typedef Halo<T, TJ, TI, TK, NT> Hc;
+    const GridDev& g = A.g;
+    const int w = threadIdx.x % Hc::LW, r0 = threadIdx.x / Hc::LW;
+    const int jj = w / 3, c = w - 3 * (w / 3);
+    const int j = j0 - 1 + jj;
+    const bool jok = (w < Hc::ROW) && j >= 0 && j < g.NJ;
+    const int plane = g.NI * g.NJ;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int row = r0 + u * RPP;
-        const int ii = row % HI, kk = row / HI;
+    for (int u = 0; u < Hc::PER; ++u) {
+        const int row = r0 + u * Hc::RPP;
+        const int ii = row % Hc::HI, kk = row / Hc::HI;
         const int i = i0 - 1 + ii, k = k0 - 1 + kk;
-        const bool in = jok && row < NROW && i >= 0 && i < g.NI && k >= 0 && k < g.NK;
+        const bool in = jok && row < Hc::NROW && i >= 0 && i < g.NI && k >= 0 && k < g.NK;
         const int gi = in ? 3 * (k * plane + i * g.NJ + j) + c : -1;
-        gidx[u] = gi;
+        H.gidx[u] = gi;
         if constexpr (MODE == MODE_DIR) {
-            v0[u] = in ? A.r[gi] : T(0);
-            v1[u] = in ? A.dinv[gi] : T(0);
-            v2[u] = (in && !first) ? p_old[gi] : T(0);
+            H.v0[u] = in ? A.r[gi] : T(0);
+            H.v1[u] = in ? A.dinv[gi] : T(0);
+            H.v2[u] = (in && !first) ? p_old[gi] : T(0);
         } else {
-            v0[u] = in ? A.u[gi] : T(0);
+            H.v0[u] = in ? A.u[gi] : T(0);
         }
     }
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int row = r0 + u * RPP;
-        if (w >= ROW || row >= NROW) continue;
-        const int ii = row % HI, kk = row / HI;
-        T val;
-        if constexpr (MODE == MODE_DIR) {
-            val = v1[u] * v0[u] + beta * v2[u];
-            if (gidx[u] >= 0 && jown && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK) p_new[gidx[u]] = val;
-        } else {
-            val = v0[u];
-        }
-        const int li = (kk * HI + ii) * ROW + w;
-        sP[li] = val;
-        if (sZero) sZero[li] = T(0);
+    for (int u = 0; u < Hc::PES; ++u) {
+        const int idx = threadIdx.x + u * NT;
+        const int ej = idx % Hc::EJ, ei = (idx / Hc::EJ) % Hc::EI, ek = idx / (Hc::EJ * Hc::EI);
+        const int je = j0 - 1 + ej, ie = i0 - 1 + ei, ke = k0 - 1 + ek;
+        const bool in = idx < Hc::NE && je >= 0 && je < g.ny && ie >= 0 && ie < g.nx && ke >= 0 && ke < g.nz;
+        H.sv[u] = in ? A.s_e[(ke * g.nx + ie) * g.ny + je] : T(0);
     }
 }
 
+template <typename T, int TJ, int TI, int TK, int NT, int MODE>
+__device__ __forceinline__ void halo_store(const Halo<T, TJ, TI, TK, NT>& H, T beta, T* p_new, T* sP, T* sS,
+                                           T* sZero) {
+    typedef Halo<T, TJ, TI, TK, NT> Hc;
+    const int w = threadIdx.x % Hc::LW, r0 = threadIdx.x / Hc::LW;
+    const int jj = w / 3;
+    const bool jown = jj >= 1 && jj <= TJ;
+#pragma unroll
+    for (int u = 0; u < Hc::PER; ++u) {
+        const int row = r0 + u * Hc::RPP;
+        if (w >= Hc::ROW || row >= Hc::NROW) continue;
+        const int ii = row % Hc::HI, kk = row / Hc::HI;
+        T val;
+        if constexpr (MODE == MODE_DIR) {
+            val = H.v1[u] * H.v0[u] + beta * H.v2[u];
+            if (H.gidx[u] >= 0 && jown && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK) p_new[H.gidx[u]] = val;
+        } else {
+            val = H.v0[u];
+        }
+        const int li = row * Hc::ROW + w;
+        sP[li] = val;
+        if (sZero) sZero[li] = T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < Hc::PES; ++u) {
+        const int idx = threadIdx.x + u * NT;
+        if (idx < Hc::NE) sS[idx] = H.sv[u];
+    }
+}
+
+// Tile range of a persistent block: the tiles are split into NXCD contiguous
+// ranges, one per XCD (blocks are dispatched to the XCDs round-robin), so the
+// blocks sharing an L2 work on neighbouring bricks and reuse each other's
+// halos; inside a range the blocks of that XCD take every nloc-th tile.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void tile_range(int ntiles, int& first_tile, int& end_tile, int& step) {
+    const int nx = (gridDim.x % NXCD == 0) ? NXCD : 1;
+    const int xcd = blockIdx.x % nx, loc = blockIdx.x / nx, nloc = gridDim.x / nx;
+    const int b = int((long long)ntiles * xcd / nx), e = int((long long)ntiles * (xcd + 1) / nx);
+    first_tile = b + loc;
+    end_tile = e;
+    step = nloc;
+}
+
+// Node-centric K*u over the tiles of a persistent block (grid <= tiles).  With
+// PF the next brick's global loads are issued before the current brick is
+// computed (more registers, fewer exposed latencies).
+// One brick of the node-centric K*u: stage (unless PF staged it already),
+// compute the owned nodes, write the outputs, accumulate the fused partials.
 template <typename T, int TJ, int TI, int TK, int MODE>
-__global__ void __launch_bounds__(TJ * TI * TK)
-k_node(NodeArgs<T> A) {
+__device__ __forceinline__ void node_brick(const NodeArgs<T>& A, int j0, int i0, int k0, const T* kp, T* sP,
+                                           const T* sS, double (&part)[2]) {
     constexpr int NT = TJ * TI * TK;
-    constexpr int HJ = TJ + 2, HI = TI + 2, HK = TK + 2;
-    constexpr int HPL = HJ * HI * HK;             // halo nodes
-    constexpr int EJ = TJ + 1, EI = TI + 1, EK = TK + 1;
-    constexpr int NE = EJ * EI * EK;
-
-    __shared__ T sP[3 * HPL];
-    __shared__ T sS[NE];
-    __shared__ double sRed[16 * 4];
-
+    typedef Halo<T, TJ, TI, TK, NT> Hc;
+    constexpr int HJ = Hc::HJ, HI = Hc::HI;
+    constexpr int EJ = Hc::EJ, EI = Hc::EI;
     const GridDev& g = A.g;
-    if constexpr (MODE == MODE_DIR) {
-        if (A.st->status != ST_RUNNING) return;
-    }
-    int j0, i0, k0;
-    tile_origin(g, blockIdx.x, TJ, TI, TK, j0, i0, k0);
-
-    T beta = T(0);
-    int first = 1;
-    T* p_new = nullptr;
-    const T* p_old = nullptr;
-    if constexpr (MODE == MODE_DIR) {
-        beta = T(A.st->beta);
-        first = A.st->first;
-        const bool odd = (A.st->k & 1) != 0;
-        p_new = odd ? A.p1 : A.p0;
-        p_old = odd ? A.p0 : A.p1;
-    }
-
-    stage_halo<T, TJ, TI, TK, MODE, NT>(A, j0, i0, k0, beta, first, p_old, p_new, sP, (T*)nullptr);
-    // ---- stage element scales (0 outside the mesh)
-    for (int idx = threadIdx.x; idx < NE; idx += NT) {
-        const int ej = idx % EJ, ei = (idx / EJ) % EI, ek = idx / (EJ * EI);
-        const int je = j0 - 1 + ej, ie = i0 - 1 + ei, ke = k0 - 1 + ek;
-        T s = T(0);
-        if (je >= 0 && je < g.ny && ie >= 0 && ie < g.nx && ke >= 0 && ke < g.nz)
-            s = A.s_e[(long long)ke * g.nx * g.ny + (long long)ie * g.ny + je];
-        sS[idx] = s;
-    }
-    __syncthreads();
-
+    const auto K0c = as_const(kp);
     const int tj = threadIdx.x % TJ;
     const int ti = (threadIdx.x / TJ) % TI;
     const int tk = threadIdx.x / (TJ * TI);
     const int j = j0 + tj, i = i0 + ti, k = k0 + tk;
-    const bool owned = (j < g.NJ && i < g.NI && k < g.NK);
-
-    double part[2] = {0.0, 0.0};
-    const auto K0c = as_const(A.K0);
-    if (owned) {
-        // Contract (u_b - u_n) instead of u_b: K0 annihilates rigid translations,
-        // so the product is unchanged, but rounding now scales with the local
-        // variation of u instead of its magnitude (matters in FP32).
-        // Brick symmetry: K0 block (a,b) = R_a K0 block (0, a^b) R_a with R_a the
-        // reflection taking corner a to corner 0, so the 64 blocks come from
-        // the first three rows of K0 (72 values) with compile-time signs.
-        // The input component d is the outer loop: 27 neighbour values and
-        // 24 table values are live at a time, accumulated per (corner, output).
-        const int hb = 3 * ((tk + 1) * HI * HJ + (ti + 1) * HJ + (tj + 1));
-        T ctr[3];
+    if (!(j < g.NJ && i < g.NI && k < g.NK)) return;
+    // Contract (u_b - u_n) instead of u_b: K0 annihilates rigid translations, so
+    // the product is unchanged, but rounding now scales with the local variation
+    // of u instead of its magnitude (matters in FP32).  Brick symmetry: K0 block
+    // (a,b) = R_a K0 block (0, a^b) R_a with R_a the reflection taking corner a
+    // to corner 0, so the 64 blocks come from the first three rows of K0 (72
+    // values) with compile-time signs.  The input component d is the outer
+    // loop: 27 neighbour values and 8 table values are live at a time.
+    const int hb = 3 * ((tk + 1) * HI * HJ + (ti + 1) * HJ + (tj + 1));
+    T ctr[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ctr[d] = sP[hb + d];
-        T acc[8][3];
+    for (int d = 0; d < 3; ++d) ctr[d] = sP[hb + d];
+    T acc[8][3];
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+    for (int a = 0; a < 8; ++a)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) acc[a][c] = T(0);
+        for (int c = 0; c < 3; ++c) acc[a][c] = T(0);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            T nb[3][3][3];
+    for (int d = 0; d < 3; ++d) {
+        T nb[3][3][3];
 #pragma unroll
-            for (int dz = 0; dz < 3; ++dz)
+        for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
-                for (int dx = 0; dx < 3; ++dx)
+            for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-                    for (int dy = 0; dy < 3; ++dy)
-                        nb[dz][dx][dy] = sP[hb + 3 * ((dz - 1) * HI * HJ + (dx - 1) * HJ + (dy - 1)) + d] - ctr[d];
+                for (int dy = 0; dy < 3; ++dy)
+                    nb[dz][dx][dy] = sP[hb + 3 * ((dz - 1) * HI * HJ + (dx - 1) * HJ + (dy - 1)) + d] - ctr[d];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                // the 8 table values of (c, d): K0[c][3x + d], x = 0..7
-                T kt[8];
+        for (int c = 0; c < 3; ++c) {
+            T kt[8];
 #pragma unroll
-                for (int x = 0; x < 8; ++x) kt[x] = K0c[c * 24 + 3 * x + d];
+            for (int x = 0; x < 8; ++x) kt[x] = K0c[c * 24 + 3 * x + d];
 #pragma unroll
-                for (int a = 0; a < 8; ++a)
+            for (int a = 0; a < 8; ++a)
 #pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        if (b == a) continue;   // u_n - u_n = 0: the self block contributes nothing
-                        const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
-                        const T kv = kt[corner_xor(a, b)];
-                        if (refl_sign(a, c) * refl_sign(a, d) > 0)
-                            acc[a][c] += kv * nb[dz][dx][dy];
-                        else
-                            acc[a][c] -= kv * nb[dz][dx][dy];
-                    }
-            }
-        }
-        T qv[3] = {T(0), T(0), T(0)};
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) qv[c] += s * acc[a][c];
-        }
-        const long long gn = 3LL * ((long long)k * g.NI * g.NJ + (long long)i * g.NJ + j);
-        if constexpr (MODE == MODE_PLAIN) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) A.y[gn + c] = qv[c];
-        } else if constexpr (MODE == MODE_DIR) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                A.q[gn + c] = qv[c];
-                part[0] += double(ctr[c]) * double(qv[c]);
-            }
-        } else {  // MODE_RESID (T = double)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double msk = A.dinv[gn + c];
-                const double r64 = (msk != 0.0) ? (A.f[gn + c] - double(qv[c])) : 0.0;
-                const float r32 = float(r64);
-                A.r32[gn + c] = r32;
-                part[0] += r64 * r64;
-                part[1] += double(r32) * double(A.dinv32[gn + c] * r32);
-            }
+                for (int b = 0; b < 8; ++b) {
+                    if (b == a) continue;   // u_n - u_n = 0: the self block contributes nothing
+                    const int dz = 1 + OZ(b) - OZ(a), dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
+                    const T kv = kt[corner_xor(a, b)];
+                    if (refl_sign(a, c) * refl_sign(a, d) > 0)
+                        acc[a][c] += kv * nb[dz][dx][dy];
+                    else
+                        acc[a][c] -= kv * nb[dz][dx][dy];
+                }
         }
     }
+    T qv[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) qv[c] += s * acc[a][c];
+    }
+    const int gn = 3 * ((k * g.NI + i) * g.NJ + j);
+    if constexpr (MODE == MODE_PLAIN) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) A.y[gn + c] = qv[c];
+    } else if constexpr (MODE == MODE_DIR) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            A.q[gn + c] = qv[c];
+            part[0] += double(ctr[c]) * double(qv[c]);
+        }
+    } else {  // MODE_RESID (T = double)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double msk = A.dinv[gn + c];
+            const double r64 = (msk != 0.0) ? (A.f[gn + c] - double(qv[c])) : 0.0;
+            const float r32 = float(r64);
+            A.r32[gn + c] = r32;
+            part[0] += r64 * r64;
+            part[1] += double(r32) * double(A.dinv32[gn + c] * r32);
+        }
+    }
+}
 
+template <typename T, int MODE>
+__device__ __forceinline__ void node_finish(const NodeArgs<T>& A, double (&part)[2], double* sRed) {
     if constexpr (MODE == MODE_DIR) {
         double v[1] = {part[0]}, tot[1];
         if (reduce_last_block<1>(v, A.partials, &A.st->cnt[0], tot, sRed)) {
@@ -305,6 +317,100 @@ k_node(NodeArgs<T> A) {
             if (threadIdx.x == 0) resid_finish(A.st, tot[0], tot[1]);
         }
     }
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& first, T*& p_new, const T*& p_old) {
+    beta = T(0);
+    first = 1;
+    p_new = nullptr;
+    p_old = nullptr;
+    if constexpr (MODE == MODE_DIR) {
+        if (A.st->status != ST_RUNNING) return false;
+        beta = T(A.st->beta);
+        first = A.st->first;
+        const bool odd = (A.st->k & 1) != 0;
+        p_new = odd ? A.p1 : A.p0;
+        p_old = odd ? A.p0 : A.p1;
+    }
+    return true;
+}
+
+// Node-centric K*u, one brick per block (grid = bricks).
+template <typename T, int TJ, int TI, int TK, int MODE>
+__global__ void __launch_bounds__(TJ * TI * TK)
+k_node(NodeArgs<T> A) {
+    constexpr int NT = TJ * TI * TK;
+    typedef Halo<T, TJ, TI, TK, NT> Hc;
+    __shared__ T sP[3 * Hc::HJ * Hc::HI * Hc::HK];
+    __shared__ T sS[Hc::NE];
+    __shared__ double sRed[16 * 4];
+    T beta;
+    int first;
+    T* p_new;
+    const T* p_old;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    int j0, i0, k0;
+    tile_origin(A.g, blockIdx.x, TJ, TI, TK, j0, i0, k0);
+    {
+        Hc H;
+        halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
+        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, sP, sS, (T*)nullptr);
+    }
+    __syncthreads();
+    double part[2] = {0.0, 0.0};
+    node_brick<T, TJ, TI, TK, MODE>(A, j0, i0, k0, A.K0, sP, sS, part);
+    node_finish<T, MODE>(A, part, sRed);
+}
+
+// Node-centric K*u, persistent: a fixed grid whose blocks walk contiguous,
+// XCD-local runs of bricks (one partial per block; neighbouring bricks share an
+// L2).  WAVES bounds the registers so that many waves stay resident.
+template <typename T> struct PersistWaves;
+template <> struct PersistWaves<double> { static constexpr int value = 3; };
+template <> struct PersistWaves<float> { static constexpr int value = 4; };
+
+template <typename T, int TJ, int TI, int TK, int MODE>
+__global__ void __launch_bounds__(TJ * TI * TK) __attribute__((amdgpu_waves_per_eu(PersistWaves<T>::value)))
+k_node_persist(NodeArgs<T> A) {
+    constexpr int NT = TJ * TI * TK;
+    typedef Halo<T, TJ, TI, TK, NT> Hc;
+    __shared__ T sP[3 * Hc::HJ * Hc::HI * Hc::HK];
+    __shared__ T sS[Hc::NE];
+    __shared__ double sRed[16 * 4];
+    T beta;
+    int first;
+    T* p_new;
+    const T* p_old;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    int tile, tend, tstep;
+    tile_range(A.g.ntiles, tile, tend, tstep);
+    double part[2] = {0.0, 0.0};
+    for (; tile < tend; tile += tstep) {
+        int j0, i0, k0;
+        tile_origin(A.g, tile, TJ, TI, TK, j0, i0, k0);
+        {
+            Hc H;
+            halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
+            halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, sP, sS, (T*)nullptr);
+        }
+        __syncthreads();
+        // opaque per-brick copy of the K0 pointer keeps the table reads in the loop
+        const T* kp = A.K0;
+        asm volatile("" : "+s"(kp));
+        node_brick<T, TJ, TI, TK, MODE>(A, j0, i0, k0, kp, sP, sS, part);
+        __syncthreads();
+    }
+    node_finish<T, MODE>(A, part, sRed);
+}
+
+// Element-batched kernels stage with the same layout (non-persistent form).
+template <typename T, int TJ, int TI, int TK, int MODE, int NT>
+__device__ __forceinline__ void stage_halo(const NodeArgs<T>& A, int j0, int i0, int k0, T beta, int first,
+                                           const T* p_old, T* p_new, T* sP, T* sS, T* sZero) {
+    Halo<T, TJ, TI, TK, NT> H;
+    halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
+    halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, sP, sS, sZero);
 }
 
 }  // namespace h8

@@ -22,7 +22,7 @@ H8_MAXITER = 4
 H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
 
-VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA = 0, 1, 2
+VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST = 0, 1, 2, 3
 
 
 class H8Grid(C.Structure):

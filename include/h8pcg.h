@@ -41,6 +41,7 @@ extern "C" {
 #define H8_VARIANT_AUTO 0   /* library default for the precision             */
 #define H8_VARIANT_NODE 1   /* node-centric gather (VALU), deterministic     */
 #define H8_VARIANT_MFMA 2   /* element-batched 24x24 contraction on MFMA     */
+#define H8_VARIANT_NODE_PERSIST 3 /* node-centric, persistent XCD-local brick runs */
 
 typedef struct h8_grid {
     int32_t nx, ny, nz;     /* elements per axis (>= 1)                      */
