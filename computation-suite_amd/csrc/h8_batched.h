@@ -1,0 +1,292 @@
+// h8_batched.h -- many structured instances x many right-hand sides in one
+// launch per PCG phase.
+//
+// A "system" is one (instance, load case) pair; its vectors are contiguous
+// segments of the batch vectors (instance i, case c at vec_off[i] + c*ndof_i).
+// Every launch covers every system; a system whose status has left RUNNING is
+// skipped (its per-system convergence mask), and the host polls one device
+// counter of still-running systems per captured chunk of iterations.
+//
+//   k_binit   : flat chunk table over systems; r = mask(b), |b|, |r|, r.z; the
+//               last block of each system applies the early-out test.
+//   k_bdir    : flat brick table over instances; a block stages the element
+//               scales of its brick once, then for each running load case of
+//               the instance stages p = dinv*r + beta*p_old, computes q = K p
+//               on the brick (same node-centric arithmetic as k_node) and
+//               writes its p.q partial; the last block of the instance forms
+//               alpha (or flags p.q <= 0) for every running case.
+//   k_bupdate : flat chunk table over systems; x, r update, r.z and r.r
+//               partials; the last block of each system applies the stopping
+//               rule of the single-system solver and retires finished systems.
+#pragma once
+#include "h8_device.h"
+#include "h8_node.h"
+
+namespace h8 {
+
+struct BInst {            // device copy of one instance
+    int nx, ny, nz, nrhs;
+    long long node_off;   // dinv at 3*node_off
+    long long elem_off;   // s_e at elem_off
+    long long vec_off;    // case c at vec_off + c*ndof
+    int tile_off, ntiles, sys_off, k0_off;   // k0_off: index of the instance's K0 (x576)
+};
+struct BTile { int inst, tile; };
+struct BChunk { int sys; int start; int len; int pad; };
+
+struct BSys {             // per-system solver state
+    double rz_old, alpha, beta, bnorm, rel, bad_pq;
+    long long k, iters;
+    int status, first;
+    unsigned cnt;         // last-block counter of this system's chunks
+    int inst;             // owning instance
+    long long off;        // vector offset
+    int ndof, case_idx;
+};
+
+struct BArgs {
+    const BInst* inst;
+    const BTile* tiles;
+    const BChunk* chunks;
+    BSys* sys;
+    unsigned* inst_cnt;   // last-block counters of the bricks of each instance
+    int* active;          // systems still RUNNING
+    double* partials;     // per system: pstride doubles
+    int pstride;
+    double tol;
+    long long maxiter;
+};
+
+template <typename T>
+struct BVecs {
+    const T* s_e;
+    const T* K0;
+    const T* dinv;
+    const T* b;
+    T* x;
+    T* r;
+    T* p0;
+    T* p1;
+    T* q;
+};
+
+__device__ inline GridDev inst_grid(const BInst& I, int TJ, int TI, int TK) {
+    GridDev g;
+    g.nx = I.nx; g.ny = I.ny; g.nz = I.nz;
+    g.NJ = I.ny + 1; g.NI = I.nx + 1; g.NK = I.nz + 1;
+    g.nnodes = (long long)g.NJ * g.NI * g.NK;
+    g.ndof = 3 * g.nnodes;
+    g.tj = (g.NJ + TJ - 1) / TJ;
+    g.ti = (g.NI + TI - 1) / TI;
+    g.tk = (g.NK + TK - 1) / TK;
+    g.ntiles = g.tj * g.ti * g.tk;
+    g.variant = 0;
+    return g;
+}
+
+// Sums NV partials of one system written by `count` contributors (fixed order).
+template <int NV>
+__device__ void sum_partials(const double* part, int count, double (&tot)[NV], double* sh) {
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int b = threadIdx.x; b < count; b += blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += ld_relaxed(&part[(size_t)b * NV + i]);
+    }
+    block_sum<NV>(acc, sh);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) tot[i] = acc[i];
+}
+
+// Ticket on a counter after this block's device-scope partial stores; true in
+// the last of `expected` blocks (which also resets the counter).
+__device__ inline bool take_ticket(unsigned* counter, unsigned expected) {
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (t == expected - 1);
+        if (s_last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return s_last != 0;
+}
+
+__device__ inline void retire(BArgs& B, BSys& S, int status) {
+    S.status = status;
+    atomicSub(B.active, 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_binit(BArgs B, BVecs<T> V) {
+    __shared__ double sRed[16 * 3];
+    const BChunk C = B.chunks[blockIdx.x];
+    BSys& S = B.sys[C.sys];
+    const BInst& I = B.inst[S.inst];
+    const T* dinv = V.dinv + 3 * I.node_off;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int e = C.start + threadIdx.x; e < C.start + C.len; e += blockDim.x) {
+        const T d = dinv[e];
+        const T bi = (d != T(0)) ? V.b[S.off + e] : T(0);
+        V.r[S.off + e] = bi;
+        V.x[S.off + e] = T(0);
+        v[0] += double(bi) * double(bi);
+        v[1] += double(bi) * double(bi);
+        v[2] += double(bi) * double(d * bi);
+    }
+    block_sum<3>(v, sRed);
+    const int nch = (S.ndof + 2047) / 2048;
+    double* part = B.partials + (size_t)C.sys * B.pstride;
+    if (threadIdx.x == 0) {
+        const int ci = C.start / 2048;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) st_relaxed(&part[3 * ci + i], v[i]);
+    }
+    if (!take_ticket(&S.cnt, nch)) return;
+    double tot[3];
+    sum_partials<3>(part, nch, tot, sRed);
+    if (threadIdx.x == 0) {
+        double bn = sqrt(tot[0]);
+        if (bn == 0.0) bn = 1.0;
+        S.bnorm = bn;
+        S.rel = sqrt(tot[1]) / bn;
+        S.k = 1;
+        S.first = 1;
+        S.beta = 0.0;
+        S.alpha = 0.0;
+        if (S.rel < B.tol) {
+            S.iters = 0;
+            retire(B, S, ST_DONE);
+        } else if (B.maxiter == 0) {
+            S.iters = 0;
+            retire(B, S, ST_MAXITER);
+        } else {
+            S.rz_old = tot[2];
+            S.status = ST_RUNNING;
+        }
+    }
+}
+
+template <typename T, int TJ, int TI, int TK>
+__global__ void __launch_bounds__(TJ * TI * TK)
+k_bdir(BArgs B, BVecs<T> V) {
+    constexpr int NT = TJ * TI * TK;
+    typedef Halo<T, TJ, TI, TK, NT> Hc;
+    __shared__ T sP[3 * Hc::HJ * Hc::HI * Hc::HK];
+    __shared__ T sS[Hc::NE];
+    __shared__ double sRed[16 * 4];
+    const BTile Tl = B.tiles[blockIdx.x];
+    const BInst& I = B.inst[Tl.inst];
+    NodeArgs<T> A{};
+    A.g = inst_grid(I, TJ, TI, TK);
+    A.s_e = V.s_e + I.elem_off;
+    A.K0 = V.K0 + (size_t)I.k0_off * 576;
+    A.dinv = V.dinv + 3 * I.node_off;
+    int j0, i0, k0;
+    tile_origin(A.g, Tl.tile, TJ, TI, TK, j0, i0, k0);
+    bool any = false;
+    for (int c = 0; c < I.nrhs; ++c) {
+        BSys& S = B.sys[I.sys_off + c];
+        double part[2] = {0.0, 0.0};
+        if (S.status == ST_RUNNING) {
+            any = true;
+            const long long off = S.off;
+            A.r = V.r + off;
+            A.p0 = V.p0 + off;
+            A.p1 = V.p1 + off;
+            A.q = V.q + off;
+            const bool odd = (S.k & 1) != 0;
+            T* p_new = odd ? A.p1 : A.p0;
+            const T* p_old = odd ? A.p0 : A.p1;
+            {
+                Hc H;
+                halo_load<T, TJ, TI, TK, NT, MODE_DIR>(A, j0, i0, k0, S.first, p_old, H);
+                halo_store<T, TJ, TI, TK, NT, MODE_DIR>(H, T(S.beta), p_new, sP, sS, (T*)nullptr);
+            }
+            __syncthreads();
+            node_brick<T, TJ, TI, TK, MODE_DIR>(A, j0, i0, k0, A.K0, sP, sS, part);
+            __syncthreads();
+        }
+        double v[1] = {part[0]};
+        block_sum<1>(v, sRed);
+        if (threadIdx.x == 0)
+            st_relaxed(&B.partials[(size_t)(I.sys_off + c) * B.pstride + Tl.tile], v[0]);
+    }
+    (void)any;
+    if (!take_ticket(&B.inst_cnt[Tl.inst], I.ntiles)) return;
+    for (int c = 0; c < I.nrhs; ++c) {
+        BSys& S = B.sys[I.sys_off + c];
+        if (S.status != ST_RUNNING) continue;
+        double tot[1];
+        sum_partials<1>(B.partials + (size_t)(I.sys_off + c) * B.pstride, I.ntiles, tot, sRed);
+        if (threadIdx.x == 0) {
+            const double pq = tot[0];
+            if (!(pq > 0.0)) {
+                S.bad_pq = pq;
+                S.iters = S.k;
+                retire(B, S, ST_NOT_SPD);
+            } else {
+                S.alpha = S.rz_old / pq;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_bupdate(BArgs B, BVecs<T> V) {
+    __shared__ double sRed[16 * 2];
+    const BChunk C = B.chunks[blockIdx.x];
+    BSys& S = B.sys[C.sys];
+    if (S.status != ST_RUNNING) return;
+    const BInst& I = B.inst[S.inst];
+    const T* dinv = V.dinv + 3 * I.node_off;
+    const T* p = ((S.k & 1) ? V.p1 : V.p0) + S.off;
+    T* x = V.x + S.off;
+    T* r = V.r + S.off;
+    const T* q = V.q + S.off;
+    const T alpha = T(S.alpha);
+    double v[2] = {0.0, 0.0};
+    for (int e = C.start + threadIdx.x; e < C.start + C.len; e += blockDim.x) {
+        const T d = dinv[e];
+        x[e] += alpha * p[e];
+        const T ri = (d != T(0)) ? T(r[e] - alpha * q[e]) : T(0);
+        r[e] = ri;
+        v[0] += double(ri) * double(d * ri);
+        v[1] += double(ri) * double(ri);
+    }
+    block_sum<2>(v, sRed);
+    const int nch = (S.ndof + 2047) / 2048;
+    double* part = B.partials + (size_t)C.sys * B.pstride;
+    if (threadIdx.x == 0) {
+        const int ci = C.start / 2048;
+        st_relaxed(&part[2 * ci + 0], v[0]);
+        st_relaxed(&part[2 * ci + 1], v[1]);
+    }
+    if (!take_ticket(&S.cnt, nch)) return;
+    double tot[2];
+    sum_partials<2>(part, nch, tot, sRed);
+    if (threadIdx.x == 0) {
+        const double rel = sqrt(tot[1]) / S.bnorm;
+        S.rel = rel;
+        const long long k = S.k;
+        if (rel < B.tol) {
+            S.iters = k;
+            retire(B, S, ST_DONE);
+        } else if (k >= B.maxiter) {
+            S.iters = k;
+            retire(B, S, ST_MAXITER);
+        } else {
+            S.beta = tot[0] / S.rz_old;
+            S.rz_old = tot[0];
+            S.first = 0;
+            S.k = k + 1;
+        }
+    }
+}
+
+}  // namespace h8

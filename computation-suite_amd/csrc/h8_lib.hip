@@ -17,6 +17,7 @@
 #include "h8_node.h"
 #include "h8_stream.h"
 #include "h8_mfma.h"
+#include "h8_batched.h"
 
 using namespace h8;
 
@@ -24,6 +25,8 @@ using namespace h8;
 // errors
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
+static thread_local double g_batched_ms = 0.0;
+static thread_local long long g_batched_launches = 0;
 
 static int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -755,3 +758,194 @@ extern "C" int h8_time_kernels(const h8_grid* g, int precision, const void* s_e,
     }
     return fail(H8_ERR_ARG, "precision must be 64 or 32");
 }
+
+// ---------------------------------------------------------------------------
+// Batched solve
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int BCH = 2048;   // update/init chunk (elements); k_binit/k_bupdate assume it
+
+struct BLayout {
+    int n_sys = 0, n_tiles = 0, n_chunks = 0, pstride = 0;
+    size_t off_r, off_p0, off_p1, off_q, off_inst, off_tiles, off_chunks, off_sys, off_icnt, off_active,
+        off_part, total;
+};
+
+int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLayout& L) {
+    if (!inst || n_inst < 1) return fail(H8_ERR_ARG, "no instances");
+    int maxt = 1, maxc = 1;
+    for (int i = 0; i < n_inst; ++i) {
+        const h8_instance& I = inst[i];
+        if (I.nx < 1 || I.ny < 1 || I.nz < 1 || I.nrhs < 1)
+            return fail(H8_ERR_ARG, "instance %d: nx, ny, nz, nrhs must be >= 1", i);
+        GridDev d;
+        h8_grid g{I.nx, I.ny, I.nz, 0};
+        int rc = make_grid(&g, d);
+        if (rc) return rc;
+        L.n_sys += I.nrhs;
+        L.n_tiles += d.ntiles;
+        const int nch = int((d.ndof + BCH - 1) / BCH);
+        L.n_chunks += nch * I.nrhs;
+        maxt = std::max(maxt, d.ntiles);
+        maxc = std::max(maxc, nch);
+    }
+    L.pstride = std::max(maxt, 3 * maxc);
+    size_t p = 0;
+    auto take = [&](size_t bytes) { size_t q = p; p += align_up(bytes); return q; };
+    L.off_r = take(sizeof(double) * total_vec);
+    L.off_p0 = take(sizeof(double) * total_vec);
+    L.off_p1 = take(sizeof(double) * total_vec);
+    L.off_q = take(sizeof(double) * total_vec);
+    L.off_inst = take(sizeof(BInst) * n_inst);
+    L.off_tiles = take(sizeof(BTile) * L.n_tiles);
+    L.off_chunks = take(sizeof(BChunk) * L.n_chunks);
+    L.off_sys = take(sizeof(BSys) * L.n_sys);
+    L.off_icnt = take(sizeof(unsigned) * n_inst);
+    L.off_active = take(sizeof(int));
+    L.off_part = take(sizeof(double) * (size_t)L.n_sys * L.pstride);
+    L.total = p;
+    return H8_OK;
+}
+}  // namespace
+
+extern "C" size_t h8_batched_workspace_bytes(const h8_instance* inst, int32_t n_inst, int64_t total_vec) {
+    BLayout L;
+    if (batched_layout(inst, n_inst, total_vec, L)) return 0;
+    return L.total;
+}
+
+extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64_t total_nodes, int64_t total_elems,
+                                  int64_t total_vec, const double* s_e, const double* K0, int32_t k0_per_instance,
+                                  const double* dinv, const double* b, double* x, double tol, int64_t maxiter,
+                                  void* ws, size_t ws_bytes, int32_t* status_out, int64_t* iters_out,
+                                  double* rel_out, void* stream) {
+    BLayout L;
+    int rc = batched_layout(inst, n_inst, total_vec, L);
+    if (rc) return rc;
+    if (!s_e || !K0 || !dinv || !b || !x || !ws) return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < L.total) return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, L.total);
+    if (!(tol >= 0.0) || maxiter < 0) return fail(H8_ERR_ARG, "bad tol / maxiter");
+    // host tables
+    std::vector<BInst> hi(n_inst);
+    std::vector<BTile> ht;
+    std::vector<BChunk> hc;
+    std::vector<BSys> hs;
+    ht.reserve(L.n_tiles);
+    hc.reserve(L.n_chunks);
+    hs.reserve(L.n_sys);
+    for (int i = 0; i < n_inst; ++i) {
+        const h8_instance& I = inst[i];
+        GridDev d;
+        h8_grid g{I.nx, I.ny, I.nz, 0};
+        make_grid(&g, d);
+        const long long m = (long long)I.nx * I.ny * I.nz;
+        if (I.node_off < 0 || I.node_off + d.nnodes > total_nodes || I.elem_off < 0 || I.elem_off + m > total_elems ||
+            I.vec_off < 0 || I.vec_off + d.ndof * I.nrhs > total_vec)
+            return fail(H8_ERR_ARG, "instance %d: offsets outside the batch arrays", i);
+        BInst& B = hi[i];
+        B.nx = I.nx; B.ny = I.ny; B.nz = I.nz; B.nrhs = I.nrhs;
+        B.node_off = I.node_off; B.elem_off = I.elem_off; B.vec_off = I.vec_off;
+        B.tile_off = int(ht.size()); B.ntiles = d.ntiles; B.sys_off = int(hs.size());
+        B.k0_off = k0_per_instance ? i : 0;
+        for (int t = 0; t < d.ntiles; ++t) ht.push_back(BTile{i, t});
+        for (int c = 0; c < I.nrhs; ++c) {
+            BSys S{};
+            S.status = ST_RUNNING;
+            S.inst = i;
+            S.off = I.vec_off + (long long)c * d.ndof;
+            S.ndof = int(d.ndof);
+            S.case_idx = c;
+            const int sysid = int(hs.size());
+            hs.push_back(S);
+            for (long long st = 0; st < d.ndof; st += BCH)
+                hc.push_back(BChunk{sysid, int(st), int(std::min<long long>(BCH, d.ndof - st)), 0});
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char* w = (char*)ws;
+    BArgs A{};
+    A.inst = (const BInst*)(w + L.off_inst);
+    A.tiles = (const BTile*)(w + L.off_tiles);
+    A.chunks = (const BChunk*)(w + L.off_chunks);
+    A.sys = (BSys*)(w + L.off_sys);
+    A.inst_cnt = (unsigned*)(w + L.off_icnt);
+    A.active = (int*)(w + L.off_active);
+    A.partials = (double*)(w + L.off_part);
+    A.pstride = L.pstride;
+    A.tol = tol;
+    A.maxiter = maxiter;
+    BVecs<double> V{};
+    V.s_e = s_e; V.K0 = K0; V.dinv = dinv; V.b = b; V.x = x;
+    V.r = (double*)(w + L.off_r); V.p0 = (double*)(w + L.off_p0); V.p1 = (double*)(w + L.off_p1);
+    V.q = (double*)(w + L.off_q);
+
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(hipMemcpyAsync((void*)A.inst, hi.data(), sizeof(BInst) * hi.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void*)A.tiles, ht.data(), sizeof(BTile) * ht.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void*)A.chunks, hc.data(), sizeof(BChunk) * hc.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void*)A.sys, hs.data(), sizeof(BSys) * hs.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, (long long)n_inst, (int*)A.inst_cnt, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, A.active, L.n_sys);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_binit<double>, dim3(L.n_chunks), dim3(256), 0, s, A, V);
+    LAUNCHCHK();
+    int active = 0;
+    HIPCHK(hipMemcpyAsync(&active, A.active, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));   // also keeps the host tables alive until the copies completed
+    long long launches = 0;
+    if (active > 0) {
+        GraphKey key{};
+        key.kind = 3;
+        HIPCHK(hipGetDevice(&key.dev));
+        key.nx = n_inst; key.ny = L.n_tiles; key.nz = L.n_chunks;
+        const void* ptrs[] = {s_e, K0, dinv, b, x, ws};
+        std::memcpy(key.ptr, ptrs, sizeof ptrs);
+        key.chunk = chunk_for(maxiter);
+        hipGraphExec_t exec;
+        rc = get_graph(key, [&](hipStream_t cs) -> int {
+            for (int it = 0; it < key.chunk; ++it) {
+                hipLaunchKernelGGL((k_bdir<double, NTJ, NTI, NTK>), dim3(L.n_tiles), dim3(NTJ * NTI * NTK), 0, cs, A,
+                                   V);
+                hipLaunchKernelGGL(k_bupdate<double>, dim3(L.n_chunks), dim3(256), 0, cs, A, V);
+            }
+            LAUNCHCHK();
+            return H8_OK;
+        }, &exec);
+        if (rc) return rc;
+        while (active > 0) {
+            HIPCHK(hipGraphLaunch(exec, s));
+            ++launches;
+            HIPCHK(hipMemcpyAsync(&active, A.active, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipMemcpy(hs.data(), A.sys, sizeof(BSys) * hs.size(), hipMemcpyDeviceToHost));
+    int first_bad = H8_OK;
+    for (int i = 0; i < L.n_sys; ++i) {
+        const int st = hs[i].status;
+        const int code = st == ST_DONE ? H8_OK : (st == ST_NOT_SPD ? H8_NOT_SPD : H8_MAXITER);
+        if (status_out) status_out[i] = code;
+        if (iters_out) iters_out[i] = hs[i].iters;
+        if (rel_out) rel_out[i] = hs[i].rel;
+        if (code != H8_OK && first_bad == H8_OK) first_bad = code;
+    }
+    g_batched_ms = ms;
+    g_batched_launches = launches;
+    if (first_bad == H8_NOT_SPD) return fail(H8_NOT_SPD, "batched PCG: p^T A p <= 0 in at least one system");
+    if (first_bad == H8_MAXITER)
+        return fail(H8_MAXITER, "batched PCG: at least one system did not converge in %lld iterations",
+                    (long long)maxiter);
+    return H8_OK;
+}
+
+extern "C" double h8_batched_last_ms(void) { return g_batched_ms; }
+extern "C" int64_t h8_batched_last_launches(void) { return g_batched_launches; }

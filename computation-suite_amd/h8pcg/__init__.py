@@ -13,8 +13,9 @@ from .fields import FEField
 from .facet_loads import list_facets, boundary_faces, face_load_shares
 from .solver import FESystem
 from ._lib import NotSPDError, H8LibraryMissing
+from .batched import solve_static_batched, solve_many
 
 __all__ = ["Material", "D_solid3d", "Mesh", "box_mesh", "structured_grid", "Hex8Solid3D", "coefficients",
            "FEField", "list_facets", "boundary_faces", "face_load_shares", "FESystem", "NotSPDError",
-           "H8LibraryMissing"]
+           "H8LibraryMissing", "solve_static_batched", "solve_many"]
 __version__ = "0.1.0"

@@ -40,11 +40,9 @@ class H8PcgStats(C.Structure):
 
 
 class H8Instance(C.Structure):
-    """One structured instance of a batched solve (see include/h8pcg.h)."""
+    """One structured instance of a batched solve (include/h8pcg.h h8_instance)."""
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nrhs", C.c_int32),
-                ("node_off", C.c_int64), ("elem_off", C.c_int64), ("vec_off", C.c_int64),
-                ("tile_off", C.c_int32), ("ntiles", C.c_int32), ("sys_off", C.c_int32),
-                ("pad", C.c_int32)]
+                ("node_off", C.c_int64), ("elem_off", C.c_int64), ("vec_off", C.c_int64)]
 
 
 _P = C.c_void_p
@@ -71,10 +69,11 @@ SIGNATURES = {
     "h8_cast_f64_f32": (C.c_int, [_P, _P, _I64, _P]),
     "h8_masked_sub_f64": (C.c_int, [_P, _P, _P, _P, _I64, _P]),
     "h8_time_kernels": (C.c_int, [_GRID, C.c_int, _P, _P, _P, _P, C.c_size_t, C.c_int, _P, _P]),
-    "h8_batched_workspace_bytes": (C.c_size_t, [C.POINTER(H8Instance), C.c_int32, C.c_int64, C.c_int64,
-                                                C.c_int32]),
-    "h8_pcg_batched_f64": (C.c_int, [C.POINTER(H8Instance), C.c_int32, _P, _P, _P, _P, _P, _D, _I64,
-                                     _P, C.c_size_t, _P, _P, _P, _P]),
+    "h8_batched_workspace_bytes": (C.c_size_t, [C.POINTER(H8Instance), C.c_int32, C.c_int64]),
+    "h8_pcg_batched_f64": (C.c_int, [C.POINTER(H8Instance), C.c_int32, C.c_int64, C.c_int64, C.c_int64, _P, _P,
+                                     C.c_int32, _P, _P, _P, _D, _I64, _P, C.c_size_t, _P, _P, _P, _P]),
+    "h8_batched_last_ms": (C.c_double, []),
+    "h8_batched_last_launches": (C.c_int64, []),
 }
 
 _lib = None

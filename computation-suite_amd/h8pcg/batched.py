@@ -1,0 +1,147 @@
+"""Batched solves on the GPU.
+
+``solve_static_batched(fs, F_batch)`` mirrors fe/batched_solve.py:105-164: one
+assembled FESystem, many load vectors (columns of F_batch), the same
+Dirichlet data (non-zero prescribed values corrected per column), a full
+(n_dof, n_batch) result with U[fixed] = u_fixed.  Instead of one factorisation
+it runs one batched Jacobi-PCG over all columns.
+
+``solve_many(systems)`` solves many *different* structured models, each with
+its own load cases, in the same batched launches (the role fe/batch.py:60-128
+plays with a process pool): every (model, load case) pair is a system with
+its own convergence mask, and all systems advance together.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import H8Instance, check, ptr
+
+
+def _dirichlet_columns(fs, F_batch):
+    """Device b columns mask(F - K u_D) and the u_D vector (None if zero)."""
+    import torch
+    op = fs._op
+    fixed = fs.fixed_dofs_array
+    u_fixed = fs._fixed_dof_values_array(fixed)
+    F_t = torch.as_tensor(np.asarray(F_batch, dtype=float), device=op.device)
+    corr = None
+    uD_t = None
+    if len(fixed) and np.any(u_fixed):
+        uD = np.zeros(fs.n_dof)
+        uD[fixed] = u_fixed
+        uD_t = torch.as_tensor(uD, device=op.device)
+        corr = op.matvec(uD_t)
+    cols = []
+    for c in range(F_t.shape[1]):
+        cols.append(op.masked(F_t[:, c].contiguous(), corr))
+    return cols, uD_t
+
+
+class _Batch:
+    """Concatenated device arrays of a batched solve."""
+
+    def __init__(self, device):
+        self.device = device
+        self.inst = []
+        self.s_e = []
+        self.K0 = []
+        self.dinv = []
+        self.b = []
+        self.node_off = self.elem_off = self.vec_off = 0
+        self.ndofs = []
+
+    def add(self, op, b_cols):
+        nx, ny, nz = op.nx, op.ny, op.nz
+        nrhs = len(b_cols)
+        self.inst.append(H8Instance(nx, ny, nz, nrhs, self.node_off, self.elem_off, self.vec_off))
+        self.s_e.append(op.s_e)
+        self.K0.append(op.K0.reshape(-1))
+        self.dinv.append(op.dinv)
+        self.b.extend(b_cols)
+        self.ndofs.append((op.n_dof, nrhs))
+        self.node_off += op.n_nodes
+        self.elem_off += op.m
+        self.vec_off += op.n_dof * nrhs
+
+    def solve(self, tol, maxiter):
+        import torch
+        n_inst = len(self.inst)
+        arr = (H8Instance * n_inst)(*self.inst)
+        s_e = torch.cat(self.s_e).contiguous()
+        K0 = torch.cat(self.K0).contiguous()
+        dinv = torch.cat(self.dinv).contiguous()
+        b = torch.cat(self.b).contiguous()
+        x = torch.empty_like(b)
+        total_vec = b.numel()
+        nbytes = _lib.lib().h8_batched_workspace_bytes(arr, n_inst, total_vec)
+        if nbytes == 0:
+            check(_lib.H8_ERR_ARG, "h8_batched_workspace_bytes")
+        ws = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        n_sys = sum(n for _, n in self.ndofs)
+        status = (C.c_int32 * n_sys)()
+        iters = (C.c_int64 * n_sys)()
+        rel = (C.c_double * n_sys)()
+        if maxiter is None:
+            maxiter = 10 * max(nd for nd, _ in self.ndofs)
+        with torch.cuda.device(self.device):
+            rc = _lib.lib().h8_pcg_batched_f64(arr, n_inst, self.node_off, self.elem_off, total_vec, ptr(s_e), ptr(K0),
+                                               1, ptr(dinv), ptr(b), ptr(x), float(tol), int(maxiter), ptr(ws),
+                                               ws.numel(), status, iters, rel, _lib.stream_handle(self.device))
+        stats = {"status": np.array(status[:]), "iterations": np.array(iters[:]), "rel_residual": np.array(rel[:]),
+                 "elapsed_ms": _lib.lib().h8_batched_last_ms(),
+                 "launches": int(_lib.lib().h8_batched_last_launches()), "rc": rc}
+        check(rc)
+        out, o = [], 0
+        for nd, nrhs in self.ndofs:
+            out.append(x[o:o + nd * nrhs].reshape(nrhs, nd).T)
+            o += nd * nrhs
+        return out, stats
+
+
+def solve_static_batched(fesystem, F_batch, tol=1e-8, maxiter=None, return_stats=False):
+    """U_batch (n_dof, n_batch) for the columns of F_batch (fe/batched_solve.py:105-164)."""
+    F_batch = np.asarray(F_batch, dtype=float)
+    if F_batch.ndim != 2 or F_batch.shape[0] != fesystem.n_dof:
+        raise ValueError(f"F_batch must have shape ({fesystem.n_dof}, n_batch), got {F_batch.shape}")
+    fesystem._require_stiffness("solve_static_batched")
+    op = fesystem._op
+    op.set_free_mask(fesystem._free_mask())
+    cols, uD = _dirichlet_columns(fesystem, F_batch)
+    batch = _Batch(op.device)
+    batch.add(op, cols)
+    (X,), stats = batch.solve(tol, maxiter)
+    if uD is not None:
+        X = X + uD[:, None]
+    U = X.cpu().numpy()
+    return (U, stats) if return_stats else U
+
+
+def solve_many(systems, tol=1e-8, maxiter=None):
+    """systems: iterable of (FESystem, F_batch (n_dof_i, k_i)).  Returns
+    ([U_i (n_dof_i, k_i) device tensors], stats).  Every FESystem must be
+    assembled on the same device."""
+    batch = None
+    uDs = []
+    for fs, F_batch in systems:
+        fs._require_stiffness("solve_many")
+        op = fs._op
+        op.set_free_mask(fs._free_mask())
+        F_batch = np.asarray(F_batch, dtype=float)
+        if F_batch.ndim == 1:
+            F_batch = F_batch[:, None]
+        cols, uD = _dirichlet_columns(fs, F_batch)
+        if batch is None:
+            batch = _Batch(op.device)
+        elif op.device != batch.device:
+            raise ValueError("solve_many: all systems must live on one GPU")
+        batch.add(op, cols)
+        uDs.append(uD)
+    if batch is None:
+        raise ValueError("solve_many: no systems given")
+    X, stats = batch.solve(tol, maxiter)
+    X = [x if uD is None else x + uD[:, None] for x, uD in zip(X, uDs)]
+    return X, stats

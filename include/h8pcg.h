@@ -135,6 +135,41 @@ int h8_elem_bilinear_f64(const h8_grid* g, const double* K0, const double* u, co
 int h8_simp_scale_f64(const double* rho, double p, double rho_min, int64_t m,
                       double* s, double* ds, void* stream);
 
+/* ---- batched: many structured instances x many load cases --------------- */
+/* One instance of a batched solve (HOST array).  Instance i has
+ * ndof_i = 3(nx+1)(ny+1)(nz+1) DOFs and nrhs load cases; its element scales
+ * are s_e[elem_off .. elem_off + nx*ny*nz), its inverse Jacobi diagonal is
+ * dinv[3*node_off .. 3*node_off + ndof_i) (zero at fixed DOFs, shared by its
+ * cases), and case c occupies b/x[vec_off + c*ndof_i .. + ndof_i). */
+typedef struct h8_instance {
+    int32_t nx, ny, nz, nrhs;
+    int64_t node_off, elem_off, vec_off;
+} h8_instance;
+
+/* Workspace (bytes) of h8_pcg_batched_f64 for this instance list. */
+size_t h8_batched_workspace_bytes(const h8_instance* inst, int32_t n_inst, int64_t total_vec);
+
+/* Jacobi-PCG of every (instance, case) system with the recurrence and
+ * stopping rule of h8_pcg_f64, all systems advanced together: each launch
+ * covers every system still running (per-system convergence masks), and the
+ * iterations are captured in HIP graphs; one host poll per graph launch reads
+ * the number of systems still running.  K0 is one 24x24 matrix
+ * (k0_per_instance = 0) or one per instance (= 1).  x receives the solutions
+ * (x0 = 0).  status_out/iters_out/rel_out are HOST arrays of length
+ * sum(nrhs) (system order: instance by instance, case by case).  Replaces the
+ * per-model loop of batch.map (fe/batch.py:60-128) and, for one instance, the
+ * multi-RHS solve_static_batched (fe/batched_solve.py:105-164).  Returns H8_OK
+ * when every system converged, else the first failing status. */
+int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64_t total_nodes, int64_t total_elems,
+                       int64_t total_vec, const double* s_e, const double* K0, int32_t k0_per_instance,
+                       const double* dinv, const double* b, double* x, double tol, int64_t maxiter,
+                       void* ws, size_t ws_bytes, int32_t* status_out, int64_t* iters_out,
+                       double* rel_out, void* stream);
+
+/* Device time (ms) and graph launches of this thread's last batched solve. */
+double h8_batched_last_ms(void);
+int64_t h8_batched_last_launches(void);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Average device time (ms) of the two kernels of one PCG iteration, each
  * launched `reps` times alone between HIP events on `stream`: avg_ms[0] = the
