@@ -76,6 +76,29 @@ def cpu_baseline(nx, ny, nz, E, F, iters):
                      "host_cpu": _cpu_name(), "host_cores": os.cpu_count()}
 
 
+KERNEL_NAMES = {  # (variant, precision) -> kernel symbol of the fused direction + K*p kernel
+    (1, 32): "k_node<float, 8, 8, 4, 1>", (1, 64): "k_node<double, 8, 8, 4, 1>",
+    (2, 32): "k_mfma<float, 16, 8, 8, 1>", (2, 64): "k_mfma<double, 8, 8, 8, 1>",
+    (3, 32): "k_node_persist<float, 8, 8, 4, 1>", (3, 64): "k_node_persist<double, 8, 8, 4, 1>",
+}
+DEFAULT_VARIANT = {32: 3, 64: 1}
+
+
+def profiled_traffic(dims, kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/pcg_kernels_<dims>_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "pcg_kernels_%dx%dx%d_traffic.md" % dims)
+    try:
+        for line in open(path):
+            if line.startswith("traffic-json:"):
+                d = json.loads(line.split(":", 1)[1])
+                if d["kernel"] == kernel:
+                    return d["bytes_per_launch"], os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def _cpu_name():
     try:
         for line in open("/proc/cpuinfo"):
@@ -164,6 +187,9 @@ def main():
     dir_bytes = 5 * 4 * n_dof + 4 * m          # read r, dinv, p_old; write p, q; read s_e
     upd_bytes = 7 * 4 * n_dof                  # read x, r, p, q, dinv; write x, r
     achieved = dir_bytes / (dir_ms * 1e-3)
+    variant = op.grid.variant or DEFAULT_VARIANT[32]
+    kname = KERNEL_NAMES.get((variant, 32), "unknown")
+    traffic, traffic_src = profiled_traffic((nx, ny, nz), kname)
     # whole-iteration contract: B_iter = 13 n_f s + m s_c (FP32), B_outer = 6 n_f 8 + 8 m
     s_ = stats[-1]
     b_solve = s_["inner_iterations"] * (13 * n_free * 4 + 4 * m) + s_["outer"] * (6 * n_free * 8 + 8 * m)
@@ -186,8 +212,9 @@ def main():
                                f"nu=0.3, clamped x=0, -1e6 consistent tip traction, true rel. residual < 1e-8",
                    "n_free": n_free, "elements": m, "delta": a.delta},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "fused direction + K*p + p.q (FP32, default variant)",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": f"{kname}: fused direction + K*p + p.q (FP32)",
                      "kernel_avg_us": dir_ms * 1e3, "bytes_per_launch": dir_bytes,
                      "update_kernel_avg_us": upd_ms * 1e3, "update_bytes_per_launch": upd_bytes,
                      "update_frac": upd_bytes / (upd_ms * 1e-3) / HBM_PEAK,

@@ -5,9 +5,10 @@
 Builds the clamped box operator (per-element scale 10**U(9,12), seed 5), then
 times the fused direction + K*p kernel and the update kernel of one PCG
 iteration (each launched `reps` times alone, captured in a HIP graph, timed
-with HIP events on the launching stream), and finally runs a plain device
-copy of --copy-mib MiB so hardware byte counters can be calibrated against a
-kernel of known traffic.  Prints one JSON line.  Run it under
+with HIP events on the launching stream), and finally runs a plain
+elementwise kernel (torch.neg, float32) over --copy-mib MiB -- it reads and
+writes exactly that many bytes -- so hardware byte counters (FETCH_SIZE,
+WRITE_SIZE) can be calibrated against a kernel of known traffic.  Prints one JSON line.  Run it under
 ``rocprofv3 --kernel-trace --stats`` or ``rocprofv3 --pmc ...``.
 """
 from __future__ import annotations
@@ -76,9 +77,10 @@ def main(argv=None):
         src = torch.ones(n, dtype=torch.float32, device=op.device)
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.neg(src, out=dst)
         torch.cuda.synchronize()
-        out["copy_bytes_each_way"] = 4 * n
+        out["calibration_kernel"] = "neg (float32)"
+        out["calibration_bytes_each_way"] = 4 * n
     print(json.dumps(out), flush=True)
 
 
