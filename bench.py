@@ -180,7 +180,7 @@ def main():
     dir_ms, upd_ms = avg[0], avg[1]
     ws64 = op.workspace(64)
     avg64 = (C.c_double * 2)()
-    _lib.check(_lib.lib().h8_time_kernels(C.byref(op.grid), 64, _lib.ptr(op.s_e), _lib.ptr(op.K0),
+    _lib.check(_lib.lib().h8_time_kernels(C.byref(op.grid), 64, _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
                                           _lib.ptr(dinv), _lib.ptr(ws64), ws64.numel(), 200, avg64,
                                           _lib.stream_handle(op.device)), "h8_time_kernels")
     n_dof = fs.n_dof

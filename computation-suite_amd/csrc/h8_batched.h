@@ -29,7 +29,7 @@ struct BInst {            // device copy of one instance
     long long node_off;   // dinv at 3*node_off
     long long elem_off;   // s_e at elem_off
     long long vec_off;    // case c at vec_off + c*ndof
-    int tile_off, ntiles, sys_off, k0_off;   // k0_off: index of the instance's K0 (x576)
+    int tile_off, ntiles, sys_off, k0_off;   // k0_off: index of the instance's K0 (x H8_K0_WORDS)
 };
 struct BTile { int inst, tile; };
 struct BChunk { int sys; int start; int len; int pad; };
@@ -183,7 +183,7 @@ k_bdir(BArgs B, BVecs<T> V) {
     NodeArgs<T> A{};
     A.g = inst_grid(I, TJ, TI, TK);
     A.s_e = V.s_e + I.elem_off;
-    A.K0 = V.K0 + (size_t)I.k0_off * 576;
+    A.K0 = V.K0 + (size_t)I.k0_off * 648;   // H8_K0_WORDS
     A.dinv = V.dinv + 3 * I.node_off;
     int j0, i0, k0;
     tile_origin(A.g, Tl.tile, TJ, TI, TK, j0, i0, k0);
@@ -204,7 +204,8 @@ k_bdir(BArgs B, BVecs<T> V) {
             {
                 Hc H;
                 halo_load<T, TJ, TI, TK, NT, MODE_DIR>(A, j0, i0, k0, S.first, p_old, H);
-                halo_store<T, TJ, TI, TK, NT, MODE_DIR>(H, T(S.beta), p_new, sP, sS, (T*)nullptr);
+                halo_store<T, TJ, TI, TK, NT, MODE_DIR>(H, T(S.beta), p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS,
+                                                         (T*)nullptr);
             }
             __syncthreads();
             node_brick<T, TJ, TI, TK, MODE_DIR>(A, j0, i0, k0, A.K0, sP, sS, part);

@@ -22,7 +22,8 @@ struct GridDev {
     long long ndof;        // 3*nnodes
     int tj, ti, tk;        // tiles per axis (node tiles)
     int ntiles;            // tj*ti*tk
-    int variant;           // 1 = node-centric VALU, 2 = element-batched MFMA
+    int variant;           // H8_VARIANT_*
+    int layers;            // node layers per block of the sector kernel (0 = choose)
 };
 
 inline void set_tiles(GridDev& d, int TJ, int TI, int TK) {
@@ -160,5 +161,44 @@ __device__ __forceinline__ void tile_origin(const GridDev& g, int tile, int TJ, 
     const int bk = rest / g.ti;
     j0 = bj * TJ; i0 = bi * TI; k0 = bk * TK;
 }
+
+// Brick of a block in a one-brick-per-block launch.  Workgroups are dealt to
+// the 8 XCDs round-robin (block b runs on XCD b % 8), so consecutive block ids
+// would put neighbouring bricks on different XCDs and every halo would be
+// fetched into several L2s.  This bijection gives XCD x the contiguous brick
+// range [x*q + min(x, r), ...) (q = n / 8, r = n % 8), which has exactly as
+// many bricks as XCD x has blocks.
+__device__ __forceinline__ int xcd_local_tile(int b, int n) {
+    const int x = b & 7, q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// Bounds-checked raw buffer access.  A buffer resource covers `bytes` bytes
+// from `p`; a load at an offset >= bytes returns 0 and a store there is
+// dropped (the hardware range check of raw buffers), so out-of-domain halo
+// nodes need no branch: their offset is set to OOB.
+constexpr int OOB = int(0x80000000u);
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), 0x00020000);
+}
+template <typename T> struct Buf;
+template <> struct Buf<float> {
+    static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+    static __device__ __forceinline__ void st(float v, __amdgpu_buffer_rsrc_t r, int off) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+    }
+};
+template <> struct Buf<double> {
+    static __device__ __forceinline__ double ld(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    static __device__ __forceinline__ void st(double v, __amdgpu_buffer_rsrc_t r, int off) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    }
+};
 
 }  // namespace h8

@@ -18,6 +18,7 @@
 #include "h8_stream.h"
 #include "h8_mfma.h"
 #include "h8_batched.h"
+#include "h8_sector.h"
 
 using namespace h8;
 
@@ -67,6 +68,13 @@ extern "C" int h8_device_info(char* buf, int buflen) {
     return H8_OK;
 }
 
+extern "C" int h8_sector_table_f64(double* K0, void* stream) {
+    if (!K0) return fail(H8_ERR_ARG, "NULL pointer");
+    hipLaunchKernelGGL(k_sector_table, dim3(1), dim3(128), 0, (hipStream_t)stream, K0);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
 // ---------------------------------------------------------------------------
 // tiling
 // ---------------------------------------------------------------------------
@@ -95,12 +103,16 @@ static int make_grid(const h8_grid* g, GridDev& d) {
     d.ti = (d.NI + NTI - 1) / NTI;
     d.tk = (d.NK + NTK - 1) / NTK;
     long long nt = (long long)d.tj * d.ti * d.tk;
-    if (nt > (1LL << 30) || d.ndof >= (1LL << 31))
-        return fail(H8_ERR_ARG, "grid too large: %lld DOFs (the kernels index with 32-bit integers)", d.ndof);
+    // byte offsets of FP64 vectors must fit the 31-bit range of buffer offsets
+    if (nt > (1LL << 30) || d.ndof >= (1LL << 28))
+        return fail(H8_ERR_ARG, "grid too large: %lld DOFs (the kernels address vectors with 32-bit byte offsets; "
+                    "at most 2^28 - 1 DOFs)", d.ndof);
     d.ntiles = int(nt);
-    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_NODE_PERSIST)
+    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_SECTOR)
         return fail(H8_ERR_ARG, "unknown kernel variant %d", g->variant);
+    if (g->layers < 0) return fail(H8_ERR_ARG, "layers must be >= 0 (0 = library choice), got %d", g->layers);
     d.variant = g->variant;
+    d.layers = g->layers;
     return H8_OK;
 }
 
@@ -110,6 +122,19 @@ static int make_grid(const h8_grid* g, GridDev& d) {
 static std::mutex g_occ_mu;
 static std::vector<std::pair<const void*, int>> g_occ;
 static int g_cus[64] = {};
+
+// Compute units of the current device (256 if the query fails).
+static int device_cus() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    if (!g_cus[dev]) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 256;
+        g_cus[dev] = p.multiProcessorCount;
+    }
+    return g_cus[dev];
+}
 
 static int persistent_grid(const void* kernel, int threads, int ntiles) {
     int dev = 0;
@@ -132,6 +157,34 @@ static int persistent_grid(const void* kernel, int threads, int ntiles) {
     return int(std::max(1LL, gsz));
 }
 
+// Sector kernel: column tiles of (SEJ-1) x (SEI-1) nodes, chunks of `layers`
+// node layers (library choice: about 4 blocks per CU, 2..16 layers).  The
+// partials buffer of the fused modes holds max(brick tiles, 2048) blocks, so
+// the launch is refused if it would need more.
+constexpr int SEJ = 16, SEI = 16;
+template <typename T, int MODE>
+static int launch_sector(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
+    GridDev d = d0;
+    const int TJ = SEJ - 1, TI = SEI - 1;
+    d.tj = (d.NJ + TJ - 1) / TJ;
+    d.ti = (d.NI + TI - 1) / TI;
+    int kd = d.layers;
+    if (kd <= 0) {
+        const long long cols = (long long)d.tj * d.ti;
+        const long long want = 4LL * device_cus();
+        kd = int(std::max(2LL, std::min(16LL, (d.NK * cols + want - 1) / want)));
+    }
+    d.tk = (d.NK + kd - 1) / kd;
+    const long long nt = (long long)d.tj * d.ti * d.tk;
+    if (MODE != MODE_PLAIN && nt > std::max<long long>(d0.ntiles, 2048))
+        return fail(H8_ERR_ARG, "sector kernel: %lld blocks exceed the partials capacity (use more layers)", nt);
+    d.ntiles = int(nt);
+    A.g = d;
+    hipLaunchKernelGGL((k_sector<T, SEJ, SEI, MODE>), dim3(d.ntiles), dim3(SEJ * SEI), 0, s, A, A.K0 + 576, kd);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
 // Launch one structured operator kernel (mode MODE) with the variant recorded
 // in d (AUTO -> the precision's default).
 template <typename T, int MODE>
@@ -146,7 +199,11 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     } else {
         set_tiles(d, NTJ, NTI, NTK);
         A.g = d;
-        if (v == H8_VARIANT_NODE_PERSIST) {
+        if (v == H8_VARIANT_SECTOR) {
+            return launch_sector<T, MODE>(d0, A, s);
+        } else if (v == H8_VARIANT_NODE_SPLIT) {
+            hipLaunchKernelGGL((k_node2<T, NTJ, NTI, NTK, MODE>), dim3(d.ntiles), dim3(2 * NTJ * NTI * NTK), 0, s, A);
+        } else if (v == H8_VARIANT_NODE_PERSIST) {
             const void* kfn = (const void*)k_node_persist<T, NTJ, NTI, NTK, MODE>;
             const int grid = persistent_grid(kfn, NTJ * NTI * NTK, d.ntiles);
             hipLaunchKernelGGL((k_node_persist<T, NTJ, NTI, NTK, MODE>), dim3(grid), dim3(NTJ * NTI * NTK), 0, s,

@@ -66,7 +66,7 @@ k_mfma(NodeArgs<T> A) {
         if (A.st->status != ST_RUNNING) return;
     }
     int j0, i0, k0;
-    tile_origin(g, blockIdx.x, TJ, TI, TK, j0, i0, k0);
+    tile_origin(g, xcd_local_tile(blockIdx.x, gridDim.x), TJ, TI, TK, j0, i0, k0);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
     T beta = T(0);

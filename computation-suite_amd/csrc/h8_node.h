@@ -108,86 +108,124 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
 // LDS node-interleaved ([k][i][j][c], the global DOF order: a staged row is one
 // contiguous LDS row and lane-adjacent nodes sit 3 words apart).  A halo row
 // (fixed i, k) is 3*HJ contiguous words in global memory; each thread owns one
-// word position w of a 32- or 64-lane row slot (j, c and the row-invariant
-// tests are computed once) and walks the rows in passes of NT/LW.  MODE_DIR
-// forms p = dinv*r + beta*p_old and writes it for the owned nodes.  The
-// (TJ+1)(TI+1)(TK+1) element scales (0 outside the mesh) are staged the same
-// way.  Indices are 32-bit (the library rejects grids with 2^31 or more DOFs).
+// word position w of a 32- or 64-lane row slot.  Rows are dealt to the RPP
+// slots so that the first HK passes take row (ii = slot, kk = pass): the
+// j and i parts of the address and their range tests are computed once per
+// thread and the k part is uniform per pass.  The remaining (HI - RPP) * HK
+// rows (ii >= RPP) follow in XP more passes.  Every access is a bounds-checked
+// raw buffer access: a word outside the domain gets an out-of-range offset, so
+// it loads 0 and needs no branch, and the direction p written for owned nodes
+// uses the same trick for the nodes a block does not own.  MODE_DIR forms
+// p = dinv*r + beta*p_old.  The (TJ+1)(TI+1)(TK+1) element scales (0 outside
+// the mesh) are staged the same way.  Byte offsets are 32-bit (the library
+// rejects grids with 2^28 or more DOFs).
 template <typename T, int TJ, int TI, int TK, int NT>
 struct Halo {
     static constexpr int HJ = TJ + 2, HI = TI + 2, HK = TK + 2;
     static constexpr int ROW = 3 * HJ;
     static constexpr int LW = ROW <= 32 ? 32 : 64;
     static constexpr int RPP = NT / LW;
-    static constexpr int NROW = HI * HK;
-    static constexpr int PER = (NROW + RPP - 1) / RPP;
+    static_assert(ROW <= 64, "halo row longer than 64 words");
+    // RPP <= HI: row (ii = slot, kk = pass) for the first HK passes, then the
+    // XR rows with ii >= RPP.  RPP > HI: plain row = slot + RPP * pass.
+    static constexpr bool SPLIT = RPP <= HI;
+    static constexpr int XW = SPLIT ? HI - RPP : 0;      // extra rows per plane
+    static constexpr int XR = XW * HK;                   // extra rows in all
+    static constexpr int XP = (XR + RPP - 1) / RPP;      // extra passes
+    static constexpr int PER = SPLIT ? HK + XP : (HI * HK + RPP - 1) / RPP;
     static constexpr int EJ = TJ + 1, EI = TI + 1, EK = TK + 1;
     static constexpr int NE = EJ * EI * EK;
     static constexpr int PES = (NE + NT - 1) / NT;
-    static_assert(ROW <= 64, "halo row longer than 64 words");
     T v0[PER], v1[PER], v2[PER];
-    int gidx[PER];
+    int own[PER];     // byte offset of an owned node's word, OOB otherwise
     T sv[PES];
 };
+
+// Halo row (ii, kk) of pass u for row slot `slot`.
+template <int HI, int HK, int RPP>
+__device__ __forceinline__ void halo_row(int u, int slot, int& ii, int& kk, bool& valid) {
+    if constexpr (RPP > HI) {
+        const int row = slot + RPP * u;
+        ii = row % HI; kk = row / HI; valid = row < HI * HK;
+        return;
+    }
+    constexpr int XW = RPP > HI ? 0 : HI - RPP;
+    if (u < HK) {
+        ii = slot; kk = u; valid = true;
+    } else {
+        const int e = (u - HK) * RPP + slot;
+        if constexpr (XW > 0) {
+            ii = RPP + e % XW; kk = e / XW; valid = e < XW * HK;
+        } else {
+            ii = 0; kk = 0; valid = false;
+        }
+    }
+}
 
 template <typename T, int TJ, int TI, int TK, int NT, int MODE>
 __device__ __forceinline__ void halo_load(const NodeArgs<T>& A, int j0, int i0, int k0, int first,
                                           const T* p_old, Halo<T, TJ, TI, TK, NT>& H) {
     typedef Halo<T, TJ, TI, TK, NT> Hc;
     const GridDev& g = A.g;
-    const int w = threadIdx.x % Hc::LW, r0 = threadIdx.x / Hc::LW;
+    const unsigned vbytes = unsigned(g.ndof) * sizeof(T);
+    const int w = threadIdx.x % Hc::LW, slot = threadIdx.x / Hc::LW;
     const int jj = w / 3, c = w - 3 * (w / 3);
     const int j = j0 - 1 + jj;
     const bool jok = (w < Hc::ROW) && j >= 0 && j < g.NJ;
-    const int plane = g.NI * g.NJ;
+    const bool jown = jj >= 1 && jj <= TJ;
+    const int plane3 = 3 * g.NI * g.NJ;
+    const int wofs = 3 * j + c;
 #pragma unroll
     for (int u = 0; u < Hc::PER; ++u) {
-        const int row = r0 + u * Hc::RPP;
-        const int ii = row % Hc::HI, kk = row / Hc::HI;
+        int ii, kk;
+        bool rv;
+        halo_row<Hc::HI, Hc::HK, Hc::RPP>(u, slot, ii, kk, rv);
         const int i = i0 - 1 + ii, k = k0 - 1 + kk;
-        const bool in = jok && row < Hc::NROW && i >= 0 && i < g.NI && k >= 0 && k < g.NK;
-        const int gi = in ? 3 * (k * plane + i * g.NJ + j) + c : -1;
-        H.gidx[u] = gi;
+        const bool in = jok && rv && i >= 0 && i < g.NI && k >= 0 && k < g.NK;
+        const int off = in ? int(sizeof(T)) * (k * plane3 + 3 * i * g.NJ + wofs) : OOB;
+        const bool owned = in && jown && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK;
+        H.own[u] = owned ? off : OOB;
         if constexpr (MODE == MODE_DIR) {
-            H.v0[u] = in ? A.r[gi] : T(0);
-            H.v1[u] = in ? A.dinv[gi] : T(0);
-            H.v2[u] = (in && !first) ? p_old[gi] : T(0);
+            H.v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), off);
+            H.v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), off);
+            H.v2[u] = first ? T(0) : Buf<T>::ld(buf_rsrc(p_old, vbytes), off);
         } else {
-            H.v0[u] = in ? A.u[gi] : T(0);
+            H.v0[u] = Buf<T>::ld(buf_rsrc(A.u, vbytes), off);
         }
     }
+    const unsigned sbytes = unsigned(g.nx) * unsigned(g.ny) * unsigned(g.nz) * sizeof(T);
 #pragma unroll
     for (int u = 0; u < Hc::PES; ++u) {
         const int idx = threadIdx.x + u * NT;
         const int ej = idx % Hc::EJ, ei = (idx / Hc::EJ) % Hc::EI, ek = idx / (Hc::EJ * Hc::EI);
         const int je = j0 - 1 + ej, ie = i0 - 1 + ei, ke = k0 - 1 + ek;
         const bool in = idx < Hc::NE && je >= 0 && je < g.ny && ie >= 0 && ie < g.nx && ke >= 0 && ke < g.nz;
-        H.sv[u] = in ? A.s_e[(ke * g.nx + ie) * g.ny + je] : T(0);
+        H.sv[u] = Buf<T>::ld(buf_rsrc(A.s_e, sbytes), in ? int(sizeof(T)) * ((ke * g.nx + ie) * g.ny + je) : OOB);
     }
 }
 
 template <typename T, int TJ, int TI, int TK, int NT, int MODE>
-__device__ __forceinline__ void halo_store(const Halo<T, TJ, TI, TK, NT>& H, T beta, T* p_new, T* sP, T* sS,
-                                           T* sZero) {
+__device__ __forceinline__ void halo_store(const Halo<T, TJ, TI, TK, NT>& H, T beta, T* p_new, unsigned vbytes,
+                                           T* sP, T* sS, T* sZero) {
     typedef Halo<T, TJ, TI, TK, NT> Hc;
-    const int w = threadIdx.x % Hc::LW, r0 = threadIdx.x / Hc::LW;
-    const int jj = w / 3;
-    const bool jown = jj >= 1 && jj <= TJ;
+    const int w = threadIdx.x % Hc::LW, slot = threadIdx.x / Hc::LW;
 #pragma unroll
     for (int u = 0; u < Hc::PER; ++u) {
-        const int row = r0 + u * Hc::RPP;
-        if (w >= Hc::ROW || row >= Hc::NROW) continue;
-        const int ii = row % Hc::HI, kk = row / Hc::HI;
+        int ii, kk;
+        bool rv;
+        halo_row<Hc::HI, Hc::HK, Hc::RPP>(u, slot, ii, kk, rv);
         T val;
         if constexpr (MODE == MODE_DIR) {
             val = H.v1[u] * H.v0[u] + beta * H.v2[u];
-            if (H.gidx[u] >= 0 && jown && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK) p_new[H.gidx[u]] = val;
+            Buf<T>::st(val, buf_rsrc(p_new, vbytes), H.own[u]);
         } else {
             val = H.v0[u];
         }
-        const int li = row * Hc::ROW + w;
-        sP[li] = val;
-        if (sZero) sZero[li] = T(0);
+        if (w < Hc::ROW && rv) {
+            const int li = (kk * Hc::HI + ii) * Hc::ROW + w;
+            sP[li] = val;
+            if (sZero) sZero[li] = T(0);
+        }
     }
 #pragma unroll
     for (int u = 0; u < Hc::PES; ++u) {
@@ -213,6 +251,35 @@ __device__ __forceinline__ void tile_range(int ntiles, int& first_tile, int& end
 // Node-centric K*u over the tiles of a persistent block (grid <= tiles).  With
 // PF the next brick's global loads are issued before the current brick is
 // computed (more registers, fewer exposed latencies).
+// Outputs of one node (global DOF gn of component 0): y, or q and p.q, or the
+// FP64 true residual with its FP32 copy and norms.
+template <typename T, int MODE>
+__device__ __forceinline__ void node_output(const NodeArgs<T>& A, int gn, const T (&qv)[3], const T (&ctr)[3],
+                                            double (&part)[2]) {
+    const GridDev& g = A.g;
+    const unsigned vbytes = unsigned(g.ndof) * sizeof(T);
+    if constexpr (MODE == MODE_PLAIN) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Buf<T>::st(qv[c], buf_rsrc(A.y, vbytes), int(sizeof(T)) * (gn + c));
+    } else if constexpr (MODE == MODE_DIR) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Buf<T>::st(qv[c], buf_rsrc(A.q, vbytes), int(sizeof(T)) * (gn + c));
+            part[0] += double(ctr[c]) * double(qv[c]);
+        }
+    } else {  // MODE_RESID (T = double)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double msk = A.dinv[gn + c];
+            const double r64 = (msk != 0.0) ? (A.f[gn + c] - double(qv[c])) : 0.0;
+            const float r32 = float(r64);
+            A.r32[gn + c] = r32;
+            part[0] += r64 * r64;
+            part[1] += double(r32) * double(A.dinv32[gn + c] * r32);
+        }
+    }
+}
+
 // One brick of the node-centric K*u: stage (unless PF staged it already),
 // compute the owned nodes, write the outputs, accumulate the fused partials.
 template <typename T, int TJ, int TI, int TK, int MODE>
@@ -223,7 +290,6 @@ __device__ __forceinline__ void node_brick(const NodeArgs<T>& A, int j0, int i0,
     constexpr int HJ = Hc::HJ, HI = Hc::HI;
     constexpr int EJ = Hc::EJ, EI = Hc::EI;
     const GridDev& g = A.g;
-    const auto K0c = as_const(kp);
     const int tj = threadIdx.x % TJ;
     const int ti = (threadIdx.x / TJ) % TI;
     const int tk = threadIdx.x / (TJ * TI);
@@ -236,10 +302,13 @@ __device__ __forceinline__ void node_brick(const NodeArgs<T>& A, int j0, int i0,
     // to corner 0, so the 64 blocks come from the first three rows of K0 (72
     // values) with compile-time signs.  The input component d is the outer
     // loop: 27 neighbour values and 8 table values are live at a time.
-    const int hb = 3 * ((tk + 1) * HI * HJ + (ti + 1) * HJ + (tj + 1));
+    // hb: LDS word of the (-1,-1,-1) neighbour, so every neighbour offset is a
+    // non-negative constant (folded into the LDS instruction's immediate)
+    const int hb = 3 * (tk * HI * HJ + ti * HJ + tj);
+    constexpr int CTR = 3 * (HI * HJ + HJ + 1);
     T ctr[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) ctr[d] = sP[hb + d];
+    for (int d = 0; d < 3; ++d) ctr[d] = sP[hb + CTR + d];
     T acc[8][3];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
@@ -254,12 +323,17 @@ __device__ __forceinline__ void node_brick(const NodeArgs<T>& A, int j0, int i0,
             for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy)
-                    nb[dz][dx][dy] = sP[hb + 3 * ((dz - 1) * HI * HJ + (dx - 1) * HJ + (dy - 1)) + d] - ctr[d];
+                    nb[dz][dx][dy] = sP[hb + 3 * (dz * HI * HJ + dx * HJ + dy) + d] - ctr[d];
+        // the table slice of this d is loaded here, not hoisted: all 72 values of
+        // an FP64 table would not fit in the scalar registers
+        const T* kpd = kp;
+        asm volatile("" : "+s"(kpd));
+        const auto K0d = as_const(kpd);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             T kt[8];
 #pragma unroll
-            for (int x = 0; x < 8; ++x) kt[x] = K0c[c * 24 + 3 * x + d];
+            for (int x = 0; x < 8; ++x) kt[x] = K0d[c * 24 + 3 * x + d];
 #pragma unroll
             for (int a = 0; a < 8; ++a)
 #pragma unroll
@@ -275,33 +349,14 @@ __device__ __forceinline__ void node_brick(const NodeArgs<T>& A, int j0, int i0,
         }
     }
     T qv[3] = {T(0), T(0), T(0)};
+    const int sb = (tk * EI + ti) * EJ + tj;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        const T s = sS[((tk + 1 - OZ(a)) * EI + (ti + 1 - OX(a))) * EJ + (tj + 1 - OY(a))];
+        const T s = sS[sb + ((1 - OZ(a)) * EI + (1 - OX(a))) * EJ + (1 - OY(a))];
 #pragma unroll
         for (int c = 0; c < 3; ++c) qv[c] += s * acc[a][c];
     }
-    const int gn = 3 * ((k * g.NI + i) * g.NJ + j);
-    if constexpr (MODE == MODE_PLAIN) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) A.y[gn + c] = qv[c];
-    } else if constexpr (MODE == MODE_DIR) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            A.q[gn + c] = qv[c];
-            part[0] += double(ctr[c]) * double(qv[c]);
-        }
-    } else {  // MODE_RESID (T = double)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double msk = A.dinv[gn + c];
-            const double r64 = (msk != 0.0) ? (A.f[gn + c] - double(qv[c])) : 0.0;
-            const float r32 = float(r64);
-            A.r32[gn + c] = r32;
-            part[0] += r64 * r64;
-            part[1] += double(r32) * double(A.dinv32[gn + c] * r32);
-        }
-    }
+    node_output<T, MODE>(A, 3 * ((k * g.NI + i) * g.NJ + j), qv, ctr, part);
 }
 
 template <typename T, int MODE>
@@ -351,15 +406,143 @@ k_node(NodeArgs<T> A) {
     const T* p_old;
     if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
     int j0, i0, k0;
-    tile_origin(A.g, blockIdx.x, TJ, TI, TK, j0, i0, k0);
+    tile_origin(A.g, xcd_local_tile(blockIdx.x, gridDim.x), TJ, TI, TK, j0, i0, k0);
     {
         Hc H;
         halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
-        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, sP, sS, (T*)nullptr);
+        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, (T*)nullptr);
     }
     __syncthreads();
     double part[2] = {0.0, 0.0};
     node_brick<T, TJ, TI, TK, MODE>(A, j0, i0, k0, A.K0, sP, sS, part);
+    node_finish<T, MODE>(A, part, sRed);
+}
+
+// Half of the node-centric product of one node: the four elements on one side
+// of it in z (H = 0: the elements above the node, where it is a bottom corner,
+// OZ(a) = 0; H = 1: the elements below it).  Same arithmetic as node_brick
+// restricted to those elements; their corners lie in two halo planes, and in
+// both cases the plane of corner b is OZ(b) planes above the first one.
+// Returns the scaled contribution of the four elements in qv and the node's
+// own value in ctr.
+template <typename T, int TJ, int TI, int TK, int H>
+__device__ __forceinline__ void node_half(const T* kp, const T* sP, const T* sS, int tj, int ti, int tk,
+                                          T (&ctr)[3], T (&qv)[3]) {
+    constexpr int HJ = TJ + 2, HI = TI + 2;
+    constexpr int EJ = TJ + 1, EI = TI + 1;
+    constexpr int PZ0 = H == 0 ? 1 : 0;      // first halo plane (counted from the node's -1 plane)
+    const int hb = 3 * (tk * HI * HJ + ti * HJ + tj);
+    constexpr int CTR = 3 * (HI * HJ + HJ + 1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ctr[d] = sP[hb + CTR + d];
+    T acc[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[a][c] = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        T nb[2][3][3];
+#pragma unroll
+        for (int pz = 0; pz < 2; ++pz)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+                    nb[pz][dx][dy] = sP[hb + 3 * ((PZ0 + pz) * HI * HJ + dx * HJ + dy) + d] - ctr[d];
+        // the table slice of this d is loaded here, not hoisted: all 72 values of
+        // an FP64 table would not fit in the scalar registers
+        const T* kpd = kp;
+        asm volatile("" : "+s"(kpd));
+        const auto K0d = as_const(kpd);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T kt[8];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) kt[x] = K0d[c * 24 + 3 * x + d];
+#pragma unroll
+            for (int ai = 0; ai < 4; ++ai) {
+                const int a = 4 * H + ai;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    if (b == a) continue;
+                    const int dx = 1 + OX(b) - OX(a), dy = 1 + OY(b) - OY(a);
+                    const T kv = kt[corner_xor(a, b)];
+                    if (refl_sign(a, c) * refl_sign(a, d) > 0)
+                        acc[ai][c] += kv * nb[OZ(b)][dx][dy];
+                    else
+                        acc[ai][c] -= kv * nb[OZ(b)][dx][dy];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) qv[c] = T(0);
+    const int sb = (tk * EI + ti) * EJ + tj;
+#pragma unroll
+    for (int ai = 0; ai < 4; ++ai) {
+        const int a = 4 * H + ai;
+        const T sc = sS[sb + ((1 - OZ(a)) * EI + (1 - OX(a))) * EJ + (1 - OY(a))];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) qv[c] += sc * acc[ai][c];
+    }
+}
+
+// Node-centric K*u with two threads per node, one brick per block: waves
+// [0, NN/64) take the elements above their nodes and waves [NN/64, 2NN/64)
+// the elements below (the half is wave-uniform, so the two code paths never
+// diverge inside a wave); the lower half leaves its sums in LDS and the upper
+// half adds them (fixed order: deterministic) and writes the outputs.  Half
+// the registers and work per thread of k_node, twice the waves.
+template <typename T> struct SplitWaves;
+template <> struct SplitWaves<double> { static constexpr int value = 4; };
+template <> struct SplitWaves<float> { static constexpr int value = 8; };
+
+template <typename T, int TJ, int TI, int TK, int MODE>
+__global__ void __launch_bounds__(2 * TJ * TI * TK) __attribute__((amdgpu_waves_per_eu(SplitWaves<T>::value)))
+k_node2(NodeArgs<T> A) {
+    constexpr int NN = TJ * TI * TK;
+    constexpr int NT = 2 * NN;
+    static_assert(NN % 64 == 0, "a half must be whole waves");
+    typedef Halo<T, TJ, TI, TK, NT> Hc;
+    __shared__ T sP[3 * Hc::HJ * Hc::HI * Hc::HK];
+    __shared__ T sS[Hc::NE];
+    __shared__ T sQ[3 * NN];
+    __shared__ double sRed[16 * 4];
+    T beta;
+    int first;
+    T* p_new;
+    const T* p_old;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    int j0, i0, k0;
+    tile_origin(A.g, xcd_local_tile(blockIdx.x, gridDim.x), TJ, TI, TK, j0, i0, k0);
+    {
+        Hc H;
+        halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
+        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, (T*)nullptr);
+    }
+    __syncthreads();
+    const int half = threadIdx.x / NN;
+    const int nidx = threadIdx.x - half * NN;
+    const int tj = nidx % TJ, ti = (nidx / TJ) % TI, tk = nidx / (TJ * TI);
+    T ctr[3], qv[3];
+    // nodes outside the grid compute on the zero halo (in-bounds LDS reads) and write nothing
+    if (half) {
+        node_half<T, TJ, TI, TK, 1>(A.K0, sP, sS, tj, ti, tk, ctr, qv);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sQ[3 * nidx + c] = qv[c];
+    } else {
+        node_half<T, TJ, TI, TK, 0>(A.K0, sP, sS, tj, ti, tk, ctr, qv);
+    }
+    __syncthreads();
+    double part[2] = {0.0, 0.0};
+    const GridDev& g = A.g;
+    const int j = j0 + tj, i = i0 + ti, k = k0 + tk;
+    if (!half && j < g.NJ && i < g.NI && k < g.NK) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) qv[c] += sQ[3 * nidx + c];
+        node_output<T, MODE>(A, 3 * ((k * g.NI + i) * g.NJ + j), qv, ctr, part);
+    }
     node_finish<T, MODE>(A, part, sRed);
 }
 
@@ -392,7 +575,7 @@ k_node_persist(NodeArgs<T> A) {
         {
             Hc H;
             halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
-            halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, sP, sS, (T*)nullptr);
+            halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, (T*)nullptr);
         }
         __syncthreads();
         // opaque per-brick copy of the K0 pointer keeps the table reads in the loop
@@ -410,7 +593,7 @@ __device__ __forceinline__ void stage_halo(const NodeArgs<T>& A, int j0, int i0,
                                            const T* p_old, T* p_new, T* sP, T* sS, T* sZero) {
     Halo<T, TJ, TI, TK, NT> H;
     halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
-    halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, sP, sS, sZero);
+    halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, sZero);
 }
 
 }  // namespace h8

@@ -1,0 +1,288 @@
+// h8_sector.h -- matrix-free K*u through the reflection sectors of the brick
+// element, marching through the grid in z.
+//
+// Sector form of the element product.  Every brick element is symmetric under
+// the reflections x -> -x, y -> -y, z -> -z about its centre; a reflection g
+// maps corner a to a^g and flips the sign of displacement component c when g
+// reflects axis c.  K0 commutes with these maps, so the Walsh-Hadamard
+// transform over the 8 corners,
+//     U^[chi][c] = sum_a (-1)^(chi . a) U[a][c]      (a, chi as 3-bit masks),
+// block-diagonalises it: output (chi, c) couples only to inputs (chi', c')
+// with chi ^ e_c == chi' ^ e_c' =: psi (e_c = the bit of axis c), i.e. into 8
+// sectors psi of 3 components each:
+//     Y^[psi ^ e_c][c] = sum_c' Kh[psi][c][c'] U^[psi ^ e_c'][c'],
+//     Kh[psi][c][c'] = (1/8) sum_x (-1)^((psi ^ e_c') . x) K0[c][3x + c']
+// (the 1/8 of the inverse transform folded in; only K0's first three rows
+// enter).  The three translation sectors psi = e_c have a zero row and column
+// c (K0 annihilates rigid translations).  One element product is then a
+// forward transform (72 add/sub), 57 multiply-adds and an inverse transform
+// (72 add/sub) instead of 576 multiply-adds.
+//
+// Marching.  A block owns a column of TJ x TI = (EJ-1) x (EI-1) nodes over the
+// node layers [kb, ke) and walks the element slabs s = kb-1 .. ke-1 (slab s
+// lies between node layers s and s+1) upwards with one thread per element of
+// the EJ x EI slab (the slab's elements cover the owned nodes plus a one-node
+// halo ring).  Node layers are staged into a 3-slot LDS ring (the direction
+// p = dinv*r + beta*p_old is formed here in MODE_DIR, and written out for the
+// owned nodes); each element writes the outputs of its four bottom corners
+// (for node layer s) and its four top corners (for layer s+1) into per-node
+// slots, and every owned node sums its slots in a fixed order (deterministic)
+// - the top-corner sum is carried to the next step in a register.  The next
+// layer's global loads are issued before the element work of a step and
+// stored into the ring after it.  Two barriers per step.
+#pragma once
+#include "h8_device.h"
+#include "h8_node.h"
+
+namespace h8 {
+
+// Corner index of the 3-bit mask (bit 0 = x offset, bit 1 = y, bit 2 = z).
+__host__ __device__ constexpr int corner_of_mask(int m) { return corner_of(m & 1, (m >> 1) & 1, (m >> 2) & 1); }
+
+// In-place unnormalised Walsh-Hadamard transform over the corner masks.
+template <typename T>
+__device__ __forceinline__ void wht8(T (&v)[8]) {
+#pragma unroll
+    for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (!(m & bit)) {
+                const T u = v[m], w = v[m | bit];
+                v[m] = u + w;
+                v[m | bit] = u - w;
+            }
+}
+
+// Element product of one element: Y[a][c] (corner a in the reference order)
+// from U[a][c] through the sectors.  kh: 72 values Kh[psi][c][c'] (uniform).
+template <typename T>
+__device__ __forceinline__ void sector_product(const T* kh, T (&U)[8][3], T scale) {
+    T V[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) V[c][m] = U[corner_of_mask(m)][c];
+        wht8(V[c]);
+    }
+    // scale the transformed input once (the product is linear)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int m = 1; m < 8; ++m) V[c][m] *= scale;
+    T W[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) W[c][0] = T(0);    // the translation outputs of sectors e_c
+    const T* kp = kh;
+    asm volatile("" : "+s"(kp));
+    const auto K = as_const(kp);
+#pragma unroll
+    for (int psi = 0; psi < 8; ++psi) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (psi == (1 << c)) continue;          // zero row of a translation sector
+            T acc = T(0);
+            bool firstterm = true;
+#pragma unroll
+            for (int c2 = 0; c2 < 3; ++c2) {
+                if (psi == (1 << c2)) continue;     // zero column of a translation sector
+                const T kv = K[psi * 9 + c * 3 + c2];
+                const T x = V[c2][psi ^ (1 << c2)];
+                if (firstterm) { acc = kv * x; firstterm = false; }
+                else acc += kv * x;
+            }
+            W[c][psi ^ (1 << c)] = acc;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        wht8(W[c]);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) U[corner_of_mask(m)][c] = W[c][m];
+    }
+}
+
+// Layer staging of the marching kernel: a node layer of the column
+// ((EJ+1) x (EI+1) nodes, 3 words each, node-interleaved [hi][hj][c] = the
+// global order of those words, row by row) is LW words; thread t owns words
+// t + NT*u.  The word's row / in-row offset and its range tests do not change
+// between layers, so they are computed once per thread (SLayer::init); per
+// layer only the k term is added.
+template <typename T, int EJ, int EI>
+struct SLayer {
+    static constexpr int NT = EJ * EI;
+    static constexpr int RW = 3 * (EJ + 1);            // words per row
+    static constexpr int LW = RW * (EI + 1);           // words per layer
+    static constexpr int PASS = (LW + NT - 1) / NT;
+    int ofs[PASS];      // byte offset of the word without the k term, OOB if out of the (i, j) domain
+    bool own[PASS];     // word of an owned node (in i, j)
+    T v0[PASS], v1[PASS], v2[PASS];
+
+    __device__ __forceinline__ void init(const GridDev& g, int j0, int i0) {
+#pragma unroll
+        for (int u = 0; u < PASS; ++u) {
+            const int w = threadIdx.x + NT * u;
+            const int row = w / RW, r = w - RW * (w / RW);
+            const int hj = r / 3;
+            const int i = i0 - 1 + row, j = j0 - 1 + hj;
+            const bool in = w < LW && i >= 0 && i < g.NI && j >= 0 && j < g.NJ;
+            ofs[u] = in ? int(sizeof(T)) * (3 * (i * g.NJ + j0 - 1) + r) : OOB;
+            own[u] = in && row >= 1 && row <= EI - 1 && hj >= 1 && hj <= EJ - 1;
+        }
+    }
+    template <int MODE>
+    __device__ __forceinline__ void load(const NodeArgs<T>& A, const T* p_old, int first, int k) {
+        const GridDev& g = A.g;
+        const unsigned vbytes = unsigned(g.ndof) * sizeof(T);
+        const bool kin = k >= 0 && k < g.NK;
+        const int kofs = int(sizeof(T)) * 3 * k * g.NI * g.NJ;
+#pragma unroll
+        for (int u = 0; u < PASS; ++u) {
+            const int off = (kin && ofs[u] != OOB) ? ofs[u] + kofs : OOB;
+            if constexpr (MODE == MODE_DIR) {
+                v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), off);
+                v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), off);
+                v2[u] = first ? T(0) : Buf<T>::ld(buf_rsrc(p_old, vbytes), off);
+            } else {
+                v0[u] = Buf<T>::ld(buf_rsrc(A.u, vbytes), off);
+            }
+        }
+    }
+    // form the layer's values, write the owned direction entries, store to LDS
+    template <int MODE>
+    __device__ __forceinline__ void store(const NodeArgs<T>& A, T beta, T* p_new, int k, bool k_owned, T* slot) {
+        const unsigned vbytes = unsigned(A.g.ndof) * sizeof(T);
+        const int kofs = int(sizeof(T)) * 3 * k * A.g.NI * A.g.NJ;
+#pragma unroll
+        for (int u = 0; u < PASS; ++u) {
+            const int w = threadIdx.x + NT * u;
+            T val;
+            if constexpr (MODE == MODE_DIR) {
+                val = v1[u] * v0[u] + beta * v2[u];
+                Buf<T>::st(val, buf_rsrc(p_new, vbytes), (k_owned && own[u]) ? ofs[u] + kofs : OOB);
+            } else {
+                val = v0[u];
+            }
+            if (w < LW) slot[w] = val;
+        }
+    }
+};
+
+template <typename T> struct SectorWaves;
+template <> struct SectorWaves<double> { static constexpr int value = 4; };
+template <> struct SectorWaves<float> { static constexpr int value = 6; };
+
+// One block per (column tile, layer chunk); A.g.{tj, ti, tk} count the column
+// tiles in j and i and the chunks of `kd` node layers.
+template <typename T, int EJ, int EI, int MODE>
+__global__ void __launch_bounds__(EJ * EI) __attribute__((amdgpu_waves_per_eu(SectorWaves<T>::value)))
+k_sector(NodeArgs<T> A, const T* kh, int kd) {
+    constexpr int NT = EJ * EI;
+    constexpr int TJ = EJ - 1, TI = EI - 1;
+    typedef SLayer<T, EJ, EI> L;
+    static_assert(NT % 64 == 0, "whole waves");
+    __shared__ T sRing[3 * L::LW];
+    __shared__ T sLo[NT * 12];
+    __shared__ T sHi[NT * 12];
+    __shared__ double sRed[16 * 4];
+    constexpr bool OFFS = sizeof(T) == 4;   // subtract the corner-0 value first (FP32 rounding)
+
+    const GridDev& g = A.g;
+    T beta;
+    int first;
+    T* p_new;
+    const T* p_old;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
+    const int bj = tile % g.tj, bi = (tile / g.tj) % g.ti, bk = tile / (g.tj * g.ti);
+    const int j0 = bj * TJ, i0 = bi * TI;
+    const int kb = bk * kd, ke = min(g.NK, kb + kd);
+
+    const int ej = threadIdx.x % EJ, ei = threadIdx.x / EJ;
+    const int je = j0 - 1 + ej, ie = i0 - 1 + ei;          // element column of this thread
+    const bool ecol = je >= 0 && je < g.ny && ie >= 0 && ie < g.nx;
+    const unsigned sbytes = unsigned(g.nx) * unsigned(g.ny) * unsigned(g.nz) * sizeof(T);
+    // owned node of this thread in the node phase: halo position (ej, ei), both >= 1
+    const int nj = j0 - 1 + ej, ni = i0 - 1 + ei;
+    const bool nown = ej >= 1 && ei >= 1 && nj < g.NJ && ni < g.NI;
+    const int lbase = 3 * (ei * (EJ + 1) + ej);             // LDS word of corner 0 within a layer
+
+    L lay;
+    lay.init(g, j0, i0);
+    // prologue: layers kb-1 and kb into ring slots 0 and 1
+    lay.template load<MODE>(A, p_old, first, kb - 1);
+    lay.template store<MODE>(A, beta, p_new, kb - 1, false, sRing);
+    lay.template load<MODE>(A, p_old, first, kb);
+    lay.template store<MODE>(A, beta, p_new, kb, kb < ke, sRing + L::LW);
+    T se_next = Buf<T>::ld(buf_rsrc(A.s_e, sbytes),
+                           (ecol && kb - 1 >= 0) ? int(sizeof(T)) * (((kb - 1) * g.nx + ie) * g.ny + je) : OOB);
+    __syncthreads();
+
+    T carry[3] = {T(0), T(0), T(0)};
+    double part[2] = {0.0, 0.0};
+    for (int s = kb - 1; s < ke; ++s) {
+        const int q = s - (kb - 1);                          // step index
+        const T* lo = sRing + (q % 3) * L::LW;               // layer s
+        const T* up = sRing + ((q + 1) % 3) * L::LW;         // layer s + 1
+        // prefetch layer s + 2 and the next slab's element scale
+        const bool more = s + 2 <= ke;
+        if (more) lay.template load<MODE>(A, p_old, first, s + 2);
+        const T se = se_next;
+        se_next = Buf<T>::ld(buf_rsrc(A.s_e, sbytes),
+                             (ecol && s + 1 < g.nz && s + 1 < ke) ? int(sizeof(T)) * (((s + 1) * g.nx + ie) * g.ny + je)
+                                                                  : OOB);
+        // ---- element phase: slab s, element (je, ie, s) (scale 0 outside the mesh)
+        {
+            T U[8][3];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const T* src = OZ(a) ? up : lo;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) U[a][c] = src[lbase + 3 * (OX(a) * (EJ + 1) + OY(a)) + c];
+            }
+            if constexpr (OFFS) {
+#pragma unroll
+                for (int a = 1; a < 8; ++a)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) U[a][c] -= U[0][c];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) U[0][c] = T(0);
+            }
+            sector_product<T>(kh, U, se);
+            // corner a of this element is node (ej + OY(a), ei + OX(a)) of the halo grid;
+            // slots exist for hj < EJ, hi < EI (owned nodes have 1 <= hj, hi <= EJ-1, EI-1)
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                if ((OY(a) && ej + 1 >= EJ) || (OX(a) && ei + 1 >= EI)) continue;
+                T* dst = OZ(a) ? sHi : sLo;
+                const int slot = ((ei + OX(a)) * EJ + ej + OY(a)) * 12 + 3 * (a & 3);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dst[slot + c] = U[a][c];
+            }
+        }
+        __syncthreads();
+        // ---- node phase: layer s outputs, layer s + 1 carry; ring refill with layer s + 2
+        {
+            T qv[3], ctr[3], top[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T sl = T(0), sh = T(0);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    sl += sLo[threadIdx.x * 12 + 3 * a + c];
+                    sh += sHi[threadIdx.x * 12 + 3 * a + c];
+                }
+                qv[c] = carry[c] + sl;
+                top[c] = sh;
+                ctr[c] = lo[lbase + c];
+            }
+            if (nown && s >= kb) node_output<T, MODE>(A, 3 * ((s * g.NI + ni) * g.NJ + nj), qv, ctr, part);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) carry[c] = top[c];
+            if (more) lay.template store<MODE>(A, beta, p_new, s + 2, s + 2 < ke, sRing + ((q + 2) % 3) * L::LW);
+        }
+        __syncthreads();
+    }
+    node_finish<T, MODE>(A, part, sRed);
+}
+
+}  // namespace h8

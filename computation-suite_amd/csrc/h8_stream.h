@@ -170,6 +170,23 @@ k_fill(long long n, T* __restrict__ x, T v) {
         x[i] = v;
 }
 
+// Reflection-sector table of K0 (72 entries after the 576 matrix entries):
+// Kh[psi][c][c'] = (1/8) sum_x (-1)^popcount((psi ^ 2^c') & mask(x)) K0[c][3x+c'],
+// mask(x) = the corner's offset bits (x, y, z).  One thread per entry.
+__global__ void __launch_bounds__(128) k_sector_table(double* K0) {
+    const int t = threadIdx.x;
+    if (t >= 72) return;
+    const int psi = t / 9, c = (t / 3) % 3, c2 = t % 3;
+    double acc = 0.0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        const int m = OX(x) | (OY(x) << 1) | (OZ(x) << 2);
+        const double v = K0[c * 24 + 3 * x + c2];
+        acc += (__builtin_popcount((psi ^ (1 << c2)) & m) & 1) ? -v : v;
+    }
+    K0[576 + t] = 0.125 * acc;
+}
+
 // Jacobi diagonal: K_ii = sum over the node's elements of s_e K0[3a+c][3a+c].
 __global__ void __launch_bounds__(STREAM_THREADS)
 k_diag(GridDev g, const double* __restrict__ s_e, const double* __restrict__ K0,

@@ -22,11 +22,13 @@ H8_MAXITER = 4
 H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
 
-VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST = 0, 1, 2, 3
+VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR = 0, 1, 2, 3, 4, 5
+K0_WORDS = 648
 
 
 class H8Grid(C.Structure):
-    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("variant", C.c_int32)]
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("variant", C.c_int32),
+                ("layers", C.c_int32)]
 
 
 class H8PcgStats(C.Structure):
@@ -56,6 +58,7 @@ SIGNATURES = {
     "h8_last_error": (C.c_char_p, []),
     "h8_version": (C.c_int, []),
     "h8_device_info": (C.c_int, [C.c_char_p, C.c_int]),
+    "h8_sector_table_f64": (C.c_int, [_P, _P]),
     "h8_ke_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
     "h8_diag_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
     "h8_spmv_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),

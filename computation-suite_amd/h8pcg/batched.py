@@ -59,7 +59,7 @@ class _Batch:
         nrhs = len(b_cols)
         self.inst.append(H8Instance(nx, ny, nz, nrhs, self.node_off, self.elem_off, self.vec_off))
         self.s_e.append(op.s_e)
-        self.K0.append(op.K0.reshape(-1))
+        self.K0.append(op.K0buf)
         self.dinv.append(op.dinv)
         self.b.extend(b_cols)
         self.ndofs.append((op.n_dof, nrhs))

@@ -41,7 +41,7 @@ def build_operator(nx, ny, nz, seed=5, device="cuda:0"):
 def time_kernels(op, precision, reps):
     from . import _lib
     ws = op.workspace(precision)
-    s_e, K0 = (op.s_e, op.K0) if precision == 64 else (op.s_e32, op.K0_32)
+    s_e, K0 = (op.s_e, op.K0buf) if precision == 64 else (op.s_e32, op.K0_32)
     avg = (C.c_double * 2)()
     _lib.check(_lib.lib().h8_time_kernels(C.byref(op.grid), precision, _lib.ptr(s_e), _lib.ptr(K0),
                                           _lib.ptr(op.dinv), _lib.ptr(ws), ws.numel(), int(reps), avg,

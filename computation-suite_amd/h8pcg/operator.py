@@ -89,18 +89,23 @@ class GridOperator:
         torch = require_gpu()
         self.device = device_of(device)
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
-        self.grid = H8Grid(self.nx, self.ny, self.nz, int(variant))
+        self.grid = H8Grid(self.nx, self.ny, self.nz, int(variant), 0)
         self.n_nodes = (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
         self.n_dof = 3 * self.n_nodes
         self.m = self.nx * self.ny * self.nz
         K0 = torch.as_tensor(K0, dtype=torch.float64, device=self.device)
         if K0.shape != (24, 24):
             raise ValueError("K0 must be 24x24")
-        self.K0 = brick_structured(K0)
+        # library layout: the 24x24 matrix then its 72-entry reflection-sector table
+        self.K0buf = torch.empty(_lib.K0_WORDS, dtype=torch.float64, device=self.device)
+        self.K0buf[:576] = brick_structured(K0).reshape(-1)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().h8_sector_table_f64(ptr(self.K0buf), self._stream()), "h8_sector_table_f64")
+        self.K0 = self.K0buf[:576].view(24, 24)
         self.s_e = torch.as_tensor(s_e, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
         if self.s_e.numel() != self.m:
             raise ValueError(f"s_e has {self.s_e.numel()} entries, the grid has {self.m} elements")
-        self.K0_32 = self.K0.float().contiguous()
+        self.K0_32 = self.K0buf.float().contiguous()
         self.s_e32 = self.s_e.float().contiguous()
         self.dinv = None
         self._ws = {}
@@ -137,7 +142,7 @@ class GridOperator:
         u = self._vec(u)
         y = torch.empty_like(u)
         with torch.cuda.device(self.device):
-            check(_lib.lib().h8_spmv_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(u), ptr(y),
+            check(_lib.lib().h8_spmv_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(u), ptr(y),
                                          self._stream()), "h8_spmv_f64")
         return y
 
@@ -162,7 +167,7 @@ class GridOperator:
         if preconditioner == "jacobi":
             flag = torch.zeros(1, dtype=torch.int32, device=self.device)
             with torch.cuda.device(self.device):
-                check(_lib.lib().h8_diag_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(fm), ptr(dinv),
+                check(_lib.lib().h8_diag_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(fm), ptr(dinv),
                                              ptr(flag), self._stream()), "")
         elif preconditioner is None or preconditioner == "none":
             dinv.copy_(fm.to(torch.float64))
@@ -194,7 +199,7 @@ class GridOperator:
         ws = self.workspace(64)
         st = H8PcgStats()
         with torch.cuda.device(self.device):
-            rc = _lib.lib().h8_pcg_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(self.dinv), ptr(b),
+            rc = _lib.lib().h8_pcg_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(self.dinv), ptr(b),
                                        ptr(x), 1 if x0 is not None else 0, float(tol), int(maxiter), ptr(ws),
                                        ws.numel(), C.byref(st), self._stream())
         stats = st.as_dict()
@@ -214,7 +219,7 @@ class GridOperator:
         ws = self.workspace(32)
         st = H8PcgStats()
         with torch.cuda.device(self.device):
-            rc = _lib.lib().h8_refine_solve(C.byref(self.grid), ptr(self.s_e), ptr(self.K0), ptr(self.s_e32),
+            rc = _lib.lib().h8_refine_solve(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32),
                                             ptr(self.K0_32), ptr(self.dinv), ptr(f), ptr(u), float(tol),
                                             int(maxiter), float(delta), ptr(ws), ws.numel(), C.byref(st),
                                             self._stream())
@@ -233,7 +238,7 @@ class GridOperator:
             raise ValueError("w must have one entry per element")
         out = torch.empty(self.m, dtype=torch.float64, device=self.device)
         with torch.cuda.device(self.device):
-            check(_lib.lib().h8_elem_bilinear_f64(C.byref(self.grid), ptr(self.K0), ptr(u), ptr(v), ptr(w),
+            check(_lib.lib().h8_elem_bilinear_f64(C.byref(self.grid), ptr(self.K0buf), ptr(u), ptr(v), ptr(w),
                                                   ptr(out), self._stream()), "h8_elem_bilinear_f64")
         return out
 

@@ -18,6 +18,11 @@
  * vectors have 3*(nx+1)(ny+1)(nz+1) entries; fixed DOFs are marked by a zero in
  * `dinv` (the inverse Jacobi diagonal), which keeps every Krylov vector zero
  * there -- the same iterates as the reduced system K_ff u_f = F_f.
+ *
+ * A `K0` argument is H8_K0_WORDS values: the 24x24 element matrix (row-major,
+ * entry 24*i + j) followed by its 72-value reflection-sector table, which
+ * h8_sector_table_f64 fills from the matrix (an FP32 K0 is the cast of the
+ * FP64 one).  Only the sector kernels read the table.
  */
 #ifndef H8PCG_H
 #define H8PCG_H
@@ -42,10 +47,15 @@ extern "C" {
 #define H8_VARIANT_NODE 1   /* node-centric gather (VALU), deterministic     */
 #define H8_VARIANT_MFMA 2   /* element-batched 24x24 contraction on MFMA     */
 #define H8_VARIANT_NODE_PERSIST 3 /* node-centric, persistent XCD-local brick runs */
+#define H8_VARIANT_NODE_SPLIT 4   /* node-centric, two threads per node (elements above / below) */
+#define H8_VARIANT_SECTOR 5       /* element products through the reflection sectors, marching in z */
+
+#define H8_K0_WORDS 648           /* 576 matrix entries + 72 sector-table entries */
 
 typedef struct h8_grid {
     int32_t nx, ny, nz;     /* elements per axis (>= 1)                      */
     int32_t variant;        /* H8_VARIANT_* for the K*u kernels              */
+    int32_t layers;         /* node layers per block of H8_VARIANT_SECTOR (0 = library choice) */
 } h8_grid;
 
 typedef struct h8_pcg_stats {
@@ -66,6 +76,12 @@ const char* h8_last_error(void);
 int h8_version(void);
 /* Writes the device name / CU count / arch of the current device. */
 int h8_device_info(char* buf, int buflen);
+
+/* Fills K0[576 .. 647] with the reflection-sector table of the 24x24 matrix in
+ * K0[0 .. 575] (device pointer): Kh[psi][c][c'] = (1/8) sum_x
+ * (-1)^popcount((psi ^ 2^c') & x) K0[c][3x + c'] for the corner bit masks x.
+ * No counterpart in the reference (a layout helper of this library). */
+int h8_sector_table_f64(double* K0, void* stream);
 
 /* ---- element stiffness ------------------------------------------------- */
 /* ke[e] = sum over the 2x2x2 Gauss rule of B^T D B detJ w, general geometry.
