@@ -80,8 +80,9 @@ KERNEL_NAMES = {  # (variant, precision) -> kernel symbol of the fused direction
     (1, 32): "k_node<float, 8, 8, 4, 1>", (1, 64): "k_node<double, 8, 8, 4, 1>",
     (2, 32): "k_mfma<float, 16, 8, 8, 1>", (2, 64): "k_mfma<double, 8, 8, 8, 1>",
     (3, 32): "k_node_persist<float, 8, 8, 4, 1>", (3, 64): "k_node_persist<double, 8, 8, 4, 1>",
+    (4, 32): "k_node2<float, 8, 8, 4, 1>", (4, 64): "k_node2<double, 8, 8, 4, 1>",
+    (5, 32): "k_sector<float, 16, 16, 1>", (5, 64): "k_sector<double, 16, 16, 1>",
 }
-DEFAULT_VARIANT = {32: 3, 64: 1}
 
 
 def profiled_traffic(dims, kernel):
@@ -136,6 +137,10 @@ def main():
     n_free = fs.n_dof - len(fs.fixed_dofs)
     m = nx * ny * nz
     log(f"setup {time.time() - t0:.1f}s: n_dof={fs.n_dof} n_free={n_free} m={m}")
+    # kernel choice per precision for this shape (timed once, outside the timed region)
+    fs.solve_static_device(tol=1e-2)           # builds dinv (free mask) for the tuner
+    tuned = op.autotune(cache=False)
+    log("autotune", tuned)
 
     def solve(precision, tol=1e-8):
         return fs.solve_static_device(tol=tol, precision=precision, delta=a.delta)
@@ -174,7 +179,7 @@ def main():
     dinv = op.dinv
     ws32 = op.workspace(32)
     avg = (C.c_double * 2)()
-    _lib.check(_lib.lib().h8_time_kernels(C.byref(op.grid), 32, _lib.ptr(op.s_e32), _lib.ptr(op.K0_32),
+    _lib.check(_lib.lib().h8_time_kernels(C.byref(op.grid32), 32, _lib.ptr(op.s_e32), _lib.ptr(op.K0_32),
                                           _lib.ptr(dinv), _lib.ptr(ws32), ws32.numel(), 200, avg,
                                           _lib.stream_handle(op.device)), "h8_time_kernels")
     dir_ms, upd_ms = avg[0], avg[1]
@@ -187,8 +192,7 @@ def main():
     dir_bytes = 5 * 4 * n_dof + 4 * m          # read r, dinv, p_old; write p, q; read s_e
     upd_bytes = 7 * 4 * n_dof                  # read x, r, p, q, dinv; write x, r
     achieved = dir_bytes / (dir_ms * 1e-3)
-    variant = op.grid.variant or DEFAULT_VARIANT[32]
-    kname = KERNEL_NAMES.get((variant, 32), "unknown")
+    kname = KERNEL_NAMES.get((op.grid32.variant, 32), "unknown")
     traffic, traffic_src = profiled_traffic((nx, ny, nz), kname)
     # whole-iteration contract: B_iter = 13 n_f s + m s_c (FP32), B_outer = 6 n_f 8 + 8 m
     s_ = stats[-1]
@@ -210,7 +214,9 @@ def main():
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"{nx}x{ny}x{nz} H8 cantilever, E=10**U(9,12) per element (seed {a.seed}), "
                                f"nu=0.3, clamped x=0, -1e6 consistent tip traction, true rel. residual < 1e-8",
-                   "n_free": n_free, "elements": m, "delta": a.delta},
+                   "n_free": n_free, "elements": m, "delta": a.delta,
+                   "kernels": {"fp32": {"variant": op.grid32.variant, "layers": op.grid32.layers},
+                               "fp64": {"variant": op.grid.variant, "layers": op.grid.layers}}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "traffic_source": traffic_src,

@@ -87,9 +87,15 @@ template <typename T> struct MfmaTile;
 template <> struct MfmaTile<float> { static constexpr int J = 16, I = 8, K = 8; };
 template <> struct MfmaTile<double> { static constexpr int J = 8, I = 8, K = 8; };
 
-template <typename T> static int default_variant();
-template <> int default_variant<float>() { return H8_VARIANT_NODE_PERSIST; }
-template <> int default_variant<double>() { return H8_VARIANT_NODE; }
+// Library default per precision and size (measured on MI355X): the sector
+// kernel in FP32; in FP64 the node-centric kernel below ~1M DOF (the sector
+// kernel's heavier FP64 register state costs more than its arithmetic saves
+// there) and the sector kernel above.
+template <typename T> static int default_variant(const GridDev& d);
+template <> int default_variant<float>(const GridDev&) { return H8_VARIANT_SECTOR; }
+template <> int default_variant<double>(const GridDev& d) {
+    return d.ndof >= 1000000 ? H8_VARIANT_SECTOR : H8_VARIANT_NODE;
+}
 
 static int make_grid(const h8_grid* g, GridDev& d) {
     if (!g) return fail(H8_ERR_ARG, "grid is NULL");
@@ -190,7 +196,7 @@ static int launch_sector(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
 template <typename T, int MODE>
 static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     GridDev d = d0;
-    const int v = d.variant == H8_VARIANT_AUTO ? default_variant<T>() : d.variant;
+    const int v = d.variant == H8_VARIANT_AUTO ? default_variant<T>(d) : d.variant;
     if (v == H8_VARIANT_MFMA) {
         constexpr int TJ = MfmaTile<T>::J, TI = MfmaTile<T>::I, TK = MfmaTile<T>::K;
         set_tiles(d, TJ, TI, TK);
@@ -218,8 +224,8 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
 
 // Streaming grid: ~8 elements per thread (two 16-byte groups in FP32, four in
 // FP64), at most 2048 blocks (the partials array holds one slot per block).
-static int stream_blocks(long long n) {
-    long long b = (n + STREAM_THREADS * 8 - 1) / (STREAM_THREADS * 8);
+static int stream_blocks(long long n, int per_thread = 8) {
+    long long b = (n + STREAM_THREADS * per_thread - 1) / (STREAM_THREADS * per_thread);
     // keep every CU busy: at least 2 blocks per CU when there is work for them
     b = std::max(b, std::min(512LL, (n + STREAM_THREADS - 1) / STREAM_THREADS));
     return int(std::max(1LL, std::min(b, 2048LL)));
@@ -379,6 +385,7 @@ static char* carve(char*& p, size_t bytes) {
 struct GraphKey {
     int kind;
     int variant;
+    int layers;
     int dev;
     long long nx, ny, nz;
     const void* ptr[12];
@@ -438,6 +445,12 @@ static int get_graph(const GraphKey& key, Body body, hipGraphExec_t* out) {
     return H8_OK;
 }
 
+// Blocks of the update kernel: the same bytes per thread in both precisions
+// (16 floats or 8 doubles of each vector), so FP32 keeps as many bytes in
+// flight per thread as FP64.
+template <typename T>
+static int update_blocks(long long n) { return stream_blocks(n, 64 / int(sizeof(T))); }
+
 // One PCG iteration = direction+product kernel (node-centric) + update kernel.
 template <typename T, bool REFINED>
 static int launch_iteration(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r,
@@ -447,7 +460,7 @@ static int launch_iteration(const GridDev& d, const T* s_e, const T* K0, const T
     A.st = st; A.partials = partials;
     int rc = launch_op<T, MODE_DIR>(d, A, s);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_pcg_update<T, REFINED>), dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s,
+    hipLaunchKernelGGL((k_pcg_update<T, REFINED>), dim3(update_blocks<T>(d.ndof)), dim3(STREAM_THREADS), 0, s,
                        d.ndof, x, r, (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials);
     LAUNCHCHK();
     return H8_OK;
@@ -552,6 +565,7 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
         GraphKey key{};
         key.kind = 1;
         key.variant = d.variant;
+        key.layers = d.layers;
         HIPCHK(hipGetDevice(&key.dev));
         key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
         const void* ptrs[] = {s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
@@ -657,6 +671,7 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     GraphKey key{};
     key.kind = 2;
     key.variant = d.variant;
+    key.layers = d.layers;
     HIPCHK(hipGetDevice(&key.dev));
     key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
     const void* ptrs[] = {s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
@@ -739,7 +754,8 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
             if (which == 0) {
                 rc = launch_op<T, MODE_DIR>(d, A, cs);
             } else {
-                hipLaunchKernelGGL((k_pcg_update<T, REFINED>), dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, x, r,
+                hipLaunchKernelGGL((k_pcg_update<T, REFINED>), dim3(update_blocks<T>(d.ndof)), dim3(STREAM_THREADS), 0,
+                                   cs, d.ndof, x, r,
                                    (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials);
             }
         }

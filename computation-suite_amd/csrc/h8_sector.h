@@ -72,11 +72,15 @@ __device__ __forceinline__ void sector_product(const T* kh, T (&U)[8][3], T scal
     T W[3][8];
 #pragma unroll
     for (int c = 0; c < 3; ++c) W[c][0] = T(0);    // the translation outputs of sectors e_c
-    const T* kp = kh;
-    asm volatile("" : "+s"(kp));
-    const auto K = as_const(kp);
+    T dep = V[0][1];
 #pragma unroll
     for (int psi = 0; psi < 8; ++psi) {
+        // this sector's 9 table values are loaded after the previous sector's
+        // products (data dependence through `dep`), so at most a few sectors'
+        // values occupy scalar registers at a time
+        const T* kp = kh;
+        asm volatile("" : "+s"(kp) : "v"(dep));
+        const auto K = as_const(kp);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             if (psi == (1 << c)) continue;          // zero row of a translation sector
@@ -91,6 +95,7 @@ __device__ __forceinline__ void sector_product(const T* kh, T (&U)[8][3], T scal
                 else acc += kv * x;
             }
             W[c][psi ^ (1 << c)] = acc;
+            dep = acc;
         }
     }
 #pragma unroll
@@ -113,8 +118,11 @@ struct SLayer {
     static constexpr int RW = 3 * (EJ + 1);            // words per row
     static constexpr int LW = RW * (EI + 1);           // words per layer
     static constexpr int PASS = (LW + NT - 1) / NT;
-    int ofs[PASS];      // byte offset of the word without the k term, OOB if out of the (i, j) domain
-    bool own[PASS];     // word of an owned node (in i, j)
+    // byte offsets without the k term: ofs = OOB outside the (i, j) domain,
+    // sofs = OOB unless the word belongs to an owned node.  Adding a k term
+    // (< 2^31 bytes) to OOB = 2^31 keeps it out of range, so no per-word flags
+    // are needed (they would occupy scalar registers as lane masks).
+    int ofs[PASS], sofs[PASS];
     T v0[PASS], v1[PASS], v2[PASS];
 
     __device__ __forceinline__ void init(const GridDev& g, int j0, int i0) {
@@ -125,44 +133,52 @@ struct SLayer {
             const int hj = r / 3;
             const int i = i0 - 1 + row, j = j0 - 1 + hj;
             const bool in = w < LW && i >= 0 && i < g.NI && j >= 0 && j < g.NJ;
-            ofs[u] = in ? int(sizeof(T)) * (3 * (i * g.NJ + j0 - 1) + r) : OOB;
-            own[u] = in && row >= 1 && row <= EI - 1 && hj >= 1 && hj <= EJ - 1;
+            const int o = int(sizeof(T)) * (3 * (i * g.NJ + j0 - 1) + r);
+            ofs[u] = in ? o : OOB;
+            sofs[u] = (in && row >= 1 && row <= EI - 1 && hj >= 1 && hj <= EJ - 1) ? o : OOB;
         }
     }
+    // global loads of layer k.  Branch-free: a layer outside the grid, and the
+    // previous direction in the first iteration, get the out-of-range bit ORed
+    // into their offsets and load zeros.
     template <int MODE>
     __device__ __forceinline__ void load(const NodeArgs<T>& A, const T* p_old, int first, int k) {
         const GridDev& g = A.g;
         const unsigned vbytes = unsigned(g.ndof) * sizeof(T);
         const bool kin = k >= 0 && k < g.NK;
-        const int kofs = int(sizeof(T)) * 3 * k * g.NI * g.NJ;
+        const unsigned kbad = kin ? 0u : unsigned(OOB);
+        const unsigned kofs = kin ? unsigned(sizeof(T)) * 3u * unsigned(k) * unsigned(g.NI) * unsigned(g.NJ) : 0u;
+        const unsigned pbad = first ? unsigned(OOB) : 0u;
 #pragma unroll
         for (int u = 0; u < PASS; ++u) {
-            const int off = (kin && ofs[u] != OOB) ? ofs[u] + kofs : OOB;
+            const unsigned off = (unsigned(ofs[u]) + kofs) | kbad;
             if constexpr (MODE == MODE_DIR) {
-                v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), off);
-                v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), off);
-                v2[u] = first ? T(0) : Buf<T>::ld(buf_rsrc(p_old, vbytes), off);
+                v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), int(off));
+                v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), int(off));
+                v2[u] = Buf<T>::ld(buf_rsrc(p_old, vbytes), int(off | pbad));
             } else {
-                v0[u] = Buf<T>::ld(buf_rsrc(A.u, vbytes), off);
+                v0[u] = Buf<T>::ld(buf_rsrc(A.u, vbytes), int(off));
             }
         }
     }
-    // form the layer's values, write the owned direction entries, store to LDS
+    // form the layer's values, write the owned direction entries (k_owned is
+    // wave-uniform), store to LDS
     template <int MODE>
     __device__ __forceinline__ void store(const NodeArgs<T>& A, T beta, T* p_new, int k, bool k_owned, T* slot) {
         const unsigned vbytes = unsigned(A.g.ndof) * sizeof(T);
-        const int kofs = int(sizeof(T)) * 3 * k * A.g.NI * A.g.NJ;
+        const unsigned kbad = k_owned ? 0u : unsigned(OOB);
+        const unsigned kofs = k_owned ? unsigned(sizeof(T)) * 3u * unsigned(k) * unsigned(A.g.NI) * unsigned(A.g.NJ) : 0u;
 #pragma unroll
         for (int u = 0; u < PASS; ++u) {
             const int w = threadIdx.x + NT * u;
             T val;
             if constexpr (MODE == MODE_DIR) {
                 val = v1[u] * v0[u] + beta * v2[u];
-                Buf<T>::st(val, buf_rsrc(p_new, vbytes), (k_owned && own[u]) ? ofs[u] + kofs : OOB);
+                Buf<T>::st(val, buf_rsrc(p_new, vbytes), int((unsigned(sofs[u]) + kofs) | kbad));
             } else {
                 val = v0[u];
             }
-            if (w < LW) slot[w] = val;
+            if (u + 1 < PASS || w < LW) slot[w] = val;
         }
     }
 };

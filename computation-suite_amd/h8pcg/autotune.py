@@ -1,0 +1,97 @@
+"""Choice of the K*u kernel (variant and layers per block) for one grid shape,
+by timing on the device.
+
+    tune(op, precision)  ->  (variant, layers, microseconds)
+
+Each candidate is timed with ``h8_time_kernels``: the fused direction + K*p
+kernel of one PCG iteration, launched `reps` times back to back in a HIP graph
+and timed with HIP events on the launching stream (the same kernel and launch
+pattern as inside a solve).  The fastest candidate wins.  Results are cached in
+a JSON file keyed by device, grid dims and precision, so a shape is timed once
+per machine: $H8PCG_TUNE_CACHE, else ~/.cache/h8pcg/tune.json (no cache when
+the file cannot be written).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+from . import _lib
+
+# (variant, layers): node-centric brick, persistent node-centric, and the
+# sector kernel at several marching depths (0 = the library's own choice)
+CANDIDATES = [
+    (_lib.VARIANT_NODE, 0),
+    (_lib.VARIANT_NODE_PERSIST, 0),
+    (_lib.VARIANT_SECTOR, 0),
+    (_lib.VARIANT_SECTOR, 2),
+    (_lib.VARIANT_SECTOR, 4),
+    (_lib.VARIANT_SECTOR, 8),
+]
+
+
+def cache_path():
+    return os.environ.get("H8PCG_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "h8pcg",
+                                                               "tune.json")
+
+
+def _device_key():
+    buf = C.create_string_buffer(256)
+    if _lib.lib().h8_device_info(buf, 256) != 0:
+        return "unknown"
+    return buf.value.decode(errors="replace").split("|cu=")[0]
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, data):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp%d" % os.getpid()
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def time_direction_kernel(op, precision, variant, layers, reps=30):
+    """Average microseconds of the fused direction kernel with this choice."""
+    import torch
+    g = _lib.H8Grid(op.nx, op.ny, op.nz, int(variant), int(layers))
+    ws = op.workspace(precision)
+    s_e, K0 = (op.s_e, op.K0buf) if precision == 64 else (op.s_e32, op.K0_32)
+    dinv = op.dinv if op.dinv is not None else torch.ones(op.n_dof, dtype=torch.float64, device=op.device)
+    avg = (C.c_double * 2)()
+    with torch.cuda.device(op.device):
+        _lib.check(_lib.lib().h8_time_kernels(C.byref(g), int(precision), _lib.ptr(s_e), _lib.ptr(K0),
+                                              _lib.ptr(dinv), _lib.ptr(ws), ws.numel(), int(reps), avg,
+                                              _lib.stream_handle(op.device)), "h8_time_kernels")
+    return avg[0] * 1e3
+
+
+def tune(op, precision, cache=True, candidates=None, reps=30):
+    key = "%s|%dx%dx%d|fp%d" % (_device_key(), op.nx, op.ny, op.nz, int(precision))
+    path = cache_path()
+    data = _load(path) if cache else {}
+    if key in data:
+        v, l, us = data[key]
+        return int(v), int(l), float(us)
+    best = None
+    for v, l in (candidates or CANDIDATES):
+        time_direction_kernel(op, precision, v, l, reps=3)        # warm-up (graph capture)
+        us = time_direction_kernel(op, precision, v, l, reps=reps)
+        if best is None or us < best[2]:
+            best = (v, l, us)
+    if cache:
+        data = _load(path)
+        data[key] = list(best)
+        _store(path, data)
+    return best

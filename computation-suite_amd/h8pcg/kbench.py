@@ -1,6 +1,6 @@
 """Kernel micro-benchmark for profiling the two PCG iteration kernels.
 
-    python -m h8pcg.kbench --dims 256,64,64 --reps 50 [--precision 32,64] [--copy-mib 256]
+    python -m h8pcg.kbench --dims 256,64,64 --reps 50 [--precision 32,64] [--variant V] [--layers L] [--copy-mib 256]
 
 Builds the clamped box operator (per-element scale 10**U(9,12), seed 5), then
 times the fused direction + K*p kernel and the update kernel of one PCG
@@ -61,11 +61,13 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--precision", default="32,64")
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--layers", type=int, default=0, help="node layers per block of the sector kernel (0 = auto)")
     ap.add_argument("--copy-mib", type=int, default=256)
     a = ap.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.dims.split(","))
     op = build_operator(nx, ny, nz)
     op.grid.variant = a.variant
+    op.grid.layers = a.layers
     out = {"dims": [nx, ny, nz], "n_dof": op.n_dof, "elements": op.m, "reps": a.reps}
     for prec in (int(p) for p in a.precision.split(",")):
         d_us, u_us = time_kernels(op, prec, a.reps)

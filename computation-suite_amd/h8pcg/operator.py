@@ -89,7 +89,10 @@ class GridOperator:
         torch = require_gpu()
         self.device = device_of(device)
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        # kernel choice per precision: `grid` for the FP64 operations, `grid32`
+        # for the FP32 ones (the refined solve's iterations); AUTO = library default
         self.grid = H8Grid(self.nx, self.ny, self.nz, int(variant), 0)
+        self.grid32 = H8Grid(self.nx, self.ny, self.nz, int(variant), 0)
         self.n_nodes = (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
         self.n_dof = 3 * self.n_nodes
         self.m = self.nx * self.ny * self.nz
@@ -152,7 +155,7 @@ class GridOperator:
         u = self._vec(u, torch.float32)
         y = torch.empty_like(u)
         with torch.cuda.device(self.device):
-            check(_lib.lib().h8_spmv_f32(C.byref(self.grid), ptr(self.s_e32), ptr(self.K0_32), ptr(u), ptr(y),
+            check(_lib.lib().h8_spmv_f32(C.byref(self.grid32), ptr(self.s_e32), ptr(self.K0_32), ptr(u), ptr(y),
                                          self._stream()), "h8_spmv_f32")
         return y
 
@@ -219,7 +222,7 @@ class GridOperator:
         ws = self.workspace(32)
         st = H8PcgStats()
         with torch.cuda.device(self.device):
-            rc = _lib.lib().h8_refine_solve(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32),
+            rc = _lib.lib().h8_refine_solve(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32),
                                             ptr(self.K0_32), ptr(self.dinv), ptr(f), ptr(u), float(tol),
                                             int(maxiter), float(delta), ptr(ws), ws.numel(), C.byref(st),
                                             self._stream())
@@ -240,6 +243,25 @@ class GridOperator:
         with torch.cuda.device(self.device):
             check(_lib.lib().h8_elem_bilinear_f64(C.byref(self.grid), ptr(self.K0buf), ptr(u), ptr(v), ptr(w),
                                                   ptr(out), self._stream()), "h8_elem_bilinear_f64")
+        return out
+
+    def set_variant(self, variant, layers=0, precision=None):
+        """Kernel variant (and sector-kernel layers per block) for FP64 (64), FP32
+        (32) or both (None) operations."""
+        for p, g in ((64, self.grid), (32, self.grid32)):
+            if precision in (None, p):
+                g.variant = int(variant)
+                g.layers = int(layers)
+
+    def autotune(self, precisions=(32, 64), cache=True):
+        """Times the candidate kernels for this grid and keeps the fastest per
+        precision (see h8pcg.autotune).  Returns {precision: (variant, layers, us)}."""
+        from .autotune import tune
+        out = {}
+        for p in precisions:
+            v, l, us = tune(self, p, cache=cache)
+            self.set_variant(v, l, p)
+            out[p] = (v, l, us)
         return out
 
     def set_scale(self, s_e):

@@ -213,3 +213,31 @@ def test_lazy_K_matches_oracle(h8, gpu):
     fs = _fesys(h8, 5, 3, 2)
     c = o.cantilever(5, 3, 2)
     assert abs(fs.K - c["K"]).max() < 1e-12 * abs(c["K"]).max()
+
+
+def test_autotune_choice_cache_and_solution(h8, gpu, tmp_path, monkeypatch):
+    say("autotune")
+    import json
+    from h8pcg import autotune
+    monkeypatch.setenv("H8PCG_TUNE_CACHE", str(tmp_path / "tune.json"))
+    nx, ny, nz = 24, 6, 6
+    fs = h8.FESystem(h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz), h8.Hex8Solid3D())
+    E = 10 ** np.random.default_rng(3).uniform(9, 11, nx * ny * nz)
+    fs.assemble_stiffness(h8.D_solid3d(h8.Material(E=1.0, nu=0.3)), element_scale=E)
+    fs.fix_dofs(fs.mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.add_nodal_force(fs.mesh.nodes_on_plane(0, float(nx)), 2, -1e3)
+    u_default = fs.solve_static_device(tol=1e-11).cpu().numpy()
+    it_default = fs.last_solve_stats["iterations"]
+    res = fs.operator.autotune()
+    cands = set(autotune.CANDIDATES)
+    for p in (32, 64):
+        v, l, us = res[p]
+        assert (v, l) in cands and us > 0
+    data = json.loads((tmp_path / "tune.json").read_text())
+    assert len(data) == 2
+    assert fs.operator.autotune() == res            # served from the cache
+    u = fs.solve_static_device(tol=1e-11).cpu().numpy()
+    assert np.linalg.norm(u - u_default) < 1e-9 * np.linalg.norm(u_default)
+    assert abs(fs.last_solve_stats["iterations"] - it_default) <= 2
+    u32 = fs.solve_static_device(tol=1e-9, precision="mixed").cpu().numpy()
+    assert np.linalg.norm(u32 - u_default) < 1e-7 * np.linalg.norm(u_default)
