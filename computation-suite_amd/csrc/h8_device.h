@@ -54,7 +54,19 @@ struct PcgState {
     int status;
     int first;         // next direction kernel uses p = z
     unsigned cnt[4];   // last-block counters (reset by the last block)
+    double best_true;  // refined: smallest true relative residual so far
+    long long stall;   // refined: replacements since it last fell by 1%
+    int stalled;       // refined: stopped because the true residual stagnated
+    long long k_rep;   // refined: iteration index at the last replacement
 };
+
+// Refined solve: replacements without a 1% drop of the true residual after
+// which the solve stops (the FP32 iterations cannot reduce it further).
+constexpr long long REFINE_STALL = 16;
+// An FP32 phase longer than this whose recurrence residual has risen above
+// the last replaced (true) residual is ended by a replacement: at the FP32
+// rounding floor the FP32 iterations can drift away instead of converging.
+constexpr long long REFINE_PHASE = 50;
 
 // ---------------------------------------------------------------------------
 // Reductions: wave shuffle tree + ordered LDS sum, then "last block" finish.

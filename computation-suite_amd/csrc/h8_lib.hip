@@ -643,6 +643,7 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     hst.tol = tol;
     hst.maxiter = maxiter;
     hst.delta = delta;
+    hst.best_true = INFINITY;
     hst.status = ST_RUNNING;
     HIPCHK(hipMemcpyAsync(w.st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pcg_init<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, f,
@@ -707,6 +708,11 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     if (hst.status == ST_NOT_SPD)
         return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
                     hst.bad_pq, hst.iters);
+    if (hst.status == ST_MAXITER && hst.stalled)
+        return fail(H8_MAXITER,
+                    "refined solve: the true relative residual stagnated at %.3e (tol=%.1e) after %lld FP32 "
+                    "iterations -- FP32 iterations cannot reduce it further; use the FP64 solve or a larger tol.",
+                    hst.true_rel, tol, (long long)hst.iters);
     if (hst.status == ST_MAXITER)
         return fail(H8_MAXITER,
                     "refined solve: did not converge in %lld iterations (true relative residual %.3e >= "

@@ -88,14 +88,28 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
     } else if (st->k - 1 >= st->maxiter) {
         st->status = ST_MAXITER;
         st->iters = st->k - 1;
+    } else if (!(st->true_rel < 0.99 * st->best_true) && ++st->stall >= REFINE_STALL) {
+        st->status = ST_MAXITER;       // stagnated (or not finite)
+        st->stalled = 1;
+        st->iters = st->k - 1;
     } else {
-        st->rmax = rn;
-        if (st->first) {
-            st->rz_old = rz;
-        } else {
-            st->beta = rz / st->rz_old;
-            st->rz_old = rz;
+        if (st->true_rel < 0.99 * st->best_true) {
+            st->best_true = st->true_rel;
+            st->stall = 0;
         }
+        st->rmax = rn;
+        st->k_rep = st->k;
+        const double b = st->first ? 0.0 : rz / st->rz_old;
+        if (!st->first && !(b < 1.0 / (st->delta * st->delta))) {
+            // the true residual has not fallen below the previous replacement's
+            // level (beta >= 1/delta^2): keeping the direction would scale it by
+            // beta at every such replacement, so restart it (p = z)
+            st->first = 1;
+            st->beta = 0.0;
+        } else {
+            st->beta = b;
+        }
+        st->rz_old = rz;
         st->rel = st->true_rel;
         st->status = ST_RUNNING;
     }

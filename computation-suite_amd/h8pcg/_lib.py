@@ -99,6 +99,10 @@ def lib():
             raise H8LibraryMissing(
                 f"libh8pcg.so not found at {LIB_PATH}: build it with "
                 f"`python computation-suite_amd/build.py` (or __graft_entry__.build()).")
+        # torch bundles its own HIP runtime with the same soname as the system
+        # one this library links against; loading torch first makes the process
+        # use a single runtime (torch's), shared by torch and this library.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name, None)

@@ -71,12 +71,14 @@ class SIMPProblem:
         self.op.set_free_mask(self.free_mask)
         return s, ds
 
-    def solve(self, rho, rhs=None, x0=None, tol=None):
-        """u with K(rho) u = rhs (default F) on the free DOFs."""
+    def solve(self, rho, rhs=None, x0=None, tol=None, precision=None):
+        """u with K(rho) u = rhs (default F) on the free DOFs (precision: the
+        problem's unless given; a warm start x0 always uses the FP64 solve)."""
         self._set_rho(rho)
         b = self.op.masked(self.F if rhs is None else rhs.to(torch_f64()).contiguous())
         tol = self.tol if tol is None else tol
-        if self.precision == "mixed" and x0 is None:
+        precision = self.precision if precision is None else precision
+        if precision == "mixed" and x0 is None:
             u, st = self.op.refined(b, tol=tol, maxiter=self.maxiter)
         else:
             u, st = self.op.pcg(b, tol=tol, maxiter=self.maxiter, x0=x0)
@@ -167,10 +169,11 @@ def displacement(rho, problem):
 
 def fd_check(problem, rho, elements, h=1e-6, tol=1e-12):
     """Cancellation-free central differences of C at the given elements:
-    C(rho+h e) - C(rho-h e) = -(s+ - s-) u-_e^T K0 u+_e (K symmetric), with the
-    perturbed solves warm-started from u(rho).  Returns (fd, u) as numpy."""
+    C(rho+h e) - C(rho-h e) = -(s+ - s-) u-_e^T K0 u+_e (K symmetric), all three
+    solves in FP64 to `tol`, the perturbed ones warm-started from u(rho).
+    Returns (fd, u) as numpy."""
     import torch
-    u = problem.solve(rho, tol=tol)
+    u = problem.solve(rho, tol=tol, precision="fp64")
     out = []
     s_all = simp_scale(rho, problem.p, problem.rho_min)
     for e in elements:

@@ -1,0 +1,138 @@
+"""Full-size runs of the SIMP-gradient and batched workloads on one GPU.
+
+    python -m h8pcg.workload_runs simp --dims 96,32,16 --rho 0.3,1.0 --seed 3 --fd 64
+    python -m h8pcg.workload_runs simp --dims 256,64,64 --rho 0.2,1.0 --seed 5 --fd 32 --precision mixed
+    python -m h8pcg.workload_runs batched --seed 4 --n 256 --tol 1e-8
+
+simp: SIMP cantilever on an nx x ny x nz unit-cube grid, x = 0 face clamped,
+F_z = -1 at the node (nx, ny/2, 0); stiffness scale s = rho_min + (1 - rho_min)
+rho^p (p = 3, rho_min = 1e-9, E0 = 1, nu = 0.3) with rho ~ U(lo, hi) per element.
+Compliance and dC/drho go through the autograd Function (forward solution
+stored); then central differences with h = 1e-6 at `fd` elements drawn by
+rng.choice(m, fd, replace=False) (a second generator, seed + 1), each from two
+FP64 solves warm-started from u at tol 1e-12, in the cancellation-free form
+C(rho + h) - C(rho - h) = -(s+ - s-) u-_e^T K0 u+_e.  Prints one JSON line with
+the timings and the largest relative difference between FD and gradient.
+
+batched: solve_many over batched_cantilevers(seed, n) (nx from {8, 12, .., 48},
+ny = nz = nx/4, 4..16 unit tip-traction load cases each, E = 10**U(9, 11) per
+element); prints systems, DOF, iterations, device time and DOF*iterations/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def simp(a):
+    import torch
+    from . import topopt
+    nx, ny, nz = (int(v) for v in a.dims.split(","))
+    lo, hi = (float(v) for v in a.rho.split(","))
+    m = nx * ny * nz
+    NJ, NI = ny + 1, nx + 1
+    n_dof = 3 * NJ * NI * (nz + 1)
+    F = np.zeros(n_dof)
+    if ny % 2:
+        raise SystemExit("simp: ny must be even (the load node is (nx, ny/2, 0))")
+    F[3 * (nx * NJ + ny // 2) + 2] = -1.0
+    k, j = np.meshgrid(np.arange(nz + 1), np.arange(NJ), indexing="ij")
+    clamp = (k * NI * NJ + j).ravel()
+    fixed = (3 * clamp[:, None] + np.arange(3)).ravel()
+    rng = np.random.default_rng(a.seed)
+    rho_np = rng.uniform(lo, hi, m)
+    t0 = time.time()
+    pb = topopt.SIMPProblem(nx, ny, nz, F, fixed, p=3.0, rho_min=1e-9, device="cuda:0", tol=a.tol,
+                            precision=a.precision)
+    if a.autotune:
+        pb.op.set_free_mask(pb.free_mask)
+        pb.op.autotune()
+    rho = torch.as_tensor(rho_np, device=pb.device).requires_grad_(True)
+    setup_s = time.time() - t0
+    _sync()
+    t1 = time.perf_counter()
+    C = topopt.compliance(rho, pb)
+    _sync()
+    t2 = time.perf_counter()
+    C.backward()
+    _sync()
+    t3 = time.perf_counter()
+    fwd = dict(pb.last_stats)
+    g = rho.grad.detach().cpu().numpy()
+    out = {"workload": "simp", "dims": [nx, ny, nz], "n_dof": n_dof, "n_free": n_dof - len(fixed), "elements": m,
+           "rho": [lo, hi], "seed": a.seed, "precision": a.precision, "tol": a.tol, "setup_s": setup_s,
+           "compliance": float(C), "forward_ms": 1e3 * (t2 - t1), "gradient_ms": 1e3 * (t3 - t2),
+           "forward_stats": {k_: fwd.get(k_) for k_ in ("iterations", "inner_iterations", "outer",
+                                                        "rel_residual", "true_rel_residual")}}
+    if a.fd > 0:
+        els = np.random.default_rng(a.seed + 1).choice(m, a.fd, replace=False)
+        t4 = time.perf_counter()
+        fd = []
+        for c0 in range(0, len(els), 8):          # in chunks, with a progress line per chunk
+            part, _ = topopt.fd_check(pb, rho.detach(), els[c0:c0 + 8], h=1e-6, tol=1e-12)
+            fd.extend(part)
+            print("[simp] fd %d/%d elements, %.1f s" % (c0 + len(part), len(els), time.perf_counter() - t4),
+                  file=sys.stderr, flush=True)
+        fd = np.array(fd)
+        _sync()
+        rel = np.abs(fd - g[els]) / np.abs(g[els])
+        out.update({"fd_elements": int(a.fd), "fd_h": 1e-6, "fd_s": time.perf_counter() - t4,
+                    "fd_max_rel_diff": float(rel.max()), "fd_median_rel_diff": float(np.median(rel))})
+    print(json.dumps(out), flush=True)
+
+
+def batched(a):
+    from .batched import solve_many
+    from .workloads import batched_cantilevers
+    t0 = time.time()
+    systems = batched_cantilevers(seed=a.seed, n=a.n, device="cuda:0")
+    setup_s = time.time() - t0
+    dofs = [fs.n_dof for fs, F in systems for _ in range(F.shape[1])]
+    free = [fs.n_dof - len(fs.fixed_dofs) for fs, F in systems for _ in range(F.shape[1])]
+    solve_many(systems[:2], tol=a.tol)          # warm-up (graph capture of a small batch)
+    _sync()
+    t1 = time.perf_counter()
+    X, st = solve_many(systems, tol=a.tol)
+    _sync()
+    wall = time.perf_counter() - t1
+    iters = st["iterations"]
+    dof_iters = float(np.dot(free, iters))
+    print(json.dumps({"workload": "batched", "seed": a.seed, "instances": a.n, "systems": len(dofs),
+                      "total_dof": int(sum(dofs)), "total_free_dof": int(sum(free)), "tol": a.tol,
+                      "setup_s": setup_s, "wall_ms": 1e3 * wall, "device_ms": st["elapsed_ms"],
+                      "launches": st["launches"], "iterations_min": int(iters.min()),
+                      "iterations_max": int(iters.max()), "iterations_mean": float(iters.mean()),
+                      "all_converged": bool(np.all(st["status"] == 0)),
+                      "dof_iters_per_s": dof_iters / wall}), flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("simp")
+    s.add_argument("--dims", default="96,32,16")
+    s.add_argument("--rho", default="0.3,1.0")
+    s.add_argument("--seed", type=int, default=3)
+    s.add_argument("--fd", type=int, default=64)
+    s.add_argument("--tol", type=float, default=1e-10)
+    s.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    s.add_argument("--autotune", action="store_true")
+    b = sub.add_parser("batched")
+    b.add_argument("--seed", type=int, default=4)
+    b.add_argument("--n", type=int, default=256)
+    b.add_argument("--tol", type=float, default=1e-8)
+    a = ap.parse_args(argv)
+    simp(a) if a.cmd == "simp" else batched(a)
+
+
+if __name__ == "__main__":
+    main()

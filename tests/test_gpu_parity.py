@@ -241,3 +241,16 @@ def test_autotune_choice_cache_and_solution(h8, gpu, tmp_path, monkeypatch):
     assert abs(fs.last_solve_stats["iterations"] - it_default) <= 2
     u32 = fs.solve_static_device(tol=1e-9, precision="mixed").cpu().numpy()
     assert np.linalg.norm(u32 - u_default) < 1e-7 * np.linalg.norm(u_default)
+
+
+def test_refined_stagnation_is_reported(h8, gpu):
+    """An unreachable tolerance ends the FP32-refined solve with a clear error
+    once the true residual stops falling (not after maxiter FP32 iterations)."""
+    say("refined stagnation")
+    nx, ny, nz = 12, 4, 4
+    fs = h8.FESystem(h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz), h8.Hex8Solid3D())
+    fs.assemble_stiffness(h8.D_solid3d(h8.Material(E=210e9, nu=0.3)))
+    fs.fix_dofs(fs.mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.add_nodal_force(fs.mesh.nodes_on_plane(0, float(nx)), 2, -1e6)
+    with pytest.raises(RuntimeError, match="stagnated"):
+        fs.solve_static_device(tol=1e-18, precision="mixed", maxiter=10**7)
