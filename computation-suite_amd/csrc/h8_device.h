@@ -58,15 +58,16 @@ struct PcgState {
     long long stall;   // refined: replacements since it last fell by 1%
     int stalled;       // refined: stopped because the true residual stagnated
     long long k_rep;   // refined: iteration index at the last replacement
+    long long max_phase;  // refined: longest FP32 phase (iterations between replacements)
 };
 
 // Refined solve: replacements without a 1% drop of the true residual after
 // which the solve stops (the FP32 iterations cannot reduce it further).
 constexpr long long REFINE_STALL = 16;
-// An FP32 phase longer than this whose recurrence residual has risen above
-// the last replaced (true) residual is ended by a replacement: at the FP32
-// rounding floor the FP32 iterations can drift away instead of converging.
-constexpr long long REFINE_PHASE = 50;
+// An FP32 phase longer than max(REFINE_PHASE, 4 x the longest phase so far) is
+// ended by a replacement: at the FP32 rounding floor the FP32 iterations can
+// drift away instead of reaching the next replacement threshold.
+constexpr long long REFINE_PHASE = 500;
 
 // ---------------------------------------------------------------------------
 // Reductions: wave shuffle tree + ordered LDS sum, then "last block" finish.

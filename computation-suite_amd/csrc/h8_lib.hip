@@ -746,6 +746,8 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
     h.bnorm = 1.0;
     h.tol = 0.0;
     h.maxiter = (1LL << 62);
+    h.rmax = INFINITY;            // no replacement trigger fires while timing
+    h.k_rep = (1LL << 62);
     NodeArgs<T> A{};
     A.g = d; A.s_e = s_e; A.K0 = K0; A.r = r; A.dinv = dinv; A.p0 = p0; A.p1 = p1; A.q = q;
     A.st = st; A.partials = partials;

@@ -81,6 +81,7 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
     const double rn = sqrt(rr);
     st->true_rel = rn / st->bnorm;
     st->outer += 1;
+    const bool progress = st->true_rel < 0.99 * st->best_true;   // false for NaN
     if (st->true_rel < st->tol) {
         st->status = ST_DONE;
         st->iters = st->k - 1;
@@ -88,22 +89,23 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
     } else if (st->k - 1 >= st->maxiter) {
         st->status = ST_MAXITER;
         st->iters = st->k - 1;
-    } else if (!(st->true_rel < 0.99 * st->best_true) && ++st->stall >= REFINE_STALL) {
+    } else if (!progress && ++st->stall >= REFINE_STALL) {
         st->status = ST_MAXITER;       // stagnated (or not finite)
         st->stalled = 1;
         st->iters = st->k - 1;
     } else {
-        if (st->true_rel < 0.99 * st->best_true) {
+        if (progress) {
             st->best_true = st->true_rel;
             st->stall = 0;
         }
-        st->rmax = rn;
+        if (st->k - st->k_rep > st->max_phase) st->max_phase = st->k - st->k_rep;
         st->k_rep = st->k;
+        st->rmax = rn;
         const double b = st->first ? 0.0 : rz / st->rz_old;
-        if (!st->first && !(b < 1.0 / (st->delta * st->delta))) {
-            // the true residual has not fallen below the previous replacement's
-            // level (beta >= 1/delta^2): keeping the direction would scale it by
-            // beta at every such replacement, so restart it (p = z)
+        if (!st->first && (!progress || !(b < 1.0 / (st->delta * st->delta)))) {
+            // no progress since the previous replacement: keeping the direction
+            // would carry (and, with beta >= 1/delta^2, amplify) a direction that
+            // no longer reduces the true residual, so restart it (p = z)
             st->first = 1;
             st->beta = 0.0;
         } else {

@@ -127,8 +127,8 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
         } else {
             st->first = 0;
             st->k = k + 1;
-            if (rel < st->tol || rn < st->delta * st->rmax || (rn > st->rmax && k - st->k_rep > REFINE_PHASE) ||
-                k >= st->maxiter) {
+            const long long phase_cap = REFINE_PHASE > 4 * st->max_phase ? REFINE_PHASE : 4 * st->max_phase;
+            if (rel < st->tol || rn < st->delta * st->rmax || k - st->k_rep > phase_cap || k >= st->maxiter) {
                 st->status = ST_REPLACE;
             } else {
                 st->beta = tot[0] / st->rz_old;
