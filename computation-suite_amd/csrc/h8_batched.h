@@ -9,18 +9,18 @@
 //
 //   k_binit   : flat chunk table over systems; r = mask(b), |b|, |r|, r.z; the
 //               last block of each system applies the early-out test.
-//   k_bdir    : flat brick table over instances; a block stages the element
-//               scales of its brick once, then for each running load case of
-//               the instance stages p = dinv*r + beta*p_old, computes q = K p
-//               on the brick (same node-centric arithmetic as k_node) and
-//               writes its p.q partial; the last block of the instance forms
-//               alpha (or flags p.q <= 0) for every running case.
+//   k_bsector : flat table of (system, column tile, layer chunk); a block
+//               runs the sector march of the single-system kernel on its
+//               system (p = dinv*r + beta*p_old formed while staging, q = K p,
+//               p.q partial); the last block of each system forms alpha (or
+//               flags p.q <= 0).
 //   k_bupdate : flat chunk table over systems; x, r update, r.z and r.r
 //               partials; the last block of each system applies the stopping
 //               rule of the single-system solver and retires finished systems.
 #pragma once
 #include "h8_device.h"
 #include "h8_node.h"
+#include "h8_sector.h"
 
 namespace h8 {
 
@@ -29,9 +29,9 @@ struct BInst {            // device copy of one instance
     long long node_off;   // dinv at 3*node_off
     long long elem_off;   // s_e at elem_off
     long long vec_off;    // case c at vec_off + c*ndof
-    int tile_off, ntiles, sys_off, k0_off;   // k0_off: index of the instance's K0 (x H8_K0_WORDS)
+    int sys_off, k0_off;  // first system; k0_off: index of the instance's K0 (x H8_K0_WORDS)
 };
-struct BTile { int inst, tile; };
+struct BSTile { int sys, tile; };   // one block of the batched sector kernel
 struct BChunk { int sys; int start; int len; int pad; };
 
 struct BSys {             // per-system solver state
@@ -46,10 +46,8 @@ struct BSys {             // per-system solver state
 
 struct BArgs {
     const BInst* inst;
-    const BTile* tiles;
     const BChunk* chunks;
     BSys* sys;
-    unsigned* inst_cnt;   // last-block counters of the bricks of each instance
     int* active;          // systems still RUNNING
     double* partials;     // per system: pstride doubles
     int pstride;
@@ -170,70 +168,52 @@ k_binit(BArgs B, BVecs<T> V) {
     }
 }
 
-template <typename T, int TJ, int TI, int TK>
-__global__ void __launch_bounds__(TJ * TI * TK)
-k_bdir(BArgs B, BVecs<T> V) {
-    constexpr int NT = TJ * TI * TK;
-    typedef Halo<T, TJ, TI, TK, NT> Hc;
-    __shared__ T sP[3 * Hc::HJ * Hc::HI * Hc::HK];
-    __shared__ T sS[Hc::NE];
+// Batched direction + K*p through the reflection sectors: one block per
+// (system, column tile, layer chunk) of a flat table; blocks of systems no
+// longer running exit at once.  Each block runs the march of the single-system
+// sector kernel on its system's vectors and instance grid, writes its p.q
+// partial, and the last block of the system (ticket on the system's counter)
+// forms alpha or retires the system on p.q <= 0.
+template <typename T, int EJ, int EI>
+__global__ void __launch_bounds__(EJ * EI) __attribute__((amdgpu_waves_per_eu(SectorWaves<T>::value)))
+k_bsector(BArgs B, BVecs<T> V, const BSTile* tiles, int kd) {
+    __shared__ SectorSmem<T, EJ, EI> sm;
     __shared__ double sRed[16 * 4];
-    const BTile Tl = B.tiles[blockIdx.x];
-    const BInst& I = B.inst[Tl.inst];
+    const BSTile Tl = tiles[blockIdx.x];
+    BSys& S = B.sys[Tl.sys];
+    if (S.status != ST_RUNNING) return;
+    const BInst& I = B.inst[S.inst];
     NodeArgs<T> A{};
-    A.g = inst_grid(I, TJ, TI, TK);
+    A.g = inst_grid(I, EJ, EI, kd);
+    sector_tiles(A.g, EJ, EI, kd);
     A.s_e = V.s_e + I.elem_off;
     A.K0 = V.K0 + (size_t)I.k0_off * 648;   // H8_K0_WORDS
     A.dinv = V.dinv + 3 * I.node_off;
-    int j0, i0, k0;
-    tile_origin(A.g, Tl.tile, TJ, TI, TK, j0, i0, k0);
-    bool any = false;
-    for (int c = 0; c < I.nrhs; ++c) {
-        BSys& S = B.sys[I.sys_off + c];
-        double part[2] = {0.0, 0.0};
-        if (S.status == ST_RUNNING) {
-            any = true;
-            const long long off = S.off;
-            A.r = V.r + off;
-            A.p0 = V.p0 + off;
-            A.p1 = V.p1 + off;
-            A.q = V.q + off;
-            const bool odd = (S.k & 1) != 0;
-            T* p_new = odd ? A.p1 : A.p0;
-            const T* p_old = odd ? A.p0 : A.p1;
-            {
-                Hc H;
-                halo_load<T, TJ, TI, TK, NT, MODE_DIR>(A, j0, i0, k0, S.first, p_old, H);
-                halo_store<T, TJ, TI, TK, NT, MODE_DIR>(H, T(S.beta), p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS,
-                                                         (T*)nullptr);
-            }
-            __syncthreads();
-            node_brick<T, TJ, TI, TK, MODE_DIR>(A, j0, i0, k0, A.K0, sP, sS, part);
-            __syncthreads();
+    A.r = V.r + S.off;
+    A.p0 = V.p0 + S.off;
+    A.p1 = V.p1 + S.off;
+    A.q = V.q + S.off;
+    const bool odd = (S.k & 1) != 0;
+    T* p_new = odd ? A.p1 : A.p0;
+    const T* p_old = odd ? A.p0 : A.p1;
+    double part[2] = {0.0, 0.0};
+    sector_march<T, EJ, EI, MODE_DIR>(A, A.K0 + 576, kd, Tl.tile, T(S.beta), S.first, p_new, p_old, sm, part);
+    double v[1] = {part[0]};
+    block_sum<1>(v, sRed);
+    double* pp = B.partials + (size_t)Tl.sys * B.pstride;
+    if (threadIdx.x == 0) st_relaxed(&pp[Tl.tile], v[0]);
+    if (!take_ticket(&S.cnt, A.g.ntiles)) return;
+    double tot[1];
+    sum_partials<1>(pp, A.g.ntiles, tot, sRed);
+    if (threadIdx.x == 0) {
+        const double pq = tot[0];
+        if (!(pq > 0.0)) {
+            S.bad_pq = pq;
+            S.iters = S.k;
+            retire(B, S, ST_NOT_SPD);
+        } else {
+            S.alpha = S.rz_old / pq;
         }
-        double v[1] = {part[0]};
-        block_sum<1>(v, sRed);
-        if (threadIdx.x == 0)
-            st_relaxed(&B.partials[(size_t)(I.sys_off + c) * B.pstride + Tl.tile], v[0]);
-    }
-    (void)any;
-    if (!take_ticket(&B.inst_cnt[Tl.inst], I.ntiles)) return;
-    for (int c = 0; c < I.nrhs; ++c) {
-        BSys& S = B.sys[I.sys_off + c];
-        if (S.status != ST_RUNNING) continue;
-        double tot[1];
-        sum_partials<1>(B.partials + (size_t)(I.sys_off + c) * B.pstride, I.ntiles, tot, sRed);
-        if (threadIdx.x == 0) {
-            const double pq = tot[0];
-            if (!(pq > 0.0)) {
-                S.bad_pq = pq;
-                S.iters = S.k;
-                retire(B, S, ST_NOT_SPD);
-            } else {
-                S.alpha = S.rz_old / pq;
-            }
-        }
-        __syncthreads();
     }
 }
 

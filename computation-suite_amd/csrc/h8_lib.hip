@@ -171,16 +171,14 @@ constexpr int SEJ = 16, SEI = 16;
 template <typename T, int MODE>
 static int launch_sector(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     GridDev d = d0;
-    const int TJ = SEJ - 1, TI = SEI - 1;
-    d.tj = (d.NJ + TJ - 1) / TJ;
-    d.ti = (d.NI + TI - 1) / TI;
+    sector_tiles(d, SEJ, SEI, 1);
     int kd = d.layers;
     if (kd <= 0) {
         const long long cols = (long long)d.tj * d.ti;
         const long long want = 4LL * device_cus();
         kd = int(std::max(2LL, std::min(16LL, (d.NK * cols + want - 1) / want)));
     }
-    d.tk = (d.NK + kd - 1) / kd;
+    sector_tiles(d, SEJ, SEI, kd);
     const long long nt = (long long)d.tj * d.ti * d.tk;
     if (MODE != MODE_PLAIN && nt > std::max<long long>(d0.ntiles, 2048))
         return fail(H8_ERR_ARG, "sector kernel: %lld blocks exceed the partials capacity (use more layers)", nt);
@@ -845,11 +843,15 @@ extern "C" int h8_time_kernels(const h8_grid* g, int precision, const void* s_e,
 // ---------------------------------------------------------------------------
 namespace {
 constexpr int BCH = 2048;   // update/init chunk (elements); k_binit/k_bupdate assume it
+// sector march of the batched direction kernel: 8 x 16 element columns (7 x 15
+// owned nodes: small columns waste few threads on the small instances of a
+// batch) and chunks of 16 node layers (a whole instance of up to 16 layers in
+// one march, without the extra slab of a chunk boundary)
+constexpr int BEJ = 8, BEI = 16, BKD = 16;
 
 struct BLayout {
     int n_sys = 0, n_tiles = 0, n_chunks = 0, pstride = 0;
-    size_t off_r, off_p0, off_p1, off_q, off_inst, off_tiles, off_chunks, off_sys, off_icnt, off_active,
-        off_part, total;
+    size_t off_r, off_p0, off_p1, off_q, off_inst, off_tiles, off_chunks, off_sys, off_active, off_part, total;
 };
 
 int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLayout& L) {
@@ -863,8 +865,9 @@ int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLa
         h8_grid g{I.nx, I.ny, I.nz, 0};
         int rc = make_grid(&g, d);
         if (rc) return rc;
+        sector_tiles(d, BEJ, BEI, BKD);
         L.n_sys += I.nrhs;
-        L.n_tiles += d.ntiles;
+        L.n_tiles += d.ntiles * I.nrhs;
         const int nch = int((d.ndof + BCH - 1) / BCH);
         L.n_chunks += nch * I.nrhs;
         maxt = std::max(maxt, d.ntiles);
@@ -878,10 +881,9 @@ int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLa
     L.off_p1 = take(sizeof(double) * total_vec);
     L.off_q = take(sizeof(double) * total_vec);
     L.off_inst = take(sizeof(BInst) * n_inst);
-    L.off_tiles = take(sizeof(BTile) * L.n_tiles);
+    L.off_tiles = take(sizeof(BSTile) * L.n_tiles);
     L.off_chunks = take(sizeof(BChunk) * L.n_chunks);
     L.off_sys = take(sizeof(BSys) * L.n_sys);
-    L.off_icnt = take(sizeof(unsigned) * n_inst);
     L.off_active = take(sizeof(int));
     L.off_part = take(sizeof(double) * (size_t)L.n_sys * L.pstride);
     L.total = p;
@@ -908,7 +910,7 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
     if (!(tol >= 0.0) || maxiter < 0) return fail(H8_ERR_ARG, "bad tol / maxiter");
     // host tables
     std::vector<BInst> hi(n_inst);
-    std::vector<BTile> ht;
+    std::vector<BSTile> ht;
     std::vector<BChunk> hc;
     std::vector<BSys> hs;
     ht.reserve(L.n_tiles);
@@ -926,10 +928,12 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
         BInst& B = hi[i];
         B.nx = I.nx; B.ny = I.ny; B.nz = I.nz; B.nrhs = I.nrhs;
         B.node_off = I.node_off; B.elem_off = I.elem_off; B.vec_off = I.vec_off;
-        B.tile_off = int(ht.size()); B.ntiles = d.ntiles; B.sys_off = int(hs.size());
+        B.sys_off = int(hs.size());
         B.k0_off = k0_per_instance ? i : 0;
-        for (int t = 0; t < d.ntiles; ++t) ht.push_back(BTile{i, t});
+        GridDev ds = d;
+        sector_tiles(ds, BEJ, BEI, BKD);
         for (int c = 0; c < I.nrhs; ++c) {
+            for (int t = 0; t < ds.ntiles; ++t) ht.push_back(BSTile{int(hs.size()), t});
             BSys S{};
             S.status = ST_RUNNING;
             S.inst = i;
@@ -946,10 +950,8 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
     char* w = (char*)ws;
     BArgs A{};
     A.inst = (const BInst*)(w + L.off_inst);
-    A.tiles = (const BTile*)(w + L.off_tiles);
     A.chunks = (const BChunk*)(w + L.off_chunks);
     A.sys = (BSys*)(w + L.off_sys);
-    A.inst_cnt = (unsigned*)(w + L.off_icnt);
     A.active = (int*)(w + L.off_active);
     A.partials = (double*)(w + L.off_part);
     A.pstride = L.pstride;
@@ -965,10 +967,10 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, s));
     HIPCHK(hipMemcpyAsync((void*)A.inst, hi.data(), sizeof(BInst) * hi.size(), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync((void*)A.tiles, ht.data(), sizeof(BTile) * ht.size(), hipMemcpyHostToDevice, s));
+    const BSTile* stiles = (const BSTile*)(w + L.off_tiles);
+    HIPCHK(hipMemcpyAsync((void*)stiles, ht.data(), sizeof(BSTile) * ht.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync((void*)A.chunks, hc.data(), sizeof(BChunk) * hc.size(), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync((void*)A.sys, hs.data(), sizeof(BSys) * hs.size(), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, (long long)n_inst, (int*)A.inst_cnt, 0);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, A.active, L.n_sys);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_binit<double>, dim3(L.n_chunks), dim3(256), 0, s, A, V);
@@ -988,8 +990,8 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
         hipGraphExec_t exec;
         rc = get_graph(key, [&](hipStream_t cs) -> int {
             for (int it = 0; it < key.chunk; ++it) {
-                hipLaunchKernelGGL((k_bdir<double, NTJ, NTI, NTK>), dim3(L.n_tiles), dim3(NTJ * NTI * NTK), 0, cs, A,
-                                   V);
+                hipLaunchKernelGGL((k_bsector<double, BEJ, BEI>), dim3(L.n_tiles), dim3(BEJ * BEI), 0, cs, A, V,
+                                   stiles, BKD);
                 hipLaunchKernelGGL(k_bupdate<double>, dim3(L.n_chunks), dim3(256), 0, cs, A, V);
             }
             LAUNCHCHK();

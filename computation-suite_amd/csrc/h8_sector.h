@@ -187,28 +187,35 @@ template <typename T> struct SectorWaves;
 template <> struct SectorWaves<double> { static constexpr int value = 4; };
 template <> struct SectorWaves<float> { static constexpr int value = 6; };
 
-// One block per (column tile, layer chunk); A.g.{tj, ti, tk} count the column
-// tiles in j and i and the chunks of `kd` node layers.
+// LDS of one march: the 3-slot layer ring and the per-node corner slots.
+template <typename T, int EJ, int EI>
+struct SectorSmem {
+    // 16-byte aligned members: a node's 12 slot values are read with 128-bit loads
+    alignas(16) T ring[3 * SLayer<T, EJ, EI>::LW];
+    alignas(16) T lo[EJ * EI * 12];
+    alignas(16) T hi[EJ * EI * 12];
+};
+
+// Column tiles of the sector march for a grid: (EJ-1) x (EI-1) nodes, chunks of kd layers.
+__host__ __device__ inline void sector_tiles(GridDev& g, int EJ, int EI, int kd) {
+    g.tj = (g.NJ + EJ - 2) / (EJ - 1);
+    g.ti = (g.NI + EI - 2) / (EI - 1);
+    g.tk = (g.NK + kd - 1) / kd;
+    g.ntiles = g.tj * g.ti * g.tk;
+}
+
+// The march of one block over `tile` (column tile + layer chunk) of grid A.g:
+// stages layers, forms p (MODE_DIR; written for owned nodes), evaluates the
+// element products through the sectors and writes each owned node's outputs
+// (node_output), accumulating the fused partials in `part`.
 template <typename T, int EJ, int EI, int MODE>
-__global__ void __launch_bounds__(EJ * EI) __attribute__((amdgpu_waves_per_eu(SectorWaves<T>::value)))
-k_sector(NodeArgs<T> A, const T* kh, int kd) {
-    constexpr int NT = EJ * EI;
+__device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, int kd, int tile, T beta, int first,
+                                             T* p_new, const T* p_old, SectorSmem<T, EJ, EI>& sm,
+                                             double (&part)[2]) {
     constexpr int TJ = EJ - 1, TI = EI - 1;
     typedef SLayer<T, EJ, EI> L;
-    static_assert(NT % 64 == 0, "whole waves");
-    __shared__ T sRing[3 * L::LW];
-    __shared__ T sLo[NT * 12];
-    __shared__ T sHi[NT * 12];
-    __shared__ double sRed[16 * 4];
     constexpr bool OFFS = sizeof(T) == 4;   // subtract the corner-0 value first (FP32 rounding)
-
     const GridDev& g = A.g;
-    T beta;
-    int first;
-    T* p_new;
-    const T* p_old;
-    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
-    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
     const int bj = tile % g.tj, bi = (tile / g.tj) % g.ti, bk = tile / (g.tj * g.ti);
     const int j0 = bj * TJ, i0 = bi * TI;
     const int kb = bk * kd, ke = min(g.NK, kb + kd);
@@ -221,6 +228,9 @@ k_sector(NodeArgs<T> A, const T* kh, int kd) {
     const int nj = j0 - 1 + ej, ni = i0 - 1 + ei;
     const bool nown = ej >= 1 && ei >= 1 && nj < g.NJ && ni < g.NI;
     const int lbase = 3 * (ei * (EJ + 1) + ej);             // LDS word of corner 0 within a layer
+    T* sRing = sm.ring;
+    T* sLo = sm.lo;
+    T* sHi = sm.hi;
 
     L lay;
     lay.init(g, j0, i0);
@@ -234,7 +244,6 @@ k_sector(NodeArgs<T> A, const T* kh, int kd) {
     __syncthreads();
 
     T carry[3] = {T(0), T(0), T(0)};
-    double part[2] = {0.0, 0.0};
     for (int s = kb - 1; s < ke; ++s) {
         const int q = s - (kb - 1);                          // step index
         const T* lo = sRing + (q % 3) * L::LW;               // layer s
@@ -298,6 +307,24 @@ k_sector(NodeArgs<T> A, const T* kh, int kd) {
         }
         __syncthreads();
     }
+}
+
+// One block per (column tile, layer chunk); A.g.{tj, ti, tk} count the column
+// tiles in j and i and the chunks of `kd` node layers.
+template <typename T, int EJ, int EI, int MODE>
+__global__ void __launch_bounds__(EJ * EI) __attribute__((amdgpu_waves_per_eu(SectorWaves<T>::value)))
+k_sector(NodeArgs<T> A, const T* kh, int kd) {
+    static_assert((EJ * EI) % 64 == 0, "whole waves");
+    __shared__ SectorSmem<T, EJ, EI> sm;
+    __shared__ double sRed[16 * 4];
+    T beta;
+    int first;
+    T* p_new;
+    const T* p_old;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    double part[2] = {0.0, 0.0};
+    sector_march<T, EJ, EI, MODE>(A, kh, kd, xcd_local_tile(blockIdx.x, gridDim.x), beta, first, p_new, p_old, sm,
+                                  part);
     node_finish<T, MODE>(A, part, sRed);
 }
 
