@@ -82,6 +82,7 @@ KERNEL_NAMES = {  # (variant, precision) -> kernel symbol of the fused direction
     (3, 32): "k_node_persist<float, 8, 8, 4, 1>", (3, 64): "k_node_persist<double, 8, 8, 4, 1>",
     (4, 32): "k_node2<float, 8, 8, 4, 1>", (4, 64): "k_node2<double, 8, 8, 4, 1>",
     (5, 32): "k_sector<float, 16, 16, 1>", (5, 64): "k_sector<double, 16, 16, 1>",
+    (6, 32): "k_sector<float, 8, 16, 1>", (6, 64): "k_sector<double, 8, 16, 1>",
 }
 
 

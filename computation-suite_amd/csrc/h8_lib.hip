@@ -114,7 +114,7 @@ static int make_grid(const h8_grid* g, GridDev& d) {
         return fail(H8_ERR_ARG, "grid too large: %lld DOFs (the kernels address vectors with 32-bit byte offsets; "
                     "at most 2^28 - 1 DOFs)", d.ndof);
     d.ntiles = int(nt);
-    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_SECTOR)
+    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_SECTOR_NARROW)
         return fail(H8_ERR_ARG, "unknown kernel variant %d", g->variant);
     if (g->layers < 0) return fail(H8_ERR_ARG, "layers must be >= 0 (0 = library choice), got %d", g->layers);
     d.variant = g->variant;
@@ -167,24 +167,23 @@ static int persistent_grid(const void* kernel, int threads, int ntiles) {
 // node layers (library choice: about 4 blocks per CU, 2..16 layers).  The
 // partials buffer of the fused modes holds max(brick tiles, 2048) blocks, so
 // the launch is refused if it would need more.
-constexpr int SEJ = 16, SEI = 16;
-template <typename T, int MODE>
+template <typename T, int MODE, int EJ, int EI>
 static int launch_sector(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     GridDev d = d0;
-    sector_tiles(d, SEJ, SEI, 1);
+    sector_tiles(d, EJ, EI, 1);
     int kd = d.layers;
     if (kd <= 0) {
         const long long cols = (long long)d.tj * d.ti;
         const long long want = 4LL * device_cus();
         kd = int(std::max(2LL, std::min(16LL, (d.NK * cols + want - 1) / want)));
     }
-    sector_tiles(d, SEJ, SEI, kd);
+    sector_tiles(d, EJ, EI, kd);
     const long long nt = (long long)d.tj * d.ti * d.tk;
     if (MODE != MODE_PLAIN && nt > std::max<long long>(d0.ntiles, 2048))
         return fail(H8_ERR_ARG, "sector kernel: %lld blocks exceed the partials capacity (use more layers)", nt);
     d.ntiles = int(nt);
     A.g = d;
-    hipLaunchKernelGGL((k_sector<T, SEJ, SEI, MODE>), dim3(d.ntiles), dim3(SEJ * SEI), 0, s, A, A.K0 + 576, kd);
+    hipLaunchKernelGGL((k_sector<T, EJ, EI, MODE>), dim3(d.ntiles), dim3(EJ * EI), 0, s, A, A.K0 + 576, kd);
     LAUNCHCHK();
     return H8_OK;
 }
@@ -204,7 +203,9 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
         set_tiles(d, NTJ, NTI, NTK);
         A.g = d;
         if (v == H8_VARIANT_SECTOR) {
-            return launch_sector<T, MODE>(d0, A, s);
+            return launch_sector<T, MODE, 16, 16>(d0, A, s);    // 16 x 16 element columns
+        } else if (v == H8_VARIANT_SECTOR_NARROW) {
+            return launch_sector<T, MODE, 8, 16>(d0, A, s);     // 8 x 16: half the LDS per block
         } else if (v == H8_VARIANT_NODE_SPLIT) {
             hipLaunchKernelGGL((k_node2<T, NTJ, NTI, NTK, MODE>), dim3(d.ntiles), dim3(2 * NTJ * NTI * NTK), 0, s, A);
         } else if (v == H8_VARIANT_NODE_PERSIST) {

@@ -22,7 +22,8 @@ H8_MAXITER = 4
 H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
 
-VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR = 0, 1, 2, 3, 4, 5
+(VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR,
+ VARIANT_SECTOR_NARROW) = 0, 1, 2, 3, 4, 5, 6
 K0_WORDS = 648
 
 

@@ -22,8 +22,8 @@ from . import _lib
 TIE = 0.02
 
 # (variant, layers), in order of preference on near-ties: node-centric brick,
-# the sector kernel with the library's own depth, persistent node-centric, and
-# the sector kernel at other marching depths
+# the sector kernel with the library's own depth, persistent node-centric, the
+# sector kernel at other marching depths, and the narrow-column sector kernel
 CANDIDATES = [
     (_lib.VARIANT_NODE, 0),
     (_lib.VARIANT_SECTOR, 0),
@@ -31,6 +31,9 @@ CANDIDATES = [
     (_lib.VARIANT_SECTOR, 2),
     (_lib.VARIANT_SECTOR, 4),
     (_lib.VARIANT_SECTOR, 8),
+    (_lib.VARIANT_SECTOR_NARROW, 0),
+    (_lib.VARIANT_SECTOR_NARROW, 4),
+    (_lib.VARIANT_SECTOR_NARROW, 8),
 ]
 
 

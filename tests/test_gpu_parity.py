@@ -57,7 +57,7 @@ def test_spmv_f64_f32_match_oracle_csr(h8, gpu, dims):
     K = o.assemble_csr(nodes, elems, o.d_solid3d(1.0, 0.3), scale=s)
     u = rng.normal(size=K.shape[0])
     ref = K @ u
-    for variant in (1, 2, 3, 4, 5):  # node-centric, element MFMA, persistent, two threads per node, sectors
+    for variant in (1, 2, 3, 4, 5, 6):  # node, element MFMA, persistent, two per node, sectors, narrow sectors
         op.grid.variant = variant
         y = op.matvec(torch.as_tensor(u, device=gpu)).cpu().numpy()
         e64 = np.abs(y - ref).max() / np.abs(ref).max()
@@ -102,7 +102,7 @@ def test_config1_pcg_matches_reference_and_spsolve(h8, gpu, golden):
     assert np.all(np.asarray(U)[fs.fixed_dofs_array] == 0.0)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 def test_pcg_variants_agree(h8, gpu, variant):
     say("pcg variant", variant)
     rng = np.random.default_rng(4)
