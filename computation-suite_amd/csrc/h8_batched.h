@@ -14,9 +14,11 @@
 //               system (p = dinv*r + beta*p_old formed while staging, q = K p,
 //               p.q partial); the last block of each system forms alpha (or
 //               flags p.q <= 0).
-//   k_bupdate : flat chunk table over systems; x, r update, r.z and r.r
+//   k_bupdate : flat chunk table over systems; r update, r.z and r.r
 //               partials; the last block of each system applies the stopping
 //               rule of the single-system solver and retires finished systems.
+//               (x += alpha p is applied by the next k_bsector while it stages
+//               p_old, and after the loop by k_bfinal.)
 #pragma once
 #include "h8_device.h"
 #include "h8_node.h"
@@ -38,6 +40,7 @@ struct BSys {             // per-system solver state
     double rz_old, alpha, beta, bnorm, rel, bad_pq;
     long long k, iters;
     int status, first;
+    int pending;          // x lacks alpha * p of the last direction kernel
     unsigned cnt;         // last-block counter of this system's chunks
     int inst;             // owning instance
     long long off;        // vector offset
@@ -155,6 +158,7 @@ k_binit(BArgs B, BVecs<T> V) {
         S.first = 1;
         S.beta = 0.0;
         S.alpha = 0.0;
+        S.pending = 0;
         if (S.rel < B.tol) {
             S.iters = 0;
             retire(B, S, ST_DONE);
@@ -193,11 +197,13 @@ k_bsector(BArgs B, BVecs<T> V, const BSTile* tiles, int kd) {
     A.p0 = V.p0 + S.off;
     A.p1 = V.p1 + S.off;
     A.q = V.q + S.off;
+    A.x = V.x + S.off;
     const bool odd = (S.k & 1) != 0;
     T* p_new = odd ? A.p1 : A.p0;
     const T* p_old = odd ? A.p0 : A.p1;
     double part[2] = {0.0, 0.0};
-    sector_march<T, EJ, EI, MODE_DIR>(A, A.K0 + 576, kd, Tl.tile, T(S.beta), S.first, p_new, p_old, sm, part);
+    sector_march<T, EJ, EI, MODE_DIRX>(A, A.K0 + 576, kd, Tl.tile, T(S.beta), S.first, p_new, p_old, T(S.alpha),
+                                      S.pending, sm, part);
     double v[1] = {part[0]};
     block_sum<1>(v, sRed);
     double* pp = B.partials + (size_t)Tl.sys * B.pstride;
@@ -210,9 +216,11 @@ k_bsector(BArgs B, BVecs<T> V, const BSTile* tiles, int kd) {
         if (!(pq > 0.0)) {
             S.bad_pq = pq;
             S.iters = S.k;
+            S.pending = 0;
             retire(B, S, ST_NOT_SPD);
         } else {
             S.alpha = S.rz_old / pq;
+            S.pending = 1;       // x += alpha p: next direction kernel, or k_bfinal
         }
     }
 }
@@ -226,15 +234,12 @@ k_bupdate(BArgs B, BVecs<T> V) {
     if (S.status != ST_RUNNING) return;
     const BInst& I = B.inst[S.inst];
     const T* dinv = V.dinv + 3 * I.node_off;
-    const T* p = ((S.k & 1) ? V.p1 : V.p0) + S.off;
-    T* x = V.x + S.off;
     T* r = V.r + S.off;
     const T* q = V.q + S.off;
     const T alpha = T(S.alpha);
     double v[2] = {0.0, 0.0};
     for (int e = C.start + threadIdx.x; e < C.start + C.len; e += blockDim.x) {
         const T d = dinv[e];
-        x[e] += alpha * p[e];
         const T ri = (d != T(0)) ? T(r[e] - alpha * q[e]) : T(0);
         r[e] = ri;
         v[0] += double(ri) * double(d * ri);
@@ -268,6 +273,20 @@ k_bupdate(BArgs B, BVecs<T> V) {
             S.k = k + 1;
         }
     }
+}
+
+// After the batched loop: x += alpha p of the last direction kernel for every
+// system whose update is still pending (flat chunk table).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_bfinal(BArgs B, BVecs<T> V) {
+    const BChunk C = B.chunks[blockIdx.x];
+    const BSys& S = B.sys[C.sys];
+    if (!S.pending) return;
+    const T* p = ((S.k & 1) ? V.p1 : V.p0) + S.off;
+    T* x = V.x + S.off;
+    const T alpha = T(S.alpha);
+    for (int e = C.start + threadIdx.x; e < C.start + C.len; e += blockDim.x) x[e] += alpha * p[e];
 }
 
 }  // namespace h8

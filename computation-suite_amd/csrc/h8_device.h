@@ -59,6 +59,8 @@ struct PcgState {
     int stalled;       // refined: stopped because the true residual stagnated
     long long k_rep;   // refined: iteration index at the last replacement
     long long max_phase;  // refined: longest FP32 phase (iterations between replacements)
+    int pending;       // x still lacks alpha * p of the last direction kernel: the next
+                       // direction kernel (or the final/fold kernel) applies it
 };
 
 // Refined solve: replacements without a 1% drop of the true residual after

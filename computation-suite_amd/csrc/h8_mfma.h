@@ -62,20 +62,22 @@ k_mfma(NodeArgs<T> A) {
     __shared__ double sRed[16 * 4];
 
     const GridDev& g = A.g;
-    if constexpr (MODE == MODE_DIR) {
+    if constexpr (is_dir(MODE)) {
         if (A.st->status != ST_RUNNING) return;
     }
     int j0, i0, k0;
     tile_origin(g, xcd_local_tile(blockIdx.x, gridDim.x), TJ, TI, TK, j0, i0, k0);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    T beta = T(0);
-    int first = 1;
+    T beta = T(0), alpha_prev = T(0);
+    int first = 1, pending = 0;
     T* p_new = nullptr;
     const T* p_old = nullptr;
-    if constexpr (MODE == MODE_DIR) {
+    if constexpr (is_dir(MODE)) {
         beta = T(A.st->beta);
         first = A.st->first;
+        alpha_prev = T(A.st->alpha);
+        pending = A.st->pending;
         const bool odd = (A.st->k & 1) != 0;
         p_new = odd ? A.p1 : A.p0;
         p_old = odd ? A.p0 : A.p1;
@@ -93,7 +95,7 @@ k_mfma(NodeArgs<T> A) {
         }
 
     // ---- stage the halo and element scales (same layout as the node kernel), zero accumulators
-    stage_halo<T, TJ, TI, TK, MODE, NT>(A, j0, i0, k0, beta, first, p_old, p_new, sP, sS, sA);
+    stage_halo<T, TJ, TI, TK, MODE, NT>(A, j0, i0, k0, beta, first, p_old, p_new, alpha_prev, pending, sP, sS, sA);
     __syncthreads();
 
     // ---- 8 parity colours; batches of 16 same-colour elements per MFMA pass
@@ -158,7 +160,7 @@ k_mfma(NodeArgs<T> A) {
             const T qv = sA[li];
             if constexpr (MODE == MODE_PLAIN) {
                 A.y[gn + c] = qv;
-            } else if constexpr (MODE == MODE_DIR) {
+            } else if constexpr (is_dir(MODE)) {
                 A.q[gn + c] = qv;
                 part[0] += double(sP[li]) * double(qv);
             } else {
@@ -172,7 +174,7 @@ k_mfma(NodeArgs<T> A) {
         }
     }
 
-    if constexpr (MODE == MODE_DIR) {
+    if constexpr (is_dir(MODE)) {
         double v[1] = {part[0]}, tot[1];
         if (reduce_last_block<1>(v, A.partials, &A.st->cnt[0], tot, sRed)) {
             if (threadIdx.x == 0) dir_finish(A.st, tot[0]);

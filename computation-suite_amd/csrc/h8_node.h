@@ -28,7 +28,10 @@
 
 namespace h8 {
 
-enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2 };
+enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3 };
+// MODE_DIRX = MODE_DIR that also applies the previous iteration's deferred
+// x += alpha p_old to the owned nodes while staging p_old.
+__host__ __device__ constexpr bool is_dir(int mode) { return mode == MODE_DIR || mode == MODE_DIRX; }
 
 template <typename T>
 struct NodeArgs {
@@ -42,6 +45,7 @@ struct NodeArgs {
     T* p0;            // DIR: direction buffers, iteration k writes p[k&1]
     T* p1;            //      and reads the previous direction from p[(k-1)&1]
     T* q;             // DIR
+    T* x;             // DIR: iterate; x += alpha p_old is applied to owned nodes while staging
     const double* f;  // RESID
     float* r32;       // RESID
     const float* dinv32;  // RESID
@@ -69,8 +73,10 @@ __device__ inline void dir_finish(PcgState* st, double pq) {
         st->bad_pq = pq;
         st->iters = st->k;
         st->status = ST_NOT_SPD;
+        st->pending = 0;
     } else {
         st->alpha = st->rz_old / pq;
+        st->pending = 1;       // x += alpha p is applied by the next direction kernel
     }
 }
 
@@ -81,6 +87,7 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
     const double rn = sqrt(rr);
     st->true_rel = rn / st->bnorm;
     st->outer += 1;
+    st->pending = 0;           // the fold before this replacement applied it
     const bool progress = st->true_rel < 0.99 * st->best_true;   // false for NaN
     if (st->true_rel < st->tol) {
         st->status = ST_DONE;
@@ -152,7 +159,7 @@ struct Halo {
     static constexpr int EJ = TJ + 1, EI = TI + 1, EK = TK + 1;
     static constexpr int NE = EJ * EI * EK;
     static constexpr int PES = (NE + NT - 1) / NT;
-    T v0[PER], v1[PER], v2[PER];
+    T v0[PER], v1[PER], v2[PER], xv[PER];
     int own[PER];     // byte offset of an owned node's word, OOB otherwise
     T sv[PES];
 };
@@ -179,7 +186,7 @@ __device__ __forceinline__ void halo_row(int u, int slot, int& ii, int& kk, bool
 }
 
 template <typename T, int TJ, int TI, int TK, int NT, int MODE>
-__device__ __forceinline__ void halo_load(const NodeArgs<T>& A, int j0, int i0, int k0, int first,
+__device__ __forceinline__ void halo_load(const NodeArgs<T>& A, int j0, int i0, int k0, int first, int pending,
                                           const T* p_old, Halo<T, TJ, TI, TK, NT>& H) {
     typedef Halo<T, TJ, TI, TK, NT> Hc;
     const GridDev& g = A.g;
@@ -201,10 +208,11 @@ __device__ __forceinline__ void halo_load(const NodeArgs<T>& A, int j0, int i0, 
         const int off = in ? int(sizeof(T)) * (k * plane3 + 3 * i * g.NJ + wofs) : OOB;
         const bool owned = in && jown && ii >= 1 && ii <= TI && kk >= 1 && kk <= TK;
         H.own[u] = owned ? off : OOB;
-        if constexpr (MODE == MODE_DIR) {
+        if constexpr (is_dir(MODE)) {
             H.v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), off);
             H.v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), off);
             H.v2[u] = first ? T(0) : Buf<T>::ld(buf_rsrc(p_old, vbytes), off);
+            if constexpr (MODE == MODE_DIRX) H.xv[u] = Buf<T>::ld(buf_rsrc(A.x, vbytes), pending ? H.own[u] : OOB);
         } else {
             H.v0[u] = Buf<T>::ld(buf_rsrc(A.u, vbytes), off);
         }
@@ -221,8 +229,8 @@ __device__ __forceinline__ void halo_load(const NodeArgs<T>& A, int j0, int i0, 
 }
 
 template <typename T, int TJ, int TI, int TK, int NT, int MODE>
-__device__ __forceinline__ void halo_store(const Halo<T, TJ, TI, TK, NT>& H, T beta, T* p_new, unsigned vbytes,
-                                           T* sP, T* sS, T* sZero) {
+__device__ __forceinline__ void halo_store(const Halo<T, TJ, TI, TK, NT>& H, T beta, T* p_new, T alpha_prev,
+                                           int pending, T* x, unsigned vbytes, T* sP, T* sS, T* sZero) {
     typedef Halo<T, TJ, TI, TK, NT> Hc;
     const int w = threadIdx.x % Hc::LW, slot = threadIdx.x / Hc::LW;
 #pragma unroll
@@ -231,9 +239,12 @@ __device__ __forceinline__ void halo_store(const Halo<T, TJ, TI, TK, NT>& H, T b
         bool rv;
         halo_row<Hc::HI, Hc::HK, Hc::RPP>(u, slot, ii, kk, rv);
         T val;
-        if constexpr (MODE == MODE_DIR) {
+        if constexpr (is_dir(MODE)) {
             val = H.v1[u] * H.v0[u] + beta * H.v2[u];
             Buf<T>::st(val, buf_rsrc(p_new, vbytes), H.own[u]);
+            // the previous iteration's x += alpha p_old, for the owned nodes
+            if constexpr (MODE == MODE_DIRX)
+                Buf<T>::st(H.xv[u] + alpha_prev * H.v2[u], buf_rsrc(x, vbytes), pending ? H.own[u] : OOB);
         } else {
             val = H.v0[u];
         }
@@ -277,7 +288,7 @@ __device__ __forceinline__ void node_output(const NodeArgs<T>& A, int gn, const 
     if constexpr (MODE == MODE_PLAIN) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) Buf<T>::st(qv[c], buf_rsrc(A.y, vbytes), int(sizeof(T)) * (gn + c));
-    } else if constexpr (MODE == MODE_DIR) {
+    } else if constexpr (is_dir(MODE)) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             Buf<T>::st(qv[c], buf_rsrc(A.q, vbytes), int(sizeof(T)) * (gn + c));
@@ -377,7 +388,7 @@ __device__ __forceinline__ void node_brick(const NodeArgs<T>& A, int j0, int i0,
 
 template <typename T, int MODE>
 __device__ __forceinline__ void node_finish(const NodeArgs<T>& A, double (&part)[2], double* sRed) {
-    if constexpr (MODE == MODE_DIR) {
+    if constexpr (is_dir(MODE)) {
         double v[1] = {part[0]}, tot[1];
         if (reduce_last_block<1>(v, A.partials, &A.st->cnt[0], tot, sRed)) {
             if (threadIdx.x == 0) dir_finish(A.st, tot[0]);
@@ -391,15 +402,20 @@ __device__ __forceinline__ void node_finish(const NodeArgs<T>& A, double (&part)
 }
 
 template <typename T, int MODE>
-__device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& first, T*& p_new, const T*& p_old) {
+__device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& first, T*& p_new, const T*& p_old,
+                                           T& alpha_prev, int& pending) {
     beta = T(0);
     first = 1;
     p_new = nullptr;
     p_old = nullptr;
-    if constexpr (MODE == MODE_DIR) {
+    alpha_prev = T(0);
+    pending = 0;
+    if constexpr (is_dir(MODE)) {
         if (A.st->status != ST_RUNNING) return false;
         beta = T(A.st->beta);
         first = A.st->first;
+        alpha_prev = T(A.st->alpha);
+        pending = A.st->pending;
         const bool odd = (A.st->k & 1) != 0;
         p_new = odd ? A.p1 : A.p0;
         p_old = odd ? A.p0 : A.p1;
@@ -416,17 +432,18 @@ k_node(NodeArgs<T> A) {
     __shared__ T sP[3 * Hc::HJ * Hc::HI * Hc::HK];
     __shared__ T sS[Hc::NE];
     __shared__ double sRed[16 * 4];
-    T beta;
-    int first;
+    T beta, alpha_prev;
+    int first, pending;
     T* p_new;
     const T* p_old;
-    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old, alpha_prev, pending)) return;
     int j0, i0, k0;
     tile_origin(A.g, xcd_local_tile(blockIdx.x, gridDim.x), TJ, TI, TK, j0, i0, k0);
     {
         Hc H;
-        halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
-        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, (T*)nullptr);
+        halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, pending, p_old, H);
+        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, alpha_prev, pending, A.x, unsigned(A.g.ndof) * sizeof(T),
+                                            sP, sS, (T*)nullptr);
     }
     __syncthreads();
     double part[2] = {0.0, 0.0};
@@ -525,17 +542,18 @@ k_node2(NodeArgs<T> A) {
     __shared__ T sS[Hc::NE];
     __shared__ T sQ[3 * NN];
     __shared__ double sRed[16 * 4];
-    T beta;
-    int first;
+    T beta, alpha_prev;
+    int first, pending;
     T* p_new;
     const T* p_old;
-    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old, alpha_prev, pending)) return;
     int j0, i0, k0;
     tile_origin(A.g, xcd_local_tile(blockIdx.x, gridDim.x), TJ, TI, TK, j0, i0, k0);
     {
         Hc H;
-        halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
-        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, (T*)nullptr);
+        halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, pending, p_old, H);
+        halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, alpha_prev, pending, A.x, unsigned(A.g.ndof) * sizeof(T),
+                                            sP, sS, (T*)nullptr);
     }
     __syncthreads();
     const int half = threadIdx.x / NN;
@@ -577,11 +595,11 @@ k_node_persist(NodeArgs<T> A) {
     __shared__ T sP[3 * Hc::HJ * Hc::HI * Hc::HK];
     __shared__ T sS[Hc::NE];
     __shared__ double sRed[16 * 4];
-    T beta;
-    int first;
+    T beta, alpha_prev;
+    int first, pending;
     T* p_new;
     const T* p_old;
-    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old, alpha_prev, pending)) return;
     int tile, tend, tstep;
     tile_range(A.g.ntiles, tile, tend, tstep);
     double part[2] = {0.0, 0.0};
@@ -590,8 +608,9 @@ k_node_persist(NodeArgs<T> A) {
         tile_origin(A.g, tile, TJ, TI, TK, j0, i0, k0);
         {
             Hc H;
-            halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
-            halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, (T*)nullptr);
+            halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, pending, p_old, H);
+            halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, alpha_prev, pending, A.x,
+                                                unsigned(A.g.ndof) * sizeof(T), sP, sS, (T*)nullptr);
         }
         __syncthreads();
         // opaque per-brick copy of the K0 pointer keeps the table reads in the loop
@@ -606,10 +625,12 @@ k_node_persist(NodeArgs<T> A) {
 // Element-batched kernels stage with the same layout (non-persistent form).
 template <typename T, int TJ, int TI, int TK, int MODE, int NT>
 __device__ __forceinline__ void stage_halo(const NodeArgs<T>& A, int j0, int i0, int k0, T beta, int first,
-                                           const T* p_old, T* p_new, T* sP, T* sS, T* sZero) {
+                                           const T* p_old, T* p_new, T alpha_prev, int pending, T* sP, T* sS,
+                                           T* sZero) {
     Halo<T, TJ, TI, TK, NT> H;
-    halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, p_old, H);
-    halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, unsigned(A.g.ndof) * sizeof(T), sP, sS, sZero);
+    halo_load<T, TJ, TI, TK, NT, MODE>(A, j0, i0, k0, first, pending, p_old, H);
+    halo_store<T, TJ, TI, TK, NT, MODE>(H, beta, p_new, alpha_prev, pending, A.x, unsigned(A.g.ndof) * sizeof(T), sP,
+                                        sS, sZero);
 }
 
 }  // namespace h8

@@ -23,7 +23,7 @@
 // lies between node layers s and s+1) upwards with one thread per element of
 // the EJ x EI slab (the slab's elements cover the owned nodes plus a one-node
 // halo ring).  Node layers are staged into a 3-slot LDS ring (the direction
-// p = dinv*r + beta*p_old is formed here in MODE_DIR, and written out for the
+// p = dinv*r + beta*p_old is formed here in the direction modes, and written out for the
 // owned nodes); each element writes the outputs of its four bottom corners
 // (for node layer s) and its four top corners (for layer s+1) into per-node
 // slots, and every owned node sums its slots in a fixed order (deterministic)
@@ -123,7 +123,7 @@ struct SLayer {
     // (< 2^31 bytes) to OOB = 2^31 keeps it out of range, so no per-word flags
     // are needed (they would occupy scalar registers as lane masks).
     int ofs[PASS], sofs[PASS];
-    T v0[PASS], v1[PASS], v2[PASS];
+    T v0[PASS], v1[PASS], v2[PASS], xv[PASS];
 
     __device__ __forceinline__ void init(const GridDev& g, int j0, int i0) {
 #pragma unroll
@@ -141,21 +141,27 @@ struct SLayer {
     // global loads of layer k.  Branch-free: a layer outside the grid, and the
     // previous direction in the first iteration, get the out-of-range bit ORed
     // into their offsets and load zeros.
+    // x_own (MODE_DIRX): the layer is owned and the previous x update is
+    // pending (uniform): the owned words of x are loaded for the update
+    // applied in store().
     template <int MODE>
-    __device__ __forceinline__ void load(const NodeArgs<T>& A, const T* p_old, int first, int k) {
+    __device__ __forceinline__ void load(const NodeArgs<T>& A, const T* p_old, int first, int k, bool x_own) {
         const GridDev& g = A.g;
         const unsigned vbytes = unsigned(g.ndof) * sizeof(T);
         const bool kin = k >= 0 && k < g.NK;
         const unsigned kbad = kin ? 0u : unsigned(OOB);
         const unsigned kofs = kin ? unsigned(sizeof(T)) * 3u * unsigned(k) * unsigned(g.NI) * unsigned(g.NJ) : 0u;
         const unsigned pbad = first ? unsigned(OOB) : 0u;
+        const unsigned xbad = x_own ? 0u : unsigned(OOB);
 #pragma unroll
         for (int u = 0; u < PASS; ++u) {
             const unsigned off = (unsigned(ofs[u]) + kofs) | kbad;
-            if constexpr (MODE == MODE_DIR) {
+            if constexpr (is_dir(MODE)) {
                 v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), int(off));
                 v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), int(off));
                 v2[u] = Buf<T>::ld(buf_rsrc(p_old, vbytes), int(off | pbad));
+                if constexpr (MODE == MODE_DIRX)
+                    xv[u] = Buf<T>::ld(buf_rsrc(A.x, vbytes), int((unsigned(sofs[u]) + kofs) | kbad | xbad));
             } else {
                 v0[u] = Buf<T>::ld(buf_rsrc(A.u, vbytes), int(off));
             }
@@ -164,17 +170,23 @@ struct SLayer {
     // form the layer's values, write the owned direction entries (k_owned is
     // wave-uniform), store to LDS
     template <int MODE>
-    __device__ __forceinline__ void store(const NodeArgs<T>& A, T beta, T* p_new, int k, bool k_owned, T* slot) {
+    __device__ __forceinline__ void store(const NodeArgs<T>& A, T beta, T* p_new, T alpha_prev, bool x_own, int k,
+                                          bool k_owned, T* slot) {
         const unsigned vbytes = unsigned(A.g.ndof) * sizeof(T);
         const unsigned kbad = k_owned ? 0u : unsigned(OOB);
         const unsigned kofs = k_owned ? unsigned(sizeof(T)) * 3u * unsigned(k) * unsigned(A.g.NI) * unsigned(A.g.NJ) : 0u;
+        const unsigned xbad = x_own ? 0u : unsigned(OOB);
 #pragma unroll
         for (int u = 0; u < PASS; ++u) {
             const int w = threadIdx.x + NT * u;
             T val;
-            if constexpr (MODE == MODE_DIR) {
+            if constexpr (is_dir(MODE)) {
                 val = v1[u] * v0[u] + beta * v2[u];
-                Buf<T>::st(val, buf_rsrc(p_new, vbytes), int((unsigned(sofs[u]) + kofs) | kbad));
+                const unsigned so = (unsigned(sofs[u]) + kofs) | kbad;
+                Buf<T>::st(val, buf_rsrc(p_new, vbytes), int(so));
+                // the previous iteration's x += alpha p_old, for the owned words
+                if constexpr (MODE == MODE_DIRX)
+                    Buf<T>::st(xv[u] + alpha_prev * v2[u], buf_rsrc(A.x, vbytes), int(so | xbad));
             } else {
                 val = v0[u];
             }
@@ -205,13 +217,13 @@ __host__ __device__ inline void sector_tiles(GridDev& g, int EJ, int EI, int kd)
 }
 
 // The march of one block over `tile` (column tile + layer chunk) of grid A.g:
-// stages layers, forms p (MODE_DIR; written for owned nodes), evaluates the
+// stages layers, forms p (direction modes; written for owned nodes), evaluates the
 // element products through the sectors and writes each owned node's outputs
 // (node_output), accumulating the fused partials in `part`.
 template <typename T, int EJ, int EI, int MODE>
 __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, int kd, int tile, T beta, int first,
-                                             T* p_new, const T* p_old, SectorSmem<T, EJ, EI>& sm,
-                                             double (&part)[2]) {
+                                             T* p_new, const T* p_old, T alpha_prev, int pending,
+                                             SectorSmem<T, EJ, EI>& sm, double (&part)[2]) {
     constexpr int TJ = EJ - 1, TI = EI - 1;
     typedef SLayer<T, EJ, EI> L;
     constexpr bool OFFS = sizeof(T) == 4;   // subtract the corner-0 value first (FP32 rounding)
@@ -235,10 +247,10 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, 
     L lay;
     lay.init(g, j0, i0);
     // prologue: layers kb-1 and kb into ring slots 0 and 1
-    lay.template load<MODE>(A, p_old, first, kb - 1);
-    lay.template store<MODE>(A, beta, p_new, kb - 1, false, sRing);
-    lay.template load<MODE>(A, p_old, first, kb);
-    lay.template store<MODE>(A, beta, p_new, kb, kb < ke, sRing + L::LW);
+    lay.template load<MODE>(A, p_old, first, kb - 1, false);
+    lay.template store<MODE>(A, beta, p_new, alpha_prev, false, kb - 1, false, sRing);
+    lay.template load<MODE>(A, p_old, first, kb, pending && kb < ke);
+    lay.template store<MODE>(A, beta, p_new, alpha_prev, pending && kb < ke, kb, kb < ke, sRing + L::LW);
     T se_next = Buf<T>::ld(buf_rsrc(A.s_e, sbytes),
                            (ecol && kb - 1 >= 0) ? int(sizeof(T)) * (((kb - 1) * g.nx + ie) * g.ny + je) : OOB);
     __syncthreads();
@@ -250,7 +262,7 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, 
         const T* up = sRing + ((q + 1) % 3) * L::LW;         // layer s + 1
         // prefetch layer s + 2 and the next slab's element scale
         const bool more = s + 2 <= ke;
-        if (more) lay.template load<MODE>(A, p_old, first, s + 2);
+        if (more) lay.template load<MODE>(A, p_old, first, s + 2, pending && s + 2 < ke);
         const T se = se_next;
         se_next = Buf<T>::ld(buf_rsrc(A.s_e, sbytes),
                              (ecol && s + 1 < g.nz && s + 1 < ke) ? int(sizeof(T)) * (((s + 1) * g.nx + ie) * g.ny + je)
@@ -303,7 +315,9 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, 
             if (nown && s >= kb) node_output<T, MODE>(A, 3 * ((s * g.NI + ni) * g.NJ + nj), qv, ctr, part);
 #pragma unroll
             for (int c = 0; c < 3; ++c) carry[c] = top[c];
-            if (more) lay.template store<MODE>(A, beta, p_new, s + 2, s + 2 < ke, sRing + ((q + 2) % 3) * L::LW);
+            if (more)
+                lay.template store<MODE>(A, beta, p_new, alpha_prev, pending && s + 2 < ke, s + 2, s + 2 < ke,
+                                         sRing + ((q + 2) % 3) * L::LW);
         }
         __syncthreads();
     }
@@ -317,14 +331,14 @@ k_sector(NodeArgs<T> A, const T* kh, int kd) {
     static_assert((EJ * EI) % 64 == 0, "whole waves");
     __shared__ SectorSmem<T, EJ, EI> sm;
     __shared__ double sRed[16 * 4];
-    T beta;
-    int first;
+    T beta, alpha_prev;
+    int first, pending;
     T* p_new;
     const T* p_old;
-    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old)) return;
+    if (!node_setup<T, MODE>(A, beta, first, p_new, p_old, alpha_prev, pending)) return;
     double part[2] = {0.0, 0.0};
-    sector_march<T, EJ, EI, MODE>(A, kh, kd, xcd_local_tile(blockIdx.x, gridDim.x), beta, first, p_new, p_old, sm,
-                                  part);
+    sector_march<T, EJ, EI, MODE>(A, kh, kd, xcd_local_tile(blockIdx.x, gridDim.x), beta, first, p_new, p_old,
+                                  alpha_prev, pending, sm, part);
     node_finish<T, MODE>(A, part, sRed);
 }
 

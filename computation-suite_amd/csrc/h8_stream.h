@@ -36,6 +36,7 @@ k_pcg_init(long long n, const T* __restrict__ b, const T* __restrict__ Ax0, cons
         st->beta = 0.0;
         st->alpha = 0.0;
         st->outer = 0;
+        st->pending = 0;
         if (st->rel < st->tol) {
             st->status = ST_DONE;
             st->iters = 0;
@@ -46,60 +47,72 @@ k_pcg_init(long long n, const T* __restrict__ b, const T* __restrict__ Ax0, cons
     }
 }
 
-// x += alpha p ; r = mask(r - alpha q) ; reduces r.z and r.r with z = dinv r.
+// r = mask(r - alpha q) ; reduces r.z and r.r with z = dinv r.
+// XUPD = false: the iterate update x += alpha p is deferred to the next
+//               direction kernel, which has p in registers while staging it
+//               (PcgState::pending), so this pass streams four vectors (r, q,
+//               dinv in; r out) -- the form for large grids.
+// XUPD = true : x += alpha p here as well (seven vectors) and the pending flag
+//               is cleared -- the form for small grids, where the direction
+//               kernel is latency-bound and the extra staging costs more.
 // REFINED = false: the stopping rule / beta of fe/iterative_solvers.py:455-466.
 // REFINED = true : additionally stops the FP32 loop for an FP64 residual
 //                  replacement when ||r|| < delta * rmax (or the recurrence
-//                  claims convergence, or maxiter is reached).
-// Vectors are read and written in 16-byte groups (2 doubles / 4 floats); the
-// workspace and torch allocations are 256-byte aligned, the tail is scalar.
-template <typename T, bool REFINED>
+//                  claims convergence, the phase is too long, or maxiter is
+//                  reached).
+// Vectors are read and written in 16-byte groups (2 doubles / 4 floats), four
+// groups per thread per pass (two when x is updated too) with all loads issued
+// first; the workspace and torch allocations are 256-byte aligned, the tail is
+// scalar.
+template <typename T, bool REFINED, bool XUPD>
 __global__ void __launch_bounds__(STREAM_THREADS)
 k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p0,
              const T* __restrict__ p1, const T* __restrict__ q, const T* __restrict__ dinv, PcgState* st,
              double* partials) {
     constexpr int V = 16 / sizeof(T);
+    constexpr int G = XUPD ? 2 : 4;
     typedef T vec __attribute__((ext_vector_type(V)));
     __shared__ double sRed[16 * 2];
     if (st->status != ST_RUNNING) return;
-    const T* __restrict__ p = (st->k & 1) ? p1 : p0;
     const T alpha = T(st->alpha);
+    const T* __restrict__ p = (st->k & 1) ? p1 : p0;
     double v[2] = {0.0, 0.0};
     const long long nvec = n / V;
     const long long gstride = (long long)gridDim.x * blockDim.x;
-    // two 16-byte groups per thread per pass, loads of both issued first
-    for (long long iv0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; iv0 < nvec; iv0 += 2 * gstride) {
-        vec dv[2], xv[2], rv[2], pv[2], qv[2];
-        bool ok[2];
+    for (long long iv0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; iv0 < nvec; iv0 += G * gstride) {
+        vec dv[G], rv[G], qv[G], xv[G], pv[G];
+        bool ok[G];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < G; ++u) {
             const long long iv = iv0 + u * gstride;
             ok[u] = iv < nvec;
             const long long j = ok[u] ? iv : iv0;
             dv[u] = reinterpret_cast<const vec*>(dinv)[j];
-            xv[u] = reinterpret_cast<const vec*>(x)[j];
             rv[u] = reinterpret_cast<const vec*>(r)[j];
-            pv[u] = reinterpret_cast<const vec*>(p)[j];
             qv[u] = reinterpret_cast<const vec*>(q)[j];
+            if constexpr (XUPD) {
+                xv[u] = reinterpret_cast<const vec*>(x)[j];
+                pv[u] = reinterpret_cast<const vec*>(p)[j];
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < G; ++u) {
             if (!ok[u]) continue;
 #pragma unroll
             for (int c = 0; c < V; ++c) {
-                xv[u][c] += alpha * pv[u][c];
+                if constexpr (XUPD) xv[u][c] += alpha * pv[u][c];
                 const T ri = (dv[u][c] != T(0)) ? T(rv[u][c] - alpha * qv[u][c]) : T(0);
                 rv[u][c] = ri;
                 v[0] += double(ri) * double(dv[u][c] * ri);
                 v[1] += double(ri) * double(ri);
             }
-            reinterpret_cast<vec*>(x)[iv0 + u * gstride] = xv[u];
             reinterpret_cast<vec*>(r)[iv0 + u * gstride] = rv[u];
+            if constexpr (XUPD) reinterpret_cast<vec*>(x)[iv0 + u * gstride] = xv[u];
         }
     }
     for (long long i = nvec * V + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gstride) {
         const T d = dinv[i];
-        x[i] += alpha * p[i];
+        if constexpr (XUPD) x[i] += alpha * p[i];
         const T ri = (d != T(0)) ? T(r[i] - alpha * q[i]) : T(0);
         r[i] = ri;
         v[0] += double(ri) * double(d * ri);
@@ -107,6 +120,7 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     }
     double tot[2];
     if (reduce_last_block<2>(v, partials, &st->cnt[1], tot, sRed) && threadIdx.x == 0) {
+        if (XUPD) st->pending = 0;   // applied above
         const double rn = sqrt(tot[1]);
         const double rel = rn / st->bnorm;
         st->rel = rel;
@@ -138,14 +152,29 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     }
 }
 
-// u64 += double(x32); x32 = 0 (fold the FP32 iterate into the FP64 solution).
+// u64 += double(x32) (+ alpha p32 when the last direction's update is still
+// pending); x32 = 0: fold the FP32 iterate into the FP64 solution.
 __global__ void __launch_bounds__(STREAM_THREADS)
-k_fold(long long n, double* __restrict__ u, float* __restrict__ x32) {
+k_fold(long long n, double* __restrict__ u, float* __restrict__ x32, const float* __restrict__ p32,
+       const PcgState* st) {
+    const double alpha = st->pending ? st->alpha : 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
-        u[i] += double(x32[i]);
+        u[i] += double(x32[i]) + alpha * double(p32[i]);
         x32[i] = 0.0f;
     }
+}
+
+// x += alpha p when the last direction's update is still pending (after the
+// last iteration of a solve).
+template <typename T>
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_pending_axpy(long long n, T* __restrict__ x, const T* __restrict__ p, const PcgState* st) {
+    if (!st->pending) return;
+    const T alpha = T(st->alpha);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] += alpha * p[i];
 }
 
 __global__ void __launch_bounds__(STREAM_THREADS)
