@@ -3,6 +3,7 @@
 // one host poll per chunk) and the mixed-precision refined solve.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -19,6 +20,7 @@
 #include "h8_mfma.h"
 #include "h8_batched.h"
 #include "h8_sector.h"
+#include "h8_resident.h"
 
 using namespace h8;
 
@@ -114,7 +116,7 @@ static int make_grid(const h8_grid* g, GridDev& d) {
         return fail(H8_ERR_ARG, "grid too large: %lld DOFs (the kernels address vectors with 32-bit byte offsets; "
                     "at most 2^28 - 1 DOFs)", d.ndof);
     d.ntiles = int(nt);
-    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_SECTOR_NARROW)
+    if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_RESIDENT)
         return fail(H8_ERR_ARG, "unknown kernel variant %d", g->variant);
     if (g->layers < 0) return fail(H8_ERR_ARG, "layers must be >= 0 (0 = library choice), got %d", g->layers);
     d.variant = g->variant;
@@ -193,7 +195,9 @@ static int launch_sector(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
 template <typename T, int MODE>
 static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     GridDev d = d0;
-    const int v = d.variant == H8_VARIANT_AUTO ? default_variant<T>(d) : d.variant;
+    // the resident solver's products outside its loop use the streaming default
+    const int v = (d.variant == H8_VARIANT_AUTO || d.variant == H8_VARIANT_RESIDENT) ? default_variant<T>(d)
+                                                                                      : d.variant;
     if constexpr (MODE == MODE_DIRX) {   // sector kernels only (defer_x)
         if (v == H8_VARIANT_SECTOR) return launch_sector<T, MODE, 16, 16>(d0, A, s);
         if (v == H8_VARIANT_SECTOR_NARROW) return launch_sector<T, MODE, 8, 16>(d0, A, s);
@@ -506,11 +510,130 @@ static int chunk_for(long long maxiter) {
     return int(std::max(1LL, std::min<long long>(maxiter, 32)));
 }
 
+// ---------------------------------------------------------------------------
+// Resident solver (one cooperative launch per solve / refined phase)
+// ---------------------------------------------------------------------------
+// run_resident's answer when the device cannot hold all blocks at once and the
+// variant was the library's choice (not an error: the streaming loop runs)
+constexpr int RES_DECLINED = -1;
+
+template <typename T> struct ResThreads;
+template <> struct ResThreads<float> { static constexpr int value = 1024; };
+template <> struct ResThreads<double> { static constexpr int value = 512; };
+
+// Brick split for the resident solver: at most one block per CU, LDS within
+// 159 KiB (the kernel's static LDS stays below 1 KiB of the CU's 160); the
+// fewest element batches (elements / threads, rounded up) per block, then the
+// fewest elements, then the fewest blocks.
+template <typename T>
+static bool res_plan(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
+    constexpr long long LDS_MAX = 159 * 1024;
+    bool found = false;
+    long long bcost = 0;
+    int bnb = 0;
+    for (int PK = 1; PK <= std::min(d.NK, max_blocks); ++PK)
+        for (int PI = 1; PI <= std::min(d.NI, max_blocks / PK); ++PI)
+            for (int PJ = 1; PJ <= std::min(d.NJ, max_blocks / (PK * PI)); ++PJ) {
+                ResGeom c{};
+                c.PI = PI; c.PJ = PJ; c.PK = PK;
+                c.BI = (d.NI + PI - 1) / PI;
+                c.BJ = (d.NJ + PJ - 1) / PJ;
+                c.BK = (d.NK + PK - 1) / PK;
+                c.HI = c.BI + 2;
+                c.HJ = c.BJ + 2;
+                c.nown = c.BI * c.BJ * c.BK;
+                c.nhalo = c.HI * c.HJ * (c.BK + 2);
+                c.nelem = (c.BI + 1) * (c.BJ + 1) * (c.BK + 1);
+                long long off = 0;
+                auto take = [&](long long bytes) {
+                    const long long o = off;
+                    off += (bytes + 15) & ~15LL;
+                    return int(o);
+                };
+                c.o_p = take(3LL * c.nhalo * sizeof(T));
+                c.o_r = take(3LL * c.nown * sizeof(T));
+                c.o_x = take(3LL * c.nown * sizeof(T));
+                c.o_d = take(3LL * c.nown * sizeof(T));
+                c.o_q = take(3LL * c.nown * sizeof(T));
+                c.o_s = take((long long)c.nelem * sizeof(T));
+                if (off > LDS_MAX) continue;
+                c.lds_bytes = int(off);
+                const int nb = PI * PJ * PK;
+                const long long cost = (long long)((c.nelem + nt - 1) / nt) * (1LL << 32) + c.nelem;
+                if (!found || cost < bcost || (cost == bcost && nb < bnb)) {
+                    found = true;
+                    bcost = cost;
+                    bnb = nb;
+                    best = c;
+                }
+            }
+    return found;
+}
+
+// The resident solver runs the iterations when asked for (H8_VARIANT_RESIDENT)
+// or, with H8_VARIANT_AUTO, whenever the grid fits it: it then keeps every
+// vector on chip and replaces two launches per iteration by two grid sums.
+template <typename T>
+static bool use_resident(const GridDev& d) {
+    if (d.variant == H8_VARIANT_RESIDENT) return true;
+    if (d.variant != H8_VARIANT_AUTO) return false;
+    ResGeom G;
+    return res_plan<T>(d, ResThreads<T>::value, device_cus(), G);
+}
+
+// Runs the iterations until the device-side status leaves RUNNING in one
+// cooperative launch of k_resident (q serves as the z exchange vector).
+template <typename T, bool REFINED>
+static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
+                        T* q, PcgState* st, double* partials, hipStream_t s, PcgState& hst, long long& launches) {
+    constexpr int NT = ResThreads<T>::value;
+    const void* fn = (const void*)k_resident<T, NT, REFINED>;
+    ResGeom G;
+    if (!res_plan<T>(d, NT, device_cus(), G))
+        return fail(H8_ERR_ARG, "resident solver: a %dx%dx%d grid does not fit in LDS (one block per CU)", d.nx,
+                    d.ny, d.nz);
+    const int nb = G.PI * G.PJ * G.PK;
+    int coop = 0, dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) return fail(H8_ERR_HIP, "resident solver: the device has no cooperative launch");
+    HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, G.lds_bytes));
+    int per = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, NT, G.lds_bytes));
+    if (per < 1 || nb > per * device_cus())
+        return fail(H8_ERR_HIP, "resident solver: %d blocks of %d threads and %d bytes of LDS cannot all be resident",
+                    nb, NT, G.lds_bytes);
+    ResArgs<T> A{};
+    A.g = d; A.G = G; A.s_e = s_e; A.kh = K0 + 576; A.dinv = dinv; A.x = x; A.r = r; A.p0 = p0; A.p1 = p1;
+    A.zx = q; A.ring = partials; A.err = reinterpret_cast<unsigned*>(partials + 9 * nb); A.st = st;
+    hipLaunchKernelGGL(k_ring_reset, dim3((9 * nb + 255) / 256), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long*>(partials), 9 * nb, A.err);
+    LAUNCHCHK();
+    void* args[] = {&A};
+    const hipError_t le = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(NT), args, unsigned(G.lds_bytes), s);
+    if (le == hipErrorCooperativeLaunchTooLarge && d.variant == H8_VARIANT_AUTO) {
+        (void)hipGetLastError();
+        return RES_DECLINED;      // the caller runs the streaming kernels instead
+    }
+    HIPCHK(le);
+    ++launches;
+    unsigned err = 0;
+    HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&err, A.err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (err) return fail(H8_ERR_HIP, "resident solver: a grid-wide sum timed out");
+    return H8_OK;
+}
+
 // Runs captured chunks until the device-side status leaves RUNNING.
 template <typename T, bool REFINED>
 static int run_chunks(const GraphKey& key, const GridDev& d, const T* s_e, const T* K0, const T* dinv,
                       T* x, T* r, T* p0, T* p1, T* q, PcgState* st, double* partials, hipStream_t s,
                       PcgState& hst, long long& launches) {
+    if (use_resident<T>(d)) {
+        const int rr = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, q, st, partials, s, hst, launches);
+        if (rr != RES_DECLINED) return rr;
+    }
     const int chunk = key.chunk;
     hipGraphExec_t exec;
     int rc = get_graph(key, [&](hipStream_t cs) -> int {
@@ -792,6 +915,32 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
     h.rmax = INFINITY;            // no replacement trigger fires while timing
     h.k_rep = (1LL << 62);
     h.pending = defer_x<T>(d) ? 1 : 0;   // as inside a solve of this size and kernel
+    if (d.variant == H8_VARIANT_RESIDENT) {
+        // the resident solver: `reps` iterations in one launch (stopped by
+        // maxiter); avg_ms[0] = time per iteration, avg_ms[1] = 0
+        h.maxiter = h.k + reps - 1;
+        hipEvent_t ev[2];
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        PcgState hh{};
+        long long launches = 0;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {      // warm-up, then timed
+            HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
+            HIPCHK(hipEventRecord(ev[0], s));
+            int rc = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, q, st, partials, s, hh, launches);
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(ev[1], s));
+            HIPCHK(hipEventSynchronize(ev[1]));
+            HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            if (hh.status == ST_NOT_SPD || hh.k - 2 + (hh.status == ST_REPLACE ? 0 : 1) < reps)
+                return fail(H8_ERR_ARG, "resident timing run stopped early (status %d at iteration %lld)", hh.status,
+                            hh.k);
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        avg_ms[0] = ms / reps;
+        avg_ms[1] = 0.0;
+        return H8_OK;
+    }
     NodeArgs<T> A{};
     A.g = d; A.s_e = s_e; A.K0 = K0; A.r = r; A.dinv = dinv; A.p0 = p0; A.p1 = p1; A.q = q; A.x = x;
     A.st = st; A.partials = partials;

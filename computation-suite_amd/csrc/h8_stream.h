@@ -47,6 +47,44 @@ k_pcg_init(long long n, const T* __restrict__ b, const T* __restrict__ Ax0, cons
     }
 }
 
+// End of an iteration, given rz = r.z and rr = r.r of the updated residual.
+// REFINED = false: the stopping rule / beta of fe/iterative_solvers.py:455-466.
+// REFINED = true : additionally stops the FP32 loop for an FP64 residual
+//                  replacement when ||r|| < delta * rmax (or the recurrence
+//                  claims convergence, the phase is too long, or maxiter is
+//                  reached).
+template <bool REFINED>
+__device__ inline void update_finish(PcgState* st, double rz, double rr) {
+    const double rn = sqrt(rr);
+    const double rel = rn / st->bnorm;
+    st->rel = rel;
+    const long long k = st->k;
+    if (!REFINED) {
+        if (rel < st->tol) {
+            st->status = ST_DONE;
+            st->iters = k;
+        } else if (k >= st->maxiter) {
+            st->status = ST_MAXITER;
+            st->iters = k;
+        } else {
+            st->beta = rz / st->rz_old;
+            st->rz_old = rz;
+            st->first = 0;
+            st->k = k + 1;
+        }
+    } else {
+        st->first = 0;
+        st->k = k + 1;
+        const long long phase_cap = REFINE_PHASE > 4 * st->max_phase ? REFINE_PHASE : 4 * st->max_phase;
+        if (rel < st->tol || rn < st->delta * st->rmax || k - st->k_rep > phase_cap || k >= st->maxiter) {
+            st->status = ST_REPLACE;
+        } else {
+            st->beta = rz / st->rz_old;
+            st->rz_old = rz;
+        }
+    }
+}
+
 // r = mask(r - alpha q) ; reduces r.z and r.r with z = dinv r.
 // XUPD = false: the iterate update x += alpha p is deferred to the next
 //               direction kernel, which has p in registers while staging it
@@ -55,11 +93,7 @@ k_pcg_init(long long n, const T* __restrict__ b, const T* __restrict__ Ax0, cons
 // XUPD = true : x += alpha p here as well (seven vectors) and the pending flag
 //               is cleared -- the form for small grids, where the direction
 //               kernel is latency-bound and the extra staging costs more.
-// REFINED = false: the stopping rule / beta of fe/iterative_solvers.py:455-466.
-// REFINED = true : additionally stops the FP32 loop for an FP64 residual
-//                  replacement when ||r|| < delta * rmax (or the recurrence
-//                  claims convergence, the phase is too long, or maxiter is
-//                  reached).
+// REFINED: the stopping rule of update_finish.
 // Vectors are read and written in 16-byte groups (2 doubles / 4 floats), four
 // groups per thread per pass (two when x is updated too) with all loads issued
 // first; the workspace and torch allocations are 256-byte aligned, the tail is
@@ -121,34 +155,7 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     double tot[2];
     if (reduce_last_block<2>(v, partials, &st->cnt[1], tot, sRed) && threadIdx.x == 0) {
         if (XUPD) st->pending = 0;   // applied above
-        const double rn = sqrt(tot[1]);
-        const double rel = rn / st->bnorm;
-        st->rel = rel;
-        const long long k = st->k;
-        if (!REFINED) {
-            if (rel < st->tol) {
-                st->status = ST_DONE;
-                st->iters = k;
-            } else if (k >= st->maxiter) {
-                st->status = ST_MAXITER;
-                st->iters = k;
-            } else {
-                st->beta = tot[0] / st->rz_old;
-                st->rz_old = tot[0];
-                st->first = 0;
-                st->k = k + 1;
-            }
-        } else {
-            st->first = 0;
-            st->k = k + 1;
-            const long long phase_cap = REFINE_PHASE > 4 * st->max_phase ? REFINE_PHASE : 4 * st->max_phase;
-            if (rel < st->tol || rn < st->delta * st->rmax || k - st->k_rep > phase_cap || k >= st->maxiter) {
-                st->status = ST_REPLACE;
-            } else {
-                st->beta = tot[0] / st->rz_old;
-                st->rz_old = tot[0];
-            }
-        }
+        update_finish<REFINED>(st, tot[0], tot[1]);
     }
 }
 

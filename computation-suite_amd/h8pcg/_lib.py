@@ -23,7 +23,7 @@ H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
 
 (VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR,
- VARIANT_SECTOR_NARROW) = 0, 1, 2, 3, 4, 5, 6
+ VARIANT_SECTOR_NARROW, VARIANT_RESIDENT) = 0, 1, 2, 3, 4, 5, 6, 7
 K0_WORDS = 648
 
 

@@ -3,10 +3,14 @@ by timing on the device.
 
     tune(op, precision)  ->  (variant, layers, microseconds)
 
-Each candidate is timed with ``h8_time_kernels``: the fused direction + K*p
-kernel of one PCG iteration, launched `reps` times back to back in a HIP graph
-and timed with HIP events on the launching stream (the same kernel and launch
-pattern as inside a solve).  The fastest candidate wins.  Results are cached in
+Each streaming candidate is timed with ``h8_time_kernels``: the fused direction
++ K*p kernel of one PCG iteration, launched `reps` times back to back in a HIP
+graph and timed with HIP events on the launching stream (the same kernel and
+launch pattern as inside a solve).  The fastest candidate wins among them.
+When the grid fits the resident solver (the whole loop in one cooperative
+launch), its time per iteration is measured the same way and compared with the
+winner's whole iteration (direction + update kernel); the faster of the two is
+the choice.  Results are cached in
 a JSON file keyed by device, grid dims and precision, so a shape is timed once
 per machine: $H8PCG_TUNE_CACHE, else ~/.cache/h8pcg/tune.json (no cache when
 the file cannot be written).
@@ -70,6 +74,13 @@ def _store(path, data):
 
 def time_direction_kernel(op, precision, variant, layers, reps=30):
     """Average microseconds of the fused direction kernel with this choice."""
+    return time_iteration(op, precision, variant, layers, reps)[0]
+
+
+def time_iteration(op, precision, variant, layers, reps=30):
+    """(direction kernel, update kernel) average microseconds with this choice;
+    for the resident solver (time per iteration, 0).  ValueError if the grid
+    does not fit the resident solver."""
     import torch
     g = _lib.H8Grid(op.nx, op.ny, op.nz, int(variant), int(layers))
     ws = op.workspace(precision)
@@ -80,10 +91,10 @@ def time_direction_kernel(op, precision, variant, layers, reps=30):
         _lib.check(_lib.lib().h8_time_kernels(C.byref(g), int(precision), _lib.ptr(s_e), _lib.ptr(K0),
                                               _lib.ptr(dinv), _lib.ptr(ws), ws.numel(), int(reps), avg,
                                               _lib.stream_handle(op.device)), "h8_time_kernels")
-    return avg[0] * 1e3
+    return avg[0] * 1e3, avg[1] * 1e3
 
 
-def tune(op, precision, cache=True, candidates=None, reps=30):
+def tune(op, precision, cache=True, candidates=None, reps=30, resident=True):
     key = "%s|%dx%dx%d|fp%d" % (_device_key(), op.nx, op.ny, op.nz, int(precision))
     path = cache_path()
     data = _load(path) if cache else {}
@@ -92,12 +103,22 @@ def tune(op, precision, cache=True, candidates=None, reps=30):
         return int(v), int(l), float(us)
     timed = []
     for v, l in (candidates or CANDIDATES):
-        time_direction_kernel(op, precision, v, l, reps=3)        # warm-up (graph capture)
-        timed.append((v, l, time_direction_kernel(op, precision, v, l, reps=reps)))
+        time_iteration(op, precision, v, l, reps=3)        # warm-up (graph capture)
+        d_us, u_us = time_iteration(op, precision, v, l, reps=reps)
+        timed.append((v, l, d_us, u_us))
     # the first candidate within TIE of the fastest: timings closer than that are
     # noise, and a stable choice keeps results reproducible between runs
     fastest = min(t[2] for t in timed)
-    best = next(t for t in timed if t[2] <= fastest * (1.0 + TIE))
+    v, l, d_us, u_us = next(t for t in timed if t[2] <= fastest * (1.0 + TIE))
+    best = (v, l, d_us)
+    if resident:
+        try:
+            time_iteration(op, precision, _lib.VARIANT_RESIDENT, 0, reps=5)
+            res_us = time_iteration(op, precision, _lib.VARIANT_RESIDENT, 0, reps=max(reps, 50))[0]
+        except ValueError:           # the grid does not fit in LDS
+            res_us = None
+        if res_us is not None and res_us < (d_us + u_us) * (1.0 - TIE):
+            best = (_lib.VARIANT_RESIDENT, 0, res_us)
     if cache:
         data = _load(path)
         data[key] = list(best)

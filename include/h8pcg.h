@@ -50,6 +50,8 @@ extern "C" {
 #define H8_VARIANT_NODE_SPLIT 4   /* node-centric, two threads per node (elements above / below) */
 #define H8_VARIANT_SECTOR 5       /* element products through the reflection sectors, marching in z */
 #define H8_VARIANT_SECTOR_NARROW 6 /* the sector kernel with 8 x 16 element columns (half the LDS) */
+#define H8_VARIANT_RESIDENT 7     /* PCG loop in one cooperative launch, vectors resident in LDS
+                                     (grids that fit one block per CU; other products: the default) */
 
 #define H8_K0_WORDS 648           /* 576 matrix entries + 72 sector-table entries */
 

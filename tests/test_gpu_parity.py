@@ -58,7 +58,7 @@ def test_spmv_f64_f32_match_oracle_csr(h8, gpu, dims):
     u = rng.normal(size=K.shape[0])
     ref = K @ u
     for variant in (1, 2, 3, 4, 5, 6):  # node, element MFMA, persistent, two per node, sectors, narrow sectors
-        op.grid.variant = variant
+        op.set_variant(variant)
         y = op.matvec(torch.as_tensor(u, device=gpu)).cpu().numpy()
         e64 = np.abs(y - ref).max() / np.abs(ref).max()
         y32 = op.matvec32(torch.as_tensor(u, device=gpu)).cpu().numpy()
@@ -102,14 +102,14 @@ def test_config1_pcg_matches_reference_and_spsolve(h8, gpu, golden):
     assert np.all(np.asarray(U)[fs.fixed_dofs_array] == 0.0)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
 def test_pcg_variants_agree(h8, gpu, variant):
     say("pcg variant", variant)
     rng = np.random.default_rng(4)
     nx, ny, nz = 16, 5, 4
     E = 10 ** rng.uniform(9, 11, nx * ny * nz)
     fs = _fesys(h8, nx, ny, nz, E_elem=E)
-    fs.operator.grid.variant = variant
+    fs.operator.set_variant(variant)
     c = o.cantilever(nx, ny, nz, E_elem=E)
     Ud = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
     for prec in ("fp64", "mixed"):
@@ -117,6 +117,47 @@ def test_pcg_variants_agree(h8, gpu, variant):
         err = np.linalg.norm(U - Ud) / np.linalg.norm(Ud)
         say(prec, fs.last_solve_stats["iterations"], err)
         assert err < 1e-6
+
+
+def test_resident_solver_golden_and_streaming_agreement(h8, gpu, golden):
+    """The one-launch resident solver (variant 7): the reference's iteration
+    count on the 20x4x4 cantilever, the spsolve solution, and agreement with the
+    streaming kernels on a grid split over many blocks."""
+    from h8pcg import _lib
+    say("resident config1")
+    fs = _fesys(h8, 20, 4, 4)
+    fs.operator.set_variant(_lib.VARIANT_RESIDENT)
+    U = np.asarray(fs.solve_static(method="pcg", tol=1e-10))
+    st = fs.last_solve_stats
+    c = o.cantilever(20, 4, 4)
+    U_direct = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
+    err = np.linalg.norm(U - U_direct) / np.linalg.norm(U_direct)
+    say("resident iters", st["iterations"], "launches", st["launches"], "err", err)
+    assert err < 1e-8
+    assert abs(st["iterations"] - int(golden["c1_facet_1e-10/iters"])) <= 2
+    assert st["launches"] == 1
+    assert np.all(U[fs.fixed_dofs_array] == 0.0)
+    rng = np.random.default_rng(7)
+    nx, ny, nz = 40, 10, 9
+    E = 10 ** rng.uniform(9, 11, nx * ny * nz)
+    fs = _fesys(h8, nx, ny, nz, E_elem=E)
+    res = {}
+    for variant in (_lib.VARIANT_SECTOR, _lib.VARIANT_RESIDENT):
+        fs.operator.set_variant(variant)
+        for prec in ("fp64", "mixed"):
+            say("resident agreement", variant, prec)
+            res[variant, prec] = (np.asarray(fs.solve_static(tol=1e-9, precision=prec)),
+                                  dict(fs.last_solve_stats))
+    for prec in ("fp64", "mixed"):
+        (Us, ss), (Ur, sr) = res[_lib.VARIANT_SECTOR, prec], res[_lib.VARIANT_RESIDENT, prec]
+        d = np.linalg.norm(Ur - Us) / np.linalg.norm(Us)
+        say(prec, "iters", ss["iterations"], sr["iterations"], "launches", ss["launches"], sr["launches"], "diff", d)
+        assert d < 1e-7
+        if prec == "fp64":
+            assert abs(sr["iterations"] - ss["iterations"]) <= 2
+            assert sr["launches"] == 1
+        else:
+            assert sr["true_rel_residual"] < 1e-9
 
 
 def test_nodal_and_heterogeneous_golden(h8, gpu, golden):
@@ -229,7 +270,7 @@ def test_autotune_choice_cache_and_solution(h8, gpu, tmp_path, monkeypatch):
     u_default = fs.solve_static_device(tol=1e-11).cpu().numpy()
     it_default = fs.last_solve_stats["iterations"]
     res = fs.operator.autotune()
-    cands = set(autotune.CANDIDATES)
+    cands = set(autotune.CANDIDATES) | {(h8._lib.VARIANT_RESIDENT, 0)}
     for p in (32, 64):
         v, l, us = res[p]
         assert (v, l) in cands and us > 0
