@@ -365,13 +365,15 @@ static size_t partial_slots(const GridDev& d) {
 }
 
 static size_t ws_bytes_64(const GridDev& d) {
-    // r, p0, p1, q, partials, state, and one spare vector (the timing helper's x)
+    // r, p0, p1, q, partials, state, one spare vector (the timing helper's x),
+    // and the resident solver's two further z-exchange vectors (q is the third)
     const size_t nv = align_up(sizeof(double) * d.ndof);
-    return 5 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
+    return 7 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
 }
 static size_t ws_bytes_ref(const GridDev& d) {
+    // r, x, p0, p1, q, dinv32, partials, state, and the two z-exchange vectors
     const size_t nv = align_up(sizeof(float) * d.ndof);
-    return 6 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
+    return 8 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
 }
 
 extern "C" size_t h8_pcg_workspace_bytes(const h8_grid* g, int precision) {
@@ -384,6 +386,20 @@ static char* carve(char*& p, size_t bytes) {
     char* q = p;
     p += align_up(bytes);
     return q;
+}
+
+// The resident solver's z-exchange ring: q and two more vectors of the workspace.
+template <typename T>
+struct ZRing {
+    T* z[3];
+};
+template <typename T>
+static ZRing<T> carve_ring(char*& p, const GridDev& d, T* q) {
+    ZRing<T> zr;
+    zr.z[0] = q;
+    zr.z[1] = (T*)carve(p, sizeof(T) * d.ndof);
+    zr.z[2] = (T*)carve(p, sizeof(T) * d.ndof);
+    return zr;
 }
 
 // ---------------------------------------------------------------------------
@@ -551,12 +567,13 @@ static bool res_plan(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
                     return int(o);
                 };
                 c.o_p = take(3LL * c.nhalo * sizeof(T));
+                c.o_z = take(3LL * c.nhalo * sizeof(T));
                 c.o_r = take(3LL * c.nown * sizeof(T));
                 c.o_x = take(3LL * c.nown * sizeof(T));
                 c.o_d = take(3LL * c.nown * sizeof(T));
                 c.o_q = take(3LL * c.nown * sizeof(T));
                 c.o_s = take((long long)c.nelem * sizeof(T));
-                if (off > LDS_MAX) continue;
+                if (off > LDS_MAX || c.nhalo >= (1 << 15)) continue;   // (index decoding needs < 2^16)
                 c.lds_bytes = int(off);
                 const int nb = PI * PJ * PK;
                 const long long cost = (long long)((c.nelem + nt - 1) / nt) * (1LL << 32) + c.nelem;
@@ -578,18 +595,19 @@ static bool use_resident(const GridDev& d) {
     if (d.variant == H8_VARIANT_RESIDENT) return true;
     if (d.variant != H8_VARIANT_AUTO) return false;
     ResGeom G;
-    return res_plan<T>(d, ResThreads<T>::value, device_cus(), G);
+    return res_plan<T>(d, ResThreads<T>::value, std::min(device_cus(), RES_MAX_BLOCKS), G);
 }
 
 // Runs the iterations until the device-side status leaves RUNNING in one
 // cooperative launch of k_resident (q serves as the z exchange vector).
 template <typename T, bool REFINED>
 static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
-                        T* q, PcgState* st, double* partials, hipStream_t s, PcgState& hst, long long& launches) {
+                        const ZRing<T>& zr, PcgState* st, double* partials, hipStream_t s, PcgState& hst,
+                        long long& launches) {
     constexpr int NT = ResThreads<T>::value;
     const void* fn = (const void*)k_resident<T, NT, REFINED>;
     ResGeom G;
-    if (!res_plan<T>(d, NT, device_cus(), G))
+    if (!res_plan<T>(d, NT, std::min(device_cus(), RES_MAX_BLOCKS), G))
         return fail(H8_ERR_ARG, "resident solver: a %dx%dx%d grid does not fit in LDS (one block per CU)", d.nx,
                     d.ny, d.nz);
     const int nb = G.PI * G.PJ * G.PK;
@@ -605,9 +623,10 @@ static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* di
                     nb, NT, G.lds_bytes);
     ResArgs<T> A{};
     A.g = d; A.G = G; A.s_e = s_e; A.kh = K0 + 576; A.dinv = dinv; A.x = x; A.r = r; A.p0 = p0; A.p1 = p1;
-    A.zx = q; A.ring = partials; A.err = reinterpret_cast<unsigned*>(partials + 9 * nb); A.st = st;
-    hipLaunchKernelGGL(k_ring_reset, dim3((9 * nb + 255) / 256), dim3(256), 0, s,
-                       reinterpret_cast<unsigned long long*>(partials), 9 * nb, A.err);
+    for (int e = 0; e < 3; ++e) A.zr[e] = zr.z[e];
+    A.ring = partials; A.err = reinterpret_cast<unsigned*>(partials + 9 * nb); A.st = st;
+    hipLaunchKernelGGL(k_ring_reset<T>, dim3(stream_blocks(3 * d.ndof)), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long*>(partials), 9 * nb, A.err, A);
     LAUNCHCHK();
     void* args[] = {&A};
     const hipError_t le = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(NT), args, unsigned(G.lds_bytes), s);
@@ -628,10 +647,10 @@ static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* di
 // Runs captured chunks until the device-side status leaves RUNNING.
 template <typename T, bool REFINED>
 static int run_chunks(const GraphKey& key, const GridDev& d, const T* s_e, const T* K0, const T* dinv,
-                      T* x, T* r, T* p0, T* p1, T* q, PcgState* st, double* partials, hipStream_t s,
-                      PcgState& hst, long long& launches) {
+                      T* x, T* r, T* p0, T* p1, T* q, const ZRing<T>& zr, PcgState* st, double* partials,
+                      hipStream_t s, PcgState& hst, long long& launches) {
     if (use_resident<T>(d)) {
-        const int rr = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, q, st, partials, s, hst, launches);
+        const int rr = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hst, launches);
         if (rr != RES_DECLINED) return rr;
     }
     const int chunk = key.chunk;
@@ -691,6 +710,8 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
     w.q = (double*)carve(p, sizeof(double) * d.ndof);
     w.partials = (double*)carve(p, sizeof(double) * partial_slots(d));
     w.st = (PcgState*)carve(p, sizeof(PcgState));
+    (void)carve(p, sizeof(double) * d.ndof);                 // the timing helper's x
+    const ZRing<double> zr = carve_ring<double>(p, d, w.q);
 
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -728,7 +749,7 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
         const void* ptrs[] = {s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
         std::memcpy(key.ptr, ptrs, sizeof ptrs);
         key.chunk = chunk_for(maxiter);
-        rc = run_chunks<double, false>(key, d, s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials, s,
+        rc = run_chunks<double, false>(key, d, s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, zr, w.st, w.partials, s,
                                        hst, launches);
         if (rc) return rc;
         // the last iteration's x += alpha p (the update kernel defers it to the next direction kernel)
@@ -787,6 +808,7 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     w.dinv32 = (float*)carve(p, sizeof(float) * d.ndof);
     w.partials = (double*)carve(p, sizeof(double) * partial_slots(d));
     w.st = (PcgState*)carve(p, sizeof(PcgState));
+    const ZRing<float> zr = carve_ring<float>(p, d, w.q);
 
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -842,7 +864,7 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     std::memcpy(key.ptr, ptrs, sizeof ptrs);
     key.chunk = chunk_for(maxiter);
     while (hst.status == ST_RUNNING) {
-        rc = run_chunks<float, true>(key, d, s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, w.st,
+        rc = run_chunks<float, true>(key, d, s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, zr, w.st,
                                      w.partials, s, hst, launches);
         if (rc) return rc;
         if (hst.status != ST_REPLACE) break;
@@ -893,7 +915,8 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
 // ---------------------------------------------------------------------------
 template <typename T, bool REFINED>
 static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
-                        T* q, PcgState* st, double* partials, int reps, double* avg_ms, hipStream_t s) {
+                        T* q, const ZRing<T>& zr, PcgState* st, double* partials, int reps, double* avg_ms,
+                        hipStream_t s) {
     // r = 1 and a zero previous direction: every launch forms p = dinv*r (zero at
     // fixed DOFs), so p^T K p > 0 and no launch takes the early exit.
     const int sb = stream_blocks(d.ndof);
@@ -927,7 +950,7 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
         for (int rep = 0; rep < 2; ++rep) {      // warm-up, then timed
             HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
             HIPCHK(hipEventRecord(ev[0], s));
-            int rc = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, q, st, partials, s, hh, launches);
+            int rc = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hh, launches);
             if (rc) return rc;
             HIPCHK(hipEventRecord(ev[1], s));
             HIPCHK(hipEventSynchronize(ev[1]));
@@ -1009,8 +1032,9 @@ extern "C" int h8_time_kernels(const h8_grid* g, int precision, const void* s_e,
         double* partials = (double*)carve(p, sizeof(double) * partial_slots(d));
         PcgState* st = (PcgState*)carve(p, sizeof(PcgState));
         double* x = (double*)carve(p, sizeof(double) * d.ndof);
+        const ZRing<double> zr = carve_ring<double>(p, d, q);
         return time_kernels<double, false>(d, (const double*)s_e, (const double*)K0, dinv64, x, r, p0, p1, q,
-                                           st, partials, reps, avg_ms, s);
+                                           zr, st, partials, reps, avg_ms, s);
     }
     if (precision == 32) {
         if (ws_bytes < ws_bytes_ref(d)) return fail(H8_ERR_WORKSPACE, "workspace too small");
@@ -1022,10 +1046,11 @@ extern "C" int h8_time_kernels(const h8_grid* g, int precision, const void* s_e,
         float* dinv32 = (float*)carve(p, sizeof(float) * d.ndof);
         double* partials = (double*)carve(p, sizeof(double) * partial_slots(d));
         PcgState* st = (PcgState*)carve(p, sizeof(PcgState));
+        const ZRing<float> zr = carve_ring<float>(p, d, q);
         hipLaunchKernelGGL(k_cast_f64_f32, dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s, d.ndof,
                            dinv64, dinv32);
         LAUNCHCHK();
-        return time_kernels<float, true>(d, (const float*)s_e, (const float*)K0, dinv32, x, r, p0, p1, q, st,
+        return time_kernels<float, true>(d, (const float*)s_e, (const float*)K0, dinv32, x, r, p0, p1, q, zr, st,
                                          partials, reps, avg_ms, s);
     }
     return fail(H8_ERR_ARG, "precision must be 64 or 32");

@@ -7,9 +7,9 @@
 // owns a brick of nodes and keeps, for the whole solve, r, x, dinv and q of
 // its nodes, p of its nodes plus a one-node halo shell, and the scales of the
 // elements touching its nodes in LDS.  An iteration exchanges only the
-// preconditioned residual z = dinv r of brick-surface nodes (device-scope
-// stores and loads, coherent across the XCDs' L2 caches) and takes two
-// grid-wide sums.
+// preconditioned residual z = dinv r of brick-surface nodes and takes two
+// grid-wide sums, all through device-scope stores and loads (coherent across
+// the XCDs' L2 caches).
 //
 // Iteration (the Hestenes-Stiefel recurrence of the streaming kernels,
 // fe/iterative_solvers.py:438-469):
@@ -22,21 +22,22 @@
 //       by corner (8 rounds in a fixed order: deterministic)
 //   p.q -> grid sum -> alpha, or stop (not SPD)
 //   x += alpha p, r = mask(r - alpha q), z = dinv r, publish surface z;
-//   r.z, r.r -> grid sum -> beta, or stop (update_finish)
+//   r.z, r.r -> grid sum -> beta, or stop (update_finish); meanwhile every
+//       block collects the z of its shell nodes
 //
-// Grid sums: every block publishes its partial in its slot of a ring entry and
-// polls all blocks' slots of that entry, then sums them in block order, so
-// every block holds the same total.  A slot holds RING_EMPTY (a signalling
-// NaN, which arithmetic never produces) until written.  Entries rotate over 3
-// iterations; after the sum of iteration j a block empties its own slot of the
-// entry used in iteration j - 1, which every block has read by then, and that
-// store completes before the block's next publication, so no block sees a
-// stale value when the entry comes round again.  The ring is filled with
-// RING_EMPTY before each launch.  Polls are bounded: a block that waits more
-// than SPIN_LIMIT polls raises the error word and leaves; the other blocks
-// then leave within the same bound, so a launch always ends.  The launch is
-// cooperative (all blocks resident at once), so in a correct run no poll
-// waits for a block that has not started.
+// Slots and sentinels.  Every exchanged word -- a block's partial sum, a
+// surface node's z -- lives in a ring of 3 entries (iteration j uses entry
+// j % 3) and holds EMPTY (a signalling-NaN pattern, which arithmetic never
+// produces) until its owner writes it, so a reader polls the word itself: a
+// value that is not EMPTY is the current one.  An owner empties its words of
+// the entry used one iteration earlier once every block is known to have read
+// them (after the next grid sum), and those stores complete before the
+// owner's next p.q publication, which every reader observes before it polls
+// that entry again.  The rings are filled with EMPTY before each launch.
+// Polls are bounded: a block that waits more than SPIN_LIMIT polls raises the
+// error word and leaves; the other blocks then leave within the same bound, so
+// a launch always ends.  The launch is cooperative (all blocks resident at
+// once), so in a correct run no poll waits for a block that has not started.
 #pragma once
 #include "h8_device.h"
 #include "h8_node.h"
@@ -47,10 +48,22 @@ namespace h8 {
 
 constexpr unsigned long long RING_EMPTY = 0x7FF4DEADBEEF0001ull;
 constexpr long long SPIN_LIMIT = 1LL << 22;
+constexpr int RES_MAX_BLOCKS = 256;   // grid sums poll up to 4 slots per lane of one wave
+
+// EMPTY pattern of an exchanged vector word (a signalling NaN of T).
+template <typename T> struct Empty;
+template <> struct Empty<float> {
+    typedef unsigned U;
+    static constexpr U bits = 0x7FA00001u;
+};
+template <> struct Empty<double> {
+    typedef unsigned long long U;
+    static constexpr U bits = RING_EMPTY;
+};
 
 // Brick split of the node grid: PJ x PI x PK parts, part a of P over N nodes
 // spans [a N / P, (a + 1) N / P).  LDS layouts use the largest extents:
-//   p     [BK + 2][HI][HJ][3]   halo brick (HI = BI + 2, HJ = BJ + 2)
+//   p, z  [BK + 2][HI][HJ][3]   halo brick (HI = BI + 2, HJ = BJ + 2); z: shell nodes
 //   r, x, dinv, q [BK][BI][BJ][3]  owned nodes
 //   s     element scales of the block's (clipped) element range, j fastest
 struct ResGeom {
@@ -58,7 +71,7 @@ struct ResGeom {
     int BI, BJ, BK;
     int HI, HJ;
     int nown, nhalo, nelem;
-    int o_p, o_r, o_x, o_d, o_q, o_s;
+    int o_p, o_z, o_r, o_x, o_d, o_q, o_s;
     int lds_bytes;
 };
 
@@ -73,7 +86,7 @@ struct ResArgs {
     T* r;
     T* p0;             // direction buffers: iteration k's p is written to p[k & 1]
     T* p1;
-    T* zx;             // z exchange (one vector)
+    T* zr[3];          // z exchange ring (three vectors)
     double* ring;      // [3][nb] (p.q), then [3][nb][2] (r.z, r.r)
     unsigned* err;     // set to 1 when a poll exceeded SPIN_LIMIT
     PcgState* st;
@@ -89,6 +102,31 @@ __device__ __forceinline__ void st_dev(T* p, T v) {
 }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Exact t / d for t, d < 2^16 through a 64-bit multiply by m = ceil(2^32 / d):
+// t m / 2^32 = t / d + t e / (d 2^32) with 0 <= e < d, and t e / 2^32 < 1 / d.
+struct FastDiv {
+    unsigned long long m;
+    __device__ void init(unsigned d) { m = ((1ull << 32) + d - 1) / d; }
+    __device__ unsigned operator()(unsigned t) const { return unsigned((t * m) >> 32); }
+};
+// (a, b, c) with t = a + n1 (b + n2 c), 0 <= a < n1, 0 <= b < n2.
+struct Decode3 {
+    int n1, n2;
+    FastDiv f1, f12;
+    __device__ void init(int a, int b) {
+        n1 = a;
+        n2 = b;
+        f1.init(unsigned(a));
+        f12.init(unsigned(a * b));
+    }
+    __device__ void operator()(int t, int& a, int& b, int& c) const {
+        const int q = int(f1(unsigned(t)));
+        c = int(f12(unsigned(t)));
+        a = t - q * n1;
+        b = q - c * n2;
+    }
+};
 
 // DPP lane move of a double (two 32-bit halves); lanes of rows outside
 // ROW_MASK receive 0.
@@ -120,17 +158,19 @@ __device__ __forceinline__ double wave_total(double v) {
 // Grid-wide sum of NV doubles through one ring entry ([nb][NV]); `done`: the
 // entry used one iteration earlier, whose slot of this block is emptied.
 // Every wave reduces its lanes, wave 0 adds the waves, publishes the block's
-// partial, polls the nb slots (lane l: slots l, l + 64, ... in order) and
-// reduces them, so `tot` holds the same total in wave 0 of every block.
+// partial, polls the nb <= 256 slots (lane l: slots l + 64 u, u < 4, all
+// loads of a round in flight together) and reduces them in that fixed order,
+// so `tot` holds the same total in wave 0 of every block.  RELEASE: the
+// block's earlier device-scope stores complete before the publication.
 // Returns false in wave 0 if a poll timed out (true in the other waves, which
 // return after the first barrier).  The caller reads `tot` in thread 0 and
 // then synchronises the block.
-template <int NV>
+template <int NV, bool RELEASE>
 __device__ bool grid_sum(const double (&v)[NV], double* entry, double* done, double* sh, double (&tot)[NV]) {
-    // the block's earlier device-scope stores (z exchange, an emptied slot)
-    // complete before its partial is published
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);
+    if constexpr (RELEASE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -145,31 +185,47 @@ __device__ bool grid_sum(const double (&v)[NV], double* entry, double* done, dou
         const double bsum = wave_total(lane < nw ? sh[lane * NV + i] : 0.0);
         if (lane == 0) st_dev(&entry[blockIdx.x * NV + i], bsum);
     }
-    double acc[NV];
+    constexpr int U = RES_MAX_BLOCKS / 64;
+    double val[U][NV];
+    bool pend[U];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int u = 0; u < U; ++u) {
+        pend[u] = lane + 64 * u < nb;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) val[u][i] = 0.0;
+    }
     int good = 1;
-    for (int b = lane; b < nb; b += 64) {
-        double val[NV];
-        for (long long spin = 0;; ++spin) {
-            bool all = true;
+    for (long long spin = 0;; ++spin) {
+        bool any = false;
 #pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                val[i] = ld_dev(&entry[b * NV + i]);
-                all = all && __builtin_bit_cast(unsigned long long, val[i]) != RING_EMPTY;
+        for (int u = 0; u < U; ++u)
+            if (pend[u]) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) val[u][i] = ld_dev(&entry[(lane + 64 * u) * NV + i]);
             }
-            if (all) break;
-            if (spin >= SPIN_LIMIT) {
-                good = 0;
-                break;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (pend[u]) {
+                bool all = true;
+#pragma unroll
+                for (int i = 0; i < NV; ++i) all = all && __builtin_bit_cast(unsigned long long, val[u][i]) != RING_EMPTY;
+                pend[u] = !all;
+                any = any || !all;
             }
-            __builtin_amdgcn_s_sleep(1);
+        if (!any) break;
+        if (spin >= SPIN_LIMIT) {
+            good = 0;
+            break;
         }
-#pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] += val[i];
+        __builtin_amdgcn_s_sleep(1);
     }
 #pragma unroll
-    for (int i = 0; i < NV; ++i) tot[i] = wave_total(acc[i]);
+    for (int i = 0; i < NV; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += val[u][i];
+        tot[i] = wave_total(acc);
+    }
     if (lane == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i)
@@ -187,7 +243,9 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
     constexpr bool OFFS = sizeof(T) == 4;   // subtract the corner-0 value first (FP32 rounding)
     const GridDev& g = A.g;
     const ResGeom& G = A.G;
+    typedef typename Empty<T>::U EU;
     T* sP = reinterpret_cast<T*>(smem + G.o_p);
+    T* sZ = reinterpret_cast<T*>(smem + G.o_z);
     T* sR = reinterpret_cast<T*>(smem + G.o_r);
     T* sX = reinterpret_cast<T*>(smem + G.o_x);
     T* sD = reinterpret_cast<T*>(smem + G.o_d);
@@ -212,11 +270,16 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
     const int ej0 = max(j0 - 1, 0), ei0 = max(i0 - 1, 0), ek0 = max(k0 - 1, 0);
     const int cj = min(j1, g.ny) - ej0, ci = min(i1, g.nx) - ei0, ck = min(k1, g.nz) - ek0;
     const int ne = cj * ci * ck;
+    Decode3 dO, dH, dE;                 // owned nodes, halo nodes, elements (j fastest)
+    dO.init(bj, bi);
+    dH.init(hj, hi);
+    dE.init(cj, ci);
     auto gdof = [&](int i, int j, int k) { return 3LL * (((long long)k * g.NI + i) * g.NJ + j); };
 
     // ---- load: owned r, x, dinv; p_old on the halo brick; element scales
     for (int t = threadIdx.x; t < no; t += NT) {
-        const int lj = t % bj, li = (t / bj) % bi, lk = t / (bj * bi);
+        int lj, li, lk;
+        dO(t, lj, li, lk);
         const int o = 3 * ((lk * G.BI + li) * G.BJ + lj);
         const long long gd = gdof(i0 + li, j0 + lj, k0 + lk);
 #pragma unroll
@@ -230,7 +293,8 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         const T* pg = ((S.k - 1) & 1) ? A.p1 : A.p0;
         const bool first = S.first != 0;
         for (int t = threadIdx.x; t < nh; t += NT) {
-            const int hj_ = t % hj, hi_ = (t / hj) % hi, hk_ = t / (hj * hi);
+            int hj_, hi_, hk_;
+            dH(t, hj_, hi_, hk_);
             const int j = j0 - 1 + hj_, i = i0 - 1 + hi_, k = k0 - 1 + hk_;
             const bool in = unsigned(j) < unsigned(g.NJ) && unsigned(i) < unsigned(g.NI) && unsigned(k) < unsigned(g.NK);
             const int o = 3 * ((hk_ * G.HI + hi_) * G.HJ + hj_);
@@ -240,7 +304,8 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         }
     }
     for (int t = threadIdx.x; t < ne; t += NT) {
-        const int lj = t % cj, li = (t / cj) % ci, lk = t / (cj * ci);
+        int lj, li, lk;
+        dE(t, lj, li, lk);
         sS[t] = A.s_e[((long long)(ek0 + lk) * g.nx + (ei0 + li)) * g.ny + (ej0 + lj)];
     }
     __syncthreads();
@@ -254,8 +319,7 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         const T beta = T(S.beta);
         const bool first = S.first != 0;
         kexec = S.k;
-        // ---- p on the halo brick, q = 0 (the exchange loads of up to PF nodes
-        // per thread are issued before any is used)
+        // ---- p on the halo brick, q = 0
         constexpr int PF = 4;
         for (int base = 0; base < nh; base += PF * NT) {
             T zv[PF][3];
@@ -266,7 +330,8 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
                 ho[u] = -1;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) zv[u][c] = T(0);
-                const int hj_ = t % hj, hi_ = (t / hj) % hi, hk_ = t / (hj * hi);
+                int hj_, hi_, hk_;
+            dH(t, hj_, hi_, hk_);
                 const int j = j0 - 1 + hj_, i = i0 - 1 + hi_, k = k0 - 1 + hk_;
                 if (t < nh && unsigned(j) < unsigned(g.NJ) && unsigned(i) < unsigned(g.NI) &&
                     unsigned(k) < unsigned(g.NK)) {
@@ -280,9 +345,9 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
                     } else if (it == 0) {   // before the first exchange
 #pragma unroll
                         for (int c = 0; c < 3; ++c) zv[u][c] = A.dinv[gd + c] * A.r[gd + c];
-                    } else {
+                    } else {                // collected during the previous iteration's grid sum
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) zv[u][c] = ld_dev(&A.zx[gd + c]);
+                        for (int c = 0; c < 3; ++c) zv[u][c] = sZ[ho[u] + c];
                     }
                 }
             }
@@ -302,7 +367,8 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
             const int t = e0 + int(threadIdx.x);
             const bool ev = t < ne;
             const int tt = ev ? t : 0;
-            const int lj = tt % cj, li = (tt / cj) % ci, lk = tt / (cj * ci);
+            int lj, li, lk;
+            dE(tt, lj, li, lk);
             const int ie = ei0 + li, je = ej0 + lj, ke = ek0 + lk;
             const int hb = ((ke - k0 + 1) * G.HI + (ie - i0 + 1)) * G.HJ + (je - j0 + 1);
             T U[8][3];
@@ -337,7 +403,8 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         const int e = it % 3, ed = (it + 2) % 3;
         double pq[1] = {0.0};
         for (int t = threadIdx.x; t < no; t += NT) {
-            const int lj = t % bj, li = (t / bj) % bi, lk = t / (bj * bi);
+            int lj, li, lk;
+        dO(t, lj, li, lk);
             const int o = 3 * ((lk * G.BI + li) * G.BJ + lj);
             const int h = 3 * (((lk + 1) * G.HI + li + 1) * G.HJ + lj + 1);
 #pragma unroll
@@ -345,7 +412,7 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         }
         {
             double tot[1];
-            const bool good = grid_sum<1>(pq, ring1 + e * nb, ring1 + ed * nb, sRed, tot);
+            const bool good = grid_sum<1, true>(pq, ring1 + e * nb, ring1 + ed * nb, sRed, tot);
             if (threadIdx.x == 0) {
                 if (good) dir_finish(&S, tot[0]);
                 else sFail = 1;
@@ -359,10 +426,15 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         if (S.status != ST_RUNNING) break;
         const T alpha = T(S.alpha);
 
-        // ---- x += alpha p, r = mask(r - alpha q), z = dinv r (surface z published)
+        // ---- x += alpha p, r = mask(r - alpha q), z = dinv r; surface z published
+        // in entry it % 3, this block's words of entry (it - 1) % 3 emptied (every
+        // block collected them before its p.q publication of this iteration)
+        T* zw = A.zr[e];
+        EU* zc = reinterpret_cast<EU*>(A.zr[ed]);
         double rz[2] = {0.0, 0.0};
         for (int t = threadIdx.x; t < no; t += NT) {
-            const int lj = t % bj, li = (t / bj) % bi, lk = t / (bj * bi);
+            int lj, li, lk;
+        dO(t, lj, li, lk);
             const int o = 3 * ((lk * G.BI + li) * G.BJ + lj);
             const int h = 3 * (((lk + 1) * G.HI + li + 1) * G.HJ + lj + 1);
             const bool surf = (lj == 0 && j0 > 0) || (lj == bj - 1 && j1 < g.NJ) || (li == 0 && i0 > 0) ||
@@ -377,12 +449,65 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
                 const T z = d * ri;
                 rz[0] += double(ri) * double(z);
                 rz[1] += double(ri) * double(ri);
-                if (surf) st_dev(&A.zx[gd + c], z);
+                if (surf) {
+                    st_dev(&zw[gd + c], z);
+                    if (it > 0) st_dev(&zc[gd + c], Empty<T>::bits);
+                }
             }
         }
         {
             double tot[2];
-            const bool good = grid_sum<2>(rz, ring2 + 2 * e * nb, ring2 + 2 * ed * nb, sRed, tot);
+            // no release: the z words carry their own EMPTY test
+            const bool good = grid_sum<2, false>(rz, ring2 + 2 * e * nb, ring2 + 2 * ed * nb, sRed, tot);
+            // every thread collects the z of its shell nodes into the z layout
+            const EU* zr = reinterpret_cast<const EU*>(A.zr[e]);
+            int zgood = 1;
+            for (int base = 0; base < nh; base += PF * NT) {
+                EU zb[PF][3];
+                int ho[PF];
+                long long gd[PF];
+                bool pend[PF];
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    const int t = base + u * NT + int(threadIdx.x);
+                    int hj_, hi_, hk_;
+            dH(t, hj_, hi_, hk_);
+                    const int j = j0 - 1 + hj_, i = i0 - 1 + hi_, k = k0 - 1 + hk_;
+                    const bool own = hj_ >= 1 && hj_ <= bj && hi_ >= 1 && hi_ <= bi && hk_ >= 1 && hk_ <= bk;
+                    pend[u] = t < nh && !own && unsigned(j) < unsigned(g.NJ) && unsigned(i) < unsigned(g.NI) &&
+                              unsigned(k) < unsigned(g.NK);
+                    ho[u] = 3 * ((hk_ * G.HI + hi_) * G.HJ + hj_);
+                    gd[u] = pend[u] ? gdof(i, j, k) : 0;
+                }
+                for (long long spin = 0;; ++spin) {
+#pragma unroll
+                    for (int u = 0; u < PF; ++u)
+                        if (pend[u]) {
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) zb[u][c] = ld_dev(&zr[gd[u] + c]);
+                        }
+                    bool any = false;
+#pragma unroll
+                    for (int u = 0; u < PF; ++u)
+                        if (pend[u]) {
+                            if (zb[u][0] != Empty<T>::bits && zb[u][1] != Empty<T>::bits &&
+                                zb[u][2] != Empty<T>::bits) {
+                                pend[u] = false;
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) sZ[ho[u] + c] = __builtin_bit_cast(T, zb[u][c]);
+                            } else {
+                                any = true;
+                            }
+                        }
+                    if (!any) break;
+                    if (spin >= SPIN_LIMIT) {
+                        zgood = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (!zgood) sFail = 1;
             if (threadIdx.x == 0) {
                 if (good) update_finish<REFINED>(&S, tot[0], tot[1]);
                 else sFail = 1;
@@ -402,7 +527,8 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
     // ---- write back the owned x, r and the direction of the last iteration
     T* pw = (kexec & 1) ? A.p1 : A.p0;
     for (int t = threadIdx.x; t < no; t += NT) {
-        const int lj = t % bj, li = (t / bj) % bi, lk = t / (bj * bi);
+        int lj, li, lk;
+        dO(t, lj, li, lk);
         const int o = 3 * ((lk * G.BI + li) * G.BJ + lj);
         const int h = 3 * (((lk + 1) * G.HI + li + 1) * G.HJ + lj + 1);
         const long long gd = gdof(i0 + li, j0 + lj, k0 + lk);
@@ -419,9 +545,17 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
     }
 }
 
-// Ring entries -> RING_EMPTY, error word -> 0 (before each launch).
-__global__ void __launch_bounds__(256) k_ring_reset(unsigned long long* ring, int n, unsigned* err) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ring[i] = RING_EMPTY;
+// Sum rings -> RING_EMPTY, z rings -> Empty<T>, error word -> 0 (before each launch).
+template <typename T>
+__global__ void __launch_bounds__(256) k_ring_reset(unsigned long long* ring, int n, unsigned* err, ResArgs<T> A) {
+    typedef typename Empty<T>::U EU;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) ring[i] = RING_EMPTY;
+    for (int e = 0; e < 3; ++e) {
+        EU* z = reinterpret_cast<EU*>(A.zr[e]);
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < A.g.ndof; i += stride)
+            z[i] = Empty<T>::bits;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) *err = 0u;
 }
 

@@ -5,7 +5,8 @@
 Builds the clamped box operator (per-element scale 10**U(9,12), seed 5), then
 times the fused direction + K*p kernel and the update kernel of one PCG
 iteration (each launched `reps` times alone, captured in a HIP graph, timed
-with HIP events on the launching stream), and finally runs a plain
+with HIP events on the launching stream) -- with --variant 7, one launch of the
+resident solver running `reps` iterations -- and finally runs a plain
 elementwise kernel (torch.neg, float32) over --copy-mib MiB -- it reads and
 writes exactly that many bytes -- so hardware byte counters (FETCH_SIZE,
 WRITE_SIZE) can be calibrated against a kernel of known traffic.  Prints one JSON line.  Run it under
@@ -72,6 +73,12 @@ def main(argv=None):
     for prec in (int(p) for p in a.precision.split(",")):
         d_us, u_us = time_kernels(op, prec, a.reps)
         b = bytes_per_launch(op, prec)
+        if a.variant == 7:
+            # the resident solver: one launch of `reps` iterations; d_us is per iteration
+            it_b = b["dir"] + b["update"]
+            out[f"fp{prec}"] = {"iteration_us": d_us, "launch_us": d_us * a.reps, "iterations_per_launch": a.reps,
+                                "iteration_bytes": it_b, "iteration_GBps": it_b / d_us * 1e-3}
+            continue
         out[f"fp{prec}"] = {"dir_us": d_us, "update_us": u_us, "dir_bytes": b["dir"], "update_bytes": b["update"],
                             "dir_GBps": b["dir"] / d_us * 1e-3, "update_GBps": b["update"] / u_us * 1e-3}
     if a.copy_mib > 0:
