@@ -361,7 +361,8 @@ struct WsRef {
 };
 
 static size_t partial_slots(const GridDev& d) {
-    return size_t(std::max<long long>(d.ntiles, 2048)) * 4;
+    // four per block of the streaming kernels; the resident solver's sum rings and error word
+    return size_t(std::max<long long>(std::max<long long>(d.ntiles, 2048) * 4, RES_RING_DOUBLES + 64));
 }
 
 static size_t ws_bytes_64(const GridDev& d) {
@@ -538,12 +539,12 @@ template <> struct ResThreads<float> { static constexpr int value = 1024; };
 template <> struct ResThreads<double> { static constexpr int value = 512; };
 
 // Brick split for the resident solver: at most one block per CU, LDS within
-// 159 KiB (the kernel's static LDS stays below 1 KiB of the CU's 160); the
+// 158 KiB (the kernel's static LDS stays below 2 KiB of the CU's 160); the
 // fewest element batches (elements / threads, rounded up) per block, then the
 // fewest elements, then the fewest blocks.
 template <typename T>
 static bool res_plan(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
-    constexpr long long LDS_MAX = 159 * 1024;
+    constexpr long long LDS_MAX = 158 * 1024;
     bool found = false;
     long long bcost = 0;
     int bnb = 0;
@@ -624,9 +625,9 @@ static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* di
     ResArgs<T> A{};
     A.g = d; A.G = G; A.s_e = s_e; A.kh = K0 + 576; A.dinv = dinv; A.x = x; A.r = r; A.p0 = p0; A.p1 = p1;
     for (int e = 0; e < 3; ++e) A.zr[e] = zr.z[e];
-    A.ring = partials; A.err = reinterpret_cast<unsigned*>(partials + 9 * nb); A.st = st;
+    A.ring = partials; A.err = reinterpret_cast<unsigned*>(partials + RES_RING_DOUBLES); A.st = st;
     hipLaunchKernelGGL(k_ring_reset<T>, dim3(stream_blocks(3 * d.ndof)), dim3(256), 0, s,
-                       reinterpret_cast<unsigned long long*>(partials), 9 * nb, A.err, A);
+                       reinterpret_cast<unsigned long long*>(partials), RES_RING_DOUBLES, A.err, A);
     LAUNCHCHK();
     void* args[] = {&A};
     const hipError_t le = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(NT), args, unsigned(G.lds_bytes), s);

@@ -49,6 +49,13 @@ namespace h8 {
 constexpr unsigned long long RING_EMPTY = 0x7FF4DEADBEEF0001ull;
 constexpr long long SPIN_LIMIT = 1LL << 22;
 constexpr int RES_MAX_BLOCKS = 256;   // grid sums poll up to 4 slots per lane of one wave
+// Every partial is published in RES_REPL copies, RES_RSTRIDE doubles apart (4 KiB:
+// different memory channels), and block b polls copy b % RES_REPL: the polls of
+// all blocks spread over the copies instead of all reading the same words.
+constexpr int RES_REPL = 8;
+constexpr int RES_RSTRIDE = 512;
+constexpr int RES_ENTRY = RES_REPL * RES_RSTRIDE;             // doubles per ring entry
+constexpr int RES_RING_DOUBLES = 2 * 3 * RES_ENTRY;           // p.q ring, then (r.z, r.r) ring
 
 // EMPTY pattern of an exchanged vector word (a signalling NaN of T).
 template <typename T> struct Empty;
@@ -87,7 +94,7 @@ struct ResArgs {
     T* p0;             // direction buffers: iteration k's p is written to p[k & 1]
     T* p1;
     T* zr[3];          // z exchange ring (three vectors)
-    double* ring;      // [3][nb] (p.q), then [3][nb][2] (r.z, r.r)
+    double* ring;      // RES_RING_DOUBLES: p.q ring, then (r.z, r.r) ring; entry [RES_REPL][RES_RSTRIDE]
     unsigned* err;     // set to 1 when a poll exceeded SPIN_LIMIT
     PcgState* st;
 };
@@ -155,12 +162,13 @@ __device__ __forceinline__ double wave_total(double v) {
     return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
 }
 
-// Grid-wide sum of NV doubles through one ring entry ([nb][NV]); `done`: the
-// entry used one iteration earlier, whose slot of this block is emptied.
-// Every wave reduces its lanes, wave 0 adds the waves, publishes the block's
-// partial, polls the nb <= 256 slots (lane l: slots l + 64 u, u < 4, all
-// loads of a round in flight together) and reduces them in that fixed order,
-// so `tot` holds the same total in wave 0 of every block.  RELEASE: the
+// Grid-wide sum of NV doubles through one ring entry ([RES_REPL][nb][NV] with
+// copies RES_RSTRIDE apart); `done`: the entry used one iteration earlier,
+// whose slots of this block are emptied.  Every wave reduces its lanes, wave 0
+// adds the waves, publishes the block's partial (lanes 0..RES_REPL-1: one copy
+// each), polls the nb <= 256 slots of its copy (lane l: slots l + 64 u, u < 4,
+// all loads of a round in flight together) and reduces them in that fixed
+// order, so `tot` holds the same total in wave 0 of every block.  RELEASE: the
 // block's earlier device-scope stores complete before the publication.
 // Returns false in wave 0 if a poll timed out (true in the other waves, which
 // return after the first barrier).  The caller reads `tot` in thread 0 and
@@ -183,8 +191,9 @@ __device__ bool grid_sum(const double (&v)[NV], double* entry, double* done, dou
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         const double bsum = wave_total(lane < nw ? sh[lane * NV + i] : 0.0);
-        if (lane == 0) st_dev(&entry[blockIdx.x * NV + i], bsum);
+        if (lane < RES_REPL) st_dev(&entry[lane * RES_RSTRIDE + blockIdx.x * NV + i], bsum);
     }
+    const double* copy = entry + (blockIdx.x % RES_REPL) * RES_RSTRIDE;
     constexpr int U = RES_MAX_BLOCKS / 64;
     double val[U][NV];
     bool pend[U];
@@ -201,7 +210,7 @@ __device__ bool grid_sum(const double (&v)[NV], double* entry, double* done, dou
         for (int u = 0; u < U; ++u)
             if (pend[u]) {
 #pragma unroll
-                for (int i = 0; i < NV; ++i) val[u][i] = ld_dev(&entry[(lane + 64 * u) * NV + i]);
+                for (int i = 0; i < NV; ++i) val[u][i] = ld_dev(&copy[(lane + 64 * u) * NV + i]);
             }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -226,10 +235,10 @@ __device__ bool grid_sum(const double (&v)[NV], double* entry, double* done, dou
         for (int u = 0; u < U; ++u) acc += val[u][i];
         tot[i] = wave_total(acc);
     }
-    if (lane == 0) {
+    if (lane < RES_REPL) {
 #pragma unroll
         for (int i = 0; i < NV; ++i)
-            st_dev(reinterpret_cast<unsigned long long*>(&done[blockIdx.x * NV + i]), RING_EMPTY);
+            st_dev(reinterpret_cast<unsigned long long*>(&done[lane * RES_RSTRIDE + blockIdx.x * NV + i]), RING_EMPTY);
     }
     return __all(good) != 0;
 }
@@ -312,7 +321,7 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
 
     const int nb = gridDim.x;
     double* ring1 = A.ring;
-    double* ring2 = A.ring + 3 * nb;
+    double* ring2 = A.ring + 3 * RES_ENTRY;
     long long kexec = S.k;
     bool ok = true;
     for (int it = 0;; ++it) {
@@ -412,7 +421,7 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         }
         {
             double tot[1];
-            const bool good = grid_sum<1, true>(pq, ring1 + e * nb, ring1 + ed * nb, sRed, tot);
+            const bool good = grid_sum<1, true>(pq, ring1 + e * RES_ENTRY, ring1 + ed * RES_ENTRY, sRed, tot);
             if (threadIdx.x == 0) {
                 if (good) dir_finish(&S, tot[0]);
                 else sFail = 1;
@@ -458,7 +467,7 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
         {
             double tot[2];
             // no release: the z words carry their own EMPTY test
-            const bool good = grid_sum<2, false>(rz, ring2 + 2 * e * nb, ring2 + 2 * ed * nb, sRed, tot);
+            const bool good = grid_sum<2, false>(rz, ring2 + e * RES_ENTRY, ring2 + ed * RES_ENTRY, sRed, tot);
             // every thread collects the z of its shell nodes into the z layout
             const EU* zr = reinterpret_cast<const EU*>(A.zr[e]);
             int zgood = 1;
