@@ -81,10 +81,17 @@ KERNEL_NAMES = {  # (variant, precision) -> kernel symbol of the fused direction
     (2, 32): "k_mfma<float, 16, 8, 8, 1>", (2, 64): "k_mfma<double, 8, 8, 8, 1>",
     (3, 32): "k_node_persist<float, 8, 8, 4, 1>", (3, 64): "k_node_persist<double, 8, 8, 4, 1>",
     (4, 32): "k_node2<float, 8, 8, 4, 1>", (4, 64): "k_node2<double, 8, 8, 4, 1>",
-    (5, 32): "k_sector<float, 16, 16, 1>", (5, 64): "k_sector<double, 16, 16, 1>",
-    (6, 32): "k_sector<float, 8, 16, 1>", (6, 64): "k_sector<double, 8, 16, 1>",
-    (7, 32): "k_resident<float, 1024, true>", (7, 64): "k_resident<double, 512, false>",
+    (5, 32): "k_sector<float, 16, 16, 1, %s>", (5, 64): "k_sector<double, 16, 16, 1, double>",
+    (6, 32): "k_sector<float, 8, 16, 1, %s>", (6, 64): "k_sector<double, 8, 16, 1, double>",
+    (7, 32): "k_resident<float, 1024, true, %s>", (7, 64): "k_resident<double, 512, false, double>",
 }
+
+
+def kernel_name(variant, precision, arith=0):
+    """Symbol of the fused direction kernel; FP32 sector and resident kernels
+    carry their element-product arithmetic (FP32 unless arith = 64)."""
+    n = KERNEL_NAMES.get((variant, precision), "unknown")
+    return n % ("double" if arith == 64 else "float") if "%s" in n else n
 RESIDENT_REPS = 100   # iterations per launch of the resident solver's timing run
 
 
@@ -123,6 +130,8 @@ def main():
     ap.add_argument("--dims", default="128,32,32")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--delta", type=float, default=0.5)
+    ap.add_argument("--arith", type=int, default=0,
+                    help="element-product arithmetic of the FP32 iterations: 0 = library (FP32), 32, 64")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); the multi-GPU "
@@ -137,6 +146,7 @@ def main():
     t0 = time.time()
     fs, E = build_system(nx, ny, nz, a.seed)
     op = fs.operator
+    op.set_arith(a.arith)
     n_free = fs.n_dof - len(fs.fixed_dofs)
     m = nx * ny * nz
     log(f"setup {time.time() - t0:.1f}s: n_dof={fs.n_dof} n_free={n_free} m={m}")
@@ -198,7 +208,7 @@ def main():
     n_dof = fs.n_dof
     dir_bytes = 5 * 4 * n_dof + 4 * m          # read r, dinv, p_old; write p, q; read s_e
     upd_bytes = 7 * 4 * n_dof                  # read x, r, p, q, dinv; write x, r
-    kname = KERNEL_NAMES.get((op.grid32.variant, 32), "unknown")
+    kname = kernel_name(op.grid32.variant, 32, op.grid32.arith)
     if resident:
         # one launch = RESIDENT_REPS iterations, each with the bytes of a streaming iteration
         launch_ms = dir_ms * RESIDENT_REPS
@@ -207,7 +217,7 @@ def main():
                  f"({RESIDENT_REPS} iterations per timed launch)")
     else:
         launch_ms, launch_bytes = dir_ms, dir_bytes
-        kdesc = f"{kname}: fused direction + K*p + p.q (FP32)"
+        kdesc = f"{kname}: fused direction + K*p + p.q (FP32 data)"
     achieved = launch_bytes / (launch_ms * 1e-3)
     traffic, traffic_src = profiled_traffic((nx, ny, nz), kname)
     # whole-iteration contract: B_iter = 13 n_f s + m s_c (FP32), B_outer = 6 n_f 8 + 8 m
@@ -231,7 +241,8 @@ def main():
         "config": {"workload": f"{nx}x{ny}x{nz} H8 cantilever, E=10**U(9,12) per element (seed {a.seed}), "
                                f"nu=0.3, clamped x=0, -1e6 consistent tip traction, true rel. residual < 1e-8",
                    "n_free": n_free, "elements": m, "delta": a.delta,
-                   "kernels": {"fp32": {"variant": op.grid32.variant, "layers": op.grid32.layers},
+                   "kernels": {"fp32": {"variant": op.grid32.variant, "layers": op.grid32.layers,
+                                        "element_products": "f64" if op.grid32.arith == 64 else "f32"},
                                "fp64": {"variant": op.grid.variant, "layers": op.grid.layers}}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
@@ -242,7 +253,7 @@ def main():
                      "update_kernel_avg_us": None if resident else upd_ms * 1e3,
                      "update_bytes_per_launch": None if resident else upd_bytes,
                      "update_frac": None if resident else upd_bytes / (upd_ms * 1e-3) / HBM_PEAK,
-                     "fp64_kernel": KERNEL_NAMES.get((op.grid.variant, 64), "unknown"),
+                     "fp64_kernel": kernel_name(op.grid.variant, 64),
                      "fp64_iteration_us": (avg64[0] + avg64[1]) * 1e3,
                      "iteration_contract_frac": b_solve / t_solve / HBM_PEAK,
                      "fp64_iteration_contract_frac": b64 / dt64 / HBM_PEAK},

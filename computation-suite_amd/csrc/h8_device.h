@@ -24,6 +24,7 @@ struct GridDev {
     int ntiles;            // tj*ti*tk
     int variant;           // H8_VARIANT_*
     int layers;            // node layers per block of the sector kernel (0 = choose)
+    int arith;             // element-product arithmetic of FP32 sector/resident kernels (0 = 32, 32, 64)
 };
 
 inline void set_tiles(GridDev& d, int TJ, int TI, int TK) {

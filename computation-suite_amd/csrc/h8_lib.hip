@@ -119,8 +119,11 @@ static int make_grid(const h8_grid* g, GridDev& d) {
     if (g->variant < H8_VARIANT_AUTO || g->variant > H8_VARIANT_RESIDENT)
         return fail(H8_ERR_ARG, "unknown kernel variant %d", g->variant);
     if (g->layers < 0) return fail(H8_ERR_ARG, "layers must be >= 0 (0 = library choice), got %d", g->layers);
+    if (g->arith != 0 && g->arith != 32 && g->arith != 64)
+        return fail(H8_ERR_ARG, "arith must be 0 (library choice), 32 or 64, got %d", g->arith);
     d.variant = g->variant;
     d.layers = g->layers;
+    d.arith = g->arith;
     return H8_OK;
 }
 
@@ -165,6 +168,19 @@ static int persistent_grid(const void* kernel, int threads, int ntiles) {
     return int(std::max(1LL, gsz));
 }
 
+// FP32 data with FP64 element products (arith 64).  The library's choice is
+// FP32 products: the FP32 iterations need 1.6-2x the FP64 count on
+// ill-conditioned systems, but that comes from rounding the recurrence
+// vectors (p, r, q, z) to FP32 -- any one of them -- not from the products;
+// FP64 products leave the count unchanged and cost registers.
+template <typename T> static bool fp64_products(const GridDev& d) {
+    return sizeof(T) == 4 && d.arith == 64;
+}
+// The FP64 sector table of a K0 buffer: at word 576 of an FP64 buffer, at
+// word 648 (bit-copied doubles, 8-byte aligned) of an FP32 one.
+static const double* k0_table64(const double* K0) { return K0 + 576; }
+static const double* k0_table64(const float* K0) { return reinterpret_cast<const double*>(K0 + H8_K0_WORDS); }
+
 // Sector kernel: column tiles of (SEJ-1) x (SEI-1) nodes, chunks of `layers`
 // node layers (library choice: about 4 blocks per CU, 2..16 layers).  The
 // partials buffer of the fused modes holds max(brick tiles, 2048) blocks, so
@@ -185,7 +201,12 @@ static int launch_sector(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
         return fail(H8_ERR_ARG, "sector kernel: %lld blocks exceed the partials capacity (use more layers)", nt);
     d.ntiles = int(nt);
     A.g = d;
-    hipLaunchKernelGGL((k_sector<T, EJ, EI, MODE>), dim3(d.ntiles), dim3(EJ * EI), 0, s, A, A.K0 + 576, kd);
+    if (fp64_products<T>(d)) {
+        hipLaunchKernelGGL((k_sector<T, EJ, EI, MODE, double>), dim3(d.ntiles), dim3(EJ * EI), 0, s, A,
+                           k0_table64(A.K0), kd);
+    } else {
+        hipLaunchKernelGGL((k_sector<T, EJ, EI, MODE>), dim3(d.ntiles), dim3(EJ * EI), 0, s, A, A.K0 + 576, kd);
+    }
     LAUNCHCHK();
     return H8_OK;
 }
@@ -534,9 +555,14 @@ static int chunk_for(long long maxiter) {
 // variant was the library's choice (not an error: the streaming loop runs)
 constexpr int RES_DECLINED = -1;
 
-template <typename T> struct ResThreads;
-template <> struct ResThreads<float> { static constexpr int value = 1024; };
-template <> struct ResThreads<double> { static constexpr int value = 512; };
+// Threads per block of k_resident by (storage, element-product arithmetic).
+template <typename T, typename TA> struct ResThreads;
+template <> struct ResThreads<float, float> { static constexpr int value = 1024; };
+template <> struct ResThreads<float, double> { static constexpr int value = 1024; };
+template <> struct ResThreads<double, double> { static constexpr int value = 512; };
+template <typename T> static int res_threads(const GridDev& d) {
+    return fp64_products<T>(d) ? ResThreads<T, double>::value : ResThreads<T, T>::value;
+}
 
 // Brick split for the resident solver: at most one block per CU, LDS within
 // 158 KiB (the kernel's static LDS stays below 2 KiB of the CU's 160); the
@@ -596,17 +622,18 @@ static bool use_resident(const GridDev& d) {
     if (d.variant == H8_VARIANT_RESIDENT) return true;
     if (d.variant != H8_VARIANT_AUTO) return false;
     ResGeom G;
-    return res_plan<T>(d, ResThreads<T>::value, std::min(device_cus(), RES_MAX_BLOCKS), G);
+    return res_plan<T>(d, res_threads<T>(d), std::min(device_cus(), RES_MAX_BLOCKS), G);
 }
 
 // Runs the iterations until the device-side status leaves RUNNING in one
 // cooperative launch of k_resident (q serves as the z exchange vector).
-template <typename T, bool REFINED>
-static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
-                        const ZRing<T>& zr, PcgState* st, double* partials, hipStream_t s, PcgState& hst,
-                        long long& launches) {
-    constexpr int NT = ResThreads<T>::value;
-    const void* fn = (const void*)k_resident<T, NT, REFINED>;
+// ev0/ev1 (optional): recorded just before and just after the launch.
+template <typename T, typename TA, bool REFINED>
+static int run_resident_k(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
+                          const ZRing<T>& zr, PcgState* st, double* partials, hipStream_t s, PcgState& hst,
+                          long long& launches, hipEvent_t ev0, hipEvent_t ev1) {
+    constexpr int NT = ResThreads<T, TA>::value;
+    const void* fn = (const void*)k_resident<T, NT, REFINED, TA>;
     ResGeom G;
     if (!res_plan<T>(d, NT, std::min(device_cus(), RES_MAX_BLOCKS), G))
         return fail(H8_ERR_ARG, "resident solver: a %dx%dx%d grid does not fit in LDS (one block per CU)", d.nx,
@@ -623,19 +650,22 @@ static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* di
         return fail(H8_ERR_HIP, "resident solver: %d blocks of %d threads and %d bytes of LDS cannot all be resident",
                     nb, NT, G.lds_bytes);
     ResArgs<T> A{};
-    A.g = d; A.G = G; A.s_e = s_e; A.kh = K0 + 576; A.dinv = dinv; A.x = x; A.r = r; A.p0 = p0; A.p1 = p1;
+    A.g = d; A.G = G; A.s_e = s_e; A.kh = K0 + 576; A.kh64 = k0_table64(K0); A.dinv = dinv; A.x = x; A.r = r;
+    A.p0 = p0; A.p1 = p1;
     for (int e = 0; e < 3; ++e) A.zr[e] = zr.z[e];
     A.ring = partials; A.err = reinterpret_cast<unsigned*>(partials + RES_RING_DOUBLES); A.st = st;
     hipLaunchKernelGGL(k_ring_reset<T>, dim3(stream_blocks(3 * d.ndof)), dim3(256), 0, s,
                        reinterpret_cast<unsigned long long*>(partials), RES_RING_DOUBLES, A.err, A);
     LAUNCHCHK();
     void* args[] = {&A};
+    if (ev0) HIPCHK(hipEventRecord(ev0, s));
     const hipError_t le = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(NT), args, unsigned(G.lds_bytes), s);
     if (le == hipErrorCooperativeLaunchTooLarge && d.variant == H8_VARIANT_AUTO) {
         (void)hipGetLastError();
         return RES_DECLINED;      // the caller runs the streaming kernels instead
     }
     HIPCHK(le);
+    if (ev1) HIPCHK(hipEventRecord(ev1, s));
     ++launches;
     unsigned err = 0;
     HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
@@ -643,6 +673,17 @@ static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* di
     HIPCHK(hipStreamSynchronize(s));
     if (err) return fail(H8_ERR_HIP, "resident solver: a grid-wide sum timed out");
     return H8_OK;
+}
+
+template <typename T, bool REFINED>
+static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
+                        const ZRing<T>& zr, PcgState* st, double* partials, hipStream_t s, PcgState& hst,
+                        long long& launches, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+    if (fp64_products<T>(d))
+        return run_resident_k<T, double, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hst, launches,
+                                                  ev0, ev1);
+    return run_resident_k<T, T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hst, launches, ev0,
+                                         ev1);
 }
 
 // Runs captured chunks until the device-side status leaves RUNNING.
@@ -950,10 +991,9 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
         float ms = 0.f;
         for (int rep = 0; rep < 2; ++rep) {      // warm-up, then timed
             HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
-            HIPCHK(hipEventRecord(ev[0], s));
-            int rc = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hh, launches);
+            int rc = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hh, launches,
+                                              ev[0], ev[1]);
             if (rc) return rc;
-            HIPCHK(hipEventRecord(ev[1], s));
             HIPCHK(hipEventSynchronize(ev[1]));
             HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
             if (hh.status == ST_NOT_SPD || hh.k - 2 + (hh.status == ST_REPLACE ? 0 : 1) < reps)

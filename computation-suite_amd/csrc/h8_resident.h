@@ -88,6 +88,7 @@ struct ResArgs {
     ResGeom G;
     const T* s_e;
     const T* kh;       // 72-entry sector table
+    const double* kh64; // the same table in FP64 (FP64 element products of FP32 data)
     const T* dinv;
     T* x;
     T* r;
@@ -243,15 +244,20 @@ __device__ bool grid_sum(const double (&v)[NV], double* entry, double* done, dou
     return __all(good) != 0;
 }
 
-template <typename T, int NT, bool REFINED>
+// TA: arithmetic of the element products (T, or double for FP32 data; the
+// outputs are rounded to T before the per-node sums).
+template <typename T, int NT, bool REFINED, typename TA = T>
 __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ PcgState S;
     __shared__ double sRed[16 * 2];
     __shared__ int sFail;
-    constexpr bool OFFS = sizeof(T) == 4;   // subtract the corner-0 value first (FP32 rounding)
+    constexpr bool OFFS = sizeof(TA) == 4;  // subtract the corner-0 value first (FP32 rounding)
     const GridDev& g = A.g;
     const ResGeom& G = A.G;
+    const TA* kh;
+    if constexpr (sizeof(TA) == sizeof(T)) kh = A.kh;
+    else kh = A.kh64;
     typedef typename Empty<T>::U EU;
     T* sP = reinterpret_cast<T*>(smem + G.o_p);
     T* sZ = reinterpret_cast<T*>(smem + G.o_z);
@@ -380,12 +386,12 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
             dE(tt, lj, li, lk);
             const int ie = ei0 + li, je = ej0 + lj, ke = ek0 + lk;
             const int hb = ((ke - k0 + 1) * G.HI + (ie - i0 + 1)) * G.HJ + (je - j0 + 1);
-            T U[8][3];
+            TA U[8][3];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const int h = 3 * (hb + (OZ(a) * G.HI + OX(a)) * G.HJ + OY(a));
 #pragma unroll
-                for (int c = 0; c < 3; ++c) U[a][c] = sP[h + c];
+                for (int c = 0; c < 3; ++c) U[a][c] = TA(sP[h + c]);
             }
             if constexpr (OFFS) {
 #pragma unroll
@@ -393,16 +399,16 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) U[a][c] -= U[0][c];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) U[0][c] = T(0);
+                for (int c = 0; c < 3; ++c) U[0][c] = TA(0);
             }
-            sector_product<T>(A.kh, U, ev ? sS[tt] : T(0));
+            sector_product<TA>(kh, U, ev ? TA(sS[tt]) : TA(0));
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const int ni = ie + OX(a) - i0, nj = je + OY(a) - j0, nk = ke + OZ(a) - k0;
                 if (ev && unsigned(ni) < unsigned(bi) && unsigned(nj) < unsigned(bj) && unsigned(nk) < unsigned(bk)) {
                     const int o = 3 * ((nk * G.BI + ni) * G.BJ + nj);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) sQ[o + c] += U[a][c];
+                    for (int c = 0; c < 3; ++c) sQ[o + c] += T(U[a][c]);
                 }
                 __syncthreads();
             }

@@ -220,13 +220,16 @@ __host__ __device__ inline void sector_tiles(GridDev& g, int EJ, int EI, int kd)
 // stages layers, forms p (direction modes; written for owned nodes), evaluates the
 // element products through the sectors and writes each owned node's outputs
 // (node_output), accumulating the fused partials in `part`.
-template <typename T, int EJ, int EI, int MODE>
-__device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, int kd, int tile, T beta, int first,
+// TA: arithmetic of the element products (T, or double for FP32 data: the
+// products' rounding, not the FP32 storage, is what slows the FP32 iterations'
+// convergence); the per-node sums of the outputs are in T.
+template <typename T, int EJ, int EI, int MODE, typename TA = T>
+__device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh, int kd, int tile, T beta, int first,
                                              T* p_new, const T* p_old, T alpha_prev, int pending,
                                              SectorSmem<T, EJ, EI>& sm, double (&part)[2]) {
     constexpr int TJ = EJ - 1, TI = EI - 1;
     typedef SLayer<T, EJ, EI> L;
-    constexpr bool OFFS = sizeof(T) == 4;   // subtract the corner-0 value first (FP32 rounding)
+    constexpr bool OFFS = sizeof(TA) == 4;  // subtract the corner-0 value first (FP32 rounding)
     const GridDev& g = A.g;
     const int bj = tile % g.tj, bi = (tile / g.tj) % g.ti, bk = tile / (g.tj * g.ti);
     const int j0 = bj * TJ, i0 = bi * TI;
@@ -269,12 +272,12 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, 
                                                                   : OOB);
         // ---- element phase: slab s, element (je, ie, s) (scale 0 outside the mesh)
         {
-            T U[8][3];
+            TA U[8][3];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 const T* src = OZ(a) ? up : lo;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) U[a][c] = src[lbase + 3 * (OX(a) * (EJ + 1) + OY(a)) + c];
+                for (int c = 0; c < 3; ++c) U[a][c] = TA(src[lbase + 3 * (OX(a) * (EJ + 1) + OY(a)) + c]);
             }
             if constexpr (OFFS) {
 #pragma unroll
@@ -282,9 +285,9 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, 
 #pragma unroll
                     for (int c = 0; c < 3; ++c) U[a][c] -= U[0][c];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) U[0][c] = T(0);
+                for (int c = 0; c < 3; ++c) U[0][c] = TA(0);
             }
-            sector_product<T>(kh, U, se);
+            sector_product<TA>(kh, U, TA(se));
             // corner a of this element is node (ej + OY(a), ei + OX(a)) of the halo grid;
             // slots exist for hj < EJ, hi < EI (owned nodes have 1 <= hj, hi <= EJ-1, EI-1)
 #pragma unroll
@@ -293,7 +296,7 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, 
                 T* dst = OZ(a) ? sHi : sLo;
                 const int slot = ((ei + OX(a)) * EJ + ej + OY(a)) * 12 + 3 * (a & 3);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) dst[slot + c] = U[a][c];
+                for (int c = 0; c < 3; ++c) dst[slot + c] = T(U[a][c]);
             }
         }
         __syncthreads();
@@ -325,9 +328,9 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const T* kh, 
 
 // One block per (column tile, layer chunk); A.g.{tj, ti, tk} count the column
 // tiles in j and i and the chunks of `kd` node layers.
-template <typename T, int EJ, int EI, int MODE>
-__global__ void __launch_bounds__(EJ * EI) __attribute__((amdgpu_waves_per_eu(SectorWaves<T>::value)))
-k_sector(NodeArgs<T> A, const T* kh, int kd) {
+template <typename T, int EJ, int EI, int MODE, typename TA = T>
+__global__ void __launch_bounds__(EJ * EI) __attribute__((amdgpu_waves_per_eu(SectorWaves<TA>::value)))
+k_sector(NodeArgs<T> A, const TA* kh, int kd) {
     static_assert((EJ * EI) % 64 == 0, "whole waves");
     __shared__ SectorSmem<T, EJ, EI> sm;
     __shared__ double sRed[16 * 4];
@@ -337,8 +340,8 @@ k_sector(NodeArgs<T> A, const T* kh, int kd) {
     const T* p_old;
     if (!node_setup<T, MODE>(A, beta, first, p_new, p_old, alpha_prev, pending)) return;
     double part[2] = {0.0, 0.0};
-    sector_march<T, EJ, EI, MODE>(A, kh, kd, xcd_local_tile(blockIdx.x, gridDim.x), beta, first, p_new, p_old,
-                                  alpha_prev, pending, sm, part);
+    sector_march<T, EJ, EI, MODE, TA>(A, kh, kd, xcd_local_tile(blockIdx.x, gridDim.x), beta, first, p_new, p_old,
+                                      alpha_prev, pending, sm, part);
     node_finish<T, MODE>(A, part, sRed);
 }
 

@@ -25,11 +25,12 @@ H8_ERR_WORKSPACE = 6
 (VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR,
  VARIANT_SECTOR_NARROW, VARIANT_RESIDENT) = 0, 1, 2, 3, 4, 5, 6, 7
 K0_WORDS = 648
+K0_WORDS_F32 = 792   # FP32 K0: cast of the FP64 K0, then the FP64 sector table (72 doubles)
 
 
 class H8Grid(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("variant", C.c_int32),
-                ("layers", C.c_int32)]
+                ("layers", C.c_int32), ("arith", C.c_int32)]
 
 
 class H8PcgStats(C.Structure):

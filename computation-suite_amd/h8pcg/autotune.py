@@ -82,7 +82,8 @@ def time_iteration(op, precision, variant, layers, reps=30):
     for the resident solver (time per iteration, 0).  ValueError if the grid
     does not fit the resident solver."""
     import torch
-    g = _lib.H8Grid(op.nx, op.ny, op.nz, int(variant), int(layers))
+    arith = op.grid32.arith if precision == 32 else 0
+    g = _lib.H8Grid(op.nx, op.ny, op.nz, int(variant), int(layers), int(arith))
     ws = op.workspace(precision)
     s_e, K0 = (op.s_e, op.K0buf) if precision == 64 else (op.s_e32, op.K0_32)
     dinv = op.dinv if op.dinv is not None else torch.ones(op.n_dof, dtype=torch.float64, device=op.device)
@@ -94,8 +95,22 @@ def time_iteration(op, precision, variant, layers, reps=30):
     return avg[0] * 1e3, avg[1] * 1e3
 
 
+def fp32_candidates(arith):
+    """Candidates of the FP32 iterations: with FP64 element products (arith 64)
+    only the kernels that compute them (the sector kernels; the node-centric
+    and MFMA kernels compute in FP32), else all of them."""
+    if arith != 64:
+        return CANDIDATES
+    return [(v, l) for v, l in CANDIDATES if v in (_lib.VARIANT_SECTOR, _lib.VARIANT_SECTOR_NARROW)]
+
+
 def tune(op, precision, cache=True, candidates=None, reps=30, resident=True):
+    arith = op.grid32.arith if precision == 32 else 0
     key = "%s|%dx%dx%d|fp%d" % (_device_key(), op.nx, op.ny, op.nz, int(precision))
+    if precision == 32 and arith == 64:
+        key += "|a64"
+    if candidates is None and precision == 32:
+        candidates = fp32_candidates(arith)
     path = cache_path()
     data = _load(path) if cache else {}
     if key in data:

@@ -44,7 +44,8 @@ def time_kernels(op, precision, reps):
     ws = op.workspace(precision)
     s_e, K0 = (op.s_e, op.K0buf) if precision == 64 else (op.s_e32, op.K0_32)
     avg = (C.c_double * 2)()
-    _lib.check(_lib.lib().h8_time_kernels(C.byref(op.grid), precision, _lib.ptr(s_e), _lib.ptr(K0),
+    g = op.grid if precision == 64 else op.grid32
+    _lib.check(_lib.lib().h8_time_kernels(C.byref(g), precision, _lib.ptr(s_e), _lib.ptr(K0),
                                           _lib.ptr(op.dinv), _lib.ptr(ws), ws.numel(), int(reps), avg,
                                           _lib.stream_handle(op.device)), "h8_time_kernels")
     return avg[0] * 1e3, avg[1] * 1e3
@@ -63,12 +64,16 @@ def main(argv=None):
     ap.add_argument("--precision", default="32,64")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--layers", type=int, default=0, help="node layers per block of the sector kernel (0 = auto)")
+    ap.add_argument("--arith", type=int, default=0, help="FP32 element-product arithmetic (0 = library, 32, 64)")
     ap.add_argument("--copy-mib", type=int, default=256)
     a = ap.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.dims.split(","))
     op = build_operator(nx, ny, nz)
     op.grid.variant = a.variant
     op.grid.layers = a.layers
+    op.grid32.variant = a.variant
+    op.grid32.layers = a.layers
+    op.set_arith(a.arith)
     out = {"dims": [nx, ny, nz], "n_dof": op.n_dof, "elements": op.m, "reps": a.reps}
     for prec in (int(p) for p in a.precision.split(",")):
         d_us, u_us = time_kernels(op, prec, a.reps)

@@ -108,7 +108,10 @@ class GridOperator:
         self.s_e = torch.as_tensor(s_e, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
         if self.s_e.numel() != self.m:
             raise ValueError(f"s_e has {self.s_e.numel()} entries, the grid has {self.m} elements")
-        self.K0_32 = self.K0buf.float().contiguous()
+        # FP32 layout: the cast of the FP64 buffer, then the FP64 sector table
+        # bit-copied (FP64 element products of FP32 data)
+        self.K0_32 = torch.cat([self.K0buf.float(), self.K0buf[576:].contiguous().view(torch.float32)])
+        assert self.K0_32.numel() == _lib.K0_WORDS_F32
         self.s_e32 = self.s_e.float().contiguous()
         self.dinv = None
         self._ws = {}
@@ -252,6 +255,14 @@ class GridOperator:
             if precision in (None, p):
                 g.variant = int(variant)
                 g.layers = int(layers)
+
+    def set_arith(self, arith):
+        """Element-product arithmetic of the FP32 sector and resident kernels:
+        0 (library choice: FP32), 32 (FP32 products) or 64 (FP64 products of
+        the FP32 data, outputs rounded to FP32)."""
+        if int(arith) not in (0, 32, 64):
+            raise ValueError(f"arith must be 0, 32 or 64, got {arith}")
+        self.grid32.arith = int(arith)
 
     def autotune(self, precisions=(32, 64), cache=True):
         """Times the candidate kernels for this grid and keeps the fastest per

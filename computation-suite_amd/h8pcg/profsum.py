@@ -2,6 +2,7 @@
 
     python -m h8pcg.profsum stats <dir with *_kernel_stats.csv> [--top 12]
     python -m h8pcg.profsum traffic --fetch <dir> --write <dir> [--calib-bytes 268435456]
+    python -m h8pcg.profsum launches <dir with *_kernel_trace.csv> --kernel NAME [--last 2]
 
 ``stats`` prints the per-kernel table of a ``--kernel-trace --stats`` run.
 ``traffic`` reads two ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs of
@@ -9,7 +10,10 @@
 each by what the same counter reads for the calibration kernel (torch's
 elementwise ``neg`` over --calib-bytes bytes, which reads and writes exactly that
 many bytes).  It prints one ``traffic-json:`` line per h8 kernel, in bytes per
-launch, that bench.py reads back.
+launch, that bench.py reads back.  ``launches`` prints the durations of the last
+--last dispatches of the kernels whose name contains NAME, in dispatch order
+(for a kernel whose launches differ in length, such as a solver that runs a
+whole solve phase per launch, the dispatches of one timing run).
 """
 from __future__ import annotations
 
@@ -80,6 +84,19 @@ def traffic(fetch_dir, write_dir, calib_bytes):
         print("traffic-json: " + json.dumps(o))
 
 
+def launches(path, kernel, last):
+    f = _one(os.path.join(path, "**", "*kernel_trace.csv"))
+    rows = [r for r in csv.DictReader(open(f)) if kernel in r["Kernel_Name"]]
+    if not rows:
+        raise SystemExit(f"profsum: no dispatch of a kernel named like {kernel!r}")
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    print(f"{len(rows)} dispatches of `{short_name(rows[0]['Kernel_Name'])}`; the last {min(last, len(rows))}:\n")
+    print("| dispatch | duration us |")
+    print("|---:|---:|")
+    for r in rows[-last:]:
+        print(f"| {r['Dispatch_Id']} | {(int(r['End_Timestamp']) - int(r['Start_Timestamp'])) / 1e3:.2f} |")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -90,9 +107,15 @@ def main(argv=None):
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
     t.add_argument("--calib-bytes", type=int, default=256 << 20)
+    ln = sub.add_parser("launches")
+    ln.add_argument("dir")
+    ln.add_argument("--kernel", required=True)
+    ln.add_argument("--last", type=int, default=2)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         stats(a.dir, a.top)
+    elif a.cmd == "launches":
+        launches(a.dir, a.kernel, a.last)
     else:
         traffic(a.fetch, a.write, a.calib_bytes)
 

@@ -21,8 +21,10 @@
  *
  * A `K0` argument is H8_K0_WORDS values: the 24x24 element matrix (row-major,
  * entry 24*i + j) followed by its 72-value reflection-sector table, which
- * h8_sector_table_f64 fills from the matrix (an FP32 K0 is the cast of the
- * FP64 one).  Only the sector kernels read the table.
+ * h8_sector_table_f64 fills from the matrix.  An FP32 K0 (h8_spmv_f32,
+ * h8_refine_solve, h8_time_kernels at precision 32) is H8_K0_WORDS_F32 words:
+ * the cast of the FP64 one followed by the FP64 table itself.  Only the sector
+ * and resident kernels read the tables.
  */
 #ifndef H8PCG_H
 #define H8PCG_H
@@ -54,11 +56,18 @@ extern "C" {
                                      (grids that fit one block per CU; other products: the default) */
 
 #define H8_K0_WORDS 648           /* 576 matrix entries + 72 sector-table entries */
+#define H8_K0_WORDS_F32 792       /* an FP32 K0: the cast of the FP64 K0 (648 words), then the
+                                     FP64 sector table bit-copied (72 doubles = 144 words, 8-byte
+                                     aligned) for FP64 element products of FP32 data */
 
 typedef struct h8_grid {
     int32_t nx, ny, nz;     /* elements per axis (>= 1)                      */
     int32_t variant;        /* H8_VARIANT_* for the K*u kernels              */
     int32_t layers;         /* node layers per block of H8_VARIANT_SECTOR (0 = library choice) */
+    int32_t arith;          /* element-product arithmetic of the FP32 sector and resident kernels:
+                               0 = library choice (32), 32 = FP32 products, 64 = FP64 products
+                               of the FP32 data (outputs rounded to FP32).  The node-centric
+                               and MFMA kernels compute in their storage precision. */
 } h8_grid;
 
 typedef struct h8_pcg_stats {

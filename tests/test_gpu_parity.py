@@ -57,15 +57,24 @@ def test_spmv_f64_f32_match_oracle_csr(h8, gpu, dims):
     K = o.assemble_csr(nodes, elems, o.d_solid3d(1.0, 0.3), scale=s)
     u = rng.normal(size=K.shape[0])
     ref = K @ u
+    # the FP32 product of the FP32-rounded input: FP64 element products leave
+    # only the rounding of the outputs and of the 8-term node sums
+    ref_u32 = K @ u.astype(np.float32).astype(np.float64)
     for variant in (1, 2, 3, 4, 5, 6):  # node, element MFMA, persistent, two per node, sectors, narrow sectors
         op.set_variant(variant)
         y = op.matvec(torch.as_tensor(u, device=gpu)).cpu().numpy()
         e64 = np.abs(y - ref).max() / np.abs(ref).max()
-        y32 = op.matvec32(torch.as_tensor(u, device=gpu)).cpu().numpy()
-        e32 = np.abs(y32 - ref).max() / np.abs(ref).max()
-        say("variant", variant, "err64", e64, "err32", e32)
+        for arith in ((32, 64) if variant in (5, 6) else (0,)):
+            op.set_arith(arith)
+            y32 = op.matvec32(torch.as_tensor(u, device=gpu)).cpu().numpy()
+            e32 = np.abs(y32 - ref).max() / np.abs(ref).max()
+            e32r = np.abs(y32 - ref_u32).max() / np.abs(ref).max()
+            say("variant", variant, "arith", arith, "err64", e64, "err32", e32, "vs K u32", e32r)
+            assert e32 <= 2e-5
+            if arith == 64:
+                assert e32r <= 1e-6
+        op.set_arith(0)
         assert e64 <= 1e-12
-        assert e32 <= 2e-5
 
 
 def _fesys(h8, nx, ny, nz, E=210e9, E_elem=None, load="facet"):
@@ -158,6 +167,36 @@ def test_resident_solver_golden_and_streaming_agreement(h8, gpu, golden):
             assert sr["launches"] == 1
         else:
             assert sr["true_rel_residual"] < 1e-9
+
+
+@pytest.mark.parametrize("variant", [5, 7])
+def test_refined_fp32_and_fp64_element_products(h8, gpu, variant):
+    """The refined solve with FP32 element products (arith 32) and with FP64
+    products of the FP32 data (arith 64, the default): both meet the FP64 true
+    residual and agree with the direct solve."""
+    say("refined arith", variant)
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 24, 6, 6
+    E = 10 ** rng.uniform(9, 12, nx * ny * nz)
+    fs = _fesys(h8, nx, ny, nz, E_elem=E)
+    fs.operator.set_variant(variant)
+    c = o.cantilever(nx, ny, nz, E_elem=E)
+    Ud = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
+    free, _ = o.partition(c["K"].shape[0], c["fixed_dofs"])
+    its = {}
+    for arith in (32, 64):
+        fs.operator.set_arith(arith)
+        U = np.asarray(fs.solve_static(tol=1e-9, precision="mixed"))
+        st = fs.last_solve_stats
+        true_res = np.linalg.norm((c["F"] - c["K"] @ U)[free]) / np.linalg.norm(c["F"][free])
+        err = np.linalg.norm(U - Ud) / np.linalg.norm(Ud)
+        its[arith] = (st["inner_iterations"], st["outer"])
+        say("arith", arith, "fp32 iters", st["inner_iterations"], "replacements", st["outer"], "true_res", true_res,
+            "err", err)
+        assert true_res < 1e-9
+        assert err < 1e-6
+    fs.operator.set_arith(0)
+    say("iterations by arithmetic", its)
 
 
 def test_nodal_and_heterogeneous_golden(h8, gpu, golden):
