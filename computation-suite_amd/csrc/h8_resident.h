@@ -474,19 +474,24 @@ __global__ void __launch_bounds__(NT) k_resident(ResArgs<T> A) {
             double tot[2];
             // no release: the z words carry their own EMPTY test
             const bool good = grid_sum<2, false>(rz, ring2 + e * RES_ENTRY, ring2 + ed * RES_ENTRY, sRed, tot);
-            // every thread collects the z of its shell nodes into the z layout
+            // waves 1.. collect the z of the shell nodes into the z layout while
+            // wave 0 runs the grid sum (a collection share of wave 0 would start
+            // only after its polls and lengthen the phase)
+            static_assert(NT >= 128, "the z collection needs waves besides wave 0");
+            constexpr int NC = NT - 64;
+            const int ct = int(threadIdx.x) - 64;
             const EU* zr = reinterpret_cast<const EU*>(A.zr[e]);
             int zgood = 1;
-            for (int base = 0; base < nh; base += PF * NT) {
+            for (int base = 0; ct >= 0 && base < nh; base += PF * NC) {
                 EU zb[PF][3];
                 int ho[PF];
                 long long gd[PF];
                 bool pend[PF];
 #pragma unroll
                 for (int u = 0; u < PF; ++u) {
-                    const int t = base + u * NT + int(threadIdx.x);
+                    const int t = base + u * NC + ct;
                     int hj_, hi_, hk_;
-            dH(t, hj_, hi_, hk_);
+                    dH(t, hj_, hi_, hk_);
                     const int j = j0 - 1 + hj_, i = i0 - 1 + hi_, k = k0 - 1 + hk_;
                     const bool own = hj_ >= 1 && hj_ <= bj && hi_ >= 1 && hi_ <= bi && hk_ >= 1 && hk_ <= bk;
                     pend[u] = t < nh && !own && unsigned(j) < unsigned(g.NJ) && unsigned(i) < unsigned(g.NI) &&
