@@ -97,16 +97,17 @@ RESIDENT_REPS = 100   # iterations per launch of the resident solver's timing ru
 
 def profiled_traffic(dims, kernel):
     """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (profiles/pcg_kernels_<dims>_traffic.md), or None."""
-    path = os.path.join(ROOT, "profiles", "pcg_kernels_%dx%dx%d_traffic.md" % dims)
-    try:
-        for line in open(path):
-            if line.startswith("traffic-json:"):
-                d = json.loads(line.split(":", 1)[1])
-                if d["kernel"] == kernel:
-                    return d["bytes_per_launch"], os.path.relpath(path, ROOT)
-    except OSError:
-        pass
+    FETCH_SIZE/WRITE_SIZE summaries (profiles/*_<dims>_traffic.md), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_%dx%dx%d_traffic.md" % dims))):
+        try:
+            for line in open(path):
+                if line.startswith("traffic-json:"):
+                    d = json.loads(line.split(":", 1)[1])
+                    if d["kernel"] == kernel:
+                        return d["bytes_per_launch"], os.path.relpath(path, ROOT)
+        except OSError:
+            pass
     return None, None
 
 
