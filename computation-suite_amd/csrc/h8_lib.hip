@@ -21,6 +21,7 @@
 #include "h8_batched.h"
 #include "h8_sector.h"
 #include "h8_resident.h"
+#include "h8_recovery.h"
 
 using namespace h8;
 
@@ -346,6 +347,44 @@ extern "C" int h8_elem_bilinear_f64(const h8_grid* g, const double* K0, const do
     hipLaunchKernelGGL(k_elem_bilinear, dim3(stream_blocks(m)), dim3(STREAM_THREADS), 0,
                        (hipStream_t)stream, d, K0, u, v, w, out);
     LAUNCHCHK();
+    return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stress / strain recovery
+// ---------------------------------------------------------------------------
+extern "C" int h8_recovery_table_f64(double hx, double hy, double hz, const double* D, double* tab,
+                                     void* stream) {
+    if (!(hx > 0.0) || !(hy > 0.0) || !(hz > 0.0))
+        return fail(H8_ERR_ARG, "recovery table: spacings must be > 0 (got %g, %g, %g)", hx, hy, hz);
+    if (!tab) return fail(H8_ERR_ARG, "NULL pointer");
+    static_assert(H8_RECOVERY_WORDS == REC_WORDS, "table size");
+    hipLaunchKernelGGL(k_recovery_table, dim3((REC_WORDS + 255) / 256), dim3(256), 0, (hipStream_t)stream, hx, hy,
+                       hz, D, tab);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_recover_f64(const h8_grid* g, const double* tab, const double* s_e, const double* u,
+                              double* nodal, double* vm_nodal, double* elem, double* vm_elem, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!tab || !u) return fail(H8_ERR_ARG, "NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (nodal || vm_nodal) {
+        constexpr int TJ = 8, TI = 8, TK = 4;
+        set_tiles(d, TJ, TI, TK);
+        hipLaunchKernelGGL((k_recover<TJ, TI, TK>), dim3(d.ntiles), dim3(TJ * TI * TK), 0, s, d, tab, s_e, u, nodal,
+                           vm_nodal);
+        LAUNCHCHK();
+    }
+    if (elem || vm_elem) {
+        const long long m = (long long)d.nx * d.ny * d.nz;
+        hipLaunchKernelGGL(k_recover_elem, dim3(unsigned((m + 255) / 256)), dim3(256), 0, s, d, tab, s_e, u, elem,
+                           vm_elem);
+        LAUNCHCHK();
+    }
     return H8_OK;
 }
 

@@ -79,7 +79,10 @@ SIGNATURES = {
                                      C.c_int32, _P, _P, _P, _D, _I64, _P, C.c_size_t, _P, _P, _P, _P]),
     "h8_batched_last_ms": (C.c_double, []),
     "h8_batched_last_launches": (C.c_int64, []),
+    "h8_recovery_table_f64": (C.c_int, [_D, _D, _D, _P, _P, _P]),
+    "h8_recover_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
+RECOVERY_WORDS = 252
 
 _lib = None
 _lock = threading.Lock()

@@ -1,6 +1,7 @@
 """Kernel micro-benchmark for profiling the two PCG iteration kernels.
 
     python -m h8pcg.kbench --dims 256,64,64 --reps 50 [--precision 32,64] [--variant V] [--layers L] [--copy-mib 256]
+    python -m h8pcg.kbench --dims 256,64,64 --reps 50 --recovery
 
 Builds the clamped box operator (per-element scale 10**U(9,12), seed 5), then
 times the fused direction + K*p kernel and the update kernel of one PCG
@@ -56,6 +57,43 @@ def bytes_per_launch(op, precision):
     return {"dir": 5 * w * op.n_dof + w * op.m, "update": 7 * w * op.n_dof}
 
 
+def recovery_bytes(nx, ny, nz):
+    """Algorithmic HBM bytes of one recovery: nodal kernel reads u and the element
+    scales once and writes 6 stresses + von Mises per node; element kernel reads
+    u and the scales once and writes 6 stresses + von Mises per element."""
+    n_nodes, m = (nx + 1) * (ny + 1) * (nz + 1), nx * ny * nz
+    return {"nodal": 24 * n_nodes + 8 * m + 56 * n_nodes, "elements": 24 * n_nodes + 8 * m + 56 * m}
+
+
+def time_recovery(nx, ny, nz, reps, device="cuda:0"):
+    """Average microseconds of h8_recover_f64 (table + kernel) for the nodal and
+    the element outputs, timed with events on the launching (current) stream."""
+    import torch
+    from .operator import recover_grid
+    from .material import D_solid3d, Material
+    rng = np.random.default_rng(5)
+    u = torch.as_tensor(rng.normal(size=3 * (nx + 1) * (ny + 1) * (nz + 1)), device=device)
+    s = 10 ** rng.uniform(9, 12, nx * ny * nz)
+    D = D_solid3d(Material(1.0, 0.3))
+    out = {}
+    for what in ("nodal", "elements"):
+        kw = dict(nodes=what == "nodal", elements=what == "elements", device=device)
+        for _ in range(3):
+            recover_grid(nx, ny, nz, u, (1.0, 1.0, 1.0), D, s, **kw)
+        # resident scale and D: the timed calls upload nothing
+        st = torch.as_tensor(s, device=device)
+        Dt = torch.as_tensor(D, device=device)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            recover_grid(nx, ny, nz, u, (1.0, 1.0, 1.0), Dt, st, **kw)
+        ev[1].record()
+        torch.cuda.synchronize()
+        out[what + "_us"] = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    return out
+
+
 def main(argv=None):
     import torch
     ap = argparse.ArgumentParser()
@@ -66,8 +104,17 @@ def main(argv=None):
     ap.add_argument("--layers", type=int, default=0, help="node layers per block of the sector kernel (0 = auto)")
     ap.add_argument("--arith", type=int, default=0, help="FP32 element-product arithmetic (0 = library, 32, 64)")
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--recovery", action="store_true", help="time the stress recovery kernels instead")
     a = ap.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.dims.split(","))
+    if a.recovery:
+        r = time_recovery(nx, ny, nz, a.reps)
+        b = recovery_bytes(nx, ny, nz)
+        r.update({"dims": [nx, ny, nz], "reps": a.reps, "nodal_bytes": b["nodal"], "elements_bytes": b["elements"],
+                  "nodal_GBps": b["nodal"] / r["nodal_us"] * 1e-3,
+                  "elements_GBps": b["elements"] / r["elements_us"] * 1e-3})
+        print(json.dumps(r), flush=True)
+        return
     op = build_operator(nx, ny, nz)
     op.grid.variant = a.variant
     op.grid.layers = a.layers

@@ -256,6 +256,10 @@ class GridOperator:
                 g.variant = int(variant)
                 g.layers = int(layers)
 
+    def recover(self, u, spacing, D=None, scale=None, nodes=True, elements=True):
+        """Stress / strain recovery on this grid (see recover_grid)."""
+        return recover_grid(self.nx, self.ny, self.nz, u, spacing, D, scale, nodes, elements, self.device)
+
     def set_arith(self, arith):
         """Element-product arithmetic of the FP32 sector and resident kernels:
         0 (library choice: FP32), 32 (FP32 products) or 64 (FP64 products of
@@ -284,3 +288,46 @@ class GridOperator:
         self.s_e32 = self.s_e.float().contiguous()
         if getattr(self, "free_mask", None) is not None and self.dinv is not None:
             self.set_free_mask(self.free_mask.cpu().numpy())
+
+
+def recover_grid(nx, ny, nz, u, spacing, D=None, scale=None, nodes=True, elements=True, device="cuda"):
+    """Stress (D: 6x6) or strain (D None) recovery on the device for the box grid
+    nx x ny x nz with brick spacing (hx, hy, hz): the lumped projection of the
+    2x2x2 Gauss-point values to the nodes and the element volume averages, with
+    their von Mises values (fe/recovery.py:234-420).  scale: per-element multiple
+    of D (None = 1).  Returns device tensors (nodal (N,6), vm_nodal (N,),
+    elem (m,6), vm_elem (m,)), None where not asked for."""
+    import torch
+    hx, hy, hz = (float(v) for v in spacing)
+    grid = H8Grid(int(nx), int(ny), int(nz), 0, 0)
+    n_nodes, m = (nx + 1) * (ny + 1) * (nz + 1), nx * ny * nz
+    f64 = dict(dtype=torch.float64, device=device)
+    uu = (u.to(**f64) if isinstance(u, torch.Tensor) else torch.as_tensor(np.asarray(u), **f64)).reshape(-1)
+    uu = uu.contiguous()
+    if uu.numel() != 3 * n_nodes:
+        raise ValueError(f"vector has {uu.numel()} entries, expected {3 * n_nodes}")
+    Dt = None
+    if D is not None:
+        Dt = (D.to(**f64) if isinstance(D, torch.Tensor) else torch.as_tensor(np.asarray(D, dtype=float), **f64))
+        if tuple(Dt.shape) != (6, 6):
+            raise ValueError(f"D must be 6x6 for Hex8Solid3D, got {tuple(Dt.shape)}")
+        Dt = Dt.contiguous()
+    st = None
+    if scale is not None:
+        st = (scale.to(**f64) if isinstance(scale, torch.Tensor)
+              else torch.as_tensor(np.asarray(scale, dtype=float), **f64)).reshape(-1).contiguous()
+        if st.numel() != m:
+            raise ValueError(f"scale has {st.numel()} entries, the grid has {m} elements")
+    tab = torch.empty(_lib.RECOVERY_WORDS, **f64)
+    nodal = torch.empty((n_nodes, 6), **f64) if nodes else None
+    vm_n = torch.empty(n_nodes, **f64) if nodes else None
+    elem = torch.empty((m, 6), **f64) if elements else None
+    vm_e = torch.empty(m, **f64) if elements else None
+    with torch.cuda.device(device):
+        stream = _lib.stream_handle(device)
+        check(_lib.lib().h8_recovery_table_f64(hx, hy, hz, ptr(Dt) or None, ptr(tab), stream),
+              "h8_recovery_table_f64")
+        check(_lib.lib().h8_recover_f64(C.byref(grid), ptr(tab), ptr(st) or None, ptr(uu), ptr(nodal) or None,
+                                        ptr(vm_n) or None, ptr(elem) or None, ptr(vm_e) or None, stream),
+              "h8_recover_f64")
+    return nodal, vm_n, elem, vm_e

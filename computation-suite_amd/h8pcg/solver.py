@@ -359,6 +359,83 @@ class FESystem:
         return FEField(data, n_nodes=len(self.mesh.nodes), dofs_per_node=3, dof_names=self.dof_names,
                        mesh=self.mesh, label=label)
 
+    # ---- derived results (fe/solver.py:413-430 -> fe/recovery.py) ---------------------------
+    _STRESS_NAMES = ("sxx", "syy", "szz", "sxy", "syz", "sxz")
+    _STRAIN_NAMES = ("exx", "eyy", "ezz", "gxy", "gyz", "gxz")
+
+    def _recover(self, U, D, kind, at, who):
+        """Device recovery (h8_recover_f64): (nodal (N,6), vm_nodal, elem (m,6),
+        vm_elem) as numpy; D of the last assemble_stiffness unless given
+        (fe/recovery.py:36-42); per-element coefficients become a scale of one D."""
+        if at not in ("nodes", "elements"):
+            raise ValueError("at must be 'nodes' or 'elements'.")
+        U = np.asarray(U)
+        if U.ndim != 1 or U.shape[0] != self.n_dof:
+            raise ValueError(f"expected a displacement vector of length {self.n_dof}, got shape {U.shape}.")
+        if np.iscomplexobj(U):
+            raise NotImplementedError("recovery of complex (harmonic) fields: pass U.real or U.imag.")
+        Dm = scale = None
+        if kind == "stress":
+            if D is None:
+                args = getattr(self, "_stiffness_args", None)
+                if args is None:
+                    raise RuntimeError("stress/strain recovery needs the constitutive matrix: call "
+                                       "assemble_stiffness() first (it is remembered) or pass D=.")
+                D = args[0]
+            if has_coefficient(D):
+                Dm, scale = factor_scales(D)
+                if len(scale) != len(self.mesh.elements):
+                    raise ValueError(f"Coefficient.by_element: {len(scale)} values but the mesh has "
+                                     f"{len(self.mesh.elements)} elements.")
+            else:
+                Dm = np.asarray(D, dtype=float)
+                if Dm.shape != (6, 6):
+                    raise ValueError(f"{who}: D must be 6x6 for Hex8Solid3D, got {Dm.shape}")
+        st = self._structure if self._structure is not None else structured_grid(self.mesh)
+        if st is None:
+            raise NotImplementedError(f"{who}(backend='hip') needs a structured box mesh")
+        self._structure = st
+        nx, ny, nz, coords0 = st
+        from .operator import recover_grid
+        out = recover_grid(nx, ny, nz, U, coords0[6] - coords0[0], Dm, scale, nodes=(at == "nodes"),
+                           elements=(at == "elements"), device=self.device)
+        return tuple(None if t is None else t.cpu().numpy() for t in out)
+
+    def stress(self, U=None, D=None, at="nodes", vectorized=True, backend="numpy", device="cpu"):
+        """Stress components [sxx, syy, szz, sxy, syz, sxz]: an FEField of nodal
+        values (at="nodes") or an (m, 6) array of element averages
+        (fe/solver.py:413-418, fe/recovery.py:378-386); computed on the GPU
+        whatever vectorized/backend/device say."""
+        nodal, _, elem, _ = self._recover(self.solve_static() if U is None else U, D, "stress", at, "stress")
+        if at == "elements":
+            return elem
+        return FEField(nodal.reshape(-1), n_nodes=len(self.mesh.nodes), dofs_per_node=6,
+                       dof_names=self._STRESS_NAMES, mesh=self.mesh, label="stress")
+
+    def strain(self, U=None, at="nodes", vectorized=True, backend="numpy", device="cpu"):
+        """Strain components [exx, eyy, ezz, gxy, gyz, gxz] (engineering shear),
+        same layout as stress (fe/solver.py:420-423, fe/recovery.py:389-397)."""
+        nodal, _, elem, _ = self._recover(self.solve_static() if U is None else U, None, "strain", at, "strain")
+        if at == "elements":
+            return elem
+        return FEField(nodal.reshape(-1), n_nodes=len(self.mesh.nodes), dofs_per_node=6,
+                       dof_names=self._STRAIN_NAMES, mesh=self.mesh, label="strain")
+
+    def von_mises(self, U=None, D=None, plane="stress", nu=None, at="nodes", vectorized=True, backend="numpy",
+                  device="cpu"):
+        """Von Mises stress: a scalar FEField at the nodes or an (m,) array of
+        element values (fe/solver.py:425-430, fe/recovery.py:400-419; plane and
+        nu matter only for 2-D elements and are validated as there)."""
+        if plane not in ("stress", "strain"):
+            raise ValueError("plane must be 'stress' or 'strain'.")
+        if plane == "strain" and nu is None:
+            raise ValueError("von_mises(plane='strain') needs the Poisson ratio nu= to form sigma_zz.")
+        _, vm_n, _, vm_e = self._recover(self.solve_static() if U is None else U, D, "stress", at, "von_mises")
+        if at == "elements":
+            return vm_e
+        return FEField(vm_n, n_nodes=len(self.mesh.nodes), dofs_per_node=1, dof_names=("von_mises",),
+                       mesh=self.mesh, label="von Mises stress")
+
     def reactions(self, U=None):
         """Support reactions R = K U - F at the constrained DOFs, zero elsewhere
         (fe/recovery.py:422-448); K U is the device matrix-free product."""

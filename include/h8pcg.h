@@ -198,6 +198,27 @@ int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64_t total_no
 double h8_batched_last_ms(void);
 int64_t h8_batched_last_launches(void);
 
+/* ---- stress / strain recovery ----------------------------------------- */
+#define H8_RECOVERY_WORDS 252     /* 216 projected shape-function gradients + D (36) */
+
+/* The recovery table of the brick element with spacing (hx, hy, hz):
+ * A[c][d][b] = sum_p N_c(p) dN_b/dx_d(p) over the 2x2x2 Gauss points for the
+ * 8 corners c, the volume average (1/8) sum_p dN_b/dx_d(p) as c = 8, then
+ * the 6x6 D (device pointer; NULL = identity, i.e. strain).  Layout helper of
+ * h8_recover_f64 (fe/recovery.py:193-231 evaluates B(p) per Gauss point). */
+int h8_recovery_table_f64(double hx, double hy, double hz, const double* D, double* tab, void* stream);
+
+/* Stress sigma = s_e D B u (strain with the identity table) recovered as the
+ * reference does (fe/recovery.py:234-330, _block_loop / _block_batched /
+ * _recover): Gauss-point values projected to the corners with the linear
+ * shape functions, sum N_a val |J| w / sum N_a |J| w over the elements sharing
+ * a node -> nodal (n_nodes x 6, Voigt xx, yy, zz, xy, yz, xz); volume averages
+ * -> elem (m x 6); their von Mises values (fe/recovery.py:364-375) -> vm_nodal
+ * (n_nodes), vm_elem (m).  Any output may be NULL; s_e NULL = 1 for every
+ * element.  Replaces stress / strain / von_mises (fe/recovery.py:378-420). */
+int h8_recover_f64(const h8_grid* g, const double* tab, const double* s_e, const double* u,
+                   double* nodal, double* vm_nodal, double* elem, double* vm_elem, void* stream);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Average device time (ms) of the two kernels of one PCG iteration, each
  * launched `reps` times alone between HIP events on `stream`: avg_ms[0] = the

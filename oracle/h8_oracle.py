@@ -32,6 +32,11 @@ PINNING (function -> independent checks):
   simp_* / sensitivities: simp_scale_grad vs central FD; closed form vs adjoint
                           solve; both vs cancellation-free FD of the forward solve
   solve_static_batched  : each column equals an independent solve
+  recover / von_mises   : the reference's own recovery outputs on the committed
+                          golden fixture (per-element D and one D); a uniform
+                          strain field gives D eps at every node and element, a
+                          rigid motion gives zero; von Mises of uniaxial stress
+                          is |s|, of pure shear sqrt(3)|t|
 """
 from __future__ import annotations
 
@@ -236,6 +241,53 @@ def add_nodal_force(F, node_ids, dof_index, total):
 def element_dof_table(elements):
     """Global DOFs 3*node + c per element, local order 3a+c (fe/solver.py:275-277)."""
     return (3 * elements[:, :, None] + np.arange(3)[None, None, :]).reshape(len(elements), 24)
+
+
+# ---------------------------------------------------------------------------
+# Recovery: fe/recovery.py:234-255 (per-element loop), 258-330 (_recover),
+# 364-375 (_mises)
+# ---------------------------------------------------------------------------
+def recover(nodes, elements, U, D=None, scale=None):
+    """(nodal (N,6), element (m,6)) stress sigma = D_e B u (or strain with D=None)
+    at the 2x2x2 Gauss points, projected to the corners with the linear shape
+    functions: nodal_a = sum N_a val |detJ| w / sum N_a |detJ| w over the elements
+    sharing node a; element = volume average (fe/recovery.py:234-255,258-330).
+    D_e = scale[e] * D (per-element material as a multiple of one D)."""
+    nodes = np.asarray(nodes, dtype=float)
+    elements = np.asarray(elements)
+    m = len(elements)
+    num = np.zeros((len(nodes), 6))
+    den = np.zeros(len(nodes))
+    elem = np.zeros((m, 6))
+    pts, wts = gauss_points(2)
+    for i, conn in enumerate(elements):
+        coords = nodes[conn]
+        ue = U[(3 * conn[:, None] + np.arange(3)[None, :]).ravel()]
+        De = None if D is None else (D if scale is None else scale[i] * D)
+        acc, vol = np.zeros(6), 0.0
+        for p, w in zip(pts, wts):
+            B, detJ = hex8_B(p, coords)
+            N, _ = hex8_shape_and_derivs(*p)
+            val = B @ ue
+            if De is not None:
+                val = De @ val
+            wt = abs(detJ) * w
+            num[conn] += wt * N[:, None] * val[None, :]
+            den[conn] += wt * N
+            acc += wt * val
+            vol += wt
+        elem[i] = acc / vol
+    nodal = np.divide(num, den[:, None], out=np.zeros_like(num), where=den[:, None] > 0)
+    return nodal, elem
+
+
+def von_mises(s):
+    """Von Mises stress of (..., 6) Voigt stresses [xx, yy, zz, xy, yz, xz]
+    (fe/recovery.py:364-375)."""
+    s = np.asarray(s)
+    sxx, syy, szz, sxy, syz, sxz = (s[..., j] for j in range(6))
+    return np.sqrt(0.5 * ((sxx - syy) ** 2 + (syy - szz) ** 2 + (szz - sxx) ** 2)
+                   + 3.0 * (sxy ** 2 + syz ** 2 + sxz ** 2))
 
 
 def assemble_csr(nodes, elements, D, scale=None, ke=None):
