@@ -22,6 +22,7 @@
 #include "h8_sector.h"
 #include "h8_resident.h"
 #include "h8_recovery.h"
+#include "h8_topopt.h"
 
 using namespace h8;
 
@@ -385,6 +386,29 @@ extern "C" int h8_recover_f64(const h8_grid* g, const double* tab, const double*
                            vm_elem);
         LAUNCHCHK();
     }
+    return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SIMP loop: sensitivity filter, OC update
+// ---------------------------------------------------------------------------
+extern "C" int h8_sens_filter_f64(const h8_grid* g, double hx, double hy, double hz, double radius,
+                                  const double* rho, const double* dc, double eps, double* out, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!rho || !dc || !out) return fail(H8_ERR_ARG, "NULL pointer");
+    if (!(hx > 0.0) || !(hy > 0.0) || !(hz > 0.0)) return fail(H8_ERR_ARG, "spacings must be > 0");
+    if (!(radius > 0.0) || !std::isfinite(radius)) return fail(H8_ERR_ARG, "filter radius must be > 0 and finite");
+    const double rmax = std::max(std::max(radius / hx, radius / hy), radius / hz);
+    if (rmax > 64.0)
+        return fail(H8_ERR_ARG, "filter radius %g spans more than 64 elements (the stencil is evaluated per element)",
+                    radius);
+    const int Ri = int(std::ceil(radius / hx)), Rj = int(std::ceil(radius / hy)), Rk = int(std::ceil(radius / hz));
+    const long long m = (long long)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_sens_filter, dim3(unsigned((m + 255) / 256)), dim3(256), 0, (hipStream_t)stream, d, hx, hy,
+                       hz, radius, Ri, Rj, Rk, rho, dc, eps, out);
+    LAUNCHCHK();
     return H8_OK;
 }
 
@@ -766,6 +790,60 @@ static void fill_stats(h8_pcg_stats* stats, const PcgState& h, float ms, long lo
     stats->elapsed_ms = ms;
     stats->inner_iterations = 0;
     stats->launches = launches;
+}
+
+// OC update (the SIMP loop's density step)
+static size_t oc_partials(long long m) { return size_t(stream_blocks(m)); }
+
+extern "C" size_t h8_oc_workspace_bytes(int64_t m) {
+    if (m < 1) return 0;
+    return align_up(sizeof(OcState)) + align_up(sizeof(double) * oc_partials(m)) + align_up(sizeof(unsigned long long));
+}
+
+extern "C" int h8_oc_update_f64(const double* rho, const double* dc, int64_t m, double volume_fraction, double move,
+                                double rho_min_bound, double rho_max_bound, double bisection_tol,
+                                int32_t max_bisection_iter, double* rho_new, double* change, void* ws,
+                                size_t ws_bytes, void* stream) {
+    if (m < 1) return fail(H8_ERR_ARG, "m must be >= 1");
+    if (!rho || !dc || !rho_new || !ws) return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < h8_oc_workspace_bytes(m)) return fail(H8_ERR_WORKSPACE, "workspace too small");
+    if (max_bisection_iter < 1) return fail(H8_ERR_ARG, "max_bisection_iter must be >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    OcState* st = (OcState*)carve(p, sizeof(OcState));
+    double* partials = (double*)carve(p, sizeof(double) * oc_partials(m));
+    unsigned long long* cmax = (unsigned long long*)carve(p, sizeof(unsigned long long));
+    OcState h{};
+    h.l1 = 1e-12;
+    h.l2 = 1.0;
+    HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
+    OcParams P{volume_fraction, move, rho_min_bound, rho_max_bound, bisection_tol, max_bisection_iter};
+    const int nb = stream_blocks(m);
+    // every pass either widens the bracket (at most 60) or bisects it (at most
+    // max_bisection_iter): launch in chunks, poll the phase between chunks
+    const long long max_pass = OC_MAX_EXPAND + (long long)max_bisection_iter;
+    for (long long done = 0; done < max_pass;) {
+        const long long chunk = std::min<long long>(16, max_pass - done);
+        for (long long i = 0; i < chunk; ++i)
+            hipLaunchKernelGGL(k_oc_pass, dim3(nb), dim3(STREAM_THREADS), 0, s, (long long)m, rho, dc, P, st, partials,
+                               rho_new);
+        LAUNCHCHK();
+        done += chunk;
+        HIPCHK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (h.phase == 2) break;
+    }
+    if (h.phase != 2) return fail(H8_ERR_HIP, "OC update: bisection did not finish in %lld passes", max_pass);
+    if (change) {
+        unsigned long long bits = 0;
+        HIPCHK(hipMemcpyAsync(cmax, &bits, sizeof bits, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_absdiff_max, dim3(nb), dim3(STREAM_THREADS), 0, s, (long long)m, rho_new, rho, cmax);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(&bits, cmax, sizeof bits, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *change = __builtin_bit_cast(double, bits);
+    }
+    return H8_OK;
 }
 
 // ---------------------------------------------------------------------------

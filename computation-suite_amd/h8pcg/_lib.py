@@ -81,6 +81,10 @@ SIGNATURES = {
     "h8_batched_last_launches": (C.c_int64, []),
     "h8_recovery_table_f64": (C.c_int, [_D, _D, _D, _P, _P, _P]),
     "h8_recover_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "h8_sens_filter_f64": (C.c_int, [_GRID, _D, _D, _D, _D, _P, _P, _D, _P, _P]),
+    "h8_oc_workspace_bytes": (C.c_size_t, [_I64]),
+    "h8_oc_update_f64": (C.c_int, [_P, _P, _I64, _D, _D, _D, _D, _D, C.c_int32, _P, C.POINTER(C.c_double), _P,
+                                   C.c_size_t, _P]),
 }
 RECOVERY_WORDS = 252
 

@@ -41,7 +41,7 @@ class SIMPProblem:
     """Clamped structured H8 body under a fixed load, stiffness scaled per element."""
 
     def __init__(self, nx, ny, nz, F, fixed_dofs, E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0), p=3.0, rho_min=1e-9,
-                 device="cuda", tol=1e-10, precision="fp64", maxiter=None, store_forward=True):
+                 device="cuda", tol=1e-10, precision="fp64", maxiter=None, store_forward=True, D=None):
         torch = require_gpu()
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.m = self.nx * self.ny * self.nz
@@ -50,7 +50,8 @@ class SIMPProblem:
         self.store_forward = store_forward
         X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
                       [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float) * np.asarray(h, dtype=float)
-        K0 = Hex8Solid3D().stiffness_batched(X[None], D_solid3d(Material(E=E0, nu=nu)), device=device)[0]
+        D0 = D_solid3d(Material(E=E0, nu=nu)) if D is None else np.asarray(D, dtype=float)
+        K0 = Hex8Solid3D().stiffness_batched(X[None], D0, device=device)[0]
         self.op = GridOperator(nx, ny, nz, K0, np.ones(self.m), device=device)
         n = self.op.n_dof
         mask = np.ones(n, dtype=np.uint8)
@@ -98,6 +99,114 @@ class SIMPProblem:
         """dL/drho_e = -s'(rho_e) lambda_e^T K0 u_e for K lambda = dL/du."""
         _, ds = simp_scale(rho, self.p, self.rho_min, with_grad=True)
         return self.op.element_bilinear(lam, u, -ds)
+
+
+def sensitivity_filter(nx, ny, nz, h, radius, rho, dc, eps=1e-9):
+    """Sigmund's filter of dc over the grid's elements (fe/topopt.py:235-252) on
+    the device: sum_j H_ej rho_j dc_j / max(max(rho_e, eps) sum_j H_ej, eps) with
+    H_ej = max(0, radius - |c_e - c_j|); h = brick spacing."""
+    import torch
+    r = rho.detach().to(torch.float64).contiguous()
+    d = dc.detach().to(device=r.device, dtype=torch.float64).contiguous()
+    if r.numel() != nx * ny * nz or d.numel() != r.numel():
+        raise ValueError(f"rho and dc need {nx * ny * nz} entries")
+    out = torch.empty_like(r)
+    g = _lib.H8Grid(int(nx), int(ny), int(nz), 0, 0)
+    with torch.cuda.device(r.device):
+        check(_lib.lib().h8_sens_filter_f64(__import__("ctypes").byref(g), float(h[0]), float(h[1]), float(h[2]),
+                                            float(radius), ptr(r), ptr(d), float(eps), ptr(out),
+                                            _lib.stream_handle(r.device)), "h8_sens_filter_f64")
+    return out
+
+
+def oc_update(rho, dc, volume_fraction, move=0.2, rho_min_bound=1e-3, rho_max_bound=1.0, bisection_tol=1e-4,
+              max_bisection_iter=100, return_change=False):
+    """Optimality-criteria density update (fe/topopt.py:258-296) on the device:
+    the reference's bracket widening and bisection on the volume multiplier,
+    every pass one kernel over the elements.  Returns rho_new (device tensor),
+    and max |rho_new - rho| when return_change."""
+    import ctypes
+    import torch
+    r = rho.detach().to(torch.float64).contiguous()
+    d = dc.detach().to(device=r.device, dtype=torch.float64).contiguous()
+    m = r.numel()
+    if d.numel() != m:
+        raise ValueError("rho and dc must have the same length")
+    out = torch.empty_like(r)
+    ws = torch.empty(int(_lib.lib().h8_oc_workspace_bytes(m)), dtype=torch.uint8, device=r.device)
+    change = ctypes.c_double(0.0)
+    with torch.cuda.device(r.device):
+        check(_lib.lib().h8_oc_update_f64(ptr(r), ptr(d), m, float(volume_fraction), float(move),
+                                          float(rho_min_bound), float(rho_max_bound), float(bisection_tol),
+                                          int(max_bisection_iter), ptr(out), ctypes.byref(change), ptr(ws),
+                                          ws.numel(), _lib.stream_handle(r.device)), "h8_oc_update_f64")
+    return (out, change.value) if return_change else out
+
+
+def topology_optimize_compliance(fesystem, D, F_ext, volume_fraction, p=3.0, rho_min=1e-3, move=0.2, n_iter=50,
+                                 filter_radius=None, sensitivity="closed_form", rho_init=None, tol=1e-3,
+                                 solve_tol=1e-10, precision="fp64", timings=None):
+    """SIMP compliance minimisation on the GPU (fe/topopt.py:302-377): solve ->
+    sensitivity -> (filter) -> OC update, until max |rho_new - rho| < tol.
+    fesystem supplies the structured mesh and the fixed DOFs (its own K is not
+    used), D the rho = 1 material.  Every step runs in libh8pcg.so; the forward
+    solve is the Jacobi-PCG to solve_tol (FP64, or precision="mixed"), warm-started
+    from the previous iteration's u.  sensitivity="adjoint" takes the gradient
+    through the general adjoint (a second solve) instead of the closed form.
+    Returns {"rho", "compliance_history", "u_final"} as the reference does.
+    timings: a dict that receives the device-synchronised seconds spent in each
+    stage ("solve", "sensitivity", "filter", "oc"), summed over the iterations
+    (None: no synchronisation between the stages)."""
+    import time
+    import torch
+    clock = [time.perf_counter()]
+
+    def lap(stage):
+        if timings is not None:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            timings[stage] = timings.get(stage, 0.0) + t - clock[0]
+            clock[0] = t
+    from .mesh import structured_grid
+    if sensitivity not in ("closed_form", "adjoint"):
+        raise ValueError(f"topology_optimize_compliance: unknown sensitivity={sensitivity!r} "
+                         f"-- expected 'closed_form' or 'adjoint'.")
+    st = structured_grid(fesystem.mesh)
+    if st is None:
+        raise NotImplementedError("topology_optimize_compliance(backend='hip') needs a structured box mesh")
+    nx, ny, nz, coords0 = st
+    h = coords0[6] - coords0[0]
+    prob = SIMPProblem(nx, ny, nz, F_ext, fesystem.fixed_dofs_array, h=h, p=p, rho_min=rho_min,
+                       device=fesystem.device, tol=solve_tol, precision=precision, D=D)
+    m = nx * ny * nz
+    rho = (torch.full((m,), float(volume_fraction), dtype=torch.float64, device=prob.device) if rho_init is None
+           else torch.as_tensor(np.asarray(rho_init, dtype=float), device=prob.device))
+    history, u = [], None
+    lap("setup")
+    for _ in range(int(n_iter)):
+        if precision == "fp64" and u is not None:
+            u = prob.solve(rho, x0=u)
+        else:
+            u = prob.solve(rho)
+        history.append(prob.compliance_value(u))
+        lap("solve")
+        if sensitivity == "closed_form":
+            dc = prob.sensitivity_closed_form(rho, u)
+        else:
+            r = rho.clone().requires_grad_(True)
+            with torch.enable_grad():
+                (prob.F * displacement(r, prob)).sum().backward()
+            dc = r.grad.detach()
+        lap("sensitivity")
+        if filter_radius is not None and filter_radius > 0:
+            dc = sensitivity_filter(nx, ny, nz, h, filter_radius, rho, dc)
+            lap("filter")
+        rho_new, change = oc_update(rho, dc, volume_fraction, move=move, rho_min_bound=rho_min, return_change=True)
+        lap("oc")
+        rho = rho_new
+        if change < tol:
+            break
+    return {"rho": rho.cpu().numpy(), "compliance_history": history, "u_final": u.cpu().numpy()}
 
 
 def torch_f64():

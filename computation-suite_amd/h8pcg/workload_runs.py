@@ -3,6 +3,7 @@
     python -m h8pcg.workload_runs simp --dims 96,32,16 --rho 0.3,1.0 --seed 3 --fd 64
     python -m h8pcg.workload_runs simp --dims 256,64,64 --rho 0.2,1.0 --seed 5 --fd 32 --precision mixed
     python -m h8pcg.workload_runs batched --seed 4 --n 256 --tol 1e-8
+    python -m h8pcg.workload_runs simploop --dims 96,32,16 --iters 20 --vf 0.4 --radius 1.5
 
 simp: SIMP cantilever on an nx x ny x nz unit-cube grid, x = 0 face clamped,
 F_z = -1 at the node (nx, ny/2, 0); stiffness scale s = rho_min + (1 - rho_min)
@@ -90,6 +91,41 @@ def simp(a):
     print(json.dumps(out), flush=True)
 
 
+def simploop(a):
+    """The whole SIMP loop (topopt.topology_optimize_compliance): the simp
+    cantilever (x = 0 clamped, F_z = -1 at (nx, ny/2, 0), E0 = 1, nu = 0.3),
+    uniform start at the volume fraction, p = 3, rho_min = 1e-3, move 0.2,
+    Sigmund filter; per-stage device time."""
+    from . import topopt
+    from .elements import Hex8Solid3D
+    from .material import D_solid3d, Material
+    from .mesh import box_mesh
+    from .solver import FESystem
+    nx, ny, nz = (int(v) for v in a.dims.split(","))
+    mesh = box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    fs = FESystem(mesh, Hex8Solid3D(), backend="hip", device="cuda:0")
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    NJ = ny + 1
+    F = np.zeros(fs.n_dof)
+    F[3 * (nx * NJ + ny // 2) + 2] = -1.0
+    D = D_solid3d(Material(E=1.0, nu=0.3))
+    tm = {}
+    _sync()
+    t0 = time.perf_counter()
+    res = topopt.topology_optimize_compliance(fs, D, F, a.vf, p=3.0, rho_min=1e-3, move=0.2, n_iter=a.iters,
+                                              filter_radius=a.radius, tol=0.0, solve_tol=a.tol,
+                                              precision=a.precision, timings=tm)
+    _sync()
+    wall = time.perf_counter() - t0
+    it = len(res["compliance_history"])
+    print(json.dumps({"workload": "simploop", "dims": [nx, ny, nz], "elements": nx * ny * nz, "iterations": it,
+                      "vf": a.vf, "radius": a.radius, "solve_tol": a.tol, "precision": a.precision,
+                      "wall_ms": 1e3 * wall, "stage_ms_per_iteration": {k: 1e3 * v / it for k, v in tm.items()
+                                                                        if k != "setup"},
+                      "compliance_first_last": [res["compliance_history"][0], res["compliance_history"][-1]],
+                      "volume": float(res["rho"].mean())}), flush=True)
+
+
 def batched(a):
     from .batched import solve_many
     from .workloads import batched_cantilevers
@@ -130,8 +166,15 @@ def main(argv=None):
     b.add_argument("--seed", type=int, default=4)
     b.add_argument("--n", type=int, default=256)
     b.add_argument("--tol", type=float, default=1e-8)
+    lp = sub.add_parser("simploop")
+    lp.add_argument("--dims", default="96,32,16")
+    lp.add_argument("--iters", type=int, default=20)
+    lp.add_argument("--vf", type=float, default=0.4)
+    lp.add_argument("--radius", type=float, default=1.5)
+    lp.add_argument("--tol", type=float, default=1e-8)
+    lp.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
     a = ap.parse_args(argv)
-    simp(a) if a.cmd == "simp" else batched(a)
+    {"simp": simp, "batched": batched, "simploop": simploop}[a.cmd](a)
 
 
 if __name__ == "__main__":

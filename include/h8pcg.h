@@ -219,6 +219,31 @@ int h8_recovery_table_f64(double hx, double hy, double hz, const double* D, doub
 int h8_recover_f64(const h8_grid* g, const double* tab, const double* s_e, const double* u,
                    double* nodal, double* vm_nodal, double* elem, double* vm_elem, void* stream);
 
+/* ---- SIMP loop: sensitivity filter, optimality-criteria update ---------- */
+/* Sigmund's sensitivity filter on the grid's elements (fe/topopt.py:235-252):
+ * out_e = sum_j H_ej rho_j dc_j / max(max(rho_e, eps) sum_j H_ej, eps) with
+ * H_ej = max(0, radius - |c_e - c_j|) between the element centroids (brick
+ * spacing hx, hy, hz); the dense m x m H of build_filter_weights is a stencil
+ * here (radius at most 64 element lengths).  rho, dc, out: m values. */
+int h8_sens_filter_f64(const h8_grid* g, double hx, double hy, double hz, double radius,
+                       const double* rho, const double* dc, double eps, double* out, void* stream);
+
+/* Workspace (bytes) of h8_oc_update_f64 for m elements. */
+size_t h8_oc_workspace_bytes(int64_t m);
+
+/* Optimality-criteria density update (fe/topopt.py:258-296): dc clipped to
+ * <= -1e-12; rho_new = clip(clip(rho sqrt(-dc / l), rho - move, rho + move),
+ * rho_min_bound, rho_max_bound) with the volume multiplier l found as the
+ * reference does -- the bracket [1e-12, 1] doubled at its upper end (at most
+ * 60 times) until mean(rho_new) <= volume_fraction, then bisected until
+ * l2 - l1 < bisection_tol max(l2, 1) or max_bisection_iter steps; rho_new is
+ * the last bisection step's.  *change (HOST pointer, may be NULL) receives
+ * max |rho_new - rho|.  Synchronizes `stream` between chunks of passes. */
+int h8_oc_update_f64(const double* rho, const double* dc, int64_t m, double volume_fraction, double move,
+                     double rho_min_bound, double rho_max_bound, double bisection_tol,
+                     int32_t max_bisection_iter, double* rho_new, double* change, void* ws,
+                     size_t ws_bytes, void* stream);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Average device time (ms) of the two kernels of one PCG iteration, each
  * launched `reps` times alone between HIP events on `stream`: avg_ms[0] = the

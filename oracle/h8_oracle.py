@@ -32,6 +32,12 @@ PINNING (function -> independent checks):
   simp_* / sensitivities: simp_scale_grad vs central FD; closed form vs adjoint
                           solve; both vs cancellation-free FD of the forward solve
   solve_static_batched  : each column equals an independent solve
+  filter_weights / sensitivity_filter / oc_update / topology_optimize :
+                          the reference's own outputs on the committed golden
+                          fixture (filter, OC update, a 6-iteration filtered SIMP
+                          run); a constant field passes the filter unchanged; the
+                          OC result holds the move limit, the bounds and the
+                          volume to the bisection tolerance
   recover / von_mises   : the reference's own recovery outputs on the committed
                           golden fixture (per-element D and one D); a uniform
                           strain field gives D eps at every node and element, a
@@ -530,3 +536,75 @@ def cantilever(nx, ny, nz, Lx=None, Ly=None, Lz=None, E=210e9, nu=0.3, load="fac
     clamp = nodes_on_plane(nodes, 0, 0.0)
     fixed = [3 * int(n) + c for n in clamp for c in range(3)]
     return dict(nodes=nodes, elements=elements, K=K, F=F, fixed_dofs=fixed, D=D)
+
+
+# ---------------------------------------------------------------------------
+# SIMP loop: fe/topopt.py:235-296 (filter, OC update), 302-377 (orchestrator)
+# ---------------------------------------------------------------------------
+def filter_weights(centroids, radius):
+    """Dense H[e, j] = max(0, radius - |c_e - c_j|) (fe/topopt.py:235-243)."""
+    diff = centroids[:, None, :] - centroids[None, :, :]
+    return np.maximum(0.0, radius - np.sqrt(np.sum(diff ** 2, axis=-1)))
+
+
+def sensitivity_filter(H, rho, dc, eps=1e-9):
+    """(H (rho dc)) / max(max(rho, eps) sum_j H, eps) (fe/topopt.py:246-252)."""
+    denom = np.maximum(rho, eps) * H.sum(axis=1)
+    return (H @ (rho * dc)) / np.maximum(denom, eps)
+
+
+def oc_update(rho, dc, volume_fraction, move=0.2, rho_min_bound=1e-3, rho_max_bound=1.0,
+              bisection_tol=1e-4, max_bisection_iter=100):
+    """Optimality-criteria update with the reference's bracket widening (at
+    most 60 doublings of the upper end) and bisection (fe/topopt.py:258-296)."""
+    dc = np.minimum(dc, -1e-12)
+    l1, l2 = 1e-12, 1.0
+
+    def trial(lam):
+        r = np.clip(rho * np.sqrt(-dc / lam), rho - move, rho + move)
+        return np.clip(r, rho_min_bound, rho_max_bound)
+
+    for _ in range(60):
+        if trial(l2).mean() <= volume_fraction:
+            break
+        l2 *= 2.0
+    rho_new = None
+    for _ in range(max_bisection_iter):
+        lmid = 0.5 * (l1 + l2)
+        rho_new = trial(lmid)
+        if rho_new.mean() > volume_fraction:
+            l1 = lmid
+        else:
+            l2 = lmid
+        if (l2 - l1) < bisection_tol * max(l2, 1.0):
+            break
+    return rho_new
+
+
+def topology_optimize(nodes, elements, D, F, fixed_dofs, volume_fraction, p=3.0, rho_min=1e-3, move=0.2,
+                      n_iter=50, filter_radius=None, rho_init=None, tol=1e-3):
+    """SIMP compliance minimisation: solve -> closed-form sensitivity ->
+    (filter) -> OC update, until max |rho_new - rho| < tol
+    (fe/topopt.py:302-377, sensitivity="closed_form"); the forward solve is
+    the direct sparse solve.  Returns (rho, compliance history, u)."""
+    ke = hex8_ke(nodes[elements[0]], D)
+    m = len(elements)
+    ke_unit = np.broadcast_to(ke, (m, 24, 24))
+    rho = np.full(m, volume_fraction) if rho_init is None else np.array(rho_init, dtype=float)
+    H = None
+    if filter_radius is not None and filter_radius > 0:
+        H = filter_weights(nodes[elements].mean(axis=1), filter_radius)
+    hist, u = [], None
+    for _ in range(n_iter):
+        K = assemble_csr(nodes, elements, None, scale=simp_scale(rho, p, rho_min), ke=ke_unit)
+        u = solve_direct(K, F, fixed_dofs)
+        hist.append(compliance(F, u))
+        dc = compliance_sensitivity_closed_form(ke_unit, elements, u, rho, p, rho_min)
+        if H is not None:
+            dc = sensitivity_filter(H, rho, dc)
+        rho_new = oc_update(rho, dc, volume_fraction, move=move, rho_min_bound=rho_min)
+        change = float(np.max(np.abs(rho_new - rho)))
+        rho = rho_new
+        if change < tol:
+            break
+    return rho, hist, u
