@@ -274,7 +274,20 @@ extern "C" int h8_ke_batched_f64(const double* X, const double* D, int64_t d_str
     if (d_stride != 0 && d_stride != 36) return fail(H8_ERR_ARG, "d_stride must be 0 or 36");
     hipStream_t s = (hipStream_t)stream;
     const long long blocks = (m + 3) / 4;
-    hipLaunchKernelGGL(k_ke, dim3((unsigned)blocks), dim3(256), 0, s, (long long)m, X, D,
+    hipLaunchKernelGGL(k_ke<false>, dim3((unsigned)blocks), dim3(256), 0, s, (long long)m, X, D,
+                       (long long)d_stride, ke);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_ke_bbar_batched_f64(const double* X, const double* D, int64_t d_stride, int64_t m,
+                                      double* ke, void* stream) {
+    if (m < 0) return fail(H8_ERR_ARG, "m < 0");
+    if (m == 0) return H8_OK;
+    if (!X || !D || !ke) return fail(H8_ERR_ARG, "NULL pointer");
+    if (d_stride != 0 && d_stride != 36) return fail(H8_ERR_ARG, "d_stride must be 0 or 36");
+    const long long blocks = (m + 3) / 4;
+    hipLaunchKernelGGL(k_ke<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (long long)m, X, D,
                        (long long)d_stride, ke);
     LAUNCHCHK();
     return H8_OK;

@@ -298,11 +298,18 @@ k_elem_bilinear(GridDev g, const double* __restrict__ K0, const double* __restri
 // each take one Gauss point (xi outer, eta middle, zeta inner, weight 1): they
 // form J = dN X, detJ and the physical gradients J^-1 dN into LDS; then every
 // lane owns 9 of the 576 (i,j) entries of sum_gp B^T D B detJ w.
+// BBAR: the B-bar (mean-dilatation) form (fe/elements/solids.py:225-272): at
+// every Gauss point the volumetric part (m m^T / 3) B of B is replaced by its
+// element volume average, i.e. column (a, c) of B gets (gbar_c(a) - g_c(a)) / 3
+// added to its three normal-strain rows, g_c(a) = dN_a/dx_c at the point and
+// gbar_c(a) = sum_q g_c(a)(q) detJ_q / sum_q detJ_q.
+template <bool BBAR>
 __global__ void __launch_bounds__(256)
 k_ke(long long m, const double* __restrict__ X, const double* __restrict__ D, long long d_stride,
      double* __restrict__ ke) {
     __shared__ double sG[4][8][3][8];
     __shared__ double sDet[4][8];
+    __shared__ double sGb[4][3][8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long long e = (long long)blockIdx.x * 4 + w;
     const bool live = e < m;
@@ -347,6 +354,18 @@ k_ke(long long m, const double* __restrict__ X, const double* __restrict__ D, lo
         sDet[w][q] = det;
     }
     __syncthreads();
+    if constexpr (BBAR) {
+        if (live && lane < 24) {
+            const int r = lane / 8, a = lane % 8;
+            double num = 0.0, vol = 0.0;
+            for (int q = 0; q < 8; ++q) {
+                num += sG[w][q][r][a] * sDet[w][q];
+                vol += sDet[w][q];
+            }
+            sGb[w][r][a] = num / vol;
+        }
+        __syncthreads();
+    }
     if (!live) return;
     const double* De = D + e * d_stride;
     double Dm[6][6];
@@ -372,6 +391,15 @@ k_ke(long long m, const double* __restrict__ X, const double* __restrict__ D, lo
             if (d == 0) { Bj[0] = gb0; Bj[3] = gb1; Bj[5] = gb2; }
             else if (d == 1) { Bj[1] = gb1; Bj[3] = gb0; Bj[4] = gb2; }
             else { Bj[2] = gb2; Bj[4] = gb1; Bj[5] = gb0; }
+            if constexpr (BBAR) {
+                const double vi = (sGb[w][c][a] - (c == 0 ? ga0 : (c == 1 ? ga1 : ga2))) / 3.0;
+                const double vj = (sGb[w][d][b] - (d == 0 ? gb0 : (d == 1 ? gb1 : gb2))) / 3.0;
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    Bi[s] += vi;
+                    Bj[s] += vj;
+                }
+            }
             double sum = 0.0;
 #pragma unroll
             for (int s = 0; s < 6; ++s) {

@@ -7,7 +7,7 @@ coefficients.by_element), and every arithmetic step runs in libh8pcg.so
 """
 from .material import Material, D_solid3d
 from .mesh import Mesh, box_mesh, structured_grid
-from .elements import Hex8Solid3D
+from .elements import Hex8Solid3D, Hex8SolidBbar
 from . import coefficients
 from .fields import FEField
 from .facet_loads import list_facets, boundary_faces, face_load_shares
@@ -15,7 +15,7 @@ from .solver import FESystem
 from ._lib import NotSPDError, H8LibraryMissing
 from .batched import solve_static_batched, solve_many
 
-__all__ = ["Material", "D_solid3d", "Mesh", "box_mesh", "structured_grid", "Hex8Solid3D", "coefficients",
+__all__ = ["Material", "D_solid3d", "Mesh", "box_mesh", "structured_grid", "Hex8Solid3D", "Hex8SolidBbar", "coefficients",
            "FEField", "list_facets", "boundary_faces", "face_load_shares", "FESystem", "NotSPDError",
            "H8LibraryMissing", "solve_static_batched", "solve_many"]
 __version__ = "0.1.0"

@@ -62,6 +62,7 @@ SIGNATURES = {
     "h8_device_info": (C.c_int, [C.c_char_p, C.c_int]),
     "h8_sector_table_f64": (C.c_int, [_P, _P]),
     "h8_ke_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
+    "h8_ke_bbar_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
     "h8_diag_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
     "h8_spmv_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),
     "h8_spmv_f32": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),

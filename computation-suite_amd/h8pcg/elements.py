@@ -1,4 +1,5 @@
-"""Hex8Solid3D: the 8-node trilinear solid element (fe/elements/solids.py:130-158).
+"""Hex8Solid3D: the 8-node trilinear solid element (fe/elements/solids.py:130-158),
+and Hex8SolidBbar, its B-bar (mean-dilatation) form (fe/elements/solids.py:225-272).
 
 The element descriptor mirrors the reference class (n_nodes, dofs_per_node,
 dim, gauss_order, DOF names).  ``stiffness`` / ``stiffness_batched`` evaluate
@@ -52,3 +53,22 @@ class Hex8Solid3D:
             raise NotImplementedError("Hex8Solid3D on the HIP backend integrates with the 2x2x2 rule only")
         ke = self.stiffness_batched(np.asarray(elem_coords, dtype=float)[None], D, device=device)
         return ke[0].cpu().numpy() * thickness
+
+
+class Hex8SolidBbar(Hex8Solid3D):
+    """The B-bar Hex8 (fe/elements/solids.py:225-272): at each of the 8 Gauss
+    points the volumetric part of B is replaced by its element volume average,
+    which relieves volumetric locking as nu -> 0.5.  Full integration only.
+    On a structured grid its brick K0 keeps the reflection symmetry, so every
+    matrix-free kernel and solve of the library serves it unchanged."""
+
+    def stiffness_batched(self, elem_coords, D, device="cuda"):
+        from .operator import element_stiffness_batched
+        return element_stiffness_batched(elem_coords, D, device=device, bbar=True)
+
+    def full_stiffness(self, elem_coords, D, thickness=1.0, device="cuda"):
+        return self.stiffness(elem_coords, D, thickness, device=device)
+
+    def reduced_stiffness(self, elem_coords, D, thickness=1.0):
+        raise NotImplementedError("Hex8SolidBbar has no reduced-integration variant: the mean-dilatation "
+                                  "correction is defined at the full 8-point rule.")

@@ -34,9 +34,9 @@ def device_of(device):
     return dev
 
 
-def element_stiffness_batched(coords, D, device="cuda"):
+def element_stiffness_batched(coords, D, device="cuda", bbar=False):
     """ke (m,24,24) for element coordinates (m,8,3) and D (6,6) or (m,6,6),
-    computed by the device kernel; returns a device tensor."""
+    computed by the device kernel (bbar: the B-bar form); returns a device tensor."""
     torch = require_gpu()
     dev = device_of(device)
     X = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64), device=dev).reshape(-1, 8, 3)
@@ -52,8 +52,9 @@ def element_stiffness_batched(coords, D, device="cuda"):
         stride = 36
     ke = torch.empty((m, 24, 24), dtype=torch.float64, device=dev)
     with torch.cuda.device(dev):
-        check(_lib.lib().h8_ke_batched_f64(ptr(X), ptr(Dt.contiguous()), stride, m, ptr(ke),
-                                           _lib.stream_handle(dev)), "h8_ke_batched_f64")
+        fn = "h8_ke_bbar_batched_f64" if bbar else "h8_ke_batched_f64"
+        check(getattr(_lib.lib(), fn)(ptr(X), ptr(Dt.contiguous()), stride, m, ptr(ke), _lib.stream_handle(dev)),
+              fn)
     return ke
 
 

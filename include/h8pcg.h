@@ -104,6 +104,13 @@ int h8_sector_table_f64(double* K0, void* stream);
 int h8_ke_batched_f64(const double* X, const double* D, int64_t d_stride, int64_t m,
                       double* ke, void* stream);
 
+/* The B-bar (mean-dilatation) Hex8 stiffness, same arguments and layout:
+ * B(p) with its volumetric part replaced by the element volume average at
+ * every Gauss point (Hex8SolidBbar._bbar_matrices / stiffness,
+ * fe/elements/solids.py:225-262). */
+int h8_ke_bbar_batched_f64(const double* X, const double* D, int64_t d_stride, int64_t m,
+                           double* ke, void* stream);
+
 /* ---- structured matrix-free operator ------------------------------------ */
 /* dinv[i] = 1/K_ii at free DOFs (free_mask[i] != 0), 0 at fixed DOFs, with
  * K_ii = sum_e s_e K0_aa.  Returns H8_ZERO_DIAG if a free diagonal is 0.

@@ -38,6 +38,12 @@ PINNING (function -> independent checks):
                           run); a constant field passes the filter unchanged; the
                           OC result holds the move limit, the bounds and the
                           volume to the bisection tolerance
+  hex8_ke_bbar          : the reference's own Hex8SolidBbar stiffness on the committed
+                          fixture (unit cube, distorted hex, near-incompressible
+                          brick); on an affine displacement field it acts as
+                          hex8_ke does (u^T ke_bbar u = V eps^T D eps: B_bar u =
+                          B u at every Gauss point); symmetric, the 6 rigid modes
+                          in its null space, rank 18
   recover / von_mises   : the reference's own recovery outputs on the committed
                           golden fixture (per-element D and one D); a uniform
                           strain field gives D eps at every node and element, a
@@ -107,6 +113,29 @@ def hex8_ke(coords, D, order=2):
     for p, w in zip(pts, wts):
         B, detJ = hex8_B(p, coords)
         ke += (B.T @ D @ B) * detJ * w
+    return ke
+
+
+def hex8_ke_bbar(coords, D):
+    """B-bar ke: at every Gauss point B_bar = B - (m m^T/3) B + the element volume
+    average of (m m^T/3) B (signed detJ weights), ke = sum B_bar^T D B_bar detJ w
+    (fe/elements/solids.py:225-262)."""
+    pts, wts = gauss_points(2)
+    mm = np.outer([1.0, 1, 1, 0, 0, 0], [1.0, 1, 1, 0, 0, 0]) / 3.0
+    Bs, dets = [], []
+    Bvol = np.zeros((6, 24))
+    V = 0.0
+    for p, w in zip(pts, wts):
+        B, detJ = hex8_B(p, coords)
+        Bs.append(B)
+        dets.append(detJ)
+        Bvol += mm @ B * detJ * w
+        V += detJ * w
+    Bvol /= V
+    ke = np.zeros((24, 24))
+    for B, detJ, w in zip(Bs, dets, wts):
+        Bb = B - mm @ B + Bvol
+        ke += Bb.T @ D @ Bb * detJ * w
     return ke
 
 
