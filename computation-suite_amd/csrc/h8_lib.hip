@@ -23,6 +23,7 @@
 #include "h8_resident.h"
 #include "h8_recovery.h"
 #include "h8_topopt.h"
+#include "h8_constrained.h"
 
 using namespace h8;
 
@@ -955,6 +956,167 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
 // ---------------------------------------------------------------------------
 // Mixed precision: FP32 PCG with FP64 reliable residual replacement
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Linear multi-point constraints (eliminated; Jacobi-PCG on T^T K T)
+// ---------------------------------------------------------------------------
+static int cmap_dev(const h8_cmap* c, CMapDev& m) {
+    if (!c) return fail(H8_ERR_ARG, "constraint map is NULL");
+    if (c->n_slave < 0 || c->n_master < 0) return fail(H8_ERR_ARG, "negative constraint counts");
+    if (c->n_slave > 0 && (!c->slave || !c->s_ptr || !c->s_master || !c->s_coef))
+        return fail(H8_ERR_ARG, "constraint map: NULL slave arrays");
+    if (c->n_master > 0 && (!c->master || !c->m_ptr || !c->m_slave || !c->m_coef))
+        return fail(H8_ERR_ARG, "constraint map: NULL master arrays");
+    if (c->n_slave + c->n_master >= (1LL << 31)) return fail(H8_ERR_ARG, "constraint map too large");
+    m = CMapDev{c->n_slave, c->n_master, c->slave, c->s_ptr, c->s_master, c->s_coef,
+                c->master, c->m_ptr, c->m_slave, c->m_coef};
+    return H8_OK;
+}
+
+extern "C" int h8_constraint_expand_f64(const h8_cmap* c, double* v, void* stream) {
+    CMapDev m;
+    int rc = cmap_dev(c, m);
+    if (rc) return rc;
+    if (!v) return fail(H8_ERR_ARG, "NULL pointer");
+    if (m.n_slave == 0) return H8_OK;
+    hipLaunchKernelGGL(k_cexpand, dim3(unsigned((m.n_slave + 255) / 256)), dim3(256), 0, (hipStream_t)stream, m, v,
+                       (const PcgState*)nullptr);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_constraint_contract_f64(const h8_cmap* c, double* v, void* stream) {
+    CMapDev m;
+    int rc = cmap_dev(c, m);
+    if (rc) return rc;
+    if (!v) return fail(H8_ERR_ARG, "NULL pointer");
+    if (m.n_master == 0) return H8_OK;
+    hipLaunchKernelGGL(k_cfix, dim3(unsigned((m.n_master + 255) / 256)), dim3(256), 0, (hipStream_t)stream, m,
+                       (double*)nullptr, v, (const PcgState*)nullptr);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_constraint_diag_f64(const h8_grid* g, const double* s_e, const double* K0, const h8_cmap* c,
+                                      double* dinv, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    CMapDev m;
+    rc = cmap_dev(c, m);
+    if (rc) return rc;
+    if (!s_e || !K0 || !dinv) return fail(H8_ERR_ARG, "NULL pointer");
+    if (m.n_master == 0) return H8_OK;
+    hipLaunchKernelGGL(k_cdiag, dim3(unsigned((m.n_master + 255) / 256)), dim3(256), 0, (hipStream_t)stream, d, s_e,
+                       K0, m, dinv);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_pcg_constrained_f64(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
+                                      const h8_cmap* c, const double* b, double* x, double tol, int64_t maxiter,
+                                      void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    CMapDev m;
+    rc = cmap_dev(c, m);
+    if (rc) return rc;
+    if (!s_e || !K0 || !dinv || !b || !x || !ws) return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < ws_bytes_64(d))
+        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, ws_bytes_64(d));
+    if (!(tol >= 0.0)) return fail(H8_ERR_ARG, "tol must be >= 0");
+    if (maxiter < 0) return fail(H8_ERR_ARG, "maxiter must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    double* r = (double*)carve(p, sizeof(double) * d.ndof);
+    double* pv = (double*)carve(p, sizeof(double) * d.ndof);
+    (void)carve(p, sizeof(double) * d.ndof);
+    double* q = (double*)carve(p, sizeof(double) * d.ndof);
+    double* partials = (double*)carve(p, sizeof(double) * partial_slots(d));
+    PcgState* st = (PcgState*)carve(p, sizeof(PcgState));
+
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    const int sb = stream_blocks(d.ndof);
+    hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, x, 0.0);
+    hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, pv, 0.0);
+    LAUNCHCHK();
+    PcgState hst{};
+    hst.tol = tol;
+    hst.maxiter = maxiter;
+    hst.status = ST_RUNNING;
+    HIPCHK(hipMemcpyAsync(st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pcg_init<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, b, (const double*)nullptr,
+                       dinv, r, st, partials);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    long long launches = 0;
+    if (hst.status == ST_RUNNING && maxiter > 0) {
+        // one iteration: p = dinv r + beta p; expand p to the slaves; q = K p;
+        // contract q to the masters and zero p at the slaves; p.q -> alpha;
+        // x += alpha p, r -= alpha q, r.z, r.r -> beta / stop
+        NodeArgs<double> A{};
+        A.g = d; A.s_e = s_e; A.K0 = K0; A.u = pv; A.y = q;
+        const unsigned gs = unsigned(std::max<long long>(1, (m.n_slave + 255) / 256));
+        const unsigned gf = unsigned(std::max<long long>(1, (m.n_master + m.n_slave + 255) / 256));
+        const int chunk = chunk_for(maxiter);
+        hipStream_t cs;
+        rc = capture_stream(&cs);
+        if (rc) return rc;
+        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+        for (int it = 0; it < chunk && rc == 0; ++it) {
+            hipLaunchKernelGGL(k_cdir, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, r, dinv, pv, st);
+            if (m.n_slave > 0) hipLaunchKernelGGL(k_cexpand, dim3(gs), dim3(256), 0, cs, m, pv, st);
+            rc = launch_op<double, MODE_PLAIN>(d, A, cs);
+            if (m.n_slave + m.n_master > 0)
+                hipLaunchKernelGGL(k_cfix, dim3(gf), dim3(256), 0, cs, m, pv, q, st);
+            hipLaunchKernelGGL(k_cpq, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, pv, q, st, partials);
+            hipLaunchKernelGGL((k_pcg_update<double, false, true>), dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, x,
+                               r, pv, pv, q, dinv, st, partials);
+        }
+        hipGraph_t graph;
+        const hipError_t ee = hipStreamEndCapture(cs, &graph);
+        if (rc) return rc;
+        if (ee != hipSuccess) return fail(H8_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+        hipGraphExec_t exec;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(H8_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        for (;;) {
+            const hipError_t le = hipGraphLaunch(exec, s);
+            if (le != hipSuccess) {
+                (void)hipGraphExecDestroy(exec);
+                return fail(H8_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(le));
+            }
+            ++launches;
+            HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (hst.status != ST_RUNNING) break;
+        }
+        (void)hipGraphExecDestroy(exec);
+    } else if (hst.status == ST_RUNNING) {
+        hst.status = ST_MAXITER;
+        hst.iters = 0;
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    fill_stats(stats, hst, ms, launches);
+    if (hst.status == ST_NOT_SPD)
+        return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
+                    hst.bad_pq, hst.iters);
+    if (hst.status == ST_MAXITER)
+        return fail(H8_MAXITER, "preconditioned_cg: did not converge in %lld iterations (rel residual %.3e).",
+                    (long long)maxiter, hst.rel);
+    return H8_OK;
+}
+
 extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const double* K064,
                                const float* s_e32, const float* K032, const double* dinv64,
                                const double* f, double* u, double tol, int64_t maxiter, double delta,

@@ -49,6 +49,13 @@ class H8Instance(C.Structure):
                 ("node_off", C.c_int64), ("elem_off", C.c_int64), ("vec_off", C.c_int64)]
 
 
+class H8CMap(C.Structure):
+    """Reduced linear constraints on the device (include/h8pcg.h h8_cmap)."""
+    _fields_ = [("n_slave", C.c_int64), ("slave", C.c_void_p), ("s_ptr", C.c_void_p), ("s_master", C.c_void_p),
+                ("s_coef", C.c_void_p), ("n_master", C.c_int64), ("master", C.c_void_p), ("m_ptr", C.c_void_p),
+                ("m_slave", C.c_void_p), ("m_coef", C.c_void_p)]
+
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _D = C.c_double
@@ -82,6 +89,11 @@ SIGNATURES = {
     "h8_batched_last_launches": (C.c_int64, []),
     "h8_recovery_table_f64": (C.c_int, [_D, _D, _D, _P, _P, _P]),
     "h8_recover_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "h8_constraint_expand_f64": (C.c_int, [C.POINTER(H8CMap), _P, _P]),
+    "h8_constraint_contract_f64": (C.c_int, [C.POINTER(H8CMap), _P, _P]),
+    "h8_constraint_diag_f64": (C.c_int, [_GRID, _P, _P, C.POINTER(H8CMap), _P, _P]),
+    "h8_pcg_constrained_f64": (C.c_int, [_GRID, _P, _P, _P, C.POINTER(H8CMap), _P, _P, _D, _I64, _P, C.c_size_t,
+                                         _STATS, _P]),
     "h8_sens_filter_f64": (C.c_int, [_GRID, _D, _D, _D, _D, _P, _P, _D, _P, _P]),
     "h8_oc_workspace_bytes": (C.c_size_t, [_I64]),
     "h8_oc_update_f64": (C.c_int, [_P, _P, _I64, _D, _D, _D, _D, _D, C.c_int32, _P, C.POINTER(C.c_double), _P,

@@ -52,6 +52,7 @@ class FESystem:
         self._D_ref = None
         self._K_cache = None
         self._structure = None
+        self._constraints = ()
         self.last_solve_stats = None
 
     # ---- validation helpers (fe/solver.py:447-496) ------------------------------------------
@@ -278,10 +279,111 @@ class FESystem:
             return op.masked(F_t, op.matvec(uD_t)), uD_t
         return op.masked(F_t), None
 
+    # ---- linear constraints (fe/solver.py:1341-1397) -----------------------------------------
+    def add_constraint(self, terms, value=0.0):
+        """General linear constraint sum_i c_i u_i = value; terms: (node, dof, coefficient).
+        Eliminated exactly in the solve (fe/solver.py:1341-1372)."""
+        if not terms:
+            raise ValueError("add_constraint: no terms given.")
+        dofs, coefs = [], []
+        for node, dof, c in terms:
+            n = self._check_nodes([node] if not isinstance(node, str) else node, "add_constraint")
+            if len(n) != 1:
+                raise ValueError("add_constraint: each term needs exactly one node.")
+            d = self._check_local_dofs(dof, "add_constraint")
+            if len(d) != 1:
+                raise ValueError("add_constraint: each term needs exactly one DOF.")
+            dofs.append(3 * int(n[0]) + int(d[0]))
+            coefs.append(float(c))
+        if len(set(dofs)) != len(dofs):
+            raise ValueError("add_constraint: a DOF appears twice in one constraint; merge its terms.")
+        if not np.any(np.asarray(coefs)):
+            raise ValueError("add_constraint: all coefficients are zero.")
+        self._constraints = tuple(self._constraints) + ((np.array(dofs), np.array(coefs), float(value)),)
+
+    def tie(self, slave_nodes, master_nodes, dof_indices, offset=None, tol=1e-8):
+        """u_slave = u_master for each slave node and the master node at x_slave +
+        offset (default: coincident nodes), fe/solver.py:1374-1393."""
+        from scipy.spatial import cKDTree
+        s = self._check_nodes(slave_nodes, "tie")
+        m = self._check_nodes(master_nodes, "tie")
+        dofs = self._check_local_dofs(dof_indices, "tie")
+        nodes = np.asarray(self.mesh.nodes, dtype=float)
+        off = np.zeros(nodes.shape[1]) if offset is None else np.asarray(offset, dtype=float)
+        dist, idx = cKDTree(nodes[m]).query(nodes[s] + off)
+        if np.any(dist > tol):
+            bad = s[np.argmax(dist)]
+            raise ValueError(f"tie: no master node within {tol:g} of slave node {int(bad)} (offset {off}).")
+        for sn, j in zip(s, idx):
+            mn = int(m[j])
+            if mn == int(sn):
+                continue
+            for d in dofs:
+                self.add_constraint([(int(sn), int(d), 1.0), (mn, int(d), -1.0)])
+
+    def clear_constraints(self):
+        """Remove all linear constraints added with add_constraint / tie."""
+        self._constraints = ()
+
+    def _solve_constrained(self, tol, maxiter, F_host):
+        """Jacobi-PCG on T^T K T over the kept DOFs (h8_pcg_constrained_f64):
+        the constraints reduced as fe/boundary.py:98-166 does, u = T x + w with w
+        the prescribed values and the slave offsets (fe/solver.py:1443-1479)."""
+        import ctypes as C
+        import torch
+        from .constraints import ConstraintMap, reduce_constraints
+        op = self._op
+        fixed = self.fixed_dofs_array
+        u_fixed = self._fixed_dof_values_array(fixed)
+        rows = [dict(zip(d.tolist(), c.tolist())) for d, c, _ in self._constraints]
+        vals = [v for _, _, v in self._constraints]
+        slave, mat, g = reduce_constraints(rows, vals, dict(zip(fixed.tolist(), u_fixed.tolist())))
+        cm = ConstraintMap(slave, mat, g, op.device)
+        mask = self._free_mask()
+        mask[cm.slave] = 0
+        op.set_free_mask(mask)
+        with torch.cuda.device(op.device):
+            check = _lib.check
+            check(_lib.lib().h8_constraint_diag_f64(C.byref(op.grid), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
+                                                    C.byref(cm.c), _lib.ptr(op.dinv), op._stream()),
+                  "h8_constraint_diag_f64")
+            if len(cm.master) and float(op.dinv[torch.as_tensor(cm.master, device=op.device)].min()) <= 0.0:
+                raise ValueError("jacobi_preconditioner: the constrained system has a zero diagonal entry.")
+            w = np.zeros(self.n_dof)
+            w[fixed] = u_fixed
+            w[cm.slave] = cm.g
+            w_t = torch.as_tensor(w, device=op.device)
+            F_t = torch.as_tensor(np.asarray(F_host, dtype=float), device=op.device)
+            b = (F_t - op.matvec(w_t)) if np.any(w) else F_t.clone()
+            b = op.masked(cm.contract(b.contiguous()))
+            n_keep = int(mask.sum())
+            if maxiter is None:
+                maxiter = 10 * n_keep
+            x = torch.zeros(self.n_dof, dtype=torch.float64, device=op.device)
+            st = _lib.H8PcgStats()
+            ws = op.workspace(64)
+            rc = _lib.lib().h8_pcg_constrained_f64(C.byref(op.grid), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
+                                                   _lib.ptr(op.dinv), C.byref(cm.c), _lib.ptr(b), _lib.ptr(x),
+                                                   float(tol), int(maxiter), _lib.ptr(ws), ws.numel(),
+                                                   C.byref(st), op._stream())
+            stats = st.as_dict()
+            stats["rc"] = rc
+            check(rc)
+            cm.expand(x)
+        stats.update({"method": "pcg", "precision": "fp64", "constraints": len(self._constraints),
+                      "slaves": int(len(cm.slave))})
+        self.last_solve_stats = stats
+        return x + w_t
+
     def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner="jacobi",
                             precision="fp64", delta=0.5, F=None):
         """solve_static, returning the full displacement as a device tensor (no host copy)."""
         self._require_stiffness("solve_static")
+        if self._constraints:
+            if method not in (None, "pcg") or preconditioner != "jacobi" or precision != "fp64" or x0 is not None:
+                raise NotImplementedError("solve_static with linear constraints supports the FP64 Jacobi-PCG "
+                                          "(method None or 'pcg', no x0) on backend='hip'.")
+            return self._solve_constrained(tol, maxiter, self.F if F is None else F)
         if method is None:
             method = "pcg"
         if method not in ("cg", "pcg"):

@@ -226,6 +226,48 @@ int h8_recovery_table_f64(double hx, double hy, double hz, const double* D, doub
 int h8_recover_f64(const h8_grid* g, const double* tab, const double* s_e, const double* u,
                    double* nodal, double* vm_nodal, double* elem, double* vm_elem, void* stream);
 
+/* ---- linear multi-point constraints --------------------------------------- */
+/* The reduced form of the constraints sum_i c_i u_i = value (add_constraint /
+ * tie, fe/solver.py:1341-1397), as the host's reduction (fe/boundary.py:98-166)
+ * leaves them: slave DOFs u_s = g_s + sum_m T_sm u_m over kept master DOFs.
+ * Device arrays: for slave t, its masters s_master[s_ptr[t] .. s_ptr[t+1]) with
+ * coefficients s_coef; the transpose by master likewise (m_ptr, m_slave,
+ * m_coef).  Masters are never slaves. */
+typedef struct h8_cmap {
+    int64_t n_slave;
+    const int64_t* slave;
+    const int32_t* s_ptr;
+    const int64_t* s_master;
+    const double* s_coef;
+    int64_t n_master;
+    const int64_t* master;
+    const int32_t* m_ptr;
+    const int64_t* m_slave;
+    const double* m_coef;
+} h8_cmap;
+
+/* v_s = sum_m T_sm v_m at every slave (the other entries unchanged). */
+int h8_constraint_expand_f64(const h8_cmap* c, double* v, void* stream);
+/* v_m += sum_s T_sm v_s at every master (T^T applied; the slave entries are
+ * left for the caller to mask). */
+int h8_constraint_contract_f64(const h8_cmap* c, double* v, void* stream);
+/* dinv_m = 1 / diag(T^T K T)_m at every master, the K entries evaluated from
+ * K0 and s_e (0 where that diagonal is not positive); the other entries of
+ * dinv (1/K_ii at kept DOFs, 0 at fixed and slave DOFs, from h8_diag_f64 with
+ * the slaves masked) are left as they are. */
+int h8_constraint_diag_f64(const h8_grid* g, const double* s_e, const double* K0, const h8_cmap* c,
+                           double* dinv, void* stream);
+/* Jacobi-PCG (the recurrence and stopping rule of h8_pcg_f64, x0 = 0) on the
+ * eliminated system T^T K T x = b over the kept DOFs, on full-length vectors
+ * zero at fixed and slave DOFs (b contracted and masked by the caller; dinv
+ * from h8_diag_f64 + h8_constraint_diag_f64).  The caller recovers
+ * u = expand(x) + w.  Replaces the constrained solve of form_linear_system +
+ * ReducedSystem.solve (fe/solver.py:1403-1479, fe/linear_system.py:101-140);
+ * workspace of h8_pcg_workspace_bytes(g, 64). */
+int h8_pcg_constrained_f64(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
+                           const h8_cmap* c, const double* b, double* x, double tol, int64_t maxiter,
+                           void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
+
 /* ---- SIMP loop: sensitivity filter, optimality-criteria update ---------- */
 /* Sigmund's sensitivity filter on the grid's elements (fe/topopt.py:235-252):
  * out_e = sum_j H_ej rho_j dc_j / max(max(rho_e, eps) sum_j H_ej, eps) with

@@ -44,6 +44,11 @@ PINNING (function -> independent checks):
                           hex8_ke does (u^T ke_bbar u = V eps^T D eps: B_bar u =
                           B u at every Gauss point); symmetric, the 6 rigid modes
                           in its null space, rank 18
+  reduce_constraints / solve_constrained :
+                          the reference's own reduced map of an overlapping
+                          constraint set and its constrained direct solve on the
+                          committed fixture; the solution satisfies every
+                          constraint and K u = F at the kept DOFs
   recover / von_mises   : the reference's own recovery outputs on the committed
                           golden fixture (per-element D and one D); a uniform
                           strain field gives D eps at every node and element, a
@@ -637,3 +642,77 @@ def topology_optimize(nodes, elements, D, F, fixed_dofs, volume_fraction, p=3.0,
         if change < tol:
             break
     return rho, hist, u
+
+
+# ---------------------------------------------------------------------------
+# Linear constraints: fe/boundary.py:98-166, fe/solver.py:1403-1479
+# ---------------------------------------------------------------------------
+def reduce_constraints(rows, values, fixed_values, tol=1e-10):
+    """(slave, {slave: {master: coef}}, {slave: g}): u_s = g_s - sum coef u_m,
+    by Gauss-Jordan elimination with largest-|entry| pivots over all rows at
+    once (fe/boundary.py:98-166)."""
+    clean, rhs = [], []
+    for r, v in zip(rows, values):
+        row, v = {}, float(v)
+        for d, c in r.items():
+            if d in fixed_values:
+                v -= c * fixed_values[d]
+            else:
+                row[d] = row.get(d, 0.0) + c
+        clean.append(row)
+        rhs.append(v)
+    cols = sorted({d for r in clean for d in r})
+    A = np.zeros((len(clean), len(cols) + 1))
+    for i, r in enumerate(clean):
+        for d, c in r.items():
+            A[i, cols.index(d)] = c
+        A[i, -1] = rhs[i]
+    scale = max(1.0, np.abs(A).max())
+    pivots, used = [], []
+    for _ in range(min(len(clean), len(cols))):
+        sub = np.abs(A[:, :-1])
+        sub[used, :] = 0.0
+        for _r, c in pivots:
+            sub[:, c] = 0.0
+        i, j = np.unravel_index(np.argmax(sub), sub.shape)
+        if sub[i, j] <= tol * scale:
+            break
+        A[i] /= A[i, j]
+        for r in range(len(clean)):
+            if r != i and A[r, j] != 0.0:
+                A[r] -= A[r, j] * A[i]
+        pivots.append((i, j))
+        used.append(i)
+    pc = {j for _, j in pivots}
+    slave = [cols[j] for _, j in pivots]
+    mat = {cols[j]: {cols[k]: A[i, k] for k in range(len(cols)) if k not in pc and abs(A[i, k]) > 1e-14}
+           for i, j in pivots}
+    g = {cols[j]: float(A[i, -1]) for i, j in pivots}
+    return np.array(slave, dtype=int), mat, g
+
+
+def solve_constrained(K, F, fixed_dofs, fixed_values, constraints):
+    """Direct solve with the constraints eliminated (fe/solver.py:1443-1479):
+    u_free = T u_keep + up, T^T K_ff T u_keep = T^T (F_f - K_fc u_c - K_ff up).
+    constraints: list of (dofs, coefs, value)."""
+    n = K.shape[0]
+    fixed = np.array(sorted(set(int(d) for d in fixed_dofs)), dtype=int)
+    fv = dict(zip(fixed.tolist(), np.asarray(fixed_values, dtype=float).tolist()))
+    free = np.setdiff1d(np.arange(n), fixed)
+    rows = [dict(zip(np.asarray(d).tolist(), np.asarray(c, dtype=float).tolist())) for d, c, _ in constraints]
+    slave, mat, g = reduce_constraints(rows, [v for _, _, v in constraints], fv)
+    keep = np.setdiff1d(free, slave)
+    kpos = {int(d): i for i, d in enumerate(keep)}
+    T = sp.lil_matrix((n, len(keep)))
+    for d in keep:
+        T[d, kpos[int(d)]] = 1.0
+    for s in slave:
+        for m, c in mat[int(s)].items():
+            T[int(s), kpos[int(m)]] = -c
+    w = np.zeros(n)
+    w[fixed] = [fv[int(d)] for d in fixed]
+    w[slave] = [g[int(s)] for s in slave]
+    T = T.tocsr()
+    Kc = (T.T @ K @ T).tocsc()
+    x = spla.spsolve(Kc, T.T @ (F - K @ w))
+    return T @ x + w

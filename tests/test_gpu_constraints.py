@@ -1,0 +1,81 @@
+"""GPU parity of the constrained solve (h8_pcg_constrained_f64 through
+FESystem.add_constraint / tie / solve_static) against the reference's own
+direct solve on the committed fixture and against the oracle on a face tie."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import h8_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "constraint_golden.npz")
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def h8(gpu):
+    import h8pcg
+    return h8pcg
+
+
+def test_constrained_solve_matches_reference(h8):
+    g = np.load(GOLD)
+    nx, ny, nz = (int(v) for v in g["solve/dims"])
+    mesh = h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    fs = h8.FESystem(mesh, h8.Hex8Solid3D(), backend="hip", device="cuda")
+    fs.assemble_stiffness(h8.coefficients.by_element([h8.D_solid3d(h8.Material(E=float(e), nu=0.3))
+                                                      for e in g["solve/E"]]))
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.fix_dofs([int(g["solve/mid"])], [0], value=1e-4)
+    fs.tie(g["solve/y0"], g["solve/y1"], [2], offset=[0.0, float(ny), 0.0])
+    a, b, c = (int(v) for v in g["solve/abc"])
+    fs.add_constraint([(a, "ux", 1.0), (b, "ux", -2.0), (c, "uy", 0.5)], value=2e-5)
+    fs.add_consistent_facet_load(h8.Hex8Solid3D(), h8.boundary_faces(mesh, 0, float(nx)), 2, -1e6 / (ny * nz))
+    assert np.allclose(fs.F, g["solve/F"], rtol=0, atol=1e-6)
+    U = np.asarray(fs.solve_static(tol=1e-12))
+    ref = g["solve/U"]
+    err = np.linalg.norm(U - ref) / np.linalg.norm(ref)
+    say("constrained vs reference", err, fs.last_solve_stats)
+    assert err < 1e-9
+    assert abs(U[3 * a] - 2 * U[3 * b] + 0.5 * U[3 * c + 1] - 2e-5) < 1e-12
+    assert U[3 * int(g["solve/mid"])] == 1e-4
+    with pytest.raises(NotImplementedError):
+        fs.solve_static(precision="mixed")
+
+
+def test_face_tie_matches_oracle(h8):
+    nx, ny, nz = 24, 6, 6
+    rng = np.random.default_rng(17)
+    E = 10 ** rng.uniform(9, 11, nx * ny * nz)
+    mesh = h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    fs = h8.FESystem(mesh, h8.Hex8Solid3D(), backend="hip", device="cuda")
+    fs.assemble_stiffness(h8.coefficients.by_element([h8.D_solid3d(h8.Material(E=float(e), nu=0.3)) for e in E]))
+    clamp = mesh.nodes_on_plane(0, 0.0)
+    fs.fix_dofs(clamp, [0, 1, 2])
+    tip = mesh.nodes_on_plane(0, float(nx))
+    centre = int(tip[np.argmin(np.linalg.norm(np.asarray(mesh.nodes)[tip] - [nx, ny / 2, nz / 2], axis=1))])
+    # the tip face moves rigidly in z with its centre node (a star of ties), and the
+    # z = 0 and z = Lz faces are tied in x (periodic-style pairs)
+    for n in tip:
+        if int(n) != centre:
+            fs.add_constraint([(int(n), "uz", 1.0), (centre, "uz", -1.0)])
+    z0 = np.array([n for n in mesh.nodes_on_plane(2, 0.0) if mesh.nodes[n, 0] > 0.5 and mesh.nodes[n, 0] < nx - 0.5])
+    fs.tie(z0, mesh.nodes_on_plane(2, float(nz)), [0], offset=[0.0, 0.0, float(nz)])
+    fs.add_nodal_force([centre], 2, -1e6)
+    say("face tie", len(fs._constraints), "constraints")
+    U = np.asarray(fs.solve_static(tol=1e-12))
+    c = o.cantilever(nx, ny, nz, E_elem=E)
+    cons = [(d, co, v) for d, co, v in fs._constraints]
+    Uo = o.solve_constrained(c["K"], np.asarray(fs.F), c["fixed_dofs"], np.zeros(len(c["fixed_dofs"])), cons)
+    err = np.linalg.norm(U - Uo) / np.linalg.norm(Uo)
+    say("face tie vs oracle", err, fs.last_solve_stats["iterations"])
+    assert err < 1e-9
+    for d, co, v in cons:
+        assert abs(co @ U[d] - v) < 1e-12 * np.abs(U).max()

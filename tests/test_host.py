@@ -111,3 +111,45 @@ def test_coefficient_factoring():
     assert np.allclose(s * D_ref[0, 0], [D_solid3d(Material(e, 0.3))[0, 0] for e in E])
     with pytest.raises(ValueError, match="scalar multiples"):
         coefficients.factor_scales(coefficients.by_element([D_solid3d(Material(1, 0.3)), D_solid3d(Material(1, 0.2))]))
+
+
+def test_constraint_reduction_matches_reference_fixture():
+    """The product's grouped constraint reduction (h8pcg.constraints) gives the
+    reference's reduced map (tests/golden/constraint_golden.npz)."""
+    import numpy as np
+    from h8pcg.constraints import reduce_constraints
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "constraint_golden.npz"))
+    rows = [{3: 1.0, 7: -1.0}, {7: 2.0, 11: -1.0, 5: 0.5}, {11: 1.0, 2: 1.0}, {20: 1.0, 21: -1.0}]
+    slave, mat, gg = reduce_constraints(rows, [0.0, 1.5, -0.25, 0.0], {2: 0.1})
+    ref = {int(s): i for i, s in enumerate(g["reduce/slave"])}
+    assert sorted(slave.tolist()) == sorted(ref)
+    masters = list(g["reduce/masters"])
+    for s in slave:
+        got = np.zeros(len(masters))
+        for m, c in mat[int(s)].items():
+            got[masters.index(m)] = c
+        assert np.allclose(got, g["reduce/mat"][ref[int(s)]], rtol=0, atol=1e-15)
+        assert np.isclose(gg[int(s)], g["reduce/g"][ref[int(s)]], rtol=0, atol=1e-15)
+    import pytest
+    with pytest.raises(ValueError, match="inconsistent"):
+        reduce_constraints([{1: 1.0, 2: -1.0}, {1: 2.0, 2: -2.0}], [0.0, 1.0], {})
+
+
+def test_add_constraint_argument_checks():
+    import numpy as np
+    import pytest
+    import h8pcg
+    mesh = h8pcg.box_mesh(2.0, 1.0, 1.0, 2, 1, 1)
+    fs = h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D(), backend="hip", device="cuda")
+    with pytest.raises(ValueError, match="no terms"):
+        fs.add_constraint([])
+    with pytest.raises(ValueError, match="twice"):
+        fs.add_constraint([(1, "ux", 1.0), (1, 0, -1.0)])
+    with pytest.raises(ValueError, match="all coefficients are zero"):
+        fs.add_constraint([(1, "ux", 0.0), (2, "ux", 0.0)])
+    fs.add_constraint([(1, "ux", 1.0), (2, "ux", -1.0)], value=0.5)
+    assert len(fs._constraints) == 1 and fs._constraints[0][2] == 0.5
+    fs.clear_constraints()
+    assert fs._constraints == ()
+    with pytest.raises(ValueError, match="no master node"):
+        fs.tie([0], [5], [0], offset=[10.0, 0.0, 0.0])
