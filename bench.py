@@ -87,10 +87,21 @@ KERNEL_NAMES = {  # (variant, precision) -> kernel symbol of the fused direction
 }
 
 
-def kernel_name(variant, precision, arith=0):
+DEFER_X_MIN_DOF = 1000000   # FP32 sector kernels apply the deferred x update from here
+
+
+def deferred_x(variant, precision, n_dof):
+    """True when the FP32 sector kernel applies the previous x += alpha p while
+    staging p (mode 3), so the update kernel streams only r, q, dinv and r."""
+    return precision == 32 and n_dof >= DEFER_X_MIN_DOF and variant in (5, 6)
+
+
+def kernel_name(variant, precision, arith=0, n_dof=0):
     """Symbol of the fused direction kernel; FP32 sector and resident kernels
     carry their element-product arithmetic (FP32 unless arith = 64)."""
     n = KERNEL_NAMES.get((variant, precision), "unknown")
+    if deferred_x(variant, precision, n_dof):
+        n = n.replace(", 1, %s>", ", 3, %s>")
     return n % ("double" if arith == 64 else "float") if "%s" in n else n
 RESIDENT_REPS = 100   # iterations per launch of the resident solver's timing run
 
@@ -207,9 +218,13 @@ def main():
     dir_ms, upd_ms = time_iteration(op.grid32, 32, RESIDENT_REPS if resident else 200)
     avg64 = time_iteration(op.grid, 64, RESIDENT_REPS if op.grid.variant == _lib.VARIANT_RESIDENT else 200)
     n_dof = fs.n_dof
-    dir_bytes = 5 * 4 * n_dof + 4 * m          # read r, dinv, p_old; write p, q; read s_e
-    upd_bytes = 7 * 4 * n_dof                  # read x, r, p, q, dinv; write x, r
-    kname = kernel_name(op.grid32.variant, 32, op.grid32.arith)
+    if deferred_x(op.grid32.variant, 32, n_dof):
+        dir_bytes = 7 * 4 * n_dof + 4 * m      # read r, dinv, p_old, x; write p, q, x; read s_e
+        upd_bytes = 4 * 4 * n_dof              # read r, q, dinv; write r
+    else:
+        dir_bytes = 5 * 4 * n_dof + 4 * m      # read r, dinv, p_old; write p, q; read s_e
+        upd_bytes = 7 * 4 * n_dof              # read x, r, p, q, dinv; write x, r
+    kname = kernel_name(op.grid32.variant, 32, op.grid32.arith, n_dof)
     if resident:
         # one launch = RESIDENT_REPS iterations, each with the bytes of a streaming iteration
         launch_ms = dir_ms * RESIDENT_REPS

@@ -52,8 +52,20 @@ def time_kernels(op, precision, reps):
     return avg[0] * 1e3, avg[1] * 1e3
 
 
+DEFER_X_MIN_DOF = 1000000     # the library defers x += alpha p into the FP32 sector kernel from here
+
+
 def bytes_per_launch(op, precision):
+    """Algorithmic bytes of the two kernels of one iteration.  Direction: read r,
+    dinv, p_old, write p, q, read s_e; update: read x, p, r, q, dinv, write x, r.
+    FP32 sector kernels on grids of DEFER_X_MIN_DOF+ DOFs apply the previous
+    x += alpha p while they stage p (read and write x there), so the update
+    streams r, q, dinv and r only."""
     w = precision // 8
+    g = op.grid if precision == 64 else op.grid32
+    deferred = (precision == 32 and op.n_dof >= DEFER_X_MIN_DOF and g.variant in (0, 5, 6))
+    if deferred:
+        return {"dir": 7 * w * op.n_dof + w * op.m, "update": 4 * w * op.n_dof}
     return {"dir": 5 * w * op.n_dof + w * op.m, "update": 7 * w * op.n_dof}
 
 
