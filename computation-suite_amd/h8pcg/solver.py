@@ -325,10 +325,12 @@ class FESystem:
         """Remove all linear constraints added with add_constraint / tie."""
         self._constraints = ()
 
-    def _solve_constrained(self, tol, maxiter, F_host):
-        """Jacobi-PCG on T^T K T over the kept DOFs (h8_pcg_constrained_f64):
-        the constraints reduced as fe/boundary.py:98-166 does, u = T x + w with w
-        the prescribed values and the slave offsets (fe/solver.py:1443-1479)."""
+    def _constrained_setup(self, F_host):
+        """Device set-up of the eliminated system (fe/solver.py:1403-1479): the
+        constraints reduced as fe/boundary.py:98-166 does (ConstraintMap, empty
+        without constraints), dinv of T^T K T (zero at fixed and slave DOFs),
+        w = prescribed values and slave offsets, and b = mask(T^T (F - K w)).
+        Returns (op, cm, w_t, b)."""
         import ctypes as C
         import torch
         from .constraints import ConstraintMap, reduce_constraints
@@ -343,10 +345,9 @@ class FESystem:
         mask[cm.slave] = 0
         op.set_free_mask(mask)
         with torch.cuda.device(op.device):
-            check = _lib.check
-            check(_lib.lib().h8_constraint_diag_f64(C.byref(op.grid), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
-                                                    C.byref(cm.c), _lib.ptr(op.dinv), op._stream()),
-                  "h8_constraint_diag_f64")
+            _lib.check(_lib.lib().h8_constraint_diag_f64(C.byref(op.grid), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
+                                                         C.byref(cm.c), _lib.ptr(op.dinv), op._stream()),
+                       "h8_constraint_diag_f64")
             if len(cm.master) and float(op.dinv[torch.as_tensor(cm.master, device=op.device)].min()) <= 0.0:
                 raise ValueError("jacobi_preconditioner: the constrained system has a zero diagonal entry.")
             w = np.zeros(self.n_dof)
@@ -356,9 +357,18 @@ class FESystem:
             F_t = torch.as_tensor(np.asarray(F_host, dtype=float), device=op.device)
             b = (F_t - op.matvec(w_t)) if np.any(w) else F_t.clone()
             b = op.masked(cm.contract(b.contiguous()))
-            n_keep = int(mask.sum())
-            if maxiter is None:
-                maxiter = 10 * n_keep
+        cm.keep_mask = mask
+        return op, cm, w_t, b
+
+    def _solve_constrained(self, tol, maxiter, F_host):
+        """Jacobi-PCG on T^T K T over the kept DOFs (h8_pcg_constrained_f64);
+        u = T x + w."""
+        import ctypes as C
+        import torch
+        op, cm, w_t, b = self._constrained_setup(F_host)
+        if maxiter is None:
+            maxiter = 10 * int(cm.keep_mask.sum())
+        with torch.cuda.device(op.device):
             x = torch.zeros(self.n_dof, dtype=torch.float64, device=op.device)
             st = _lib.H8PcgStats()
             ws = op.workspace(64)
@@ -368,12 +378,29 @@ class FESystem:
                                                    C.byref(st), op._stream())
             stats = st.as_dict()
             stats["rc"] = rc
-            check(rc)
+            _lib.check(rc)
             cm.expand(x)
         stats.update({"method": "pcg", "precision": "fp64", "constraints": len(self._constraints),
                       "slaves": int(len(cm.slave))})
         self.last_solve_stats = stats
         return x + w_t
+
+    def form_linear_system(self, F=None, K=None):
+        """The eliminated system K_r u_r = F_r over the kept DOFs as a
+        ReducedSystem (fe/solver.py:1403-1479): K_r is a scipy LinearOperator
+        whose products run on the GPU (T^T K T, matrix-free), F_r the reduced
+        load; .recover(u_r) gives the full U.  "Bring your own solver":
+        rs.recover(scipy.sparse.linalg.cg(rs.K, rs.F)[0]).  K= (another full
+        matrix) is not available on the matrix-free backend."""
+        from .linear_system import ReducedSystem
+        if K is not None:
+            raise NotImplementedError("form_linear_system(K=...): backend='hip' is matrix-free (K = sum_e s_e K0)")
+        self._require_stiffness("form_linear_system")
+        Fv = self.F if F is None else np.asarray(F, dtype=float)
+        if Fv.shape != (self.n_dof,):
+            raise ValueError(f"F must have shape ({self.n_dof},), got {Fv.shape}")
+        op, cm, w_t, b = self._constrained_setup(Fv)
+        return ReducedSystem(self, op, cm, w_t.cpu().numpy(), b)
 
     def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner="jacobi",
                             precision="fp64", delta=0.5, F=None):

@@ -79,3 +79,53 @@ def test_face_tie_matches_oracle(h8):
     assert err < 1e-9
     for d, co, v in cons:
         assert abs(co @ U[d] - v) < 1e-12 * np.abs(U).max()
+
+
+def test_form_linear_system_bring_your_own_solver(h8):
+    """form_linear_system: the reduced operator (products on the GPU) and load
+    equal the oracle's T^T K T and T^T (F - K w); scipy's CG on them, recovered,
+    is the reference's constrained solution."""
+    import scipy.sparse.linalg as spla
+    g = np.load(GOLD)
+    nx, ny, nz = (int(v) for v in g["solve/dims"])
+    mesh = h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    fs = h8.FESystem(mesh, h8.Hex8Solid3D(), backend="hip", device="cuda")
+    fs.assemble_stiffness(h8.coefficients.by_element([h8.D_solid3d(h8.Material(E=float(e), nu=0.3))
+                                                      for e in g["solve/E"]]))
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.fix_dofs([int(g["solve/mid"])], [0], value=1e-4)
+    fs.tie(g["solve/y0"], g["solve/y1"], [2], offset=[0.0, float(ny), 0.0])
+    a, b, c = (int(v) for v in g["solve/abc"])
+    fs.add_constraint([(a, "ux", 1.0), (b, "ux", -2.0), (c, "uy", 0.5)], value=2e-5)
+    fs.add_consistent_facet_load(h8.Hex8Solid3D(), h8.boundary_faces(mesh, 0, float(nx)), 2, -1e6 / (ny * nz))
+    rs = fs.form_linear_system()
+    # the oracle's reduced system with the same elimination
+    co = o.cantilever(nx, ny, nz, E_elem=g["solve/E"])
+    fixed = fs.fixed_dofs_array
+    fv = dict(zip(fixed.tolist(), fs._fixed_dof_values_array(fixed).tolist()))
+    rows = [dict(zip(d.tolist(), cc.tolist())) for d, cc, _ in fs._constraints]
+    slave, mat, gg = o.reduce_constraints(rows, [v for _, _, v in fs._constraints], fv)
+    keep = np.setdiff1d(np.setdiff1d(np.arange(fs.n_dof), fixed), slave)
+    assert np.array_equal(rs.free, keep)
+    T = np.zeros((fs.n_dof, len(keep)))
+    T[keep, np.arange(len(keep))] = 1.0
+    kpos = {int(d): i for i, d in enumerate(keep)}
+    for s in slave:
+        for m, cf in mat[int(s)].items():
+            T[int(s), kpos[int(m)]] = -cf
+    w = np.zeros(fs.n_dof)
+    w[fixed] = [fv[int(d)] for d in fixed]
+    w[slave] = [gg[int(s)] for s in slave]
+    K = co["K"].toarray()
+    Kr = T.T @ K @ T
+    Fr = T.T @ (fs.F - K @ w)
+    v = np.random.default_rng(3).normal(size=len(keep))
+    say("form_linear_system", len(keep), np.abs(rs.K @ v - Kr @ v).max() / np.abs(Kr @ v).max(),
+        np.abs(rs.F - Fr).max() / np.abs(Fr).max())
+    assert np.abs(rs.K @ v - Kr @ v).max() <= 1e-12 * np.abs(Kr @ v).max()
+    assert np.abs(rs.F - Fr).max() <= 1e-12 * np.abs(Fr).max()
+    x, info = spla.cg(rs.K, rs.F, rtol=1e-13, maxiter=20000)
+    U = np.asarray(rs.recover(x))
+    ref = g["solve/U"]
+    assert info == 0 and np.linalg.norm(U - ref) / np.linalg.norm(ref) < 1e-8
+    assert np.linalg.norm(np.asarray(rs.solve(tol=1e-12)) - ref) / np.linalg.norm(ref) < 1e-9
