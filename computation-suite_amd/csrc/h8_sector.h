@@ -202,7 +202,9 @@ template <> struct SectorWaves<float> { static constexpr int value = 6; };
 // LDS of one march: the 3-slot layer ring and the per-node corner slots.
 template <typename T, int EJ, int EI>
 struct SectorSmem {
-    // 16-byte aligned members: a node's 12 slot values are read with 128-bit loads
+    // corner slots component-major: value k (corner a & 3, component c: k = 3(a & 3) + c)
+    // of node n at k * EJ * EI + n, so lane-adjacent nodes hit adjacent LDS banks both
+    // when the elements write them and when the nodes sum them
     alignas(16) T ring[3 * SLayer<T, EJ, EI>::LW];
     alignas(16) T lo[EJ * EI * 12];
     alignas(16) T hi[EJ * EI * 12];
@@ -294,9 +296,9 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh,
             for (int a = 0; a < 8; ++a) {
                 if ((OY(a) && ej + 1 >= EJ) || (OX(a) && ei + 1 >= EI)) continue;
                 T* dst = OZ(a) ? sHi : sLo;
-                const int slot = ((ei + OX(a)) * EJ + ej + OY(a)) * 12 + 3 * (a & 3);
+                const int node = (ei + OX(a)) * EJ + ej + OY(a);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) dst[slot + c] = T(U[a][c]);
+                for (int c = 0; c < 3; ++c) dst[(3 * (a & 3) + c) * (EJ * EI) + node] = T(U[a][c]);
             }
         }
         __syncthreads();
@@ -308,8 +310,8 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh,
                 T sl = T(0), sh = T(0);
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    sl += sLo[threadIdx.x * 12 + 3 * a + c];
-                    sh += sHi[threadIdx.x * 12 + 3 * a + c];
+                    sl += sLo[(3 * a + c) * (EJ * EI) + threadIdx.x];
+                    sh += sHi[(3 * a + c) * (EJ * EI) + threadIdx.x];
                 }
                 qv[c] = carry[c] + sl;
                 top[c] = sh;
