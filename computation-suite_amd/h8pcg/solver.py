@@ -279,6 +279,47 @@ class FESystem:
             return op.masked(F_t, op.matvec(uD_t)), uD_t
         return op.masked(F_t), None
 
+    def _rigid_null_basis(self):
+        """Orthonormal basis (n_dof, r) of the rigid-body motions that vanish at every
+        fixed DOF -- the null space of Kff for a connected mesh with positive element
+        scales -- or None when the supports remove all six.  fe/solver.py:1583-1678
+        finds the same directions as near-zero eigenvalues of a dense Kff; here they
+        follow from the node coordinates: translations and rotations about the
+        centroid, combined to vanish on the fixed set."""
+        fixed = self.fixed_dofs_array
+        key = (len(fixed), hash(fixed.tobytes()))
+        cache = getattr(self, "_rigid_cache", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        X = np.asarray(self.mesh.nodes, dtype=float)
+        Y = X - X.mean(axis=0)
+        Y /= max(float(np.ptp(X, axis=0).max()), 1e-300)
+
+        def rig(dofs):
+            a, c = dofs // 3, dofs % 3
+            y = Y[a]
+            R = np.zeros((len(dofs), 6))
+            R[np.arange(len(dofs)), c] = 1.0
+            R[:, 3] = np.where(c == 1, -y[:, 2], np.where(c == 2, y[:, 1], 0.0))
+            R[:, 4] = np.where(c == 0, y[:, 2], np.where(c == 2, -y[:, 0], 0.0))
+            R[:, 5] = np.where(c == 0, -y[:, 1], np.where(c == 1, y[:, 0], 0.0))
+            return R
+
+        if len(fixed) == 0:
+            N = np.eye(6)
+        else:
+            _, s, Vt = np.linalg.svd(rig(fixed), full_matrices=True)
+            rank = int(np.sum(s > 1e-10 * max(float(s[0]), 1.0)))
+            N = Vt[rank:].T
+        Q = None
+        if N.shape[1] > 0:
+            M = rig(np.arange(self.n_dof)) @ N
+            M[fixed] = 0.0
+            Q, _ = np.linalg.qr(M)
+            Q[fixed] = 0.0      # zero rows stay zero (Householder leaves roundoff there)
+        self._rigid_cache = (key, Q)
+        return Q
+
     # ---- linear constraints (fe/solver.py:1341-1397) -----------------------------------------
     def add_constraint(self, terms, value=0.0):
         """General linear constraint sum_i c_i u_i = value; terms: (node, dof, coefficient).
@@ -432,6 +473,24 @@ class FESystem:
         op = self._op
         op.set_free_mask(self._free_mask(), preconditioner="jacobi" if preconditioner else None)
         b, uD = self._dirichlet_rhs(op, self.F if F is None else F)
+        # Supports that leave rigid-body motions free make Kff singular.  As in
+        # fe/solver.py:1645-1678: a load with a component along them (beyond 1e-6 of
+        # max|Ff|) is an under-constrained model; otherwise the result is the
+        # minimum-norm solution -- that component is dropped from the load (so the
+        # system is consistent) and from the PCG iterate.
+        Q = self._rigid_null_basis()
+        if Q is not None:
+            import torch
+            Q = torch.as_tensor(Q, device=b.device)
+            proj = Q.T @ b
+            bmax = float(b.abs().max())
+            if bmax > 0 and float(proj.abs().max()) > 1e-6 * bmax:
+                raise np.linalg.LinAlgError(
+                    f"solve_static: the free-DOF stiffness matrix is singular ({Q.shape[1]} rigid-body "
+                    f"mode(s) left free by the supports) and the load is not self-equilibrated against "
+                    f"them -- an under-constrained model (likely a missing boundary condition). Add "
+                    f"supports, or balance the load to zero net force and moment.")
+            b = b - Q @ proj
         n_free = self.n_dof - len(self.fixed_dofs)
         if maxiter is None:
             maxiter = 10 * n_free
@@ -452,6 +511,8 @@ class FESystem:
             if x0 is not None:
                 raise ValueError("x0 is not supported with precision='mixed'")
             x, stats = op.refined(b, tol=tol, maxiter=maxiter, delta=delta)
+        if Q is not None:
+            x = x - Q @ (Q.T @ x)
         if uD is not None:
             x = x + uD
         stats["method"] = method

@@ -32,6 +32,10 @@ PINNING (function -> independent checks):
   simp_* / sensitivities: simp_scale_grad vs central FD; closed form vs adjoint
                           solve; both vs cancellation-free FD of the forward solve
   solve_static_batched  : each column equals an independent solve
+  eigen_solve           : equals scipy.linalg.pinv(Kff) @ Ff (the minimum-norm
+                          solution) on a free and a partly supported H8 body; the
+                          count of rigid modes is 6 / 3; K U = F on the full
+                          system; an unbalanced load raises LinAlgError
   filter_weights / sensitivity_filter / oc_update / topology_optimize :
                           the reference's own outputs on the committed golden
                           fixture (filter, OC update, a 6-iteration filtered SIMP
@@ -481,6 +485,27 @@ def solve_direct(K, F, fixed_dofs, fixed_values=None):
         U[fixed] = u_fixed
     U[free] = spla.spsolve(Kff.tocsc(), Ff)
     return U
+
+
+def eigen_solve(Kff, Ff, tol=1e-8):
+    """The singular-Kff fallback of the direct solve (fe/solver.py:1583-1678):
+    symmetric eigendecomposition; eigenvalues below tol * max(max|lambda|, 1)
+    are rigid-body directions.  A load whose projection on them exceeds
+    1e-6 max|Ff| raises LinAlgError ("not self-equilibrated"); otherwise those
+    directions are dropped, which is the minimum-norm (Moore-Penrose) solution."""
+    A = Kff.toarray() if sp.issparse(Kff) else np.asarray(Kff)
+    lam, V = np.linalg.eigh(A)
+    scale = float(np.max(np.abs(lam))) if lam.size else 1.0
+    rigid = np.abs(lam) < tol * max(scale, 1.0)
+    if rigid.any():
+        res = V[:, rigid].T @ Ff
+        mx = float(np.max(np.abs(Ff))) if Ff.size else 0.0
+        if mx > 0 and np.max(np.abs(res)) > 1e-6 * mx:
+            raise np.linalg.LinAlgError(
+                f"singular free-DOF stiffness ({int(rigid.sum())} rigid-body modes) and a load that is not "
+                f"self-equilibrated against them: under-constrained model")
+    coef = np.where(rigid, 0.0, (V.T @ Ff) / np.where(rigid, 1.0, lam))
+    return V @ coef, int(rigid.sum())
 
 
 # ---------------------------------------------------------------------------

@@ -334,3 +334,72 @@ def test_refined_stagnation_is_reported(h8, gpu):
     fs.add_nodal_force(fs.mesh.nodes_on_plane(0, float(nx)), 2, -1e6)
     with pytest.raises(RuntimeError, match="stagnated"):
         fs.solve_static_device(tol=1e-18, precision="mixed", maxiter=10**7)
+
+
+def test_free_body_minimum_norm_and_under_constrained(h8, gpu):
+    """fe/tests/test_singular_system.py:61-162 and fe/tests/test_api_safety.py:
+    214-236 on H8.  A free body (no supports) under equal and opposite end
+    tension solves to the minimum-norm solution of the oracle's eigen_solve
+    (K U = F, no mean drift, the bar elongates), in both precisions; with only
+    u_x fixed on one face 3 rigid modes stay free and the result again equals
+    eigen_solve on Kff; an unbalanced load raises LinAlgError; a prescribed
+    displacement with zero load does not warn."""
+    import warnings
+    say("free body")
+    nx, ny, nz, Lx = 6, 3, 3, 4.0
+    mesh = h8.box_mesh(Lx, 1.0, 1.0, nx, ny, nz)
+    nodes, elems = o.box_mesh(Lx, 1.0, 1.0, nx, ny, nz)
+    K = o.assemble_csr(nodes, elems, o.d_solid3d(210e9, 0.3))
+    left, right = mesh.nodes_on_plane(0, 0.0), mesh.nodes_on_plane(0, Lx)
+
+    def system():
+        fs = h8.FESystem(mesh, h8.Hex8Solid3D())
+        fs.assemble_stiffness(h8.D_solid3d(h8.Material(E=210e9, nu=0.3)))
+        return fs
+
+    fs = system()
+    fs.add_nodal_force(right, 0, 1000.0)
+    fs.add_nodal_force(left, 0, -1000.0)
+    F = np.asarray(fs.F)
+    Uref, nr = o.eigen_solve(K, F)
+    assert nr == 6
+    for prec in ("fp64", "mixed"):
+        U = np.asarray(fs.solve_static(tol=1e-10, precision=prec))
+        say("free body", prec, fs.last_solve_stats["iterations"])
+        assert np.max(np.abs(U - Uref)) < 1e-8 * np.max(np.abs(Uref))
+        assert np.max(np.abs(K @ U - F)) < 1e-6 * np.max(np.abs(F))
+        for c in range(3):
+            assert abs(np.mean(U[c::3])) < 1e-6 * np.max(np.abs(U))
+        assert np.mean(U[3 * right]) - np.mean(U[3 * left]) > 0
+
+    # partial support: u_x on the x = 0 face (translations y, z and the rotation
+    # about x stay free); an end load in x is balanced against them
+    fs2 = system()
+    fs2.fix_dofs(left, [0])
+    fs2.add_nodal_force(right, 0, 1000.0)
+    U2 = np.asarray(fs2.solve_static(tol=1e-10))
+    fixed = fs2.fixed_dofs_array
+    free, fx = o.partition(K.shape[0], fixed)
+    Kff, Ff = o.reduced_system(K, np.asarray(fs2.F), free, fx)
+    uf, nr2 = o.eigen_solve(Kff, Ff)
+    assert nr2 == 3
+    assert np.max(np.abs(U2[free] - uf)) < 1e-8 * np.max(np.abs(uf))
+    assert np.all(U2[fixed] == 0.0)
+
+    # unbalanced loads on a free body
+    fs.add_nodal_force(right, 1, -10.0)
+    with pytest.raises(np.linalg.LinAlgError, match="under-constrained"):
+        fs.solve_static()
+    fs3 = system()
+    fs3.add_nodal_force(right, 0, 1000.0)
+    with pytest.raises(np.linalg.LinAlgError, match="(?i)not self-equilibrated"):
+        fs3.solve_static()
+
+    # prescribed displacement, zero load: no warning, non-zero field
+    fs4 = system()
+    fs4.fix_dofs(left, [0, 1, 2])
+    fs4.fix_dofs(right, [0], value=1e-4)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        U4 = np.asarray(fs4.solve_static())
+    assert np.any(U4)

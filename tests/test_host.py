@@ -153,3 +153,37 @@ def test_add_constraint_argument_checks():
     assert fs._constraints == ()
     with pytest.raises(ValueError, match="no master node"):
         fs.tie([0], [5], [0], offset=[10.0, 0.0, 0.0])
+
+
+def test_rigid_null_basis_spans_null_space_of_Kff():
+    """The rigid motions left free by the supports span the null space of Kff
+    (checked against the oracle's assembled K and the dimension of its null
+    space by eigenvalues, the test fe/solver.py:1583-1678 applies)."""
+    import h8pcg
+    mesh = h8pcg.box_mesh(3.0, 1.0, 1.0, 3, 2, 2)
+    nodes, elems = o.box_mesh(3.0, 1.0, 1.0, 3, 2, 2)
+    K = o.assemble_csr(nodes, elems, o.d_solid3d(1.0, 0.3)).toarray()
+
+    def null_dim(fixed):
+        free, _ = o.partition(K.shape[0], np.asarray(fixed, dtype=np.int64))
+        lam = np.linalg.eigvalsh(K[np.ix_(free, free)])
+        return int(np.sum(np.abs(lam) < 1e-8 * np.abs(lam).max())), free
+
+    x0 = mesh.nodes_on_plane(0, 0.0)
+    cases = [([], []), (x0, [0]), (x0, [0, 1, 2]), ([0], [0, 1, 2]), (mesh.select_nodes(x=0.0, y=0.0), [0, 1, 2])]
+    dims = []
+    for nodes_fixed, comps in cases:
+        fs = h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D())
+        if len(nodes_fixed):
+            fs.fix_dofs(nodes_fixed, comps)
+        Q = fs._rigid_null_basis()
+        r = 0 if Q is None else Q.shape[1]
+        nd, free = null_dim(fs.fixed_dofs_array)
+        assert r == nd
+        dims.append(r)
+        if Q is not None:
+            assert np.allclose(Q.T @ Q, np.eye(r), atol=1e-12)
+            assert np.all(Q[fs.fixed_dofs_array] == 0.0)
+            assert np.max(np.abs(K[np.ix_(free, free)] @ Q[free])) < 1e-12 * np.abs(K).max()
+        assert fs._rigid_null_basis() is Q          # cached per fixed set
+    assert dims == [6, 3, 0, 3, 1]

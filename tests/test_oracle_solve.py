@@ -156,3 +156,57 @@ def test_batched_columns_equal_single_solves():
     for j in range(3):
         u1 = o.solve_direct(c["K"], Fb[:, j], c["fixed_dofs"], vals)
         assert np.linalg.norm(U[:, j] - u1) / np.linalg.norm(u1) < 1e-12
+
+
+def _free_body(nx=6, ny=3, nz=3, Lx=4.0):
+    nodes, elems = o.box_mesh(Lx, 1.0, 1.0, nx, ny, nz)
+    K = o.assemble_csr(nodes, elems, o.d_solid3d(210e9, 0.3))
+    F = np.zeros(K.shape[0])
+    left, right = o.nodes_on_plane(nodes, 0, 0.0), o.nodes_on_plane(nodes, 0, Lx)
+    F[3 * right] += 1000.0 / len(right)
+    F[3 * left] -= 1000.0 / len(left)
+    return nodes, K, F, left, right
+
+
+def test_eigen_solve_free_body_is_minimum_norm():
+    """fe/tests/test_singular_system.py:61-117 on a free H8 body: no Dirichlet
+    DOFs, equal and opposite x-tension on the end faces.  Six rigid modes; the
+    result is pinv(K) F (independent routine), balances K U = F, carries no
+    mean translation and elongates the bar."""
+    import scipy.linalg as sla
+    nodes, K, F, left, right = _free_body()
+    U, nr = o.eigen_solve(K, F)
+    assert nr == 6
+    Up = sla.pinv(K.toarray()) @ F
+    assert np.max(np.abs(U - Up)) < 1e-8 * np.max(np.abs(Up))
+    assert np.max(np.abs(K @ U - F)) < 1e-6 * np.max(np.abs(F))
+    for c in range(3):
+        assert abs(np.mean(U[c::3])) < 1e-9 * np.max(np.abs(U))
+    assert np.mean(U[3 * right]) - np.mean(U[3 * left]) > 0
+
+
+def test_eigen_solve_partial_support_and_unbalanced_load():
+    """Only u_x fixed on the x=0 face: translations y, z and the rotation about x
+    stay free (3 rigid modes); an x-load is balanced against them.  A y-load on
+    the free body is not balanced (fe/tests/test_singular_system.py:143-162)."""
+    import scipy.linalg as sla
+    nodes, K, F, left, right = _free_body()
+    fixed = 3 * left
+    F2 = np.zeros_like(F)
+    F2[3 * right] = 1000.0 / len(right)
+    free, fx = o.partition(K.shape[0], fixed)
+    Kff, Ff = o.reduced_system(K, F2, free, fx)
+    uf, nr = o.eigen_solve(Kff, Ff)
+    assert nr == 3
+    up = sla.pinv(Kff.toarray()) @ Ff
+    assert np.max(np.abs(uf - up)) < 1e-8 * np.max(np.abs(up))
+    F3 = np.zeros_like(F)
+    F3[3 * right + 1] = 1000.0 / len(right)
+    with pytest.raises(np.linalg.LinAlgError, match="not self-equilibrated"):
+        o.eigen_solve(K, F3)
+    # a planted mistake (keeping the rigid directions instead of dropping them)
+    # would leave the answer off the minimum-norm solution
+    lam, V = np.linalg.eigh(K.toarray())
+    bad = V @ ((V.T @ F) / np.where(np.abs(lam) < 1e-8 * lam.max(), lam.max(), lam) + np.where(
+        np.abs(lam) < 1e-8 * lam.max(), 1e-3, 0.0))
+    assert np.max(np.abs(bad - sla.pinv(K.toarray()) @ F)) > 1e-6
