@@ -11,11 +11,12 @@ from .elements import Hex8Solid3D, Hex8SolidBbar
 from . import coefficients
 from .fields import FEField
 from .facet_loads import list_facets, boundary_faces, face_load_shares
-from .solver import FESystem
+from .solver import FESystem, fesystem_solve_static_torch
 from ._lib import NotSPDError, H8LibraryMissing
 from .batched import solve_static_batched, solve_many
 
 __all__ = ["Material", "D_solid3d", "Mesh", "box_mesh", "structured_grid", "Hex8Solid3D", "Hex8SolidBbar", "coefficients",
-           "FEField", "list_facets", "boundary_faces", "face_load_shares", "FESystem", "NotSPDError",
+           "FEField", "list_facets", "boundary_faces", "face_load_shares", "FESystem", "fesystem_solve_static_torch",
+           "NotSPDError",
            "H8LibraryMissing", "solve_static_batched", "solve_many"]
 __version__ = "0.1.0"

@@ -643,6 +643,32 @@ class FESystem:
         return self.field(out, label="reaction")
 
 
+def fesystem_solve_static_torch(sysobj, method="cg", device=None, tol=1e-8, max_iter=None):
+    """fe/torch_sparse_solver.py:268-314: the free-DOF solve of an assembled
+    FESystem through the GPU path, returned as a full (n_dof,) numpy array, zero
+    at every fixed DOF.  method='cg' is the Jacobi-preconditioned CG of
+    fe/torch_sparse_solver.py:165-262 (the same recurrence and stopping rule,
+    max_iter 10 * free DOFs); here it runs matrix-free in the HIP kernels.  It
+    raises RuntimeError when it does not converge and ValueError on a
+    non-positive diagonal, as the reference does.  `device` is accepted for
+    signature compatibility: the solve runs on the FESystem's device.  The
+    reference solves Ff = F[free] with no Dirichlet correction; a system with
+    non-zero prescribed values raises NotImplementedError instead (as
+    fe/solver.py:1848-1856 does for its torch backend), and method='dense' (a
+    dense torch factorisation) is not part of this backend."""
+    if method == "dense":
+        raise NotImplementedError("fesystem_solve_static_torch(method='dense') is not supported on the HIP "
+                                  "backend; use method='cg'.")
+    if method != "cg":
+        raise ValueError(f"fesystem_solve_static_torch: unknown method={method!r}, expected 'cg' or 'dense'")
+    fixed = sysobj.fixed_dofs_array
+    if np.any(sysobj._fixed_dof_values_array(fixed)):
+        raise NotImplementedError("fesystem_solve_static_torch: non-zero prescribed displacements are not "
+                                  "supported (the reference path drops them); use FESystem.solve_static().")
+    x = sysobj.solve_static_device(method="pcg", tol=tol, maxiter=max_iter)
+    return x.cpu().numpy()
+
+
 def _proportional(A, B):
     """c with B = c A (ValueError if not proportional)."""
     piv = np.unravel_index(np.argmax(np.abs(A)), A.shape)

@@ -71,3 +71,46 @@ def test_uneven_instances_and_masks(gpu):
     fs.F = F[:, 0].copy()
     fs.solve_static(tol=1e-10)
     assert abs(fs.last_solve_stats["iterations"] - st["iterations"][2]) <= 2
+
+
+def test_batched_dirichlet_reference_checks(gpu):
+    """fe/tests/test_batched_nonhomogeneous_dirichlet.py:53-147 on H8: the
+    prescribed value appears in every column; a one-column batch equals
+    solve_static; an explicit value=0.0 gives bit-identical results to an
+    omitted value."""
+    import h8pcg
+    from h8pcg.batched import solve_static_batched
+    nx, ny, nz = 8, 3, 2
+    mesh = h8pcg.box_mesh(2.0, 1.0, 0.5, nx, ny, nz)
+    left, right = mesh.nodes_on_plane(0, 0.0), mesh.nodes_on_plane(0, 2.0)
+
+    def build(value=None, zero_explicit=False):
+        fs = h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D())
+        fs.assemble_stiffness(h8pcg.D_solid3d(h8pcg.Material(E=2.1e11, nu=0.3)))
+        if zero_explicit:
+            fs.fix_dofs(left, [0, 1, 2], value=0.0)
+        else:
+            fs.fix_dofs(left, [0, 1, 2])
+        if value is not None:
+            fs.fix_dofs(right, [0], value=value)
+        return fs
+
+    rng = np.random.default_rng(0)
+    fs = build(0.015)
+    Fb = rng.normal(size=(fs.n_dof, 4)) * 1000.0
+    U = solve_static_batched(fs, Fb, tol=1e-12)
+    assert np.all(U[3 * right, :] == 0.015)
+
+    F = np.zeros(fs.n_dof)
+    F[3 * right + 1] = -500.0 / len(right)
+    fs1 = build(0.01)
+    U1 = solve_static_batched(fs1, F[:, None], tol=1e-10)
+    fs_ref = build(0.01)
+    fs_ref.F[:] = F
+    Uref = np.asarray(fs_ref.solve_static(tol=1e-10))
+    assert np.max(np.abs(U1[:, 0] - Uref)) < 1e-8 * np.max(np.abs(Uref))
+
+    Fz = rng.normal(size=(fs.n_dof, 3)) * 1000.0
+    Ua = solve_static_batched(build(), Fz)
+    Ub = solve_static_batched(build(zero_explicit=True), Fz)
+    assert np.array_equal(Ua, Ub)
