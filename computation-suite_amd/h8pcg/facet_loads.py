@@ -15,8 +15,44 @@ _XA = np.array([-1.0, 1.0, 1.0, -1.0])
 _EA = np.array([-1.0, -1.0, 1.0, 1.0])
 
 
-def list_facets(formulation=None):
-    return FACE_TABLE
+# natural coordinates of the Hex8 corners (fe/elements/solids.py:135-137 order)
+_HEX8_NAT = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                      [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+
+
+def _face_map(loc):
+    """(a, b) -> (xi, eta, zeta) of a face: the bilinear interpolation of its
+    corners' natural coordinates, listed in `loc` order, which is affine on a face
+    of the parent cube.  Facet corner k sits at (_XA[k], _EA[k])."""
+    C = _HEX8_NAT[list(loc)]
+
+    def nat(a, b):
+        N = 0.25 * (1 + a * _XA) * (1 + b * _EA)
+        return tuple(float(v) for v in N @ C)
+    return nat
+
+
+def quad4_shape(p):
+    """N (4,) and dN/d(a, b) (2, 4) of the bilinear facet at (a, b)."""
+    a, b = p
+    N = 0.25 * (1 + a * _XA) * (1 + b * _EA)
+    return N, np.vstack([0.25 * _XA * (1 + b * _EA), 0.25 * _EA * (1 + a * _XA)])
+
+
+_FACET_SHAPE_FNS = {"quad4": quad4_shape}
+_FACETS = [(loc, fam, _face_map(loc)) for loc, fam in FACE_TABLE]
+
+
+def list_facets(formulation):
+    """(local node indices, family, natural-coordinate map) of every face of a
+    Hex8 parent (fe/facet_loads.py:191-211): `conn[list(idx)]` gives the global
+    node tuple add_consistent_facet_load expects.  ValueError for any other
+    element type, as the reference (whose table is keyed by the exact class)."""
+    from .elements import Hex8Solid3D
+    if type(formulation) is not Hex8Solid3D:
+        raise ValueError(f"facet_loads: no facet table for {type(formulation).__name__} -- this backend's "
+                         f"facet table covers Hex8Solid3D.")
+    return list(_FACETS)
 
 
 def _gauss(n):

@@ -195,6 +195,9 @@ class FESystem:
             raise ValueError(
                 f"assemble_stiffness: unknown method={method!r} -- expected "
                 f"'full' (default), 'reduced', or 'hourglass_stabilized'.")
+        if method != "full" and vectorized:
+            raise ValueError(f"assemble_stiffness: vectorized=True only supports method='full' "
+                             f"(got method={method!r}).")
         if method != "full":
             raise NotImplementedError(f"assemble_stiffness(method={method!r}) is not available on backend='hip'")
         kw = dict(kwargs)

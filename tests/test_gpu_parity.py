@@ -25,6 +25,11 @@ def h8(gpu):
 
 
 def test_ke_kernel_matches_oracle(h8, gpu):
+    """The batched element kernel (J, detJ and the physical shape gradients in
+    registers, fe/vectorized_assembly.py:94-214) against the oracle's per-element
+    B^T D B sum on distorted hexes: the role of fe/tests/test_mesh_transform.py:
+    67-85 and fe/tests/test_torch_vectorized_assembly.py:127-159 (batched shape
+    gradients and stiffness equal the per-element ones)."""
     say("ke kernel")
     rng = np.random.default_rng(0)
     base = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)

@@ -102,6 +102,9 @@ def test_validation_errors():
         fs.solve_static()
     with pytest.raises(ValueError, match="unknown method"):
         fs.assemble_stiffness(np.eye(6), method="bogus")
+    # fe/tests/test_vectorized_assembly.py:98-103
+    with pytest.raises(ValueError, match="vectorized=True only supports"):
+        fs.assemble_stiffness(np.eye(6), method="reduced", vectorized=True)
 
 
 def test_coefficient_factoring():
@@ -187,3 +190,33 @@ def test_rigid_null_basis_spans_null_space_of_Kff():
             assert np.max(np.abs(K[np.ix_(free, free)] @ Q[free])) < 1e-12 * np.abs(K).max()
         assert fs._rigid_null_basis() is Q          # cached per fixed set
     assert dims == [6, 3, 0, 3, 1]
+
+
+def test_hex8_face_tables():
+    """fe/tests/test_facet_loads.py:101-119: on every face's natural-coordinate
+    map the element's shape functions vanish at the other four nodes and equal
+    the bilinear facet's at the face's nodes, in the table's order."""
+    import h8pcg
+    from h8pcg import facet_loads
+    elem = h8pcg.Hex8Solid3D()
+    rng = np.random.default_rng(8)
+    facets = facet_loads.list_facets(elem)
+    assert len(facets) == 6
+    for local_idx, family, natcoord_map in facets:
+        others = [i for i in range(8) if i not in local_idx]
+        shape_fn = facet_loads._FACET_SHAPE_FNS[family]
+        for _ in range(8):
+            a, b = rng.uniform(-0.9, 0.9, 2)
+            N, _ = elem.shape_and_derivs(natcoord_map(a, b))
+            Nf, _ = shape_fn((a, b))
+            assert np.max(np.abs(N[others])) < 1e-10
+            assert np.max(np.abs(N[list(local_idx)] - Nf)) < 1e-9
+    # the outward normal of each face (cross product of the facet's tangents
+    # in table order) points away from the element centre
+    cube = np.array(facet_loads._HEX8_NAT)
+    for local_idx, _, nat in facets:
+        P = np.array([nat(a, b) for a, b in [(-1, -1), (1, -1), (-1, 1)]])
+        n = np.cross(P[1] - P[0], P[2] - P[0])
+        assert np.dot(n, cube[list(local_idx)].mean(axis=0)) > 0
+    with pytest.raises(ValueError, match="no facet table"):
+        facet_loads.list_facets(h8pcg.Hex8SolidBbar())

@@ -36,6 +36,8 @@ def test_facet_shares():
 
 
 def test_assembly_equals_dense_loop():
+    """fe/tests/test_vectorized_assembly.py:62-95: the vectorised assembly equals
+    the per-element loop on a distorted hex patch."""
     nodes, elems = o.box_mesh(2.0, 1.0, 1.0, 2, 1, 2)
     nodes = nodes + 0.03 * np.random.default_rng(0).uniform(-1, 1, nodes.shape)
     D = o.d_solid3d(1.0, 0.3)
@@ -46,6 +48,8 @@ def test_assembly_equals_dense_loop():
 
 
 def test_pcg_matches_direct_and_golden(golden):
+    """fe/tests/test_iterative_solvers.py:110-116: CG agrees with the direct solve;
+    plus the reference's own iteration count and history on the fixture."""
     c = o.cantilever(20, 4, 4)
     U_direct = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
     hist = []
@@ -78,6 +82,7 @@ def test_pcg_golden_nodal_and_heterogeneous(golden):
 
 
 def test_pcg_errors():
+    """fe/tests/test_iterative_solvers.py:118-128: an indefinite matrix raises."""
     A = np.diag([1.0, -2.0, 3.0])
     import scipy.sparse as sp
     with pytest.raises(np.linalg.LinAlgError):
@@ -120,6 +125,7 @@ def _simp_solve(nodes, elems, ke, rho, F, fixed, p=3.0, rmin=1e-9):
 
 
 def test_simp_scale_grad_fd():
+    """fe/tests/test_topopt.py:59-66."""
     rho = np.array([0.2, 0.5, 0.8])
     h = 1e-6
     fd = (o.simp_scale(rho + h, 3.0, 1e-3) - o.simp_scale(rho - h, 3.0, 1e-3)) / (2 * h)
@@ -127,6 +133,8 @@ def test_simp_scale_grad_fd():
 
 
 def test_compliance_gradient_closed_form_vs_adjoint_vs_fd():
+    """fe/tests/test_topopt.py:72-122: closed form vs central differences of the
+    forward solve, and vs the adjoint sensitivity."""
     nodes, elems, D, ke, rho, F, fixed = _simp_case()
     u, K = _simp_solve(nodes, elems, ke, rho, F, fixed)
     g = o.compliance_sensitivity_closed_form(ke, elems, u, rho, 3.0, 1e-9)
@@ -148,6 +156,8 @@ def test_compliance_gradient_closed_form_vs_adjoint_vs_fd():
 
 
 def test_batched_columns_equal_single_solves():
+    """fe/tests/test_batched_solve.py:66-115: each column of a batched solve
+    equals an independent solve."""
     c = o.cantilever(6, 2, 2)
     rng = np.random.default_rng(0)
     Fb = rng.normal(size=(c["K"].shape[0], 3))
