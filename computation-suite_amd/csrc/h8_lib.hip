@@ -648,12 +648,29 @@ template <typename T> static int res_threads(const GridDev& d) {
     return fp64_products<T>(d) ? ResThreads<T, double>::value : ResThreads<T, T>::value;
 }
 
+// Most elements touched by one part of the split a N / P of N nodes: its nodes
+// plus one, less one on each side that lies on the grid boundary.
+static int res_axis_elems(int N, int P) {
+    int m = 0;
+    for (int a = 0; a < P; ++a) {
+        const int s = int((long long)(a + 1) * N / P - (long long)a * N / P);
+        m = std::max(m, s + 1 - (a == 0) - (a == P - 1));
+    }
+    return m;
+}
+
 // Brick split for the resident solver: at most one block per CU, LDS within
-// 158 KiB (the kernel's static LDS stays below 2 KiB of the CU's 160); the
-// fewest element batches (elements / threads, rounded up) per block, then the
-// fewest elements, then the fewest blocks.
+// 158 KiB (the kernel's static LDS stays below 2 KiB of the CU's 160).  Plans
+// are ranked by the element batches (elements / threads, rounded up) of the
+// busiest block, then by W = its elements + 3 x its shell nodes on faces
+// across j, then by the fewest blocks.  j is the fastest axis of the node
+// numbering, so a face across j is one scattered 12- or 24-byte word group per
+// node in the z exchange, where faces across i or k are runs along j.  On
+// MI355X at 128x32x32 (FP32, 100-iteration launches, s_memrealtime stamps):
+// 32x1x8 bricks (W = 800) 12.1 us per iteration, 16x2x8 (996) 12.4,
+// 26x3x3 (1410) 13.3, 16x4x4 (1455) 13.6.
 template <typename T>
-static bool res_plan(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
+static bool res_plan_search(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
     constexpr long long LDS_MAX = 158 * 1024;
     bool found = false;
     long long bcost = 0;
@@ -687,7 +704,10 @@ static bool res_plan(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
                 if (off > LDS_MAX || c.nhalo >= (1 << 15)) continue;   // (index decoding needs < 2^16)
                 c.lds_bytes = int(off);
                 const int nb = PI * PJ * PK;
-                const long long cost = (long long)((c.nelem + nt - 1) / nt) * (1LL << 32) + c.nelem;
+                const long long emax = (long long)res_axis_elems(d.NI, PI) * res_axis_elems(d.NJ, PJ) *
+                                       res_axis_elems(d.NK, PK);
+                const long long jface = (long long)std::min(PJ - 1, 2) * c.HI * (c.BK + 2);
+                const long long cost = ((emax + nt - 1) / nt) * (1LL << 32) + emax + 3 * jface;
                 if (!found || cost < bcost || (cost == bcost && nb < bnb)) {
                     found = true;
                     bcost = cost;
@@ -696,6 +716,32 @@ static bool res_plan(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
                 }
             }
     return found;
+}
+
+// res_plan_search, remembered per (grid, threads, block limit, word size): the
+// search visits thousands of splits and runs before every resident launch (one
+// per FP32 phase of a refined solve).
+template <typename T>
+static bool res_plan(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
+    struct Entry {
+        int NI, NJ, NK, nt, max_blocks;
+        bool found;
+        ResGeom geom;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> memo;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const Entry& e : memo)
+        if (e.NI == d.NI && e.NJ == d.NJ && e.NK == d.NK && e.nt == nt && e.max_blocks == max_blocks) {
+            best = e.geom;
+            return e.found;
+        }
+    Entry e{d.NI, d.NJ, d.NK, nt, max_blocks, false, ResGeom{}};
+    e.found = res_plan_search<T>(d, nt, max_blocks, e.geom);
+    if (memo.size() >= 64) memo.erase(memo.begin());
+    memo.push_back(e);
+    best = e.geom;
+    return e.found;
 }
 
 // The resident solver runs the iterations when asked for (H8_VARIANT_RESIDENT)
