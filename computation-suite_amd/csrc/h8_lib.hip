@@ -761,7 +761,7 @@ static bool use_resident(const GridDev& d) {
 template <typename T, typename TA, bool REFINED>
 static int run_resident_k(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
                           const ZRing<T>& zr, PcgState* st, double* partials, hipStream_t s, PcgState& hst,
-                          long long& launches, hipEvent_t ev0, hipEvent_t ev1) {
+                          long long& launches, hipEvent_t ev0, hipEvent_t ev1, bool sync) {
     constexpr int NT = ResThreads<T, TA>::value;
     const void* fn = (const void*)k_resident<T, NT, REFINED, TA>;
     ResGeom G;
@@ -797,6 +797,7 @@ static int run_resident_k(const GridDev& d, const T* s_e, const T* K0, const T* 
     HIPCHK(le);
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
     ++launches;
+    if (!sync) return H8_OK;      // the caller reads the state and the error word
     unsigned err = 0;
     HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&err, A.err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -808,12 +809,13 @@ static int run_resident_k(const GridDev& d, const T* s_e, const T* K0, const T* 
 template <typename T, bool REFINED>
 static int run_resident(const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0, T* p1,
                         const ZRing<T>& zr, PcgState* st, double* partials, hipStream_t s, PcgState& hst,
-                        long long& launches, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+                        long long& launches, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                        bool sync = true) {
     if (fp64_products<T>(d))
         return run_resident_k<T, double, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hst, launches,
-                                                  ev0, ev1);
+                                                  ev0, ev1, sync);
     return run_resident_k<T, T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hst, launches, ev0,
-                                         ev1);
+                                         ev1, sync);
 }
 
 // Runs captured chunks until the device-side status leaves RUNNING.
@@ -1170,6 +1172,77 @@ extern "C" int h8_pcg_constrained_f64(const h8_grid* g, const double* s_e, const
     return H8_OK;
 }
 
+// FP32 phases of the refined solve on the resident solver.  A phase is one
+// resident launch with the fold and the FP64 residual queued behind it (both
+// act only when the phase stopped for a replacement); its state and error word
+// are copied to a pinned host slot and an event is recorded.  Phase j + 1 is
+// queued before the host waits for phase j, so the host's reading and
+// queueing overlap the GPU's work; a phase queued after the solve ended does
+// nothing (the resident kernel returns at once unless the state is RUNNING).
+struct RefineSlot {
+    PcgState st;
+    unsigned err;
+};
+
+static int queue_refine_phase(const GridDev& d, const float* s_e32, const float* K032, const WsRef& w,
+                              const ZRing<float>& zr, double* u, const NodeArgs<double>& R, hipStream_t s,
+                              PcgState& hst, long long& launches, RefineSlot* slot, hipEvent_t ev) {
+    int rc = run_resident<float, true>(d, s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, zr, w.st, w.partials, s,
+                                       hst, launches, nullptr, nullptr, false);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fold, dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s, d.ndof, u, w.x,
+                       (const float*)w.p0, (const float*)w.p1, (const PcgState*)w.st);
+    LAUNCHCHK();
+    NodeArgs<double> Rq = R;
+    Rq.only_on_replace = 1;
+    rc = launch_op<double, MODE_RESID>(d, Rq, s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(&slot->st, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&slot->err, reinterpret_cast<unsigned*>(w.partials + RES_RING_DOUBLES), sizeof(unsigned),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(ev, s));
+    return H8_OK;
+}
+
+static int refine_resident_phases(const GridDev& d, const float* s_e32, const float* K032, const WsRef& w,
+                                  const ZRing<float>& zr, double* u, const NodeArgs<double>& R, hipStream_t s,
+                                  PcgState& hst, long long& launches, RefineSlot* hs, hipEvent_t* ev) {
+    int cur = 0;
+    int rc = queue_refine_phase(d, s_e32, K032, w, zr, u, R, s, hst, launches, &hs[0], ev[0]);
+    if (rc) return rc;                                   // RES_DECLINED: nothing ran
+    for (;;) {
+        rc = queue_refine_phase(d, s_e32, K032, w, zr, u, R, s, hst, launches, &hs[cur ^ 1], ev[cur ^ 1]);
+        if (rc == RES_DECLINED) return fail(H8_ERR_HIP, "resident solver: a cooperative launch was refused");
+        if (rc) return rc;
+        HIPCHK(hipEventSynchronize(ev[cur]));
+        hst = hs[cur].st;
+        if (hs[cur].err) return fail(H8_ERR_HIP, "resident solver: a grid-wide sum timed out");
+        if (hst.status != ST_RUNNING) {
+            --launches;                                  // the phase queued last did nothing
+            return H8_OK;
+        }
+        cur ^= 1;
+    }
+}
+
+static int refine_resident(const GridDev& d, const float* s_e32, const float* K032, const WsRef& w,
+                           const ZRing<float>& zr, double* u, const NodeArgs<double>& R, hipStream_t s,
+                           PcgState& hst, long long& launches) {
+    RefineSlot* hs = nullptr;
+    HIPCHK(hipHostMalloc((void**)&hs, 2 * sizeof(RefineSlot), hipHostMallocDefault));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = H8_OK;
+    for (int i = 0; i < 2 && rc == H8_OK; ++i)
+        if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+            rc = fail(H8_ERR_HIP, "hipEventCreateWithFlags failed");
+    if (rc == H8_OK) rc = refine_resident_phases(d, s_e32, K032, w, zr, u, R, s, hst, launches, hs, ev);
+    (void)hipStreamSynchronize(s);       // the phase queued last (and, on an error, all queued work)
+    for (int i = 0; i < 2; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    (void)hipHostFree(hs);
+    return rc;
+}
+
 extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const double* K064,
                                const float* s_e32, const float* K032, const double* dinv64,
                                const double* f, double* u, double tol, int64_t maxiter, double delta,
@@ -1237,7 +1310,6 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
         // first residual (u = 0): r32 = float(mask f), rz_old, rmax
         rc = launch_op<double, MODE_RESID>(d, R, s);
         if (rc) return rc;
-        ++resid_calls;
         HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -1250,6 +1322,10 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     const void* ptrs[] = {s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
     std::memcpy(key.ptr, ptrs, sizeof ptrs);
     key.chunk = chunk_for(maxiter);
+    if (hst.status == ST_RUNNING && use_resident<float>(d)) {
+        rc = refine_resident(d, s_e32, K032, w, zr, u, R, s, hst, launches);
+        if (rc && rc != RES_DECLINED) return rc;     // declined: the streaming loop below runs
+    }
     while (hst.status == ST_RUNNING) {
         rc = run_chunks<float, true>(key, d, s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, zr, w.st,
                                      w.partials, s, hst, launches);
@@ -1258,15 +1334,15 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
         // fold the FP32 iterate (with the pending x += alpha p of the last
         // direction, p = p[k-1], the update kernel having advanced k) into u,
         // then replace the residual in FP64
-        hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, w.x,
-                           ((hst.k - 1) & 1) ? (const float*)w.p1 : (const float*)w.p0, (const PcgState*)w.st);
+        hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, w.x, (const float*)w.p0,
+                           (const float*)w.p1, (const PcgState*)w.st);
         LAUNCHCHK();
         rc = launch_op<double, MODE_RESID>(d, R, s);
         if (rc) return rc;
-        ++resid_calls;
         HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
+    resid_calls = hst.outer;      // residual evaluations (the first one and every replacement)
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;

@@ -65,6 +65,9 @@ k_mfma(NodeArgs<T> A) {
     if constexpr (is_dir(MODE)) {
         if (A.st->status != ST_RUNNING) return;
     }
+    if constexpr (MODE == MODE_RESID) {
+        if (A.only_on_replace && A.st->status != ST_REPLACE) return;
+    }
     int j0, i0, k0;
     tile_origin(g, xcd_local_tile(blockIdx.x, gridDim.x), TJ, TI, TK, j0, i0, k0);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -51,6 +51,7 @@ struct NodeArgs {
     const float* dinv32;  // RESID
     PcgState* st;
     double* partials;
+    int only_on_replace;  // RESID: every block returns at once unless st asks for a replacement
 };
 
 // Corner whose offsets are the XOR of corners a and b, and the sign of
@@ -410,6 +411,9 @@ __device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& f
     p_old = nullptr;
     alpha_prev = T(0);
     pending = 0;
+    if constexpr (MODE == MODE_RESID) {
+        if (A.only_on_replace && A.st->status != ST_REPLACE) return false;
+    }
     if constexpr (is_dir(MODE)) {
         if (A.st->status != ST_RUNNING) return false;
         beta = T(A.st->beta);

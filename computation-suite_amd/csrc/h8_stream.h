@@ -159,11 +159,15 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     }
 }
 
-// u64 += double(x32) (+ alpha p32 when the last direction's update is still
-// pending); x32 = 0: fold the FP32 iterate into the FP64 solution.
+// When the FP32 loop stopped for a residual replacement: u64 += double(x32)
+// (+ alpha p32 when the last direction's update is still pending, p32 the
+// direction of iteration k - 1); x32 = 0: fold the FP32 iterate into the FP64
+// solution.  In any other state it changes nothing.
 __global__ void __launch_bounds__(STREAM_THREADS)
-k_fold(long long n, double* __restrict__ u, float* __restrict__ x32, const float* __restrict__ p32,
-       const PcgState* st) {
+k_fold(long long n, double* __restrict__ u, float* __restrict__ x32, const float* __restrict__ p0,
+       const float* __restrict__ p1, const PcgState* st) {
+    if (st->status != ST_REPLACE) return;
+    const float* __restrict__ p32 = ((st->k - 1) & 1) ? p1 : p0;
     const double alpha = st->pending ? st->alpha : 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
