@@ -323,18 +323,32 @@ extern "C" int h8_spmv_f32(const h8_grid* g, const float* s_e, const float* K0,
     return launch_node_plain<float>(d, s_e, K0, u, y, (hipStream_t)stream);
 }
 
-extern "C" int h8_diag_f64(const h8_grid* g, const double* s_e, const double* K0,
-                           const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream) {
+static int diag_launch(const h8_grid* g, const double* s_e, const double* K0, const uint8_t* free_mask,
+                       double* dinv, int32_t* d_flag, hipStream_t s, bool clear) {
     GridDev d;
     int rc = make_grid(g, d);
     if (rc) return rc;
     if (!s_e || !K0 || !free_mask || !dinv || !d_flag) return fail(H8_ERR_ARG, "NULL pointer");
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, (int*)d_flag, 0);
-    LAUNCHCHK();
+    if (clear) {
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, (int*)d_flag, 0);
+        LAUNCHCHK();
+    }
     hipLaunchKernelGGL(k_diag, dim3(stream_blocks(d.nnodes)), dim3(STREAM_THREADS), 0, s, d, s_e, K0,
                        free_mask, dinv, (int*)d_flag);
     LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_diag_async_f64(const h8_grid* g, const double* s_e, const double* K0,
+                                 const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream) {
+    return diag_launch(g, s_e, K0, free_mask, dinv, d_flag, (hipStream_t)stream, false);
+}
+
+extern "C" int h8_diag_f64(const h8_grid* g, const double* s_e, const double* K0,
+                           const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    int rc = diag_launch(g, s_e, K0, free_mask, dinv, d_flag, s, true);
+    if (rc) return rc;
     int32_t flag = 0;
     HIPCHK(hipMemcpyAsync(&flag, d_flag, sizeof flag, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -431,6 +445,17 @@ extern "C" int h8_cast_f64_f32(const double* x, float* y, int64_t n, void* strea
     if (!x || !y) return fail(H8_ERR_ARG, "NULL pointer");
     hipLaunchKernelGGL(k_cast_f64_f32, dim3(stream_blocks(n)), dim3(STREAM_THREADS), 0,
                        (hipStream_t)stream, (long long)n, x, y);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h8_masked_sub_cols_f64(const double* a, const double* b, const double* dinv, double* out,
+                                      int64_t n, int64_t ncols, void* stream) {
+    if (n < 0 || ncols < 0) return fail(H8_ERR_ARG, "n or ncols < 0");
+    if (n == 0 || ncols == 0) return H8_OK;
+    if (!a || !dinv || !out) return fail(H8_ERR_ARG, "NULL pointer");
+    hipLaunchKernelGGL(k_masked_sub_cols, dim3(stream_blocks(n * ncols)), dim3(STREAM_THREADS), 0,
+                       (hipStream_t)stream, (long long)n, (long long)ncols, a, b, dinv, out);
     LAUNCHCHK();
     return H8_OK;
 }

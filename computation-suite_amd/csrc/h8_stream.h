@@ -203,6 +203,19 @@ k_masked_sub(long long n, const double* __restrict__ a, const double* __restrict
         out[i] = (dinv[i] != 0.0) ? (b ? a[i] - b[i] : a[i]) : 0.0;
 }
 
+// The same for ncols columns of length n stored one after another (a, out);
+// b (length n, may be NULL) and the mask are shared by every column.
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_masked_sub_cols(long long n, long long ncols, const double* __restrict__ a, const double* __restrict__ b,
+                  const double* __restrict__ dinv, double* __restrict__ out) {
+    const long long total = n * ncols;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long i = t % n;
+        out[t] = (dinv[i] != 0.0) ? (b ? a[t] - b[i] : a[t]) : 0.0;
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(STREAM_THREADS)
 k_fill(long long n, T* __restrict__ x, T v) {

@@ -71,6 +71,7 @@ SIGNATURES = {
     "h8_ke_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
     "h8_ke_bbar_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
     "h8_diag_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
+    "h8_diag_async_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
     "h8_spmv_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),
     "h8_spmv_f32": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),
     "h8_pcg_workspace_bytes": (C.c_size_t, [_GRID, C.c_int]),
@@ -81,6 +82,7 @@ SIGNATURES = {
     "h8_simp_scale_f64": (C.c_int, [_P, _D, _D, _I64, _P, _P, _P]),
     "h8_cast_f64_f32": (C.c_int, [_P, _P, _I64, _P]),
     "h8_masked_sub_f64": (C.c_int, [_P, _P, _P, _P, _I64, _P]),
+    "h8_masked_sub_cols_f64": (C.c_int, [_P, _P, _P, _P, _I64, _I64, _P]),
     "h8_time_kernels": (C.c_int, [_GRID, C.c_int, _P, _P, _P, _P, C.c_size_t, C.c_int, _P, _P]),
     "h8_batched_workspace_bytes": (C.c_size_t, [C.POINTER(H8Instance), C.c_int32, C.c_int64]),
     "h8_pcg_batched_f64": (C.c_int, [C.POINTER(H8Instance), C.c_int32, C.c_int64, C.c_int64, C.c_int64, _P, _P,

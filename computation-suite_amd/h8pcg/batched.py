@@ -27,7 +27,8 @@ def _dirichlet_columns(fs, F_batch):
     op = fs._op
     fixed = fs.fixed_dofs_array
     u_fixed = fs._fixed_dof_values_array(fixed)
-    F_t = torch.as_tensor(np.asarray(F_batch, dtype=float), device=op.device)
+    # columns as rows of one (k, n_dof) array: one upload and one masking launch
+    F_t = torch.as_tensor(np.ascontiguousarray(np.asarray(F_batch, dtype=float).T), device=op.device)
     corr = None
     uD_t = None
     if len(fixed) and np.any(u_fixed):
@@ -35,10 +36,8 @@ def _dirichlet_columns(fs, F_batch):
         uD[fixed] = u_fixed
         uD_t = torch.as_tensor(uD, device=op.device)
         corr = op.matvec(uD_t)
-    cols = []
-    for c in range(F_t.shape[1]):
-        cols.append(op.masked(F_t[:, c].contiguous(), corr))
-    return cols, uD_t
+    B = op.masked_cols(F_t, corr)
+    return list(B.unbind(0)), uD_t
 
 
 class _Batch:
@@ -124,12 +123,16 @@ def solve_many(systems, tol=1e-8, maxiter=None):
     """systems: iterable of (FESystem, F_batch (n_dof_i, k_i)).  Returns
     ([U_i (n_dof_i, k_i) device tensors], stats).  Every FESystem must be
     assembled on the same device."""
+    import torch
     batch = None
     uDs = []
+    flag = None
     for fs, F_batch in systems:
         fs._require_stiffness("solve_many")
         op = fs._op
-        op.set_free_mask(fs._free_mask())
+        if flag is None:
+            flag = torch.zeros(1, dtype=torch.int32, device=op.device)
+        op.set_free_mask(fs._free_mask(), zero_diag_flag=flag)
         F_batch = np.asarray(F_batch, dtype=float)
         if F_batch.ndim == 1:
             F_batch = F_batch[:, None]
@@ -142,6 +145,8 @@ def solve_many(systems, tol=1e-8, maxiter=None):
         uDs.append(uD)
     if batch is None:
         raise ValueError("solve_many: no systems given")
+    if int(flag.item()):
+        raise ValueError("jacobi_preconditioner: A has a zero diagonal entry.")
     X, stats = batch.solve(tol, maxiter)
     X = [x if uD is None else x + uD[:, None] for x, uD in zip(X, uDs)]
     return X, stats

@@ -163,15 +163,23 @@ class GridOperator:
                                          self._stream()), "h8_spmv_f32")
         return y
 
-    def set_free_mask(self, free_mask, preconditioner="jacobi"):
-        """Builds dinv: 1/K_ii at free DOFs (jacobi) or 1 (no preconditioner), 0 at fixed."""
+    def set_free_mask(self, free_mask, preconditioner="jacobi", zero_diag_flag=None):
+        """Builds dinv: 1/K_ii at free DOFs (jacobi) or 1 (no preconditioner), 0 at fixed.
+        With zero_diag_flag (a zeroed int32 device tensor shared by many models)
+        the zero-diagonal check is left to the caller: the flag becomes 1 and no
+        host wait happens here."""
         import torch
         fm = torch.as_tensor(np.asarray(free_mask, dtype=np.uint8), device=self.device).reshape(-1).contiguous()
         if fm.numel() != self.n_dof:
             raise ValueError("free mask length mismatch")
         self.free_mask = fm
         dinv = torch.empty(self.n_dof, dtype=torch.float64, device=self.device)
-        if preconditioner == "jacobi":
+        if preconditioner == "jacobi" and zero_diag_flag is not None:
+            with torch.cuda.device(self.device):
+                check(_lib.lib().h8_diag_async_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(fm),
+                                                   ptr(dinv), ptr(zero_diag_flag), self._stream()),
+                      "h8_diag_async_f64")
+        elif preconditioner == "jacobi":
             flag = torch.zeros(1, dtype=torch.int32, device=self.device)
             with torch.cuda.device(self.device):
                 check(_lib.lib().h8_diag_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(fm), ptr(dinv),
@@ -182,6 +190,16 @@ class GridOperator:
             raise ValueError(f"unknown preconditioner {preconditioner!r}")
         self.dinv = dinv
         return dinv
+
+    def masked_cols(self, A, b=None):
+        """mask(A[c] - b) for every row c of A (ncols, n_dof), one launch."""
+        import torch
+        A = A.contiguous()
+        out = torch.empty_like(A)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().h8_masked_sub_cols_f64(ptr(A), ptr(b), ptr(self.dinv), ptr(out), self.n_dof,
+                                                    A.shape[0], self._stream()), "h8_masked_sub_cols_f64")
+        return out
 
     def masked(self, a, b=None):
         """mask(a - b) with mask = free DOFs."""

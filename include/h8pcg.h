@@ -118,6 +118,13 @@ int h8_ke_bbar_batched_f64(const double* X, const double* D, int64_t d_stride, i
  * (fe/iterative_solvers.py:193-202,305-321). */
 int h8_diag_f64(const h8_grid* g, const double* s_e, const double* K0,
                 const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream);
+/* h8_diag_f64 without the host wait, for many models in a row (the per-model
+ * jacobi_preconditioner of solve_static_batched / batch.map, fe/batched_solve.py:
+ * 105-164, fe/batch.py:60-128): ORs 1 into *d_flag where a free diagonal is 0.
+ * The caller zeroes *d_flag before the first call and checks it once after the
+ * last (H8_ZERO_DIAG semantics). */
+int h8_diag_async_f64(const h8_grid* g, const double* s_e, const double* K0,
+                      const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream);
 
 /* y = K u over the whole grid (every DOF), K = assembled sum_e s_e K0.
  * Replaces the CSR product A @ p of preconditioned_cg and _cg_torch
@@ -310,6 +317,11 @@ int h8_cast_f64_f32(const double* x, float* y, int64_t n, void* stream);
 /* out = mask ? (a - b) : 0 with mask = (dinv != 0)  (b may be NULL -> a) */
 int h8_masked_sub_f64(const double* a, const double* b, const double* dinv, double* out,
                       int64_t n, void* stream);
+/* The same for ncols columns of length n stored one after another in a and out,
+ * b (length n, may be NULL) shared: the Dirichlet-corrected right-hand sides
+ * F[:, c] - K u_D of a batch (fe/batched_solve.py:105-164), one launch per model. */
+int h8_masked_sub_cols_f64(const double* a, const double* b, const double* dinv, double* out,
+                           int64_t n, int64_t ncols, void* stream);
 
 #ifdef __cplusplus
 }

@@ -114,3 +114,36 @@ def test_batched_dirichlet_reference_checks(gpu):
     Ua = solve_static_batched(build(), Fz)
     Ub = solve_static_batched(build(zero_explicit=True), Fz)
     assert np.array_equal(Ua, Ub)
+
+
+def test_solve_many_zero_diagonal_and_columns(gpu):
+    """solve_many checks every model's Jacobi diagonal once, after all of them
+    (the jacobi_preconditioner ValueError of fe/iterative_solvers.py:193-202),
+    and builds each model's right-hand sides in one masking launch: a model
+    with a load case of zeros next to real ones solves each column as a single
+    solve_static would."""
+    import h8pcg
+    from h8pcg.batched import solve_many
+    mesh = h8pcg.box_mesh(4.0, 2.0, 2.0, 4, 2, 2)
+
+    def model(scale):
+        fs = h8pcg.FESystem(mesh, h8pcg.Hex8Solid3D())
+        fs.assemble_stiffness(h8pcg.D_solid3d(h8pcg.Material(E=1.0, nu=0.3)), element_scale=scale)
+        fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+        return fs
+
+    good, bad = model(np.ones(16)), model(np.zeros(16))
+    F = np.zeros((good.n_dof, 3))
+    tip = mesh.nodes_on_plane(0, 4.0)
+    F[3 * tip + 2, 0] = -1.0
+    F[3 * tip + 1, 2] = 0.5
+    with pytest.raises(ValueError, match="zero diagonal"):
+        solve_many([(good, F), (bad, F)], tol=1e-10)
+    X, st = solve_many([(good, F)], tol=1e-10)
+    U = X[0].cpu().numpy()
+    assert np.all(U[:, 1] == 0.0) and np.all(st["status"] == 0)
+    for c in (0, 2):
+        ref = model(np.ones(16))
+        ref.F[:] = F[:, c]
+        Uc = np.asarray(ref.solve_static(tol=1e-10))
+        assert np.max(np.abs(U[:, c] - Uc)) < 1e-8 * np.max(np.abs(Uc))
