@@ -29,7 +29,11 @@
 // slots, and every owned node sums its slots in a fixed order (deterministic)
 // - the top-corner sum is carried to the next step in a register.  The next
 // layer's global loads are issued before the element work of a step and
-// stored into the ring after it.  Two barriers per step.
+// stored into the ring after it.  Two barriers per step.  With FP64 storage
+// one slot array serves the bottom and then the top corners (four barriers
+// per step): half the slot LDS, so three blocks fit per CU instead of two
+// (FP64 direction kernel at 3.26M DOF on MI355X: 51.6 -> 48.1 us; the FP32
+// kernel, already at four blocks per CU, lost more to the barriers).
 #pragma once
 #include "h8_device.h"
 #include "h8_node.h"
@@ -202,12 +206,14 @@ template <> struct SectorWaves<float> { static constexpr int value = 6; };
 // LDS of one march: the 3-slot layer ring and the per-node corner slots.
 template <typename T, int EJ, int EI>
 struct SectorSmem {
+    // FP64: the top corners reuse `lo` after the bottom corners have been summed
+    static constexpr bool SPLIT = sizeof(T) == 8;
     // corner slots component-major: value k (corner a & 3, component c: k = 3(a & 3) + c)
     // of node n at k * EJ * EI + n, so lane-adjacent nodes hit adjacent LDS banks both
     // when the elements write them and when the nodes sum them
     alignas(16) T ring[3 * SLayer<T, EJ, EI>::LW];
     alignas(16) T lo[EJ * EI * 12];
-    alignas(16) T hi[EJ * EI * 12];
+    alignas(16) T hi[SPLIT ? 4 : EJ * EI * 12];
 };
 
 // Column tiles of the sector march for a grid: (EJ-1) x (EI-1) nodes, chunks of kd layers.
@@ -273,57 +279,79 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh,
                              (ecol && s + 1 < g.nz && s + 1 < ke) ? int(sizeof(T)) * (((s + 1) * g.nx + ie) * g.ny + je)
                                                                   : OOB);
         // ---- element phase: slab s, element (je, ie, s) (scale 0 outside the mesh)
-        {
-            TA U[8][3];
+        TA U[8][3];
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const T* src = OZ(a) ? up : lo;
+        for (int a = 0; a < 8; ++a) {
+            const T* src = OZ(a) ? up : lo;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) U[a][c] = TA(src[lbase + 3 * (OX(a) * (EJ + 1) + OY(a)) + c]);
-            }
-            if constexpr (OFFS) {
+            for (int c = 0; c < 3; ++c) U[a][c] = TA(src[lbase + 3 * (OX(a) * (EJ + 1) + OY(a)) + c]);
+        }
+        if constexpr (OFFS) {
 #pragma unroll
-                for (int a = 1; a < 8; ++a)
+            for (int a = 1; a < 8; ++a)
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) U[a][c] -= U[0][c];
+                for (int c = 0; c < 3; ++c) U[a][c] -= U[0][c];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) U[0][c] = TA(0);
-            }
-            sector_product<TA>(kh, U, TA(se));
-            // corner a of this element is node (ej + OY(a), ei + OX(a)) of the halo grid;
-            // slots exist for hj < EJ, hi < EI (owned nodes have 1 <= hj, hi <= EJ-1, EI-1)
+            for (int c = 0; c < 3; ++c) U[0][c] = TA(0);
+        }
+        sector_product<TA>(kh, U, TA(se));
+        // corner a of this element is node (ej + OY(a), ei + OX(a)) of the halo grid;
+        // slots exist for hj < EJ, hi < EI (owned nodes have 1 <= hj, hi <= EJ-1, EI-1)
+        auto put = [&](int a0, T* dst) {
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
+            for (int a = a0; a < a0 + 4; ++a) {
                 if ((OY(a) && ej + 1 >= EJ) || (OX(a) && ei + 1 >= EI)) continue;
-                T* dst = OZ(a) ? sHi : sLo;
                 const int node = (ei + OX(a)) * EJ + ej + OY(a);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) dst[(3 * (a & 3) + c) * (EJ * EI) + node] = T(U[a][c]);
             }
-        }
-        __syncthreads();
-        // ---- node phase: layer s outputs, layer s + 1 carry; ring refill with layer s + 2
-        {
-            T qv[3], ctr[3], top[3];
+        };
+        auto sum = [&](const T* src, int c) {
+            T v = T(0);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) v += src[(3 * a + c) * (EJ * EI) + threadIdx.x];
+            return v;
+        };
+        auto output = [&](const T (&sl)[3]) {
+            T qv[3], ctr[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                T sl = T(0), sh = T(0);
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    sl += sLo[(3 * a + c) * (EJ * EI) + threadIdx.x];
-                    sh += sHi[(3 * a + c) * (EJ * EI) + threadIdx.x];
-                }
-                qv[c] = carry[c] + sl;
-                top[c] = sh;
+                qv[c] = carry[c] + sl[c];
                 ctr[c] = lo[lbase + c];
             }
             if (nown && s >= kb) node_output<T, MODE>(A, 3 * ((s * g.NI + ni) * g.NJ + nj), qv, ctr, part);
+        };
+        if constexpr (!SectorSmem<T, EJ, EI>::SPLIT) {
+            put(0, sLo);
+            put(4, sHi);
+            __syncthreads();
+            // ---- node phase: layer s outputs, layer s + 1 carry
+            T sl[3], sh[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) carry[c] = top[c];
-            if (more)
-                lay.template store<MODE>(A, beta, p_new, alpha_prev, pending && s + 2 < ke, s + 2, s + 2 < ke,
-                                         sRing + ((q + 2) % 3) * L::LW);
+            for (int c = 0; c < 3; ++c) {
+                sl[c] = sum(sLo, c);
+                sh[c] = sum(sHi, c);
+            }
+            output(sl);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) carry[c] = sh[c];
+        } else {
+            put(0, sLo);
+            __syncthreads();
+            T sl[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sl[c] = sum(sLo, c);
+            output(sl);
+            __syncthreads();
+            put(4, sLo);
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 3; ++c) carry[c] = sum(sLo, c);
         }
+        // ring refill with layer s + 2 (its slot is not read in this step)
+        if (more)
+            lay.template store<MODE>(A, beta, p_new, alpha_prev, pending && s + 2 < ke, s + 2, s + 2 < ke,
+                                     sRing + ((q + 2) % 3) * L::LW);
         __syncthreads();
     }
 }
