@@ -1268,10 +1268,10 @@ static int refine_resident(const GridDev& d, const float* s_e32, const float* K0
     return rc;
 }
 
-extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const double* K064,
-                               const float* s_e32, const float* K032, const double* dinv64,
-                               const double* f, double* u, double tol, int64_t maxiter, double delta,
-                               void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+static int refine_solve(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                        const float* K032, const double* dinv64, const double* f, double* u, double tol,
+                        int64_t maxiter, double delta, void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream,
+                        bool warm) {
     GridDev d;
     int rc = make_grid(g, d);
     if (rc) return rc;
@@ -1302,7 +1302,11 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     const int sb = stream_blocks(d.ndof);
     hipLaunchKernelGGL(k_cast_f64_f32, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, dinv64, w.dinv32);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, 0.0);
+    if (warm)      // the initial guess, zero at the fixed DOFs (dinv64 = 0 there)
+        hipLaunchKernelGGL(k_masked_sub, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, (const double*)u,
+                           (const double*)nullptr, dinv64, u);
+    else
+        hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, 0.0);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_fill<float>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, w.x, 0.0f);
     LAUNCHCHK();
@@ -1321,6 +1325,12 @@ extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const doub
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (warm && hst.status == ST_DONE) {
+        // the early out above assumes a zero guess (||f|| below tol): with an
+        // initial guess the first true residual decides
+        hst.status = ST_RUNNING;
+        HIPCHK(hipMemcpyAsync(w.st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    }
     if (hst.status == ST_RUNNING && maxiter == 0) {
         hst.status = ST_MAXITER;
         hst.iters = 0;
@@ -1499,6 +1509,23 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
     avg_ms[0] = ms[0] / reps;
     avg_ms[1] = ms[1] / reps;
     return H8_OK;
+}
+
+
+extern "C" int h8_refine_solve(const h8_grid* g, const double* s_e64, const double* K064,
+                               const float* s_e32, const float* K032, const double* dinv64,
+                               const double* f, double* u, double tol, int64_t maxiter, double delta,
+                               void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+    return refine_solve(g, s_e64, K064, s_e32, K032, dinv64, f, u, tol, maxiter, delta, ws, ws_bytes, stats, stream,
+                        false);
+}
+
+extern "C" int h8_refine_solve_x0(const h8_grid* g, const double* s_e64, const double* K064,
+                                  const float* s_e32, const float* K032, const double* dinv64,
+                                  const double* f, double* u, double tol, int64_t maxiter, double delta,
+                                  void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+    return refine_solve(g, s_e64, K064, s_e32, K032, dinv64, f, u, tol, maxiter, delta, ws, ws_bytes, stats, stream,
+                        true);
 }
 
 extern "C" int h8_time_kernels(const h8_grid* g, int precision, const void* s_e, const void* K0,

@@ -78,6 +78,8 @@ SIGNATURES = {
     "h8_pcg_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, C.c_int, _D, _I64, _P, C.c_size_t, _STATS, _P]),
     "h8_refine_solve": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, _P, C.c_size_t,
                                   _STATS, _P]),
+    "h8_refine_solve_x0": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, _P, C.c_size_t,
+                                     _STATS, _P]),
     "h8_elem_bilinear_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
     "h8_simp_scale_f64": (C.c_int, [_P, _D, _D, _I64, _P, _P, _P]),
     "h8_cast_f64_f32": (C.c_int, [_P, _P, _I64, _P]),

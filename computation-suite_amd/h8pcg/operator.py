@@ -232,22 +232,23 @@ class GridOperator:
         check(rc)
         return x, stats
 
-    def refined(self, f, tol=1e-8, maxiter=None, delta=0.5):
-        """FP32 PCG iterations with FP64 residual replacement; returns (u, stats)."""
+    def refined(self, f, tol=1e-8, maxiter=None, delta=0.5, x0=None):
+        """FP32 PCG iterations with FP64 residual replacement; returns (u, stats).
+        x0: initial guess (full length; its fixed-DOF entries are ignored)."""
         import torch
         if self.dinv is None:
             raise RuntimeError("set_free_mask() must be called before solving")
         f = self._vec(f)
-        u = torch.zeros_like(f)
+        u = torch.zeros_like(f) if x0 is None else self._vec(x0).clone()
         if maxiter is None:
             maxiter = 10 * int(self.free_mask.sum().item())
         ws = self.workspace(32)
         st = H8PcgStats()
         with torch.cuda.device(self.device):
-            rc = _lib.lib().h8_refine_solve(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32),
-                                            ptr(self.K0_32), ptr(self.dinv), ptr(f), ptr(u), float(tol),
-                                            int(maxiter), float(delta), ptr(ws), ws.numel(), C.byref(st),
-                                            self._stream())
+            fn = _lib.lib().h8_refine_solve if x0 is None else _lib.lib().h8_refine_solve_x0
+            rc = fn(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32), ptr(self.K0_32),
+                    ptr(self.dinv), ptr(f), ptr(u), float(tol), int(maxiter), float(delta), ptr(ws), ws.numel(),
+                    C.byref(st), self._stream())
         stats = st.as_dict()
         stats["rc"] = rc
         check(rc)

@@ -497,23 +497,21 @@ class FESystem:
         n_free = self.n_dof - len(self.fixed_dofs)
         if maxiter is None:
             maxiter = 10 * n_free
+        x0_t = None
+        if x0 is not None:
+            x0 = np.asarray(x0, dtype=float)
+            full = np.zeros(self.n_dof)
+            if x0.shape == (n_free,):
+                full[self.free_dofs] = x0
+            elif x0.shape == (self.n_dof,):
+                full[self.free_dofs] = x0[self.free_dofs]
+            else:
+                raise ValueError(f"x0 must have {n_free} (free) or {self.n_dof} entries")
+            x0_t = full
         if precision == "fp64":
-            x0_t = None
-            if x0 is not None:
-                x0 = np.asarray(x0, dtype=float)
-                full = np.zeros(self.n_dof)
-                if x0.shape == (n_free,):
-                    full[self.free_dofs] = x0
-                elif x0.shape == (self.n_dof,):
-                    full[self.free_dofs] = x0[self.free_dofs]
-                else:
-                    raise ValueError(f"x0 must have {n_free} (free) or {self.n_dof} entries")
-                x0_t = full
             x, stats = op.pcg(b, tol=tol, maxiter=maxiter, x0=x0_t)
         else:
-            if x0 is not None:
-                raise ValueError("x0 is not supported with precision='mixed'")
-            x, stats = op.refined(b, tol=tol, maxiter=maxiter, delta=delta)
+            x, stats = op.refined(b, tol=tol, maxiter=maxiter, delta=delta, x0=x0_t)
         if Q is not None:
             x = x - Q @ (Q.T @ x)
         if uD is not None:

@@ -74,13 +74,13 @@ class SIMPProblem:
 
     def solve(self, rho, rhs=None, x0=None, tol=None, precision=None):
         """u with K(rho) u = rhs (default F) on the free DOFs (precision: the
-        problem's unless given; a warm start x0 always uses the FP64 solve)."""
+        problem's unless given; x0: warm start, in either precision)."""
         self._set_rho(rho)
         b = self.op.masked(self.F if rhs is None else rhs.to(torch_f64()).contiguous())
         tol = self.tol if tol is None else tol
         precision = self.precision if precision is None else precision
-        if precision == "mixed" and x0 is None:
-            u, st = self.op.refined(b, tol=tol, maxiter=self.maxiter)
+        if precision == "mixed":
+            u, st = self.op.refined(b, tol=tol, maxiter=self.maxiter, x0=x0)
         else:
             u, st = self.op.pcg(b, tol=tol, maxiter=self.maxiter, x0=x0)
         self.last_stats = st
@@ -184,10 +184,7 @@ def topology_optimize_compliance(fesystem, D, F_ext, volume_fraction, p=3.0, rho
     history, u = [], None
     lap("setup")
     for _ in range(int(n_iter)):
-        if precision == "fp64" and u is not None:
-            u = prob.solve(rho, x0=u)
-        else:
-            u = prob.solve(rho)
+        u = prob.solve(rho, x0=u)
         history.append(prob.compliance_value(u))
         lap("solve")
         if sensitivity == "closed_form":
@@ -289,8 +286,8 @@ def fd_check(problem, rho, elements, h=1e-6, tol=1e-12):
         rp = rho.clone(); rm = rho.clone()
         rp[e] += h
         rm[e] -= h
-        up = problem.solve(rp, x0=u, tol=tol)
-        um = problem.solve(rm, x0=u, tol=tol)
+        up = problem.solve(rp, x0=u, tol=tol, precision="fp64")
+        um = problem.solve(rm, x0=u, tol=tol, precision="fp64")
         sp_ = simp_scale(rp[e:e + 1], problem.p, problem.rho_min)
         sm_ = simp_scale(rm[e:e + 1], problem.p, problem.rho_min)
         w = torch.zeros(problem.m, dtype=torch.float64, device=problem.device)

@@ -162,6 +162,15 @@ int h8_refine_solve(const h8_grid* g, const double* s_e64, const double* K064,
                     const float* s_e32, const float* K032, const double* dinv64,
                     const double* f, double* u, double tol, int64_t maxiter, double delta,
                     void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
+/* h8_refine_solve started from the guess in u (its fixed-DOF entries are
+ * zeroed first): the first FP64 true residual is f - K u, so a guess that
+ * already meets tol returns after it with no FP32 iteration.  The role of the
+ * x0 argument of preconditioned_cg (fe/iterative_solvers.py:362-469) and of the
+ * warm-started forward solves of the SIMP loop (fe/topopt.py:302-377). */
+int h8_refine_solve_x0(const h8_grid* g, const double* s_e64, const double* K064,
+                       const float* s_e32, const float* K032, const double* dinv64,
+                       const double* f, double* u, double tol, int64_t maxiter, double delta,
+                       void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
 
 /* ---- element bilinear forms (SIMP sensitivities) ----------------------- */
 /* out[e] = w[e] * u_e^T K0 v_e for every element (u_e = u at the element's 24

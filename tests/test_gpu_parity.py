@@ -468,3 +468,35 @@ def test_method_dispatch_matches_direct(h8, gpu):
         h8.fesystem_solve_static_torch(build(), method="lu")
     with pytest.raises(NotImplementedError, match="prescribed"):
         h8.fesystem_solve_static_torch(build(prescribed=True))
+
+
+def test_refined_warm_start(h8, gpu):
+    """h8_refine_solve_x0 (the x0 of preconditioned_cg, fe/iterative_solvers.py:
+    362-469, for the FP32-refined solve): from the FP64 solution the first true
+    residual already meets tol, so no FP32 iteration runs and u is the guess;
+    from a perturbed guess (garbage at the fixed DOFs, which are ignored) it
+    converges in fewer FP32 iterations than from zero, to the same tolerance."""
+    say("refined warm start")
+    nx, ny, nz = 24, 6, 6
+    mesh = h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    fs = h8.FESystem(mesh, h8.Hex8Solid3D())
+    E = 10 ** np.random.default_rng(3).uniform(9, 11, nx * ny * nz)
+    fs.assemble_stiffness(h8.D_solid3d(h8.Material(E=1.0, nu=0.3)), element_scale=E)
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.add_nodal_force(mesh.nodes_on_plane(0, float(nx)), 2, -1e6)
+    U64 = np.asarray(fs.solve_static(tol=1e-12))
+    Uc = np.asarray(fs.solve_static(precision="mixed", tol=1e-8))
+    it_cold = fs.last_solve_stats["iterations"]
+    Uw = np.asarray(fs.solve_static(precision="mixed", tol=1e-8, x0=U64))
+    assert fs.last_solve_stats["iterations"] == 0 and np.array_equal(Uw, U64)
+    rng = np.random.default_rng(4)
+    x0 = U64 + 1e-3 * np.abs(U64).max() * rng.standard_normal(U64.shape)
+    x0[fs.fixed_dofs_array] = 1.0
+    Up = np.asarray(fs.solve_static(precision="mixed", tol=1e-8, x0=x0))
+    st = fs.last_solve_stats
+    say("cold", it_cold, "warm", st["iterations"])
+    assert 0 < st["iterations"] < it_cold
+    assert st["true_rel_residual"] <= 1e-8
+    assert np.all(Up[fs.fixed_dofs_array] == 0.0)
+    ref = np.abs(U64).max()
+    assert np.abs(Up - U64).max() < 1e-5 * ref and np.abs(Uc - U64).max() < 1e-5 * ref

@@ -143,3 +143,28 @@ def test_oc_positive_dc_and_forty_iteration_run(h8):
     assert len(hist) >= 2 and hist[-1] < 0.5 * hist[0]
     assert abs(res["rho"].mean() - 0.4) < 0.03
     assert np.all(res["rho"] >= 1e-2 - 1e-9) and np.all(res["rho"] <= 1.0 + 1e-9)
+
+
+def test_mixed_precision_loop_matches_fp64(h8):
+    """The SIMP loop with FP32-refined forward solves, warm-started from the
+    previous design's displacement like the FP64 loop, follows the FP64 loop's
+    compliance history and densities (fe/topopt.py:302-377)."""
+    from h8pcg import topopt
+    nx, ny, nz = 12, 4, 4
+    mesh = h8.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        fs = h8.FESystem(mesh, h8.Hex8Solid3D(), backend="hip", device="cuda")
+        fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+        F = np.zeros(fs.n_dof)
+        tip = mesh.nodes_on_plane(0, float(nx))
+        F[3 * tip + 2] = -1.0 / len(tip)
+        say("loop", prec)
+        out[prec] = topopt.topology_optimize_compliance(fs, h8.D_solid3d(h8.Material(E=1.0, nu=0.3)), F, 0.4,
+                                                        p=3.0, rho_min=1e-2, move=0.2, n_iter=8, filter_radius=1.5,
+                                                        sensitivity="closed_form", solve_tol=1e-10,
+                                                        precision=prec)
+    h64, hm = np.asarray(out["fp64"]["compliance_history"]), np.asarray(out["mixed"]["compliance_history"])
+    assert len(h64) == len(hm)
+    assert np.allclose(hm, h64, rtol=1e-7, atol=0)
+    assert np.abs(out["mixed"]["rho"] - out["fp64"]["rho"]).max() < 1e-6
