@@ -29,11 +29,10 @@
 // slots, and every owned node sums its slots in a fixed order (deterministic)
 // - the top-corner sum is carried to the next step in a register.  The next
 // layer's global loads are issued before the element work of a step and
-// stored into the ring after it.  Two barriers per step.  With FP64 storage
-// one slot array serves the bottom and then the top corners (four barriers
-// per step): half the slot LDS, so three blocks fit per CU instead of two
-// (FP64 direction kernel at 3.26M DOF on MI355X: 51.6 -> 48.1 us; the FP32
-// kernel, already at four blocks per CU, lost more to the barriers).
+// stored into the ring after it.  Two barriers per step.  Corner outputs are
+// summed in pairs along j across lanes (DPP) before they reach LDS, which
+// halves the slot LDS: six FP32 blocks or three FP64 blocks fit per CU
+// instead of four and two.
 #pragma once
 #include "h8_device.h"
 #include "h8_node.h"
@@ -206,15 +205,26 @@ template <> struct SectorWaves<float> { static constexpr int value = 6; };
 // LDS of one march: the 3-slot layer ring and the per-node corner slots.
 template <typename T, int EJ, int EI>
 struct SectorSmem {
-    // FP64: the top corners reuse `lo` after the bottom corners have been summed
-    static constexpr bool SPLIT = sizeof(T) == 8;
-    // corner slots component-major: value k (corner a & 3, component c: k = 3(a & 3) + c)
-    // of node n at k * EJ * EI + n, so lane-adjacent nodes hit adjacent LDS banks both
-    // when the elements write them and when the nodes sum them
+    // pair slots of the bottom (lo) and top (hi) node layer, component-major: value
+    // k = 3 x + c (x = the corner column OX, c the component) of thread t at
+    // k * EJ * EI + t, so lane-adjacent threads hit adjacent LDS banks
     alignas(16) T ring[3 * SLayer<T, EJ, EI>::LW];
-    alignas(16) T lo[EJ * EI * 12];
-    alignas(16) T hi[SPLIT ? 4 : EJ * EI * 12];
+    alignas(16) T lo[EJ * EI * 6];
+    alignas(16) T hi[EJ * EI * 6];
 };
+
+// The value of the lane one to the left within a 16-lane DPP row (row_shr:1);
+// the row's first lane gets 0.
+__device__ __forceinline__ float lane_left(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xF, 0xF,
+                                                                 false));
+}
+__device__ __forceinline__ double lane_left(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(u)), 0x111, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(u >> 32)), 0x111, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32));
+}
 
 // Column tiles of the sector march for a grid: (EJ-1) x (EI-1) nodes, chunks of kd layers.
 __host__ __device__ inline void sector_tiles(GridDev& g, int EJ, int EI, int kd) {
@@ -295,58 +305,44 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh,
             for (int c = 0; c < 3; ++c) U[0][c] = TA(0);
         }
         sector_product<TA>(kh, U, TA(se));
-        // corner a of this element is node (ej + OY(a), ei + OX(a)) of the halo grid;
-        // slots exist for hj < EJ, hi < EI (owned nodes have 1 <= hj, hi <= EJ-1, EI-1)
-        auto put = [&](int a0, T* dst) {
+        // Corner a of this element is node (ej + OY(a), ei + OX(a)) of the halo grid.
+        // Pairs along j are summed in registers: node (ej, ei) of a layer takes this
+        // element's corner (OX, OY) = (0, 0) plus the (0, 1) corner of element ej - 1,
+        // the lane to the left (threads are j-fastest; ej = 0 takes nothing), and
+        // node (ej, ei + 1) the (1, 0) and (1, 1) corners likewise.  A node then
+        // adds its own pair and the OX = 1 pair of thread (ej, ei - 1): the same
+        // four corners in a fixed order (deterministic), from half the slots.
+        // Corners on nodes beyond the tile (ej + 1 = EJ or ei + 1 = EI) are not used.
+        constexpr int NT = EJ * EI;
 #pragma unroll
-            for (int a = a0; a < a0 + 4; ++a) {
-                if ((OY(a) && ej + 1 >= EJ) || (OX(a) && ei + 1 >= EI)) continue;
-                const int node = (ei + OX(a)) * EJ + ej + OY(a);
+        for (int z = 0; z < 2; ++z) {
+            T* dst = z ? sHi : sLo;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) dst[(3 * (a & 3) + c) * (EJ * EI) + node] = T(U[a][c]);
+            for (int c = 0; c < 3; ++c) {
+                T l3 = lane_left(T(U[4 * z + 3][c])), l2 = lane_left(T(U[4 * z + 2][c]));
+                if (ej == 0) {
+                    l3 = T(0);
+                    l2 = T(0);
+                }
+                dst[c * NT + threadIdx.x] = T(U[4 * z + 0][c]) + l3;
+                dst[(3 + c) * NT + threadIdx.x] = T(U[4 * z + 1][c]) + l2;
             }
-        };
-        auto sum = [&](const T* src, int c) {
-            T v = T(0);
-#pragma unroll
-            for (int a = 0; a < 4; ++a) v += src[(3 * a + c) * (EJ * EI) + threadIdx.x];
-            return v;
-        };
-        auto output = [&](const T (&sl)[3]) {
+        }
+        __syncthreads();
+        // ---- node phase: layer s outputs, layer s + 1 carry
+        {
+            const int tw = threadIdx.x >= EJ ? int(threadIdx.x) - EJ : int(threadIdx.x);   // thread (ej, ei - 1)
+            const T wmask = threadIdx.x >= EJ ? T(1) : T(0);
             T qv[3], ctr[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                qv[c] = carry[c] + sl[c];
+                const T sl = sLo[c * NT + threadIdx.x] + wmask * sLo[(3 + c) * NT + tw];
+                const T sh = sHi[c * NT + threadIdx.x] + wmask * sHi[(3 + c) * NT + tw];
+                qv[c] = carry[c] + sl;
                 ctr[c] = lo[lbase + c];
+                carry[c] = sh;
             }
             if (nown && s >= kb) node_output<T, MODE>(A, 3 * ((s * g.NI + ni) * g.NJ + nj), qv, ctr, part);
-        };
-        if constexpr (!SectorSmem<T, EJ, EI>::SPLIT) {
-            put(0, sLo);
-            put(4, sHi);
-            __syncthreads();
-            // ---- node phase: layer s outputs, layer s + 1 carry
-            T sl[3], sh[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                sl[c] = sum(sLo, c);
-                sh[c] = sum(sHi, c);
-            }
-            output(sl);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) carry[c] = sh[c];
-        } else {
-            put(0, sLo);
-            __syncthreads();
-            T sl[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sl[c] = sum(sLo, c);
-            output(sl);
-            __syncthreads();
-            put(4, sLo);
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < 3; ++c) carry[c] = sum(sLo, c);
         }
         // ring refill with layer s + 2 (its slot is not read in this step)
         if (more)
