@@ -687,13 +687,14 @@ static int res_axis_elems(int N, int P) {
 // Brick split for the resident solver: at most one block per CU, LDS within
 // 158 KiB (the kernel's static LDS stays below 2 KiB of the CU's 160).  Plans
 // are ranked by the element batches (elements / threads, rounded up) of the
-// busiest block, then by W = its elements + 3 x its shell nodes on faces
+// busiest block, then by W = its elements + 1.5 x its shell nodes on faces
 // across j, then by the fewest blocks.  j is the fastest axis of the node
 // numbering, so a face across j is one scattered 12- or 24-byte word group per
 // node in the z exchange, where faces across i or k are runs along j.  On
 // MI355X at 128x32x32 (FP32, 100-iteration launches, s_memrealtime stamps):
-// 32x1x8 bricks (W = 800) 12.1 us per iteration, 16x2x8 (996) 12.4,
-// 26x3x3 (1410) 13.3, 16x4x4 (1455) 13.6.
+// 32x1x8 bricks (W = 800) 12.1 us per iteration, 16x2x8 (881) 12.4,
+// 26x3x3 (1137) 13.3, 16x4x4 (1092) 13.6; FP64 (two element passes):
+// 32x2x4 (881) 17.1, 16x2x8 (881) 17.0, 22x1x11 (896) 17.4, 26x3x3 17.5.
 template <typename T>
 static bool res_plan_search(const GridDev& d, int nt, int max_blocks, ResGeom& best) {
     constexpr long long LDS_MAX = 158 * 1024;
@@ -732,7 +733,7 @@ static bool res_plan_search(const GridDev& d, int nt, int max_blocks, ResGeom& b
                 const long long emax = (long long)res_axis_elems(d.NI, PI) * res_axis_elems(d.NJ, PJ) *
                                        res_axis_elems(d.NK, PK);
                 const long long jface = (long long)std::min(PJ - 1, 2) * c.HI * (c.BK + 2);
-                const long long cost = ((emax + nt - 1) / nt) * (1LL << 32) + emax + 3 * jface;
+                const long long cost = ((emax + nt - 1) / nt) * (1LL << 32) + 2 * emax + 3 * jface;   // 2 W
                 if (!found || cost < bcost || (cost == bcost && nb < bnb)) {
                     found = true;
                     bcost = cost;
