@@ -2,8 +2,9 @@
 
 A drop-in for fea_engine's Hex8 assemble / Jacobi-PCG solve path: the host API
 mirrors fea_engine (FESystem, box_mesh, Hex8Solid3D, Material, D_solid3d,
-coefficients.by_element), and every arithmetic step runs in libh8pcg.so
-(hand-written HIP kernels) on the GPU.
+coefficients.by_element), and the element, operator, solver, recovery and SIMP
+arithmetic runs in libh8pcg.so (hand-written HIP kernels) on the GPU; the host
+builds meshes, loads, the constraint map and the rigid-mode basis.
 """
 from .material import Material, D_solid3d
 from .mesh import Mesh, box_mesh, structured_grid
