@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -59,6 +60,20 @@ static int fail(int code, const char* fmt, ...) {
             return fail(H8_ERR_HIP, "kernel launch failed: %s (%s:%d)",                \
                         hipGetErrorString(e_), __FILE__, __LINE__);                    \
     } while (0)
+
+// A HIP event destroyed with its scope, so every early return (HIPCHK,
+// argument errors) releases it.
+struct Ev {
+    hipEvent_t e = nullptr;
+    Ev() = default;
+    Ev(const Ev&) = delete;
+    Ev& operator=(const Ev&) = delete;
+    ~Ev() {
+        if (e) (void)hipEventDestroy(e);
+    }
+    hipError_t create(unsigned flags = hipEventDefault) { return hipEventCreateWithFlags(&e, flags); }
+    operator hipEvent_t() const { return e; }
+};
 
 extern "C" const char* h8_last_error(void) { return g_err.c_str(); }
 extern "C" int h8_version(void) { return 1; }
@@ -533,20 +548,40 @@ struct GraphKey {
     int kind;
     int variant;
     int layers;
+    int arith;
     int dev;
     long long nx, ny, nz;
     const void* ptr[12];
+    // values a captured launch reads from its by-value arguments (bit patterns:
+    // tolerances, iteration limits, table sizes) -- a graph replays them as
+    // captured, so two calls that differ in any of them need different graphs
+    unsigned long long extra[6];
     int chunk;
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
+
+// A captured graph, destroyed when the last holder lets go: the cache holds
+// one reference, and a solve that replays it holds another for as long as it
+// launches it, so an eviction by another host thread never destroys an exec
+// that is still being launched.
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    ~GraphExec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+typedef std::shared_ptr<GraphExec> GraphRef;
 struct GraphEntry {
     GraphKey key;
-    hipGraphExec_t exec;
+    GraphRef exec;
     unsigned long long last_use;
 };
 
+// The cache and the capture streams live on the heap and are released only by
+// h8_release() (never by static destructors, which would run after the HIP
+// runtime has been torn down at process exit).
 static std::mutex g_graph_mu;
-static std::vector<GraphEntry> g_graphs;
+static std::vector<GraphEntry>* g_graphs = new std::vector<GraphEntry>();
 static unsigned long long g_clock = 0;
 static hipStream_t g_capture_stream[64] = {};
 
@@ -554,41 +589,93 @@ static int capture_stream(hipStream_t* out) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(H8_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_graph_mu);
     if (!g_capture_stream[dev]) HIPCHK(hipStreamCreateWithFlags(&g_capture_stream[dev], hipStreamNonBlocking));
     *out = g_capture_stream[dev];
     return H8_OK;
 }
 
+// Captures body(capture stream) into an instantiated graph.  Captures share the
+// device's capture stream, one at a time.
+static std::mutex g_capture_mu;
 template <typename Body>
-static int get_graph(const GraphKey& key, Body body, hipGraphExec_t* out) {
-    std::lock_guard<std::mutex> lk(g_graph_mu);
-    for (auto& e : g_graphs)
-        if (e.key == key) {
-            e.last_use = ++g_clock;
-            *out = e.exec;
-            return H8_OK;
-        }
+static int capture_graph(Body body, GraphExec& out) {
     hipStream_t cs;
     int rc = capture_stream(&cs);
     if (rc) return rc;
+    std::lock_guard<std::mutex> ck(g_capture_mu);
     HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
     rc = body(cs);
     hipGraph_t graph;
-    hipError_t ee = hipStreamEndCapture(cs, &graph);
-    if (rc) return rc;
-    if (ee != hipSuccess) return fail(H8_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
-    hipGraphExec_t exec;
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ie != hipSuccess) return fail(H8_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-    if (g_graphs.size() >= 16) {
-        auto it = std::min_element(g_graphs.begin(), g_graphs.end(),
-                                   [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
-        (void)hipGraphExecDestroy(it->exec);
-        g_graphs.erase(it);
+    const hipError_t ee = hipStreamEndCapture(cs, &graph);
+    if (rc) {
+        if (ee == hipSuccess) (void)hipGraphDestroy(graph);
+        return rc;
     }
-    g_graphs.push_back(GraphEntry{key, exec, ++g_clock});
-    *out = exec;
+    if (ee != hipSuccess) return fail(H8_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+    const hipError_t ie = hipGraphInstantiate(&out.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        out.exec = nullptr;
+        return fail(H8_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    }
+    return H8_OK;
+}
+
+// The graph of `key` from the cache (16 entries, least recently used evicted),
+// or captured from body(capture stream) and cached.
+template <typename Body>
+static int get_graph(const GraphKey& key, Body body, GraphRef* out) {
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        for (auto& e : *g_graphs)
+            if (e.key == key) {
+                e.last_use = ++g_clock;
+                *out = e.exec;
+                return H8_OK;
+            }
+    }
+    auto ref = std::make_shared<GraphExec>();
+    const int rc = capture_graph(body, *ref);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    if (g_graphs->size() >= 16) {
+        auto it = std::min_element(g_graphs->begin(), g_graphs->end(),
+                                   [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+        g_graphs->erase(it);
+    }
+    g_graphs->push_back(GraphEntry{key, ref, ++g_clock});
+    *out = ref;
+    return H8_OK;
+}
+
+static unsigned long long dbits(double v) { return __builtin_bit_cast(unsigned long long, v); }
+
+// Destroys the cached graphs and the capture streams (the only HIP objects the
+// library keeps between calls).  The host binding calls it at interpreter
+// exit, before the HIP runtime is torn down; any later call re-creates what it
+// needs.
+extern "C" int h8_release(void) {
+    std::vector<GraphEntry> drop;
+    hipStream_t streams[64];
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        drop.swap(*g_graphs);
+        for (int i = 0; i < 64; ++i) {
+            streams[i] = g_capture_stream[i];
+            g_capture_stream[i] = nullptr;
+        }
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    drop.clear();
+    for (int i = 0; i < 64; ++i)
+        if (streams[i]) {
+            (void)hipSetDevice(i);
+            (void)hipStreamSynchronize(streams[i]);
+            (void)hipStreamDestroy(streams[i]);
+        }
+    (void)hipSetDevice(dev);
     return H8_OK;
 }
 
@@ -854,7 +941,7 @@ static int run_chunks(const GraphKey& key, const GridDev& d, const T* s_e, const
         if (rr != RES_DECLINED) return rr;
     }
     const int chunk = key.chunk;
-    hipGraphExec_t exec;
+    GraphRef exec;
     int rc = get_graph(key, [&](hipStream_t cs) -> int {
         for (int it = 0; it < chunk; ++it) {
             int r1 = launch_iteration<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, q, st, partials, cs);
@@ -864,7 +951,7 @@ static int run_chunks(const GraphKey& key, const GridDev& d, const T* s_e, const
     }, &exec);
     if (rc) return rc;
     for (;;) {
-        HIPCHK(hipGraphLaunch(exec, s));
+        HIPCHK(hipGraphLaunch(exec->exec, s));
         ++launches;
         HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -967,9 +1054,9 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
     (void)carve(p, sizeof(double) * d.ndof);                 // the timing helper's x
     const ZRing<double> zr = carve_ring<double>(p, d, w.q);
 
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    Ev e0, e1;
+    HIPCHK(e0.create());
+    HIPCHK(e1.create());
     HIPCHK(hipEventRecord(e0, s));
 
     PcgState hst{};
@@ -998,6 +1085,7 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
         key.kind = 1;
         key.variant = d.variant;
         key.layers = d.layers;
+        key.arith = d.arith;
         HIPCHK(hipGetDevice(&key.dev));
         key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
         const void* ptrs[] = {s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
@@ -1020,8 +1108,6 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     fill_stats(stats, hst, ms, launches);
     if (hst.status == ST_NOT_SPD)
         return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
@@ -1116,9 +1202,9 @@ extern "C" int h8_pcg_constrained_f64(const h8_grid* g, const double* s_e, const
     double* partials = (double*)carve(p, sizeof(double) * partial_slots(d));
     PcgState* st = (PcgState*)carve(p, sizeof(PcgState));
 
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    Ev e0, e1;
+    HIPCHK(e0.create());
+    HIPCHK(e1.create());
     HIPCHK(hipEventRecord(e0, s));
     const int sb = stream_blocks(d.ndof);
     hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, x, 0.0);
@@ -1144,40 +1230,30 @@ extern "C" int h8_pcg_constrained_f64(const h8_grid* g, const double* s_e, const
         const unsigned gs = unsigned(std::max<long long>(1, (m.n_slave + 255) / 256));
         const unsigned gf = unsigned(std::max<long long>(1, (m.n_master + m.n_slave + 255) / 256));
         const int chunk = chunk_for(maxiter);
-        hipStream_t cs;
-        rc = capture_stream(&cs);
-        if (rc) return rc;
-        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
-        for (int it = 0; it < chunk && rc == 0; ++it) {
-            hipLaunchKernelGGL(k_cdir, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, r, dinv, pv, st);
-            if (m.n_slave > 0) hipLaunchKernelGGL(k_cexpand, dim3(gs), dim3(256), 0, cs, m, pv, st);
-            rc = launch_op<double, MODE_PLAIN>(d, A, cs);
-            if (m.n_slave + m.n_master > 0)
-                hipLaunchKernelGGL(k_cfix, dim3(gf), dim3(256), 0, cs, m, pv, q, st);
-            hipLaunchKernelGGL(k_cpq, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, pv, q, st, partials);
-            hipLaunchKernelGGL((k_pcg_update<double, false, true>), dim3(update_blocks<double>(d.ndof)), dim3(STREAM_THREADS), 0, cs, d.ndof, x,
-                               r, pv, pv, q, dinv, st, partials);
-        }
-        hipGraph_t graph;
-        const hipError_t ee = hipStreamEndCapture(cs, &graph);
-        if (rc) return rc;
-        if (ee != hipSuccess) return fail(H8_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
-        hipGraphExec_t exec;
-        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ie != hipSuccess) return fail(H8_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-        for (;;) {
-            const hipError_t le = hipGraphLaunch(exec, s);
-            if (le != hipSuccess) {
-                (void)hipGraphExecDestroy(exec);
-                return fail(H8_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(le));
+        GraphExec exec;       // destroyed on every return
+        rc = capture_graph([&](hipStream_t cs) -> int {
+            for (int it = 0; it < chunk; ++it) {
+                hipLaunchKernelGGL(k_cdir, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, r, dinv, pv, st);
+                if (m.n_slave > 0) hipLaunchKernelGGL(k_cexpand, dim3(gs), dim3(256), 0, cs, m, pv, st);
+                const int r1 = launch_op<double, MODE_PLAIN>(d, A, cs);
+                if (r1) return r1;
+                if (m.n_slave + m.n_master > 0)
+                    hipLaunchKernelGGL(k_cfix, dim3(gf), dim3(256), 0, cs, m, pv, q, st);
+                hipLaunchKernelGGL(k_cpq, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, pv, q, st, partials);
+                hipLaunchKernelGGL((k_pcg_update<double, false, true>), dim3(update_blocks<double>(d.ndof)),
+                                   dim3(STREAM_THREADS), 0, cs, d.ndof, x, r, pv, pv, q, dinv, st, partials);
             }
+            LAUNCHCHK();
+            return H8_OK;
+        }, exec);
+        if (rc) return rc;
+        for (;;) {
+            HIPCHK(hipGraphLaunch(exec.exec, s));
             ++launches;
             HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (hst.status != ST_RUNNING) break;
         }
-        (void)hipGraphExecDestroy(exec);
     } else if (hst.status == ST_RUNNING) {
         hst.status = ST_MAXITER;
         hst.iters = 0;
@@ -1186,8 +1262,6 @@ extern "C" int h8_pcg_constrained_f64(const h8_grid* g, const double* s_e, const
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     fill_stats(stats, hst, ms, launches);
     if (hst.status == ST_NOT_SPD)
         return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
@@ -1296,9 +1370,9 @@ static int refine_solve(const h8_grid* g, const double* s_e64, const double* K06
     w.st = (PcgState*)carve(p, sizeof(PcgState));
     const ZRing<float> zr = carve_ring<float>(p, d, w.q);
 
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    Ev e0, e1;
+    HIPCHK(e0.create());
+    HIPCHK(e1.create());
     HIPCHK(hipEventRecord(e0, s));
     const int sb = stream_blocks(d.ndof);
     hipLaunchKernelGGL(k_cast_f64_f32, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, dinv64, w.dinv32);
@@ -1353,6 +1427,7 @@ static int refine_solve(const h8_grid* g, const double* s_e64, const double* K06
     key.kind = 2;
     key.variant = d.variant;
     key.layers = d.layers;
+    key.arith = d.arith;       // FP32 or FP64 element products (a different kernel)
     HIPCHK(hipGetDevice(&key.dev));
     key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
     const void* ptrs[] = {s_e32, K032, w.dinv32, w.x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
@@ -1383,8 +1458,6 @@ static int refine_solve(const h8_grid* g, const double* s_e64, const double* K06
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     fill_stats(stats, hst, ms, launches);
     if (stats) {
         stats->outer = int(std::max(0LL, resid_calls - 1));
@@ -1441,8 +1514,8 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
         // the resident solver: `reps` iterations in one launch (stopped by
         // maxiter); avg_ms[0] = time per iteration, avg_ms[1] = 0
         h.maxiter = h.k + reps - 1;
-        hipEvent_t ev[2];
-        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        Ev ev[2];
+        for (auto& e : ev) HIPCHK(e.create());
         PcgState hh{};
         long long launches = 0;
         float ms = 0.f;
@@ -1457,7 +1530,6 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
                 return fail(H8_ERR_ARG, "resident timing run stopped early (status %d at iteration %lld)", hh.status,
                             hh.k);
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
         avg_ms[0] = ms / reps;
         avg_ms[1] = 0.0;
         return H8_OK;
@@ -1466,34 +1538,31 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
     A.g = d; A.s_e = s_e; A.K0 = K0; A.r = r; A.dinv = dinv; A.p0 = p0; A.p1 = p1; A.q = q; A.x = x;
     A.st = st; A.partials = partials;
     // the launches are captured into graphs, as in the solves
-    hipStream_t cs;
-    int rc = capture_stream(&cs);
-    if (rc) return rc;
-    hipGraphExec_t ex[2];
+    GraphExec ex[2];      // destroyed on every return
     for (int which = 0; which < 2; ++which) {
-        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
-        for (int i = 0; i < reps && rc == 0; ++i) {
-            if (which == 0) {
-                rc = launch_dir<T>(d, A, cs);
-            } else {
-                launch_update<T, REFINED>(d, x, r, p0, p1, q, dinv, st, partials, cs);
+        const int rc = capture_graph([&](hipStream_t cs) -> int {
+            for (int i = 0; i < reps; ++i) {
+                if (which == 0) {
+                    const int r1 = launch_dir<T>(d, A, cs);
+                    if (r1) return r1;
+                } else {
+                    launch_update<T, REFINED>(d, x, r, p0, p1, q, dinv, st, partials, cs);
+                }
             }
-        }
-        hipGraph_t gr;
-        HIPCHK(hipStreamEndCapture(cs, &gr));
+            LAUNCHCHK();
+            return H8_OK;
+        }, ex[which]);
         if (rc) return rc;
-        HIPCHK(hipGraphInstantiate(&ex[which], gr, nullptr, nullptr, 0));
-        (void)hipGraphDestroy(gr);
     }
-    hipEvent_t ev[4];
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    Ev ev[4];
+    for (auto& e : ev) HIPCHK(e.create());
     double ms[2];
     for (int which = 0; which < 2; ++which) {
         HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
-        HIPCHK(hipGraphLaunch(ex[which], s));   // warm
+        HIPCHK(hipGraphLaunch(ex[which].exec, s));   // warm
         HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, s));
         HIPCHK(hipEventRecord(ev[2 * which], s));
-        HIPCHK(hipGraphLaunch(ex[which], s));
+        HIPCHK(hipGraphLaunch(ex[which].exec, s));
         HIPCHK(hipEventRecord(ev[2 * which + 1], s));
         HIPCHK(hipEventSynchronize(ev[2 * which + 1]));
         float m = 0.f;
@@ -1504,9 +1573,6 @@ static int time_kernels(const GridDev& d, const T* s_e, const T* K0, const T* di
         if (hh.status != ST_RUNNING)
             return fail(H8_ERR_ARG, "timing loop %d left the running state (%d)", which, hh.status);
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    (void)hipGraphExecDestroy(ex[0]);
-    (void)hipGraphExecDestroy(ex[1]);
     avg_ms[0] = ms[0] / reps;
     avg_ms[1] = ms[1] / reps;
     return H8_OK;
@@ -1695,9 +1761,9 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
     V.r = (double*)(w + L.off_r); V.p0 = (double*)(w + L.off_p0); V.p1 = (double*)(w + L.off_p1);
     V.q = (double*)(w + L.off_q);
 
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    Ev e0, e1;
+    HIPCHK(e0.create());
+    HIPCHK(e1.create());
     HIPCHK(hipEventRecord(e0, s));
     HIPCHK(hipMemcpyAsync((void*)A.inst, hi.data(), sizeof(BInst) * hi.size(), hipMemcpyHostToDevice, s));
     const BSTile* stiles = (const BSTile*)(w + L.off_tiles);
@@ -1720,7 +1786,12 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
         const void* ptrs[] = {s_e, K0, dinv, b, x, ws};
         std::memcpy(key.ptr, ptrs, sizeof ptrs);
         key.chunk = chunk_for(maxiter);
-        hipGraphExec_t exec;
+        // k_bupdate reads tol and maxiter from the captured arguments, and the
+        // workspace offsets of the vectors depend on total_vec
+        const unsigned long long extra[] = {dbits(tol), (unsigned long long)maxiter, (unsigned long long)total_vec,
+                                            (unsigned long long)L.n_sys, (unsigned long long)L.pstride};
+        std::memcpy(key.extra, extra, sizeof extra);
+        GraphRef exec;
         rc = get_graph(key, [&](hipStream_t cs) -> int {
             for (int it = 0; it < key.chunk; ++it) {
                 hipLaunchKernelGGL((k_bsector<double, BEJ, BEI>), dim3(L.n_tiles), dim3(BEJ * BEI), 0, cs, A, V,
@@ -1732,7 +1803,7 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
         }, &exec);
         if (rc) return rc;
         while (active > 0) {
-            HIPCHK(hipGraphLaunch(exec, s));
+            HIPCHK(hipGraphLaunch(exec->exec, s));
             ++launches;
             HIPCHK(hipMemcpyAsync(&active, A.active, sizeof(int), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -1745,8 +1816,6 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     HIPCHK(hipMemcpy(hs.data(), A.sys, sizeof(BSys) * hs.size(), hipMemcpyDeviceToHost));
     int first_bad = H8_OK;
     for (int i = 0; i < L.n_sys; ++i) {

@@ -6,6 +6,7 @@ the error is raised here, loudly.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import threading
@@ -67,6 +68,7 @@ SIGNATURES = {
     "h8_last_error": (C.c_char_p, []),
     "h8_version": (C.c_int, []),
     "h8_device_info": (C.c_int, [C.c_char_p, C.c_int]),
+    "h8_release": (C.c_int, []),
     "h8_sector_table_f64": (C.c_int, [_P, _P]),
     "h8_ke_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
     "h8_ke_bbar_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
@@ -137,7 +139,15 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        # release the cached graphs and capture streams while the HIP runtime is
+        # still alive (Python's atexit handlers run before the C runtime's)
+        atexit.register(_release)
     return _lib
+
+
+def _release():
+    if _lib is not None:
+        _lib.h8_release()
 
 
 def exported_symbols():

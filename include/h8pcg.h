@@ -88,6 +88,12 @@ const char* h8_last_error(void);
 int h8_version(void);
 /* Writes the device name / CU count / arch of the current device. */
 int h8_device_info(char* buf, int buflen);
+/* Destroys what the library keeps between calls: its cached HIP graphs (the
+ * captured PCG iteration chunks) and its per-device capture streams.  Call it
+ * before the HIP runtime is torn down (the Python binding registers it with
+ * atexit); the next solve re-creates what it needs.  No counterpart in the
+ * reference (it keeps no device state). */
+int h8_release(void);
 
 /* Fills K0[576 .. 647] with the reflection-sector table of the 24x24 matrix in
  * K0[0 .. 575] (device pointer): Kh[psi][c][c'] = (1/8) sum_x

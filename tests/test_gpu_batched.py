@@ -147,3 +147,28 @@ def test_solve_many_zero_diagonal_and_columns(gpu):
         ref.F[:] = F[:, c]
         Uc = np.asarray(ref.solve_static(tol=1e-10))
         assert np.max(np.abs(U[:, c] - Uc)) < 1e-8 * np.max(np.abs(Uc))
+
+
+def test_batched_graph_replays_follow_tol_and_maxiter(gpu):
+    """Two batched solves of the same shapes (so the same cached graph and, with
+    torch's caching allocator, usually the same buffers) at tol 1e-6 and then
+    1e-12: the second solve stops at its own tolerance, and a third call with a
+    smaller iteration limit stops at that limit."""
+    import h8pcg
+    from h8pcg.batched import solve_many
+    from h8pcg.workloads import cantilever, tip_traction_loads
+    rng = np.random.default_rng(3)
+    systems = []
+    for nx, k in [(8, 3), (12, 2)]:
+        E = 10 ** rng.uniform(9, 11, nx * (nx // 4) ** 2)
+        fs = cantilever(nx, nx // 4, nx // 4, E_elem=E, load="none")
+        systems.append((fs, tip_traction_loads(fs, nx, nx // 4, nx // 4, k, rng)))
+    _, st6 = solve_many(systems, tol=1e-6)
+    _, st12 = solve_many(systems, tol=1e-12)
+    print("[gpu-test] rel 1e-6", st6["rel_residual"].max(), "rel 1e-12", st12["rel_residual"].max(), flush=True)
+    assert np.all(st6["status"] == 0) and np.all(st12["status"] == 0)
+    assert np.all(st6["rel_residual"] < 1e-6)
+    assert np.all(st12["rel_residual"] < 1e-12)
+    assert np.all(st12["iterations"] > st6["iterations"])
+    with pytest.raises(RuntimeError, match="did not converge"):
+        solve_many(systems, tol=1e-12, maxiter=5)

@@ -130,7 +130,7 @@ def test_pcg_variants_agree(h8, gpu, variant):
         U = np.asarray(fs.solve_static(tol=1e-9, precision=prec))
         err = np.linalg.norm(U - Ud) / np.linalg.norm(Ud)
         say(prec, fs.last_solve_stats["iterations"], err)
-        assert err < 1e-6
+        assert err < 1e-8
 
 
 def test_resident_solver_golden_and_streaming_agreement(h8, gpu, golden):
@@ -176,9 +176,10 @@ def test_resident_solver_golden_and_streaming_agreement(h8, gpu, golden):
 
 @pytest.mark.parametrize("variant", [5, 7])
 def test_refined_fp32_and_fp64_element_products(h8, gpu, variant):
-    """The refined solve with FP32 element products (arith 32) and with FP64
-    products of the FP32 data (arith 64, the default): both meet the FP64 true
-    residual and agree with the direct solve."""
+    """The refined solve with FP32 element products (arith 32, the library's
+    choice) and with FP64 products of the FP32 data (arith 64): both meet the
+    FP64 true residual and agree with the direct solve, and the two legs run
+    different kernels (their results are not bitwise equal)."""
     say("refined arith", variant)
     rng = np.random.default_rng(11)
     nx, ny, nz = 24, 6, 6
@@ -189,9 +190,11 @@ def test_refined_fp32_and_fp64_element_products(h8, gpu, variant):
     Ud = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
     free, _ = o.partition(c["K"].shape[0], c["fixed_dofs"])
     its = {}
+    sols = {}
     for arith in (32, 64):
         fs.operator.set_arith(arith)
-        U = np.asarray(fs.solve_static(tol=1e-9, precision="mixed"))
+        U = np.asarray(fs.solve_static(tol=1e-9, precision="mixed", preconditioner="jacobi"))
+        sols[arith] = U
         st = fs.last_solve_stats
         true_res = np.linalg.norm((c["F"] - c["K"] @ U)[free]) / np.linalg.norm(c["F"][free])
         err = np.linalg.norm(U - Ud) / np.linalg.norm(Ud)
@@ -199,9 +202,11 @@ def test_refined_fp32_and_fp64_element_products(h8, gpu, variant):
         say("arith", arith, "fp32 iters", st["inner_iterations"], "replacements", st["outer"], "true_res", true_res,
             "err", err)
         assert true_res < 1e-9
-        assert err < 1e-6
+        assert err < 1e-8
     fs.operator.set_arith(0)
     say("iterations by arithmetic", its)
+    # a graph captured for one arithmetic is never replayed for the other
+    assert not np.array_equal(sols[32], sols[64])
 
 
 def test_nodal_and_heterogeneous_golden(h8, gpu, golden):
@@ -233,9 +238,10 @@ def test_mixed_precision_refined_reaches_fp64_accuracy(h8, gpu):
     say("refined stats", st, "true_res", true_res, "err", err)
     assert true_res < 1e-8
     assert abs(true_res - st["true_rel_residual"]) < 1e-3 * max(true_res, 1e-12) + 1e-13
+    assert err < 1e-8
     U64 = np.asarray(fs.solve_static(tol=1e-8))
     say("fp64 iters", fs.last_solve_stats["iterations"])
-    assert np.linalg.norm(U - U64) / np.linalg.norm(Ud) < 1e-6
+    assert np.linalg.norm(U - U64) / np.linalg.norm(Ud) < 1e-8
 
 
 def test_nonhomogeneous_dirichlet_and_reactions(h8, gpu):
