@@ -62,6 +62,9 @@ struct PcgState {
     long long max_phase;  // refined: longest FP32 phase (iterations between replacements)
     int pending;       // x still lacks alpha * p of the last direction kernel: the next
                        // direction kernel (or the final/fold kernel) applies it
+    int mg;            // refined solve preconditioned by the multigrid V-cycle: beta and
+                       // rz_old come from r.z of the cycle's z (k_mg_rz), not from dinv r
+    int replaced;      // multigrid: the residual was replaced since the last r.z
 };
 
 // Refined solve: replacements without a 1% drop of the true residual after

@@ -25,6 +25,7 @@
 #include "h8_recovery.h"
 #include "h8_topopt.h"
 #include "h8_constrained.h"
+#include "h8_mg.h"
 
 using namespace h8;
 
@@ -237,10 +238,10 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     // the resident solver's products outside its loop use the streaming default
     const int v = (d.variant == H8_VARIANT_AUTO || d.variant == H8_VARIANT_RESIDENT) ? default_variant<T>(d)
                                                                                       : d.variant;
-    if constexpr (MODE == MODE_DIRX) {   // sector kernels only (defer_x)
+    if constexpr (MODE == MODE_DIRX || MODE == MODE_DIRZ) {   // sector kernels only
         if (v == H8_VARIANT_SECTOR) return launch_sector<T, MODE, 16, 16>(d0, A, s);
         if (v == H8_VARIANT_SECTOR_NARROW) return launch_sector<T, MODE, 8, 16>(d0, A, s);
-        return fail(H8_ERR_ARG, "deferred x update needs a sector kernel (variant %d)", v);
+        return fail(H8_ERR_ARG, "direction mode %d needs a sector kernel (variant %d)", MODE, v);
     } else if (v == H8_VARIANT_MFMA) {
         constexpr int TJ = MfmaTile<T>::J, TI = MfmaTile<T>::I, TK = MfmaTile<T>::K;
         set_tiles(d, TJ, TI, TK);
@@ -1479,6 +1480,409 @@ static int refine_solve(const h8_grid* g, const double* s_e64, const double* K06
                     "tol=%.1e).",
                     (long long)maxiter, hst.true_rel, tol);
     return H8_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid-preconditioned refined solve
+// ---------------------------------------------------------------------------
+// The hierarchy of a grid: level 0 is the grid itself; every further level
+// halves the axes whose element count is even, until a level has at most
+// MG_COARSE_DOF DOFs or no axis can be halved.  Usable when there are at least
+// two levels and the coarsest has at most MG_DENSE_MAX DOFs.
+struct MgPlan {
+    int nlev = 0;
+    MgLevel L[MG_MAX_LEVELS];
+    MgConst* cst = nullptr;       // [MG_MAX_LEVELS]
+    double* M = nullptr;          // the 8 child matrices of level 1, [8][576]
+    unsigned* emask = nullptr;    // free-DOF bits of every fine element
+    int* flag = nullptr;          // singular coarsest operator
+    double* A = nullptr;          // coarsest level, dense (inverted in place)
+    float* Ainv = nullptr;
+};
+
+static bool mg_plan(const GridDev& d, MgPlan& P) {
+    int nx = d.nx, ny = d.ny, nz = d.nz;
+    P.nlev = 0;
+    for (;;) {
+        MgLevel& L = P.L[P.nlev++];
+        L = MgLevel{};
+        L.nx = nx; L.ny = ny; L.nz = nz;
+        L.NJ = ny + 1; L.NI = nx + 1; L.NK = nz + 1;
+        L.nnodes = (long long)L.NJ * L.NI * L.NK;
+        L.ndof = 3 * L.nnodes;
+        const int rx = (nx % 2 == 0) ? 2 : 1, ry = (ny % 2 == 0) ? 2 : 1, rz = (nz % 2 == 0) ? 2 : 1;
+        if (L.ndof <= MG_COARSE_DOF || (rx == 1 && ry == 1 && rz == 1) || P.nlev == MG_MAX_LEVELS) break;
+        L.rx = rx; L.ry = ry; L.rz = rz;
+        nx /= rx; ny /= ry; nz /= rz;
+    }
+    return P.nlev >= 2 && P.L[P.nlev - 1].ndof <= MG_DENSE_MAX;
+}
+
+// Carves the hierarchy's buffers from `base` (nullptr: sizes only) and
+// returns the bytes needed.  Level 0 borrows the solve's z, r32 and dinv32
+// (set by the caller); its product scratch y and direction d are carved here.
+static size_t mg_layout(MgPlan& P, char* base) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* q = base ? base + off : nullptr;
+        off += align_up(bytes);
+        return q;
+    };
+    P.cst = (MgConst*)take(sizeof(MgConst) * MG_MAX_LEVELS);
+    P.M = (double*)take(sizeof(double) * 8 * 576);
+    P.emask = (unsigned*)take(sizeof(unsigned) * (size_t)P.L[0].nx * P.L[0].ny * P.L[0].nz);
+    P.flag = (int*)take(sizeof(int));
+    P.L[0].y = (float*)take(sizeof(float) * P.L[0].ndof);
+    P.L[0].d = (float*)take(sizeof(float) * P.L[0].ndof);
+    for (int l = 1; l < P.nlev; ++l) {
+        MgLevel& L = P.L[l];
+        const size_t nv = sizeof(float) * L.ndof;
+        L.S = (float*)take(sizeof(float) * MG_STENCIL * L.nnodes);
+        L.dinv = (float*)take(nv);
+        L.x0 = (float*)take(nv);
+        L.x1 = (float*)take(nv);
+        L.b = (float*)take(nv);
+        L.d = (float*)take(nv);
+        L.y = (float*)take(nv);
+    }
+    const size_t nc = P.L[P.nlev - 1].ndof;
+    P.A = (double*)take(sizeof(double) * nc * nc);
+    P.Ainv = (float*)take(sizeof(float) * nc * nc);
+    return off;
+}
+
+static unsigned mg_blocks(long long n) { return unsigned(std::max(1LL, std::min((n + 255) / 256, 4096LL))); }
+
+extern "C" size_t h8_mg_workspace_bytes(const h8_grid* g) {
+    GridDev d;
+    if (make_grid(g, d)) return 0;
+    MgPlan P;
+    if (!mg_plan(d, P)) return 0;
+    return mg_layout(P, nullptr);
+}
+
+// Builds the hierarchy for the current element scales and free mask (stream
+// ordered; *P.flag tells whether the coarsest operator was singular).
+static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const double* K064, const float* s_e32,
+                    hipStream_t s) {
+    const int nl = P.nlev;
+    const MgLevel& L0 = P.L[0];
+    hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(64), 0, s, (long long)(4 * MG_MAX_LEVELS),
+                       reinterpret_cast<unsigned*>(P.cst), 0u);
+    hipLaunchKernelGGL(k_mg_mc, dim3(1), dim3(256), 0, s, K064, L0.rx, L0.ry, L0.rz, P.M);
+    const long long m = (long long)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_mg_elem_mask, dim3(mg_blocks(m)), dim3(256), 0, s, d, (const float*)L0.dinv, P.emask);
+    hipLaunchKernelGGL(k_mg_gersh0, dim3(mg_blocks(d.nnodes)), dim3(256), 0, s, d, s_e32, K064,
+                       (const float*)L0.dinv, P.cst);
+    hipLaunchKernelGGL(k_mg_galerkin0, dim3(mg_blocks(27 * P.L[1].nnodes)), dim3(256), 0, s, d, P.L[1], L0.rx, L0.ry,
+                       L0.rz, s_e64, K064, (const double*)P.M, (const unsigned*)P.emask);
+    hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[1].nnodes)), dim3(256), 0, s, P.L[1], P.cst + 1);
+    for (int l = 1; l + 1 < nl; ++l) {
+        const MgLevel& F = P.L[l];
+        hipLaunchKernelGGL(k_mg_rap, dim3(mg_blocks(27 * P.L[l + 1].nnodes)), dim3(256), 0, s, F, P.L[l + 1], F.rx,
+                           F.ry, F.rz);
+        hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[l + 1].nnodes)), dim3(256), 0, s, P.L[l + 1],
+                           P.cst + l + 1);
+    }
+    const MgLevel& Lc = P.L[nl - 1];
+    const long long nc = Lc.ndof;
+    hipLaunchKernelGGL(k_fill<double>, dim3(mg_blocks(nc * nc)), dim3(256), 0, s, nc * nc, P.A, 0.0);
+    hipLaunchKernelGGL(k_mg_dense, dim3(mg_blocks(nc)), dim3(256), 0, s, Lc, P.A);
+    if (nc <= MG_LDS_GJ) {
+        const unsigned lds = unsigned((nc * nc + 2 * nc) * sizeof(double));
+        HIPCHK(hipFuncSetAttribute((const void*)k_mg_gauss_jordan<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int((MG_LDS_GJ * MG_LDS_GJ + 2 * MG_LDS_GJ) * sizeof(double))));
+        hipLaunchKernelGGL(k_mg_gauss_jordan<true>, dim3(1), dim3(1024), lds, s, P.A, int(nc), P.Ainv, P.flag);
+    } else {
+        hipLaunchKernelGGL(k_mg_gauss_jordan<false>, dim3(1), dim3(1024), unsigned(2 * nc * sizeof(double)), s, P.A,
+                           int(nc), P.Ainv, P.flag);
+    }
+    hipLaunchKernelGGL(k_mg_consts, dim3(1), dim3(64), 0, s, P.cst, nl);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+// z = V(r): one V-cycle (degree-2 Chebyshev smoothing before and after the
+// coarse correction).  P.L[0] holds b = r32, x0 = x1 = z, dinv = dinv32.
+// prod0: the level-0 product y = K z (sector kernel, FP32, with the state).
+static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>& prod0, const PcgState* st,
+                     hipStream_t s) {
+    const int nl = P.nlev;
+    const MgLevel* L = P.L;
+    const MgConst* cst = P.cst;
+    auto spmv0 = [&]() { return launch_op<float, MODE_PLAIN>(d32, prod0, s); };
+    int rc;
+    for (int l = 0; l + 1 < nl; ++l) {
+        const MgLevel& F = L[l];
+        const MgLevel& C = L[l + 1];
+        const unsigned nb = mg_blocks(F.nnodes), vb = mg_blocks(F.ndof);
+        // pre-smoothing from x = 0: x = d = c0 dinv b, then the second step
+        hipLaunchKernelGGL(k_mg_cheb_vec, dim3(vb), dim3(256), 0, s, F.ndof, F.x0, (const float*)F.b,
+                           (const float*)nullptr, F.d, (const float*)F.dinv, cst + l, 0, 1, st);
+        if (l == 0) {
+            if ((rc = spmv0())) return rc;
+            hipLaunchKernelGGL(k_mg_cheb_vec, dim3(vb), dim3(256), 0, s, F.ndof, F.x0, (const float*)F.b,
+                               (const float*)F.y, F.d, (const float*)F.dinv, cst, 1, 0, st);
+            if ((rc = spmv0())) return rc;             // y = K z for the residual
+        } else {
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x0, F.x1, cst + l, 1, st);
+            hipLaunchKernelGGL(k_mg_spmv_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x1, F.y, st);
+        }
+        hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.nnodes)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
+                           (const float*)F.b, (const float*)F.y, (const float*)F.dinv, C.b, st);
+    }
+    const MgLevel& Lc = L[nl - 1];
+    hipLaunchKernelGGL(k_mg_dense_solve, dim3(mg_blocks(Lc.ndof)), dim3(256), 0, s, (const float*)P.Ainv,
+                       int(Lc.ndof), (const float*)Lc.b, Lc.x1, st);
+    for (int l = nl - 2; l >= 0; --l) {
+        const MgLevel& F = L[l];
+        const MgLevel& C = L[l + 1];
+        const unsigned nb = mg_blocks(F.nnodes), vb = mg_blocks(F.ndof);
+        hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz, F.x1,
+                           (const float*)C.x1, (const float*)F.dinv, st);
+        if (l == 0) {
+            for (int step = 0; step < 2; ++step) {
+                if ((rc = spmv0())) return rc;
+                hipLaunchKernelGGL(k_mg_cheb_vec, dim3(vb), dim3(256), 0, s, F.ndof, F.x0, (const float*)F.b,
+                                   (const float*)F.y, F.d, (const float*)F.dinv, cst, step, 0, st);
+            }
+        } else {
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x1, F.x0, cst + l, 0, st);
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x0, F.x1, cst + l, 1, st);
+        }
+    }
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+// The refined solve of refine_solve with z = V(r) in the FP32 iterations:
+// per iteration p = z + beta p (sector kernel, MODE_DIRZ), q = K p, alpha;
+// x += alpha p, r -= alpha q and the replacement test; on a replacement the
+// fold and the FP64 residual (both no-ops otherwise); z = V(r); r.z -> beta.
+// All of it is captured in graphs of MG_CHUNK iterations.
+constexpr int MG_CHUNK = 8;
+
+static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                           const float* K032, const double* dinv64, const double* f, double* u, double tol,
+                           int64_t maxiter, double delta, int warm, void* ws, size_t ws_bytes, void* mg_ws,
+                           size_t mg_ws_bytes, h8_pcg_stats* stats, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e64 || !K064 || !s_e32 || !K032 || !dinv64 || !f || !u || !ws || !mg_ws)
+        return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < ws_bytes_ref(d))
+        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, ws_bytes_ref(d));
+    if (!(tol > 0.0)) return fail(H8_ERR_ARG, "tol must be > 0");
+    if (!(delta > 0.0 && delta < 1.0)) return fail(H8_ERR_ARG, "delta must be in (0, 1)");
+    if (maxiter < 0) return fail(H8_ERR_ARG, "maxiter must be >= 0");
+    MgPlan P;
+    if (!mg_plan(d, P))
+        return fail(H8_ERR_ARG, "multigrid: a %dx%dx%d grid does not coarsen to at most %lld DOFs by halving its even "
+                    "axes", d.nx, d.ny, d.nz, (long long)MG_DENSE_MAX);
+    const size_t need = mg_layout(P, nullptr);
+    if (mg_ws_bytes < need) return fail(H8_ERR_WORKSPACE, "multigrid workspace too small: %zu < %zu", mg_ws_bytes, need);
+    mg_layout(P, (char*)mg_ws);
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    WsRef w;
+    w.r = (float*)carve(p, sizeof(float) * d.ndof);
+    w.x = (float*)carve(p, sizeof(float) * d.ndof);
+    w.p0 = (float*)carve(p, sizeof(float) * d.ndof);
+    w.p1 = (float*)carve(p, sizeof(float) * d.ndof);
+    w.q = (float*)carve(p, sizeof(float) * d.ndof);
+    w.dinv32 = (float*)carve(p, sizeof(float) * d.ndof);
+    w.partials = (double*)carve(p, sizeof(double) * partial_slots(d));
+    w.st = (PcgState*)carve(p, sizeof(PcgState));
+    float* z = (float*)carve(p, sizeof(float) * d.ndof);      // the first z-exchange vector of the resident solver
+    MgLevel& L0 = P.L[0];
+    L0.b = w.r;
+    L0.x0 = L0.x1 = z;
+    L0.dinv = w.dinv32;
+    // FP32 products on the sector kernel (the direction mode needs it)
+    GridDev d32 = d;
+    if (d32.variant != H8_VARIANT_SECTOR && d32.variant != H8_VARIANT_SECTOR_NARROW) {
+        d32.variant = H8_VARIANT_SECTOR;
+        d32.layers = 0;
+    }
+
+    Ev e0, e1;
+    HIPCHK(e0.create());
+    HIPCHK(e1.create());
+    HIPCHK(hipEventRecord(e0, s));
+    const int sb = stream_blocks(d.ndof);
+    hipLaunchKernelGGL(k_cast_f64_f32, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, dinv64, w.dinv32);
+    if (warm)
+        hipLaunchKernelGGL(k_masked_sub, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, (const double*)u,
+                           (const double*)nullptr, dinv64, u);
+    else
+        hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, 0.0);
+    hipLaunchKernelGGL(k_fill<float>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, w.x, 0.0f);
+    LAUNCHCHK();
+    rc = mg_setup(d, P, s_e64, K064, s_e32, s);
+    if (rc) return rc;
+    PcgState hst{};
+    hst.tol = tol;
+    hst.maxiter = maxiter;
+    hst.delta = delta;
+    hst.best_true = INFINITY;
+    hst.status = ST_RUNNING;
+    hst.mg = 1;
+    HIPCHK(hipMemcpyAsync(w.st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pcg_init<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, f,
+                       (const double*)nullptr, dinv64, (double*)w.p0, w.st, w.partials);
+    LAUNCHCHK();
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&flag, P.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (flag)
+        return fail(H8_MG_DECLINED, "multigrid: the coarsest operator is singular (the supports leave rigid-body "
+                    "motions free); use the Jacobi preconditioner");
+    if (warm && hst.status == ST_DONE) {
+        hst.status = ST_RUNNING;
+        HIPCHK(hipMemcpyAsync(w.st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    }
+    if (hst.status == ST_RUNNING && maxiter == 0) {
+        hst.status = ST_MAXITER;
+        hst.iters = 0;
+    }
+    NodeArgs<double> R{};
+    R.g = d; R.s_e = s_e64; R.K0 = K064; R.u = u; R.dinv = dinv64; R.f = f; R.r32 = w.r;
+    R.dinv32 = w.dinv32; R.st = w.st; R.partials = w.partials;
+    NodeArgs<float> prod0{};
+    prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y; prod0.st = w.st;
+    NodeArgs<float> D{};
+    D.g = d32; D.s_e = s_e32; D.K0 = K032; D.r = z; D.dinv = w.dinv32; D.p0 = w.p0; D.p1 = w.p1; D.q = w.q;
+    D.x = w.x; D.st = w.st; D.partials = w.partials;
+    long long launches = 0;
+    hst.true_rel = hst.rel;
+    if (hst.status == ST_RUNNING) {
+        // first residual, z = V(r), r.z
+        rc = launch_op<double, MODE_RESID>(d, R, s);
+        if (rc) return rc;
+        rc = mg_vcycle(d32, P, prod0, w.st, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_mg_rz, dim3(stream_blocks(d.ndof)), dim3(256), 0, s, d.ndof, (const float*)w.r,
+                           (const float*)z, w.st, w.partials);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (hst.status == ST_RUNNING) {
+        GraphKey key{};
+        key.kind = 4;
+        key.variant = d32.variant;
+        key.layers = d32.layers;
+        key.arith = d.arith;
+        HIPCHK(hipGetDevice(&key.dev));
+        key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
+        const void* ptrs[] = {s_e64, K064, s_e32, K032, dinv64, f, u, ws, mg_ws};
+        std::memcpy(key.ptr, ptrs, sizeof ptrs);
+        key.extra[0] = (unsigned long long)g->variant;    // the FP64 residual's kernel follows the caller's grid
+        key.chunk = int(std::min<long long>(MG_CHUNK, maxiter));
+        GraphRef exec;
+        NodeArgs<double> Rq = R;
+        Rq.only_on_replace = 1;
+        rc = get_graph(key, [&](hipStream_t cs) -> int {
+            for (int it = 0; it < key.chunk; ++it) {
+                int r1 = launch_op<float, MODE_DIRZ>(d32, D, cs);
+                if (r1) return r1;
+                hipLaunchKernelGGL((k_pcg_update<float, true, true, true>), dim3(update_blocks<float>(d.ndof)),
+                                   dim3(STREAM_THREADS), 0, cs, d.ndof, w.x, w.r, (const float*)w.p0,
+                                   (const float*)w.p1, (const float*)w.q, (const float*)w.dinv32, w.st, w.partials);
+                hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, u, w.x, (const float*)w.p0,
+                                   (const float*)w.p1, (const PcgState*)w.st);
+                if ((r1 = launch_op<double, MODE_RESID>(d, Rq, cs))) return r1;
+                if ((r1 = mg_vcycle(d32, P, prod0, w.st, cs))) return r1;
+                hipLaunchKernelGGL(k_mg_rz, dim3(stream_blocks(d.ndof)), dim3(256), 0, cs, d.ndof, (const float*)w.r,
+                                   (const float*)z, w.st, w.partials);
+            }
+            LAUNCHCHK();
+            return H8_OK;
+        }, &exec);
+        if (rc) return rc;
+        for (;;) {
+            HIPCHK(hipGraphLaunch(exec->exec, s));
+            ++launches;
+            HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (hst.status != ST_RUNNING) break;
+        }
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    fill_stats(stats, hst, ms, launches);
+    if (stats) {
+        stats->outer = int(std::max(0LL, hst.outer - 1));
+        stats->true_rel_residual = hst.true_rel;
+        stats->inner_iterations = hst.iters;
+        stats->bnorm = hst.bnorm;
+    }
+    if (hst.status == ST_NOT_SPD)
+        return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
+                    hst.bad_pq, hst.iters);
+    if (hst.status == ST_MAXITER && hst.stalled)
+        return fail(H8_MAXITER,
+                    "refined solve: the true relative residual stagnated at %.3e (tol=%.1e) after %lld FP32 "
+                    "iterations -- FP32 iterations cannot reduce it further; use the FP64 solve or a larger tol.",
+                    hst.true_rel, tol, (long long)hst.iters);
+    if (hst.status == ST_MAXITER)
+        return fail(H8_MAXITER,
+                    "refined solve: did not converge in %lld iterations (true relative residual %.3e >= "
+                    "tol=%.1e).",
+                    (long long)maxiter, hst.true_rel, tol);
+    return H8_OK;
+}
+
+// One V-cycle z = V(r) after building the hierarchy (diagnostics: the
+// preconditioner of h8_refine_solve_mg as an operator).
+extern "C" int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                                const float* K032, const double* dinv64, const float* r, float* z, void* ws,
+                                size_t ws_bytes, void* mg_ws, size_t mg_ws_bytes, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e64 || !K064 || !s_e32 || !K032 || !dinv64 || !r || !z || !ws || !mg_ws)
+        return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < ws_bytes_ref(d))
+        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, ws_bytes_ref(d));
+    MgPlan P;
+    if (!mg_plan(d, P)) return fail(H8_ERR_ARG, "multigrid: the grid does not coarsen far enough");
+    const size_t need = mg_layout(P, nullptr);
+    if (mg_ws_bytes < need) return fail(H8_ERR_WORKSPACE, "multigrid workspace too small: %zu < %zu", mg_ws_bytes, need);
+    mg_layout(P, (char*)mg_ws);
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    (void)carve(p, sizeof(float) * d.ndof * 5);
+    float* dinv32 = (float*)carve(p, sizeof(float) * d.ndof);
+    hipLaunchKernelGGL(k_cast_f64_f32, dim3(stream_blocks(d.ndof)), dim3(STREAM_THREADS), 0, s, d.ndof, dinv64, dinv32);
+    LAUNCHCHK();
+    MgLevel& L0 = P.L[0];
+    L0.b = const_cast<float*>(r);
+    L0.x0 = L0.x1 = z;
+    L0.dinv = dinv32;
+    GridDev d32 = d;
+    d32.variant = H8_VARIANT_SECTOR;
+    d32.layers = 0;
+    if ((rc = mg_setup(d, P, s_e64, K064, s_e32, s))) return rc;
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, P.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (flag) return fail(H8_MG_DECLINED, "multigrid: the coarsest operator is singular");
+    NodeArgs<float> prod0{};
+    prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y;
+    return mg_vcycle(d32, P, prod0, nullptr, s);
+}
+
+extern "C" int h8_refine_solve_mg(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                                  const float* K032, const double* dinv64, const double* f, double* u, double tol,
+                                  int64_t maxiter, double delta, int use_x0, void* ws, size_t ws_bytes, void* mg_ws,
+                                  size_t mg_ws_bytes, h8_pcg_stats* stats, void* stream) {
+    return refine_solve_mg(g, s_e64, K064, s_e32, K032, dinv64, f, u, tol, maxiter, delta, use_x0 != 0, ws, ws_bytes,
+                           mg_ws, mg_ws_bytes, stats, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,544 @@
+// h8_mg.h -- geometric multigrid V-cycle used as the preconditioner of the
+// FP32 iterations of the refined solve (M^-1 r instead of dinv * r).
+//
+// Hierarchy.  Level 0 is the solve's own matrix-free operator K = sum_e s_e K0
+// on the box grid.  Level l+1 halves every axis of level l whose element count
+// is even (the others are kept), and its operator is the Galerkin product
+// A_{l+1} = P_l^T A_l P_l with P_l the trilinear interpolation from the
+// coarse nodes, restricted to the free DOFs of level l (P_l = M_l P, M_l the
+// free mask).  A coarse DOF none of whose fine neighbours is free has a zero
+// row and column and is masked (dinv = 0) on its level.  Coarse operators are
+// stored assembled, as a 27-point stencil of 3x3 blocks per node:
+//     S[(nb * 9 + 3 c + d) * nnodes + n] = A[3 n + c][3 m + d],
+//     m = n + (dk * NI + di) * NJ + dj, nb = ((dk + 1) * 3 + di + 1) * 3 + dj + 1,
+// FP32 (the preconditioner's precision; the Galerkin sums are formed in FP64).
+// The coarsest level (a few hundred DOFs) is inverted densely once per setup.
+//
+// Level 1 from the elements.  On a nested brick grid the trilinear P is
+// element-local: a coarse element E (r_x r_y r_z fine elements, children c)
+// maps its 8 corners to the corners of each child by P_c (24 x 24), so
+//     A_1 = sum_E sum_c s_{E,c} P_c^T M_e K0 M_e P_c,
+// and when all 24 DOFs of child e are free this is s_e M_c with the 8 fixed
+// matrices M_c = P_c^T K0 P_c (computed once per setup from K0).
+//
+// Smoother: Chebyshev iteration of degree 2 on D^-1 A over [lam/30, lam],
+// lam = the Gershgorin bound max_i sum_j |A_ij| / A_ii of each level (no
+// power iterations), the same polynomial before and after the coarse
+// correction, so the V-cycle is a symmetric positive definite operator (the
+// FP32 PCG stays a CG method).  Step s of a smoothing pass:
+//     d = ca_s d + cb_s dinv (b - A x),  x = x + d,
+// (ca_0, cb_0) = (0, 1/theta), (ca_1, cb_1) = (rho1 rho0, 2 rho1 / delta),
+// theta = (lam + lam/30) / 2, delta = (lam - lam/30) / 2, rho0 = delta/theta,
+// rho1 = 1 / (2 theta/delta - rho0) (Saad, Iterative Methods for Sparse
+// Linear Systems, 2nd ed., Alg. 12.1).
+//
+// Every kernel of a cycle returns at once unless the solve state is RUNNING,
+// so a captured chunk of iterations costs nothing after convergence.
+#pragma once
+#include "h8_device.h"
+
+namespace h8 {
+
+constexpr int MG_MAX_LEVELS = 12;
+constexpr int MG_STENCIL = 243;            // 27 neighbours x 3 x 3
+constexpr long long MG_COARSE_DOF = 128;   // stop coarsening at or below this many DOFs
+constexpr int MG_LDS_GJ = 128;             // coarsest levels up to this many DOFs are inverted in LDS
+constexpr long long MG_DENSE_MAX = 512;    // the coarsest level must be this small (dense inverse)
+constexpr float MG_CHEB_RATIO = 30.0f;     // smoothing interval [lam / ratio, lam]
+
+// One level of the hierarchy (by value in kernel arguments).
+struct MgLevel {
+    int nx, ny, nz;        // elements
+    int NJ, NI, NK;        // nodes per axis
+    long long nnodes, ndof;
+    int rx, ry, rz;        // ratios to the next coarser level (1 or 2; 0 on the coarsest)
+    float* S;              // stencil (levels >= 1)
+    float* dinv;           // inverse diagonal, 0 at masked DOFs (level 0: the solve's)
+    float* x0;             // iterates (levels >= 1: x1 holds a cycle's result)
+    float* x1;
+    float* b;              // right-hand side (levels >= 1)
+    float* d;              // Chebyshev direction
+    float* y;              // A x
+};
+
+// Per-level smoother constants, in device memory (written by the setup kernels).
+struct MgConst {
+    unsigned lam_bits;     // Gershgorin bound (bits of a positive float; atomicMax)
+    float c0, c1a, c1b;
+};
+
+__device__ __forceinline__ bool mg_live(const PcgState* st) { return !st || st->status == ST_RUNNING; }
+
+__device__ __forceinline__ void mg_decode(const MgLevel& L, long long n, int& j, int& i, int& k) {
+    j = int(n % L.NJ);
+    const long long q = n / L.NJ;
+    i = int(q % L.NI);
+    k = int(q / L.NI);
+}
+
+// Interpolation weight of fine position f (in fine element units, within one
+// coarse element of ratio r) from coarse corner C (0 or 1).
+__host__ __device__ constexpr double mg_w(int f, int C, int r) {
+    return r == 1 ? (f == C ? 1.0 : 0.0) : (f == 2 * C ? 1.0 : (f == 1 ? 0.5 : 0.0));
+}
+
+// M_c = P_c^T K0 P_c for the (up to 8) children c of a coarse element with
+// ratios (rx, ry, rz); child c = cx + 2 cy + 4 cz, entries [c][24][24].
+__global__ void __launch_bounds__(256) k_mg_mc(const double* __restrict__ K0, int rx, int ry, int rz,
+                                               double* __restrict__ M) {
+    for (int t = threadIdx.x; t < 8 * 576; t += blockDim.x) {
+        const int c = t / 576, row = (t / 24) % 24, col = t % 24;
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        if (cx >= rx || cy >= ry || cz >= rz) {
+            M[t] = 0.0;
+            continue;
+        }
+        const int A = row / 3, i = row % 3, B = col / 3, j = col % 3;
+        double acc = 0.0;
+        for (int a = 0; a < 8; ++a) {
+            const double pa = mg_w(cx + OX(a), OX(A), rx) * mg_w(cy + OY(a), OY(A), ry) * mg_w(cz + OZ(a), OZ(A), rz);
+            if (pa == 0.0) continue;
+            for (int b = 0; b < 8; ++b) {
+                const double pb =
+                    mg_w(cx + OX(b), OX(B), rx) * mg_w(cy + OY(b), OY(B), ry) * mg_w(cz + OZ(b), OZ(B), rz);
+                if (pb != 0.0) acc += pa * K0[(3 * a + i) * 24 + 3 * b + j] * pb;
+            }
+        }
+        M[t] = acc;
+    }
+}
+
+// Gershgorin bound of D^-1 K on level 0 from the elements: row (n, c) has
+// sum_j |K_ij| <= sum_e s_e sum_(b,d) |K0[3a+c][3b+d]| (a = the node's corner
+// in e).  Free rows only; the bound's bits go to cst->lam_bits by atomicMax.
+__global__ void __launch_bounds__(256) k_mg_gersh0(GridDev g, const float* __restrict__ s_e,
+                                                   const double* __restrict__ K0, const float* __restrict__ dinv,
+                                                   MgConst* cst) {
+    __shared__ float rs[24];
+    if (threadIdx.x < 24) {
+        double acc = 0.0;
+        for (int j = 0; j < 24; ++j) acc += fabs(K0[threadIdx.x * 24 + j]);
+        rs[threadIdx.x] = float(acc);
+    }
+    __syncthreads();
+    float best = 0.0f;
+    for (long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x; nd < g.nnodes;
+         nd += (long long)gridDim.x * blockDim.x) {
+        const int j = int(nd % g.NJ), i = int((nd / g.NJ) % g.NI), k = int(nd / ((long long)g.NJ * g.NI));
+        float sum[3] = {0.f, 0.f, 0.f};
+        for (int a = 0; a < 8; ++a) {
+            const int je = j - OY(a), ie = i - OX(a), ke = k - OZ(a);
+            if (je < 0 || je >= g.ny || ie < 0 || ie >= g.nx || ke < 0 || ke >= g.nz) continue;
+            const float s = s_e[((long long)ke * g.nx + ie) * g.ny + je];
+            for (int c = 0; c < 3; ++c) sum[c] += s * rs[3 * a + c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const float di = dinv[3 * nd + c];
+            if (di != 0.0f) best = fmaxf(best, 1.0001f * sum[c] * di);
+        }
+    }
+    if (best > 0.0f) atomicMax(&cst->lam_bits, __float_as_uint(best));
+}
+
+// Free-DOF bits of every fine element (bit 3a + c: corner a, component c).
+__global__ void __launch_bounds__(256) k_mg_elem_mask(GridDev g, const float* __restrict__ dinv0,
+                                                      unsigned* __restrict__ emask) {
+    const long long m = (long long)g.nx * g.ny * g.nz;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int je = int(e % g.ny), ie = int((e / g.ny) % g.nx), ke = int(e / ((long long)g.ny * g.nx));
+        unsigned mask = 0;
+        for (int a = 0; a < 8; ++a) {
+            const long long fn = ((long long)(ke + OZ(a)) * g.NI + (ie + OX(a))) * g.NJ + (je + OY(a));
+            for (int c = 0; c < 3; ++c)
+                if (dinv0[3 * fn + c] != 0.0f) mask |= 1u << (3 * a + c);
+        }
+        emask[e] = mask;
+    }
+}
+
+// Level-1 stencil from the fine elements: one thread per (coarse node I,
+// neighbour block nb).  The block A_1[I][J] sums, over the coarse elements E
+// holding both I (corner A) and J = I + nb (corner B), and over E's children
+// e, s_e (P_c^T M_e K0 M_e P_c)[3A + i][3B + j] -- from M_c when the child
+// has no fixed DOF, else evaluated with its mask.
+__global__ void __launch_bounds__(256) k_mg_galerkin0(GridDev g, MgLevel C, int rx, int ry, int rz,
+                                                      const double* __restrict__ s_e, const double* __restrict__ K0,
+                                                      const double* __restrict__ M, const unsigned* __restrict__ emask) {
+    const long long total = 27LL * C.nnodes;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int nb = int(t / C.nnodes);
+        const long long In = t - (long long)nb * C.nnodes;
+        int Ij, Ii, Ik;
+        mg_decode(C, In, Ij, Ii, Ik);
+        const int dj = nb % 3 - 1, di = (nb / 3) % 3 - 1, dk = nb / 9 - 1;
+        const int Jj = Ij + dj, Ji = Ii + di, Jk = Ik + dk;
+        double blk[9];
+        for (int q = 0; q < 9; ++q) blk[q] = 0.0;
+        if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
+            // coarse elements E (origin) with E <= I, J <= E + 1 on every axis
+            const int ej0 = max(max(Ij, Jj) - 1, 0), ej1 = min(min(Ij, Jj), C.ny - 1);
+            const int ei0 = max(max(Ii, Ji) - 1, 0), ei1 = min(min(Ii, Ji), C.nx - 1);
+            const int ek0 = max(max(Ik, Jk) - 1, 0), ek1 = min(min(Ik, Jk), C.nz - 1);
+            for (int Ek = ek0; Ek <= ek1; ++Ek)
+                for (int Ei = ei0; Ei <= ei1; ++Ei)
+                    for (int Ej = ej0; Ej <= ej1; ++Ej) {
+                        const int A = corner_of(Ii - Ei, Ij - Ej, Ik - Ek);
+                        const int B = corner_of(Ji - Ei, Jj - Ej, Jk - Ek);
+                        for (int cz = 0; cz < rz; ++cz)
+                            for (int cy = 0; cy < ry; ++cy)
+                                for (int cx = 0; cx < rx; ++cx) {
+                                    const int fe_i = Ei * rx + cx, fe_j = Ej * ry + cy, fe_k = Ek * rz + cz;
+                                    const long long fe = ((long long)fe_k * g.nx + fe_i) * g.ny + fe_j;
+                                    const double s = s_e[fe];
+                                    const unsigned mask = emask[fe];     // free bits of the child's 24 DOFs
+                                    if (mask == 0xFFFFFFu) {
+                                        const double* Mc = M + (cx + 2 * cy + 4 * cz) * 576;
+                                        for (int i = 0; i < 3; ++i)
+                                            for (int j = 0; j < 3; ++j)
+                                                blk[3 * i + j] += s * Mc[(3 * A + i) * 24 + 3 * B + j];
+                                    } else if (mask) {
+                                        for (int a = 0; a < 8; ++a) {
+                                            const double pa = mg_w(cx + OX(a), OX(A), rx) *
+                                                              mg_w(cy + OY(a), OY(A), ry) * mg_w(cz + OZ(a), OZ(A), rz);
+                                            if (pa == 0.0) continue;
+                                            for (int b = 0; b < 8; ++b) {
+                                                const double pb = mg_w(cx + OX(b), OX(B), rx) *
+                                                                  mg_w(cy + OY(b), OY(B), ry) *
+                                                                  mg_w(cz + OZ(b), OZ(B), rz);
+                                                if (pb == 0.0) continue;
+                                                for (int i = 0; i < 3; ++i) {
+                                                    if (!((mask >> (3 * a + i)) & 1u)) continue;
+                                                    for (int j = 0; j < 3; ++j)
+                                                        if ((mask >> (3 * b + j)) & 1u)
+                                                            blk[3 * i + j] +=
+                                                                s * pa * pb * K0[(3 * a + i) * 24 + 3 * b + j];
+                                                }
+                                            }
+                                        }
+                                    }
+                                }
+                    }
+        }
+        for (int q = 0; q < 9; ++q) C.S[(long long)(nb * 9 + q) * C.nnodes + In] = float(blk[q]);
+    }
+}
+
+// Stencil of level l+1 from the stencil of level l (A_{l+1} = P^T A_l P; A_l
+// is zero at its masked DOFs, so P needs no mask here): one thread per
+// (coarse node I, neighbour block nb).
+__global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, int ry, int rz) {
+    const long long total = 27LL * C.nnodes;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int nb = int(t / C.nnodes);
+        const long long In = t - (long long)nb * C.nnodes;
+        int Ij, Ii, Ik;
+        mg_decode(C, In, Ij, Ii, Ik);
+        const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
+        double blk[9];
+        for (int q = 0; q < 9; ++q) blk[q] = 0.0;
+        if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
+            // fine nodes of I's support: offsets -1..1 on halved axes, 0 on kept ones
+            for (int ok = -(rz - 1); ok <= rz - 1; ++ok)
+                for (int oi = -(rx - 1); oi <= rx - 1; ++oi)
+                    for (int oj = -(ry - 1); oj <= ry - 1; ++oj) {
+                        const int fj = Ij * ry + oj, fi = Ii * rx + oi, fk = Ik * rz + ok;
+                        if (fj < 0 || fj >= F.NJ || fi < 0 || fi >= F.NI || fk < 0 || fk >= F.NK) continue;
+                        const double wi = (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
+                        const long long fn = ((long long)fk * F.NI + fi) * F.NJ + fj;
+                        for (int s = 0; s < 27; ++s) {
+                            const int gj = fj + s % 3 - 1, gi = fi + (s / 3) % 3 - 1, gk = fk + s / 9 - 1;
+                            // g in J's support
+                            const int uj = gj - Jj * ry, ui = gi - Ji * rx, uk = gk - Jk * rz;
+                            if (uj < -(ry - 1) || uj > ry - 1 || ui < -(rx - 1) || ui > rx - 1 || uk < -(rz - 1) ||
+                                uk > rz - 1)
+                                continue;
+                            if (gj < 0 || gj >= F.NJ || gi < 0 || gi >= F.NI || gk < 0 || gk >= F.NK) continue;
+                            const double w = wi * (uj ? 0.5 : 1.0) * (ui ? 0.5 : 1.0) * (uk ? 0.5 : 1.0);
+                            for (int q = 0; q < 9; ++q) blk[q] += w * double(F.S[(long long)(s * 9 + q) * F.nnodes + fn]);
+                        }
+                    }
+        }
+        for (int q = 0; q < 9; ++q) C.S[(long long)(nb * 9 + q) * C.nnodes + In] = float(blk[q]);
+    }
+}
+
+// Inverse diagonal (0 where the row is zero: a masked DOF) and the Gershgorin
+// bound of a stencil level.
+__global__ void __launch_bounds__(256) k_mg_diag(MgLevel L, MgConst* cst) {
+    float best = 0.0f;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < L.nnodes;
+         n += (long long)gridDim.x * blockDim.x) {
+        for (int c = 0; c < 3; ++c) {
+            const float dg = L.S[(long long)(13 * 9 + 4 * c) * L.nnodes + n];
+            const float di = dg > 0.0f ? 1.0f / dg : 0.0f;
+            L.dinv[3 * n + c] = di;
+            if (di != 0.0f) {
+                float sum = 0.0f;
+                for (int nb = 0; nb < 27; ++nb)
+                    for (int d = 0; d < 3; ++d) sum += fabsf(L.S[(long long)(nb * 9 + 3 * c + d) * L.nnodes + n]);
+                best = fmaxf(best, 1.0001f * sum * di);
+            }
+        }
+    }
+    if (best > 0.0f) atomicMax(&cst->lam_bits, __float_as_uint(best));
+}
+
+// Chebyshev coefficients of every level from its bound.
+__global__ void k_mg_consts(MgConst* cst, int nlev) {
+    const int l = threadIdx.x;
+    if (l >= nlev) return;
+    float lam = __uint_as_float(cst[l].lam_bits);
+    if (!(lam > 0.0f)) lam = 1.0f;     // a level with no free DOF: any value
+    const float a = lam / MG_CHEB_RATIO;
+    const float theta = 0.5f * (lam + a), delta = 0.5f * (lam - a);
+    const float sigma = theta / delta, rho0 = 1.0f / sigma, rho1 = 1.0f / (2.0f * sigma - rho0);
+    cst[l].c0 = 1.0f / theta;
+    cst[l].c1a = rho1 * rho0;
+    cst[l].c1b = 2.0f * rho1 / delta;
+}
+
+// Dense matrix of the coarsest level (FP64, row-major, n = L.ndof), identity
+// on masked rows; A must be zero on entry.
+__global__ void __launch_bounds__(256) k_mg_dense(MgLevel L, double* __restrict__ A) {
+    const long long n = L.ndof;
+    for (long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+         row += (long long)gridDim.x * blockDim.x) {
+        const long long nd = row / 3;
+        const int c = int(row % 3);
+        if (L.dinv[row] == 0.0f) {
+            A[row * n + row] = 1.0;
+            continue;
+        }
+        int j, i, k;
+        mg_decode(L, nd, j, i, k);
+        for (int nb = 0; nb < 27; ++nb) {
+            const int mj = j + nb % 3 - 1, mi = i + (nb / 3) % 3 - 1, mk = k + nb / 9 - 1;
+            if (mj < 0 || mj >= L.NJ || mi < 0 || mi >= L.NI || mk < 0 || mk >= L.NK) continue;
+            const long long m = ((long long)mk * L.NI + mi) * L.NJ + mj;
+            for (int d = 0; d < 3; ++d)
+                if (L.dinv[3 * m + d] != 0.0f)
+                    A[row * n + 3 * m + d] = double(L.S[(long long)(nb * 9 + 3 * c + d) * L.nnodes + nd]);
+        }
+    }
+}
+
+// In-place Gauss-Jordan inversion of the SPD n x n matrix A (one block, no
+// pivoting: the pivots of an SPD matrix are positive), then Ainv = float(A).
+// *flag = 1 if a pivot is not above 1e-10 of its row's original diagonal (a
+// safety net for an exactly singular coarse operator; the host does not use
+// the V-cycle when the supports leave rigid-body motions free).  INLDS: the
+// matrix is held in LDS (n <= MG_LDS_GJ; dynamic LDS 8 n^2 + 16 n bytes),
+// else in global memory (dynamic LDS 16 n bytes).
+template <bool INLDS>
+__global__ void __launch_bounds__(1024) k_mg_gauss_jordan(double* __restrict__ Ag, int n, float* __restrict__ Ainv,
+                                                         int* flag) {
+    extern __shared__ double smem_gj[];
+    double* fcol = smem_gj;         // column k before the step
+    double* diag0 = smem_gj + n;    // the original diagonal
+    double* A = INLDS ? smem_gj + 2 * n : Ag;
+    __shared__ double s_piv;
+    __shared__ int s_bad;
+    const long long nn = (long long)n * n;
+    if (threadIdx.x == 0) s_bad = 0;
+    if (INLDS)
+        for (long long t = threadIdx.x; t < nn; t += blockDim.x) A[t] = Ag[t];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) diag0[j] = A[(long long)j * n + j];
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) {
+            double p = A[(long long)k * n + k];
+            if (!(p > 1e-10 * diag0[k])) {
+                s_bad = 1;
+                p = 1.0;
+            }
+            s_piv = 1.0 / p;
+        }
+        __syncthreads();
+        const double ip = s_piv;
+        // scale row k (A[k][k] becomes 1 / pivot), keep the old column k
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            fcol[j] = A[(long long)j * n + k];
+            A[(long long)k * n + j] = (j == k) ? ip : A[(long long)k * n + j] * ip;
+        }
+        __syncthreads();
+        for (long long t = threadIdx.x; t < nn; t += blockDim.x) {
+            const int i = int(t / n), j = int(t - (long long)i * n);
+            if (i == k) continue;
+            const double f = fcol[i];
+            if (j == k) A[t] = -f * ip;
+            else A[t] -= f * A[(long long)k * n + j];
+        }
+        __syncthreads();
+    }
+    for (long long t = threadIdx.x; t < nn; t += blockDim.x) Ainv[t] = float(A[t]);
+    if (threadIdx.x == 0) *flag = s_bad;
+}
+
+// x = Ainv b on the coarsest level (one thread per row; Ainv is symmetric, so
+// row r is read as column r: adjacent threads read adjacent words).
+__global__ void __launch_bounds__(256) k_mg_dense_solve(const float* __restrict__ Ainv, int n,
+                                                        const float* __restrict__ b, float* __restrict__ x,
+                                                        const PcgState* st) {
+    if (!mg_live(st)) return;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    float acc = 0.0f;
+    for (int j = 0; j < n; ++j) acc += Ainv[(long long)j * n + row] * b[j];
+    x[row] = acc;
+}
+
+// Chebyshev step on a vector already multiplied (y = A x; ignored, with x,
+// when zero_x): d = ca d + cb dinv (b - y), x = x + d.
+__global__ void __launch_bounds__(256) k_mg_cheb_vec(long long n, float* __restrict__ x, const float* __restrict__ b,
+                                                     const float* __restrict__ y, float* __restrict__ d,
+                                                     const float* __restrict__ dinv, const MgConst* cst, int step,
+                                                     int zero_x, const PcgState* st) {
+    if (!mg_live(st)) return;
+    const float ca = step ? cst->c1a : 0.0f, cb = step ? cst->c1b : cst->c0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float r = zero_x ? b[i] : b[i] - y[i];
+        const float dn = (zero_x ? 0.0f : ca * d[i]) + cb * dinv[i] * r;
+        d[i] = dn;
+        x[i] = zero_x ? dn : x[i] + dn;
+    }
+}
+
+// y = A x at node n of a stencil level (3 components).
+__device__ __forceinline__ void mg_apply(const MgLevel& L, const float* __restrict__ x, long long n, float (&y)[3]) {
+    int j, i, k;
+    mg_decode(L, n, j, i, k);
+    y[0] = y[1] = y[2] = 0.0f;
+#pragma unroll
+    for (int nb = 0; nb < 27; ++nb) {
+        const int mj = j + nb % 3 - 1, mi = i + (nb / 3) % 3 - 1, mk = k + nb / 9 - 1;
+        if (mj < 0 || mj >= L.NJ || mi < 0 || mi >= L.NI || mk < 0 || mk >= L.NK) continue;
+        const long long m = 3 * (((long long)mk * L.NI + mi) * L.NJ + mj);
+        const float x0 = x[m], x1 = x[m + 1], x2 = x[m + 2];
+        const float* s = L.S + (long long)(nb * 9) * L.nnodes + n;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            y[c] += s[(3 * c) * L.nnodes] * x0 + s[(3 * c + 1) * L.nnodes] * x1 + s[(3 * c + 2) * L.nnodes] * x2;
+    }
+}
+
+// Chebyshev step on a stencil level: x_out = x_in + d, d = ca d + cb dinv (b - A x_in).
+__global__ void __launch_bounds__(256) k_mg_cheb_st(MgLevel L, const float* __restrict__ xin, float* __restrict__ xout,
+                                                    const MgConst* cst, int step, const PcgState* st) {
+    if (!mg_live(st)) return;
+    const float ca = step ? cst->c1a : 0.0f, cb = step ? cst->c1b : cst->c0;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < L.nnodes;
+         n += (long long)gridDim.x * blockDim.x) {
+        float y[3];
+        mg_apply(L, xin, n, y);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long q = 3 * n + c;
+            const float dn = ca * L.d[q] + cb * L.dinv[q] * (L.b[q] - y[c]);
+            L.d[q] = dn;
+            xout[q] = xin[q] + dn;
+        }
+    }
+}
+
+// y = A x on a stencil level.
+__global__ void __launch_bounds__(256) k_mg_spmv_st(MgLevel L, const float* __restrict__ x, float* __restrict__ y,
+                                                    const PcgState* st) {
+    if (!mg_live(st)) return;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < L.nnodes;
+         n += (long long)gridDim.x * blockDim.x) {
+        float v[3];
+        mg_apply(L, x, n, v);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[3 * n + c] = v[c];
+    }
+}
+
+// Restriction of the masked residual: bC = P^T M_F (bF - yF).
+__global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C, int rx, int ry, int rz,
+                                                     const float* __restrict__ bF, const float* __restrict__ yF,
+                                                     const float* __restrict__ maskF, float* __restrict__ bC,
+                                                     const PcgState* st) {
+    if (!mg_live(st)) return;
+    for (long long In = (long long)blockIdx.x * blockDim.x + threadIdx.x; In < C.nnodes;
+         In += (long long)gridDim.x * blockDim.x) {
+        int Ij, Ii, Ik;
+        mg_decode(C, In, Ij, Ii, Ik);
+        float acc[3] = {0.f, 0.f, 0.f};
+        for (int ok = -(rz - 1); ok <= rz - 1; ++ok)
+            for (int oi = -(rx - 1); oi <= rx - 1; ++oi)
+                for (int oj = -(ry - 1); oj <= ry - 1; ++oj) {
+                    const int fj = Ij * ry + oj, fi = Ii * rx + oi, fk = Ik * rz + ok;
+                    if (fj < 0 || fj >= F.NJ || fi < 0 || fi >= F.NI || fk < 0 || fk >= F.NK) continue;
+                    const float w = (oj ? 0.5f : 1.0f) * (oi ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f);
+                    const long long f = 3 * (((long long)fk * F.NI + fi) * F.NJ + fj);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        if (maskF[f + c] != 0.0f) acc[c] += w * (bF[f + c] - yF[f + c]);
+                }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bC[3 * In + c] = acc[c];
+    }
+}
+
+// Coarse correction: xF += M_F P xC.
+__global__ void __launch_bounds__(256) k_mg_prolong(MgLevel F, MgLevel C, int rx, int ry, int rz,
+                                                    float* __restrict__ xF, const float* __restrict__ xC,
+                                                    const float* __restrict__ maskF, const PcgState* st) {
+    if (!mg_live(st)) return;
+    for (long long fn = (long long)blockIdx.x * blockDim.x + threadIdx.x; fn < F.nnodes;
+         fn += (long long)gridDim.x * blockDim.x) {
+        int fj, fi, fk;
+        mg_decode(F, fn, fj, fi, fk);
+        // parents per axis: f = r I (weight 1) or, on a halved axis at an odd f, I = (f -+ 1) / 2 (0.5 each)
+        const int j0 = fj / ry, i0 = fi / rx, k0 = fk / rz;
+        const int nj = (ry == 2 && (fj & 1)) ? 2 : 1, ni = (rx == 2 && (fi & 1)) ? 2 : 1,
+                  nk = (rz == 2 && (fk & 1)) ? 2 : 1;
+        const float w = (nj == 2 ? 0.5f : 1.0f) * (ni == 2 ? 0.5f : 1.0f) * (nk == 2 ? 0.5f : 1.0f);
+        float acc[3] = {0.f, 0.f, 0.f};
+        for (int a = 0; a < nk; ++a)
+            for (int b = 0; b < ni; ++b)
+                for (int e = 0; e < nj; ++e) {
+                    const long long I = 3 * (((long long)(k0 + a) * C.NI + (i0 + b)) * C.NJ + (j0 + e));
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) acc[c] += w * xC[I + c];
+                }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (maskF[3 * fn + c] != 0.0f) xF[3 * fn + c] += acc[c];
+    }
+}
+
+// r.z of the FP32 iterations with the V-cycle's z; the last block sets beta
+// (0 when the direction restarts; a replacement that makes beta >= 1/delta^2
+// restarts it too) and rz_old.
+__global__ void __launch_bounds__(256) k_mg_rz(long long n, const float* __restrict__ r, const float* __restrict__ z,
+                                               PcgState* st, double* partials) {
+    __shared__ double sRed[16];
+    if (st->status != ST_RUNNING) return;
+    double v[1] = {0.0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        v[0] += double(r[i]) * double(z[i]);
+    double tot[1];
+    if (reduce_last_block<1>(v, partials, &st->cnt[3], tot, sRed) && threadIdx.x == 0) {
+        const double rz = tot[0];
+        double beta = 0.0;
+        if (!st->first) {
+            beta = rz / st->rz_old;
+            if (!(beta >= 0.0) || (st->replaced && !(beta < 1.0 / (st->delta * st->delta)))) {
+                st->first = 1;
+                beta = 0.0;
+            }
+        }
+        st->beta = beta;
+        st->rz_old = rz;
+        st->replaced = 0;
+    }
+}
+
+}  // namespace h8

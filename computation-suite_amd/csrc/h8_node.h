@@ -28,10 +28,12 @@
 
 namespace h8 {
 
-enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3 };
+enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3, MODE_DIRZ = 4 };
 // MODE_DIRX = MODE_DIR that also applies the previous iteration's deferred
 // x += alpha p_old to the owned nodes while staging p_old.
-__host__ __device__ constexpr bool is_dir(int mode) { return mode == MODE_DIR || mode == MODE_DIRX; }
+// MODE_DIRZ = MODE_DIR with a given preconditioned residual: p = z + beta p_old,
+// z in NodeArgs::r (the multigrid-preconditioned loop; sector kernels only).
+__host__ __device__ constexpr bool is_dir(int mode) { return mode == MODE_DIR || mode == MODE_DIRX || mode == MODE_DIRZ; }
 
 template <typename T>
 struct NodeArgs {
@@ -109,6 +111,15 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
         if (st->k - st->k_rep > st->max_phase) st->max_phase = st->k - st->k_rep;
         st->k_rep = st->k;
         st->rmax = rn;
+        st->rel = st->true_rel;
+        st->status = ST_RUNNING;
+        if (st->mg) {
+            // beta and rz_old follow from the cycle's z of the replaced residual
+            // (k_mg_rz); a replacement without progress restarts the direction
+            st->replaced = 1;
+            if (!progress) st->first = 1;
+            return;
+        }
         const double b = st->first ? 0.0 : rz / st->rz_old;
         if (!st->first && (!progress || !(b < 1.0 / (st->delta * st->delta)))) {
             // no progress since the previous replacement: keeping the direction
@@ -413,6 +424,11 @@ __device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& f
     pending = 0;
     if constexpr (MODE == MODE_RESID) {
         if (A.only_on_replace && A.st->status != ST_REPLACE) return false;
+    }
+    if constexpr (MODE == MODE_PLAIN) {
+        // a product inside a solve's captured iterations (the multigrid cycle)
+        // passes the solve state and does nothing once the solve has stopped
+        if (A.st && A.st->status != ST_RUNNING) return false;
     }
     if constexpr (is_dir(MODE)) {
         if (A.st->status != ST_RUNNING) return false;

@@ -159,7 +159,11 @@ struct SLayer {
 #pragma unroll
         for (int u = 0; u < PASS; ++u) {
             const unsigned off = (unsigned(ofs[u]) + kofs) | kbad;
-            if constexpr (is_dir(MODE)) {
+            if constexpr (MODE == MODE_DIRZ) {
+                v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), int(off));      // z
+                v1[u] = T(1);
+                v2[u] = Buf<T>::ld(buf_rsrc(p_old, vbytes), int(off | pbad));
+            } else if constexpr (is_dir(MODE)) {
                 v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), int(off));
                 v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), int(off));
                 v2[u] = Buf<T>::ld(buf_rsrc(p_old, vbytes), int(off | pbad));

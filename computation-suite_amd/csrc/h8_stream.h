@@ -53,7 +53,9 @@ k_pcg_init(long long n, const T* __restrict__ b, const T* __restrict__ Ax0, cons
 //                  replacement when ||r|| < delta * rmax (or the recurrence
 //                  claims convergence, the phase is too long, or maxiter is
 //                  reached).
-template <bool REFINED>
+// MG: the multigrid-preconditioned refined loop, whose beta comes later from
+//     the cycle's z (k_mg_rz): only the replacement decision is taken here.
+template <bool REFINED, bool MG = false>
 __device__ inline void update_finish(PcgState* st, double rz, double rr) {
     const double rn = sqrt(rr);
     const double rel = rn / st->bnorm;
@@ -78,7 +80,7 @@ __device__ inline void update_finish(PcgState* st, double rz, double rr) {
         const long long phase_cap = REFINE_PHASE > 4 * st->max_phase ? REFINE_PHASE : 4 * st->max_phase;
         if (rel < st->tol || rn < st->delta * st->rmax || k - st->k_rep > phase_cap || k >= st->maxiter) {
             st->status = ST_REPLACE;
-        } else {
+        } else if (!MG) {
             st->beta = rz / st->rz_old;
             st->rz_old = rz;
         }
@@ -98,7 +100,7 @@ __device__ inline void update_finish(PcgState* st, double rz, double rr) {
 // groups per thread per pass (two when x is updated too) with all loads issued
 // first; the workspace and torch allocations are 256-byte aligned, the tail is
 // scalar.
-template <typename T, bool REFINED, bool XUPD>
+template <typename T, bool REFINED, bool XUPD, bool MG = false>
 __global__ void __launch_bounds__(STREAM_THREADS)
 k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p0,
              const T* __restrict__ p1, const T* __restrict__ q, const T* __restrict__ dinv, PcgState* st,
@@ -155,7 +157,7 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     double tot[2];
     if (reduce_last_block<2>(v, partials, &st->cnt[1], tot, sRed) && threadIdx.x == 0) {
         if (XUPD) st->pending = 0;   // applied above
-        update_finish<REFINED>(st, tot[0], tot[1]);
+        update_finish<REFINED, MG>(st, tot[0], tot[1]);
     }
 }
 

@@ -22,6 +22,7 @@ H8_NOT_SPD = 3
 H8_MAXITER = 4
 H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
+H8_MG_DECLINED = 7
 
 (VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR,
  VARIANT_SECTOR_NARROW, VARIANT_RESIDENT) = 0, 1, 2, 3, 4, 5, 6, 7
@@ -82,6 +83,10 @@ SIGNATURES = {
                                   _STATS, _P]),
     "h8_refine_solve_x0": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, _P, C.c_size_t,
                                      _STATS, _P]),
+    "h8_mg_workspace_bytes": (C.c_size_t, [_GRID]),
+    "h8_refine_solve_mg": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, C.c_int, _P, C.c_size_t, _P,
+                                     C.c_size_t, _STATS, _P]),
+    "h8_mg_vcycle_f32": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "h8_elem_bilinear_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
     "h8_simp_scale_f64": (C.c_int, [_P, _D, _D, _I64, _P, _P, _P]),
     "h8_cast_f64_f32": (C.c_int, [_P, _P, _I64, _P]),

@@ -232,27 +232,79 @@ class GridOperator:
         check(rc)
         return x, stats
 
-    def refined(self, f, tol=1e-8, maxiter=None, delta=0.5, x0=None):
+    def mg_available(self):
+        """True when the grid coarsens far enough for the multigrid preconditioner."""
+        return _lib.lib().h8_mg_workspace_bytes(C.byref(self.grid32)) > 0
+
+    def mg_workspace(self):
+        import torch
+        ws = self._ws.get("mg")
+        if ws is None:
+            nbytes = _lib.lib().h8_mg_workspace_bytes(C.byref(self.grid32))
+            if nbytes == 0:
+                raise ValueError(f"multigrid: a {self.nx}x{self.ny}x{self.nz} grid does not coarsen far enough "
+                                 f"(halving its even axes)")
+            ws = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+            self._ws["mg"] = ws
+        return ws
+
+    def refined(self, f, tol=1e-8, maxiter=None, delta=0.5, x0=None, preconditioner="jacobi"):
         """FP32 PCG iterations with FP64 residual replacement; returns (u, stats).
-        x0: initial guess (full length; its fixed-DOF entries are ignored)."""
+        x0: initial guess (full length; its fixed-DOF entries are ignored).
+        preconditioner: 'jacobi' (z = dinv r) or 'mg' (z = one multigrid
+        V-cycle; falls back to 'jacobi' when the coarse operator is singular,
+        stats['preconditioner'] says which ran)."""
         import torch
         if self.dinv is None:
             raise RuntimeError("set_free_mask() must be called before solving")
+        if preconditioner not in ("jacobi", "mg"):
+            raise ValueError(f"preconditioner must be 'jacobi' or 'mg', got {preconditioner!r}")
         f = self._vec(f)
         u = torch.zeros_like(f) if x0 is None else self._vec(x0).clone()
         if maxiter is None:
             maxiter = 10 * int(self.free_mask.sum().item())
         ws = self.workspace(32)
         st = H8PcgStats()
+        used = preconditioner
         with torch.cuda.device(self.device):
-            fn = _lib.lib().h8_refine_solve if x0 is None else _lib.lib().h8_refine_solve_x0
-            rc = fn(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32), ptr(self.K0_32),
-                    ptr(self.dinv), ptr(f), ptr(u), float(tol), int(maxiter), float(delta), ptr(ws), ws.numel(),
-                    C.byref(st), self._stream())
+            rc = None
+            if preconditioner == "mg":
+                mws = self.mg_workspace()
+                rc = _lib.lib().h8_refine_solve_mg(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf),
+                                                   ptr(self.s_e32), ptr(self.K0_32), ptr(self.dinv), ptr(f), ptr(u),
+                                                   float(tol), int(maxiter), float(delta), 0 if x0 is None else 1,
+                                                   ptr(ws), ws.numel(), ptr(mws), mws.numel(), C.byref(st),
+                                                   self._stream())
+                if rc == _lib.H8_MG_DECLINED:
+                    used = "jacobi"
+                    if x0 is not None:
+                        u = self._vec(x0).clone()
+            if used == "jacobi":
+                fn = _lib.lib().h8_refine_solve if x0 is None else _lib.lib().h8_refine_solve_x0
+                rc = fn(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32), ptr(self.K0_32),
+                        ptr(self.dinv), ptr(f), ptr(u), float(tol), int(maxiter), float(delta), ptr(ws), ws.numel(),
+                        C.byref(st), self._stream())
         stats = st.as_dict()
         stats["rc"] = rc
+        stats["preconditioner"] = used
         check(rc)
         return u, stats
+
+    def mg_vcycle(self, r):
+        """z = V(r): one V-cycle of the multigrid preconditioner (FP32), after
+        building the hierarchy for the current scales and free mask."""
+        import torch
+        if self.dinv is None:
+            raise RuntimeError("set_free_mask() must be called first")
+        r = self._vec(r, torch.float32)
+        z = torch.zeros_like(r)
+        ws = self.workspace(32)
+        mws = self.mg_workspace()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().h8_mg_vcycle_f32(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32),
+                                              ptr(self.K0_32), ptr(self.dinv), ptr(r), ptr(z), ptr(ws), ws.numel(),
+                                              ptr(mws), mws.numel(), self._stream()), "h8_mg_vcycle_f32")
+        return z
 
     def element_bilinear(self, u, v, w):
         """out_e = w_e u_e^T K0 v_e (device)."""

@@ -446,11 +446,19 @@ class FESystem:
         op, cm, w_t, b = self._constrained_setup(Fv)
         return ReducedSystem(self, op, cm, w_t.cpu().numpy(), b)
 
-    def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner="jacobi",
+    def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner=None,
                             precision="fp64", delta=0.5, F=None):
-        """solve_static, returning the full displacement as a device tensor (no host copy)."""
+        """solve_static, returning the full displacement as a device tensor (no host copy).
+        preconditioner: None (the default: 'jacobi' in FP64, the reference's
+        choice; 'mg' with precision='mixed'), 'jacobi', or 'mg' (geometric
+        multigrid V-cycle, precision='mixed' only)."""
         self._require_stiffness("solve_static")
+        if preconditioner is None:
+            preconditioner = "mg" if precision == "mixed" and method in (None, "pcg") else "jacobi"
         if self._constraints:
+            if preconditioner == "mg" and precision == "mixed":
+                raise NotImplementedError("solve_static with linear constraints supports the FP64 Jacobi-PCG "
+                                          "(precision='fp64') on backend='hip'.")
             if method not in (None, "pcg") or preconditioner != "jacobi" or precision != "fp64" or x0 is not None:
                 raise NotImplementedError("solve_static with linear constraints supports the FP64 Jacobi-PCG "
                                           "(method None or 'pcg', no x0) on backend='hip'.")
@@ -467,14 +475,20 @@ class FESystem:
             raise NotImplementedError(
                 f"solve_static(method='pcg', preconditioner={preconditioner!r}) is not supported with "
                 f"backend='hip'; use preconditioner='jacobi'.")
+        elif preconditioner == "mg":
+            if precision != "mixed":
+                raise NotImplementedError("solve_static(preconditioner='mg') is the preconditioner of the FP32 "
+                                          "iterations: use precision='mixed' (the FP64 solve is the reference's "
+                                          "Jacobi-PCG).")
         elif preconditioner != "jacobi":
             raise ValueError(
                 f"solve_static(method='pcg'): unknown preconditioner="
-                f"{preconditioner!r} -- choose 'jacobi', 'ssor', or 'ic0'.")
+                f"{preconditioner!r} -- choose 'jacobi', 'ssor', 'ic0' or 'mg'.")
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         op = self._op
         op.set_free_mask(self._free_mask(), preconditioner="jacobi" if preconditioner else None)
+        mg = preconditioner == "mg"
         b, uD = self._dirichlet_rhs(op, self.F if F is None else F)
         # Supports that leave rigid-body motions free make Kff singular.  As in
         # fe/solver.py:1645-1678: a load with a component along them (beyond 1e-6 of
@@ -494,6 +508,9 @@ class FESystem:
                     f"them -- an under-constrained model (likely a missing boundary condition). Add "
                     f"supports, or balance the load to zero net force and moment.")
             b = b - Q @ proj
+            mg = False            # the coarse operators would be singular too
+        if mg and not op.mg_available():
+            mg = False            # the grid does not coarsen far enough
         n_free = self.n_dof - len(self.fixed_dofs)
         if maxiter is None:
             maxiter = 10 * n_free
@@ -511,7 +528,8 @@ class FESystem:
         if precision == "fp64":
             x, stats = op.pcg(b, tol=tol, maxiter=maxiter, x0=x0_t)
         else:
-            x, stats = op.refined(b, tol=tol, maxiter=maxiter, delta=delta, x0=x0_t)
+            x, stats = op.refined(b, tol=tol, maxiter=maxiter, delta=delta, x0=x0_t,
+                                  preconditioner="mg" if mg else "jacobi")
         if Q is not None:
             x = x - Q @ (Q.T @ x)
         if uD is not None:

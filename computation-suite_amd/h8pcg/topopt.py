@@ -41,13 +41,17 @@ class SIMPProblem:
     """Clamped structured H8 body under a fixed load, stiffness scaled per element."""
 
     def __init__(self, nx, ny, nz, F, fixed_dofs, E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0), p=3.0, rho_min=1e-9,
-                 device="cuda", tol=1e-10, precision="fp64", maxiter=None, store_forward=True, D=None):
+                 device="cuda", tol=1e-10, precision="fp64", maxiter=None, store_forward=True, D=None,
+                 preconditioner=None):
         torch = require_gpu()
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.m = self.nx * self.ny * self.nz
         self.p, self.rho_min = float(p), float(rho_min)
         self.tol, self.precision, self.maxiter = tol, precision, maxiter
         self.store_forward = store_forward
+        # FP32 iterations (precision "mixed"): the multigrid V-cycle where the grid
+        # coarsens far enough (None = that choice), else "jacobi"
+        self.preconditioner = preconditioner
         X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
                       [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float) * np.asarray(h, dtype=float)
         D0 = D_solid3d(Material(E=E0, nu=nu)) if D is None else np.asarray(D, dtype=float)
@@ -80,7 +84,8 @@ class SIMPProblem:
         tol = self.tol if tol is None else tol
         precision = self.precision if precision is None else precision
         if precision == "mixed":
-            u, st = self.op.refined(b, tol=tol, maxiter=self.maxiter, x0=x0)
+            pc = self.preconditioner or ("mg" if self.op.mg_available() else "jacobi")
+            u, st = self.op.refined(b, tol=tol, maxiter=self.maxiter, x0=x0, preconditioner=pc)
         else:
             u, st = self.op.pcg(b, tol=tol, maxiter=self.maxiter, x0=x0)
         self.last_stats = st

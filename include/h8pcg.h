@@ -44,6 +44,8 @@ extern "C" {
 #define H8_MAXITER 4        /* RuntimeError: not converged in maxiter        */
 #define H8_ZERO_DIAG 5      /* ValueError: zero Jacobi diagonal at a free DOF */
 #define H8_ERR_WORKSPACE 6  /* ValueError: workspace too small               */
+#define H8_MG_DECLINED 7    /* the multigrid preconditioner does not apply (singular
+                               coarse operator): solve with the Jacobi form instead */
 
 #define H8_VARIANT_AUTO 0   /* library default for the precision             */
 #define H8_VARIANT_NODE 1   /* node-centric gather (VALU), deterministic     */
@@ -177,6 +179,37 @@ int h8_refine_solve_x0(const h8_grid* g, const double* s_e64, const double* K064
                        const float* s_e32, const float* K032, const double* dinv64,
                        const double* f, double* u, double tol, int64_t maxiter, double delta,
                        void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
+
+/* Geometric multigrid preconditioner of the FP32 iterations.  The hierarchy
+ * halves the even axes of the grid until a level has at most 128 DOFs; every
+ * coarse operator is the Galerkin product P^T A P with trilinear P restricted
+ * to the free DOFs (27-point block stencils, FP32), the coarsest is inverted
+ * densely, smoothing is degree-2 Chebyshev-Jacobi on Gershgorin bounds.
+ * h8_mg_workspace_bytes: the hierarchy's workspace (bytes), 0 when the grid
+ * does not coarsen to at most 512 DOFs that way (then only the Jacobi form
+ * applies).  No counterpart in the reference (its PCG preconditioners are
+ * Jacobi, SSOR and IC(0), fe/iterative_solvers.py:193-360). */
+size_t h8_mg_workspace_bytes(const h8_grid* g);
+
+/* h8_refine_solve (use_x0 = 0) or h8_refine_solve_x0 (use_x0 = 1) with the
+ * V-cycle as the preconditioner of the FP32 iterations (z = V(r) instead of
+ * dinv r); same arguments, stopping rule, statistics and errors, plus the
+ * hierarchy's workspace mg_ws.  The hierarchy is built from s_e64, K064 and
+ * the free mask (dinv64 != 0) on every call.  Returns H8_MG_DECLINED (nothing
+ * solved) when the coarsest operator is singular: supports that leave
+ * rigid-body motions free.  Returns the u of h8_refine_solve to within tol. */
+int h8_refine_solve_mg(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                       const float* K032, const double* dinv64, const double* f, double* u, double tol,
+                       int64_t maxiter, double delta, int use_x0, void* ws, size_t ws_bytes, void* mg_ws,
+                       size_t mg_ws_bytes, h8_pcg_stats* stats, void* stream);
+
+/* One V-cycle z = V(r) (FP32 vectors, r zero at fixed DOFs) after building the
+ * hierarchy: the preconditioner of h8_refine_solve_mg as an operator, for
+ * checking it.  ws: a workspace of h8_pcg_workspace_bytes(g, 32).
+ * Diagnostics helper with no counterpart in the reference. */
+int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                     const float* K032, const double* dinv64, const float* r, float* z, void* ws,
+                     size_t ws_bytes, void* mg_ws, size_t mg_ws_bytes, void* stream);
 
 /* ---- element bilinear forms (SIMP sensitivities) ----------------------- */
 /* out[e] = w[e] * u_e^T K0 v_e for every element (u_e = u at the element's 24

@@ -741,3 +741,127 @@ def solve_constrained(K, F, fixed_dofs, fixed_values, constraints):
     Kc = (T.T @ K @ T).tocsc()
     x = spla.spsolve(Kc, T.T @ (F - K @ w))
     return T @ x + w
+
+
+# ---------------------------------------------------------------------------
+# Multigrid V-cycle (no counterpart in the reference: the checker of the HIP
+# library's multigrid preconditioner, h8_mg_vcycle_f32 / h8_refine_solve_mg).
+# The coarse operators here are scipy triple products P^T A P of assembled
+# matrices, independent of the library's element-local and stencil forms.
+# ---------------------------------------------------------------------------
+MG_COARSE_DOF = 128
+MG_DENSE_MAX = 512
+MG_CHEB_RATIO = 30.0
+
+
+def _mg_prolong_1d(n, r):
+    """(n+1) x (n/r+1) linear interpolation along one axis (r = 1: identity)."""
+    if r == 1:
+        return sp.identity(n + 1, format="csr")
+    rows, cols, vals = [], [], []
+    for i in range(n + 1):
+        if i % 2 == 0:
+            rows.append(i)
+            cols.append(i // 2)
+            vals.append(1.0)
+        else:
+            rows += [i, i]
+            cols += [i // 2, i // 2 + 1]
+            vals += [0.5, 0.5]
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n + 1, n // 2 + 1))
+
+
+def mg_levels(nx, ny, nz):
+    """Element counts of the hierarchy: halve the even axes until a level has
+    at most MG_COARSE_DOF DOFs or no axis is even.  Returns [(nx, ny, nz, rx,
+    ry, rz)] (ratios 0 on the coarsest) or None when the coarsest level has
+    more than MG_DENSE_MAX DOFs or there is one level only."""
+    out = []
+    while True:
+        ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+        r = [2 if v % 2 == 0 else 1 for v in (nx, ny, nz)]
+        if ndof <= MG_COARSE_DOF or r == [1, 1, 1] or len(out) == 11:
+            out.append((nx, ny, nz, 0, 0, 0))
+            break
+        out.append((nx, ny, nz, *r))
+        nx, ny, nz = nx // r[0], ny // r[1], nz // r[2]
+    last = out[-1]
+    if len(out) < 2 or 3 * (last[0] + 1) * (last[1] + 1) * (last[2] + 1) > MG_DENSE_MAX:
+        return None
+    return out
+
+
+def mg_hierarchy(K, free, dims, K_abs=None, store32=False):
+    """Levels of the V-cycle for the full-length K (n x n) with free mask
+    `free` (bool, n) on the box grid dims = (nx, ny, nz) (node order
+    k (nx+1)(ny+1) + i (ny+1) + j, DOF 3 node + c).  A_0 = M K M; A_{l+1} =
+    P_l^T A_l P_l with P_l = M_l (Pz x Px x Py x I3).  Each level keeps A,
+    dinv (0 at masked DOFs), lam (Gershgorin bound of D^-1 A over the free rows,
+    times 1.0001; level 0 from K_abs = the assembly of s_e |K0| when given).
+    store32: every coarse operator rounded to FP32 as soon as it is formed
+    (the library stores them so; the next product and the dense inverse of
+    the coarsest level then start from the rounded values)."""
+    plan = mg_levels(*dims)
+    if plan is None:
+        raise ValueError("grid does not coarsen far enough")
+    m = np.asarray(free, dtype=float)
+    Mm = sp.diags(m)
+    A = (Mm @ K @ Mm).tocsr()
+    levels = []
+    for li, (nx, ny, nz, rx, ry, rz) in enumerate(plan):
+        diag = A.diagonal()
+        dinv = np.where(diag > 0, 1.0 / np.where(diag > 0, diag, 1.0), 0.0)
+        if li == 0 and K_abs is not None:
+            rs = np.asarray(abs(K_abs).sum(axis=1)).ravel()
+        else:
+            rs = np.asarray(abs(A).sum(axis=1)).ravel()
+        lam = 1.0001 * float(np.max(rs * dinv))
+        lev = {"A": A, "dinv": dinv, "lam": lam, "dims": (nx, ny, nz)}
+        levels.append(lev)
+        if rx == 0:
+            break
+        Pn = sp.kron(sp.kron(_mg_prolong_1d(nz, rz), _mg_prolong_1d(nx, rx)), _mg_prolong_1d(ny, ry))
+        P = (sp.diags((dinv != 0).astype(float)) @ sp.kron(Pn, sp.identity(3))).tocsr()
+        lev["P"] = P
+        A = (P.T @ A @ P).tocsr()
+        if store32:
+            A.data = A.data.astype(np.float32).astype(np.float64)
+    c = levels[-1]
+    Ad = c["A"].toarray()
+    masked = c["dinv"] == 0
+    Ad[masked, :] = 0.0
+    Ad[:, masked] = 0.0
+    Ad[masked, masked] = 1.0
+    c["Ainv"] = np.linalg.inv(Ad)
+    for lev in levels:
+        lam = lev["lam"]
+        a = lam / MG_CHEB_RATIO
+        theta, delta = 0.5 * (lam + a), 0.5 * (lam - a)
+        sigma = theta / delta
+        rho0 = 1.0 / sigma
+        rho1 = 1.0 / (2.0 * sigma - rho0)
+        lev["cheb"] = ((0.0, 1.0 / theta), (rho1 * rho0, 2.0 * rho1 / delta))
+    return levels
+
+
+def _mg_smooth(lev, x, b, d, zero_x):
+    """Two Chebyshev steps d = ca d + cb dinv (b - A x), x += d (Saad, Iterative
+    Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1)."""
+    for s, (ca, cb) in enumerate(lev["cheb"]):
+        r = b if (zero_x and s == 0) else b - lev["A"] @ x
+        d = (0.0 if (zero_x and s == 0) else ca * d) + cb * lev["dinv"] * r
+        x = d.copy() if (zero_x and s == 0) else x + d
+    return x, d
+
+
+def mg_vcycle(levels, r, level=0):
+    """z = V(r): Chebyshev(2) pre-smoothing from x = 0, coarse correction with
+    the restricted residual, Chebyshev(2) post-smoothing (same polynomial)."""
+    lev = levels[level]
+    if level == len(levels) - 1:
+        return lev["Ainv"] @ r
+    x, d = _mg_smooth(lev, np.zeros_like(r), r, np.zeros_like(r), True)
+    bc = lev["P"].T @ (r - lev["A"] @ x)
+    x = x + lev["P"] @ mg_vcycle(levels, bc, level + 1)
+    x, _ = _mg_smooth(lev, x, r, d, False)
+    return x

@@ -25,3 +25,9 @@ def gpu():
     if not torch.cuda.is_available():
         pytest.fail("gpu test collected on a machine without a visible GPU")
     return torch.device("cuda", 0)
+
+
+@pytest.fixture(scope="session")
+def h8(gpu):
+    import h8pcg
+    return h8pcg
