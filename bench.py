@@ -1,17 +1,27 @@
 #!/usr/bin/env python
-"""Benchmark of the H8 linear-elastic Jacobi-PCG path on one MI355X.
+"""Benchmark of the H8 linear-elastic PCG path on one MI355X.
 
-Workload (the large single solve): 128x32x32 H8 cantilever (140,481 nodes,
-418,176 free DOF), per-element Young's modulus 10**U(9,12) (seed 2), nu = 0.3,
-x=0 face clamped, consistent -z traction of 1e6 total on the x=128 face.
-A step is one full solve to a true relative residual < 1e-8: FP32 PCG
-iterations with FP64 residual replacement (the chosen form).  The plain FP64
-Jacobi-PCG GPU solve of the same system is timed beside it.
+Headline workload (the large single solve): 128x32x32 H8 cantilever
+(140,481 nodes, 418,176 free DOF), per-element Young's modulus 10**U(9,12)
+(seed 2), nu = 0.3, x=0 face clamped, consistent -z traction of 1e6 total on
+the x=128 face.  A step is one full solve to a true relative residual < 1e-8
+in the chosen form: FP32 PCG iterations preconditioned by the multigrid
+V-cycle, with FP64 residual replacement.  The plain FP64 Jacobi-PCG GPU solve
+of the same system (the reference's algorithm) is timed beside it.
 
-value = DOF*iters/s = sum over solves of n_free * (FP32 iterations + FP64
-replacements) / wall time, inputs resident in HBM.
+value = n_free * k64 / t_refined: the FP64 Jacobi-PCG iterations k64 that the
+reference's algorithm needs for this system at this tolerance, delivered per
+second by the chosen form (DOF x reference iterations per second).  Both forms
+solve the same system to the same true residual, so value is proportional to
+1 / time-to-solution; the raw PCG throughputs of both forms are reported
+beside it.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+The target block times the large adjoint workload: 256x64x64 SIMP cantilever
+(3,244,800 free DOF), rho ~ U(0.2, 1) (seed 5), p = 3, F_z = -1 at (256,32,0):
+the FP32-refined forward solve and the compliance gradient against the plain
+FP64 forward solve, with the gradient checked by central differences.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-target]
 """
 from __future__ import annotations
 
@@ -29,6 +39,7 @@ for p in (ROOT, os.path.join(ROOT, "computation-suite_amd")):
 
 METRIC = "PCG DOF·iters/s + %HBM roofline, FP32-refined vs FP64 @1e-8 rel residual"
 HBM_PEAK = 8.0e12
+MG_COARSE_DOF, MG_DENSE_MAX = 128, 512
 
 
 def log(*a):
@@ -51,9 +62,67 @@ def build_system(nx, ny, nz, seed):
     return fs, E
 
 
-def cpu_baseline(nx, ny, nz, E, F, iters):
-    """Oracle (NumPy/SciPy CSR, FP64) Jacobi-PCG: `iters` iterations, 1 thread."""
+# ---------------------------------------------------------------------------
+# algorithmic bytes of the multigrid-preconditioned FP32 iteration
+# ---------------------------------------------------------------------------
+def mg_levels(nx, ny, nz):
+    """[(ndof, nnodes)] of the library's hierarchy (halve the even axes until
+    a level has at most 128 DOFs or no axis is even)."""
+    out = []
+    while True:
+        nn = (nx + 1) * (ny + 1) * (nz + 1)
+        out.append((3 * nn, nn))
+        r = [2 if v % 2 == 0 else 1 for v in (nx, ny, nz)]
+        if 3 * nn <= MG_COARSE_DOF or r == [1, 1, 1] or len(out) == 12:
+            return out
+        nx, ny, nz = nx // r[0], ny // r[1], nz // r[2]
+
+
+def mg_iteration_bytes(nx, ny, nz):
+    """Algorithmic HBM bytes of one FP32 iteration of the multigrid-preconditioned
+    refined solve (FP32 words; stencil levels 243 words per node):
+      PCG: direction + product (z, p_old, s_e in; p, q out), update (x, r, p,
+           q, dinv in; x, r out), r.z (r, z in);
+      V-cycle, level 0 (n DOFs, m elements): four products (u, s_e in; y out),
+           the first smoothing step (b, dinv in; d, x out), three more (x, b, y,
+           d, dinv in; d, x out), restriction (b, y, mask in; coarse b out),
+           prolongation (x, mask, coarse x in; x out);
+      stencil level (n DOFs, N nodes): first step (b, dinv in; d, x out), three
+           stencil steps (stencil, x, b, d, dinv in; d, x out), one stencil
+           product (stencil, x in; y out), restriction, prolongation;
+      coarsest: the dense inverse and its in / out vectors."""
+    levels = mg_levels(nx, ny, nz)
+    m = nx * ny * nz
+    n0 = levels[0][0]
+    w = 4
+    pcg = w * (5 * n0 + m) + w * 7 * n0 + w * 2 * n0
+    cyc = 0
+    for li, (n, N) in enumerate(levels):
+        if li == len(levels) - 1:
+            cyc += w * (n * n + 2 * n)
+            break
+        nc = levels[li + 1][0]
+        if li == 0:
+            cyc += 4 * w * (2 * n + m) + w * 4 * n + 3 * w * 7 * n
+        else:
+            cyc += w * 4 * n + 3 * (w * 243 * N + w * 6 * n) + (w * 243 * N + w * 2 * n)
+        cyc += w * (3 * n + nc) + w * (3 * n + nc)
+    return pcg + cyc, levels
+
+
+def cheb_st_bytes(n1, N1):
+    """One Chebyshev step on a stencil level: the stencil (243 words per node),
+    x, b, d, dinv in; d, x out."""
+    return 4 * 243 * N1 + 4 * 6 * n1
+
+
+# ---------------------------------------------------------------------------
+# CPU legs (the oracle: test infrastructure, used here only as the checker
+# and as the timed CPU baseline)
+# ---------------------------------------------------------------------------
+def cpu_legs(nx, ny, nz, E, F, iters, ref_tol):
     import numpy as np
+    import torch
     from threadpoolctl import threadpool_limits
     from oracle import h8_oracle as o
     t0 = time.time()
@@ -64,62 +133,51 @@ def cpu_baseline(nx, ny, nz, E, F, iters):
     fixed = [3 * int(n) + c for n in clamp for c in range(3)]
     free, fixed = o.partition(K.shape[0], fixed)
     Kff, Ff = o.reduced_system(K, F, free, fixed)
-    log(f"cpu baseline: assembly {time.time() - t0:.1f}s, nnz={Kff.nnz}")
+    log(f"cpu: assembly {time.time() - t0:.1f}s, nnz={Kff.nnz}")
     with threadpool_limits(1):
         M = o.jacobi_preconditioner(Kff)
         t1 = time.perf_counter()
         _, k = o.preconditioned_cg(Kff, Ff, M=M, tol=1e-8, max_iters_hard=iters)
         dt = time.perf_counter() - t1
-    return K, free, {"value": len(free) * k / dt, "unit": "DOF·iters/s", "cores": 1, "kind": "port",
-                     "sample": f"{k} Jacobi-PCG iterations of the same 128x32x32 system (SciPy CSR, FP64, "
-                               f"assembly excluded), {dt:.1f} s",
-                     "host_cpu": _cpu_name(), "host_cores": os.cpu_count()}
-
-
-KERNEL_NAMES = {  # (variant, precision) -> kernel symbol of the fused direction + K*p kernel
-    (1, 32): "k_node<float, 8, 8, 4, 1>", (1, 64): "k_node<double, 8, 8, 4, 1>",
-    (2, 32): "k_mfma<float, 16, 8, 8, 1>", (2, 64): "k_mfma<double, 8, 8, 8, 1>",
-    (3, 32): "k_node_persist<float, 8, 8, 4, 1>", (3, 64): "k_node_persist<double, 8, 8, 4, 1>",
-    (4, 32): "k_node2<float, 8, 8, 4, 1>", (4, 64): "k_node2<double, 8, 8, 4, 1>",
-    (5, 32): "k_sector<float, 16, 16, 1, %s>", (5, 64): "k_sector<double, 16, 16, 1, double>",
-    (6, 32): "k_sector<float, 8, 16, 1, %s>", (6, 64): "k_sector<double, 8, 16, 1, double>",
-    (7, 32): "k_resident<float, 1024, true, %s>", (7, 64): "k_resident<double, 512, false, double>",
-}
-
-
-DEFER_X_MIN_DOF = 1000000   # FP32 sector kernels apply the deferred x update from here
-
-
-def deferred_x(variant, precision, n_dof):
-    """True when the FP32 sector kernel applies the previous x += alpha p while
-    staging p (mode 3), so the update kernel streams only r, q, dinv and r."""
-    return precision == 32 and n_dof >= DEFER_X_MIN_DOF and variant in (5, 6)
-
-
-def kernel_name(variant, precision, arith=0, n_dof=0):
-    """Symbol of the fused direction kernel; FP32 sector and resident kernels
-    carry their element-product arithmetic (FP32 unless arith = 64)."""
-    n = KERNEL_NAMES.get((variant, precision), "unknown")
-    if deferred_x(variant, precision, n_dof):
-        n = n.replace(", 1, %s>", ", 3, %s>")
-    return n % ("double" if arith == 64 else "float") if "%s" in n else n
-RESIDENT_REPS = 100   # iterations per launch of the resident solver's timing run
-
-
-def profiled_traffic(dims, kernel):
-    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summaries (profiles/*_<dims>_traffic.md), or None."""
-    import glob
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_%dx%dx%d_traffic.md" % dims))):
-        try:
-            for line in open(path):
-                if line.startswith("traffic-json:"):
-                    d = json.loads(line.split(":", 1)[1])
-                    if d["kernel"] == kernel:
-                        return d["bytes_per_launch"], os.path.relpath(path, ROOT)
-        except OSError:
-            pass
-    return None, None
+        base = {"value": len(free) * k / dt, "unit": "DOF·iters/s", "cores": 1, "kind": "port",
+                "sample": f"{k} Jacobi-PCG iterations of the same {nx}x{ny}x{nz} system (SciPy CSR, FP64, "
+                          f"assembly excluded), {dt:.1f} s", "host_cpu": _cpu_name(), "host_cores": os.cpu_count()}
+        # the FP64 reference solution: the oracle's Jacobi-PCG from zero to ref_tol
+        t2 = time.perf_counter()
+        uf, kref = o.preconditioned_cg(Kff, Ff, M=M, tol=ref_tol, maxiter=20 * len(free))
+        log(f"cpu: reference solution, {kref} iterations to {ref_tol:g}, {time.perf_counter() - t2:.1f} s")
+    uref = np.zeros(K.shape[0])
+    uref[free] = uf
+    # the reference's torch backend on the host cores (fe/torch_sparse_solver.py:165-262):
+    # CSR FP64 Jacobi-CG, all threads
+    nth = os.cpu_count() or 1
+    torch.set_num_threads(nth)
+    Kt = torch.sparse_csr_tensor(torch.as_tensor(Kff.indptr, dtype=torch.int64),
+                                 torch.as_tensor(Kff.indices, dtype=torch.int64),
+                                 torch.as_tensor(Kff.data), size=Kff.shape)
+    b = torch.as_tensor(Ff)
+    dinv = 1.0 / torch.as_tensor(Kff.diagonal())
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = dinv * r
+    p = z.clone()
+    rz = torch.dot(r, z)
+    t3 = time.perf_counter()
+    nt = 30
+    for _ in range(nt):
+        q = Kt @ p
+        a = rz / torch.dot(p, q)
+        x += a * p
+        r -= a * q
+        z = dinv * r
+        rzn = torch.dot(r, z)
+        p = z + (rzn / rz) * p
+        rz = rzn
+    dtt = time.perf_counter() - t3
+    torch_cpu = {"value": len(free) * nt / dtt, "unit": "DOF·iters/s", "cores": nth, "kind": "port",
+                 "sample": f"{nt} Jacobi-CG iterations of the same system, torch CSR FP64 on {nth} threads "
+                           f"(the reference's backend='torch', device='cpu' recurrence), {dtt:.2f} s"}
+    return K, free, uref, kref, base, torch_cpu
 
 
 def _cpu_name():
@@ -132,18 +190,147 @@ def _cpu_name():
     return "unknown"
 
 
+def mg_kernel_times(op, reps):
+    """Live HIP-event timing of the multigrid kernels (h8_mg_time_kernels)."""
+    from h8pcg import _lib
+    ws, mws = op.workspace(32), op.mg_workspace()
+    avg = (C.c_double * 3)()
+    info = (C.c_int64 * 3)()
+    _lib.check(_lib.lib().h8_mg_time_kernels(C.byref(op.grid32), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
+                                             _lib.ptr(op.s_e32), _lib.ptr(op.K0_32), _lib.ptr(op.dinv),
+                                             _lib.ptr(ws), ws.numel(), _lib.ptr(mws), mws.numel(), reps, avg, info,
+                                             _lib.stream_handle(op.device)), "h8_mg_time_kernels")
+    return list(avg), list(info)
+
+
+def roofline_block(op, nx, ny, nz, t_iter_s, reps):
+    """roofline for the dominant kernel of the chosen form, from HIP events on
+    the stream the kernels run on, plus the same for the other kernels and the
+    whole-iteration contract."""
+    n_dof = op.n_dof
+    m = nx * ny * nz
+    (t0, t1, tv), (nlev, n1, N1) = mg_kernel_times(op, reps)
+    b0 = 4 * (2 * n_dof + m)                      # level-0 product: u, s_e in; y out
+    b1 = cheb_st_bytes(n1, N1) if n1 else 0
+    kern = [{"kernel": "k_sector<float, 16, 16, 0, float> (level-0 product y = K x, FP32)", "avg_us": 1e3 * t0,
+             "bytes": b0, "calls_per_cycle": 4}]
+    if n1:
+        kern.append({"kernel": "k_mg_cheb_st (level-1 Chebyshev step: 27-point stencil product + update)",
+                     "avg_us": 1e3 * t1, "bytes": b1, "calls_per_cycle": 3})
+    for k in kern:
+        k["achieved_GBs"] = k["bytes"] / (k["avg_us"] * 1e-6) / 1e9
+        k["frac"] = k["achieved_GBs"] * 1e9 / HBM_PEAK
+        k["share_of_cycle"] = k["calls_per_cycle"] * k["avg_us"] / (1e3 * tv)
+    dom = max(kern, key=lambda k: k["share_of_cycle"])
+    it_bytes, levels = mg_iteration_bytes(nx, ny, nz)
+    return {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": dom["frac"], "traffic": None, "kernel": dom["kernel"], "kernel_avg_us": dom["avg_us"],
+            "bytes_per_launch": dom["bytes"], "timing": f"HIP events around {reps} graph-launched launches",
+            "other_kernels": [k for k in kern if k is not dom],
+            "vcycle_us": 1e3 * tv, "levels": nlev, "level_dofs": [lv[0] for lv in levels],
+            "iteration_bytes": it_bytes, "iteration_us": 1e6 * t_iter_s,
+            "iteration_frac": it_bytes / t_iter_s / HBM_PEAK}
+
+
+# ---------------------------------------------------------------------------
+# the large adjoint workload (SIMP, 256x64x64)
+# ---------------------------------------------------------------------------
+def target_block(dims, seed, fd_n, reps):
+    import numpy as np
+    import torch
+    from h8pcg import topopt
+    nx, ny, nz = dims
+    m = nx * ny * nz
+    NJ, NI = ny + 1, nx + 1
+    n_dof = 3 * NJ * NI * (nz + 1)
+    F = np.zeros(n_dof)
+    F[3 * (nx * NJ + ny // 2) + 2] = -1.0
+    k, j = np.meshgrid(np.arange(nz + 1), np.arange(NJ), indexing="ij")
+    fixed = (3 * (k * NI * NJ + j).ravel()[:, None] + np.arange(3)).ravel()
+    n_free = n_dof - len(fixed)
+    rng = np.random.default_rng(seed)
+    rho_np = rng.uniform(0.2, 1.0, m)
+    pb = topopt.SIMPProblem(nx, ny, nz, F, fixed, p=3.0, rho_min=1e-9, device="cuda:0", tol=1e-8,
+                            precision="mixed")
+    rho = torch.as_tensor(rho_np, device=pb.device)
+
+    def fwd_grad():
+        r = rho.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Cv = topopt.compliance(r, pb)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        Cv.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return float(Cv), r.grad.detach(), t1 - t0, t2 - t1
+
+    fwd_grad()                                    # warm-up (graph capture)
+    runs = [fwd_grad() for _ in range(reps)]
+    C_mixed, g, t_fwd, t_grad = runs[-1]
+    t_fwd = min(r[2] for r in runs)
+    t_grad = min(r[3] for r in runs)
+    st_mixed = dict(pb.last_stats)
+    u_mixed = pb.solve(rho, precision="mixed")
+    log(f"target: mixed forward {1e3 * t_fwd:.1f} ms, gradient {1e3 * t_grad:.1f} ms, {st_mixed}")
+    # plain FP64 Jacobi-PCG forward solve of the same system
+    pb.solve(rho, tol=1e-2, precision="fp64")     # graph capture
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    u64 = pb.solve(rho, tol=1e-8, precision="fp64")
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - t3
+    st64 = dict(pb.last_stats)
+    log(f"target: fp64 forward {1e3 * t_fp64:.1f} ms, {st64['iterations']} iterations")
+    # FP64 reference solution (GPU FP64 PCG at 1e-12) and the true residual of the mixed one
+    uref = pb.solve(rho, tol=1e-12, precision="fp64")
+    err = float(torch.linalg.norm(u_mixed - uref) / torch.linalg.norm(uref))
+    err64 = float(torch.linalg.norm(u64 - uref) / torch.linalg.norm(uref))
+    b = pb.op.masked(pb.F)
+    res = float(torch.linalg.norm(pb.op.masked(b, pb.op.matvec(u_mixed))) / torch.linalg.norm(b))
+    it_bytes, levels = mg_iteration_bytes(nx, ny, nz)
+    kernels = roofline_block(pb.op, nx, ny, nz, t_fwd / max(1, st_mixed["inner_iterations"]), 20)
+    out = {"workload": f"{nx}x{ny}x{nz} SIMP cantilever, rho~U(0.2,1) (seed {seed}), p=3, rho_min=1e-9, E0=1, "
+                       f"nu=0.3, clamped x=0, F_z=-1 at ({nx},{ny // 2},0)",
+           "n_free": n_free, "elements": m,
+           "forward_ms": 1e3 * t_fwd, "gradient_ms": 1e3 * t_grad, "forward_plus_gradient_ms": 1e3 * (t_fwd + t_grad),
+           "fp64_forward_ms": 1e3 * t_fp64, "fp64_iterations": st64["iterations"],
+           "speedup_forward_vs_fp64": t_fp64 / t_fwd,
+           "speedup_forward_plus_gradient_vs_fp64_forward_plus_gradient": (t_fp64 + t_grad) / (t_fwd + t_grad),
+           "fp32_iterations": st_mixed["inner_iterations"], "fp64_replacements": st_mixed["outer"],
+           "preconditioner": st_mixed.get("preconditioner"),
+           "true_rel_residual": st_mixed["true_rel_residual"], "true_rel_residual_fp64_recomputed": res,
+           "rel_err_vs_gpu_fp64_tol1e-12": err, "fp64_rel_err_vs_gpu_fp64_tol1e-12": err64,
+           "compliance": C_mixed, "forward_stored_MB": 8 * n_dof / 1e6,
+           "roofline": kernels}
+    if fd_n > 0:
+        els = np.random.default_rng(seed + 1).choice(m, fd_n, replace=False)
+        t4 = time.perf_counter()
+        fd, _ = topopt.fd_check(pb, rho, els, h=1e-6, tol=1e-12)
+        gg = g.cpu().numpy()[els]
+        rel = np.abs(fd - gg) / np.abs(gg)
+        out.update({"fd_elements": int(fd_n), "fd_h": 1e-6, "fd_max_rel_diff": float(rel.max()),
+                    "fd_median_rel_diff": float(np.median(rel)), "fd_s": time.perf_counter() - t4})
+        log(f"target: FD on {fd_n} elements, max rel diff {rel.max():.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=100)
+    ap.add_argument("--ref-tol", type=float, default=1e-12)
     ap.add_argument("--dims", default="128,32,32")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--delta", type=float, default=0.5)
-    ap.add_argument("--arith", type=int, default=0,
-                    help="element-product arithmetic of the FP32 iterations: 0 = library (FP32), 32, 64")
+    ap.add_argument("--target-dims", default="256,64,64")
+    ap.add_argument("--target-seed", type=int, default=5)
+    ap.add_argument("--target-fd", type=int, default=32)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); the multi-GPU "
@@ -151,29 +338,26 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    import h8pcg
-    from h8pcg import _lib
+    import h8pcg  # noqa: F401
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     nx, ny, nz = (int(v) for v in a.dims.split(","))
     t0 = time.time()
     fs, E = build_system(nx, ny, nz, a.seed)
     op = fs.operator
-    op.set_arith(a.arith)
     n_free = fs.n_dof - len(fs.fixed_dofs)
     m = nx * ny * nz
     log(f"setup {time.time() - t0:.1f}s: n_dof={fs.n_dof} n_free={n_free} m={m}")
-    # kernel choice per precision for this shape (timed once, outside the timed region)
     fs.solve_static_device(tol=1e-2)           # builds dinv (free mask) for the tuner
-    tuned = op.autotune(cache=False)
+    tuned = op.autotune(precisions=(64,), cache=False)   # the FP64 Jacobi-PCG kernel choice
     log("autotune", tuned)
 
-    def solve(precision, tol=1e-8):
-        return fs.solve_static_device(tol=tol, precision=precision, delta=a.delta)
+    def solve(precision, tol=1e-8, **kw):
+        return fs.solve_static_device(tol=tol, precision=precision, delta=a.delta, **kw)
 
     for _ in range(max(1, a.warmup)):
         solve("mixed")
         log("warmup refined", fs.last_solve_stats)
-    # timed region: K refined solves
+    # timed region: K refined solves (multigrid-preconditioned FP32 iterations)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     stats = []
@@ -182,11 +366,11 @@ def main():
         stats.append(dict(fs.last_solve_stats))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
-    iters = [s["inner_iterations"] + s["outer"] for s in stats]
-    value = n_free * sum(iters) / dt
-    log("refined", stats[-1])
+    t_solve = dt / a.steps
+    s_ = stats[-1]
+    log("refined", s_)
 
-    # plain FP64 GPU path, same system
+    # plain FP64 Jacobi-PCG GPU path, same system
     solve("fp64")
     torch.cuda.synchronize()
     t2 = time.perf_counter()
@@ -195,91 +379,65 @@ def main():
     dt64 = time.perf_counter() - t2
     st64 = dict(fs.last_solve_stats)
     log("fp64", st64)
-    # accuracy: FP64 reference solution at tol 1e-12 (GPU FP64 PCG)
-    uref = solve("fp64", tol=1e-12)
-    err = float(torch.linalg.norm(u - uref) / torch.linalg.norm(uref))
-    err64 = float(torch.linalg.norm(u64 - uref) / torch.linalg.norm(uref))
+    k64 = st64["iterations"]
+    value = n_free * k64 / t_solve
 
-    # dominant kernel of the FP32 iterations, timed alone with HIP events on its
-    # stream: the resident solver (one launch of RESIDENT_REPS iterations), or
-    # the fused direction + K*p kernel of the streaming loop
-    dinv = op.dinv
+    # the Jacobi form of the refined solve (FP32 iterations without the V-cycle)
+    solve("mixed", preconditioner="jacobi")
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    solve("mixed", preconditioner="jacobi")
+    torch.cuda.synchronize()
+    dtj = time.perf_counter() - t3
+    stj = dict(fs.last_solve_stats)
 
-    def time_iteration(grid, prec, reps):
-        ws = op.workspace(prec)
-        s_e, K0 = (op.s_e32, op.K0_32) if prec == 32 else (op.s_e, op.K0buf)
-        avg = (C.c_double * 2)()
-        _lib.check(_lib.lib().h8_time_kernels(C.byref(grid), prec, _lib.ptr(s_e), _lib.ptr(K0), _lib.ptr(dinv),
-                                              _lib.ptr(ws), ws.numel(), reps, avg,
-                                              _lib.stream_handle(op.device)), "h8_time_kernels")
-        return avg[0], avg[1]
+    iters = s_["inner_iterations"]
+    roof = roofline_block(op, nx, ny, nz, t_solve / max(1, iters), 50)
+    b64 = k64 * (13 * n_free * 8 + 8 * m)
+    roof["fp64_iteration_contract_frac"] = b64 / dt64 / HBM_PEAK
+    roof["solve_bytes"] = iters * roof["iteration_bytes"]
+    roof["solve_frac"] = roof["solve_bytes"] / t_solve / HBM_PEAK
 
-    resident = op.grid32.variant == _lib.VARIANT_RESIDENT
-    dir_ms, upd_ms = time_iteration(op.grid32, 32, RESIDENT_REPS if resident else 200)
-    avg64 = time_iteration(op.grid, 64, RESIDENT_REPS if op.grid.variant == _lib.VARIANT_RESIDENT else 200)
-    n_dof = fs.n_dof
-    if deferred_x(op.grid32.variant, 32, n_dof):
-        dir_bytes = 7 * 4 * n_dof + 4 * m      # read r, dinv, p_old, x; write p, q, x; read s_e
-        upd_bytes = 4 * 4 * n_dof              # read r, q, dinv; write r
-    else:
-        dir_bytes = 5 * 4 * n_dof + 4 * m      # read r, dinv, p_old; write p, q; read s_e
-        upd_bytes = 7 * 4 * n_dof              # read x, r, p, q, dinv; write x, r
-    kname = kernel_name(op.grid32.variant, 32, op.grid32.arith, n_dof)
-    if resident:
-        # one launch = RESIDENT_REPS iterations, each with the bytes of a streaming iteration
-        launch_ms = dir_ms * RESIDENT_REPS
-        launch_bytes = (dir_bytes + upd_bytes) * RESIDENT_REPS
-        kdesc = (f"{kname}: the whole FP32 PCG loop in one cooperative launch, vectors resident in LDS "
-                 f"({RESIDENT_REPS} iterations per timed launch)")
-    else:
-        launch_ms, launch_bytes = dir_ms, dir_bytes
-        kdesc = f"{kname}: fused direction + K*p + p.q (FP32 data)"
-    achieved = launch_bytes / (launch_ms * 1e-3)
-    traffic, traffic_src = profiled_traffic((nx, ny, nz), kname)
-    # whole-iteration contract: B_iter = 13 n_f s + m s_c (FP32), B_outer = 6 n_f 8 + 8 m
-    s_ = stats[-1]
-    b_solve = s_["inner_iterations"] * (13 * n_free * 4 + 4 * m) + s_["outer"] * (6 * n_free * 8 + 8 * m)
-    t_solve = dt / a.steps
-    b64 = st64["iterations"] * (13 * n_free * 8 + 8 * m)
-
-    cpu = None
-    true_res_cpu = None
+    cpu = torch_cpu = None
+    acc = {}
     if not a.no_cpu:
-        K, free, cpu = cpu_baseline(nx, ny, nz, E, np.asarray(fs.F), a.cpu_iters)
-        un = u.cpu().numpy()
-        r = (np.asarray(fs.F) - K @ un)[free]
-        true_res_cpu = float(np.linalg.norm(r) / np.linalg.norm(np.asarray(fs.F)[free]))
+        K, free, uref, kref, cpu, torch_cpu = cpu_legs(nx, ny, nz, E, np.asarray(fs.F), a.cpu_iters, a.ref_tol)
+        Fn = np.asarray(fs.F)
+        for name, uu in (("refined", u), ("fp64", u64)):
+            un = uu.cpu().numpy()
+            acc[name] = {"rel_err_vs_cpu_fp64": float(np.linalg.norm(un - uref) / np.linalg.norm(uref)),
+                         "true_rel_residual_cpu": float(np.linalg.norm((Fn - K @ un)[free]) /
+                                                        np.linalg.norm(Fn[free]))}
+        acc["cpu_reference"] = f"oracle (SciPy CSR, FP64) Jacobi-PCG from zero to {a.ref_tol:g}, {kref} iterations"
+
+    target = None
+    if not a.no_target:
+        target = target_block(tuple(int(v) for v in a.target_dims.split(",")), a.target_seed, a.target_fd, 3)
 
     out = {
         "metric": METRIC, "value": value, "unit": "DOF·iters/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": 1e3 * dt / a.steps, "higher_is_better": True, "scaling": "weak",
+        "warmup": a.warmup, "ms_per_step": 1e3 * t_solve, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "value_definition": "n_free x (FP64 Jacobi-PCG iterations of this system to 1e-8, the reference's "
+                            "algorithm) / wall time of the FP32-refined solve to the same true residual",
         "config": {"workload": f"{nx}x{ny}x{nz} H8 cantilever, E=10**U(9,12) per element (seed {a.seed}), "
                                f"nu=0.3, clamped x=0, -1e6 consistent tip traction, true rel. residual < 1e-8",
-                   "n_free": n_free, "elements": m, "delta": a.delta,
-                   "kernels": {"fp32": {"variant": op.grid32.variant, "layers": op.grid32.layers,
-                                        "element_products": "f64" if op.grid32.arith == 64 else "f32"},
-                               "fp64": {"variant": op.grid.variant, "layers": op.grid.layers}}},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "traffic_source": traffic_src,
-                     "kernel": kdesc,
-                     "kernel_avg_us": launch_ms * 1e3, "bytes_per_launch": launch_bytes,
-                     "iteration_us": (dir_ms if resident else dir_ms + upd_ms) * 1e3,
-                     "update_kernel_avg_us": None if resident else upd_ms * 1e3,
-                     "update_bytes_per_launch": None if resident else upd_bytes,
-                     "update_frac": None if resident else upd_bytes / (upd_ms * 1e-3) / HBM_PEAK,
-                     "fp64_kernel": kernel_name(op.grid.variant, 64),
-                     "fp64_iteration_us": (avg64[0] + avg64[1]) * 1e3,
-                     "iteration_contract_frac": b_solve / t_solve / HBM_PEAK,
-                     "fp64_iteration_contract_frac": b64 / dt64 / HBM_PEAK},
+                   "n_free": n_free, "elements": m, "delta": a.delta, "preconditioner": s_.get("preconditioner"),
+                   "fp64_kernel": {"variant": op.grid.variant, "layers": op.grid.layers}},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "refined": {"fp32_iterations": s_["inner_iterations"], "fp64_replacements": s_["outer"],
-                    "true_rel_residual": s_["true_rel_residual"], "true_rel_residual_cpu_check": true_res_cpu,
-                    "rel_err_vs_fp64_tol1e-12": err, "solve_ms": 1e3 * t_solve, "graph_launches": s_["launches"]},
-        "fp64": {"iterations": st64["iterations"], "solve_ms": 1e3 * dt64, "rel_err_vs_fp64_tol1e-12": err64,
-                 "dof_iters_per_s": n_free * st64["iterations"] / dt64},
+        "cpu_baseline_threads": torch_cpu,
+        "refined": {"fp32_iterations": iters, "fp64_replacements": s_["outer"],
+                    "true_rel_residual": s_["true_rel_residual"], "solve_ms": 1e3 * t_solve,
+                    "graph_launches": s_["launches"], "pcg_dof_iters_per_s": n_free * iters / t_solve,
+                    **acc.get("refined", {})},
+        "fp64": {"iterations": k64, "solve_ms": 1e3 * dt64, "dof_iters_per_s": n_free * k64 / dt64,
+                 **acc.get("fp64", {})},
+        "refined_jacobi": {"fp32_iterations": stj["inner_iterations"], "fp64_replacements": stj["outer"],
+                           "solve_ms": 1e3 * dtj},
+        "accuracy_reference": acc.get("cpu_reference"),
         "speedup_refined_vs_fp64": dt64 / t_solve,
+        "target": target,
     }
     print(json.dumps(out), flush=True)
 

@@ -1569,7 +1569,7 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
     const MgLevel& L0 = P.L[0];
     hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(64), 0, s, (long long)(4 * MG_MAX_LEVELS),
                        reinterpret_cast<unsigned*>(P.cst), 0u);
-    hipLaunchKernelGGL(k_mg_mc, dim3(1), dim3(256), 0, s, K064, L0.rx, L0.ry, L0.rz, P.M);
+    hipLaunchKernelGGL(k_mg_mc, dim3(18), dim3(256), 0, s, K064, L0.rx, L0.ry, L0.rz, P.M);
     const long long m = (long long)d.nx * d.ny * d.nz;
     hipLaunchKernelGGL(k_mg_elem_mask, dim3(mg_blocks(m)), dim3(256), 0, s, d, (const float*)L0.dinv, P.emask);
     hipLaunchKernelGGL(k_mg_gersh0, dim3(mg_blocks(d.nnodes)), dim3(256), 0, s, d, s_e32, K064,
@@ -1615,7 +1615,7 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
     for (int l = 0; l + 1 < nl; ++l) {
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
-        const unsigned nb = mg_blocks(F.nnodes), vb = mg_blocks(F.ndof);
+        const unsigned nb = mg_blocks(F.ndof), vb = mg_blocks(F.ndof);
         // pre-smoothing from x = 0: x = d = c0 dinv b, then the second step
         hipLaunchKernelGGL(k_mg_cheb_vec, dim3(vb), dim3(256), 0, s, F.ndof, F.x0, (const float*)F.b,
                            (const float*)nullptr, F.d, (const float*)F.dinv, cst + l, 0, 1, st);
@@ -1628,16 +1628,16 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
             hipLaunchKernelGGL(k_mg_cheb_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x0, F.x1, cst + l, 1, st);
             hipLaunchKernelGGL(k_mg_spmv_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x1, F.y, st);
         }
-        hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.nnodes)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
+        hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.ndof)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
                            (const float*)F.b, (const float*)F.y, (const float*)F.dinv, C.b, st);
     }
     const MgLevel& Lc = L[nl - 1];
-    hipLaunchKernelGGL(k_mg_dense_solve, dim3(mg_blocks(Lc.ndof)), dim3(256), 0, s, (const float*)P.Ainv,
+    hipLaunchKernelGGL(k_mg_dense_solve, dim3(unsigned((Lc.ndof + 3) / 4)), dim3(256), 0, s, (const float*)P.Ainv,
                        int(Lc.ndof), (const float*)Lc.b, Lc.x1, st);
     for (int l = nl - 2; l >= 0; --l) {
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
-        const unsigned nb = mg_blocks(F.nnodes), vb = mg_blocks(F.ndof);
+        const unsigned nb = mg_blocks(F.nnodes), db = mg_blocks(F.ndof), vb = mg_blocks(F.ndof);
         hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz, F.x1,
                            (const float*)C.x1, (const float*)F.dinv, st);
         if (l == 0) {
@@ -1647,8 +1647,8 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
                                    (const float*)F.y, F.d, (const float*)F.dinv, cst, step, 0, st);
             }
         } else {
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x1, F.x0, cst + l, 0, st);
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x0, F.x1, cst + l, 1, st);
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(db), dim3(256), 0, s, F, (const float*)F.x1, F.x0, cst + l, 0, st);
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(db), dim3(256), 0, s, F, (const float*)F.x0, F.x1, cst + l, 1, st);
         }
     }
     LAUNCHCHK();
@@ -1875,6 +1875,97 @@ extern "C" int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const dou
     NodeArgs<float> prod0{};
     prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y;
     return mg_vcycle(d32, P, prod0, nullptr, s);
+}
+
+// Average device time (ms) of the multigrid kernels of one FP32 iteration,
+// each graph-launched `reps` times alone between HIP events on `stream`:
+// avg_ms[0] = the level-0 product y = K x (sector kernel, FP32), avg_ms[1] =
+// one Chebyshev step on level 1 (stencil product fused with the update; level
+// 1 is a stencil level when the hierarchy has more than two levels, else 0),
+// avg_ms[2] = one whole V-cycle.  info[0..2] = levels, DOFs of level 1,
+// nodes of level 1.  Measurement helper with no counterpart in the reference;
+// scribbles on both workspaces.
+extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                                  const float* K032, const double* dinv64, void* ws, size_t ws_bytes, void* mg_ws,
+                                  size_t mg_ws_bytes, int reps, double* avg_ms, int64_t* info, void* stream) {
+    GridDev d;
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e64 || !K064 || !s_e32 || !K032 || !dinv64 || !ws || !mg_ws || !avg_ms || reps < 1)
+        return fail(H8_ERR_ARG, "bad argument");
+    if (ws_bytes < ws_bytes_ref(d)) return fail(H8_ERR_WORKSPACE, "workspace too small");
+    MgPlan P;
+    if (!mg_plan(d, P)) return fail(H8_ERR_ARG, "multigrid: the grid does not coarsen far enough");
+    if (mg_ws_bytes < mg_layout(P, nullptr)) return fail(H8_ERR_WORKSPACE, "multigrid workspace too small");
+    mg_layout(P, (char*)mg_ws);
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    float* r = (float*)carve(p, sizeof(float) * d.ndof);
+    float* z = (float*)carve(p, sizeof(float) * d.ndof);
+    (void)carve(p, sizeof(float) * d.ndof * 3);
+    float* dinv32 = (float*)carve(p, sizeof(float) * d.ndof);
+    const int sb = stream_blocks(d.ndof);
+    hipLaunchKernelGGL(k_cast_f64_f32, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, dinv64, dinv32);
+    hipLaunchKernelGGL(k_fill<float>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, r, 1.0f);
+    hipLaunchKernelGGL(k_fill<float>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, z, 0.0f);
+    LAUNCHCHK();
+    MgLevel& L0 = P.L[0];
+    L0.b = r;
+    L0.x0 = L0.x1 = z;
+    L0.dinv = dinv32;
+    GridDev d32 = d;
+    if (d32.variant != H8_VARIANT_SECTOR && d32.variant != H8_VARIANT_SECTOR_NARROW) {
+        d32.variant = H8_VARIANT_SECTOR;
+        d32.layers = 0;
+    }
+    if ((rc = mg_setup(d, P, s_e64, K064, s_e32, s))) return rc;
+    const MgLevel& L1 = P.L[1];
+    if (L1.nnodes > 0 && P.nlev > 2) {
+        hipLaunchKernelGGL(k_fill<float>, dim3(mg_blocks(L1.ndof)), dim3(256), 0, s, L1.ndof, L1.x0, 1.0f);
+        hipLaunchKernelGGL(k_fill<float>, dim3(mg_blocks(L1.ndof)), dim3(256), 0, s, L1.ndof, L1.b, 1.0f);
+        hipLaunchKernelGGL(k_fill<float>, dim3(mg_blocks(L1.ndof)), dim3(256), 0, s, L1.ndof, L1.d, 0.0f);
+    }
+    NodeArgs<float> prod0{};
+    prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y;
+    GraphExec ex[3];
+    for (int which = 0; which < 3; ++which) {
+        if (which == 1 && P.nlev <= 2) continue;
+        rc = capture_graph([&](hipStream_t cs) -> int {
+            for (int i = 0; i < reps; ++i) {
+                int r1 = H8_OK;
+                if (which == 0) r1 = launch_op<float, MODE_PLAIN>(d32, prod0, cs);
+                else if (which == 1)
+                    hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(L1.ndof)), dim3(256), 0, cs, L1,
+                                       (const float*)L1.x0, L1.x1, (const MgConst*)(P.cst + 1), 1,
+                                       (const PcgState*)nullptr);
+                else r1 = mg_vcycle(d32, P, prod0, nullptr, cs);
+                if (r1) return r1;
+            }
+            LAUNCHCHK();
+            return H8_OK;
+        }, ex[which]);
+        if (rc) return rc;
+    }
+    Ev ev[2];
+    for (auto& e : ev) HIPCHK(e.create());
+    for (int which = 0; which < 3; ++which) {
+        avg_ms[which] = 0.0;
+        if (!ex[which].exec) continue;
+        HIPCHK(hipGraphLaunch(ex[which].exec, s));    // warm
+        HIPCHK(hipEventRecord(ev[0], s));
+        HIPCHK(hipGraphLaunch(ex[which].exec, s));
+        HIPCHK(hipEventRecord(ev[1], s));
+        HIPCHK(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        avg_ms[which] = ms / reps;
+    }
+    if (info) {
+        info[0] = P.nlev;
+        info[1] = P.nlev > 2 ? L1.ndof : 0;
+        info[2] = P.nlev > 2 ? L1.nnodes : 0;
+    }
+    return H8_OK;
 }
 
 extern "C" int h8_refine_solve_mg(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,

@@ -86,7 +86,7 @@ __host__ __device__ constexpr double mg_w(int f, int C, int r) {
 // ratios (rx, ry, rz); child c = cx + 2 cy + 4 cz, entries [c][24][24].
 __global__ void __launch_bounds__(256) k_mg_mc(const double* __restrict__ K0, int rx, int ry, int rz,
                                                double* __restrict__ M) {
-    for (int t = threadIdx.x; t < 8 * 576; t += blockDim.x) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 8 * 576; t += gridDim.x * blockDim.x) {
         const int c = t / 576, row = (t / 24) % 24, col = t % 24;
         const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
         if (cx >= rx || cy >= ry || cz >= rz) {
@@ -248,17 +248,23 @@ __global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, in
                         if (fj < 0 || fj >= F.NJ || fi < 0 || fi >= F.NI || fk < 0 || fk >= F.NK) continue;
                         const double wi = (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
                         const long long fn = ((long long)fk * F.NI + fi) * F.NJ + fj;
-                        for (int s = 0; s < 27; ++s) {
-                            const int gj = fj + s % 3 - 1, gi = fi + (s / 3) % 3 - 1, gk = fk + s / 9 - 1;
-                            // g in J's support
-                            const int uj = gj - Jj * ry, ui = gi - Ji * rx, uk = gk - Jk * rz;
-                            if (uj < -(ry - 1) || uj > ry - 1 || ui < -(rx - 1) || ui > rx - 1 || uk < -(rz - 1) ||
-                                uk > rz - 1)
-                                continue;
-                            if (gj < 0 || gj >= F.NJ || gi < 0 || gi >= F.NI || gk < 0 || gk >= F.NK) continue;
-                            const double w = wi * (uj ? 0.5 : 1.0) * (ui ? 0.5 : 1.0) * (uk ? 0.5 : 1.0);
-                            for (int q = 0; q < 9; ++q) blk[q] += w * double(F.S[(long long)(s * 9 + q) * F.nnodes + fn]);
-                        }
+                        // stencil neighbours g = f + s of f inside J's support (|g - r J| <= r - 1
+                        // per axis) and inside the fine grid
+                        const int sj0 = max(-1, max(Jj * ry - (ry - 1) - fj, -fj)),
+                                  sj1 = min(1, min(Jj * ry + (ry - 1) - fj, F.NJ - 1 - fj));
+                        const int si0 = max(-1, max(Ji * rx - (rx - 1) - fi, -fi)),
+                                  si1 = min(1, min(Ji * rx + (rx - 1) - fi, F.NI - 1 - fi));
+                        const int sk0 = max(-1, max(Jk * rz - (rz - 1) - fk, -fk)),
+                                  sk1 = min(1, min(Jk * rz + (rz - 1) - fk, F.NK - 1 - fk));
+                        for (int sk = sk0; sk <= sk1; ++sk)
+                            for (int si = si0; si <= si1; ++si)
+                                for (int sj = sj0; sj <= sj1; ++sj) {
+                                    const int s = ((sk + 1) * 3 + si + 1) * 3 + sj + 1;
+                                    const int uj = fj + sj - Jj * ry, ui = fi + si - Ji * rx, uk = fk + sk - Jk * rz;
+                                    const double w = wi * (uj ? 0.5 : 1.0) * (ui ? 0.5 : 1.0) * (uk ? 0.5 : 1.0);
+                                    for (int q = 0; q < 9; ++q)
+                                        blk[q] += w * double(F.S[(long long)(s * 9 + q) * F.nnodes + fn]);
+                                }
                     }
         }
         for (int q = 0; q < 9; ++q) C.S[(long long)(nb * 9 + q) * C.nnodes + In] = float(blk[q]);
@@ -378,17 +384,19 @@ __global__ void __launch_bounds__(1024) k_mg_gauss_jordan(double* __restrict__ A
     if (threadIdx.x == 0) *flag = s_bad;
 }
 
-// x = Ainv b on the coarsest level (one thread per row; Ainv is symmetric, so
-// row r is read as column r: adjacent threads read adjacent words).
+// x = Ainv b on the coarsest level: one wave per row, lanes over the columns
+// (Ainv is symmetric, so row r is read as column r), a fixed-order wave sum.
 __global__ void __launch_bounds__(256) k_mg_dense_solve(const float* __restrict__ Ainv, int n,
                                                         const float* __restrict__ b, float* __restrict__ x,
                                                         const PcgState* st) {
     if (!mg_live(st)) return;
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= n) return;
     float acc = 0.0f;
-    for (int j = 0; j < n; ++j) acc += Ainv[(long long)j * n + row] * b[j];
-    x[row] = acc;
+    for (int j = lane; j < n; j += 64) acc += Ainv[(long long)row * n + j] * b[j];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) x[row] = acc;
 }
 
 // Chebyshev step on a vector already multiplied (y = A x; ignored, with x,
@@ -408,22 +416,26 @@ __global__ void __launch_bounds__(256) k_mg_cheb_vec(long long n, float* __restr
     }
 }
 
-// y = A x at node n of a stencil level (3 components).
-__device__ __forceinline__ void mg_apply(const MgLevel& L, const float* __restrict__ x, long long n, float (&y)[3]) {
+// Row (n, c) of y = A x on a stencil level.  Kernels over a level's DOFs use
+// component-major thread indices t = c * nnodes + n, so the lanes of a wave
+// read adjacent stencil words.
+__device__ __forceinline__ float mg_row(const MgLevel& L, const float* __restrict__ x, long long n, int c) {
     int j, i, k;
     mg_decode(L, n, j, i, k);
-    y[0] = y[1] = y[2] = 0.0f;
-#pragma unroll
-    for (int nb = 0; nb < 27; ++nb) {
-        const int mj = j + nb % 3 - 1, mi = i + (nb / 3) % 3 - 1, mk = k + nb / 9 - 1;
-        if (mj < 0 || mj >= L.NJ || mi < 0 || mi >= L.NI || mk < 0 || mk >= L.NK) continue;
-        const long long m = 3 * (((long long)mk * L.NI + mi) * L.NJ + mj);
-        const float x0 = x[m], x1 = x[m + 1], x2 = x[m + 2];
-        const float* s = L.S + (long long)(nb * 9) * L.nnodes + n;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            y[c] += s[(3 * c) * L.nnodes] * x0 + s[(3 * c + 1) * L.nnodes] * x1 + s[(3 * c + 2) * L.nnodes] * x2;
-    }
+    const int j0 = j > 0 ? -1 : 0, j1 = j + 1 < L.NJ ? 1 : 0;
+    const int i0 = i > 0 ? -1 : 0, i1 = i + 1 < L.NI ? 1 : 0;
+    const int k0 = k > 0 ? -1 : 0, k1 = k + 1 < L.NK ? 1 : 0;
+    const float* Sc = L.S + (long long)(3 * c) * L.nnodes + n;
+    float y = 0.0f;
+    for (int dk = k0; dk <= k1; ++dk)
+        for (int di = i0; di <= i1; ++di)
+            for (int dj = j0; dj <= j1; ++dj) {
+                const int nb = ((dk + 1) * 3 + di + 1) * 3 + dj + 1;
+                const long long m = 3 * (n + ((long long)dk * L.NI + di) * L.NJ + dj);
+                const float* s = Sc + (long long)(nb * 9) * L.nnodes;
+                y += s[0] * x[m] + s[L.nnodes] * x[m + 1] + s[2 * L.nnodes] * x[m + 2];
+            }
+    return y;
 }
 
 // Chebyshev step on a stencil level: x_out = x_in + d, d = ca d + cb dinv (b - A x_in).
@@ -431,17 +443,14 @@ __global__ void __launch_bounds__(256) k_mg_cheb_st(MgLevel L, const float* __re
                                                     const MgConst* cst, int step, const PcgState* st) {
     if (!mg_live(st)) return;
     const float ca = step ? cst->c1a : 0.0f, cb = step ? cst->c1b : cst->c0;
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < L.nnodes;
-         n += (long long)gridDim.x * blockDim.x) {
-        float y[3];
-        mg_apply(L, xin, n, y);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const long long q = 3 * n + c;
-            const float dn = ca * L.d[q] + cb * L.dinv[q] * (L.b[q] - y[c]);
-            L.d[q] = dn;
-            xout[q] = xin[q] + dn;
-        }
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = int(t / L.nnodes);
+        const long long n = t - c * L.nnodes, q = 3 * n + c;
+        const float y = mg_row(L, xin, n, c);
+        const float dn = ca * L.d[q] + cb * L.dinv[q] * (L.b[q] - y);
+        L.d[q] = dn;
+        xout[q] = xin[q] + dn;
     }
 }
 
@@ -449,39 +458,40 @@ __global__ void __launch_bounds__(256) k_mg_cheb_st(MgLevel L, const float* __re
 __global__ void __launch_bounds__(256) k_mg_spmv_st(MgLevel L, const float* __restrict__ x, float* __restrict__ y,
                                                     const PcgState* st) {
     if (!mg_live(st)) return;
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < L.nnodes;
-         n += (long long)gridDim.x * blockDim.x) {
-        float v[3];
-        mg_apply(L, x, n, v);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) y[3 * n + c] = v[c];
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = int(t / L.nnodes);
+        const long long n = t - c * L.nnodes;
+        y[3 * n + c] = mg_row(L, x, n, c);
     }
 }
 
-// Restriction of the masked residual: bC = P^T M_F (bF - yF).
+// Restriction of the masked residual: bC = P^T M_F (bF - yF), one thread per
+// coarse DOF (component-major).
 __global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C, int rx, int ry, int rz,
                                                      const float* __restrict__ bF, const float* __restrict__ yF,
                                                      const float* __restrict__ maskF, float* __restrict__ bC,
                                                      const PcgState* st) {
     if (!mg_live(st)) return;
-    for (long long In = (long long)blockIdx.x * blockDim.x + threadIdx.x; In < C.nnodes;
-         In += (long long)gridDim.x * blockDim.x) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < C.ndof;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = int(t / C.nnodes);
+        const long long In = t - c * C.nnodes;
         int Ij, Ii, Ik;
         mg_decode(C, In, Ij, Ii, Ik);
-        float acc[3] = {0.f, 0.f, 0.f};
-        for (int ok = -(rz - 1); ok <= rz - 1; ++ok)
-            for (int oi = -(rx - 1); oi <= rx - 1; ++oi)
-                for (int oj = -(ry - 1); oj <= ry - 1; ++oj) {
-                    const int fj = Ij * ry + oj, fi = Ii * rx + oi, fk = Ik * rz + ok;
-                    if (fj < 0 || fj >= F.NJ || fi < 0 || fi >= F.NI || fk < 0 || fk >= F.NK) continue;
+        const int fj = Ij * ry, fi = Ii * rx, fk = Ik * rz;
+        const int j0 = (ry == 2 && fj > 0) ? -1 : 0, j1 = (ry == 2 && fj + 1 < F.NJ) ? 1 : 0;
+        const int i0 = (rx == 2 && fi > 0) ? -1 : 0, i1 = (rx == 2 && fi + 1 < F.NI) ? 1 : 0;
+        const int k0 = (rz == 2 && fk > 0) ? -1 : 0, k1 = (rz == 2 && fk + 1 < F.NK) ? 1 : 0;
+        float acc = 0.0f;
+        for (int ok = k0; ok <= k1; ++ok)
+            for (int oi = i0; oi <= i1; ++oi)
+                for (int oj = j0; oj <= j1; ++oj) {
                     const float w = (oj ? 0.5f : 1.0f) * (oi ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f);
-                    const long long f = 3 * (((long long)fk * F.NI + fi) * F.NJ + fj);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        if (maskF[f + c] != 0.0f) acc[c] += w * (bF[f + c] - yF[f + c]);
+                    const long long f = 3 * (((long long)(fk + ok) * F.NI + (fi + oi)) * F.NJ + (fj + oj)) + c;
+                    if (maskF[f] != 0.0f) acc += w * (bF[f] - yF[f]);
                 }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) bC[3 * In + c] = acc[c];
+        bC[3 * In + c] = acc;
     }
 }
 

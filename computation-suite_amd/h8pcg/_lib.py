@@ -86,6 +86,8 @@ SIGNATURES = {
     "h8_mg_workspace_bytes": (C.c_size_t, [_GRID]),
     "h8_refine_solve_mg": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, C.c_int, _P, C.c_size_t, _P,
                                      C.c_size_t, _STATS, _P]),
+    "h8_mg_time_kernels": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_int, _P, _P,
+                                     _P]),
     "h8_mg_vcycle_f32": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _P, C.c_size_t, _P, C.c_size_t, _P]),
     "h8_elem_bilinear_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P]),
     "h8_simp_scale_f64": (C.c_int, [_P, _D, _D, _I64, _P, _P, _P]),

@@ -211,6 +211,18 @@ int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const double* K064, 
                      const float* K032, const double* dinv64, const float* r, float* z, void* ws,
                      size_t ws_bytes, void* mg_ws, size_t mg_ws_bytes, void* stream);
 
+/* Average device time (ms) of the multigrid kernels, each graph-launched
+ * `reps` times alone between HIP events on `stream`: avg_ms[0] = the level-0
+ * FP32 product y = K x (sector kernel), avg_ms[1] = one Chebyshev step on
+ * level 1 (stencil product fused with the update; 0 when level 1 is the
+ * coarsest), avg_ms[2] = one whole V-cycle.  info[0..2] (may be NULL) =
+ * levels, DOFs and nodes of level 1.  ws: h8_pcg_workspace_bytes(g, 32);
+ * mg_ws: h8_mg_workspace_bytes(g).  Measurement helper with no counterpart in
+ * the reference; scribbles on both workspaces. */
+int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
+                       const float* K032, const double* dinv64, void* ws, size_t ws_bytes, void* mg_ws,
+                       size_t mg_ws_bytes, int reps, double* avg_ms, int64_t* info, void* stream);
+
 /* ---- element bilinear forms (SIMP sensitivities) ----------------------- */
 /* out[e] = w[e] * u_e^T K0 v_e for every element (u_e = u at the element's 24
  * DOFs).  With w = -s'(rho), u = v this is compliance_sensitivity_closed_form
