@@ -1032,18 +1032,36 @@ extern "C" int h8_oc_update_f64(const double* rho, const double* dc, int64_t m, 
 // ---------------------------------------------------------------------------
 // FP64 Jacobi-PCG
 // ---------------------------------------------------------------------------
-extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
-                          const double* b, double* x, int use_x0, double tol, int64_t maxiter,
-                          void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
-    GridDev d;
-    int rc = make_grid(g, d);
-    if (rc) return rc;
-    if (!s_e || !K0 || !dinv || !b || !x || !ws) return fail(H8_ERR_ARG, "NULL pointer");
-    if (ws_bytes < ws_bytes_64(d))
-        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, ws_bytes_64(d));
-    if (!(tol >= 0.0)) return fail(H8_ERR_ARG, "tol must be >= 0");
-    if (maxiter < 0) return fail(H8_ERR_ARG, "maxiter must be >= 0");
-    hipStream_t s = (hipStream_t)stream;
+// The iterations of run_chunks launched directly on the stream, `chunk` per
+// host poll, with no captured graph (nothing instantiated or allocated).
+template <typename T, bool REFINED>
+static int run_direct(int chunk, const GridDev& d, const T* s_e, const T* K0, const T* dinv, T* x, T* r, T* p0,
+                      T* p1, T* q, const ZRing<T>& zr, PcgState* st, double* partials, hipStream_t s, PcgState& hst,
+                      long long& launches) {
+    if (use_resident<T>(d)) {
+        const int rr = run_resident<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, zr, st, partials, s, hst, launches);
+        if (rr != RES_DECLINED) return rr;
+    }
+    for (;;) {
+        for (int it = 0; it < chunk; ++it) {
+            const int r1 = launch_iteration<T, REFINED>(d, s_e, K0, dinv, x, r, p0, p1, q, st, partials, s);
+            if (r1) return r1;
+        }
+        ++launches;
+        HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hst.status != ST_RUNNING) break;
+    }
+    return H8_OK;
+}
+
+// Jacobi-PCG in FP64 on a workspace of ws_bytes_64(d) bytes at ws.  noalloc:
+// no events (elapsed_ms = -1) and no captured graphs, so the call creates,
+// instantiates and allocates nothing.
+static int pcg_f64_impl(const GridDev& d, const double* s_e, const double* K0, const double* dinv, const double* b,
+                        double* x, int use_x0, double tol, int64_t maxiter, void* ws, h8_pcg_stats* stats,
+                        hipStream_t s, bool noalloc) {
+    int rc;
     char* p = (char*)ws;
     Ws64 w;
     w.r = (double*)carve(p, sizeof(double) * d.ndof);
@@ -1056,9 +1074,11 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
     const ZRing<double> zr = carve_ring<double>(p, d, w.q);
 
     Ev e0, e1;
-    HIPCHK(e0.create());
-    HIPCHK(e1.create());
-    HIPCHK(hipEventRecord(e0, s));
+    if (!noalloc) {
+        HIPCHK(e0.create());
+        HIPCHK(e1.create());
+        HIPCHK(hipEventRecord(e0, s));
+    }
 
     PcgState hst{};
     hst.tol = tol;
@@ -1082,18 +1102,23 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
     HIPCHK(hipStreamSynchronize(s));
     long long launches = 0;
     if (hst.status == ST_RUNNING && maxiter > 0) {
-        GraphKey key{};
-        key.kind = 1;
-        key.variant = d.variant;
-        key.layers = d.layers;
-        key.arith = d.arith;
-        HIPCHK(hipGetDevice(&key.dev));
-        key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
-        const void* ptrs[] = {s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
-        std::memcpy(key.ptr, ptrs, sizeof ptrs);
-        key.chunk = chunk_for(maxiter);
-        rc = run_chunks<double, false>(key, d, s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, zr, w.st, w.partials, s,
-                                       hst, launches);
+        if (noalloc) {
+            rc = run_direct<double, false>(chunk_for(maxiter), d, s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, zr, w.st,
+                                           w.partials, s, hst, launches);
+        } else {
+            GraphKey key{};
+            key.kind = 1;
+            key.variant = d.variant;
+            key.layers = d.layers;
+            key.arith = d.arith;
+            HIPCHK(hipGetDevice(&key.dev));
+            key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
+            const void* ptrs[] = {s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, w.st, w.partials};
+            std::memcpy(key.ptr, ptrs, sizeof ptrs);
+            key.chunk = chunk_for(maxiter);
+            rc = run_chunks<double, false>(key, d, s_e, K0, dinv, x, w.r, w.p0, w.p1, w.q, zr, w.st, w.partials, s,
+                                           hst, launches);
+        }
         if (rc) return rc;
         // the last iteration's x += alpha p (the update kernel defers it to the next direction kernel)
         if (hst.pending) {
@@ -1105,10 +1130,14 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
         hst.status = ST_MAXITER;
         hst.iters = 0;
     }
-    HIPCHK(hipEventRecord(e1, s));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    float ms = -1.f;
+    if (!noalloc) {
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    } else {
+        HIPCHK(hipStreamSynchronize(s));
+    }
     fill_stats(stats, hst, ms, launches);
     if (hst.status == ST_NOT_SPD)
         return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
@@ -1119,6 +1148,72 @@ extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0,
                     "%.3e >= tol=%.1e).",
                     (long long)maxiter, hst.rel, tol);
     return H8_OK;
+}
+
+static int pcg_f64_args(const h8_grid* g, GridDev& d, const double* s_e, const double* K0, const double* dinv,
+                        const double* b, double* x, double tol, int64_t maxiter) {
+    int rc = make_grid(g, d);
+    if (rc) return rc;
+    if (!s_e || !K0 || !dinv || !b || !x) return fail(H8_ERR_ARG, "NULL pointer");
+    if (!(tol >= 0.0)) return fail(H8_ERR_ARG, "tol must be >= 0");
+    if (maxiter < 0) return fail(H8_ERR_ARG, "maxiter must be >= 0");
+    return H8_OK;
+}
+
+extern "C" int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
+                          const double* b, double* x, int use_x0, double tol, int64_t maxiter,
+                          void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+    GridDev d;
+    int rc = pcg_f64_args(g, d, s_e, K0, dinv, b, x, tol, maxiter);
+    if (rc) return rc;
+    if (!ws) return fail(H8_ERR_ARG, "NULL pointer");
+    if (ws_bytes < ws_bytes_64(d))
+        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, ws_bytes_64(d));
+    return pcg_f64_impl(d, s_e, K0, dinv, b, x, use_x0, tol, maxiter, ws, stats, (hipStream_t)stream, false);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 Jacobi-PCG that allocates nothing: the workspace of h8_pcg_f64 in a
+// file-scope device array (each GPU holds its own copy until the process ends)
+// ---------------------------------------------------------------------------
+static_assert(H8_SCRATCH_BYTES % 256 == 0, "scratch size");
+static __device__ __align__(256) unsigned char scratch[H8_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current (host code never passes the symbol itself).
+static int scratch_address(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(H8_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(H8_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return H8_OK;
+}
+
+extern "C" int h8_pcg_f64_noalloc(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
+                                  const double* b, double* x, int use_x0, double tol, int64_t maxiter,
+                                  h8_pcg_stats* stats, void* stream) {
+    GridDev d;
+    int rc = pcg_f64_args(g, d, s_e, K0, dinv, b, x, tol, maxiter);
+    if (rc) return rc;
+    const size_t need = ws_bytes_64(d);
+    if (need > H8_SCRATCH_BYTES)
+        return fail(H8_ERR_WORKSPACE,
+                    "h8_pcg_f64_noalloc: a %dx%dx%d grid needs %zu bytes of workspace, more than the library's "
+                    "static scratch of %zu bytes; use h8_pcg_f64 with a workspace",
+                    d.nx, d.ny, d.nz, need, (size_t)H8_SCRATCH_BYTES);
+    void* ws = nullptr;
+    if ((rc = scratch_address(&ws))) return rc;
+    return pcg_f64_impl(d, s_e, K0, dinv, b, x, use_x0, tol, maxiter, ws, stats, (hipStream_t)stream, true);
 }
 
 // ---------------------------------------------------------------------------

@@ -23,6 +23,7 @@ H8_MAXITER = 4
 H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
 H8_MG_DECLINED = 7
+SCRATCH_BYTES = 32 << 20   # H8_SCRATCH_BYTES: the static workspace of h8_pcg_f64_noalloc
 
 (VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR,
  VARIANT_SECTOR_NARROW, VARIANT_RESIDENT) = 0, 1, 2, 3, 4, 5, 6, 7
@@ -79,6 +80,7 @@ SIGNATURES = {
     "h8_spmv_f32": (C.c_int, [_GRID, _P, _P, _P, _P, _P]),
     "h8_pcg_workspace_bytes": (C.c_size_t, [_GRID, C.c_int]),
     "h8_pcg_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, C.c_int, _D, _I64, _P, C.c_size_t, _STATS, _P]),
+    "h8_pcg_f64_noalloc": (C.c_int, [_GRID, _P, _P, _P, _P, _P, C.c_int, _D, _I64, _STATS, _P]),
     "h8_refine_solve": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, _P, C.c_size_t,
                                   _STATS, _P]),
     "h8_refine_solve_x0": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _D, _P, C.c_size_t,

@@ -159,6 +159,22 @@ int h8_pcg_f64(const h8_grid* g, const double* s_e, const double* K0, const doub
                const double* b, double* x, int use_x0, double tol, int64_t maxiter,
                void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
 
+/* h8_pcg_f64 without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops): the
+ * workspace is a file-scope device array of H8_SCRATCH_BYTES bytes that each
+ * GPU holds until the process ends, and the call creates no events, captures
+ * no graphs and allocates nothing (elapsed_ms = -1).  Grids whose
+ * h8_pcg_workspace_bytes(g, 64) exceeds H8_SCRATCH_BYTES are refused with
+ * H8_ERR_WORKSPACE (about 590k DOFs).  Same recurrence, kernels and results
+ * as h8_pcg_f64.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (two host threads, or two streams) race.  No
+ * counterpart in the reference (its solvers allocate per call); it exists for
+ * callers that cannot allocate. */
+#define H8_SCRATCH_BYTES (32u << 20)
+int h8_pcg_f64_noalloc(const h8_grid* g, const double* s_e, const double* K0, const double* dinv,
+                       const double* b, double* x, int use_x0, double tol, int64_t maxiter,
+                       h8_pcg_stats* stats, void* stream);
+
 /* Mixed precision: FP32 Jacobi-PCG iterations (s_e32, K032, FP32 vectors,
  * FP64 dot products) whose residual is replaced by the FP64 true residual
  * r = f - K u (s_e64, K064) and whose iterate is folded into the FP64 u
