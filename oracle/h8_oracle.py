@@ -865,3 +865,18 @@ def mg_vcycle(levels, r, level=0):
     x = x + lev["P"] @ mg_vcycle(levels, bc, level + 1)
     x, _ = _mg_smooth(lev, x, r, d, False)
     return x
+
+
+def element_matvec(elements, ke, scale, u, chunk=65536):
+    """K u = sum_e scale_e ke u_e without forming K (the product the assembly
+    of fe/vectorized_assembly.py:217-288 and fe/topopt.py:214-220 defines),
+    in chunks of elements; ke one (24, 24) matrix.  Pinned to assemble_csr @ u."""
+    u = np.asarray(u, dtype=float)
+    out = np.zeros_like(u)
+    elements = np.asarray(elements)
+    for c0 in range(0, len(elements), chunk):
+        el = elements[c0:c0 + chunk]
+        dofs = (3 * el[:, :, None] + np.arange(3)).reshape(len(el), 24)
+        ye = (u[dofs] @ ke.T) * np.asarray(scale[c0:c0 + chunk], dtype=float)[:, None]
+        out += np.bincount(dofs.ravel(), weights=ye.ravel(), minlength=len(u))
+    return out
