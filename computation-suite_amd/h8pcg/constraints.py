@@ -26,47 +26,52 @@ from ._lib import check, ptr
 
 
 def _reduce_group(rows, rhs, tol):
-    cols = sorted({d for r in rows for d in r})
-    m = len(rows)
-    if not cols:
+    """Eliminate one connected group of constraint rows ({dof: coefficient},
+    right-hand side): Gauss-Jordan on the augmented table with, at every step,
+    the entry of largest magnitude among the rows and columns not yet used as
+    the pivot -- the elimination order of fe/boundary.py:98-166, so the same
+    DOFs become slaves.  Returns (slaves, {slave: {master: coefficient}},
+    {slave: value})."""
+    dofs = sorted({dof for row in rows for dof in row})
+    if not dofs:
         for v in rhs:
             if abs(v) > tol * max(1.0, abs(v)):
-                raise ValueError("inconsistent constraint: all its DOFs are prescribed and the equation is not "
-                                 "satisfied.")
+                raise ValueError("inconsistent constraint: every DOF it involves is prescribed and the "
+                                 "prescribed values do not satisfy it.")
         return [], {}, {}
-    cidx = {d: j for j, d in enumerate(cols)}
-    work = np.zeros((m, len(cols) + 1))
-    for i, r in enumerate(rows):
-        for d, c in r.items():
-            work[i, cidx[d]] = c
-        work[i, -1] = rhs[i]
-    scale = max(1.0, np.abs(work).max())
-    pivots, used = [], []
-    for _ in range(min(m, len(cols))):
-        sub = np.abs(work[:, :-1])
-        sub[used, :] = 0.0
-        for _r, c in pivots:
-            sub[:, c] = 0.0
-        i, j = np.unravel_index(np.argmax(sub), sub.shape)
-        if sub[i, j] <= tol * scale:
+    col_of = {dof: j for j, dof in enumerate(dofs)}
+    ncol = len(dofs)
+    table = np.zeros((len(rows), ncol + 1))
+    for i, row in enumerate(rows):
+        for dof, coef in row.items():
+            table[i, col_of[dof]] = coef
+        table[i, ncol] = rhs[i]
+    magnitude = max(1.0, np.abs(table).max())
+    pivot_row, pivot_col = [], []
+    for _ in range(min(len(rows), ncol)):
+        cand = np.abs(table[:, :ncol])
+        cand[pivot_row, :] = 0.0
+        cand[:, pivot_col] = 0.0
+        i, j = np.unravel_index(np.argmax(cand), cand.shape)
+        if cand[i, j] <= tol * magnitude:
             break
-        work[i] /= work[i, j]
-        for r in range(m):
-            if r != i and work[r, j] != 0.0:
-                work[r] -= work[r, j] * work[i]
-        pivots.append((i, j))
-        used.append(i)
-    for r in range(m):
-        if r not in used and abs(work[r, -1]) > 1e-8 * scale:
-            raise ValueError("inconsistent linear constraints (conflicting equations).")
-    pcols = {j for _, j in pivots}
-    slave, mat, g = [], {}, {}
-    for i, j in pivots:
-        s = cols[j]
-        slave.append(s)
-        mat[s] = {cols[k]: work[i, k] for k in range(len(cols)) if k not in pcols and abs(work[i, k]) > 1e-14}
-        g[s] = float(work[i, -1])
-    return slave, mat, g
+        table[i] /= table[i, j]
+        others = np.flatnonzero((np.arange(len(rows)) != i) & (table[:, j] != 0.0))
+        for r in others:
+            table[r] -= table[r, j] * table[i]
+        pivot_row.append(i)
+        pivot_col.append(j)
+    leftover = [r for r in range(len(rows)) if r not in pivot_row]
+    if any(abs(table[r, ncol]) > 1e-8 * magnitude for r in leftover):
+        raise ValueError("inconsistent linear constraints: the equations contradict one another.")
+    masters = [j for j in range(ncol) if j not in pivot_col]
+    slaves, coeffs, values = [], {}, {}
+    for i, j in zip(pivot_row, pivot_col):
+        s = dofs[j]
+        slaves.append(s)
+        coeffs[s] = {dofs[k]: table[i, k] for k in masters if abs(table[i, k]) > 1e-14}
+        values[s] = float(table[i, ncol])
+    return slaves, coeffs, values
 
 
 def reduce_constraints(rows, values, fixed_values, tol=1e-10):

@@ -22,50 +22,52 @@ class Mesh:
     node_sets: dict = field(default_factory=dict)
     structure: tuple | None = None   # (nx, ny, nz, Lx, Ly, Lz) for box meshes
 
-    # ---- node sets / selection (fe/mesh.py:206-264) ------------------------
+    # ---- node sets / selection (the contract of fe/mesh.py:206-264) -------------
+    # A node set is a sorted array of unique node ids stored under a name; the
+    # solver's node arguments accept those names.  Errors are ValueError.
     def add_node_set(self, name, node_ids):
-        if not isinstance(name, str) or not name:
-            raise ValueError("add_node_set: the set name must be a non-empty string.")
-        ids = np.asarray(node_ids).reshape(-1)
+        if not (isinstance(name, str) and name):
+            raise ValueError("add_node_set: a node set needs a non-empty string name.")
+        ids = np.asarray(node_ids).ravel()
         if ids.size == 0:
-            raise ValueError(f"add_node_set({name!r}): no nodes given -- the selection matched nothing "
-                             f"(check the coordinate value / tolerance).")
-        if ids.dtype.kind not in "iu":
-            if ids.dtype.kind == "f" and np.all(ids == np.round(ids)):
-                ids = ids.astype(int)
-            else:
-                raise ValueError(f"add_node_set({name!r}): node ids must be integers.")
-        ids = np.unique(ids.astype(int))
+            raise ValueError(f"add_node_set({name!r}): the set would be empty (a selection that matched no "
+                             f"node; check the coordinate and tol).")
+        if ids.dtype.kind == "f" and np.array_equal(ids, np.floor(ids)):
+            ids = ids.astype(np.int64)
+        elif ids.dtype.kind not in "iu":
+            raise ValueError(f"add_node_set({name!r}): node ids have to be whole numbers.")
+        ids = np.unique(ids.astype(np.int64))
         if ids[0] < 0 or ids[-1] >= len(self.nodes):
-            raise ValueError(f"add_node_set({name!r}): node id outside the mesh "
-                             f"(valid range 0..{len(self.nodes) - 1}).")
+            raise ValueError(f"add_node_set({name!r}): ids run outside 0..{len(self.nodes) - 1}.")
         self.node_sets[name] = ids
         return self
 
     def node_set(self, name):
-        try:
-            return self.node_sets[name]
-        except KeyError:
-            raise ValueError(f"no node set named {name!r}; available: {sorted(self.node_sets)}") from None
+        if name not in self.node_sets:
+            raise ValueError(f"unknown node set {name!r} (defined: {sorted(self.node_sets)})")
+        return self.node_sets[name]
 
     def select_nodes(self, x=None, y=None, z=None, tol=1e-9, name=None):
-        dim = self.nodes.shape[1]
-        mask = np.ones(len(self.nodes), dtype=bool)
-        for axis, spec in enumerate((x, y, z)):
-            if spec is None:
+        """Ids of the nodes matching every given coordinate: a value (within tol)
+        or a closed interval (lo, hi) widened by tol; registered as a node set
+        when `name` is given."""
+        coords = self.nodes
+        keep = np.ones(len(coords), dtype=bool)
+        for axis, label, want in ((0, "x", x), (1, "y", y), (2, "z", z)):
+            if want is None:
                 continue
-            if axis >= dim:
-                raise ValueError(f"select_nodes: '{'xyz'[axis]}' given but the mesh has only {dim} "
+            if axis >= coords.shape[1]:
+                raise ValueError(f"select_nodes: {label}= given for a mesh with {coords.shape[1]} "
                                  f"coordinate(s).")
-            col = self.nodes[:, axis]
-            if np.ndim(spec) == 0:
-                mask &= np.abs(col - spec) < tol
-            else:
-                lo, hi = spec
-                if lo > hi:
-                    raise ValueError(f"select_nodes: {'xyz'[axis]}=({lo}, {hi}) has lo > hi.")
-                mask &= (col >= lo - tol) & (col <= hi + tol)
-        ids = np.where(mask)[0]
+            c = coords[:, axis]
+            if np.ndim(want) == 0:
+                keep &= np.abs(c - want) < tol
+                continue
+            lo, hi = want
+            if hi < lo:
+                raise ValueError(f"select_nodes: the interval {label}=({lo}, {hi}) is reversed.")
+            keep &= (c >= lo - tol) & (c <= hi + tol)
+        ids = np.flatnonzero(keep)
         if name is not None:
             self.add_node_set(name, ids)
         return ids

@@ -24,6 +24,61 @@ from .fields import FEField
 from .mesh import structured_grid
 
 
+def _integer_ids(values, what, who):
+    """A flat int array from ints, integral floats, sets or arrays; ValueError otherwise."""
+    if isinstance(values, (set, frozenset)):
+        values = sorted(values)
+    arr = np.asarray(values).ravel()
+    if arr.size == 0 or arr.dtype.kind in "iu":
+        return arr.astype(np.int64)
+    if arr.dtype.kind == "f" and np.array_equal(arr, np.floor(arr)):
+        return arr.astype(np.int64)
+    raise ValueError(f"{who}: {what} have to be whole numbers (got dtype {arr.dtype}).")
+
+
+def _node_selection(mesh, spec, who):
+    """Node ids of `spec`: ids, or one or more node-set names of `mesh`."""
+    names = [spec] if isinstance(spec, str) else None
+    if names is None and isinstance(spec, (list, tuple)) and spec and all(isinstance(v, str) for v in spec):
+        names = list(spec)
+    if names is not None:
+        sets = getattr(mesh, "node_sets", {})
+        absent = [nm for nm in names if nm not in sets]
+        if absent:
+            raise ValueError(f"{who}: unknown node set {absent} (the mesh defines {sorted(sets)}); create it "
+                             f"first with mesh.add_node_set or mesh.select_nodes(..., name=...).")
+        spec = np.unique(np.concatenate([np.asarray(sets[nm], dtype=np.int64) for nm in names]))
+    ids = _integer_ids(spec, "node ids", who)
+    if ids.size == 0:
+        raise ValueError(f"{who}: no nodes selected -- the id list is empty, which is what a coordinate "
+                         f"selector returns when it matched no node (check its value and tol).")
+    n = len(mesh.nodes)
+    outside = np.unique(ids[(ids < 0) | (ids >= n)])
+    if outside.size:
+        raise ValueError(f"{who}: node ids {outside[:5].tolist()} are outside the mesh, whose ids run "
+                         f"0..{n - 1}.")
+    return ids
+
+
+def _component_selection(spec, npn, by_name, who):
+    """Local component indices of `spec`: an index, a name, or a collection of
+    either; each must lie in 0..npn-1."""
+    if isinstance(spec, (set, frozenset)):
+        items = sorted(spec, key=str)
+    elif isinstance(spec, str) or np.ndim(spec) == 0:
+        items = [spec]
+    else:
+        items = list(spec)
+    comps = _integer_ids([by_name(v) if isinstance(v, str) else v for v in items], "component indices", who)
+    if comps.size == 0:
+        raise ValueError(f"{who}: no DOF indices given.")
+    wrong = sorted(set(comps[(comps < 0) | (comps >= npn)].tolist()))
+    if wrong:
+        raise ValueError(f"{who}: component index {wrong} does not exist -- Hex8Solid3D has {npn} DOF(s) per "
+                         f"node, numbered 0..{npn - 1}.")
+    return comps
+
+
 class FESystem:
     def __init__(self, mesh, elem_formulation, thickness=1.0, sparse=False, backend="hip", device="cuda"):
         if backend != "hip":
@@ -55,18 +110,10 @@ class FESystem:
         self._constraints = ()
         self.last_solve_stats = None
 
-    # ---- validation helpers (fe/solver.py:447-496) ------------------------------------------
-    @staticmethod
-    def _as_int_array(values, what, who):
-        arr = np.asarray(list(values) if isinstance(values, (set, frozenset)) else values)
-        arr = arr.reshape(-1)
-        if arr.size and arr.dtype.kind not in "iu":
-            if arr.dtype.kind == "f" and np.all(arr == np.round(arr)):
-                arr = arr.astype(int)
-            else:
-                raise ValueError(f"{who}: {what} must be integers, got {arr.dtype}.")
-        return arr.astype(int)
-
+    # ---- argument resolution (the contract of fe/solver.py:447-496) -------------------------
+    # Node arguments: integer ids, or the name(s) of node sets registered on the
+    # mesh.  Component arguments: local indices or names ("uy").  Both raise
+    # ValueError; the message fragments the reference's tests match on are kept.
     @property
     def dof_names(self):
         return self.elem.local_dof_names()
@@ -81,58 +128,18 @@ class FESystem:
             raise ValueError(f"{who}: {err}") from None
 
     def _check_nodes(self, node_ids, who):
-        if isinstance(node_ids, str) or (isinstance(node_ids, (list, tuple)) and len(node_ids) > 0
-                                         and all(isinstance(n, str) for n in node_ids)):
-            names = [node_ids] if isinstance(node_ids, str) else list(node_ids)
-            sets = getattr(self.mesh, "node_sets", {})
-            missing = [n for n in names if n not in sets]
-            if missing:
-                raise ValueError(f"{who}: unknown node set(s) {missing}; available: {sorted(sets)}. "
-                                 f"Register one with mesh.add_node_set(name, ids) or "
-                                 f"mesh.select_nodes(..., name=name).")
-            node_ids = np.unique(np.concatenate([np.asarray(sets[n], dtype=int) for n in names]))
-        nodes = self._as_int_array(node_ids, "node ids", who)
-        if nodes.size == 0:
-            raise ValueError(
-                f"{who}: no nodes selected (node_ids is empty). A selector such as "
-                f"mesh.nodes_on_line(axis, value) matched nothing -- check the coordinate "
-                f"value and its tolerance (tol=).")
-        n_nodes = len(self.mesh.nodes)
-        bad = nodes[(nodes < 0) | (nodes >= n_nodes)]
-        if bad.size:
-            raise ValueError(
-                f"{who}: node id(s) {sorted(set(bad.tolist()))[:5]} outside the mesh "
-                f"(valid range 0..{n_nodes - 1}).")
-        return nodes
+        return _node_selection(self.mesh, node_ids, who)
 
     def _check_local_dofs(self, dof_indices, who):
-        if isinstance(dof_indices, (set, frozenset)):
-            items = list(dof_indices)
-        elif isinstance(dof_indices, str) or np.ndim(dof_indices) == 0:
-            items = [dof_indices]
-        else:
-            items = list(dof_indices)
-        resolved = [self._dof_index_by_name(it, who) if isinstance(it, str) else it for it in items]
-        dofs = self._as_int_array(np.array(resolved), "local DOF indices", who)
-        if dofs.size == 0:
-            raise ValueError(f"{who}: no DOF indices given.")
-        bad = dofs[(dofs < 0) | (dofs >= self.npn)]
-        if bad.size:
-            raise ValueError(
-                f"{who}: local DOF index {sorted(set(bad.tolist()))} is out of range -- "
-                f"Hex8Solid3D has {self.npn} DOF(s) per node, so valid indices "
-                f"are 0..{self.npn - 1}.")
-        return dofs
+        return _component_selection(dof_indices, self.npn, lambda nm: self._dof_index_by_name(nm, who), who)
 
     # ---- boundary conditions and loads (fe/solver.py:1109-1270) ----------------------------
     def fix_dofs(self, node_ids, dof_indices, value=0.0):
         nodes = self._check_nodes(node_ids, "fix_dofs")
-        dofs = self._check_local_dofs(dof_indices, "fix_dofs")
-        for n in nodes:
-            for k in dofs:
-                dof = self.npn * int(n) + int(k)
-                self.fixed_dofs.add(dof)
-                self.fixed_dof_values[dof] = float(value)
+        comps = self._check_local_dofs(dof_indices, "fix_dofs")
+        gdofs = (self.npn * nodes[:, None] + comps[None, :]).ravel().tolist()
+        self.fixed_dofs.update(gdofs)
+        self.fixed_dof_values.update(dict.fromkeys(gdofs, float(value)))
 
     def add_nodal_force(self, node_ids, dof_index, total_force):
         nodes = self._check_nodes(node_ids, "add_nodal_force")
@@ -176,13 +183,13 @@ class FESystem:
 
     # ---- assembly (fe/solver.py:498-518,569-697) -------------------------------------------
     def _begin_assembly(self, accumulate, who):
-        if "K" in self._assembled and not accumulate and "K" not in self._warned_reassembly:
-            warnings.warn(
-                f"{who}: K was already assembled, so it is being REPLACED (earlier "
-                f"versions silently added a second copy, doubling the matrix). Pass "
-                f"accumulate=True to add to the existing matrix instead.",
-                UserWarning, stacklevel=3)
-            self._warned_reassembly.add("K")
+        # a repeated assemble_stiffness replaces K (accumulate=True adds to it);
+        # the first replacement warns once
+        if accumulate or "K" not in self._assembled or "K" in self._warned_reassembly:
+            return
+        self._warned_reassembly.add("K")
+        warnings.warn(f"{who}: K was already assembled; this call replaces it (pass accumulate=True to add "
+                      f"the new contribution instead).", UserWarning, stacklevel=3)
 
     def assemble_stiffness(self, D, method="full", vectorized=False, accumulate=False, **kwargs):
         """Stores K0 (24x24, device) and s_e (per element, device).
@@ -191,13 +198,13 @@ class FESystem:
         material (the matrices must be scalar multiples of one another).
         kwargs: thickness= (scales every element), gauss_order= (2 only),
         element_scale= (an (m,) array multiplying each element, e.g. a SIMP scale)."""
-        if method not in ("full", "reduced", "hourglass_stabilized"):
-            raise ValueError(
-                f"assemble_stiffness: unknown method={method!r} -- expected "
-                f"'full' (default), 'reduced', or 'hourglass_stabilized'.")
-        if method != "full" and vectorized:
-            raise ValueError(f"assemble_stiffness: vectorized=True only supports method='full' "
-                             f"(got method={method!r}).")
+        known = ("full", "reduced", "hourglass_stabilized")
+        if method not in known:
+            raise ValueError(f"assemble_stiffness: unknown method {method!r}; the integration methods are "
+                             f"{', '.join(map(repr, known))}.")
+        if vectorized and method != "full":
+            raise ValueError(f"assemble_stiffness: vectorized=True only supports method='full', and "
+                             f"method={method!r} was asked for.")
         if method != "full":
             raise NotImplementedError(f"assemble_stiffness(method={method!r}) is not available on backend='hip'")
         kw = dict(kwargs)
@@ -249,10 +256,10 @@ class FESystem:
         self._stiffness_args = (D, dict(kwargs))
 
     def _require_stiffness(self, who):
-        if self._op is None:
-            raise RuntimeError(
-                f"{who}: the global stiffness matrix K is all zeros -- call "
-                f"assemble_stiffness(...) before solving.")
+        if self._op is not None:
+            return
+        raise RuntimeError(f"{who}: nothing is assembled yet (K is all zeros); run assemble_stiffness(...) "
+                           f"first.")
 
     @property
     def operator(self):
