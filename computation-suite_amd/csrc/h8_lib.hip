@@ -238,7 +238,7 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     // the resident solver's products outside its loop use the streaming default
     const int v = (d.variant == H8_VARIANT_AUTO || d.variant == H8_VARIANT_RESIDENT) ? default_variant<T>(d)
                                                                                       : d.variant;
-    if constexpr (MODE == MODE_DIRX || MODE == MODE_DIRZ) {   // sector kernels only
+    if constexpr (MODE == MODE_DIRX || MODE == MODE_DIRZ || MODE == MODE_CHEB) {   // sector kernels only
         if (v == H8_VARIANT_SECTOR) return launch_sector<T, MODE, 16, 16>(d0, A, s);
         if (v == H8_VARIANT_SECTOR_NARROW) return launch_sector<T, MODE, 8, 16>(d0, A, s);
         return fail(H8_ERR_ARG, "direction mode %d needs a sector kernel (variant %d)", MODE, v);
@@ -1614,8 +1614,9 @@ static bool mg_plan(const GridDev& d, MgPlan& P) {
 }
 
 // Carves the hierarchy's buffers from `base` (nullptr: sizes only) and
-// returns the bytes needed.  Level 0 borrows the solve's z, r32 and dinv32
-// (set by the caller); its product scratch y and direction d are carved here.
+// returns the bytes needed.  Level 0 borrows the solve's z (its x1), r32 and
+// dinv32 (set by the caller); its product scratch y, direction d and the
+// other iterate x0 are carved here.
 static size_t mg_layout(MgPlan& P, char* base) {
     size_t off = 0;
     auto take = [&](size_t bytes) -> char* {
@@ -1629,6 +1630,7 @@ static size_t mg_layout(MgPlan& P, char* base) {
     P.flag = (int*)take(sizeof(int));
     P.L[0].y = (float*)take(sizeof(float) * P.L[0].ndof);
     P.L[0].d = (float*)take(sizeof(float) * P.L[0].ndof);
+    P.L[0].x0 = (float*)take(sizeof(float) * P.L[0].ndof);
     for (int l = 1; l < P.nlev; ++l) {
         MgLevel& L = P.L[l];
         const size_t nv = sizeof(float) * L.ndof;
@@ -1697,53 +1699,85 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
     return H8_OK;
 }
 
+// First level of the cycle that runs inside k_mg_bottom: the trailing levels
+// of at most MG_BOTTOM_NODES nodes (never level 0, the matrix-free grid).
+static int mg_bottom_level(const MgPlan& P) {
+    int lb = P.nlev - 1;
+    while (lb > 1 && P.L[lb - 1].nnodes <= MG_BOTTOM_NODES) --lb;
+    return lb;
+}
+
 // z = V(r): one V-cycle (degree-2 Chebyshev smoothing before and after the
-// coarse correction).  P.L[0] holds b = r32, x0 = x1 = z, dinv = dinv32.
-// prod0: the level-0 product y = K z (sector kernel, FP32, with the state).
+// coarse correction).  P.L[0] holds b = r32, x1 = z, dinv = dinv32; prod0 the
+// level-0 operator arguments (FP32 sector kernel; st and partials of the
+// solve).  Level 0 smooths with the fused sector Chebyshev step; a stencil
+// level gets its first step from the restriction into it; the small levels
+// run in one k_mg_bottom launch.  fuse_rz: the last step also reduces r.z and
+// sets beta (the multigrid iteration's k_mg_rz).
 static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>& prod0, const PcgState* st,
-                     hipStream_t s) {
-    const int nl = P.nlev;
+                     hipStream_t s, bool fuse_rz) {
+    const int nl = P.nlev, lb = mg_bottom_level(P);
     const MgLevel* L = P.L;
     const MgConst* cst = P.cst;
-    auto spmv0 = [&]() { return launch_op<float, MODE_PLAIN>(d32, prod0, s); };
+    const MgLevel& L0 = L[0];
+    auto plain0 = [&](const float* u) {
+        NodeArgs<float> A = prod0;
+        A.u = u;
+        A.y = L0.y;
+        return launch_op<float, MODE_PLAIN>(d32, A, s);
+    };
+    auto cheb0 = [&](const float* u, float* out, int step, int rz) {
+        NodeArgs<float> A = prod0;
+        A.u = u;
+        A.dinv = L0.dinv;
+        A.cheb = &cst[0].c0;
+        A.cb_b = L0.b;
+        A.cb_d = L0.d;
+        A.cb_out = out;
+        A.cheb_step = step;
+        A.cheb_rz = rz;
+        return launch_op<float, MODE_CHEB>(d32, A, s);
+    };
     int rc;
-    for (int l = 0; l + 1 < nl; ++l) {
+    for (int l = 0; l < lb; ++l) {
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
-        const unsigned nb = mg_blocks(F.ndof), vb = mg_blocks(F.ndof);
-        // pre-smoothing from x = 0: x = d = c0 dinv b, then the second step
-        hipLaunchKernelGGL(k_mg_cheb_vec, dim3(vb), dim3(256), 0, s, F.ndof, F.x0, (const float*)F.b,
-                           (const float*)nullptr, F.d, (const float*)F.dinv, cst + l, 0, 1, st);
         if (l == 0) {
-            if ((rc = spmv0())) return rc;
-            hipLaunchKernelGGL(k_mg_cheb_vec, dim3(vb), dim3(256), 0, s, F.ndof, F.x0, (const float*)F.b,
-                               (const float*)F.y, F.d, (const float*)F.dinv, cst, 1, 0, st);
-            if ((rc = spmv0())) return rc;             // y = K z for the residual
+            // x0 = d = c0 dinv b; x1 = x0 + d'; y = K x1
+            hipLaunchKernelGGL(k_mg_cheb_vec, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F.ndof, F.x0,
+                               (const float*)F.b, (const float*)nullptr, F.d, (const float*)F.dinv, cst, 0, 1, st);
+            if ((rc = cheb0(F.x0, F.x1, 1, 0))) return rc;
+            if ((rc = plain0(F.x1))) return rc;
         } else {
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x0, F.x1, cst + l, 1, st);
-            hipLaunchKernelGGL(k_mg_spmv_st, dim3(nb), dim3(256), 0, s, F, (const float*)F.x1, F.y, st);
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x0, F.x1,
+                               cst + l, 1, st);
+            hipLaunchKernelGGL(k_mg_spmv_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x1, F.y,
+                               st);
         }
         hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.ndof)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
-                           (const float*)F.b, (const float*)F.y, (const float*)F.dinv, C.b, st);
+                           (const float*)F.b, (const float*)F.y, (const float*)F.dinv, cst + l + 1,
+                           (l + 1 < lb) ? 1 : 0, st);
     }
-    const MgLevel& Lc = L[nl - 1];
-    hipLaunchKernelGGL(k_mg_dense_solve, dim3(unsigned((Lc.ndof + 3) / 4)), dim3(256), 0, s, (const float*)P.Ainv,
-                       int(Lc.ndof), (const float*)Lc.b, Lc.x1, st);
-    for (int l = nl - 2; l >= 0; --l) {
+    MgBottom B{};
+    for (int l = 0; l < nl; ++l) B.L[l] = L[l];
+    B.lb = lb;
+    B.nl = nl;
+    B.cst = cst;
+    B.Ainv = P.Ainv;
+    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(MG_BOTTOM_THREADS), 0, s, B, st);
+    for (int l = lb - 1; l >= 0; --l) {
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
-        const unsigned nb = mg_blocks(F.nnodes), db = mg_blocks(F.ndof), vb = mg_blocks(F.ndof);
-        hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz, F.x1,
+        hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(F.nnodes)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz, F.x1,
                            (const float*)C.x1, (const float*)F.dinv, st);
         if (l == 0) {
-            for (int step = 0; step < 2; ++step) {
-                if ((rc = spmv0())) return rc;
-                hipLaunchKernelGGL(k_mg_cheb_vec, dim3(vb), dim3(256), 0, s, F.ndof, F.x0, (const float*)F.b,
-                                   (const float*)F.y, F.d, (const float*)F.dinv, cst, step, 0, st);
-            }
+            if ((rc = cheb0(F.x1, F.x0, 0, 0))) return rc;
+            if ((rc = cheb0(F.x0, F.x1, 1, fuse_rz ? 1 : 0))) return rc;
         } else {
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(db), dim3(256), 0, s, F, (const float*)F.x1, F.x0, cst + l, 0, st);
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(db), dim3(256), 0, s, F, (const float*)F.x0, F.x1, cst + l, 1, st);
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x1, F.x0,
+                               cst + l, 0, st);
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x0, F.x1,
+                               cst + l, 1, st);
         }
     }
     LAUNCHCHK();
@@ -1792,7 +1826,7 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     float* z = (float*)carve(p, sizeof(float) * d.ndof);      // the first z-exchange vector of the resident solver
     MgLevel& L0 = P.L[0];
     L0.b = w.r;
-    L0.x0 = L0.x1 = z;
+    L0.x1 = z;
     L0.dinv = w.dinv32;
     // FP32 products on the sector kernel (the direction mode needs it)
     GridDev d32 = d;
@@ -1847,6 +1881,7 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     R.dinv32 = w.dinv32; R.st = w.st; R.partials = w.partials;
     NodeArgs<float> prod0{};
     prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y; prod0.st = w.st;
+    prod0.partials = w.partials;
     NodeArgs<float> D{};
     D.g = d32; D.s_e = s_e32; D.K0 = K032; D.r = z; D.dinv = w.dinv32; D.p0 = w.p0; D.p1 = w.p1; D.q = w.q;
     D.x = w.x; D.st = w.st; D.partials = w.partials;
@@ -1856,11 +1891,8 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
         // first residual, z = V(r), r.z
         rc = launch_op<double, MODE_RESID>(d, R, s);
         if (rc) return rc;
-        rc = mg_vcycle(d32, P, prod0, w.st, s);
+        rc = mg_vcycle(d32, P, prod0, w.st, s, true);     // with r.z and beta
         if (rc) return rc;
-        hipLaunchKernelGGL(k_mg_rz, dim3(stream_blocks(d.ndof)), dim3(256), 0, s, d.ndof, (const float*)w.r,
-                           (const float*)z, w.st, w.partials);
-        LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -1889,9 +1921,7 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
                 hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, u, w.x, (const float*)w.p0,
                                    (const float*)w.p1, (const PcgState*)w.st);
                 if ((r1 = launch_op<double, MODE_RESID>(d, Rq, cs))) return r1;
-                if ((r1 = mg_vcycle(d32, P, prod0, w.st, cs))) return r1;
-                hipLaunchKernelGGL(k_mg_rz, dim3(stream_blocks(d.ndof)), dim3(256), 0, cs, d.ndof, (const float*)w.r,
-                                   (const float*)z, w.st, w.partials);
+                if ((r1 = mg_vcycle(d32, P, prod0, w.st, cs, true))) return r1;
             }
             LAUNCHCHK();
             return H8_OK;
@@ -1957,7 +1987,7 @@ extern "C" int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const dou
     LAUNCHCHK();
     MgLevel& L0 = P.L[0];
     L0.b = const_cast<float*>(r);
-    L0.x0 = L0.x1 = z;
+    L0.x1 = z;
     L0.dinv = dinv32;
     GridDev d32 = d;
     d32.variant = H8_VARIANT_SECTOR;
@@ -1969,7 +1999,7 @@ extern "C" int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const dou
     if (flag) return fail(H8_MG_DECLINED, "multigrid: the coarsest operator is singular");
     NodeArgs<float> prod0{};
     prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y;
-    return mg_vcycle(d32, P, prod0, nullptr, s);
+    return mg_vcycle(d32, P, prod0, nullptr, s, false);
 }
 
 // Average device time (ms) of the multigrid kernels of one FP32 iteration,
@@ -2006,7 +2036,7 @@ extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const d
     LAUNCHCHK();
     MgLevel& L0 = P.L[0];
     L0.b = r;
-    L0.x0 = L0.x1 = z;
+    L0.x1 = z;
     L0.dinv = dinv32;
     GridDev d32 = d;
     if (d32.variant != H8_VARIANT_SECTOR && d32.variant != H8_VARIANT_SECTOR_NARROW) {
@@ -2033,7 +2063,7 @@ extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const d
                     hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(L1.ndof)), dim3(256), 0, cs, L1,
                                        (const float*)L1.x0, L1.x1, (const MgConst*)(P.cst + 1), 1,
                                        (const PcgState*)nullptr);
-                else r1 = mg_vcycle(d32, P, prod0, nullptr, cs);
+                else r1 = mg_vcycle(d32, P, prod0, nullptr, cs, false);
                 if (r1) return r1;
             }
             LAUNCHCHK();

@@ -36,6 +36,7 @@
 // so a captured chunk of iterations costs nothing after convergence.
 #pragma once
 #include "h8_device.h"
+#include "h8_node.h"
 
 namespace h8 {
 
@@ -45,6 +46,8 @@ constexpr long long MG_COARSE_DOF = 128;   // stop coarsening at or below this m
 constexpr int MG_LDS_GJ = 128;             // coarsest levels up to this many DOFs are inverted in LDS
 constexpr long long MG_DENSE_MAX = 512;    // the coarsest level must be this small (dense inverse)
 constexpr float MG_CHEB_RATIO = 30.0f;     // smoothing interval [lam / ratio, lam]
+constexpr long long MG_BOTTOM_NODES = 256;   // levels this small run in one workgroup (k_mg_bottom)
+constexpr int MG_BOTTOM_THREADS = 512;
 
 // One level of the hierarchy (by value in kernel arguments).
 struct MgLevel {
@@ -422,19 +425,26 @@ __global__ void __launch_bounds__(256) k_mg_cheb_vec(long long n, float* __restr
 __device__ __forceinline__ float mg_row(const MgLevel& L, const float* __restrict__ x, long long n, int c) {
     int j, i, k;
     mg_decode(L, n, j, i, k);
-    const int j0 = j > 0 ? -1 : 0, j1 = j + 1 < L.NJ ? 1 : 0;
-    const int i0 = i > 0 ? -1 : 0, i1 = i + 1 < L.NI ? 1 : 0;
-    const int k0 = k > 0 ? -1 : 0, k1 = k + 1 < L.NK ? 1 : 0;
+    // Branch-free over all 27 neighbours, so every load of the row is issued at
+    // once: the stencil blocks of neighbours outside the grid are exactly zero
+    // (the setup writes them so), and their x index is clamped to the node itself.
+    const long long sj = L.NJ, si = (long long)L.NI * L.NJ;
+    // (one plane of 9 neighbours per pass: its 54 loads in flight together, at
+    // a third of the registers of all 27)
     const float* Sc = L.S + (long long)(3 * c) * L.nnodes + n;
     float y = 0.0f;
-    for (int dk = k0; dk <= k1; ++dk)
-        for (int di = i0; di <= i1; ++di)
-            for (int dj = j0; dj <= j1; ++dj) {
-                const int nb = ((dk + 1) * 3 + di + 1) * 3 + dj + 1;
-                const long long m = 3 * (n + ((long long)dk * L.NI + di) * L.NJ + dj);
-                const float* s = Sc + (long long)(nb * 9) * L.nnodes;
-                y += s[0] * x[m] + s[L.nnodes] * x[m + 1] + s[2 * L.nnodes] * x[m + 2];
-            }
+#pragma unroll 1
+    for (int dk = -1; dk <= 1; ++dk) {
+        const bool kin = unsigned(k + dk) < unsigned(L.NK);
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+            const int dj = p % 3 - 1, di = p / 3 - 1;
+            const bool in = kin && unsigned(j + dj) < unsigned(L.NJ) && unsigned(i + di) < unsigned(L.NI);
+            const long long m = 3 * (in ? n + dk * si + di * sj + dj : n);
+            const float* s = Sc + (long long)(((dk + 1) * 9 + p) * 9) * L.nnodes;
+            y += s[0] * x[m] + s[L.nnodes] * x[m + 1] + s[2 * L.nnodes] * x[m + 2];
+        }
+    }
     return y;
 }
 
@@ -466,32 +476,80 @@ __global__ void __launch_bounds__(256) k_mg_spmv_st(MgLevel L, const float* __re
     }
 }
 
-// Restriction of the masked residual: bC = P^T M_F (bF - yF), one thread per
-// coarse DOF (component-major).
+// Coarse DOF t (component-major) of the restricted masked residual
+// P^T M_F (bF - yF).
+__device__ __forceinline__ float mg_restrict_dof(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
+                                                 const float* __restrict__ bF, const float* __restrict__ yF,
+                                                 const float* __restrict__ maskF, long long t, long long& qC) {
+    const int c = int(t / C.nnodes);
+    const long long In = t - c * C.nnodes;
+    int Ij, Ii, Ik;
+    mg_decode(C, In, Ij, Ii, Ik);
+    const int fj = Ij * ry, fi = Ii * rx, fk = Ik * rz;
+    // all 27 offsets, branch-free: weight 0 (and the index clamped to the
+    // centre) for an offset along a kept axis or outside the grid
+    float acc = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        const int oj = o % 3 - 1, oi = (o / 3) % 3 - 1, ok = o / 9 - 1;
+        const bool in = (oj == 0 || (ry == 2 && unsigned(fj + oj) < unsigned(F.NJ))) &&
+                        (oi == 0 || (rx == 2 && unsigned(fi + oi) < unsigned(F.NI))) &&
+                        (ok == 0 || (rz == 2 && unsigned(fk + ok) < unsigned(F.NK)));
+        const float w = in ? (oj ? 0.5f : 1.0f) * (oi ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f) : 0.0f;
+        const long long f = 3 * (in ? ((long long)(fk + ok) * F.NI + (fi + oi)) * F.NJ + (fj + oj)
+                                    : ((long long)fk * F.NI + fi) * F.NJ + fj) + c;
+        acc += (maskF[f] != 0.0f) ? w * (bF[f] - yF[f]) : 0.0f;
+    }
+    qC = 3 * In + c;
+    return acc;
+}
+
+// Fine node fn of the coarse correction xF += M_F P xC.
+__device__ __forceinline__ void mg_prolong_node(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
+                                                float* __restrict__ xF, const float* __restrict__ xC,
+                                                const float* __restrict__ maskF, long long fn) {
+    int fj, fi, fk;
+    mg_decode(F, fn, fj, fi, fk);
+    // parents per axis: f = r I (weight 1) or, on a halved axis at an odd f,
+    // I = (f -+ 1) / 2 (0.5 each); all 8 candidates read, the absent second
+    // parent of an axis with weight 0 at the first one's index
+    const int j0 = fj / ry, i0 = fi / rx, k0 = fk / rz;
+    const bool tj = ry == 2 && (fj & 1), ti = rx == 2 && (fi & 1), tk = rz == 2 && (fk & 1);
+    float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int e = a & 1, b = (a >> 1) & 1, g = a >> 2;
+        const float w = (tj ? 0.5f : (e ? 0.0f : 1.0f)) * (ti ? 0.5f : (b ? 0.0f : 1.0f)) *
+                        (tk ? 0.5f : (g ? 0.0f : 1.0f));
+        const long long I = 3 * (((long long)(k0 + (tk ? g : 0)) * C.NI + (i0 + (ti ? b : 0))) * C.NJ +
+                                 (j0 + (tj ? e : 0)));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += w * xC[I + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        if (maskF[3 * fn + c] != 0.0f) xF[3 * fn + c] += acc[c];
+}
+
+// Restriction bC = P^T M_F (bF - yF), one thread per coarse DOF; with pre0
+// the coarse level's first smoothing step from zero follows at once in the
+// same thread: x0 = d = c0 dinv b (cstC = the coarse level's constants).
 __global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C, int rx, int ry, int rz,
                                                      const float* __restrict__ bF, const float* __restrict__ yF,
-                                                     const float* __restrict__ maskF, float* __restrict__ bC,
+                                                     const float* __restrict__ maskF, const MgConst* cstC, int pre0,
                                                      const PcgState* st) {
     if (!mg_live(st)) return;
+    const float c0 = pre0 ? cstC->c0 : 0.0f;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < C.ndof;
          t += (long long)gridDim.x * blockDim.x) {
-        const int c = int(t / C.nnodes);
-        const long long In = t - c * C.nnodes;
-        int Ij, Ii, Ik;
-        mg_decode(C, In, Ij, Ii, Ik);
-        const int fj = Ij * ry, fi = Ii * rx, fk = Ik * rz;
-        const int j0 = (ry == 2 && fj > 0) ? -1 : 0, j1 = (ry == 2 && fj + 1 < F.NJ) ? 1 : 0;
-        const int i0 = (rx == 2 && fi > 0) ? -1 : 0, i1 = (rx == 2 && fi + 1 < F.NI) ? 1 : 0;
-        const int k0 = (rz == 2 && fk > 0) ? -1 : 0, k1 = (rz == 2 && fk + 1 < F.NK) ? 1 : 0;
-        float acc = 0.0f;
-        for (int ok = k0; ok <= k1; ++ok)
-            for (int oi = i0; oi <= i1; ++oi)
-                for (int oj = j0; oj <= j1; ++oj) {
-                    const float w = (oj ? 0.5f : 1.0f) * (oi ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f);
-                    const long long f = 3 * (((long long)(fk + ok) * F.NI + (fi + oi)) * F.NJ + (fj + oj)) + c;
-                    if (maskF[f] != 0.0f) acc += w * (bF[f] - yF[f]);
-                }
-        bC[3 * In + c] = acc;
+        long long q;
+        const float b = mg_restrict_dof(F, C, rx, ry, rz, bF, yF, maskF, t, q);
+        C.b[q] = b;
+        if (pre0) {
+            const float d = c0 * C.dinv[q] * b;
+            C.d[q] = d;
+            C.x0[q] = d;
+        }
     }
 }
 
@@ -501,31 +559,90 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MgLevel F, MgLevel C, int rx
                                                     const float* __restrict__ maskF, const PcgState* st) {
     if (!mg_live(st)) return;
     for (long long fn = (long long)blockIdx.x * blockDim.x + threadIdx.x; fn < F.nnodes;
-         fn += (long long)gridDim.x * blockDim.x) {
-        int fj, fi, fk;
-        mg_decode(F, fn, fj, fi, fk);
-        // parents per axis: f = r I (weight 1) or, on a halved axis at an odd f, I = (f -+ 1) / 2 (0.5 each)
-        const int j0 = fj / ry, i0 = fi / rx, k0 = fk / rz;
-        const int nj = (ry == 2 && (fj & 1)) ? 2 : 1, ni = (rx == 2 && (fi & 1)) ? 2 : 1,
-                  nk = (rz == 2 && (fk & 1)) ? 2 : 1;
-        const float w = (nj == 2 ? 0.5f : 1.0f) * (ni == 2 ? 0.5f : 1.0f) * (nk == 2 ? 0.5f : 1.0f);
-        float acc[3] = {0.f, 0.f, 0.f};
-        for (int a = 0; a < nk; ++a)
-            for (int b = 0; b < ni; ++b)
-                for (int e = 0; e < nj; ++e) {
-                    const long long I = 3 * (((long long)(k0 + a) * C.NI + (i0 + b)) * C.NJ + (j0 + e));
+         fn += (long long)gridDim.x * blockDim.x)
+        mg_prolong_node(F, C, rx, ry, rz, xF, xC, maskF, fn);
+}
+
+// The small levels of a cycle in one workgroup: levels lb .. nl-1 (the last
+// the coarsest, inverted densely), given b of level lb; leaves that level's
+// result in its x1.  Phases are separated by workgroup barriers (all waves of
+// a workgroup share one CU and its L1, so their global stores are visible to
+// each other after the barrier).
+struct MgBottom {
+    MgLevel L[MG_MAX_LEVELS];
+    int lb, nl;
+    const MgConst* cst;
+    const float* Ainv;
+};
+
+__device__ __forceinline__ void mg_bottom_cheb(const MgLevel& L, const MgConst& k, const float* xin, float* xout,
+                                               int step) {
+    const float ca = step ? k.c1a : 0.0f, cb = step ? k.c1b : k.c0;
+    for (long long t = threadIdx.x; t < L.ndof; t += blockDim.x) {
+        const int c = int(t / L.nnodes);
+        const long long n = t - c * L.nnodes, q = 3 * n + c;
+        const float dn = (step ? ca * L.d[q] : 0.0f) + cb * L.dinv[q] * (L.b[q] - mg_row(L, xin, n, c));
+        L.d[q] = dn;
+        xout[q] = xin[q] + dn;
+    }
+}
+
+__global__ void __launch_bounds__(MG_BOTTOM_THREADS) k_mg_bottom(MgBottom B, const PcgState* st) {
+    if (!mg_live(st)) return;
+    for (int l = B.lb; l + 1 < B.nl; ++l) {
+        const MgLevel& F = B.L[l];
+        const MgLevel& C = B.L[l + 1];
+        const MgConst& k = B.cst[l];
+        for (long long q = threadIdx.x; q < F.ndof; q += blockDim.x) {
+            const float d = k.c0 * F.dinv[q] * F.b[q];
+            F.d[q] = d;
+            F.x0[q] = d;
+        }
+        __syncthreads();
+        mg_bottom_cheb(F, k, F.x0, F.x1, 1);
+        __syncthreads();
+        for (long long t = threadIdx.x; t < F.ndof; t += blockDim.x) {
+            const int c = int(t / F.nnodes);
+            const long long n = t - c * F.nnodes;
+            F.y[3 * n + c] = mg_row(F, F.x1, n, c);
+        }
+        __syncthreads();
+        for (long long t = threadIdx.x; t < C.ndof; t += blockDim.x) {
+            long long q;
+            const float b = mg_restrict_dof(F, C, F.rx, F.ry, F.rz, F.b, F.y, F.dinv, t, q);
+            C.b[q] = b;
+        }
+        __syncthreads();
+    }
+    {
+        // coarsest: x1 = Ainv b, one wave per row
+        const MgLevel& Lc = B.L[B.nl - 1];
+        const int n = int(Lc.ndof), lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int row = wv; row < n; row += nw) {
+            float acc = 0.0f;
+            for (int j = lane; j < n; j += 64) acc += B.Ainv[(long long)row * n + j] * Lc.b[j];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) acc[c] += w * xC[I + c];
-                }
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if (maskF[3 * fn + c] != 0.0f) xF[3 * fn + c] += acc[c];
+            for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if (lane == 0) Lc.x1[row] = acc;
+        }
+        __syncthreads();
+    }
+    for (int l = B.nl - 2; l >= B.lb; --l) {
+        const MgLevel& F = B.L[l];
+        const MgLevel& C = B.L[l + 1];
+        const MgConst& k = B.cst[l];
+        for (long long fn = threadIdx.x; fn < F.nnodes; fn += blockDim.x)
+            mg_prolong_node(F, C, F.rx, F.ry, F.rz, F.x1, C.x1, F.dinv, fn);
+        __syncthreads();
+        mg_bottom_cheb(F, k, F.x1, F.x0, 0);
+        __syncthreads();
+        mg_bottom_cheb(F, k, F.x0, F.x1, 1);
+        __syncthreads();
     }
 }
 
 // r.z of the FP32 iterations with the V-cycle's z; the last block sets beta
-// (0 when the direction restarts; a replacement that makes beta >= 1/delta^2
-// restarts it too) and rz_old.
+// (mg_rz_finish).
 __global__ void __launch_bounds__(256) k_mg_rz(long long n, const float* __restrict__ r, const float* __restrict__ z,
                                                PcgState* st, double* partials) {
     __shared__ double sRed[16];
@@ -535,20 +652,7 @@ __global__ void __launch_bounds__(256) k_mg_rz(long long n, const float* __restr
          i += (long long)gridDim.x * blockDim.x)
         v[0] += double(r[i]) * double(z[i]);
     double tot[1];
-    if (reduce_last_block<1>(v, partials, &st->cnt[3], tot, sRed) && threadIdx.x == 0) {
-        const double rz = tot[0];
-        double beta = 0.0;
-        if (!st->first) {
-            beta = rz / st->rz_old;
-            if (!(beta >= 0.0) || (st->replaced && !(beta < 1.0 / (st->delta * st->delta)))) {
-                st->first = 1;
-                beta = 0.0;
-            }
-        }
-        st->beta = beta;
-        st->rz_old = rz;
-        st->replaced = 0;
-    }
+    if (reduce_last_block<1>(v, partials, &st->cnt[3], tot, sRed) && threadIdx.x == 0) mg_rz_finish(st, tot[0]);
 }
 
 }  // namespace h8

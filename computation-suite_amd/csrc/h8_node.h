@@ -28,11 +28,14 @@
 
 namespace h8 {
 
-enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3, MODE_DIRZ = 4 };
+enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3, MODE_DIRZ = 4, MODE_CHEB = 5 };
 // MODE_DIRX = MODE_DIR that also applies the previous iteration's deferred
 // x += alpha p_old to the owned nodes while staging p_old.
 // MODE_DIRZ = MODE_DIR with a given preconditioned residual: p = z + beta p_old,
 // z in NodeArgs::r (the multigrid-preconditioned loop; sector kernels only).
+// MODE_CHEB = a Chebyshev smoothing step of the multigrid cycle on the grid:
+// d = ca d + cb dinv (b - K x), x_out = x + d (x = NodeArgs::u, sector kernels
+// only); with cheb_rz also the reduction r.z of x_out with b and the beta step.
 __host__ __device__ constexpr bool is_dir(int mode) { return mode == MODE_DIR || mode == MODE_DIRX || mode == MODE_DIRZ; }
 
 template <typename T>
@@ -54,6 +57,12 @@ struct NodeArgs {
     PcgState* st;
     double* partials;
     int only_on_replace;  // RESID: every block returns at once unless st asks for a replacement
+    const float* cheb;    // CHEB: the level's coefficients {c0, c1a, c1b} (device)
+    const T* cb_b;        // CHEB: right-hand side
+    T* cb_d;              // CHEB: Chebyshev direction (updated in place)
+    T* cb_out;            // CHEB: x_out (not u: other blocks still read u)
+    int cheb_step;        // CHEB: 0 = first step of a pass (ca = 0, cb = c0), 1 = second
+    int cheb_rz;          // CHEB: reduce b.x_out and finish the iteration's beta
 };
 
 // Corner whose offsets are the XOR of corners a and b, and the sign of
@@ -81,6 +90,23 @@ __device__ inline void dir_finish(PcgState* st, double pq) {
         st->alpha = st->rz_old / pq;
         st->pending = 1;       // x += alpha p is applied by the next direction kernel
     }
+}
+
+// End of the multigrid-preconditioned FP32 iteration, given rz = r.z with
+// z the V-cycle's output: beta = 0 when the direction restarts (p = z), else
+// rz / rz_old, and a replacement that makes beta >= 1/delta^2 restarts it.
+__device__ inline void mg_rz_finish(PcgState* st, double rz) {
+    double beta = 0.0;
+    if (!st->first) {
+        beta = rz / st->rz_old;
+        if (!(beta >= 0.0) || (st->replaced && !(beta < 1.0 / (st->delta * st->delta)))) {
+            st->first = 1;
+            beta = 0.0;
+        }
+    }
+    st->beta = beta;
+    st->rz_old = rz;
+    st->replaced = 0;
 }
 
 // Last block of an FP64 residual replacement: converged on the true residual,
@@ -306,6 +332,18 @@ __device__ __forceinline__ void node_output(const NodeArgs<T>& A, int gn, const 
             Buf<T>::st(qv[c], buf_rsrc(A.q, vbytes), int(sizeof(T)) * (gn + c));
             part[0] += double(ctr[c]) * double(qv[c]);
         }
+    } else if constexpr (MODE == MODE_CHEB) {
+        const T ca = A.cheb_step ? T(A.cheb[1]) : T(0), cb = A.cheb_step ? T(A.cheb[2]) : T(A.cheb[0]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int q = gn + c;
+            const T b = A.cb_b[q];
+            const T dn = (A.cheb_step ? ca * A.cb_d[q] : T(0)) + cb * A.dinv[q] * (b - qv[c]);
+            A.cb_d[q] = dn;
+            const T xo = ctr[c] + dn;
+            A.cb_out[q] = xo;
+            part[0] += double(b) * double(xo);
+        }
     } else {  // MODE_RESID (T = double)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -405,6 +443,13 @@ __device__ __forceinline__ void node_finish(const NodeArgs<T>& A, double (&part)
         if (reduce_last_block<1>(v, A.partials, &A.st->cnt[0], tot, sRed)) {
             if (threadIdx.x == 0) dir_finish(A.st, tot[0]);
         }
+    } else if constexpr (MODE == MODE_CHEB) {
+        if (A.cheb_rz) {
+            double v[1] = {part[0]}, tot[1];
+            if (reduce_last_block<1>(v, A.partials, &A.st->cnt[3], tot, sRed)) {
+                if (threadIdx.x == 0) mg_rz_finish(A.st, tot[0]);
+            }
+        }
     } else if constexpr (MODE == MODE_RESID) {
         double v[2] = {part[0], part[1]}, tot[2];
         if (reduce_last_block<2>(v, A.partials, &A.st->cnt[2], tot, sRed)) {
@@ -425,7 +470,7 @@ __device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& f
     if constexpr (MODE == MODE_RESID) {
         if (A.only_on_replace && A.st->status != ST_REPLACE) return false;
     }
-    if constexpr (MODE == MODE_PLAIN) {
+    if constexpr (MODE == MODE_PLAIN || MODE == MODE_CHEB) {
         // a product inside a solve's captured iterations (the multigrid cycle)
         // passes the solve state and does nothing once the solve has stopped
         if (A.st && A.st->status != ST_RUNNING) return false;
