@@ -72,11 +72,15 @@ struct MgConst {
 
 __device__ __forceinline__ bool mg_live(const PcgState* st) { return !st || st->status == ST_RUNNING; }
 
+// (j, i, k) of node n.  32-bit arithmetic: the plan keeps every index of a
+// stencil level below 2^31 (243 x its nodes), and 64-bit division is a long
+// software sequence on the GPU.
 __device__ __forceinline__ void mg_decode(const MgLevel& L, long long n, int& j, int& i, int& k) {
-    j = int(n % L.NJ);
-    const long long q = n / L.NJ;
-    i = int(q % L.NI);
-    k = int(q / L.NI);
+    const unsigned nn = unsigned(n), NJ = unsigned(L.NJ), NI = unsigned(L.NI);
+    const unsigned q = nn / NJ;
+    j = int(nn - q * NJ);
+    k = int(q / NI);
+    i = int(q - unsigned(k) * NI);
 }
 
 // Interpolation weight of fine position f (in fine element units, within one
@@ -127,7 +131,9 @@ __global__ void __launch_bounds__(256) k_mg_gersh0(GridDev g, const float* __res
     float best = 0.0f;
     for (long long nd = (long long)blockIdx.x * blockDim.x + threadIdx.x; nd < g.nnodes;
          nd += (long long)gridDim.x * blockDim.x) {
-        const int j = int(nd % g.NJ), i = int((nd / g.NJ) % g.NI), k = int(nd / ((long long)g.NJ * g.NI));
+        const unsigned q = unsigned(nd) / unsigned(g.NJ);
+        const int j = int(unsigned(nd) - q * unsigned(g.NJ)), k = int(q / unsigned(g.NI));
+        const int i = int(q - unsigned(k) * unsigned(g.NI));
         float sum[3] = {0.f, 0.f, 0.f};
         for (int a = 0; a < 8; ++a) {
             const int je = j - OY(a), ie = i - OX(a), ke = k - OZ(a);
@@ -149,7 +155,9 @@ __global__ void __launch_bounds__(256) k_mg_elem_mask(GridDev g, const float* __
     const long long m = (long long)g.nx * g.ny * g.nz;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m;
          e += (long long)gridDim.x * blockDim.x) {
-        const int je = int(e % g.ny), ie = int((e / g.ny) % g.nx), ke = int(e / ((long long)g.ny * g.nx));
+        const unsigned q = unsigned(e) / unsigned(g.ny);
+        const int je = int(unsigned(e) - q * unsigned(g.ny)), ke = int(q / unsigned(g.nx));
+        const int ie = int(q - unsigned(ke) * unsigned(g.nx));
         unsigned mask = 0;
         for (int a = 0; a < 8; ++a) {
             const long long fn = ((long long)(ke + OZ(a)) * g.NI + (ie + OX(a))) * g.NJ + (je + OY(a));
@@ -160,88 +168,113 @@ __global__ void __launch_bounds__(256) k_mg_elem_mask(GridDev g, const float* __
     }
 }
 
-// Level-1 stencil from the fine elements: one thread per (coarse node I,
-// neighbour block nb).  The block A_1[I][J] sums, over the coarse elements E
-// holding both I (corner A) and J = I + nb (corner B), and over E's children
-// e, s_e (P_c^T M_e K0 M_e P_c)[3A + i][3B + j] -- from M_c when the child
-// has no fixed DOF, else evaluated with its mask.
-__global__ void __launch_bounds__(256) k_mg_galerkin0(GridDev g, MgLevel C, int rx, int ry, int rz,
-                                                      const double* __restrict__ s_e, const double* __restrict__ K0,
-                                                      const double* __restrict__ M, const unsigned* __restrict__ emask) {
-    const long long total = 27LL * C.nnodes;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int nb = int(t / C.nnodes);
-        const long long In = t - (long long)nb * C.nnodes;
-        int Ij, Ii, Ik;
-        mg_decode(C, In, Ij, Ii, Ik);
-        const int dj = nb % 3 - 1, di = (nb / 3) % 3 - 1, dk = nb / 9 - 1;
-        const int Jj = Ij + dj, Ji = Ii + di, Jk = Ik + dk;
-        double blk[9];
-        for (int q = 0; q < 9; ++q) blk[q] = 0.0;
-        if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
-            // coarse elements E (origin) with E <= I, J <= E + 1 on every axis
-            const int ej0 = max(max(Ij, Jj) - 1, 0), ej1 = min(min(Ij, Jj), C.ny - 1);
-            const int ei0 = max(max(Ii, Ji) - 1, 0), ei1 = min(min(Ii, Ji), C.nx - 1);
-            const int ek0 = max(max(Ik, Jk) - 1, 0), ek1 = min(min(Ik, Jk), C.nz - 1);
-            for (int Ek = ek0; Ek <= ek1; ++Ek)
-                for (int Ei = ei0; Ei <= ei1; ++Ei)
-                    for (int Ej = ej0; Ej <= ej1; ++Ej) {
-                        const int A = corner_of(Ii - Ei, Ij - Ej, Ik - Ek);
-                        const int B = corner_of(Ji - Ei, Jj - Ej, Jk - Ek);
-                        for (int cz = 0; cz < rz; ++cz)
-                            for (int cy = 0; cy < ry; ++cy)
-                                for (int cx = 0; cx < rx; ++cx) {
-                                    const int fe_i = Ei * rx + cx, fe_j = Ej * ry + cy, fe_k = Ek * rz + cz;
-                                    const long long fe = ((long long)fe_k * g.nx + fe_i) * g.ny + fe_j;
-                                    const double s = s_e[fe];
-                                    const unsigned mask = emask[fe];     // free bits of the child's 24 DOFs
-                                    if (mask == 0xFFFFFFu) {
-                                        const double* Mc = M + (cx + 2 * cy + 4 * cz) * 576;
-                                        for (int i = 0; i < 3; ++i)
-                                            for (int j = 0; j < 3; ++j)
-                                                blk[3 * i + j] += s * Mc[(3 * A + i) * 24 + 3 * B + j];
-                                    } else if (mask) {
-                                        for (int a = 0; a < 8; ++a) {
-                                            const double pa = mg_w(cx + OX(a), OX(A), rx) *
-                                                              mg_w(cy + OY(a), OY(A), ry) * mg_w(cz + OZ(a), OZ(A), rz);
-                                            if (pa == 0.0) continue;
-                                            for (int b = 0; b < 8; ++b) {
-                                                const double pb = mg_w(cx + OX(b), OX(B), rx) *
-                                                                  mg_w(cy + OY(b), OY(B), ry) *
-                                                                  mg_w(cz + OZ(b), OZ(B), rz);
-                                                if (pb == 0.0) continue;
-                                                for (int i = 0; i < 3; ++i) {
-                                                    if (!((mask >> (3 * a + i)) & 1u)) continue;
-                                                    for (int j = 0; j < 3; ++j)
-                                                        if ((mask >> (3 * b + j)) & 1u)
-                                                            blk[3 * i + j] +=
-                                                                s * pa * pb * K0[(3 * a + i) * 24 + 3 * b + j];
-                                                }
-                                            }
-                                        }
-                                    }
-                                }
+// Level-1 element matrices: Kc_E = sum over the children e of coarse element
+// E of s_e (P_c^T M_e K0 M_e P_c), FP64, [E][24][24]; one thread per entry,
+// the lanes of a wave over the entries of one element (its children's scales
+// and masks are wave-uniform loads).  A child with all 24 DOFs free
+// contributes s_e M_c (M_c staged in LDS); a child with fixed DOFs is
+// evaluated with its mask.
+__global__ void __launch_bounds__(256) k_mg_coarse_elements(GridDev g, MgLevel C, int rx, int ry, int rz,
+                                                            const double* __restrict__ s_e,
+                                                            const double* __restrict__ K0,
+                                                            const double* __restrict__ Mg,
+                                                            const unsigned* __restrict__ emask,
+                                                            double* __restrict__ Kc) {
+    __shared__ double M[8 * 576];
+    for (int t = threadIdx.x; t < 8 * 576; t += blockDim.x) M[t] = Mg[t];
+    __syncthreads();
+    const unsigned mc = unsigned(C.nx) * unsigned(C.ny) * unsigned(C.nz);
+    const unsigned cny = unsigned(C.ny), cnx = unsigned(C.nx);
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < 576u * mc; t += gridDim.x * blockDim.x) {
+        const unsigned E = t / 576u, ent = t - E * 576u;
+        const int row = int(ent / 24u), col = int(ent % 24u);
+        const int A = row / 3, i = row % 3, B = col / 3, j = col % 3;
+        const unsigned q = E / cny;
+        const int Ej = int(E - q * cny), Ek = int(q / cnx), Ei = int(q - unsigned(Ek) * cnx);
+        double acc = 0.0;
+        for (int cz = 0; cz < rz; ++cz)
+            for (int cy = 0; cy < ry; ++cy)
+                for (int cx = 0; cx < rx; ++cx) {
+                    const long long fe = ((long long)(Ek * rz + cz) * g.nx + (Ei * rx + cx)) * g.ny + (Ej * ry + cy);
+                    const double sv = s_e[fe];
+                    const unsigned mask = emask[fe];
+                    if (mask == 0xFFFFFFu) {
+                        acc += sv * M[(cx + 2 * cy + 4 * cz) * 576 + int(ent)];
+                    } else if (mask) {
+                        for (int a = 0; a < 8; ++a) {
+                            if (!((mask >> (3 * a + i)) & 1u)) continue;
+                            const double pa =
+                                mg_w(cx + OX(a), OX(A), rx) * mg_w(cy + OY(a), OY(A), ry) * mg_w(cz + OZ(a), OZ(A), rz);
+                            if (pa == 0.0) continue;
+                            for (int b = 0; b < 8; ++b) {
+                                if (!((mask >> (3 * b + j)) & 1u)) continue;
+                                const double pb = mg_w(cx + OX(b), OX(B), rx) * mg_w(cy + OY(b), OY(B), ry) *
+                                                  mg_w(cz + OZ(b), OZ(B), rz);
+                                if (pb != 0.0) acc += sv * pa * pb * K0[(3 * a + i) * 24 + 3 * b + j];
+                            }
+                        }
                     }
+                }
+        Kc[t] = acc;
+    }
+}
+
+// Level-1 stencil from the element matrices: every entry (coarse node I,
+// neighbour block nb, entry 3 c + d) sums, over the coarse elements E that
+// hold both I (corner A) and J = I + nb (corner B), Kc_E[3A + c][3B + d] in a
+// fixed order.  A block takes GT consecutive nodes: its lanes walk the 243
+// entries of a node (nearby words of a few element matrices), the results
+// are transposed in LDS and stored node-contiguous.
+constexpr int MG_GT = 32;
+__global__ void __launch_bounds__(256) k_mg_galerkin0(MgLevel C, const double* __restrict__ Kc) {
+    __shared__ float tile[243 * MG_GT];
+    for (long long I0 = (long long)blockIdx.x * MG_GT; I0 < C.nnodes; I0 += (long long)gridDim.x * MG_GT) {
+        const int nt = int(min((long long)MG_GT, C.nnodes - I0));
+        for (int k = threadIdx.x; k < 243 * nt; k += blockDim.x) {
+            const int in = k / 243, e = k - in * 243;
+            const int nb = e / 9, c = (e % 9) / 3, d = e % 3;
+            int Ij, Ii, Ik;
+            mg_decode(C, I0 + in, Ij, Ii, Ik);
+            const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
+            double acc = 0.0;
+            if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
+                const int ej0 = max(max(Ij, Jj) - 1, 0), ej1 = min(min(Ij, Jj), C.ny - 1);
+                const int ei0 = max(max(Ii, Ji) - 1, 0), ei1 = min(min(Ii, Ji), C.nx - 1);
+                const int ek0 = max(max(Ik, Jk) - 1, 0), ek1 = min(min(Ik, Jk), C.nz - 1);
+                for (int Ek = ek0; Ek <= ek1; ++Ek)
+                    for (int Ei = ei0; Ei <= ei1; ++Ei)
+                        for (int Ej = ej0; Ej <= ej1; ++Ej) {
+                            const int A = corner_of(Ii - Ei, Ij - Ej, Ik - Ek);
+                            const int B = corner_of(Ji - Ei, Jj - Ej, Jk - Ek);
+                            const long long E = ((long long)Ek * C.nx + Ei) * C.ny + Ej;
+                            acc += Kc[E * 576 + (3 * A + c) * 24 + 3 * B + d];
+                        }
+            }
+            tile[e * MG_GT + in] = float(acc);
         }
-        for (int q = 0; q < 9; ++q) C.S[(long long)(nb * 9 + q) * C.nnodes + In] = float(blk[q]);
+        __syncthreads();
+        for (int k = threadIdx.x; k < 243 * MG_GT; k += blockDim.x) {
+            const int e = k / MG_GT, in = k - e * MG_GT;
+            if (in < nt) C.S[(long long)e * C.nnodes + I0 + in] = tile[k];
+        }
+        __syncthreads();
     }
 }
 
 // Stencil of level l+1 from the stencil of level l (A_{l+1} = P^T A_l P; A_l
-// is zero at its masked DOFs, so P needs no mask here): one thread per
-// (coarse node I, neighbour block nb).
+// is zero at its masked DOFs, so P needs no mask here): one thread per stencil
+// entry (coarse node I, neighbour block nb, entry q = 3 c + d of the block).
 __global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, int ry, int rz) {
-    const long long total = 27LL * C.nnodes;
+    const long long total = 243LL * C.nnodes;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        const int nb = int(t / C.nnodes);
-        const long long In = t - (long long)nb * C.nnodes;
+        const int e = int(unsigned(t) / unsigned(C.nnodes));   // nb * 9 + q
+        const int nb = e / 9, q = e % 9;
+        const long long In = t - (long long)e * C.nnodes;
         int Ij, Ii, Ik;
         mg_decode(C, In, Ij, Ii, Ik);
         const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
-        double blk[9];
-        for (int q = 0; q < 9; ++q) blk[q] = 0.0;
+        double acc = 0.0;
         if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
             // fine nodes of I's support: offsets -1..1 on halved axes, 0 on kept ones
             for (int ok = -(rz - 1); ok <= rz - 1; ++ok)
@@ -262,15 +295,14 @@ __global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, in
                         for (int sk = sk0; sk <= sk1; ++sk)
                             for (int si = si0; si <= si1; ++si)
                                 for (int sj = sj0; sj <= sj1; ++sj) {
-                                    const int s = ((sk + 1) * 3 + si + 1) * 3 + sj + 1;
+                                    const int sn = ((sk + 1) * 3 + si + 1) * 3 + sj + 1;
                                     const int uj = fj + sj - Jj * ry, ui = fi + si - Ji * rx, uk = fk + sk - Jk * rz;
                                     const double w = wi * (uj ? 0.5 : 1.0) * (ui ? 0.5 : 1.0) * (uk ? 0.5 : 1.0);
-                                    for (int q = 0; q < 9; ++q)
-                                        blk[q] += w * double(F.S[(long long)(s * 9 + q) * F.nnodes + fn]);
+                                    acc += w * double(F.S[(long long)(sn * 9 + q) * F.nnodes + fn]);
                                 }
                     }
         }
-        for (int q = 0; q < 9; ++q) C.S[(long long)(nb * 9 + q) * C.nnodes + In] = float(blk[q]);
+        C.S[(long long)e * C.nnodes + In] = float(acc);
     }
 }
 
@@ -455,7 +487,7 @@ __global__ void __launch_bounds__(256) k_mg_cheb_st(MgLevel L, const float* __re
     const float ca = step ? cst->c1a : 0.0f, cb = step ? cst->c1b : cst->c0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
          t += (long long)gridDim.x * blockDim.x) {
-        const int c = int(t / L.nnodes);
+        const int c = int(unsigned(t) / unsigned(L.nnodes));
         const long long n = t - c * L.nnodes, q = 3 * n + c;
         const float y = mg_row(L, xin, n, c);
         const float dn = ca * L.d[q] + cb * L.dinv[q] * (L.b[q] - y);
@@ -470,7 +502,7 @@ __global__ void __launch_bounds__(256) k_mg_spmv_st(MgLevel L, const float* __re
     if (!mg_live(st)) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
          t += (long long)gridDim.x * blockDim.x) {
-        const int c = int(t / L.nnodes);
+        const int c = int(unsigned(t) / unsigned(L.nnodes));
         const long long n = t - c * L.nnodes;
         y[3 * n + c] = mg_row(L, x, n, c);
     }
@@ -481,7 +513,7 @@ __global__ void __launch_bounds__(256) k_mg_spmv_st(MgLevel L, const float* __re
 __device__ __forceinline__ float mg_restrict_dof(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
                                                  const float* __restrict__ bF, const float* __restrict__ yF,
                                                  const float* __restrict__ maskF, long long t, long long& qC) {
-    const int c = int(t / C.nnodes);
+    const int c = int(unsigned(t) / unsigned(C.nnodes));
     const long long In = t - c * C.nnodes;
     int Ij, Ii, Ik;
     mg_decode(C, In, Ij, Ii, Ik);
@@ -579,7 +611,7 @@ __device__ __forceinline__ void mg_bottom_cheb(const MgLevel& L, const MgConst& 
                                                int step) {
     const float ca = step ? k.c1a : 0.0f, cb = step ? k.c1b : k.c0;
     for (long long t = threadIdx.x; t < L.ndof; t += blockDim.x) {
-        const int c = int(t / L.nnodes);
+        const int c = int(unsigned(t) / unsigned(L.nnodes));
         const long long n = t - c * L.nnodes, q = 3 * n + c;
         const float dn = (step ? ca * L.d[q] : 0.0f) + cb * L.dinv[q] * (L.b[q] - mg_row(L, xin, n, c));
         L.d[q] = dn;
@@ -602,7 +634,7 @@ __global__ void __launch_bounds__(MG_BOTTOM_THREADS) k_mg_bottom(MgBottom B, con
         mg_bottom_cheb(F, k, F.x0, F.x1, 1);
         __syncthreads();
         for (long long t = threadIdx.x; t < F.ndof; t += blockDim.x) {
-            const int c = int(t / F.nnodes);
+            const int c = int(unsigned(t) / unsigned(F.nnodes));
             const long long n = t - c * F.nnodes;
             F.y[3 * n + c] = mg_row(F, F.x1, n, c);
         }

@@ -2,6 +2,7 @@
 
     python -m h8pcg.kbench --dims 256,64,64 --reps 50 [--precision 32,64] [--variant V] [--layers L] [--copy-mib 256]
     python -m h8pcg.kbench --dims 256,64,64 --reps 50 --recovery
+    python -m h8pcg.kbench --dims 256,64,64 --reps 20 --mg
 
 Builds the clamped box operator (per-element scale 10**U(9,12), seed 5), then
 times the fused direction + K*p kernel and the update kernel of one PCG
@@ -11,7 +12,9 @@ resident solver running `reps` iterations -- and finally runs a plain
 elementwise kernel (torch.neg, float32) over --copy-mib MiB -- it reads and
 writes exactly that many bytes -- so hardware byte counters (FETCH_SIZE,
 WRITE_SIZE) can be calibrated against a kernel of known traffic.  Prints one JSON line.  Run it under
-``rocprofv3 --kernel-trace --stats`` or ``rocprofv3 --pmc ...``.
+``rocprofv3 --kernel-trace --stats`` or ``rocprofv3 --pmc ...``.  With --mg it
+times the multigrid kernels of the FP32 iterations instead (h8_mg_time_kernels:
+the level-0 product, one level-1 Chebyshev step, one whole V-cycle).
 """
 from __future__ import annotations
 
@@ -106,6 +109,26 @@ def time_recovery(nx, ny, nz, reps, device="cuda:0"):
     return out
 
 
+def time_mg(op, reps):
+    """The multigrid kernels (h8_mg_time_kernels) with their algorithmic bytes:
+    level-0 product (u, s_e in; y out), level-1 Chebyshev step (243 stencil
+    words per node; x, b, d, dinv in; d, x out)."""
+    from . import _lib
+    ws, mws = op.workspace(32), op.mg_workspace()
+    avg = (C.c_double * 3)()
+    info = (C.c_int64 * 3)()
+    _lib.check(_lib.lib().h8_mg_time_kernels(C.byref(op.grid32), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
+                                             _lib.ptr(op.s_e32), _lib.ptr(op.K0_32), _lib.ptr(op.dinv),
+                                             _lib.ptr(ws), ws.numel(), _lib.ptr(mws), mws.numel(), int(reps), avg,
+                                             info, _lib.stream_handle(op.device)), "h8_mg_time_kernels")
+    b0 = 4 * (2 * op.n_dof + op.m)
+    b1 = 4 * 243 * info[2] + 4 * 6 * info[1]
+    us = [1e3 * v for v in avg]
+    return {"levels": info[0], "level1_dof": info[1], "product_us": us[0], "product_bytes": b0,
+            "product_GBps": b0 / us[0] * 1e-3, "cheb_st_us": us[1], "cheb_st_bytes": b1,
+            "cheb_st_GBps": (b1 / us[1] * 1e-3) if us[1] else None, "vcycle_us": us[2]}
+
+
 def main(argv=None):
     import torch
     ap = argparse.ArgumentParser()
@@ -117,6 +140,7 @@ def main(argv=None):
     ap.add_argument("--arith", type=int, default=0, help="FP32 element-product arithmetic (0 = library, 32, 64)")
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--recovery", action="store_true", help="time the stress recovery kernels instead")
+    ap.add_argument("--mg", action="store_true", help="time the multigrid kernels instead")
     a = ap.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.dims.split(","))
     if a.recovery:
@@ -134,7 +158,9 @@ def main(argv=None):
     op.grid32.layers = a.layers
     op.set_arith(a.arith)
     out = {"dims": [nx, ny, nz], "n_dof": op.n_dof, "elements": op.m, "reps": a.reps}
-    for prec in (int(p) for p in a.precision.split(",")):
+    if a.mg:
+        out["mg"] = time_mg(op, a.reps)
+    for prec in ((int(p) for p in a.precision.split(",")) if not a.mg else ()):
         d_us, u_us = time_kernels(op, prec, a.reps)
         b = bytes_per_launch(op, prec)
         if a.variant == 7:

@@ -248,12 +248,19 @@ class GridOperator:
             self._ws["mg"] = ws
         return ws
 
-    def refined(self, f, tol=1e-8, maxiter=None, delta=0.5, x0=None, preconditioner="jacobi"):
+    # residual-replacement threshold of the refined solve per preconditioner:
+    # with the V-cycle the FP32 residual falls about 2x per iteration, so 0.1
+    # replaces every few iterations (0.5 would replace every other one); with
+    # Jacobi, fewer replacements let the FP32 recurrence drift (MI355X sweeps)
+    DELTA = {"jacobi": 0.5, "mg": 0.1}
+
+    def refined(self, f, tol=1e-8, maxiter=None, delta=None, x0=None, preconditioner="jacobi"):
         """FP32 PCG iterations with FP64 residual replacement; returns (u, stats).
         x0: initial guess (full length; its fixed-DOF entries are ignored).
         preconditioner: 'jacobi' (z = dinv r) or 'mg' (z = one multigrid
         V-cycle; falls back to 'jacobi' when the coarse operator is singular,
-        stats['preconditioner'] says which ran)."""
+        stats['preconditioner'] says which ran).  delta: replacement threshold
+        (None: DELTA[preconditioner])."""
         import torch
         if self.dinv is None:
             raise RuntimeError("set_free_mask() must be called before solving")
@@ -266,13 +273,15 @@ class GridOperator:
         ws = self.workspace(32)
         st = H8PcgStats()
         used = preconditioner
+        d_mg = float(self.DELTA["mg"] if delta is None else delta)
+        d_j = float(self.DELTA["jacobi"] if delta is None else delta)
         with torch.cuda.device(self.device):
             rc = None
             if preconditioner == "mg":
                 mws = self.mg_workspace()
                 rc = _lib.lib().h8_refine_solve_mg(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf),
                                                    ptr(self.s_e32), ptr(self.K0_32), ptr(self.dinv), ptr(f), ptr(u),
-                                                   float(tol), int(maxiter), float(delta), 0 if x0 is None else 1,
+                                                   float(tol), int(maxiter), d_mg, 0 if x0 is None else 1,
                                                    ptr(ws), ws.numel(), ptr(mws), mws.numel(), C.byref(st),
                                                    self._stream())
                 if rc == _lib.H8_MG_DECLINED:
@@ -282,7 +291,7 @@ class GridOperator:
             if used == "jacobi":
                 fn = _lib.lib().h8_refine_solve if x0 is None else _lib.lib().h8_refine_solve_x0
                 rc = fn(C.byref(self.grid32), ptr(self.s_e), ptr(self.K0buf), ptr(self.s_e32), ptr(self.K0_32),
-                        ptr(self.dinv), ptr(f), ptr(u), float(tol), int(maxiter), float(delta), ptr(ws), ws.numel(),
+                        ptr(self.dinv), ptr(f), ptr(u), float(tol), int(maxiter), d_j, ptr(ws), ws.numel(),
                         C.byref(st), self._stream())
         stats = st.as_dict()
         stats["rc"] = rc

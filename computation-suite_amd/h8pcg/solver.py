@@ -454,7 +454,7 @@ class FESystem:
         return ReducedSystem(self, op, cm, w_t.cpu().numpy(), b)
 
     def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner=None,
-                            precision="fp64", delta=0.5, F=None):
+                            precision="fp64", delta=None, F=None):
         """solve_static, returning the full displacement as a device tensor (no host copy).
         preconditioner: None (the default: 'jacobi' in FP64, the reference's
         choice; 'mg' with precision='mixed'), 'jacobi', or 'mg' (geometric
