@@ -39,7 +39,6 @@ for p in (ROOT, os.path.join(ROOT, "computation-suite_amd")):
 
 METRIC = "PCG DOF·iters/s + %HBM roofline, FP32-refined vs FP64 @1e-8 rel residual"
 HBM_PEAK = 8.0e12
-MG_COARSE_DOF, MG_DENSE_MAX = 128, 512
 
 
 def log(*a):
@@ -60,65 +59,6 @@ def build_system(nx, ny, nz, seed):
     fs.add_consistent_facet_load(h8pcg.Hex8Solid3D(), h8pcg.boundary_faces(mesh, 0, float(nx)), 2,
                                  -1e6 / (ny * nz))
     return fs, E
-
-
-# ---------------------------------------------------------------------------
-# algorithmic bytes of the multigrid-preconditioned FP32 iteration
-# ---------------------------------------------------------------------------
-def mg_levels(nx, ny, nz):
-    """[(ndof, nnodes)] of the library's hierarchy (halve the even axes until
-    a level has at most 128 DOFs or no axis is even)."""
-    out = []
-    while True:
-        nn = (nx + 1) * (ny + 1) * (nz + 1)
-        out.append((3 * nn, nn))
-        r = [2 if v % 2 == 0 else 1 for v in (nx, ny, nz)]
-        if 3 * nn <= MG_COARSE_DOF or r == [1, 1, 1] or len(out) == 12:
-            return out
-        nx, ny, nz = nx // r[0], ny // r[1], nz // r[2]
-
-
-def mg_iteration_bytes(nx, ny, nz):
-    """Algorithmic HBM bytes of one FP32 iteration of the multigrid-preconditioned
-    refined solve (FP32 words, n DOFs and m elements on level 0; a stencil
-    level has 243 words per node), the kernels as the library runs them:
-      PCG: direction + product (z, p_old, s_e in; p, q out), update (x, r, p,
-           q, dinv in; x, r out);
-      level 0: first smoothing step (b, dinv in; d, x0 out), three fused
-           product + Chebyshev steps (u, s_e, b, d, dinv in; d, x out), one
-           product (u, s_e in; y out), restriction (b, y, mask in; coarse b, d,
-           x0 out), prolongation (x, mask, coarse x in; x out);
-      stencil level (n DOFs, N nodes): three Chebyshev steps (stencil, x, b, d,
-           dinv in; d, x out), one product (stencil, x in; y out), restriction
-           and prolongation as above;
-      coarsest: the dense inverse and its in / out vectors.
-    Returns (bytes per iteration, bytes per FP64 residual replacement, levels);
-    a replacement folds x into u (u, x in; u, x out) and forms r = f - K u in
-    FP64 (u, f, dinv, s_e in; r32 out)."""
-    levels = mg_levels(nx, ny, nz)
-    m = nx * ny * nz
-    n0 = levels[0][0]
-    w = 4
-    pcg = w * (4 * n0 + m) + w * 7 * n0
-    cyc = 0
-    for li, (n, N) in enumerate(levels):
-        if li == len(levels) - 1:
-            cyc += w * (n * n + 2 * n)
-            break
-        nc = levels[li + 1][0]
-        if li == 0:
-            cyc += w * 4 * n + 3 * w * (7 * n + m) + w * (2 * n + m)
-        else:
-            cyc += 3 * (w * 243 * N + w * 6 * n) + (w * 243 * N + w * 2 * n)
-        cyc += w * (3 * n + 3 * nc) + w * (3 * n + nc)
-    rep = 8 * 2 * n0 + 4 * 2 * n0 + 8 * 3 * n0 + 8 * m + 4 * n0
-    return pcg + cyc, rep, levels
-
-
-def cheb_st_bytes(n1, N1):
-    """One Chebyshev step on a stencil level: the stencil (243 words per node),
-    x, b, d, dinv in; d, x out."""
-    return 4 * 243 * N1 + 4 * 6 * n1
 
 
 # ---------------------------------------------------------------------------
@@ -195,45 +135,31 @@ def _cpu_name():
     return "unknown"
 
 
-def mg_kernel_times(op, reps):
-    """Live HIP-event timing of the multigrid kernels (h8_mg_time_kernels)."""
-    from h8pcg import _lib
-    ws, mws = op.workspace(32), op.mg_workspace()
-    avg = (C.c_double * 3)()
-    info = (C.c_int64 * 3)()
-    _lib.check(_lib.lib().h8_mg_time_kernels(C.byref(op.grid32), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
-                                             _lib.ptr(op.s_e32), _lib.ptr(op.K0_32), _lib.ptr(op.dinv),
-                                             _lib.ptr(ws), ws.numel(), _lib.ptr(mws), mws.numel(), reps, avg, info,
-                                             _lib.stream_handle(op.device)), "h8_mg_time_kernels")
-    return list(avg), list(info)
-
-
 def roofline_block(op, nx, ny, nz, t_iter_s, reps):
-    """roofline for the dominant kernel of the chosen form, from HIP events on
-    the stream the kernels run on, plus the same for the other kernels and the
-    whole-iteration contract."""
-    n_dof = op.n_dof
-    m = nx * ny * nz
-    (t0, t1, tv), (nlev, n1, N1) = mg_kernel_times(op, reps)
-    b0 = 4 * (2 * n_dof + m)                      # level-0 product: u, s_e in; y out
-    b1 = cheb_st_bytes(n1, N1) if n1 else 0
-    kern = [{"kernel": "k_sector<float, 16, 16, 0, float> (level-0 product y = K x, FP32)", "avg_us": 1e3 * t0,
-             "bytes": b0, "calls_per_cycle": 4}]
-    if n1:
-        kern.append({"kernel": "k_mg_cheb_st (level-1 Chebyshev step: 27-point stencil product + update)",
-                     "avg_us": 1e3 * t1, "bytes": b1, "calls_per_cycle": 3})
-    for k in kern:
-        k["achieved_GBs"] = k["bytes"] / (k["avg_us"] * 1e-6) / 1e9
-        k["frac"] = k["achieved_GBs"] * 1e9 / HBM_PEAK
-        k["share_of_cycle"] = k["calls_per_cycle"] * k["avg_us"] / (1e3 * tv)
-    dom = max(kern, key=lambda k: k["share_of_cycle"])
-    it_bytes, _, levels = mg_iteration_bytes(nx, ny, nz)
+    """roofline of the dominant kernel of the chosen form: every kernel of the
+    V-cycle timed alone with HIP events on the stream it runs on
+    (h8pcg.kbench.time_mg), its share of the measured FP32 iteration = launches
+    per cycle x average duration / iteration time; the kernel with the largest
+    share is the dominant one.  Also the whole-iteration contract."""
+    from h8pcg import kbench
+    t = kbench.time_mg(op, reps)
+    it_us = 1e6 * t_iter_s
+    kern = []
+    for slot, k in sorted(t["kernels"].items()):
+        if not k["calls"] or not k["avg_us"]:
+            continue
+        kern.append({"kernel": k["kernel"], "avg_us": k["avg_us"], "bytes": k["bytes"], "calls_per_cycle": k["calls"],
+                     "achieved_GBs": k["GBps"], "frac": k["GBps"] * 1e9 / HBM_PEAK,
+                     "share_of_iteration": k["calls"] * k["avg_us"] / it_us})
+    dom = max(kern, key=lambda k: k["share_of_iteration"])
+    it_bytes, _, levels = kbench.mg_iteration_bytes(nx, ny, nz)
     return {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": dom["frac"], "traffic": None, "kernel": dom["kernel"], "kernel_avg_us": dom["avg_us"],
-            "bytes_per_launch": dom["bytes"], "timing": f"HIP events around {reps} graph-launched launches",
+            "bytes_per_launch": dom["bytes"], "share_of_iteration": dom["share_of_iteration"],
+            "timing": f"HIP events around {reps} graph-launched launches of each kernel",
             "other_kernels": [k for k in kern if k is not dom],
-            "vcycle_us": 1e3 * tv, "levels": nlev, "level_dofs": [lv[0] for lv in levels],
-            "iteration_bytes": it_bytes, "iteration_us": 1e6 * t_iter_s,
+            "vcycle_us": t["vcycle_us"], "levels": t["levels"], "level_dofs": [lv[0] for lv in levels],
+            "iteration_bytes": it_bytes, "iteration_us": it_us,
             "iteration_frac": it_bytes / t_iter_s / HBM_PEAK}
 
 
@@ -308,7 +234,8 @@ def target_block(dims, seed, fd_n, reps):
            "rel_err_vs_gpu_fp64_tol1e-12": err, "fp64_rel_err_vs_gpu_fp64_tol1e-12": err64,
            "compliance": C_mixed, "forward_stored_MB": 8 * n_dof / 1e6,
            "roofline": kernels}
-    it_b, rep_b, _ = mg_iteration_bytes(nx, ny, nz)
+    from h8pcg import kbench
+    it_b, rep_b, _ = kbench.mg_iteration_bytes(nx, ny, nz)
     kernels["solve_bytes"] = st_mixed["inner_iterations"] * it_b + st_mixed["outer"] * rep_b
     kernels["solve_frac"] = kernels["solve_bytes"] / t_fwd / HBM_PEAK
     if fd_n > 0:
@@ -399,11 +326,12 @@ def main():
     dtj = time.perf_counter() - t3
     stj = dict(fs.last_solve_stats)
 
+    from h8pcg import kbench
     iters = s_["inner_iterations"]
     roof = roofline_block(op, nx, ny, nz, t_solve / max(1, iters), 50)
     b64 = k64 * (13 * n_free * 8 + 8 * m)
     roof["fp64_iteration_contract_frac"] = b64 / dt64 / HBM_PEAK
-    roof["solve_bytes"] = iters * roof["iteration_bytes"] + s_["outer"] * mg_iteration_bytes(nx, ny, nz)[1]
+    roof["solve_bytes"] = iters * roof["iteration_bytes"] + s_["outer"] * kbench.mg_iteration_bytes(nx, ny, nz)[1]
     roof["solve_frac"] = roof["solve_bytes"] / t_solve / HBM_PEAK
 
     cpu = torch_cpu = None

@@ -238,7 +238,8 @@ static int launch_op(const GridDev& d0, NodeArgs<T> A, hipStream_t s) {
     // the resident solver's products outside its loop use the streaming default
     const int v = (d.variant == H8_VARIANT_AUTO || d.variant == H8_VARIANT_RESIDENT) ? default_variant<T>(d)
                                                                                       : d.variant;
-    if constexpr (MODE == MODE_DIRX || MODE == MODE_DIRZ || MODE == MODE_CHEB) {   // sector kernels only
+    if constexpr (MODE == MODE_DIRX || MODE == MODE_DIRZ || MODE == MODE_CHEB || MODE == MODE_CHEB0 ||
+                  MODE == MODE_MGRES) {   // sector kernels only
         if (v == H8_VARIANT_SECTOR) return launch_sector<T, MODE, 16, 16>(d0, A, s);
         if (v == H8_VARIANT_SECTOR_NARROW) return launch_sector<T, MODE, 8, 16>(d0, A, s);
         return fail(H8_ERR_ARG, "direction mode %d needs a sector kernel (variant %d)", MODE, v);
@@ -1663,12 +1664,37 @@ extern "C" size_t h8_mg_workspace_bytes(const h8_grid* g) {
     return mg_layout(P, nullptr);
 }
 
+// First level of the cycle that runs inside k_mg_bottom: the trailing levels
+// of at most MG_BOTTOM_NODES nodes (never level 0, the matrix-free grid).
+static int mg_bottom_level(const MgPlan& P) {
+    int lb = P.nlev - 1;
+    while (lb > 1 && P.L[lb - 1].nnodes <= MG_BOTTOM_NODES) --lb;
+    return lb;
+}
+
+// Arguments of k_mg_bottom (the LDS layout of its staged form, or none
+// when the small levels do not fit).
+static MgBottom mg_bottom_args(const MgPlan& P) {
+    MgBottom B{};
+    for (int l = 0; l < P.nlev; ++l) B.L[l] = P.L[l];
+    B.lb = mg_bottom_level(P);
+    B.nl = P.nlev;
+    B.cst = P.cst;
+    B.Ainv = P.Ainv;
+    const long long w = mg_bottom_layout(B.L, B.lb, B.nl, B.lds_off);
+    B.lds_words = (w * (long long)sizeof(float) <= MG_BOTTOM_LDS_BYTES) ? int(w) : 0;
+    return B;
+}
+
 // Builds the hierarchy for the current element scales and free mask (stream
 // ordered; *P.flag tells whether the coarsest operator was singular).
 static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const double* K064, const float* s_e32,
                     hipStream_t s) {
     const int nl = P.nlev;
     const MgLevel& L0 = P.L[0];
+    if (mg_bottom_args(P).lds_words)
+        HIPCHK(hipFuncSetAttribute((const void*)k_mg_bottom<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   MG_BOTTOM_LDS_BYTES));
     hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(64), 0, s, (long long)(4 * MG_MAX_LEVELS),
                        reinterpret_cast<unsigned*>(P.cst), 0u);
     hipLaunchKernelGGL(k_mg_mc, dim3(18), dim3(256), 0, s, K064, L0.rx, L0.ry, L0.rz, P.M);
@@ -1707,12 +1733,18 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
     return H8_OK;
 }
 
-// First level of the cycle that runs inside k_mg_bottom: the trailing levels
-// of at most MG_BOTTOM_NODES nodes (never level 0, the matrix-free grid).
-static int mg_bottom_level(const MgPlan& P) {
-    int lb = P.nlev - 1;
-    while (lb > 1 && P.L[lb - 1].nnodes <= MG_BOTTOM_NODES) --lb;
-    return lb;
+// The small levels' kernel: staged in LDS when they fit.
+static int launch_bottom(const MgPlan& P, const PcgState* st, hipStream_t s) {
+    const MgBottom B = mg_bottom_args(P);
+    if (B.lds_words) {
+        // (the kernel's dynamic-LDS limit is raised by mg_setup, outside any capture)
+        const unsigned lds = unsigned(B.lds_words) * sizeof(float);
+        hipLaunchKernelGGL(k_mg_bottom<true>, dim3(1), dim3(MG_BOTTOM_THREADS), lds, s, B, st);
+    } else {
+        hipLaunchKernelGGL(k_mg_bottom<false>, dim3(1), dim3(MG_BOTTOM_THREADS), 0, s, B, st);
+    }
+    LAUNCHCHK();
+    return H8_OK;
 }
 
 // z = V(r): one V-cycle (degree-2 Chebyshev smoothing before and after the
@@ -1728,11 +1760,22 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
     const MgLevel* L = P.L;
     const MgConst* cst = P.cst;
     const MgLevel& L0 = L[0];
-    auto plain0 = [&](const float* u) {
+    auto resid0 = [&](const float* u) {          // L0.y = M (b - K u)
         NodeArgs<float> A = prod0;
         A.u = u;
         A.y = L0.y;
-        return launch_op<float, MODE_PLAIN>(d32, A, s);
+        A.cb_b = L0.b;
+        A.dinv = L0.dinv;
+        return launch_op<float, MODE_MGRES>(d32, A, s);
+    };
+    auto first0 = [&](float* out) {              // x0 = c0 dinv b, then one step: out = x0 + d
+        NodeArgs<float> A = prod0;
+        A.dinv = L0.dinv;
+        A.cheb = &cst[0].c0;
+        A.cb_b = L0.b;
+        A.cb_d = L0.d;
+        A.cb_out = out;
+        return launch_op<float, MODE_CHEB0>(d32, A, s);
     };
     auto cheb0 = [&](const float* u, float* out, int step, int rz) {
         NodeArgs<float> A = prod0;
@@ -1751,28 +1794,19 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
         if (l == 0) {
-            // x0 = d = c0 dinv b; x1 = x0 + d'; y = K x1
-            hipLaunchKernelGGL(k_mg_cheb_vec, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F.ndof, F.x0,
-                               (const float*)F.b, (const float*)nullptr, F.d, (const float*)F.dinv, cst, 0, 1, st);
-            if ((rc = cheb0(F.x0, F.x1, 1, 0))) return rc;
-            if ((rc = plain0(F.x1))) return rc;
+            // x0 = d = c0 dinv b and x1 = x0 + d' in one pass; y = M (b - K x1)
+            if ((rc = first0(F.x1))) return rc;
+            if ((rc = resid0(F.x1))) return rc;
         } else {
             hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x0, F.x1,
                                cst + l, 1, st);
-            hipLaunchKernelGGL(k_mg_spmv_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x1, F.y,
+            hipLaunchKernelGGL(k_mg_resid_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x1, F.y,
                                st);
         }
-        hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.ndof)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
-                           (const float*)F.b, (const float*)F.y, (const float*)F.dinv, cst + l + 1,
-                           (l + 1 < lb) ? 1 : 0, st);
+        hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.nnodes)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
+                           (const float*)F.y, cst + l + 1, (l + 1 < lb) ? 1 : 0, st);
     }
-    MgBottom B{};
-    for (int l = 0; l < nl; ++l) B.L[l] = L[l];
-    B.lb = lb;
-    B.nl = nl;
-    B.cst = cst;
-    B.Ainv = P.Ainv;
-    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(MG_BOTTOM_THREADS), 0, s, B, st);
+    if ((rc = launch_bottom(P, st, s))) return rc;
     for (int l = lb - 1; l >= 0; --l) {
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
@@ -2060,33 +2094,65 @@ extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const d
     }
     NodeArgs<float> prod0{};
     prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y;
-    GraphExec ex[3];
-    for (int which = 0; which < 3; ++which) {
-        if (which == 1 && P.nlev <= 2) continue;
-        rc = capture_graph([&](hipStream_t cs) -> int {
-            for (int i = 0; i < reps; ++i) {
-                int r1 = H8_OK;
-                if (which == 0) r1 = launch_op<float, MODE_PLAIN>(d32, prod0, cs);
-                else if (which == 1)
-                    hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(L1.ndof)), dim3(256), 0, cs, L1,
-                                       (const float*)L1.x0, L1.x1, (const MgConst*)(P.cst + 1), 1,
-                                       (const PcgState*)nullptr);
-                else r1 = mg_vcycle(d32, P, prod0, nullptr, cs, false);
-                if (r1) return r1;
-            }
-            LAUNCHCHK();
+    const bool has1 = P.nlev > 2;           // level 1 is a smoothed stencil level
+    const int lb = mg_bottom_level(P);
+    // one kernel (or the whole cycle) per slot, `reps` launches per graph
+    auto body = [&](int which, hipStream_t cs) -> int {
+        NodeArgs<float> A = prod0;
+        A.dinv = L0.dinv; A.cheb = &P.cst[0].c0; A.cb_b = L0.b; A.cb_d = L0.d;
+        switch (which) {
+        case 0: return launch_op<float, MODE_PLAIN>(d32, prod0, cs);
+        case 1:
+            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(L1.ndof)), dim3(256), 0, cs, L1, (const float*)L1.x0,
+                               L1.x1, (const MgConst*)(P.cst + 1), 1, (const PcgState*)nullptr);
             return H8_OK;
-        }, ex[which]);
-        if (rc) return rc;
-    }
+        case 2: return mg_vcycle(d32, P, prod0, nullptr, cs, false);
+        case 3:
+            A.u = L0.x0; A.cb_out = L0.x1; A.cheb_step = 1;
+            return launch_op<float, MODE_CHEB>(d32, A, cs);
+        case 4:
+            A.cb_out = L0.x1;
+            return launch_op<float, MODE_CHEB0>(d32, A, cs);
+        case 5:
+            A.u = L0.x1; A.y = L0.y;
+            return launch_op<float, MODE_MGRES>(d32, A, cs);
+        case 6:
+            hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(L1.nnodes)), dim3(256), 0, cs, L0, L1, L0.rx, L0.ry,
+                               L0.rz, (const float*)L0.y, (const MgConst*)(P.cst + 1), 1, (const PcgState*)nullptr);
+            return H8_OK;
+        case 7:
+            hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(L0.nnodes)), dim3(256), 0, cs, L0, L1, L0.rx, L0.ry,
+                               L0.rz, L0.x1, (const float*)L1.x1, (const float*)L0.dinv, (const PcgState*)nullptr);
+            return H8_OK;
+        case 8: return launch_bottom(P, nullptr, cs);
+        default:
+            hipLaunchKernelGGL(k_mg_resid_st, dim3(mg_blocks(L1.ndof)), dim3(256), 0, cs, L1, (const float*)L1.x1,
+                               L1.y, (const PcgState*)nullptr);
+            return H8_OK;
+        }
+    };
+    // slots that need level 1 as a smoothed level (1, 9) or a restriction from
+    // level 0 into a smoothed level (6 with pre0)
+    auto skip = [&](int which) { return (which == 1 || which == 9 || which == 6) && (!has1 || lb < 1); };
     Ev ev[2];
     for (auto& e : ev) HIPCHK(e.create());
-    for (int which = 0; which < 3; ++which) {
+    // the whole cycle first: one launch fills every level's vectors
+    const int order[H8_MG_TIMES] = {2, 0, 1, 3, 4, 5, 6, 7, 8, 9};
+    for (int o = 0; o < H8_MG_TIMES; ++o) {
+        const int which = order[o];
         avg_ms[which] = 0.0;
-        if (!ex[which].exec) continue;
-        HIPCHK(hipGraphLaunch(ex[which].exec, s));    // warm
+        if (skip(which)) continue;
+        GraphExec ex;
+        rc = capture_graph([&](hipStream_t cs) -> int {
+            for (int i = 0; i < reps; ++i)
+                if (int r1 = body(which, cs)) return r1;
+            LAUNCHCHK();
+            return H8_OK;
+        }, ex);
+        if (rc) return rc;
+        HIPCHK(hipGraphLaunch(ex.exec, s));    // warm
         HIPCHK(hipEventRecord(ev[0], s));
-        HIPCHK(hipGraphLaunch(ex[which].exec, s));
+        HIPCHK(hipGraphLaunch(ex.exec, s));
         HIPCHK(hipEventRecord(ev[1], s));
         HIPCHK(hipEventSynchronize(ev[1]));
         float ms = 0.f;
@@ -2095,8 +2161,8 @@ extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const d
     }
     if (info) {
         info[0] = P.nlev;
-        info[1] = P.nlev > 2 ? L1.ndof : 0;
-        info[2] = P.nlev > 2 ? L1.nnodes : 0;
+        info[1] = has1 ? L1.ndof : 0;
+        info[2] = has1 ? L1.nnodes : 0;
     }
     return H8_OK;
 }

@@ -61,7 +61,7 @@ struct MgLevel {
     float* x1;
     float* b;              // right-hand side (levels >= 1)
     float* d;              // Chebyshev direction
-    float* y;              // A x
+    float* y;              // masked residual M (b - A x1) after the pre-smoothing (the restriction's input)
 };
 
 // Per-level smoother constants, in device memory (written by the setup kernels).
@@ -434,23 +434,6 @@ __global__ void __launch_bounds__(256) k_mg_dense_solve(const float* __restrict_
     if (lane == 0) x[row] = acc;
 }
 
-// Chebyshev step on a vector already multiplied (y = A x; ignored, with x,
-// when zero_x): d = ca d + cb dinv (b - y), x = x + d.
-__global__ void __launch_bounds__(256) k_mg_cheb_vec(long long n, float* __restrict__ x, const float* __restrict__ b,
-                                                     const float* __restrict__ y, float* __restrict__ d,
-                                                     const float* __restrict__ dinv, const MgConst* cst, int step,
-                                                     int zero_x, const PcgState* st) {
-    if (!mg_live(st)) return;
-    const float ca = step ? cst->c1a : 0.0f, cb = step ? cst->c1b : cst->c0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const float r = zero_x ? b[i] : b[i] - y[i];
-        const float dn = (zero_x ? 0.0f : ca * d[i]) + cb * dinv[i] * r;
-        d[i] = dn;
-        x[i] = zero_x ? dn : x[i] + dn;
-    }
-}
-
 // Row (n, c) of y = A x on a stencil level.  Kernels over a level's DOFs use
 // component-major thread indices t = c * nnodes + n, so the lanes of a wave
 // read adjacent stencil words.
@@ -496,31 +479,32 @@ __global__ void __launch_bounds__(256) k_mg_cheb_st(MgLevel L, const float* __re
     }
 }
 
-// y = A x on a stencil level.
-__global__ void __launch_bounds__(256) k_mg_spmv_st(MgLevel L, const float* __restrict__ x, float* __restrict__ y,
-                                                    const PcgState* st) {
+// Masked residual r = (dinv != 0) (b - A x) on a stencil level (the input of
+// the restriction).
+__global__ void __launch_bounds__(256) k_mg_resid_st(MgLevel L, const float* __restrict__ x, float* __restrict__ r,
+                                                     const PcgState* st) {
     if (!mg_live(st)) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
          t += (long long)gridDim.x * blockDim.x) {
         const int c = int(unsigned(t) / unsigned(L.nnodes));
-        const long long n = t - c * L.nnodes;
-        y[3 * n + c] = mg_row(L, x, n, c);
+        const long long n = t - c * L.nnodes, q = 3 * n + c;
+        const float y = mg_row(L, x, n, c);
+        r[q] = L.dinv[q] != 0.0f ? L.b[q] - y : 0.0f;
     }
 }
 
-// Coarse DOF t (component-major) of the restricted masked residual
-// P^T M_F (bF - yF).
-__device__ __forceinline__ float mg_restrict_dof(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
-                                                 const float* __restrict__ bF, const float* __restrict__ yF,
-                                                 const float* __restrict__ maskF, long long t, long long& qC) {
-    const int c = int(unsigned(t) / unsigned(C.nnodes));
-    const long long In = t - c * C.nnodes;
+// Coarse node In of the restriction bC = P^T rF (rF the fine level's masked
+// residual), all three components: the 27 fine nodes around the node's fine
+// position (weights 1, 1/2, 1/4, 1/8 by the number of offset axes; a kept axis
+// has only the centre), each a 12-byte read.
+__device__ __forceinline__ void mg_restrict_node(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
+                                                 const float* __restrict__ rF, long long In, float (&acc)[3]) {
     int Ij, Ii, Ik;
     mg_decode(C, In, Ij, Ii, Ik);
     const int fj = Ij * ry, fi = Ii * rx, fk = Ik * rz;
     // all 27 offsets, branch-free: weight 0 (and the index clamped to the
     // centre) for an offset along a kept axis or outside the grid
-    float acc = 0.0f;
+    acc[0] = acc[1] = acc[2] = 0.0f;
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
         const int oj = o % 3 - 1, oi = (o / 3) % 3 - 1, ok = o / 9 - 1;
@@ -529,11 +513,10 @@ __device__ __forceinline__ float mg_restrict_dof(const MgLevel& F, const MgLevel
                         (ok == 0 || (rz == 2 && unsigned(fk + ok) < unsigned(F.NK)));
         const float w = in ? (oj ? 0.5f : 1.0f) * (oi ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f) : 0.0f;
         const long long f = 3 * (in ? ((long long)(fk + ok) * F.NI + (fi + oi)) * F.NJ + (fj + oj)
-                                    : ((long long)fk * F.NI + fi) * F.NJ + fj) + c;
-        acc += (maskF[f] != 0.0f) ? w * (bF[f] - yF[f]) : 0.0f;
+                                    : ((long long)fk * F.NI + fi) * F.NJ + fj);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += w * rF[f + c];
     }
-    qC = 3 * In + c;
-    return acc;
 }
 
 // Fine node fn of the coarse correction xF += M_F P xC.
@@ -563,24 +546,27 @@ __device__ __forceinline__ void mg_prolong_node(const MgLevel& F, const MgLevel&
         if (maskF[3 * fn + c] != 0.0f) xF[3 * fn + c] += acc[c];
 }
 
-// Restriction bC = P^T M_F (bF - yF), one thread per coarse DOF; with pre0
-// the coarse level's first smoothing step from zero follows at once in the
-// same thread: x0 = d = c0 dinv b (cstC = the coarse level's constants).
+// Restriction bC = P^T rF, one thread per coarse node; with pre0 the coarse
+// level's first smoothing step from zero follows at once in the same thread:
+// x0 = d = c0 dinv b (cstC = the coarse level's constants).
 __global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C, int rx, int ry, int rz,
-                                                     const float* __restrict__ bF, const float* __restrict__ yF,
-                                                     const float* __restrict__ maskF, const MgConst* cstC, int pre0,
+                                                     const float* __restrict__ rF, const MgConst* cstC, int pre0,
                                                      const PcgState* st) {
     if (!mg_live(st)) return;
     const float c0 = pre0 ? cstC->c0 : 0.0f;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < C.ndof;
-         t += (long long)gridDim.x * blockDim.x) {
-        long long q;
-        const float b = mg_restrict_dof(F, C, rx, ry, rz, bF, yF, maskF, t, q);
-        C.b[q] = b;
-        if (pre0) {
-            const float d = c0 * C.dinv[q] * b;
-            C.d[q] = d;
-            C.x0[q] = d;
+    for (long long In = (long long)blockIdx.x * blockDim.x + threadIdx.x; In < C.nnodes;
+         In += (long long)gridDim.x * blockDim.x) {
+        float b[3];
+        mg_restrict_node(F, C, rx, ry, rz, rF, In, b);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long q = 3 * In + c;
+            C.b[q] = b[c];
+            if (pre0) {
+                const float d = c0 * C.dinv[q] * b[c];
+                C.d[q] = d;
+                C.x0[q] = d;
+            }
         }
     }
 }
@@ -597,15 +583,87 @@ __global__ void __launch_bounds__(256) k_mg_prolong(MgLevel F, MgLevel C, int rx
 
 // The small levels of a cycle in one workgroup: levels lb .. nl-1 (the last
 // the coarsest, inverted densely), given b of level lb; leaves that level's
-// result in its x1.  Phases are separated by workgroup barriers (all waves of
-// a workgroup share one CU and its L1, so their global stores are visible to
-// each other after the barrier).
+// result in its x1.  Phases are separated by workgroup barriers.  Staged form
+// (lds_words > 0): the levels' stencils, inverse diagonals and vectors and the
+// dense inverse are copied into LDS once per launch (all loads of a thread in
+// flight together), every phase then runs on LDS, and only level lb's x1 is
+// written back.  Unstaged form: the phases work on the levels' global vectors
+// (all waves of a workgroup share one CU and its L1, so their global stores
+// are visible to each other after the barrier).
+constexpr int MG_BOTTOM_LDS_BYTES = 152 * 1024;   // LDS budget of the staged form
+
 struct MgBottom {
     MgLevel L[MG_MAX_LEVELS];
     int lb, nl;
     const MgConst* cst;
     const float* Ainv;
+    // staged form: word offset of each level's block in LDS (a level above
+    // the coarsest: S, dinv, b, x0, x1, d, y; the coarsest: b, x1, Ainv)
+    int lds_off[MG_MAX_LEVELS];
+    int lds_words;
 };
+
+// LDS words of the staged form of levels lb .. nl-1 (offsets into off); every
+// array starts at a multiple of 4 words (16-byte copies).
+__host__ __device__ constexpr long long mg_pad4(long long w) { return (w + 3) & ~3LL; }
+__host__ __device__ inline long long mg_bottom_layout(const MgLevel* L, int lb, int nl, int* off) {
+    long long w = 0;
+    for (int l = lb; l < nl; ++l) {
+        if (off) off[l] = int(w);
+        const long long n = mg_pad4(L[l].ndof);
+        w += (l + 1 < nl) ? mg_pad4(243 * L[l].nnodes) + 6 * n : mg_pad4(L[l].ndof * L[l].ndof) + 2 * n;
+    }
+    return w;
+}
+
+template <bool STAGED>
+__device__ __forceinline__ MgLevel mg_bottom_view(const MgBottom& B, float* smem, int l) {
+    MgLevel L = B.L[l];
+    if constexpr (STAGED) {
+        float* p = smem + B.lds_off[l];
+        const long long n = mg_pad4(L.ndof);
+        if (l + 1 < B.nl) {
+            L.S = p;
+            p += mg_pad4(243 * L.nnodes);
+            L.dinv = p;
+            L.b = p + n;
+            L.x0 = p + 2 * n;
+            L.x1 = p + 3 * n;
+            L.d = p + 4 * n;
+            L.y = p + 5 * n;
+        } else {
+            // (every pointer of a staged view points into LDS, so the compiler
+            // addresses all of them as LDS; the coarsest uses b and x1 only)
+            L.S = L.dinv = L.x0 = L.d = L.y = p;
+            L.b = p;
+            L.x1 = p + n;      // the dense inverse follows at p + 2 n
+        }
+    }
+    return L;
+}
+
+// dst[0..cnt) = src[0..cnt) (both 16-byte aligned): 16-byte loads, 8 of them
+// in flight per thread, then the tail words
+__device__ __forceinline__ void mg_stage(float* dst, const float* __restrict__ src, long long cnt) {
+    constexpr int U = 8;
+    const long long n4 = cnt >> 2;
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    for (long long i0 = threadIdx.x; i0 < n4; i0 += (long long)U * blockDim.x) {
+        float4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + (long long)u * blockDim.x;
+            v[u] = i < n4 ? s4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + (long long)u * blockDim.x;
+            if (i < n4) d4[i] = v[u];
+        }
+    }
+    for (long long i = 4 * n4 + threadIdx.x; i < cnt; i += blockDim.x) dst[i] = src[i];
+}
 
 __device__ __forceinline__ void mg_bottom_cheb(const MgLevel& L, const MgConst& k, const float* xin, float* xout,
                                                int step) {
@@ -619,12 +677,32 @@ __device__ __forceinline__ void mg_bottom_cheb(const MgLevel& L, const MgConst& 
     }
 }
 
+template <bool STAGED>
 __global__ void __launch_bounds__(MG_BOTTOM_THREADS) k_mg_bottom(MgBottom B, const PcgState* st) {
+    extern __shared__ float mg_smem[];
+    float* smem = mg_smem;
     if (!mg_live(st)) return;
+    const float* Ainv = B.Ainv;
+    if constexpr (STAGED) {
+        for (int l = B.lb; l + 1 < B.nl; ++l) {
+            const MgLevel V = mg_bottom_view<true>(B, smem, l);
+            mg_stage(V.S, B.L[l].S, 243 * B.L[l].nnodes);
+            mg_stage(V.dinv, B.L[l].dinv, V.ndof);
+        }
+        {
+            const MgLevel V = mg_bottom_view<true>(B, smem, B.lb);
+            mg_stage(V.b, B.L[B.lb].b, V.ndof);
+            const MgLevel Vc = mg_bottom_view<true>(B, smem, B.nl - 1);
+            float* sA = Vc.x1 + mg_pad4(Vc.ndof);
+            mg_stage(sA, B.Ainv, Vc.ndof * Vc.ndof);
+            Ainv = sA;
+        }
+        __syncthreads();
+    }
     for (int l = B.lb; l + 1 < B.nl; ++l) {
-        const MgLevel& F = B.L[l];
-        const MgLevel& C = B.L[l + 1];
-        const MgConst& k = B.cst[l];
+        const MgLevel F = mg_bottom_view<STAGED>(B, smem, l);
+        const MgLevel C = mg_bottom_view<STAGED>(B, smem, l + 1);
+        const MgConst k = B.cst[l];
         for (long long q = threadIdx.x; q < F.ndof; q += blockDim.x) {
             const float d = k.c0 * F.dinv[q] * F.b[q];
             F.d[q] = d;
@@ -635,34 +713,47 @@ __global__ void __launch_bounds__(MG_BOTTOM_THREADS) k_mg_bottom(MgBottom B, con
         __syncthreads();
         for (long long t = threadIdx.x; t < F.ndof; t += blockDim.x) {
             const int c = int(unsigned(t) / unsigned(F.nnodes));
-            const long long n = t - c * F.nnodes;
-            F.y[3 * n + c] = mg_row(F, F.x1, n, c);
+            const long long n = t - c * F.nnodes, q = 3 * n + c;
+            const float y = mg_row(F, F.x1, n, c);
+            F.y[q] = F.dinv[q] != 0.0f ? F.b[q] - y : 0.0f;
         }
         __syncthreads();
-        for (long long t = threadIdx.x; t < C.ndof; t += blockDim.x) {
-            long long q;
-            const float b = mg_restrict_dof(F, C, F.rx, F.ry, F.rz, F.b, F.y, F.dinv, t, q);
-            C.b[q] = b;
+        for (long long In = threadIdx.x; In < C.nnodes; In += blockDim.x) {
+            float b[3];
+            mg_restrict_node(F, C, F.rx, F.ry, F.rz, F.y, In, b);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) C.b[3 * In + c] = b[c];
         }
         __syncthreads();
     }
     {
-        // coarsest: x1 = Ainv b, one wave per row
-        const MgLevel& Lc = B.L[B.nl - 1];
-        const int n = int(Lc.ndof), lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        for (int row = wv; row < n; row += nw) {
-            float acc = 0.0f;
-            for (int j = lane; j < n; j += 64) acc += B.Ainv[(long long)row * n + j] * Lc.b[j];
+        // coarsest: x1 = Ainv b (Ainv symmetric), a fixed-order sum per row
+        const MgLevel Lc = mg_bottom_view<STAGED>(B, smem, B.nl - 1);
+        const int n = int(Lc.ndof);
+        if constexpr (STAGED) {
+            // thread per row, column j of Ainv read as row j (adjacent threads, adjacent words)
+            for (int row = threadIdx.x; row < n; row += blockDim.x) {
+                float acc = 0.0f;
+                for (int j = 0; j < n; ++j) acc += Ainv[j * n + row] * Lc.b[j];
+                Lc.x1[row] = acc;
+            }
+        } else {
+            // one wave per row, lanes over the columns
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+            for (int row = wv; row < n; row += nw) {
+                float acc = 0.0f;
+                for (int j = lane; j < n; j += 64) acc += Ainv[(long long)row * n + j] * Lc.b[j];
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-            if (lane == 0) Lc.x1[row] = acc;
+                for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+                if (lane == 0) Lc.x1[row] = acc;
+            }
         }
         __syncthreads();
     }
     for (int l = B.nl - 2; l >= B.lb; --l) {
-        const MgLevel& F = B.L[l];
-        const MgLevel& C = B.L[l + 1];
-        const MgConst& k = B.cst[l];
+        const MgLevel F = mg_bottom_view<STAGED>(B, smem, l);
+        const MgLevel C = mg_bottom_view<STAGED>(B, smem, l + 1);
+        const MgConst k = B.cst[l];
         for (long long fn = threadIdx.x; fn < F.nnodes; fn += blockDim.x)
             mg_prolong_node(F, C, F.rx, F.ry, F.rz, F.x1, C.x1, F.dinv, fn);
         __syncthreads();
@@ -670,6 +761,10 @@ __global__ void __launch_bounds__(MG_BOTTOM_THREADS) k_mg_bottom(MgBottom B, con
         __syncthreads();
         mg_bottom_cheb(F, k, F.x0, F.x1, 1);
         __syncthreads();
+    }
+    if constexpr (STAGED) {
+        const MgLevel V = mg_bottom_view<true>(B, smem, B.lb);
+        for (long long q = threadIdx.x; q < V.ndof; q += blockDim.x) B.L[B.lb].x1[q] = V.x1[q];
     }
 }
 

@@ -28,7 +28,8 @@
 
 namespace h8 {
 
-enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3, MODE_DIRZ = 4, MODE_CHEB = 5 };
+enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3, MODE_DIRZ = 4, MODE_CHEB = 5, MODE_MGRES = 6,
+       MODE_CHEB0 = 7 };
 // MODE_DIRX = MODE_DIR that also applies the previous iteration's deferred
 // x += alpha p_old to the owned nodes while staging p_old.
 // MODE_DIRZ = MODE_DIR with a given preconditioned residual: p = z + beta p_old,
@@ -36,6 +37,11 @@ enum { MODE_PLAIN = 0, MODE_DIR = 1, MODE_RESID = 2, MODE_DIRX = 3, MODE_DIRZ = 
 // MODE_CHEB = a Chebyshev smoothing step of the multigrid cycle on the grid:
 // d = ca d + cb dinv (b - K x), x_out = x + d (x = NodeArgs::u, sector kernels
 // only); with cheb_rz also the reduction r.z of x_out with b and the beta step.
+// MODE_CHEB0 = the cycle's first two Chebyshev steps from x = 0 in one pass:
+// x0 = d = c0 dinv b is formed while staging (NodeArgs::cb_b, dinv), then
+// d = c1a d + c1b dinv (b - K x0), x_out = x0 + d.
+// MODE_MGRES = the masked residual of the cycle, y = (dinv != 0) (b - K u),
+// the input of the restriction (sector kernels only).
 __host__ __device__ constexpr bool is_dir(int mode) { return mode == MODE_DIR || mode == MODE_DIRX || mode == MODE_DIRZ; }
 
 template <typename T>
@@ -332,6 +338,21 @@ __device__ __forceinline__ void node_output(const NodeArgs<T>& A, int gn, const 
             Buf<T>::st(qv[c], buf_rsrc(A.q, vbytes), int(sizeof(T)) * (gn + c));
             part[0] += double(ctr[c]) * double(qv[c]);
         }
+    } else if constexpr (MODE == MODE_MGRES) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int q = gn + c;
+            Buf<T>::st(A.dinv[q] != T(0) ? A.cb_b[q] - qv[c] : T(0), buf_rsrc(A.y, vbytes), int(sizeof(T)) * q);
+        }
+    } else if constexpr (MODE == MODE_CHEB0) {
+        const T ca = T(A.cheb[1]), cb = T(A.cheb[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int q = gn + c;
+            const T dn = ca * ctr[c] + cb * A.dinv[q] * (A.cb_b[q] - qv[c]);
+            A.cb_d[q] = dn;
+            A.cb_out[q] = ctr[c] + dn;
+        }
     } else if constexpr (MODE == MODE_CHEB) {
         const T ca = A.cheb_step ? T(A.cheb[1]) : T(0), cb = A.cheb_step ? T(A.cheb[2]) : T(A.cheb[0]);
 #pragma unroll
@@ -470,11 +491,12 @@ __device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& f
     if constexpr (MODE == MODE_RESID) {
         if (A.only_on_replace && A.st->status != ST_REPLACE) return false;
     }
-    if constexpr (MODE == MODE_PLAIN || MODE == MODE_CHEB) {
+    if constexpr (MODE == MODE_PLAIN || MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_MGRES) {
         // a product inside a solve's captured iterations (the multigrid cycle)
         // passes the solve state and does nothing once the solve has stopped
         if (A.st && A.st->status != ST_RUNNING) return false;
     }
+    if constexpr (MODE == MODE_CHEB0) beta = T(A.cheb[0]);     // the staging scale c0
     if constexpr (is_dir(MODE)) {
         if (A.st->status != ST_RUNNING) return false;
         beta = T(A.st->beta);

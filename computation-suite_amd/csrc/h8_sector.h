@@ -159,7 +159,10 @@ struct SLayer {
 #pragma unroll
         for (int u = 0; u < PASS; ++u) {
             const unsigned off = (unsigned(ofs[u]) + kofs) | kbad;
-            if constexpr (MODE == MODE_DIRZ) {
+            if constexpr (MODE == MODE_CHEB0) {
+                v0[u] = Buf<T>::ld(buf_rsrc(A.cb_b, vbytes), int(off));   // b
+                v1[u] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), int(off));
+            } else if constexpr (MODE == MODE_DIRZ) {
                 v0[u] = Buf<T>::ld(buf_rsrc(A.r, vbytes), int(off));      // z
                 v1[u] = T(1);
                 v2[u] = Buf<T>::ld(buf_rsrc(p_old, vbytes), int(off | pbad));
@@ -194,6 +197,8 @@ struct SLayer {
                 // the previous iteration's x += alpha p_old, for the owned words
                 if constexpr (MODE == MODE_DIRX)
                     Buf<T>::st(xv[u] + alpha_prev * v2[u], buf_rsrc(A.x, vbytes), int(so | xbad));
+            } else if constexpr (MODE == MODE_CHEB0) {
+                val = beta * v1[u] * v0[u];        // x0 = c0 dinv b (beta carries c0)
             } else {
                 val = v0[u];
             }

@@ -23,6 +23,7 @@ H8_MAXITER = 4
 H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
 H8_MG_DECLINED = 7
+MG_TIMES = 10             # H8_MG_TIMES: slots of h8_mg_time_kernels
 SCRATCH_BYTES = 32 << 20   # H8_SCRATCH_BYTES: the static workspace of h8_pcg_f64_noalloc
 
 (VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR,

@@ -27,7 +27,8 @@ TIE = 0.02
 
 # (variant, layers), in order of preference on near-ties: node-centric brick,
 # the sector kernel with the library's own depth, persistent node-centric, the
-# sector kernel at other marching depths, and the narrow-column sector kernel
+# sector kernel at other marching depths, the narrow-column sector kernel, and
+# the element-batched MFMA kernel (the 24x24 contraction on the matrix cores)
 CANDIDATES = [
     (_lib.VARIANT_NODE, 0),
     (_lib.VARIANT_SECTOR, 0),
@@ -38,6 +39,7 @@ CANDIDATES = [
     (_lib.VARIANT_SECTOR_NARROW, 0),
     (_lib.VARIANT_SECTOR_NARROW, 4),
     (_lib.VARIANT_SECTOR_NARROW, 8),
+    (_lib.VARIANT_MFMA, 0),
 ]
 
 

@@ -14,7 +14,8 @@ writes exactly that many bytes -- so hardware byte counters (FETCH_SIZE,
 WRITE_SIZE) can be calibrated against a kernel of known traffic.  Prints one JSON line.  Run it under
 ``rocprofv3 --kernel-trace --stats`` or ``rocprofv3 --pmc ...``.  With --mg it
 times the multigrid kernels of the FP32 iterations instead (h8_mg_time_kernels:
-the level-0 product, one level-1 Chebyshev step, one whole V-cycle).
+each kernel of the V-cycle alone, and one whole V-cycle, with its algorithmic
+bytes).
 """
 from __future__ import annotations
 
@@ -109,24 +110,148 @@ def time_recovery(nx, ny, nz, reps, device="cuda:0"):
     return out
 
 
+# ---------------------------------------------------------------------------
+# algorithmic bytes of the multigrid-preconditioned FP32 iteration (FP32 words;
+# n DOFs and m elements on level 0, a stencil level holds 243 words per node)
+# ---------------------------------------------------------------------------
+MG_COARSE_DOF = 128       # the library halves even axes until a level has at most this many DOFs
+MG_BOTTOM_NODES = 256     # levels after the first with at most this many nodes run in one workgroup
+W = 4
+
+
+def mg_levels(nx, ny, nz):
+    """[(ndof, nnodes)] of the library's hierarchy."""
+    out = []
+    while True:
+        nn = (nx + 1) * (ny + 1) * (nz + 1)
+        out.append((3 * nn, nn))
+        r = [2 if v % 2 == 0 else 1 for v in (nx, ny, nz)]
+        if 3 * nn <= MG_COARSE_DOF or r == [1, 1, 1] or len(out) == 12:
+            return out
+        nx, ny, nz = nx // r[0], ny // r[1], nz // r[2]
+
+
+def mg_bottom_level(levels):
+    lb = len(levels) - 1
+    while lb > 1 and levels[lb - 1][1] <= MG_BOTTOM_NODES:
+        lb -= 1
+    return lb
+
+
+def _restrict(nF, nC, pre0):
+    return W * (nF + nC * (1 + 3 * pre0))        # r in (+ coarse dinv); coarse b (+ d, x0) out
+
+
+def _prolong(nF, nC):
+    return W * (3 * nF + nC)                      # coarse x, mask, x in; x out
+
+
+def _st(n, N, reads, writes):
+    return W * (243 * N + (reads + writes) * n)
+
+
+def mg_cycle_kernels(nx, ny, nz):
+    """The kernels of one V-cycle with their algorithmic bytes per launch and
+    launches per cycle, keyed by the slots of h8_mg_time_kernels:
+      3 level-0 Chebyshev step (u, s_e, b, d, dinv in; d, x out),
+      4 level-0 first step pair from zero (b, dinv, s_e in; d, x out),
+      5 level-0 masked residual (u, s_e, b, dinv in; r out),
+      6 restriction 0 -> 1 (+ level 1's first step: r, coarse dinv in; b, d, x0 out),
+      7 prolongation 1 -> 0,
+      1 level-1 Chebyshev step (stencil, x, b, d, dinv in; d, x out),
+      9 level-1 masked residual (stencil, x, b, dinv in; r out),
+      8 the single-workgroup kernel of the small levels."""
+    levels = mg_levels(nx, ny, nz)
+    m = nx * ny * nz
+    n0 = levels[0][0]
+    lb = mg_bottom_level(levels)
+    out = {
+        3: {"kernel": "k_sector<float,16,16,MODE_CHEB> (level-0 Chebyshev step)", "bytes": W * (6 * n0 + m),
+            "calls": 2},
+        4: {"kernel": "k_sector<float,16,16,MODE_CHEB0> (level-0 first step pair)", "bytes": W * (4 * n0 + m),
+            "calls": 1},
+        5: {"kernel": "k_sector<float,16,16,MODE_MGRES> (level-0 masked residual)", "bytes": W * (4 * n0 + m),
+            "calls": 1},
+    }
+    if len(levels) > 1:
+        n1, N1 = levels[1]
+        out[6] = {"kernel": "k_mg_restrict (0 -> 1)", "bytes": _restrict(n0, n1, 1 if lb > 1 else 0), "calls": 1}
+        out[7] = {"kernel": "k_mg_prolong (1 -> 0)", "bytes": _prolong(n0, n1), "calls": 1}
+        if lb > 1:
+            out[1] = {"kernel": "k_mg_cheb_st (level-1 Chebyshev step)", "bytes": _st(n1, N1, 4, 2), "calls": 3}
+            out[9] = {"kernel": "k_mg_resid_st (level-1 masked residual)", "bytes": _st(n1, N1, 3, 1), "calls": 1}
+        out[8] = {"kernel": "k_mg_bottom (levels %d..%d in one workgroup)" % (lb, len(levels) - 1),
+                  "bytes": _bottom_bytes(levels, lb), "calls": 1}
+    return out
+
+
+def _bottom_bytes(levels, lb):
+    b = 0
+    for li in range(lb, len(levels)):
+        n, N = levels[li]
+        if li == len(levels) - 1:
+            b += W * (n * n + 2 * n)
+            break
+        nc = levels[li + 1][0]
+        b += W * 4 * n + _st(n, N, 4, 2) + _st(n, N, 3, 1) + _restrict(n, nc, 0)
+        b += _prolong(n, nc) + _st(n, N, 3, 2) + _st(n, N, 4, 2)
+    return b
+
+
+def mg_iteration_bytes(nx, ny, nz):
+    """Algorithmic HBM bytes of one FP32 iteration of the multigrid-preconditioned
+    refined solve, the kernels as the library runs them:
+      PCG: direction + product (z, p_old, s_e in; p, q out), update (x, r, p,
+           q, dinv in; x, r out);
+      level 0: first step pair, masked residual, restriction, prolongation,
+           two Chebyshev steps (mg_cycle_kernels);
+      stencil level l (1 <= l < the bottom level): first step in the
+           restriction into it, one Chebyshev step and the masked residual,
+           restriction, prolongation, two Chebyshev steps;
+      the small levels as the single-workgroup kernel reads them.
+    Returns (bytes per iteration, bytes per FP64 residual replacement, levels);
+    a replacement folds x into u (u, x in; u, x out) and forms r = f - K u in
+    FP64 (u, f, dinv, s_e in; r32 out)."""
+    levels = mg_levels(nx, ny, nz)
+    m = nx * ny * nz
+    n0 = levels[0][0]
+    lb = mg_bottom_level(levels)
+    pcg = W * (4 * n0 + m) + W * 7 * n0
+    cyc = 0
+    if len(levels) == 1:
+        cyc = W * (n0 * n0 + 2 * n0)
+    else:
+        cyc = sum(k["bytes"] * k["calls"] for key, k in mg_cycle_kernels(nx, ny, nz).items() if key not in (1, 9))
+        for li in range(1, lb):
+            n, N = levels[li]
+            nc = levels[li + 1][0]
+            cyc += _st(n, N, 4, 2) + _st(n, N, 3, 1) + _st(n, N, 3, 2) + _st(n, N, 4, 2)
+            if li + 1 < len(levels):
+                cyc += _restrict(n, nc, 1 if li + 1 < lb else 0) + _prolong(n, nc)
+    rep = 8 * 2 * n0 + 4 * 2 * n0 + 8 * 3 * n0 + 8 * m + 4 * n0
+    return pcg + cyc, rep, levels
+
+
 def time_mg(op, reps):
-    """The multigrid kernels (h8_mg_time_kernels) with their algorithmic bytes:
-    level-0 product (u, s_e in; y out), level-1 Chebyshev step (243 stencil
-    words per node; x, b, d, dinv in; d, x out)."""
+    """The multigrid kernels (h8_mg_time_kernels, HIP events on the launching
+    stream) with their algorithmic bytes: {slot: {kernel, bytes, calls,
+    avg_us, GBps}}, plus the whole V-cycle and the level hierarchy."""
     from . import _lib
     ws, mws = op.workspace(32), op.mg_workspace()
-    avg = (C.c_double * 3)()
+    avg = (C.c_double * _lib.MG_TIMES)()
     info = (C.c_int64 * 3)()
     _lib.check(_lib.lib().h8_mg_time_kernels(C.byref(op.grid32), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
                                              _lib.ptr(op.s_e32), _lib.ptr(op.K0_32), _lib.ptr(op.dinv),
                                              _lib.ptr(ws), ws.numel(), _lib.ptr(mws), mws.numel(), int(reps), avg,
                                              info, _lib.stream_handle(op.device)), "h8_mg_time_kernels")
-    b0 = 4 * (2 * op.n_dof + op.m)
-    b1 = 4 * 243 * info[2] + 4 * 6 * info[1]
-    us = [1e3 * v for v in avg]
-    return {"levels": info[0], "level1_dof": info[1], "product_us": us[0], "product_bytes": b0,
-            "product_GBps": b0 / us[0] * 1e-3, "cheb_st_us": us[1], "cheb_st_bytes": b1,
-            "cheb_st_GBps": (b1 / us[1] * 1e-3) if us[1] else None, "vcycle_us": us[2]}
+    kern = mg_cycle_kernels(op.nx, op.ny, op.nz)
+    kern[0] = {"kernel": "k_sector<float,16,16,MODE_PLAIN> (level-0 product y = K x)",
+               "bytes": W * (2 * op.n_dof + op.m), "calls": 0}
+    for slot, k in kern.items():
+        k["avg_us"] = 1e3 * avg[slot]
+        k["GBps"] = k["bytes"] / k["avg_us"] * 1e-3 if k["avg_us"] > 0 else None
+    return {"levels": int(info[0]), "level1_dof": int(info[1]), "level1_nodes": int(info[2]),
+            "vcycle_us": 1e3 * avg[2], "kernels": kern}
 
 
 def main(argv=None):

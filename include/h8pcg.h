@@ -228,13 +228,24 @@ int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const double* K064, 
                      size_t ws_bytes, void* mg_ws, size_t mg_ws_bytes, void* stream);
 
 /* Average device time (ms) of the multigrid kernels, each graph-launched
- * `reps` times alone between HIP events on `stream`: avg_ms[0] = the level-0
- * FP32 product y = K x (sector kernel), avg_ms[1] = one Chebyshev step on
- * level 1 (stencil product fused with the update; 0 when level 1 is the
- * coarsest), avg_ms[2] = one whole V-cycle.  info[0..2] (may be NULL) =
- * levels, DOFs and nodes of level 1.  ws: h8_pcg_workspace_bytes(g, 32);
- * mg_ws: h8_mg_workspace_bytes(g).  Measurement helper with no counterpart in
- * the reference; scribbles on both workspaces. */
+ * `reps` times alone between HIP events on `stream` (after one whole cycle
+ * has filled every level's vectors); avg_ms has H8_MG_TIMES slots:
+ *   0 the level-0 FP32 product y = K x (sector kernel, plain mode),
+ *   1 one Chebyshev step on level 1 (stencil product fused with the update),
+ *   2 one whole V-cycle,
+ *   3 one level-0 Chebyshev step (sector kernel),
+ *   4 the level-0 first step pair from zero (sector kernel),
+ *   5 the level-0 masked residual (sector kernel),
+ *   6 the restriction 0 -> 1 with level 1's first step,
+ *   7 the prolongation 1 -> 0,
+ *   8 the single-workgroup kernel of the small levels,
+ *   9 the level-1 masked residual (stencil product);
+ * a slot that does not apply to the grid's hierarchy is 0.  info[0..2] (may
+ * be NULL) = levels, DOFs and nodes of level 1.  ws:
+ * h8_pcg_workspace_bytes(g, 32); mg_ws: h8_mg_workspace_bytes(g).
+ * Measurement helper with no counterpart in the reference; scribbles on both
+ * workspaces. */
+#define H8_MG_TIMES 10
 int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
                        const float* K032, const double* dinv64, void* ws, size_t ws_bytes, void* mg_ws,
                        size_t mg_ws_bytes, int reps, double* avg_ms, int64_t* info, void* stream);
