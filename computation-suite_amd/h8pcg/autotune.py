@@ -141,3 +141,32 @@ def tune(op, precision, cache=True, candidates=None, reps=30, resident=True):
         data[key] = list(best)
         _store(path, data)
     return best
+
+
+def main(argv=None):
+    """python -m h8pcg.autotune --table --dims 256,64,64: every candidate timed
+    (direction and update kernel, both precisions) on kbench's clamped box
+    operator, one JSON line each."""
+    import argparse
+    from . import kbench
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--table", action="store_true", help="time every candidate (no cache)")
+    ap.add_argument("--dims", default="256,64,64")
+    ap.add_argument("--reps", type=int, default=30)
+    a = ap.parse_args(argv)
+    dims = tuple(int(v) for v in a.dims.split(","))
+    op = kbench.build_operator(*dims)
+    if not a.table:
+        for prec in (32, 64):
+            print(json.dumps({"dims": dims, "precision": prec, "choice": list(tune(op, prec))}), flush=True)
+        return
+    for prec in (32, 64):
+        for v, l in CANDIDATES:
+            time_iteration(op, prec, v, l, reps=3)
+            d_us, u_us = time_iteration(op, prec, v, l, reps=a.reps)
+            print(json.dumps({"dims": dims, "precision": prec, "variant": v, "layers": l, "dir_us": d_us,
+                              "upd_us": u_us}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,10 +1,14 @@
 """Summaries of rocprofv3 CSV output as short markdown (for the profiles folder).
 
-    python -m h8pcg.profsum stats <dir with *_kernel_stats.csv> [--top 12]
-    python -m h8pcg.profsum traffic --fetch <dir> --write <dir> [--calib-bytes 268435456]
+    python -m h8pcg.profsum stats <dir with *_kernel_stats.csv> [--top 12] [--by-grid]
+    python -m h8pcg.profsum traffic --fetch <dir> --write <dir> [--calib-bytes 268435456] [--by-grid]
     python -m h8pcg.profsum launches <dir with *_kernel_trace.csv> --kernel NAME [--last 2]
+    python -m h8pcg.profsum runs <dir with *_kernel_trace.csv> --kernel NAME [--grid G] [--min-run 20]
 
 ``stats`` prints the per-kernel table of a ``--kernel-trace --stats`` run.
+With ``--by-grid`` both ``stats`` and ``traffic`` keep one row per (kernel,
+grid size) -- the kernels of a multigrid cycle run once per level, and the grid
+size tells the levels apart.
 ``traffic`` reads two ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs of
 ``python -m h8pcg.kbench``, averages each counter per dispatch (KiB), and scales
 each by what the same counter reads for the calibration kernel (torch's
@@ -13,7 +17,11 @@ many bytes).  It prints one ``traffic-json:`` line per h8 kernel, in bytes per
 launch, that bench.py reads back.  ``launches`` prints the durations of the last
 --last dispatches of the kernels whose name contains NAME, in dispatch order
 (for a kernel whose launches differ in length, such as a solver that runs a
-whole solve phase per launch, the dispatches of one timing run).
+whole solve phase per launch, the dispatches of one timing run).  ``runs`` finds
+the runs of at least --min-run consecutive dispatches of one kernel (name
+containing NAME, grid size G if given) -- the back-to-back launches of a timing
+graph -- and prints each run's mean duration, for comparison with the HIP-event
+average a timing helper reports for the same launches.
 """
 from __future__ import annotations
 
@@ -37,7 +45,19 @@ def short_name(name):
     return n.replace("h8::", "")
 
 
-def stats(path, top):
+def stats(path, top, by_grid=False):
+    if by_grid:
+        f = _one(os.path.join(path, "**", "*kernel_trace.csv"))
+        acc = collections.defaultdict(list)
+        for r in csv.DictReader(open(f)):
+            acc[(r["Kernel_Name"], int(r["Grid_Size_X"]))].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        total = sum(sum(v) for v in acc.values())
+        print("| kernel | grid | calls | avg us | min us | max us | % of kernel time |")
+        print("|---|---:|---:|---:|---:|---:|---:|")
+        for (k, g), v in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:top]:
+            print(f"| `{short_name(k)[:70]}` | {g} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {min(v) / 1e3:.2f} | "
+                  f"{max(v) / 1e3:.2f} | {100.0 * sum(v) / total:.1f} |")
+        return
     f = _one(os.path.join(path, "**", "*kernel_stats.csv"))
     rows = list(csv.DictReader(open(f)))
     print("| kernel | calls | avg us | min us | max us | % of kernel time |")
@@ -47,38 +67,45 @@ def stats(path, top):
               f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
 
 
-def per_dispatch(path, counter):
+def per_dispatch(path, counter, by_grid=False):
     f = _one(os.path.join(path, "**", "*counter_collection.csv"))
     acc = collections.defaultdict(lambda: collections.defaultdict(float))
     for r in csv.DictReader(open(f)):
         if r["Counter_Name"] != counter:
             continue
-        acc[r["Kernel_Name"]][r["Dispatch_Id"]] += float(r["Counter_Value"])
+        key = (r["Kernel_Name"], int(r["Grid_Size"])) if by_grid else (r["Kernel_Name"], None)
+        acc[key][r["Dispatch_Id"]] += float(r["Counter_Value"])
     return {k: sum(v.values()) / len(v) for k, v in acc.items()}
 
 
-def traffic(fetch_dir, write_dir, calib_bytes):
-    fe = per_dispatch(fetch_dir, "FETCH_SIZE")
-    wr = per_dispatch(write_dir, "WRITE_SIZE")
-    calib = [k for k in fe if "neg_kernel" in k]
-    if not calib:
-        raise SystemExit("profsum: the calibration kernel (neg) is missing from the FETCH run")
-    ck = calib[0]
+def traffic(fetch_dir, write_dir, calib_bytes, by_grid=False):
+    fe = per_dispatch(fetch_dir, "FETCH_SIZE", by_grid)
+    wr = per_dispatch(write_dir, "WRITE_SIZE", by_grid)
+    calib = [k for k in fe if "neg_kernel" in k[0]]
+    wcal = [k for k in wr if "neg_kernel" in k[0]]
+    if not calib or not wcal:
+        raise SystemExit("profsum: the calibration kernel (neg) is missing from the FETCH or WRITE run")
+    ck, wk = max(calib, key=lambda k: fe[k]), max(wcal, key=lambda k: wr[k])
     f_scale = calib_bytes / (fe[ck] * 1024.0)
-    w_scale = calib_bytes / (wr[ck] * 1024.0)
+    w_scale = calib_bytes / (wr[wk] * 1024.0)
     print(f"Calibration kernel `neg` (float32, {calib_bytes} B read and written): FETCH_SIZE reads "
-          f"{fe[ck]:.0f} KiB, WRITE_SIZE {wr[ck]:.0f} KiB per dispatch -> scale FETCH by {f_scale:.3f}, "
+          f"{fe[ck]:.0f} KiB, WRITE_SIZE {wr[wk]:.0f} KiB per dispatch -> scale FETCH by {f_scale:.3f}, "
           f"WRITE by {w_scale:.3f}.\n")
-    print("| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes read (scaled) | written (scaled) | total |")
-    print("|---|---:|---:|---:|---:|---:|")
+    grid = "grid | " if by_grid else ""
+    print(f"| kernel | {grid}FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes read (scaled) | written (scaled) | total |")
+    print("|---|" + ("---:|" if by_grid else "") + "---:|---:|---:|---:|---:|")
     out = []
     for k in fe:
-        if "h8::" not in k or k not in wr:
+        if "h8::" not in k[0] or k not in wr:
             continue
         rb = fe[k] * 1024.0 * f_scale
         wb = wr[k] * 1024.0 * w_scale
-        print(f"| `{short_name(k)}` | {fe[k]:.0f} | {wr[k]:.0f} | {rb:.0f} | {wb:.0f} | {rb + wb:.0f} |")
-        out.append({"kernel": short_name(k), "read_bytes": rb, "write_bytes": wb, "bytes_per_launch": rb + wb})
+        g = f"{k[1]} | " if by_grid else ""
+        print(f"| `{short_name(k[0])}` | {g}{fe[k]:.0f} | {wr[k]:.0f} | {rb:.0f} | {wb:.0f} | {rb + wb:.0f} |")
+        o = {"kernel": short_name(k[0]), "read_bytes": rb, "write_bytes": wb, "bytes_per_launch": rb + wb}
+        if by_grid:
+            o["grid"] = k[1]
+        out.append(o)
     print()
     for o in out:
         print("traffic-json: " + json.dumps(o))
@@ -97,27 +124,60 @@ def launches(path, kernel, last):
         print(f"| {r['Dispatch_Id']} | {(int(r['End_Timestamp']) - int(r['Start_Timestamp'])) / 1e3:.2f} |")
 
 
+def runs(path, kernel, grid, min_run):
+    f = _one(os.path.join(path, "**", "*kernel_trace.csv"))
+    rows = sorted(csv.DictReader(open(f)), key=lambda r: int(r["Start_Timestamp"]))
+    out, cur = [], []
+    for r in rows:
+        hit = kernel in r["Kernel_Name"] and (grid is None or int(r["Grid_Size_X"]) == grid)
+        if hit and cur and r["Kernel_Name"] == cur[-1]["Kernel_Name"] and r["Grid_Size_X"] == cur[-1]["Grid_Size_X"]:
+            cur.append(r)
+            continue
+        if len(cur) >= min_run:
+            out.append(cur)
+        cur = [r] if hit else []
+    if len(cur) >= min_run:
+        out.append(cur)
+    if not out:
+        raise SystemExit(f"profsum: no run of {min_run} consecutive dispatches of {kernel!r}")
+    print("| kernel | grid | first dispatch | launches | mean us | min us | max us |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for run in out:
+        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in run]
+        print(f"| `{short_name(run[0]['Kernel_Name'])[:60]}` | {run[0]['Grid_Size_X']} | {run[0]['Dispatch_Id']} | "
+              f"{len(d)} | {sum(d) / len(d):.2f} | {min(d):.2f} | {max(d):.2f} |")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("dir")
     s.add_argument("--top", type=int, default=12)
+    s.add_argument("--by-grid", action="store_true")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
     t.add_argument("--calib-bytes", type=int, default=256 << 20)
+    t.add_argument("--by-grid", action="store_true")
     ln = sub.add_parser("launches")
     ln.add_argument("dir")
     ln.add_argument("--kernel", required=True)
     ln.add_argument("--last", type=int, default=2)
+    rn = sub.add_parser("runs")
+    rn.add_argument("dir")
+    rn.add_argument("--kernel", required=True)
+    rn.add_argument("--grid", type=int, default=None)
+    rn.add_argument("--min-run", type=int, default=20)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
-        stats(a.dir, a.top)
+        stats(a.dir, a.top, a.by_grid)
     elif a.cmd == "launches":
         launches(a.dir, a.kernel, a.last)
+    elif a.cmd == "runs":
+        runs(a.dir, a.kernel, a.grid, a.min_run)
     else:
-        traffic(a.fetch, a.write, a.calib_bytes)
+        traffic(a.fetch, a.write, a.calib_bytes, a.by_grid)
 
 
 if __name__ == "__main__":

@@ -211,9 +211,11 @@ size_t h8_mg_workspace_bytes(const h8_grid* g);
  * V-cycle as the preconditioner of the FP32 iterations (z = V(r) instead of
  * dinv r); same arguments, stopping rule, statistics and errors, plus the
  * hierarchy's workspace mg_ws.  The hierarchy is built from s_e64, K064 and
- * the free mask (dinv64 != 0) on every call.  Returns H8_MG_DECLINED (nothing
- * solved) when the coarsest operator is singular: supports that leave
- * rigid-body motions free.  Returns the u of h8_refine_solve to within tol. */
+ * the free mask (dinv64 != 0) on every call.  Returns H8_MG_DECLINED when the
+ * coarsest operator is singular (supports that leave rigid-body motions free):
+ * nothing solved, u holds the initial guess with its fixed entries zeroed (0
+ * without use_x0), as the Jacobi form starts from.  Returns the u of
+ * h8_refine_solve to within tol. */
 int h8_refine_solve_mg(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
                        const float* K032, const double* dinv64, const double* f, double* u, double tol,
                        int64_t maxiter, double delta, int use_x0, void* ws, size_t ws_bytes, void* mg_ws,

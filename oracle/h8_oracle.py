@@ -58,6 +58,17 @@ PINNING (function -> independent checks):
                           strain field gives D eps at every node and element, a
                           rigid motion gives zero; von Mises of uniaxial stress
                           is |s|, of pure shear sqrt(3)|t|
+  element_matvec        : equals assemble_csr(...) @ u for every chunk size
+  mg_levels / mg_hierarchy / mg_vcycle (the multigrid preconditioner of the
+  FP32 iterations; the reference has none -- its checker role is the GPU
+  V-cycle) :              the level plan follows the halving rule; the
+                          prolongation reproduces every linear field exactly;
+                          on a free body every Galerkin operator annihilates the
+                          six rigid-body modes of its grid and is symmetric; the
+                          two Chebyshev steps equal the scaled Chebyshev
+                          polynomial T2((theta - lam)/delta) / T2(theta/delta);
+                          the V-cycle is symmetric positive definite and PCG
+                          with it reaches spsolve's solution (tests/test_oracle_mg.py)
 """
 from __future__ import annotations
 
