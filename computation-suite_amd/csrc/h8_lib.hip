@@ -1591,7 +1591,6 @@ struct MgPlan {
     MgConst* cst = nullptr;       // [MG_MAX_LEVELS]
     double* M = nullptr;          // the 8 child matrices of level 1, [8][576]
     unsigned* emask = nullptr;    // free-DOF bits of every fine element
-    double* Kc = nullptr;         // level-1 element matrices, [coarse element][24][24]
     int* flag = nullptr;          // singular coarsest operator
     double* A = nullptr;          // coarsest level, dense (inverted in place)
     float* Ainv = nullptr;
@@ -1612,9 +1611,8 @@ static bool mg_plan(const GridDev& d, MgPlan& P) {
         L.rx = rx; L.ry = ry; L.rz = rz;
         nx /= rx; ny /= ry; nz /= rz;
     }
-    // the MG kernels index stencil levels and the level-1 element matrices in 32 bits
-    const long long mc1 = P.nlev >= 2 ? (long long)P.L[1].nx * P.L[1].ny * P.L[1].nz : 0;
-    if (P.nlev >= 2 && (243LL * P.L[1].nnodes >= (1LL << 31) || 576LL * mc1 >= (1LL << 31))) return false;
+    // the MG kernels index stencil levels in 32 bits
+    if (P.nlev >= 2 && 243LL * P.L[1].nnodes >= (1LL << 31)) return false;
     return P.nlev >= 2 && P.L[P.nlev - 1].ndof <= MG_DENSE_MAX;
 }
 
@@ -1633,7 +1631,6 @@ static size_t mg_layout(MgPlan& P, char* base) {
     P.M = (double*)take(sizeof(double) * 8 * 576);
     P.emask = (unsigned*)take(sizeof(unsigned) * (size_t)P.L[0].nx * P.L[0].ny * P.L[0].nz);
     P.flag = (int*)take(sizeof(int));
-    P.Kc = (double*)take(sizeof(double) * 576 * (size_t)P.L[1].nx * P.L[1].ny * P.L[1].nz);
     P.L[0].y = (float*)take(sizeof(float) * P.L[0].ndof);
     P.L[0].d = (float*)take(sizeof(float) * P.L[0].ndof);
     P.L[0].x0 = (float*)take(sizeof(float) * P.L[0].ndof);
@@ -1702,11 +1699,13 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
     hipLaunchKernelGGL(k_mg_elem_mask, dim3(mg_blocks(m)), dim3(256), 0, s, d, (const float*)L0.dinv, P.emask);
     hipLaunchKernelGGL(k_mg_gersh0, dim3(mg_blocks(d.nnodes)), dim3(256), 0, s, d, s_e32, K064,
                        (const float*)L0.dinv, P.cst);
-    const long long mc1 = (long long)P.L[1].nx * P.L[1].ny * P.L[1].nz;
-    hipLaunchKernelGGL(k_mg_coarse_elements, dim3(mg_blocks(576 * mc1)), dim3(256), 0, s, d, P.L[1], L0.rx, L0.ry,
-                       L0.rz, s_e64, K064, (const double*)P.M, (const unsigned*)P.emask, P.Kc);
-    hipLaunchKernelGGL(k_mg_galerkin0, dim3(unsigned(std::min<long long>((P.L[1].nnodes + MG_GT - 1) / MG_GT, 8192))),
-                       dim3(256), 0, s, P.L[1], (const double*)P.Kc);
+    {
+        const MgLevel& L1 = P.L[1];
+        const long long runs = (L1.NJ + MG_GT - 1) / MG_GT;
+        const long long nblk = runs * L1.NI * L1.NK;
+        hipLaunchKernelGGL(k_mg_level1, dim3(unsigned(std::min<long long>(nblk, 16384))), dim3(256), 0, s, d, L1,
+                           L0.rx, L0.ry, L0.rz, s_e64, K064, (const double*)P.M, (const unsigned*)P.emask);
+    }
     hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[1].nnodes)), dim3(256), 0, s, P.L[1], P.cst + 1);
     for (int l = 1; l + 1 < nl; ++l) {
         const MgLevel& F = P.L[l];

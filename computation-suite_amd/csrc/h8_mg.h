@@ -168,96 +168,110 @@ __global__ void __launch_bounds__(256) k_mg_elem_mask(GridDev g, const float* __
     }
 }
 
-// Level-1 element matrices: Kc_E = sum over the children e of coarse element
-// E of s_e (P_c^T M_e K0 M_e P_c), FP64, [E][24][24]; one thread per entry,
-// the lanes of a wave over the entries of one element (its children's scales
-// and masks are wave-uniform loads).  A child with all 24 DOFs free
-// contributes s_e M_c (M_c staged in LDS); a child with fixed DOFs is
-// evaluated with its mask.
-__global__ void __launch_bounds__(256) k_mg_coarse_elements(GridDev g, MgLevel C, int rx, int ry, int rz,
-                                                            const double* __restrict__ s_e,
-                                                            const double* __restrict__ K0,
-                                                            const double* __restrict__ Mg,
-                                                            const unsigned* __restrict__ emask,
-                                                            double* __restrict__ Kc) {
-    __shared__ double M[8 * 576];
-    for (int t = threadIdx.x; t < 8 * 576; t += blockDim.x) M[t] = Mg[t];
-    __syncthreads();
-    const unsigned mc = unsigned(C.nx) * unsigned(C.ny) * unsigned(C.nz);
-    const unsigned cny = unsigned(C.ny), cnx = unsigned(C.nx);
-    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < 576u * mc; t += gridDim.x * blockDim.x) {
-        const unsigned E = t / 576u, ent = t - E * 576u;
-        const int row = int(ent / 24u), col = int(ent % 24u);
-        const int A = row / 3, i = row % 3, B = col / 3, j = col % 3;
-        const unsigned q = E / cny;
-        const int Ej = int(E - q * cny), Ek = int(q / cnx), Ei = int(q - unsigned(Ek) * cnx);
-        double acc = 0.0;
-        for (int cz = 0; cz < rz; ++cz)
-            for (int cy = 0; cy < ry; ++cy)
-                for (int cx = 0; cx < rx; ++cx) {
-                    const long long fe = ((long long)(Ek * rz + cz) * g.nx + (Ei * rx + cx)) * g.ny + (Ej * ry + cy);
-                    const double sv = s_e[fe];
-                    const unsigned mask = emask[fe];
-                    if (mask == 0xFFFFFFu) {
-                        acc += sv * M[(cx + 2 * cy + 4 * cz) * 576 + int(ent)];
-                    } else if (mask) {
-                        for (int a = 0; a < 8; ++a) {
-                            if (!((mask >> (3 * a + i)) & 1u)) continue;
-                            const double pa =
-                                mg_w(cx + OX(a), OX(A), rx) * mg_w(cy + OY(a), OY(A), ry) * mg_w(cz + OZ(a), OZ(A), rz);
-                            if (pa == 0.0) continue;
-                            for (int b = 0; b < 8; ++b) {
-                                if (!((mask >> (3 * b + j)) & 1u)) continue;
-                                const double pb = mg_w(cx + OX(b), OX(B), rx) * mg_w(cy + OY(b), OY(B), ry) *
-                                                  mg_w(cz + OZ(b), OZ(B), rz);
-                                if (pb != 0.0) acc += sv * pa * pb * K0[(3 * a + i) * 24 + 3 * b + j];
-                            }
-                        }
-                    }
-                }
-        Kc[t] = acc;
-    }
-}
-
-// Level-1 stencil from the element matrices: every entry (coarse node I,
-// neighbour block nb, entry 3 c + d) sums, over the coarse elements E that
-// hold both I (corner A) and J = I + nb (corner B), Kc_E[3A + c][3B + d] in a
-// fixed order.  A block takes GT consecutive nodes: its lanes walk the 243
-// entries of a node (nearby words of a few element matrices), the results
-// are transposed in LDS and stored node-contiguous.
+// Level-1 stencil straight from the fine elements: entry (coarse node I,
+// neighbour block nb, 3 c + d) of A_1 = P^T A_0 P sums, over the coarse
+// elements E holding both I (corner A) and J = I + nb (corner B) and over E's
+// fine children e (child position ch), s_e (P_ch^T K0 P_ch)[3A + c][3B + d]
+// -- the fixed child blocks M (k_mg_mc) -- when every DOF of e is free; a
+// child with fixed DOFs is evaluated with its free mask (K0 read from global
+// memory: only boundary elements take that path).  A block takes a run of up
+// to MG_GT coarse nodes along j at one (i, k) and stages M and the scales and
+// masks of the fine elements around the run in LDS; a thread owns one (node,
+// neighbour) pair and accumulates its nine entries in FP64 (coarse elements,
+// then children, in a fixed order), so the element loops and the scale and
+// mask loads are shared by nine products.  Adjacent threads hold adjacent
+// nodes of one neighbour block, so the stores of an entry are contiguous.
 constexpr int MG_GT = 32;
-__global__ void __launch_bounds__(256) k_mg_galerkin0(MgLevel C, const double* __restrict__ Kc) {
-    __shared__ float tile[243 * MG_GT];
-    for (long long I0 = (long long)blockIdx.x * MG_GT; I0 < C.nnodes; I0 += (long long)gridDim.x * MG_GT) {
-        const int nt = int(min((long long)MG_GT, C.nnodes - I0));
-        for (int k = threadIdx.x; k < 243 * nt; k += blockDim.x) {
-            const int in = k / 243, e = k - in * 243;
-            const int nb = e / 9, c = (e % 9) / 3, d = e % 3;
-            int Ij, Ii, Ik;
-            mg_decode(C, I0 + in, Ij, Ii, Ik);
+constexpr int MG_L1_ELEMS = (MG_GT + 1) * 2 * 4 * 4;   // fine elements around a run, at most
+__global__ void __launch_bounds__(256) k_mg_level1(GridDev g, MgLevel C, int rx, int ry, int rz,
+                                                   const double* __restrict__ s_e, const double* __restrict__ K0,
+                                                   const double* __restrict__ Mg, const unsigned* __restrict__ emask) {
+    __shared__ double M[8 * 576];
+    __shared__ double sS[MG_L1_ELEMS];
+    __shared__ unsigned sMask[MG_L1_ELEMS];
+    for (int t = threadIdx.x; t < 8 * 576; t += blockDim.x) M[t] = Mg[t];
+    const int runs = (C.NJ + MG_GT - 1) / MG_GT;              // runs per j-row, balanced lengths
+    const int rlen = (C.NJ + runs - 1) / runs;
+    const long long nblk = (long long)runs * C.NI * C.NK;
+    for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int r = int(b % runs);
+        const long long q = b / runs;
+        const int Ii = int(q % C.NI), Ik = int(q / C.NI);
+        const int Ij0 = r * rlen, nt = min(rlen, C.NJ - Ij0);
+        if (nt <= 0) continue;                                 // (uniform in the block)
+        // fine elements around the run: j in [(Ij0-1) ry, (Ij0+nt) ry), i in
+        // [(Ii-1) rx, (Ii+1) rx), k in [(Ik-1) rz, (Ik+1) rz), clipped to the grid
+        const int ej0 = max(0, (Ij0 - 1) * ry), ej1 = min(g.ny, (Ij0 + nt) * ry);
+        const int ei0 = max(0, (Ii - 1) * rx), ei1 = min(g.nx, (Ii + 1) * rx);
+        const int ek0 = max(0, (Ik - 1) * rz), ek1 = min(g.nz, (Ik + 1) * rz);
+        const int wj = ej1 - ej0, wi = ei1 - ei0, wk = ek1 - ek0;
+        __syncthreads();                                       // the previous run is done with sS
+        for (int t = threadIdx.x; t < wj * wi * wk; t += blockDim.x) {
+            const int lj = t % wj, li = (t / wj) % wi, lk = t / (wj * wi);
+            const long long fe = ((long long)(ek0 + lk) * g.nx + (ei0 + li)) * g.ny + (ej0 + lj);
+            sS[t] = s_e[fe];
+            sMask[t] = emask[fe];
+        }
+        __syncthreads();
+        const long long I0 = ((long long)Ik * C.NI + Ii) * C.NJ + Ij0;
+        for (int k = threadIdx.x; k < 27 * nt; k += blockDim.x) {
+            const int nb = k / nt, in = k - nb * nt;
+            const int Ij = Ij0 + in;
             const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
-            double acc = 0.0;
+            double acc[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) acc[e] = 0.0;
             if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
-                const int ej0 = max(max(Ij, Jj) - 1, 0), ej1 = min(min(Ij, Jj), C.ny - 1);
-                const int ei0 = max(max(Ii, Ji) - 1, 0), ei1 = min(min(Ii, Ji), C.nx - 1);
-                const int ek0 = max(max(Ik, Jk) - 1, 0), ek1 = min(min(Ik, Jk), C.nz - 1);
-                for (int Ek = ek0; Ek <= ek1; ++Ek)
-                    for (int Ei = ei0; Ei <= ei1; ++Ei)
-                        for (int Ej = ej0; Ej <= ej1; ++Ej) {
+                const int Ej0 = max(max(Ij, Jj) - 1, 0), Ej1 = min(min(Ij, Jj), C.ny - 1);
+                const int Ei0 = max(max(Ii, Ji) - 1, 0), Ei1 = min(min(Ii, Ji), C.nx - 1);
+                const int Ek0 = max(max(Ik, Jk) - 1, 0), Ek1 = min(min(Ik, Jk), C.nz - 1);
+                for (int Ek = Ek0; Ek <= Ek1; ++Ek)
+                    for (int Ei = Ei0; Ei <= Ei1; ++Ei)
+                        for (int Ej = Ej0; Ej <= Ej1; ++Ej) {
                             const int A = corner_of(Ii - Ei, Ij - Ej, Ik - Ek);
                             const int B = corner_of(Ji - Ei, Jj - Ej, Jk - Ek);
-                            const long long E = ((long long)Ek * C.nx + Ei) * C.ny + Ej;
-                            acc += Kc[E * 576 + (3 * A + c) * 24 + 3 * B + d];
+#pragma unroll
+                            for (int ch = 0; ch < 8; ++ch) {
+                                const int cx = ch & 1, cy = (ch >> 1) & 1, cz = ch >> 2;
+                                if (cx >= rx || cy >= ry || cz >= rz) continue;
+                                const int le = ((Ek * rz + cz - ek0) * wi + (Ei * rx + cx - ei0)) * wj +
+                                               (Ej * ry + cy - ej0);
+                                const double sv = sS[le];
+                                const unsigned mask = sMask[le];
+                                if (mask == 0xFFFFFFu) {
+                                    const double* m = M + ch * 576 + (3 * A) * 24 + 3 * B;
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                                        for (int d = 0; d < 3; ++d) acc[3 * c + d] += sv * m[c * 24 + d];
+                                } else if (mask) {
+                                    for (int a = 0; a < 8; ++a) {
+                                        const double pa = mg_w(cx + OX(a), OX(A), rx) * mg_w(cy + OY(a), OY(A), ry) *
+                                                          mg_w(cz + OZ(a), OZ(A), rz);
+                                        if (pa == 0.0) continue;
+                                        for (int bb = 0; bb < 8; ++bb) {
+                                            const double pb = mg_w(cx + OX(bb), OX(B), rx) *
+                                                              mg_w(cy + OY(bb), OY(B), ry) *
+                                                              mg_w(cz + OZ(bb), OZ(B), rz);
+                                            if (pb == 0.0) continue;
+                                            const double w = sv * pa * pb;
+#pragma unroll
+                                            for (int c = 0; c < 3; ++c) {
+                                                if (!((mask >> (3 * a + c)) & 1u)) continue;
+#pragma unroll
+                                                for (int d = 0; d < 3; ++d)
+                                                    if ((mask >> (3 * bb + d)) & 1u)
+                                                        acc[3 * c + d] += w * K0[(3 * a + c) * 24 + 3 * bb + d];
+                                            }
+                                        }
+                                    }
+                                }
+                            }
                         }
             }
-            tile[e * MG_GT + in] = float(acc);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) C.S[(long long)(nb * 9 + e) * C.nnodes + I0 + in] = float(acc[e]);
         }
-        __syncthreads();
-        for (int k = threadIdx.x; k < 243 * MG_GT; k += blockDim.x) {
-            const int e = k / MG_GT, in = k - e * MG_GT;
-            if (in < nt) C.S[(long long)e * C.nnodes + I0 + in] = tile[k];
-        }
-        __syncthreads();
     }
 }
 
