@@ -275,6 +275,138 @@ k_bupdate(BArgs B, BVecs<T> V) {
     }
 }
 
+// Persistent batched solve: one launch for the whole batch.  Every workgroup
+// takes whole systems from a queue (`order`, largest first; `queue` the next
+// index) and runs each one's Jacobi-PCG to the end with the recurrence and
+// stopping rule of k_binit / k_bsector / k_bupdate: the sector march of the
+// direction kernel over the instance's column tiles (x += alpha p_old applied
+// while staging, as MODE_DIRX), p.q, alpha, the update, r.z and r.r, the
+// stopping test, beta.  Every reduction stays inside the workgroup (per-thread
+// sums in a fixed node order, then block_sum), so no system waits on another
+// workgroup and a system's result does not depend on which workgroup runs it.
+// A workgroup ends when the queue is empty.
+template <typename T, int EJ, int EI>
+__global__ void __launch_bounds__(EJ * EI) __attribute__((amdgpu_waves_per_eu(SectorWaves<T>::value)))
+k_bsolve(BArgs B, BVecs<T> V, const int* __restrict__ order, int n_sys, int* queue, int kd) {
+    constexpr int NT = EJ * EI;
+    __shared__ SectorSmem<T, EJ, EI> sm;
+    __shared__ double sRed[16 * 4];
+    __shared__ int sSys;
+    for (;;) {
+        __syncthreads();                      // everyone has read the previous sSys
+        if (threadIdx.x == 0) {
+            const int t = atomicAdd(queue, 1);
+            sSys = t < n_sys ? order[t] : -1;
+        }
+        __syncthreads();
+        const int sid = sSys;
+        if (sid < 0) return;
+        BSys& S = B.sys[sid];
+        const BInst& I = B.inst[S.inst];
+        NodeArgs<T> A{};
+        A.g = inst_grid(I, EJ, EI, kd);
+        sector_tiles(A.g, EJ, EI, kd);
+        A.s_e = V.s_e + I.elem_off;
+        A.K0 = V.K0 + (size_t)I.k0_off * 648;   // H8_K0_WORDS
+        A.dinv = V.dinv + 3 * I.node_off;
+        A.r = V.r + S.off;
+        A.p0 = V.p0 + S.off;
+        A.p1 = V.p1 + S.off;
+        A.q = V.q + S.off;
+        A.x = V.x + S.off;
+        const T* b = V.b + S.off;
+        const T* dinv = A.dinv;
+        T* r = V.r + S.off;
+        T* x = A.x;
+        const int n = S.ndof;
+        // init: r = mask(b), x = 0, |b|^2, |r|^2, r.z
+        double v3[3] = {0.0, 0.0, 0.0};
+        for (int e = threadIdx.x; e < n; e += NT) {
+            const T d = dinv[e];
+            const T bi = (d != T(0)) ? b[e] : T(0);
+            r[e] = bi;
+            x[e] = T(0);
+            v3[0] += double(bi) * double(bi);
+            v3[1] += double(bi) * double(bi);
+            v3[2] += double(bi) * double(d * bi);
+        }
+        block_sum<3>(v3, sRed);
+        double bn = sqrt(v3[0]);
+        if (bn == 0.0) bn = 1.0;
+        double rel = sqrt(v3[1]) / bn, rz_old = v3[2], alpha = 0.0, beta = 0.0;
+        long long k = 1, iters = 0;
+        int first = 1, pending = 0, status = ST_RUNNING;
+        if (rel < B.tol) {
+            status = ST_DONE;
+        } else if (B.maxiter == 0) {
+            status = ST_MAXITER;
+        }
+        while (status == ST_RUNNING) {
+            const bool odd = (k & 1) != 0;
+            T* p_new = odd ? A.p1 : A.p0;
+            const T* p_old = odd ? A.p0 : A.p1;
+            double part[2] = {0.0, 0.0};
+            for (int tile = 0; tile < A.g.ntiles; ++tile) {
+                sector_march<T, EJ, EI, MODE_DIRX>(A, A.K0 + 576, kd, tile, T(beta), first, p_new, p_old, T(alpha),
+                                                  pending, sm, part);
+                __syncthreads();
+            }
+            double v1[1] = {part[0]};
+            block_sum<1>(v1, sRed);               // (its barriers also publish q, p and x to the workgroup)
+            const double pq = v1[0];
+            if (!(pq > 0.0)) {
+                S.bad_pq = pq;
+                iters = k;
+                pending = 0;
+                status = ST_NOT_SPD;
+                break;
+            }
+            alpha = rz_old / pq;
+            pending = 1;                          // x += alpha p: next march, or after the loop
+            const T ta = T(alpha);
+            double v2[2] = {0.0, 0.0};
+            for (int e = threadIdx.x; e < n; e += NT) {
+                const T d = dinv[e];
+                const T ri = (d != T(0)) ? T(r[e] - ta * A.q[e]) : T(0);
+                r[e] = ri;
+                v2[0] += double(ri) * double(d * ri);
+                v2[1] += double(ri) * double(ri);
+            }
+            block_sum<2>(v2, sRed);
+            rel = sqrt(v2[1]) / bn;
+            if (rel < B.tol) {
+                iters = k;
+                status = ST_DONE;
+            } else if (k >= B.maxiter) {
+                iters = k;
+                status = ST_MAXITER;
+            } else {
+                beta = v2[0] / rz_old;
+                rz_old = v2[0];
+                first = 0;
+                ++k;
+            }
+        }
+        if (pending) {
+            // the last direction's x += alpha p (k_bfinal of the launched form)
+            const T* p = ((k & 1) ? A.p1 : A.p0);
+            const T ta = T(alpha);
+            for (int e = threadIdx.x; e < n; e += NT) x[e] += ta * p[e];
+        }
+        if (threadIdx.x == 0) {
+            S.status = status;
+            S.iters = iters;
+            S.rel = rel;
+            S.k = k;
+            S.alpha = alpha;
+            S.beta = beta;
+            S.rz_old = rz_old;
+            S.bnorm = bn;
+            S.pending = 0;
+        }
+    }
+}
+
 // After the batched loop: x += alpha p of the last direction kernel for every
 // system whose update is still pending (flat chunk table).
 template <typename T>

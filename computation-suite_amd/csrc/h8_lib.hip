@@ -2343,7 +2343,8 @@ constexpr int BEJ = 8, BEI = 16, BKD = 16;
 
 struct BLayout {
     int n_sys = 0, n_tiles = 0, n_chunks = 0, pstride = 0;
-    size_t off_r, off_p0, off_p1, off_q, off_inst, off_tiles, off_chunks, off_sys, off_active, off_part, total;
+    size_t off_r, off_p0, off_p1, off_q, off_inst, off_tiles, off_chunks, off_sys, off_active, off_part, off_order,
+        off_queue, total;
 };
 
 int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLayout& L) {
@@ -2378,6 +2379,8 @@ int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLa
     L.off_sys = take(sizeof(BSys) * L.n_sys);
     L.off_active = take(sizeof(int));
     L.off_part = take(sizeof(double) * (size_t)L.n_sys * L.pstride);
+    L.off_order = take(sizeof(int) * (size_t)L.n_sys);
+    L.off_queue = take(sizeof(int));
     L.total = p;
     return H8_OK;
 }
@@ -2389,11 +2392,27 @@ extern "C" size_t h8_batched_workspace_bytes(const h8_instance* inst, int32_t n_
     return L.total;
 }
 
-extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64_t total_nodes, int64_t total_elems,
-                                  int64_t total_vec, const double* s_e, const double* K0, int32_t k0_per_instance,
-                                  const double* dinv, const double* b, double* x, double tol, int64_t maxiter,
-                                  void* ws, size_t ws_bytes, int32_t* status_out, int64_t* iters_out,
-                                  double* rel_out, void* stream) {
+// The persistent form: k_bsolve<double, EJ, EI> over the queue, one launch.
+template <int EJ, int EI>
+static int batched_persistent(const BArgs& A, const BVecs<double>& V, int* order, int n_sys, int* queue,
+                              hipStream_t s) {
+    const void* fn = (const void*)k_bsolve<double, EJ, EI>;
+    int per = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, EJ * EI, 0));
+    const int grid = std::max(1, std::min(n_sys, std::max(1, per) * device_cus()));
+    hipLaunchKernelGGL((k_bsolve<double, EJ, EI>), dim3(grid), dim3(EJ * EI), 0, s, A, V, (const int*)order, n_sys,
+                       queue, BKD);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+static int pcg_batched(const h8_instance* inst, int32_t n_inst, int64_t total_nodes, int64_t total_elems,
+                       int64_t total_vec, const double* s_e, const double* K0, int32_t k0_per_instance,
+                       const double* dinv, const double* b, double* x, double tol, int64_t maxiter, void* ws,
+                       size_t ws_bytes, int32_t* status_out, int64_t* iters_out, double* rel_out, void* stream,
+                       int form) {
+    if (form < H8_BATCHED_LAUNCHED || form > H8_BATCHED_PERSISTENT_16X16)
+        return fail(H8_ERR_ARG, "batched form %d: expected 0 (launched), 1 or 2 (persistent)", form);
     BLayout L;
     int rc = batched_layout(inst, n_inst, total_vec, L);
     if (rc) return rc;
@@ -2465,12 +2484,31 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
     HIPCHK(hipMemcpyAsync((void*)A.sys, hs.data(), sizeof(BSys) * hs.size(), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, A.active, L.n_sys);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_binit<double>, dim3(L.n_chunks), dim3(256), 0, s, A, V);
-    LAUNCHCHK();
     int active = 0;
-    HIPCHK(hipMemcpyAsync(&active, A.active, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (form == H8_BATCHED_LAUNCHED) {
+        hipLaunchKernelGGL(k_binit<double>, dim3(L.n_chunks), dim3(256), 0, s, A, V);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(&active, A.active, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
     HIPCHK(hipStreamSynchronize(s));   // also keeps the host tables alive until the copies completed
     long long launches = 0;
+    if (form != H8_BATCHED_LAUNCHED) {
+        // the persistent form re-initialises every system itself: systems in
+        // decreasing size (the longest solves start first), one launch
+        std::vector<int> ord(L.n_sys);
+        for (int i = 0; i < L.n_sys; ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return hs[a].ndof > hs[c].ndof; });
+        int* dord = (int*)(w + L.off_order);
+        int* dq = (int*)(w + L.off_queue);
+        HIPCHK(hipMemcpyAsync(dord, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, dq, 0);
+        LAUNCHCHK();
+        rc = form == H8_BATCHED_PERSISTENT_8X16 ? batched_persistent<8, 16>(A, V, dord, L.n_sys, dq, s)
+                                                : batched_persistent<16, 16>(A, V, dord, L.n_sys, dq, s);
+        if (rc) return rc;
+        launches = 1;
+        HIPCHK(hipStreamSynchronize(s));   // the order table stays alive until its copy completed
+    }
     if (active > 0) {
         GraphKey key{};
         key.kind = 3;
@@ -2526,6 +2564,24 @@ extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64
         return fail(H8_MAXITER, "batched PCG: at least one system did not converge in %lld iterations",
                     (long long)maxiter);
     return H8_OK;
+}
+
+extern "C" int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64_t total_nodes, int64_t total_elems,
+                                  int64_t total_vec, const double* s_e, const double* K0, int32_t k0_per_instance,
+                                  const double* dinv, const double* b, double* x, double tol, int64_t maxiter,
+                                  void* ws, size_t ws_bytes, int32_t* status_out, int64_t* iters_out,
+                                  double* rel_out, void* stream) {
+    return pcg_batched(inst, n_inst, total_nodes, total_elems, total_vec, s_e, K0, k0_per_instance, dinv, b, x, tol,
+                       maxiter, ws, ws_bytes, status_out, iters_out, rel_out, stream, H8_BATCHED_LAUNCHED);
+}
+
+extern "C" int h8_pcg_batched_form_f64(const h8_instance* inst, int32_t n_inst, int64_t total_nodes,
+                                       int64_t total_elems, int64_t total_vec, const double* s_e, const double* K0,
+                                       int32_t k0_per_instance, const double* dinv, const double* b, double* x,
+                                       double tol, int64_t maxiter, void* ws, size_t ws_bytes, int32_t* status_out,
+                                       int64_t* iters_out, double* rel_out, int32_t form, void* stream) {
+    return pcg_batched(inst, n_inst, total_nodes, total_elems, total_vec, s_e, K0, k0_per_instance, dinv, b, x, tol,
+                       maxiter, ws, ws_bytes, status_out, iters_out, rel_out, stream, form);
 }
 
 extern "C" double h8_batched_last_ms(void) { return g_batched_ms; }

@@ -24,6 +24,7 @@ H8_ZERO_DIAG = 5
 H8_ERR_WORKSPACE = 6
 H8_MG_DECLINED = 7
 MG_TIMES = 10             # H8_MG_TIMES: slots of h8_mg_time_kernels
+BATCHED_LAUNCHED, BATCHED_PERSISTENT_8X16, BATCHED_PERSISTENT_16X16 = 0, 1, 2   # H8_BATCHED_*
 SCRATCH_BYTES = 32 << 20   # H8_SCRATCH_BYTES: the static workspace of h8_pcg_f64_noalloc
 
 (VARIANT_AUTO, VARIANT_NODE, VARIANT_MFMA, VARIANT_NODE_PERSIST, VARIANT_NODE_SPLIT, VARIANT_SECTOR,
@@ -101,6 +102,9 @@ SIGNATURES = {
     "h8_batched_workspace_bytes": (C.c_size_t, [C.POINTER(H8Instance), C.c_int32, C.c_int64]),
     "h8_pcg_batched_f64": (C.c_int, [C.POINTER(H8Instance), C.c_int32, C.c_int64, C.c_int64, C.c_int64, _P, _P,
                                      C.c_int32, _P, _P, _P, _D, _I64, _P, C.c_size_t, _P, _P, _P, _P]),
+    "h8_pcg_batched_form_f64": (C.c_int, [C.POINTER(H8Instance), C.c_int32, C.c_int64, C.c_int64, C.c_int64, _P,
+                                          _P, C.c_int32, _P, _P, _P, _D, _I64, _P, C.c_size_t, _P, _P, _P,
+                                          C.c_int32, _P]),
     "h8_batched_last_ms": (C.c_double, []),
     "h8_batched_last_launches": (C.c_int64, []),
     "h8_recovery_table_f64": (C.c_int, [_D, _D, _D, _P, _P, _P]),

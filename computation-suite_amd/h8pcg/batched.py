@@ -9,7 +9,16 @@ it runs one batched Jacobi-PCG over all columns.
 ``solve_many(systems)`` solves many *different* structured models, each with
 its own load cases, in the same batched launches (the role fe/batch.py:60-128
 plays with a process pool): every (model, load case) pair is a system with
-its own convergence mask, and all systems advance together.
+its own convergence mask.
+
+Forms (h8_pcg_batched_form_f64): every system advanced together, two launches
+per iteration in captured graphs ("launched"), or one persistent launch whose
+workgroups take whole systems from a queue and run each to the end, with the
+direction kernel's sector march on 8 x 16 or 16 x 16 element columns.  The
+batched autotuner (``choose_form``) times the three on the batch itself (40
+iterations of every system, tol 0) the first time a batch of that composition
+is seen on a device, and keeps the fastest; the choice is cached with the
+kernel autotuner's (h8pcg.autotune.cache_path()).
 """
 from __future__ import annotations
 
@@ -66,7 +75,19 @@ class _Batch:
         self.elem_off += op.m
         self.vec_off += op.n_dof * nrhs
 
-    def solve(self, tol, maxiter):
+    def signature(self):
+        """The batch composition: sorted (nx, ny, nz, nrhs) counts."""
+        import collections
+        c = collections.Counter((i.nx, i.ny, i.nz, i.nrhs) for i in self.inst)
+        return ";".join("%dx%dx%dr%d*%d" % (k + (v,)) for k, v in sorted(c.items()))
+
+    def solve(self, tol, maxiter, form=None):
+        """form: _lib.BATCHED_* or None (the batched autotuner's choice)."""
+        if form is None:
+            form = choose_form(self)
+        return self._solve(tol, maxiter, int(form))
+
+    def _solve(self, tol, maxiter, form, check_rc=True):
         import torch
         n_inst = len(self.inst)
         arr = (H8Instance * n_inst)(*self.inst)
@@ -87,13 +108,15 @@ class _Batch:
         if maxiter is None:
             maxiter = 10 * max(nd for nd, _ in self.ndofs)
         with torch.cuda.device(self.device):
-            rc = _lib.lib().h8_pcg_batched_f64(arr, n_inst, self.node_off, self.elem_off, total_vec, ptr(s_e), ptr(K0),
-                                               1, ptr(dinv), ptr(b), ptr(x), float(tol), int(maxiter), ptr(ws),
-                                               ws.numel(), status, iters, rel, _lib.stream_handle(self.device))
+            rc = _lib.lib().h8_pcg_batched_form_f64(arr, n_inst, self.node_off, self.elem_off, total_vec, ptr(s_e),
+                                                    ptr(K0), 1, ptr(dinv), ptr(b), ptr(x), float(tol), int(maxiter),
+                                                    ptr(ws), ws.numel(), status, iters, rel, int(form),
+                                                    _lib.stream_handle(self.device))
         stats = {"status": np.array(status[:]), "iterations": np.array(iters[:]), "rel_residual": np.array(rel[:]),
                  "elapsed_ms": _lib.lib().h8_batched_last_ms(),
-                 "launches": int(_lib.lib().h8_batched_last_launches()), "rc": rc}
-        check(rc)
+                 "launches": int(_lib.lib().h8_batched_last_launches()), "rc": rc, "form": FORM_NAMES[form]}
+        if check_rc:
+            check(rc)
         out, o = [], 0
         for nd, nrhs in self.ndofs:
             out.append(x[o:o + nd * nrhs].reshape(nrhs, nd).T)
@@ -101,8 +124,43 @@ class _Batch:
         return out, stats
 
 
-def solve_static_batched(fesystem, F_batch, tol=1e-8, maxiter=None, return_stats=False):
-    """U_batch (n_dof, n_batch) for the columns of F_batch (fe/batched_solve.py:105-164)."""
+FORM_NAMES = {_lib.BATCHED_LAUNCHED: "launched", _lib.BATCHED_PERSISTENT_8X16: "persistent-8x16",
+              _lib.BATCHED_PERSISTENT_16X16: "persistent-16x16"}
+# candidates in order of preference on near-ties: the one persistent launch
+# first, then the launched form, then the narrow-column persistent form
+CANDIDATES = (_lib.BATCHED_PERSISTENT_16X16, _lib.BATCHED_LAUNCHED, _lib.BATCHED_PERSISTENT_8X16)
+TUNE_ITERS = 40
+TIE = 0.02
+
+
+def choose_form(batch, cache=True):
+    """The batched autotuner: the fastest form for this batch's composition on
+    this device, timed once (TUNE_ITERS iterations of every system at tol 0,
+    the solve's own device time) and then served from the cache; the first of
+    CANDIDATES within TIE of the fastest wins."""
+    from . import autotune
+    key = "%s|batched|%s" % (autotune._device_key(), batch.signature())
+    path = autotune.cache_path()
+    data = autotune._load(path) if cache else {}
+    if key in data:
+        return int(data[key][0])
+    timed = []
+    for form in CANDIDATES:
+        batch._solve(0.0, TUNE_ITERS, form, check_rc=False)          # warm-up (graphs, code objects)
+        _, st = batch._solve(0.0, TUNE_ITERS, form, check_rc=False)
+        timed.append((form, st["elapsed_ms"]))
+    fastest = min(t for _, t in timed)
+    form, ms = next((f, t) for f, t in timed if t <= fastest * (1.0 + TIE))
+    if cache:
+        data = autotune._load(path)
+        data[key] = [form, ms, {FORM_NAMES[f]: t for f, t in timed}]
+        autotune._store(path, data)
+    return form
+
+
+def solve_static_batched(fesystem, F_batch, tol=1e-8, maxiter=None, return_stats=False, form=None):
+    """U_batch (n_dof, n_batch) for the columns of F_batch (fe/batched_solve.py:105-164).
+    form: _lib.BATCHED_* or None (the batched autotuner's choice)."""
     F_batch = np.asarray(F_batch, dtype=float)
     if F_batch.ndim != 2 or F_batch.shape[0] != fesystem.n_dof:
         raise ValueError(f"F_batch must have shape ({fesystem.n_dof}, n_batch), got {F_batch.shape}")
@@ -112,17 +170,18 @@ def solve_static_batched(fesystem, F_batch, tol=1e-8, maxiter=None, return_stats
     cols, uD = _dirichlet_columns(fesystem, F_batch)
     batch = _Batch(op.device)
     batch.add(op, cols)
-    (X,), stats = batch.solve(tol, maxiter)
+    (X,), stats = batch.solve(tol, maxiter, form)
     if uD is not None:
         X = X + uD[:, None]
     U = X.cpu().numpy()
     return (U, stats) if return_stats else U
 
 
-def solve_many(systems, tol=1e-8, maxiter=None):
+def solve_many(systems, tol=1e-8, maxiter=None, form=None):
     """systems: iterable of (FESystem, F_batch (n_dof_i, k_i)).  Returns
     ([U_i (n_dof_i, k_i) device tensors], stats).  Every FESystem must be
-    assembled on the same device."""
+    assembled on the same device.  form: _lib.BATCHED_* or None (the batched
+    autotuner's choice)."""
     import torch
     batch = None
     uDs = []
@@ -147,6 +206,6 @@ def solve_many(systems, tol=1e-8, maxiter=None):
         raise ValueError("solve_many: no systems given")
     if int(flag.item()):
         raise ValueError("jacobi_preconditioner: A has a zero diagonal entry.")
-    X, stats = batch.solve(tol, maxiter)
+    X, stats = batch.solve(tol, maxiter, form)
     X = [x if uD is None else x + uD[:, None] for x, uD in zip(X, uDs)]
     return X, stats

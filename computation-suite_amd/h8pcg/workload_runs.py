@@ -134,10 +134,12 @@ def batched(a):
     setup_s = time.time() - t0
     dofs = [fs.n_dof for fs, F in systems for _ in range(F.shape[1])]
     free = [fs.n_dof - len(fs.fixed_dofs) for fs, F in systems for _ in range(F.shape[1])]
-    solve_many(systems[:2], tol=a.tol)          # warm-up (graph capture of a small batch)
+    from .batched import FORM_NAMES
+    form = None if a.form == "auto" else {v: k for k, v in FORM_NAMES.items()}[a.form]
+    solve_many(systems, tol=a.tol, form=form)   # warm-up (the batched autotuner's timing, graph capture)
     _sync()
     t1 = time.perf_counter()
-    X, st = solve_many(systems, tol=a.tol)
+    X, st = solve_many(systems, tol=a.tol, form=form)
     _sync()
     wall = time.perf_counter() - t1
     iters = st["iterations"]
@@ -145,7 +147,7 @@ def batched(a):
     print(json.dumps({"workload": "batched", "seed": a.seed, "instances": a.n, "systems": len(dofs),
                       "total_dof": int(sum(dofs)), "total_free_dof": int(sum(free)), "tol": a.tol,
                       "setup_s": setup_s, "wall_ms": 1e3 * wall, "device_ms": st["elapsed_ms"],
-                      "launches": st["launches"], "iterations_min": int(iters.min()),
+                      "launches": st["launches"], "form": st["form"], "iterations_min": int(iters.min()),
                       "iterations_max": int(iters.max()), "iterations_mean": float(iters.mean()),
                       "all_converged": bool(np.all(st["status"] == 0)),
                       "dof_iters_per_s": dof_iters / wall}), flush=True)
@@ -166,6 +168,8 @@ def main(argv=None):
     b.add_argument("--seed", type=int, default=4)
     b.add_argument("--n", type=int, default=256)
     b.add_argument("--tol", type=float, default=1e-8)
+    b.add_argument("--form", default="auto", choices=("auto", "launched", "persistent-8x16", "persistent-16x16"),
+                   help="batched form (auto: the batched autotuner's choice)")
     lp = sub.add_parser("simploop")
     lp.add_argument("--dims", default="96,32,16")
     lp.add_argument("--iters", type=int, default=20)

@@ -297,6 +297,28 @@ int h8_pcg_batched_f64(const h8_instance* inst, int32_t n_inst, int64_t total_no
                        void* ws, size_t ws_bytes, int32_t* status_out, int64_t* iters_out,
                        double* rel_out, void* stream);
 
+/* h8_pcg_batched_f64 in a chosen form (same arguments and results):
+ *   H8_BATCHED_LAUNCHED (0): h8_pcg_batched_f64 itself -- every system
+ *     advanced together, two launches per iteration in captured graphs;
+ *   H8_BATCHED_PERSISTENT_8X16 (1), _16X16 (2): one persistent launch; each
+ *     workgroup takes whole systems from a queue (largest first) and runs each
+ *     one's PCG to the end with every reduction inside the workgroup, the
+ *     direction kernel's sector march on 8 x 16 or 16 x 16 element columns.
+ * The iterations of every form are those of h8_pcg_f64 up to the summation
+ * order of the dot products.  The choice between them is the project's
+ * batched autotuner's (h8pcg.batched).  The workspace is
+ * h8_batched_workspace_bytes for every form.  Replaces, like
+ * h8_pcg_batched_f64, the per-model loop of batch.map (fe/batch.py:60-128)
+ * and the multi-RHS solve_static_batched (fe/batched_solve.py:105-164). */
+#define H8_BATCHED_LAUNCHED 0
+#define H8_BATCHED_PERSISTENT_8X16 1
+#define H8_BATCHED_PERSISTENT_16X16 2
+int h8_pcg_batched_form_f64(const h8_instance* inst, int32_t n_inst, int64_t total_nodes, int64_t total_elems,
+                            int64_t total_vec, const double* s_e, const double* K0, int32_t k0_per_instance,
+                            const double* dinv, const double* b, double* x, double tol, int64_t maxiter,
+                            void* ws, size_t ws_bytes, int32_t* status_out, int64_t* iters_out,
+                            double* rel_out, int32_t form, void* stream);
+
 /* Device time (ms) and graph launches of this thread's last batched solve. */
 double h8_batched_last_ms(void);
 int64_t h8_batched_last_launches(void);

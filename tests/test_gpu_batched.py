@@ -9,7 +9,11 @@ from oracle import h8_oracle as o
 pytestmark = pytest.mark.gpu
 
 
-def test_solve_static_batched_matches_oracle(gpu):
+FORMS = [0, 1, 2]   # launched (graphs of two kernels per iteration), persistent 8x16, persistent 16x16
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_solve_static_batched_matches_oracle(gpu, form):
     import h8pcg
     from h8pcg.batched import solve_static_batched
     nx, ny, nz = 10, 3, 3
@@ -22,8 +26,8 @@ def test_solve_static_batched_matches_oracle(gpu):
     rng = np.random.default_rng(1)
     Fb = rng.normal(size=(fs.n_dof, 5)) * 1e3
     Fb[:, 2] = 0.0                           # a column driven by the prescribed values only
-    U, st = solve_static_batched(fs, Fb, tol=1e-11, return_stats=True)
-    print("[gpu-test] batched stats", st["iterations"], st["elapsed_ms"], flush=True)
+    U, st = solve_static_batched(fs, Fb, tol=1e-11, return_stats=True, form=form)
+    print("[gpu-test] batched stats", st["form"], st["iterations"], st["elapsed_ms"], flush=True)
     c = o.cantilever(nx, ny, nz, E_elem=E, load="nodal", total=0.0)
     fixed = sorted(fs.fixed_dofs)
     vals = {d: fs.fixed_dof_values[d] for d in fixed}
@@ -34,7 +38,8 @@ def test_solve_static_batched_matches_oracle(gpu):
     assert np.all(st["status"] == 0)
 
 
-def test_uneven_instances_and_masks(gpu):
+@pytest.mark.parametrize("form", FORMS)
+def test_uneven_instances_and_masks(gpu, form):
     import torch
     import h8pcg
     from h8pcg.batched import solve_many
@@ -52,8 +57,10 @@ def test_uneven_instances_and_masks(gpu):
         systems.append((fs, F))
         c = o.cantilever(nx, ny, nz, E_elem=E, load="nodal", total=0.0)
         refs.append((c, F))
-    X, st = solve_many(systems, tol=1e-10)
-    print("[gpu-test] solve_many", st["iterations"].tolist(), st["elapsed_ms"], st["launches"], flush=True)
+    X, st = solve_many(systems, tol=1e-10, form=form)
+    print("[gpu-test] solve_many", st["form"], st["iterations"].tolist(), st["elapsed_ms"], st["launches"], flush=True)
+    if form != 0:
+        assert st["launches"] == 1
     assert np.all(st["status"] == 0)
     s = 0
     for (c, F), x in zip(refs, X):
@@ -172,3 +179,58 @@ def test_batched_graph_replays_follow_tol_and_maxiter(gpu):
     assert np.all(st12["iterations"] > st6["iterations"])
     with pytest.raises(RuntimeError, match="did not converge"):
         solve_many(systems, tol=1e-12, maxiter=5)
+
+
+def test_batched_forms_agree_and_the_tuner_caches_its_choice(gpu, tmp_path, monkeypatch):
+    """The three forms solve the same systems to the same tolerance with the
+    same recurrence (iteration counts within 2: only the summation order of the
+    dot products differs) and agree to 1e-9; the batched autotuner picks one of
+    them, records it for this batch composition and serves it from the cache."""
+    import json
+    from h8pcg import batched
+    from h8pcg.workloads import cantilever, tip_traction_loads
+    monkeypatch.setenv("H8PCG_TUNE_CACHE", str(tmp_path / "tune.json"))
+    rng = np.random.default_rng(11)
+    systems = []
+    for nx, k in [(24, 3), (8, 2), (16, 4), (12, 1)]:
+        ny = nz = nx // 4
+        E = 10 ** rng.uniform(9, 11, nx * ny * nz)
+        fs = cantilever(nx, ny, nz, E_elem=E, load="none")
+        systems.append((fs, tip_traction_loads(fs, nx, ny, nz, k, rng)))
+    out = {}
+    for form in FORMS:
+        X, st = batched.solve_many(systems, tol=1e-10, form=form)
+        assert np.all(st["status"] == 0)
+        out[form] = ([x.cpu().numpy() for x in X], st["iterations"].copy())
+    ref_x, ref_it = out[0]
+    for form in FORMS[1:]:
+        xs, its = out[form]
+        assert np.max(np.abs(its - ref_it)) <= 2, (form, its, ref_it)
+        for a, b in zip(xs, ref_x):
+            assert np.linalg.norm(a - b) <= 1e-9 * np.linalg.norm(b)
+    X, st = batched.solve_many(systems, tol=1e-10)        # tuned
+    data = json.load(open(tmp_path / "tune.json"))
+    keys = [k for k in data if "|batched|" in k]
+    print("[gpu-test] batched tuner", st["form"], data[keys[0]], flush=True)
+    assert len(keys) == 1 and data[keys[0]][0] in FORMS
+    assert st["form"] == batched.FORM_NAMES[data[keys[0]][0]]
+    X2, st2 = batched.solve_many(systems, tol=1e-10)      # served from the cache
+    assert st2["form"] == st["form"]
+    assert all(np.array_equal(a.cpu().numpy(), b.cpu().numpy()) for a, b in zip(X, X2))
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_persistent_form_stops_at_maxiter(gpu, form):
+    from h8pcg import batched
+    from h8pcg.workloads import cantilever, tip_traction_loads
+    rng = np.random.default_rng(12)
+    fs = cantilever(16, 4, 4, E_elem=10 ** rng.uniform(9, 11, 256), load="none")
+    F = tip_traction_loads(fs, 16, 4, 4, 3, rng)
+    with pytest.raises(RuntimeError):
+        batched.solve_many([(fs, F)], tol=1e-12, maxiter=7, form=form)
+    fs._op.set_free_mask(fs._free_mask())
+    cols, _ = batched._dirichlet_columns(fs, F)
+    b = batched._Batch(fs._op.device)
+    b.add(fs._op, cols)
+    _, st = b._solve(1e-12, 7, form, check_rc=False)
+    assert np.all(st["iterations"] == 7) and np.all(st["status"] != 0)
