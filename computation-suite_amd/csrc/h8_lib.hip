@@ -19,6 +19,7 @@
 #include "h8_node.h"
 #include "h8_stream.h"
 #include "h8_mfma.h"
+#include "h8_hex20.h"
 #include "h8_batched.h"
 #include "h8_sector.h"
 #include "h8_resident.h"
@@ -2586,3 +2587,169 @@ extern "C" int h8_pcg_batched_form_f64(const h8_instance* inst, int32_t n_inst, 
 
 extern "C" double h8_batched_last_ms(void) { return g_batched_ms; }
 extern "C" int64_t h8_batched_last_launches(void) { return g_batched_launches; }
+
+// ---------------------------------------------------------------------------
+// Hex20: element stiffness, and the structured-box operator, diagonal and PCG
+// (h8_hex20.h)
+// ---------------------------------------------------------------------------
+extern "C" int h8_ke20_batched_f64(const double* X, const double* D, int64_t d_stride, int64_t m, double* ke,
+                                   void* stream) {
+    if (m < 0) return fail(H8_ERR_ARG, "m < 0");
+    if (m == 0) return H8_OK;
+    if (!X || !D || !ke) return fail(H8_ERR_ARG, "NULL pointer");
+    if (d_stride != 0 && d_stride != 36) return fail(H8_ERR_ARG, "d_stride must be 0 or 36");
+    if (m >= (1LL << 31)) return fail(H8_ERR_ARG, "too many elements");
+    hipLaunchKernelGGL(k_ke20, dim3(unsigned(m)), dim3(256), 0, (hipStream_t)stream, (long long)m, X, D,
+                       (long long)d_stride, ke);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+static unsigned h20_blocks(const H20Grid& g) {
+    return unsigned(std::min<long long>((g.nnodes + 63) / 64, H20_MAX_BLOCKS));
+}
+
+static int h20_args(int32_t nx, int32_t ny, int32_t nz, H20Grid& g) {
+    if (nx < 1 || ny < 1 || nz < 1) return fail(H8_ERR_ARG, "hex20 box: nx, ny, nz must be >= 1");
+    g = h20_grid(nx, ny, nz);
+    if (g.ndof >= (1LL << 31)) return fail(H8_ERR_ARG, "hex20 box: %lld DOFs, the limit is 2^31", g.ndof);
+    return H8_OK;
+}
+
+// y = K u (diag: y = the diagonal of K) over every DOF of the box, one launch
+static void h20_node_launch(const H20Grid& g, bool diag, const double* s_e, const double* K0, const double* u,
+                            double* y, hipStream_t s) {
+    const unsigned blocks = h20_blocks(g);
+    if (diag)
+        hipLaunchKernelGGL(k_h20_node<true>, dim3(blocks), dim3(H20_THREADS), 0, s, g, s_e, K0, u, y);
+    else
+        hipLaunchKernelGGL(k_h20_node<false>, dim3(blocks), dim3(H20_THREADS), 0, s, g, s_e, K0, u, y);
+}
+
+extern "C" int h20_spmv_f64(int32_t nx, int32_t ny, int32_t nz, const double* s_e, const double* K0,
+                            const double* u, double* y, void* stream) {
+    H20Grid g;
+    int rc = h20_args(nx, ny, nz, g);
+    if (rc) return rc;
+    if (!s_e || !K0 || !u || !y) return fail(H8_ERR_ARG, "NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    h20_node_launch(g, false, s_e, K0, u, y, s);
+    LAUNCHCHK();
+    return H8_OK;
+}
+
+extern "C" int h20_diag_f64(int32_t nx, int32_t ny, int32_t nz, const double* s_e, const double* K0,
+                            const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream) {
+    H20Grid g;
+    int rc = h20_args(nx, ny, nz, g);
+    if (rc) return rc;
+    if (!s_e || !K0 || !free_mask || !dinv || !d_flag) return fail(H8_ERR_ARG, "NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const int sb = stream_blocks(g.ndof);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(STREAM_THREADS), 0, s, 1LL, (int*)d_flag, 0);
+    h20_node_launch(g, true, s_e, K0, nullptr, dinv, s);
+    hipLaunchKernelGGL(k_h20_dinv, dim3(sb), dim3(256), 0, s, g.ndof, (const double*)dinv, free_mask, dinv,
+                       (int*)d_flag);
+    LAUNCHCHK();
+    int32_t flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, d_flag, sizeof flag, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (flag) return fail(H8_ZERO_DIAG, "jacobi_preconditioner: A has a zero diagonal entry.");
+    return H8_OK;
+}
+
+static size_t h20_ws_bytes(const H20Grid& g) {
+    return 4 * align_up(sizeof(double) * g.ndof) + align_up(sizeof(double) * 4 * 2048) + align_up(sizeof(PcgState));
+}
+
+extern "C" size_t h20_pcg_workspace_bytes(int32_t nx, int32_t ny, int32_t nz) {
+    H20Grid g;
+    if (h20_args(nx, ny, nz, g)) return 0;
+    return h20_ws_bytes(g);
+}
+
+extern "C" int h20_pcg_f64(int32_t nx, int32_t ny, int32_t nz, const double* s_e, const double* K0,
+                           const double* dinv, const double* b, double* x, double tol, int64_t maxiter, void* ws,
+                           size_t ws_bytes, h8_pcg_stats* stats, void* stream) {
+    H20Grid g;
+    int rc = h20_args(nx, ny, nz, g);
+    if (rc) return rc;
+    if (!s_e || !K0 || !dinv || !b || !x || !ws) return fail(H8_ERR_ARG, "NULL pointer");
+    if (!(tol >= 0.0)) return fail(H8_ERR_ARG, "tol must be >= 0");
+    if (maxiter < 0) return fail(H8_ERR_ARG, "maxiter must be >= 0");
+    if (ws_bytes < h20_ws_bytes(g))
+        return fail(H8_ERR_WORKSPACE, "workspace too small: %zu < %zu", ws_bytes, h20_ws_bytes(g));
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)ws;
+    double* r = (double*)carve(p, sizeof(double) * g.ndof);
+    double* p0 = (double*)carve(p, sizeof(double) * g.ndof);
+    double* p1 = (double*)carve(p, sizeof(double) * g.ndof);
+    double* q = (double*)carve(p, sizeof(double) * g.ndof);
+    double* partials = (double*)carve(p, sizeof(double) * 4 * 2048);
+    PcgState* st = (PcgState*)carve(p, sizeof(PcgState));
+    const long long n = g.ndof;
+    const int sb = stream_blocks(n);
+    Ev e0, e1;
+    HIPCHK(e0.create());
+    HIPCHK(e1.create());
+    HIPCHK(hipEventRecord(e0, s));
+    PcgState hst{};
+    hst.tol = tol;
+    hst.maxiter = maxiter;
+    hst.status = ST_RUNNING;
+    HIPCHK(hipMemcpyAsync(st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, n, x, 0.0);
+    hipLaunchKernelGGL(k_pcg_init<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, n, b, (const double*)nullptr, dinv,
+                       r, st, partials);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    long long launches = 0;
+    if (hst.status == ST_RUNNING && maxiter > 0) {
+        GraphKey key{};
+        key.kind = 6;
+        HIPCHK(hipGetDevice(&key.dev));
+        key.nx = nx; key.ny = ny; key.nz = nz;
+        const void* ptrs[] = {s_e, K0, dinv, x, r, p0, p1, q, st, partials};
+        std::memcpy(key.ptr, ptrs, sizeof ptrs);
+        key.chunk = chunk_for(maxiter);
+        GraphRef exec;
+        rc = get_graph(key, [&](hipStream_t cs) -> int {
+            for (int it = 0; it < key.chunk; ++it) {
+                hipLaunchKernelGGL(k_h20_dir, dim3(sb), dim3(256), 0, cs, n, (const double*)r, dinv, p0, p1,
+                                   (const PcgState*)st);
+                hipLaunchKernelGGL(k_h20_node_pcg, dim3(h20_blocks(g)), dim3(H20_THREADS), 0, cs, g, s_e, K0,
+                                   (const double*)p0, (const double*)p1, q, st, partials);
+                hipLaunchKernelGGL((k_pcg_update<double, false, true>), dim3(sb), dim3(STREAM_THREADS), 0, cs, n, x,
+                                   r, (const double*)p0, (const double*)p1, (const double*)q, dinv, st, partials);
+            }
+            LAUNCHCHK();
+            return H8_OK;
+        }, &exec);
+        if (rc) return rc;
+        for (;;) {
+            HIPCHK(hipGraphLaunch(exec->exec, s));
+            ++launches;
+            HIPCHK(hipMemcpyAsync(&hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (hst.status != ST_RUNNING) break;
+        }
+    } else if (hst.status == ST_RUNNING) {
+        hst.status = ST_MAXITER;
+        hst.iters = 0;
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    fill_stats(stats, hst, ms, launches);
+    if (hst.status == ST_NOT_SPD)
+        return fail(H8_NOT_SPD, "preconditioned_cg: p^T A p = %.3e <= 0 at iteration %lld -- A is not SPD.",
+                    hst.bad_pq, hst.iters);
+    if (hst.status == ST_MAXITER)
+        return fail(H8_MAXITER,
+                    "preconditioned_cg: did not converge in %lld iterations (final relative residual "
+                    "%.3e >= tol=%.1e).",
+                    (long long)maxiter, hst.rel, tol);
+    return H8_OK;
+}

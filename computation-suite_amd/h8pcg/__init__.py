@@ -9,6 +9,7 @@ builds meshes, loads, the constraint map and the rigid-mode basis.
 from .material import Material, D_solid3d
 from .mesh import Mesh, box_mesh, structured_grid
 from .elements import Hex8Solid3D, Hex8SolidBbar
+from .hex20 import Hex20Solid3D, box_mesh_hex20
 from . import coefficients
 from .fields import FEField
 from .facet_loads import list_facets, boundary_faces, face_load_shares
@@ -16,7 +17,7 @@ from .solver import FESystem, fesystem_solve_static_torch
 from ._lib import NotSPDError, H8LibraryMissing
 from .batched import solve_static_batched, solve_many
 
-__all__ = ["Material", "D_solid3d", "Mesh", "box_mesh", "structured_grid", "Hex8Solid3D", "Hex8SolidBbar", "coefficients",
+__all__ = ["Material", "D_solid3d", "Mesh", "box_mesh", "structured_grid", "Hex8Solid3D", "Hex8SolidBbar", "Hex20Solid3D", "box_mesh_hex20", "coefficients",
            "FEField", "list_facets", "boundary_faces", "face_load_shares", "FESystem", "fesystem_solve_static_torch",
            "NotSPDError",
            "H8LibraryMissing", "solve_static_batched", "solve_many"]

@@ -106,6 +106,12 @@ SIGNATURES = {
                                           _P, C.c_int32, _P, _P, _P, _D, _I64, _P, C.c_size_t, _P, _P, _P,
                                           C.c_int32, _P]),
     "h8_batched_last_ms": (C.c_double, []),
+    "h8_ke20_batched_f64": (C.c_int, [_P, _P, _I64, _I64, _P, _P]),
+    "h20_spmv_f64": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P]),
+    "h20_diag_f64": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P]),
+    "h20_pcg_workspace_bytes": (C.c_size_t, [C.c_int32, C.c_int32, C.c_int32]),
+    "h20_pcg_f64": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _D, _I64, _P, C.c_size_t, _P,
+                              _P]),
     "h8_batched_last_launches": (C.c_int64, []),
     "h8_recovery_table_f64": (C.c_int, [_D, _D, _D, _P, _P, _P]),
     "h8_recover_f64": (C.c_int, [_GRID, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -86,9 +86,12 @@ class FESystem:
                 f"FESystem: unknown backend={backend!r} -- this package provides backend='hip' "
                 f"(matrix-free Jacobi-PCG on an AMD GPU); the reference's 'scipy', 'torch' and "
                 f"'auto' backends live in fea_engine itself.")
-        if not isinstance(elem_formulation, Hex8Solid3D):
+        from .hex20 import Hex20Solid3D
+        if not isinstance(elem_formulation, (Hex8Solid3D, Hex20Solid3D)):
             raise NotImplementedError(
-                f"FESystem(backend='hip') supports Hex8Solid3D only, got {type(elem_formulation).__name__}")
+                f"FESystem(backend='hip') supports Hex8Solid3D, Hex8SolidBbar and Hex20Solid3D, got "
+                f"{type(elem_formulation).__name__}")
+        self._hex20 = isinstance(elem_formulation, Hex20Solid3D)
         _lib.lib()   # fail fast when the device library is missing (as backend='torch' does for torch)
         self.backend = backend
         self.device = device
@@ -153,6 +156,9 @@ class FESystem:
         dof_index = int(self._check_local_dofs(dof_index, "add_consistent_facet_load")[0])
         if not facets:
             return
+        if self._hex20:
+            raise NotImplementedError("add_consistent_facet_load with Hex20Solid3D (8-node faces) is not "
+                                      "available on backend='hip'; apply nodal forces (add_nodal_force).")
         if any(len(f) != 4 for f in facets):
             raise ValueError("add_consistent_facet_load: Hex8 faces have 4 nodes")
         ids = np.asarray(facets, dtype=int)
@@ -216,7 +222,10 @@ class FESystem:
         kw.pop("device", None)
         if kw:
             raise TypeError(f"assemble_stiffness: unsupported keyword(s) {sorted(kw)}")
-        if gauss_order not in (None, 2):
+        if self._hex20:
+            if gauss_order not in (None, 3):
+                raise NotImplementedError("backend='hip' integrates Hex20 with the 3x3x3 rule only")
+        elif gauss_order not in (None, 2):
             raise NotImplementedError("backend='hip' integrates Hex8 with the 2x2x2 rule only")
         self._begin_assembly(accumulate, "assemble_stiffness")
         m = len(self.mesh.elements)
@@ -235,21 +244,30 @@ class FESystem:
             if es.shape != (m,):
                 raise ValueError(f"element_scale must have one entry per element ({m})")
             s = s * es
-        st = self._structure if self._structure is not None else structured_grid(self.mesh)
-        if st is None:
-            raise NotImplementedError(
-                "FESystem(backend='hip'): the matrix-free operator needs a structured box mesh "
-                "(box_mesh connectivity with congruent bricks)")
+        if self._hex20:
+            from .hex20 import structured_hex20, Hex20Operator
+            st = self._structure if self._structure is not None else structured_hex20(self.mesh)
+            if st is None:
+                raise NotImplementedError(
+                    "FESystem(backend='hip') with Hex20Solid3D needs a structured box mesh "
+                    "(box_mesh_hex20 connectivity with congruent bricks)")
+            operator_cls = Hex20Operator
+        else:
+            st = self._structure if self._structure is not None else structured_grid(self.mesh)
+            if st is None:
+                raise NotImplementedError(
+                    "FESystem(backend='hip'): the matrix-free operator needs a structured box mesh "
+                    "(box_mesh connectivity with congruent bricks)")
+            from .operator import GridOperator as operator_cls
         self._structure = st
         nx, ny, nz, coords0 = st
-        from .operator import GridOperator
         if accumulate and self._op is not None:
             ratio = _proportional(self._D_ref, D_ref)
             s = self._op.s_e.cpu().numpy() + ratio * s
             self._op.set_scale(s)
         else:
             ke0 = self.elem.stiffness_batched(coords0[None], D_ref, device=self.device)[0]
-            self._op = GridOperator(nx, ny, nz, ke0, s, device=self.device)
+            self._op = operator_cls(nx, ny, nz, ke0, s, device=self.device)
             self._D_ref = D_ref
         self._assembled.add("K")
         self._K_cache = None
@@ -271,6 +289,8 @@ class FESystem:
         """The assembled global stiffness as scipy CSR, built on demand on the GPU
         (COO values s_e*K0 summed by torch's coalesce) and copied to the host."""
         self._require_stiffness("K")
+        if self._K_cache is None and self._hex20:
+            self._K_cache = self._op.csr(np.asarray(self.mesh.elements))
         if self._K_cache is None:
             self._K_cache = _materialize_csr(self._op, np.asarray(self.mesh.elements))
         return self._K_cache
@@ -460,6 +480,9 @@ class FESystem:
         choice; 'mg' with precision='mixed'), 'jacobi', or 'mg' (geometric
         multigrid V-cycle, precision='mixed' only)."""
         self._require_stiffness("solve_static")
+        if self._hex20 and (precision != "fp64" or preconditioner == "mg" or self._constraints):
+            raise NotImplementedError("Hex20Solid3D on backend='hip' solves with the FP64 Jacobi-PCG "
+                                      "(precision='fp64', no linear constraints).")
         if preconditioner is None:
             preconditioner = "mg" if precision == "mixed" and method in (None, "pcg") else "jacobi"
         if self._constraints:
@@ -583,6 +606,8 @@ class FESystem:
         """Device recovery (h8_recover_f64): (nodal (N,6), vm_nodal, elem (m,6),
         vm_elem) as numpy; D of the last assemble_stiffness unless given
         (fe/recovery.py:36-42); per-element coefficients become a scale of one D."""
+        if self._hex20:
+            raise NotImplementedError(f"{who} with Hex20Solid3D is not available on backend='hip'.")
         if at not in ("nodes", "elements"):
             raise ValueError("at must be 'nodes' or 'elements'.")
         U = np.asarray(U)

@@ -411,6 +411,46 @@ int h8_oc_update_f64(const double* rho, const double* dc, int64_t m, double volu
                      int32_t max_bisection_iter, double* rho_new, double* change, void* ws,
                      size_t ws_bytes, void* stream);
 
+/* ---- Hex20: the 20-node serendipity hexahedron ------------------------- */
+/* ke[e] = sum over the 3x3x3 Gauss rule of B^T D B detJ w for the 20-node
+ * element, general geometry.  X: (m,20,3) nodal coordinates in the element's
+ * node order (corners 0-7 as Hex8, mid-edge nodes 8-11 on the bottom edges
+ * (0,1) (1,2) (2,3) (3,0), 12-15 on the top edges (4,5) (5,6) (6,7) (7,4),
+ * 16-19 on the vertical edges (0,4) .. (3,7)); D: (6,6) when d_stride == 0,
+ * else (m,36) with d_stride = 36; ke: (m,60,60).  Replaces Element.stiffness
+ * + Hex20Solid3D.shape_functions / B_matrix (fe/elements/base.py:494-514,
+ * fe/elements/solids.py:331-406). */
+int h8_ke20_batched_f64(const double* X, const double* D, int64_t d_stride, int64_t m,
+                        double* ke, void* stream);
+
+/* The Hex20 box of nx x ny x nz congruent bricks: nodes numbered corners
+ * (i,j,k) -> (k NI + i) NJ + j, then x-edge midpoints Nc + (k nx + i) NJ + j,
+ * y-edge midpoints Nc + Nx + (k NI + i) ny + j, z-edge midpoints
+ * Nc + Nx + Ny + (k NI + i) NJ + j (NI = nx+1, NJ = ny+1, NK = nz+1,
+ * Nc = NI NJ NK, Nx = nx NJ NK, Ny = NI ny NK); DOF 3 node + c; element
+ * (i,j,k) -> (k nx + i) ny + j.  K0: the (60,60) brick stiffness
+ * (h8_ke20_batched_f64), s_e: per-element scale.  y = K u over every DOF,
+ * in 8 parity-colour passes (deterministic).  Replaces the CSR product A @ p
+ * of preconditioned_cg on a Hex20 mesh (fe/iterative_solvers.py:439-441). */
+int h20_spmv_f64(int32_t nx, int32_t ny, int32_t nz, const double* s_e, const double* K0,
+                 const double* u, double* y, void* stream);
+
+/* dinv for the Hex20 box as h8_diag_f64 (1/K_ii at free DOFs, 0 at fixed;
+ * H8_ZERO_DIAG on a zero free diagonal).  Replaces jacobi_preconditioner
+ * (fe/iterative_solvers.py:193-202) on a Hex20 mesh. */
+int h20_diag_f64(int32_t nx, int32_t ny, int32_t nz, const double* s_e, const double* K0,
+                 const uint8_t* free_mask, double* dinv, int32_t* d_flag, void* stream);
+
+/* Workspace (bytes) of h20_pcg_f64; 0 for an invalid box. */
+size_t h20_pcg_workspace_bytes(int32_t nx, int32_t ny, int32_t nz);
+
+/* Jacobi-PCG on the Hex20 box, FP64, x0 = 0: the same iteration, stopping
+ * test and errors as h8_pcg_f64 (preconditioned_cg, fe/iterative_solvers.py:
+ * 434-469): b is zero at fixed DOFs, dinv from h20_diag_f64. */
+int h20_pcg_f64(int32_t nx, int32_t ny, int32_t nz, const double* s_e, const double* K0,
+                const double* dinv, const double* b, double* x, double tol, int64_t maxiter,
+                void* ws, size_t ws_bytes, h8_pcg_stats* stats, void* stream);
+
 /* ---- measurement ------------------------------------------------------- */
 /* Average device time (ms) of the two kernels of one PCG iteration, each
  * launched `reps` times alone between HIP events on `stream`: avg_ms[0] = the

@@ -69,6 +69,16 @@ PINNING (function -> independent checks):
                           polynomial T2((theta - lam)/delta) / T2(theta/delta);
                           the V-cycle is symmetric positive definite and PCG
                           with it reaches spsolve's solution (tests/test_oracle_mg.py)
+  hex20_shape_and_derivs / hex20_B / hex20_ke / hex20_box_by_midpoints /
+  assemble_csr_npe :      partition of unity, Kronecker property at the 20
+                          nodes, the isoparametric identity, derivatives = central
+                          FD of N; ke symmetric with the 6 rigid-body modes in its
+                          null space and rank 54, the constant-strain patch test
+                          (energy V eps^T D eps, B u = eps at the Gauss points), the
+                          brick integrated exactly by the 3x3x3 rule (equal to
+                          4x4x4); the library's box numbering equals the
+                          midpoint construction node by node (tests/test_oracle_hex20.py);
+                          assemble_csr_npe through the GPU parity of K u
 """
 from __future__ import annotations
 
@@ -891,3 +901,101 @@ def element_matvec(elements, ke, scale, u, chunk=65536):
         ye = (u[dofs] @ ke.T) * np.asarray(scale[c0:c0 + chunk], dtype=float)[:, None]
         out += np.bincount(dofs.ravel(), weights=ye.ravel(), minlength=len(u))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Hex20 (20-node serendipity hexahedron): fe/elements/solids.py:331-406 with
+# the generic Gauss loop of fe/elements/base.py:494-514 at gauss_order 3
+# ---------------------------------------------------------------------------
+# natural coordinates of the 20 nodes (fe/elements/solids.py:347-352)
+XI20 = np.array([-1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1, -1], dtype=float)
+ETA20 = np.array([-1, -1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1], dtype=float)
+ZETA20 = np.array([-1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1, 0, 0, 0, 0], dtype=float)
+
+
+def hex20_shape_and_derivs(xi, eta, zeta):
+    """N (20,) and dN/d(xi,eta,zeta) (3,20): serendipity shape functions
+    (fe/elements/solids.py:354-385), vectorised over the node kinds."""
+    a, b, c = xi * XI20, eta * ETA20, zeta * ZETA20
+    corner = (XI20 != 0) & (ETA20 != 0) & (ZETA20 != 0)
+    ex, ey, ez = XI20 == 0, ETA20 == 0, ZETA20 == 0
+    N = np.where(corner, 0.125 * (1 + a) * (1 + b) * (1 + c) * (a + b + c - 2), 0.0)
+    N = np.where(ex, 0.25 * (1 - xi ** 2) * (1 + b) * (1 + c), N)
+    N = np.where(ey, 0.25 * (1 + a) * (1 - eta ** 2) * (1 + c), N)
+    N = np.where(ez, 0.25 * (1 + a) * (1 + b) * (1 - zeta ** 2), N)
+    dx = np.where(corner, 0.125 * XI20 * (1 + b) * (1 + c) * (2 * a + b + c - 1), 0.0)
+    dy = np.where(corner, 0.125 * ETA20 * (1 + a) * (1 + c) * (a + 2 * b + c - 1), 0.0)
+    dz = np.where(corner, 0.125 * ZETA20 * (1 + a) * (1 + b) * (a + b + 2 * c - 1), 0.0)
+    dx = np.where(ex, -0.5 * xi * (1 + b) * (1 + c), dx)
+    dy = np.where(ex, 0.25 * (1 - xi ** 2) * ETA20 * (1 + c), dy)
+    dz = np.where(ex, 0.25 * (1 - xi ** 2) * (1 + b) * ZETA20, dz)
+    dx = np.where(ey, 0.25 * XI20 * (1 - eta ** 2) * (1 + c), dx)
+    dy = np.where(ey, -0.5 * eta * (1 + a) * (1 + c), dy)
+    dz = np.where(ey, 0.25 * (1 + a) * (1 - eta ** 2) * ZETA20, dz)
+    dx = np.where(ez, 0.25 * XI20 * (1 + b) * (1 - zeta ** 2), dx)
+    dy = np.where(ez, 0.25 * (1 + a) * ETA20 * (1 - zeta ** 2), dy)
+    dz = np.where(ez, -0.5 * zeta * (1 + a) * (1 + b), dz)
+    return N, np.vstack([dx, dy, dz])
+
+
+def hex20_B(xi, eta, zeta, X):
+    """B (6, 60) and detJ at one point (fe/elements/solids.py:387-401): Voigt
+    rows xx, yy, zz, xy, yz, zx, columns 3 a + c."""
+    _, dN = hex20_shape_and_derivs(xi, eta, zeta)
+    J = dN @ X
+    G = np.linalg.solve(J, dN)
+    B = np.zeros((6, 60))
+    B[0, 0::3], B[1, 1::3], B[2, 2::3] = G[0], G[1], G[2]
+    B[3, 0::3], B[3, 1::3] = G[1], G[0]
+    B[4, 1::3], B[4, 2::3] = G[2], G[1]
+    B[5, 0::3], B[5, 2::3] = G[2], G[0]
+    return B, np.linalg.det(J)
+
+
+def hex20_ke(X, D, order=3):
+    """sum over the order^3 Gauss rule of B^T D B detJ w (fe/elements/base.py:494-514)."""
+    pts, wts = gauss_points(order)
+    ke = np.zeros((60, 60))
+    for (xi, eta, zeta), w in zip(pts, wts):
+        B, det = hex20_B(xi, eta, zeta, np.asarray(X, dtype=float))
+        ke += B.T @ D @ B * det * w
+    return ke
+
+
+def hex20_box_by_midpoints(Lx, Ly, Lz, nx, ny, nz):
+    """A Hex20 box built as the reference's own tests build quadratic meshes
+    (fe/tests/test_benchmarks_convergence.py:42-62): the Hex8 box, then per
+    element the mid-edge nodes at the straight-edge midpoints, numbered in
+    order of first appearance and shared through their coordinates."""
+    nodes8, elems8 = box_mesh(Lx, Ly, Lz, nx, ny, nz)
+    nodes = [tuple(x) for x in nodes8]
+    index = {tuple(np.round(x, 9)): i for i, x in enumerate(nodes8)}
+    conn = []
+    for e in elems8:
+        X = nodes8[e]
+        row = list(e)
+        for a in range(8, 20):
+            N8, _ = hex8_shape_and_derivs(XI20[a], ETA20[a], ZETA20[a])
+            x = N8 @ X
+            key = tuple(np.round(x, 9))
+            if key not in index:
+                index[key] = len(nodes)
+                nodes.append(tuple(x))
+            row.append(index[key])
+        conn.append(row)
+    return np.array(nodes, dtype=float), np.array(conn, dtype=np.int64)
+
+
+def assemble_csr_npe(n_nodes, elements, ke):
+    """Global K (CSR) of elements with any node count: COO duplicate-sum of ke
+    (m, 3 npe, 3 npe) over the element DOF tables (3 node + c)."""
+    elements = np.asarray(elements)
+    m, npe = elements.shape
+    nd = 3 * npe
+    g = (3 * elements[:, :, None] + np.arange(3)).reshape(m, nd)
+    rows = np.repeat(g, nd, axis=1).ravel()
+    cols = np.tile(g, (1, nd)).ravel()
+    n = 3 * n_nodes
+    K = sp.coo_matrix((np.asarray(ke).reshape(m, nd * nd).ravel(), (rows, cols)), shape=(n, n)).tocsr()
+    K.sum_duplicates()
+    return K
