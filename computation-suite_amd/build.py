@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "h8pcg", "libh8pcg.so")
 SOURCES = ["h8_lib.hip"]
-HEADERS = ["h8_device.h", "h8_node.h", "h8_stream.h", "h8_mfma.h", "h8_batched.h", "h8_sector.h", "h8_resident.h", "h8_recovery.h", "h8_topopt.h", "h8_constrained.h", "h8_mg.h"]
+HEADERS = ["h8_device.h", "h8_node.h", "h8_stream.h", "h8_mfma.h", "h8_batched.h", "h8_sector.h", "h8_resident.h", "h8_recovery.h", "h8_topopt.h", "h8_constrained.h", "h8_mg.h", "h8_hex20.h"]
 
 
 def hipcc() -> str:

@@ -19,8 +19,8 @@
 //
 // The operator y = sum_e s_e K0 u_e is node-centric: one launch, each row
 // gathered from the elements around its node in a fixed order (no atomics,
-// deterministic), with the K0 row of a (slot, component) pair uniform across a
-// wave so it is read through the scalar cache.
+// deterministic), with the K0 rows of a slot uniform across a wave so they are
+// read through the scalar cache.
 #pragma once
 #include "h8_device.h"
 
@@ -185,7 +185,8 @@ k_ke20(long long m, const double* __restrict__ X, const double* __restrict__ D, 
 // up to 8 (corner) or 4 (edge) elements around it; slot s names one of them:
 // its offset (di, dj, dk) from (i, j, k) and the local index a of the node in
 // it, the same for every node of the family -- so a K0 row address depends on
-// (s, c) only, which are uniform across a wave.
+// s only, which is uniform across a wave (the rows are read through the
+// scalar cache).
 struct H20Slot {
     int di, dj, dk, a;
 };
@@ -226,27 +227,28 @@ __device__ __forceinline__ int h20_node32(const H20Grid& g, int i, int j, int k,
     return int(g.nc + g.nxe + g.nye) + (k * g.NI + gi) * g.NJ + gj;
 }
 
-// Component c (wave-uniform) of row n (family FAM at (i, j, k)): the sum over
-// the elements around the node of s_e K0[3a + c][.] u_e; DIAG: of s_e
-// K0[3a + c][3a + c].  Elements in a fixed order, each one's 60 products in a
-// fixed order: the result does not depend on the launch.
+// Rows 3n .. 3n+2 of node n (family FAM at (i, j, k)): the sum over the
+// elements around the node of s_e K0[3a + c][.] u_e; DIAG: of s_e
+// K0[3a + c][3a + c].  The element's 60 values are loaded once for the three
+// rows (all loads in flight before the products); elements in a fixed order,
+// each one's 60 products in a fixed order: the result does not depend on the
+// launch.
 template <int FAM, bool DIAG>
-__device__ __forceinline__ double h20_row(const H20Grid& g, int i, int j, int k, int c,
-                                          const double* __restrict__ s_e, const double* __restrict__ K0,
-                                          const double* __restrict__ u) {
+__device__ __forceinline__ void h20_rows(const H20Grid& g, int i, int j, int k,
+                                         const double* __restrict__ s_e, const double* __restrict__ K0,
+                                         const double* __restrict__ u, double (&acc)[3]) {
     constexpr int NS = FAM == 0 ? 8 : 4;
-    double acc = 0.0;
 #pragma unroll 1
     for (int s = 0; s < NS; ++s) {
         const H20Slot sl = h20_slot(FAM, s);
         const int ei = i + sl.di, ej = j + sl.dj, ek = k + sl.dk;
         if (ei < 0 || ej < 0 || ek < 0 || ei >= g.nx || ej >= g.ny || ek >= g.nz) continue;
         const double se = s_e[(ek * g.nx + ei) * g.ny + ej];
-        const double* __restrict__ Kr = K0 + (3 * sl.a + c) * 60;
+        const double* __restrict__ Kr = K0 + 3 * sl.a * 60;
         if constexpr (DIAG) {
-            acc += se * Kr[3 * sl.a + c];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] += se * Kr[c * 60 + 3 * sl.a + c];
         } else {
-            // the element's 60 values first (all loads in flight), then the row
             double ue[60];
 #pragma unroll
             for (int b = 0; b < 20; ++b) {
@@ -255,40 +257,46 @@ __device__ __forceinline__ double h20_row(const H20Grid& g, int i, int j, int k,
                 ue[3 * b + 1] = ub[1];
                 ue[3 * b + 2] = ub[2];
             }
-            double e = 0.0;
 #pragma unroll
-            for (int t = 0; t < 60; ++t) e += Kr[t] * ue[t];
-            acc += se * e;
+            for (int c = 0; c < 3; ++c) {
+                double e = 0.0;
+#pragma unroll
+                for (int t = 0; t < 60; ++t) e += Kr[c * 60 + t] * ue[t];
+                acc[c] += se * e;
+            }
         }
     }
-    return acc;
 }
 
 template <bool DIAG>
-__device__ __forceinline__ double h20_node_row(const H20Grid& g, int n, int c, const double* __restrict__ s_e,
-                                               const double* __restrict__ K0, const double* __restrict__ u) {
+__device__ __forceinline__ void h20_node_rows(const H20Grid& g, int n, const double* __restrict__ s_e,
+                                              const double* __restrict__ K0, const double* __restrict__ u,
+                                              double (&acc)[3]) {
+    acc[0] = acc[1] = acc[2] = 0.0;
     if (n < g.nc) {
         const int j = n % g.NJ, t = n / g.NJ;
-        return h20_row<0, DIAG>(g, t % g.NI, j, t / g.NI, c, s_e, K0, u);
+        h20_rows<0, DIAG>(g, t % g.NI, j, t / g.NI, s_e, K0, u, acc);
+        return;
     }
     n -= int(g.nc);
     if (n < g.nxe) {
         const int j = n % g.NJ, t = n / g.NJ;
-        return h20_row<1, DIAG>(g, t % g.nx, j, t / g.nx, c, s_e, K0, u);
+        h20_rows<1, DIAG>(g, t % g.nx, j, t / g.nx, s_e, K0, u, acc);
+        return;
     }
     n -= int(g.nxe);
     if (n < g.nye) {
         const int j = n % g.ny, t = n / g.ny;
-        return h20_row<2, DIAG>(g, t % g.NI, j, t / g.NI, c, s_e, K0, u);
+        h20_rows<2, DIAG>(g, t % g.NI, j, t / g.NI, s_e, K0, u, acc);
+        return;
     }
     n -= int(g.nye);
     const int j = n % g.NJ, t = n / g.NJ;
-    return h20_row<3, DIAG>(g, t % g.NI, j, t / g.NI, c, s_e, K0, u);
+    h20_rows<3, DIAG>(g, t % g.NI, j, t / g.NI, s_e, K0, u, acc);
 }
 
-// Block shape of the node kernels: 3 waves over 64 consecutive nodes, wave c
-// computing component c; at most H20_MAX_BLOCKS blocks, striding over the nodes.
-constexpr int H20_THREADS = 192;
+// One thread per node; at most H20_MAX_BLOCKS blocks, striding over the nodes.
+constexpr int H20_THREADS = 256;
 constexpr int H20_MAX_BLOCKS = 2048;
 
 // y = K u (DIAG: y = the diagonal of K), every DOF.
@@ -296,11 +304,12 @@ template <bool DIAG>
 __global__ void __launch_bounds__(H20_THREADS)
 k_h20_node(H20Grid g, const double* __restrict__ s_e, const double* __restrict__ K0,
            const double* __restrict__ u, double* __restrict__ y) {
-    const int c = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    for (long long n0 = (long long)blockIdx.x * 64; n0 < g.nnodes; n0 += (long long)gridDim.x * 64) {
-        const int n = int(n0) + lane;
-        if (n < g.nnodes) y[3 * (long long)n + c] = h20_node_row<DIAG>(g, n, c, s_e, K0, u);
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < g.nnodes;
+         n += (long long)gridDim.x * blockDim.x) {
+        double acc[3];
+        h20_node_rows<DIAG>(g, int(n), s_e, K0, u, acc);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[3 * n + c] = acc[c];
     }
 }
 
@@ -314,15 +323,15 @@ k_h20_node_pcg(H20Grid g, const double* __restrict__ s_e, const double* __restri
     __shared__ double sRed[16];
     if (st->status != ST_RUNNING) return;
     const double* __restrict__ p = (st->k & 1) ? p1 : p0;
-    const int c = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
     double v[1] = {0.0};
-    for (long long n0 = (long long)blockIdx.x * 64; n0 < g.nnodes; n0 += (long long)gridDim.x * 64) {
-        const int n = int(n0) + lane;
-        if (n < g.nnodes) {
-            const double qv = h20_node_row<false>(g, n, c, s_e, K0, p);
-            q[3 * (long long)n + c] = qv;
-            v[0] += p[3 * (long long)n + c] * qv;
+    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < g.nnodes;
+         n += (long long)gridDim.x * blockDim.x) {
+        double acc[3];
+        h20_node_rows<false>(g, int(n), s_e, K0, p, acc);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            q[3 * n + c] = acc[c];
+            v[0] += p[3 * n + c] * acc[c];
         }
     }
     double tot[1];

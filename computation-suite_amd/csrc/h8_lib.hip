@@ -2606,7 +2606,7 @@ extern "C" int h8_ke20_batched_f64(const double* X, const double* D, int64_t d_s
 }
 
 static unsigned h20_blocks(const H20Grid& g) {
-    return unsigned(std::min<long long>((g.nnodes + 63) / 64, H20_MAX_BLOCKS));
+    return unsigned(std::min<long long>((g.nnodes + H20_THREADS - 1) / H20_THREADS, H20_MAX_BLOCKS));
 }
 
 static int h20_args(int32_t nx, int32_t ny, int32_t nz, H20Grid& g) {

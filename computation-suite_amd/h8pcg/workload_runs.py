@@ -4,6 +4,7 @@
     python -m h8pcg.workload_runs simp --dims 256,64,64 --rho 0.2,1.0 --seed 5 --fd 32 --precision mixed
     python -m h8pcg.workload_runs batched --seed 4 --n 256 --tol 1e-8
     python -m h8pcg.workload_runs simploop --dims 96,32,16 --iters 20 --vf 0.4 --radius 1.5
+    python -m h8pcg.workload_runs hex20 --dims 64,16,16 --tol 1e-8
 
 simp: SIMP cantilever on an nx x ny x nz unit-cube grid, x = 0 face clamped,
 F_z = -1 at the node (nx, ny/2, 0); stiffness scale s = rho_min + (1 - rho_min)
@@ -14,6 +15,11 @@ rng.choice(m, fd, replace=False) (a second generator, seed + 1), each from two
 FP64 solves warm-started from u at tol 1e-12, in the cancellation-free form
 C(rho + h) - C(rho - h) = -(s+ - s-) u-_e^T K0 u+_e.  Prints one JSON line with
 the timings and the largest relative difference between FD and gradient.
+
+hex20: a Hex20 cantilever of nx x ny x nz unit bricks (box_mesh_hex20, E = 210e9,
+nu = 0.3, x = 0 clamped, -1e6 in z spread over the x = nx nodes), FP64
+Jacobi-PCG to tol; prints DOFs, iterations, device time per iteration and the
+average time of the operator K u alone.
 
 batched: solve_many over batched_cantilevers(seed, n) (nx from {8, 12, .., 48},
 ny = nz = nx/4, 4..16 unit tip-traction load cases each, E = 10**U(9, 11) per
@@ -153,6 +159,43 @@ def batched(a):
                       "dof_iters_per_s": dof_iters / wall}), flush=True)
 
 
+def hex20(a):
+    import torch
+    from .hex20 import Hex20Solid3D, box_mesh_hex20
+    from .material import D_solid3d, Material
+    from .solver import FESystem
+    nx, ny, nz = (int(v) for v in a.dims.split(","))
+    mesh = box_mesh_hex20(float(nx), float(ny), float(nz), nx, ny, nz)
+    fs = FESystem(mesh, Hex20Solid3D())
+    fs.assemble_stiffness(D_solid3d(Material(E=210e9, nu=0.3)))
+    fs.fix_dofs(mesh.nodes_on_plane(0, 0.0), [0, 1, 2])
+    fs.add_nodal_force(mesh.nodes_on_plane(0, float(nx)), 2, -1e6)
+    fs.solve_static_device(tol=a.tol)          # warm-up (graph capture)
+    _sync()
+    t0 = time.perf_counter()
+    fs.solve_static_device(tol=a.tol)
+    _sync()
+    wall = time.perf_counter() - t0
+    st = fs.last_solve_stats
+    op = fs.operator
+    u = torch.randn(op.n_dof, dtype=torch.float64, device=op.device)
+    for _ in range(3):
+        op.matvec(u)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.reps):
+        op.matvec(u)
+    e1.record()
+    _sync()
+    n_free = fs.n_dof - len(fs.fixed_dofs)
+    print(json.dumps({"workload": "hex20", "dims": [nx, ny, nz], "elements": nx * ny * nz, "dof": fs.n_dof,
+                      "free_dof": n_free, "tol": a.tol, "iterations": st["iterations"],
+                      "rel_residual": st["rel_residual"], "device_ms": st["elapsed_ms"], "wall_ms": 1e3 * wall,
+                      "us_per_iteration": 1e3 * st["elapsed_ms"] / max(st["iterations"], 1),
+                      "matvec_us": 1e3 * e0.elapsed_time(e1) / a.reps,
+                      "dof_iters_per_s": n_free * st["iterations"] / (1e-3 * st["elapsed_ms"])}), flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -177,8 +220,12 @@ def main(argv=None):
     lp.add_argument("--radius", type=float, default=1.5)
     lp.add_argument("--tol", type=float, default=1e-8)
     lp.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    h = sub.add_parser("hex20")
+    h.add_argument("--dims", default="64,16,16")
+    h.add_argument("--tol", type=float, default=1e-8)
+    h.add_argument("--reps", type=int, default=50)
     a = ap.parse_args(argv)
-    {"simp": simp, "batched": batched, "simploop": simploop}[a.cmd](a)
+    {"simp": simp, "batched": batched, "simploop": simploop, "hex20": hex20}[a.cmd](a)
 
 
 if __name__ == "__main__":

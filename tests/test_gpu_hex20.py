@@ -128,6 +128,12 @@ def test_cantilever_solve_matches_spsolve_and_oracle_pcg(h8, gpu, dims, hetero):
     assert np.all(U[fixed] == 0.0)
     # the K property is the oracle's matrix
     assert abs(fs.K - K).max() <= 1e-12 * abs(K).max()
+    # reactions: K U - F at the supports, balancing the applied load
+    R = np.asarray(fs.reactions(U))
+    R_ref = np.zeros_like(R)
+    R_ref[fixed] = (K @ U_direct - F)[fixed]
+    assert np.abs(R - R_ref).max() < 1e-7 * np.abs(R_ref).max()
+    assert abs(R[2::3].sum() + F[2::3].sum()) < 1e-8 * abs(F[2::3].sum())
 
 
 def _boundary_dirichlet_solve(h8, dims, field, nu):
