@@ -207,8 +207,14 @@ struct SLayer {
     }
 };
 
+// Waves per SIMD the register allocator aims for.  FP64: the 16 x 16 column's
+// LDS ring (45.9 KB per 256-thread block) fits three blocks in a CU's 160 KB,
+// so three waves is the most the sector kernel can have whatever its
+// registers (118 VGPRs would allow four); asking for four changed nothing in
+// the sector kernel's time and pushed the persistent batched kernel, which
+// shares this target, to 201 VGPRs and two waves.
 template <typename T> struct SectorWaves;
-template <> struct SectorWaves<double> { static constexpr int value = 4; };
+template <> struct SectorWaves<double> { static constexpr int value = 3; };
 template <> struct SectorWaves<float> { static constexpr int value = 6; };
 
 // LDS of one march: the 3-slot layer ring and the per-node corner slots.

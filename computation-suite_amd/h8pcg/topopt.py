@@ -9,7 +9,11 @@ element contraction runs in libh8pcg.so.
 Two torch.autograd.Functions sit on top:
   * ``compliance(rho, problem)``  -> C = F.u.  Backward uses the self-adjoint
     closed form dC/drho_e = -s'(rho_e) u_e^T K0 u_e (no second solve), with u
-    stored by the forward pass (or recomputed when store_forward=False).
+    stored by the forward pass or recomputed by a second solve in backward.
+    store_forward=True / False fixes the choice; "auto" (the default) stores
+    u while the device's allocated bytes plus u stay within working_set_bytes
+    (80 GiB by default) and recomputes it beyond that, so many forward passes
+    awaiting their backward cannot push the working set past the cap.
   * ``displacement(rho, problem)`` -> u (full length).  Backward solves the
     adjoint K lambda = dL/du (masked) with the same PCG and returns
     dL/drho_e = -s'(rho_e) lambda_e^T K0 u_e.
@@ -41,14 +45,17 @@ class SIMPProblem:
     """Clamped structured H8 body under a fixed load, stiffness scaled per element."""
 
     def __init__(self, nx, ny, nz, F, fixed_dofs, E0=1.0, nu=0.3, h=(1.0, 1.0, 1.0), p=3.0, rho_min=1e-9,
-                 device="cuda", tol=1e-10, precision="fp64", maxiter=None, store_forward=True, D=None,
-                 preconditioner=None):
+                 device="cuda", tol=1e-10, precision="fp64", maxiter=None, store_forward="auto", D=None,
+                 preconditioner=None, working_set_bytes=80 * 2**30):
         torch = require_gpu()
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.m = self.nx * self.ny * self.nz
         self.p, self.rho_min = float(p), float(rho_min)
         self.tol, self.precision, self.maxiter = tol, precision, maxiter
+        if store_forward not in (True, False, "auto"):
+            raise ValueError("store_forward must be True, False or 'auto'")
         self.store_forward = store_forward
+        self.working_set_bytes = int(working_set_bytes)
         # FP32 iterations (precision "mixed"): the multigrid V-cycle where the grid
         # coarsens far enough (None = that choice), else "jacobi"
         self.preconditioner = preconditioner
@@ -68,6 +75,14 @@ class SIMPProblem:
     @property
     def device(self):
         return self.op.device
+
+    def keep_forward(self, u):
+        """Whether a forward pass stores u for its backward (else recomputes it)."""
+        if self.store_forward != "auto":
+            return bool(self.store_forward)
+        import torch
+        need = u.numel() * u.element_size()
+        return torch.cuda.memory_allocated(u.device) + need <= self.working_set_bytes
 
     def _set_rho(self, rho):
         s, ds = simp_scale(rho, self.p, self.rho_min, with_grad=True)
@@ -224,7 +239,8 @@ def _functions():
         def forward(ctx, rho, problem):
             u = problem.solve(rho)
             ctx.problem = problem
-            if problem.store_forward:
+            ctx.stored = problem.keep_forward(u)
+            if ctx.stored:
                 ctx.save_for_backward(rho, u)
             else:
                 ctx.save_for_backward(rho)
@@ -233,7 +249,7 @@ def _functions():
         @staticmethod
         def backward(ctx, gC):
             problem = ctx.problem
-            if problem.store_forward:
+            if ctx.stored:
                 rho, u = ctx.saved_tensors
             else:
                 (rho,) = ctx.saved_tensors

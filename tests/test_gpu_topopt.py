@@ -53,11 +53,19 @@ def test_compliance_and_closed_form_gradient(gpu):
     print("[gpu-test] C", float(C), C_ref, "grad err", np.abs(g - g_ref).max() / np.abs(g_ref).max(), flush=True)
     assert abs(float(C) - C_ref) < 1e-9 * abs(C_ref)
     assert np.abs(g - g_ref).max() < 1e-8 * np.abs(g_ref).max()
-    # recomputing the forward instead of storing it gives the same gradient
-    pb2 = problem(c, gpu, tol=1e-12, store_forward=False)
-    rho2 = torch.as_tensor(c["rho"], device=gpu).requires_grad_(True)
-    topopt.compliance(rho2, pb2).backward()
-    assert np.abs(rho2.grad.cpu().numpy() - g).max() < 1e-9 * np.abs(g).max()
+    # recomputing the forward instead of storing it gives the same gradient:
+    # forced, and by the size rule (a working-set cap below what is allocated)
+    assert pb.keep_forward(torch.zeros(10, dtype=torch.float64, device=gpu))
+    for kw in (dict(store_forward=False), dict(working_set_bytes=1)):
+        pb2 = problem(c, gpu, tol=1e-12, **kw)
+        assert not pb2.keep_forward(torch.zeros(10, dtype=torch.float64, device=gpu))
+        rho2 = torch.as_tensor(c["rho"], device=gpu).requires_grad_(True)
+        solves = pb2.solves
+        topopt.compliance(rho2, pb2).backward()
+        assert pb2.solves == solves + 2          # the forward, and its recomputation in backward
+        assert np.abs(rho2.grad.cpu().numpy() - g).max() < 1e-9 * np.abs(g).max()
+    with pytest.raises(ValueError):
+        problem(c, gpu, store_forward="sometimes")
 
 
 def test_general_adjoint_displacement_objective(gpu):
