@@ -390,9 +390,19 @@ extern "C" int h8_elem_bilinear_f64(const h8_grid* g, const double* K0, const do
     int rc = make_grid(g, d);
     if (rc) return rc;
     if (!K0 || !u || !v || !w || !out) return fail(H8_ERR_ARG, "NULL pointer");
-    const long long m = (long long)d.nx * d.ny * d.nz;
-    hipLaunchKernelGGL(k_elem_bilinear, dim3(stream_blocks(m)), dim3(STREAM_THREADS), 0,
-                       (hipStream_t)stream, d, K0, u, v, w, out);
+    // column tiles x chunks of kd slabs: about 1024 blocks, at least 4 slabs per chunk
+    const int tiles = ((d.ny + BIL_BJ - 1) / BIL_BJ) * ((d.nx + BIL_BI - 1) / BIL_BI);
+    const int want = std::max(1, (1024 + tiles - 1) / tiles);
+    const int kd = std::max(4, (d.nz + want - 1) / want);
+    const int chunks = (d.nz + kd - 1) / kd;
+    const long long blocks = (long long)tiles * chunks;
+    if (blocks >= (1LL << 31)) return fail(H8_ERR_ARG, "grid too large for the element kernel");
+    if (u == v)
+        hipLaunchKernelGGL(k_elem_bilinear<true>, dim3(unsigned(blocks)), dim3(BIL_BJ * BIL_BI), 0,
+                           (hipStream_t)stream, d, K0, u, v, w, out, kd);
+    else
+        hipLaunchKernelGGL(k_elem_bilinear<false>, dim3(unsigned(blocks)), dim3(BIL_BJ * BIL_BI), 0,
+                           (hipStream_t)stream, d, K0, u, v, w, out, kd);
     LAUNCHCHK();
     return H8_OK;
 }

@@ -109,6 +109,138 @@ __device__ __forceinline__ void sector_product(const T* kh, T (&U)[8][3], T scal
     }
 }
 
+// ---- element bilinear forms out_e = w_e u_e^T K0 v_e -------------------------
+// Through the sectors: with U^ = WHT(u_e) and V^ = WHT(v_e) per component,
+//     u_e^T K0 v_e = sum_psi sum_{c,c'} U^[c][psi ^ e_c] Kh[psi][c][c'] V^[c'][psi ^ e_c']
+// (Kh carries the 1/8 of the inverse transform): two forward transforms
+// (144 add/sub), 57 + 21 multiply-adds, instead of 576.
+// A block of BJ x BI threads owns a column of BJ x BI elements and marches
+// up through `kd` element slabs; the (BJ+1) x (BI+1) node layers of u and v
+// are staged once each into a two-slot LDS ring (component planes), the next
+// layer's loads in flight while the current slab is computed.  SAME: u == v
+// (the compliance form), one vector staged.
+constexpr int BIL_BJ = 16, BIL_BI = 16;
+
+template <bool SAME>
+__global__ void __launch_bounds__(BIL_BJ * BIL_BI)
+k_elem_bilinear(GridDev g, const double* __restrict__ K0, const double* __restrict__ u,
+                const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ out, int kd) {
+    constexpr int NT = BIL_BJ * BIL_BI, LJ = BIL_BJ + 1, LI = BIL_BI + 1, LN = LJ * LI, LW = 3 * LN;
+    constexpr int PER = (LW + NT - 1) / NT;     // staged words per thread and vector
+    constexpr int NV = SAME ? 1 : 2;
+    __shared__ double ring[2][NV][LW];          // [slot][vector][c * LN + node]
+    const int tj = (g.ny + BIL_BJ - 1) / BIL_BJ, ti = (g.nx + BIL_BI - 1) / BIL_BI;
+    const int bj = blockIdx.x % tj, bi = (blockIdx.x / tj) % ti, bk = blockIdx.x / (tj * ti);
+    const int j0 = bj * BIL_BJ, i0 = bi * BIL_BI, k0 = bk * kd;
+    if (k0 >= g.nz) return;
+    const int k1 = min(k0 + kd, g.nz);          // element slabs [k0, k1), node layers k0 .. k1
+    const int t = threadIdx.x;
+    // the words this thread stages: global order [hi][hj][c] of the layer's rows
+    int goff[PER], loff[PER];
+    bool ok[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int wd = t + q * NT;
+        const int c = wd % 3, nd = wd / 3, hj = nd % LJ, hi = nd / LJ;
+        ok[q] = wd < LW && j0 + hj <= g.ny && i0 + hi <= g.nx;
+        goff[q] = ok[q] ? 3 * ((i0 + hi) * g.NJ + j0 + hj) + c : 0;
+        loff[q] = c * LN + nd;
+    }
+    const long long layer = 3LL * g.NI * g.NJ;
+    double reg[NV][PER];
+    auto load = [&](int kl) {
+        const double* __restrict__ ul = u + kl * layer;
+        const double* __restrict__ vl = v + kl * layer;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            reg[0][q] = ok[q] ? ul[goff[q]] : 0.0;
+            if constexpr (!SAME) reg[NV - 1][q] = ok[q] ? vl[goff[q]] : 0.0;
+        }
+    };
+    auto store = [&](int slot) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (t + q * NT < LW) {
+#pragma unroll
+                for (int vv = 0; vv < NV; ++vv) ring[slot][vv][loff[q]] = reg[vv][q];
+            }
+    };
+    const int ej = t % BIL_BJ, ei = t / BIL_BJ;
+    const bool mine = j0 + ej < g.ny && i0 + ei < g.nx;
+    const auto kh = as_const(K0 + 576);
+    load(k0);
+    store(k0 & 1);
+    load(k0 + 1);
+    for (int k = k0; k < k1; ++k) {
+        __syncthreads();                        // the slot of layer k - 1 is free
+        store((k + 1) & 1);
+        __syncthreads();
+        if (k + 2 <= k1) load(k + 2);           // in flight during this slab
+        if (mine) {
+            // v's transform and the sector products first (W[c][psi ^ e_c]), then
+            // u's transform and the dot: at most 45 values live at a time
+            double W[3][8];
+            {
+                double V[3][8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int slot = (k + (m >> 2)) & 1;
+                    const int nd = (ei + (m & 1)) * LJ + ej + ((m >> 1) & 1);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) V[c][m] = ring[slot][NV - 1][c * LN + nd];
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) wht8(V[c]);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) W[c][0] = 0.0;
+#pragma unroll
+                for (int psi = 0; psi < 8; ++psi)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        if (psi == (1 << c)) continue;      // zero row of a translation sector
+                        double y = 0.0;
+#pragma unroll
+                        for (int c2 = 0; c2 < 3; ++c2) {
+                            if (psi == (1 << c2)) continue; // zero column
+                            y += kh[psi * 9 + c * 3 + c2] * V[c2][psi ^ (1 << c2)];
+                        }
+                        W[c][psi ^ (1 << c)] = y;
+                    }
+                if constexpr (SAME) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int m = 1; m < 8; ++m) acc += V[c][m] * W[c][m];
+                    W[0][0] = acc;
+                }
+            }
+            double acc;
+            if constexpr (SAME) {
+                acc = W[0][0];
+            } else {
+                double U[3][8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int slot = (k + (m >> 2)) & 1;
+                    const int nd = (ei + (m & 1)) * LJ + ej + ((m >> 1) & 1);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) U[c][m] = ring[slot][0][c * LN + nd];
+                }
+                acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    wht8(U[c]);
+#pragma unroll
+                    for (int m = 1; m < 8; ++m) acc += U[c][m] * W[c][m];
+                }
+            }
+            const long long e = ((long long)k * g.nx + i0 + ei) * g.ny + j0 + ej;
+            out[e] = w[e] * acc;
+        }
+    }
+}
+
 // Layer staging of the marching kernel: a node layer of the column
 // ((EJ+1) x (EI+1) nodes, 3 words each, node-interleaved [hi][hj][c] = the
 // global order of those words, row by row) is LW words; thread t owns words

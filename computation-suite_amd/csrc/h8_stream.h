@@ -280,39 +280,6 @@ k_simp(long long m, const double* __restrict__ rho, double p, double rho_min,
     }
 }
 
-// out_e = w_e * u_e^T K0 v_e, one thread per element, element order e.
-__global__ void __launch_bounds__(STREAM_THREADS)
-k_elem_bilinear(GridDev g, const double* __restrict__ K0, const double* __restrict__ u,
-                const double* __restrict__ v, const double* __restrict__ w, double* __restrict__ out) {
-    const long long m = (long long)g.nx * g.ny * g.nz;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int je = int(e % g.ny);
-        const int ie = int((e / g.ny) % g.nx);
-        const int ke = int(e / ((long long)g.ny * g.nx));
-        double ue[24], ve[24];
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const long long nd = (long long)(ke + OZ(a)) * g.NI * g.NJ + (long long)(ie + OX(a)) * g.NJ + (je + OY(a));
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ue[3 * a + c] = u[3 * nd + c];
-                ve[3 * a + c] = v[3 * nd + c];
-            }
-        }
-        double acc = 0.0;
-        const auto K0c = as_const(K0);
-#pragma unroll
-        for (int r = 0; r < 24; ++r) {
-            double y = 0.0;
-#pragma unroll
-            for (int cc = 0; cc < 24; ++cc) y += K0c[r * 24 + cc] * ve[cc];
-            acc += ue[r] * y;
-        }
-        out[e] = w[e] * acc;
-    }
-}
-
 // General-geometry element stiffness, one wavefront per element.  Lanes 0..7
 // each take one Gauss point (xi outer, eta middle, zeta inner, weight 1): they
 // form J = dN X, detJ and the physical gradients J^-1 dN into LDS; then every

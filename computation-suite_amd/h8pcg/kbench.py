@@ -3,6 +3,7 @@
     python -m h8pcg.kbench --dims 256,64,64 --reps 50 [--precision 32,64] [--variant V] [--layers L] [--copy-mib 256]
     python -m h8pcg.kbench --dims 256,64,64 --reps 50 --recovery
     python -m h8pcg.kbench --dims 256,64,64 --reps 20 --mg
+    python -m h8pcg.kbench --dims 256,64,64 --reps 50 --bilinear
 
 Builds the clamped box operator (per-element scale 10**U(9,12), seed 5), then
 times the fused direction + K*p kernel and the update kernel of one PCG
@@ -107,6 +108,33 @@ def time_recovery(nx, ny, nz, reps, device="cuda:0"):
         ev[1].record()
         torch.cuda.synchronize()
         out[what + "_us"] = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    return out
+
+
+def time_bilinear(op, reps):
+    """Average microseconds of h8_elem_bilinear_f64 with u == v (the compliance
+    sensitivity: one vector read) and with u != v (the adjoint product), and
+    their algorithmic HBM bytes: the vectors (8 B per DOF each), w in and out
+    (8 B per element each)."""
+    import torch
+    rng = np.random.default_rng(7)
+    u = torch.as_tensor(rng.normal(size=op.n_dof), device=op.device)
+    v = torch.as_tensor(rng.normal(size=op.n_dof), device=op.device)
+    w = torch.as_tensor(rng.normal(size=op.m), device=op.device)
+    out = {}
+    for name, b in (("same", u), ("two", v)):
+        for _ in range(3):
+            op.element_bilinear(u, b, w)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            op.element_bilinear(u, b, w)
+        ev[1].record()
+        torch.cuda.synchronize()
+        us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+        nbytes = (1 if name == "same" else 2) * 8 * op.n_dof + 16 * op.m
+        out[name] = {"us": us, "bytes": nbytes, "GBps": nbytes / us * 1e-3}
     return out
 
 
@@ -266,6 +294,7 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--recovery", action="store_true", help="time the stress recovery kernels instead")
     ap.add_argument("--mg", action="store_true", help="time the multigrid kernels instead")
+    ap.add_argument("--bilinear", action="store_true", help="time the element bilinear form instead")
     a = ap.parse_args(argv)
     nx, ny, nz = (int(v) for v in a.dims.split(","))
     if a.recovery:
@@ -283,9 +312,11 @@ def main(argv=None):
     op.grid32.layers = a.layers
     op.set_arith(a.arith)
     out = {"dims": [nx, ny, nz], "n_dof": op.n_dof, "elements": op.m, "reps": a.reps}
+    if a.bilinear:
+        out["bilinear"] = time_bilinear(op, a.reps)
     if a.mg:
         out["mg"] = time_mg(op, a.reps)
-    for prec in ((int(p) for p in a.precision.split(",")) if not a.mg else ()):
+    for prec in ((int(p) for p in a.precision.split(",")) if not (a.mg or a.bilinear) else ()):
         d_us, u_us = time_kernels(op, prec, a.reps)
         b = bytes_per_launch(op, prec)
         if a.variant == 7:
