@@ -1757,6 +1757,23 @@ static int launch_bottom(const MgPlan& P, const PcgState* st, hipStream_t s) {
     return H8_OK;
 }
 
+// Stencil-level sweeps: the quad form on small levels, the row form on large ones.
+static void launch_cheb_st(const MgLevel& L, const float* xin, float* xout, const MgConst* cst, int step,
+                           const PcgState* st, hipStream_t s) {
+    if (L.ndof <= MG_QUAD_DOF)
+        hipLaunchKernelGGL(k_mg_cheb_st<true>, dim3(mg_blocks(4 * L.ndof)), dim3(256), 0, s, L, xin, xout, cst, step,
+                           st);
+    else
+        hipLaunchKernelGGL(k_mg_cheb_st<false>, dim3(mg_blocks(L.ndof)), dim3(256), 0, s, L, xin, xout, cst, step,
+                           st);
+}
+static void launch_resid_st(const MgLevel& L, const float* x, float* r, const PcgState* st, hipStream_t s) {
+    if (L.ndof <= MG_QUAD_DOF)
+        hipLaunchKernelGGL(k_mg_resid_st<true>, dim3(mg_blocks(4 * L.ndof)), dim3(256), 0, s, L, x, r, st);
+    else
+        hipLaunchKernelGGL(k_mg_resid_st<false>, dim3(mg_blocks(L.ndof)), dim3(256), 0, s, L, x, r, st);
+}
+
 // z = V(r): one V-cycle (degree-2 Chebyshev smoothing before and after the
 // coarse correction).  P.L[0] holds b = r32, x1 = z, dinv = dinv32; prod0 the
 // level-0 operator arguments (FP32 sector kernel; st and partials of the
@@ -1808,10 +1825,8 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
             if ((rc = first0(F.x1))) return rc;
             if ((rc = resid0(F.x1))) return rc;
         } else {
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x0, F.x1,
-                               cst + l, 1, st);
-            hipLaunchKernelGGL(k_mg_resid_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x1, F.y,
-                               st);
+            launch_cheb_st(F, F.x0, F.x1, cst + l, 1, st, s);
+            launch_resid_st(F, F.x1, F.y, st, s);
         }
         hipLaunchKernelGGL(k_mg_restrict, dim3(mg_blocks(C.nnodes)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
                            (const float*)F.y, cst + l + 1, (l + 1 < lb) ? 1 : 0, st);
@@ -1826,10 +1841,8 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
             if ((rc = cheb0(F.x1, F.x0, 0, 0))) return rc;
             if ((rc = cheb0(F.x0, F.x1, 1, fuse_rz ? 1 : 0))) return rc;
         } else {
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x1, F.x0,
-                               cst + l, 0, st);
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(F.ndof)), dim3(256), 0, s, F, (const float*)F.x0, F.x1,
-                               cst + l, 1, st);
+            launch_cheb_st(F, F.x1, F.x0, cst + l, 0, st, s);
+            launch_cheb_st(F, F.x0, F.x1, cst + l, 1, st, s);
         }
     }
     LAUNCHCHK();
@@ -1928,8 +1941,17 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
         hst.status = ST_MAXITER;
         hst.iters = 0;
     }
+    // The FP64 residual runs every iteration and returns at once unless a
+    // replacement is due; the sector kernel's no-op launch is the cheapest
+    // (the node kernel's 8x8x4 bricks launch about 1.7x as many workgroups),
+    // so it is used whatever the caller's FP64 variant.
+    GridDev d64 = d;
+    if (d64.variant != H8_VARIANT_SECTOR && d64.variant != H8_VARIANT_SECTOR_NARROW) {
+        d64.variant = H8_VARIANT_SECTOR;
+        d64.layers = 0;
+    }
     NodeArgs<double> R{};
-    R.g = d; R.s_e = s_e64; R.K0 = K064; R.u = u; R.dinv = dinv64; R.f = f; R.r32 = w.r;
+    R.g = d64; R.s_e = s_e64; R.K0 = K064; R.u = u; R.dinv = dinv64; R.f = f; R.r32 = w.r;
     R.dinv32 = w.dinv32; R.st = w.st; R.partials = w.partials;
     NodeArgs<float> prod0{};
     prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y; prod0.st = w.st;
@@ -1941,7 +1963,7 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     hst.true_rel = hst.rel;
     if (hst.status == ST_RUNNING) {
         // first residual, z = V(r), r.z
-        rc = launch_op<double, MODE_RESID>(d, R, s);
+        rc = launch_op<double, MODE_RESID>(d64, R, s);
         if (rc) return rc;
         rc = mg_vcycle(d32, P, prod0, w.st, s, true);     // with r.z and beta
         if (rc) return rc;
@@ -1958,7 +1980,8 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
         key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
         const void* ptrs[] = {s_e64, K064, s_e32, K032, dinv64, f, u, ws, mg_ws};
         std::memcpy(key.ptr, ptrs, sizeof ptrs);
-        key.extra[0] = (unsigned long long)g->variant;    // the FP64 residual's kernel follows the caller's grid
+        key.extra[0] = (unsigned long long)d64.variant;   // the FP64 residual's kernel
+        key.extra[1] = (unsigned long long)d64.layers;
         key.chunk = int(std::min<long long>(MG_CHUNK, maxiter));
         GraphRef exec;
         NodeArgs<double> Rq = R;
@@ -1972,7 +1995,7 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
                                    (const float*)w.p1, (const float*)w.q, (const float*)w.dinv32, w.st, w.partials);
                 hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, u, w.x, (const float*)w.p0,
                                    (const float*)w.p1, (const PcgState*)w.st);
-                if ((r1 = launch_op<double, MODE_RESID>(d, Rq, cs))) return r1;
+                if ((r1 = launch_op<double, MODE_RESID>(d64, Rq, cs))) return r1;
                 if ((r1 = mg_vcycle(d32, P, prod0, w.st, cs, true))) return r1;
             }
             LAUNCHCHK();
@@ -2113,8 +2136,7 @@ extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const d
         switch (which) {
         case 0: return launch_op<float, MODE_PLAIN>(d32, prod0, cs);
         case 1:
-            hipLaunchKernelGGL(k_mg_cheb_st, dim3(mg_blocks(L1.ndof)), dim3(256), 0, cs, L1, (const float*)L1.x0,
-                               L1.x1, (const MgConst*)(P.cst + 1), 1, (const PcgState*)nullptr);
+            launch_cheb_st(L1, L1.x0, L1.x1, P.cst + 1, 1, nullptr, cs);
             return H8_OK;
         case 2: return mg_vcycle(d32, P, prod0, nullptr, cs, false);
         case 3:
@@ -2136,8 +2158,7 @@ extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const d
             return H8_OK;
         case 8: return launch_bottom(P, nullptr, cs);
         default:
-            hipLaunchKernelGGL(k_mg_resid_st, dim3(mg_blocks(L1.ndof)), dim3(256), 0, cs, L1, (const float*)L1.x1,
-                               L1.y, (const PcgState*)nullptr);
+            launch_resid_st(L1, L1.x1, L1.y, nullptr, cs);
             return H8_OK;
         }
     };

@@ -477,66 +477,216 @@ __device__ __forceinline__ float mg_row(const MgLevel& L, const float* __restric
     return y;
 }
 
+// The stencil-level sweeps of the cycle (k_mg_cheb_st, k_mg_resid_st) on a
+// small level are bound by dependent memory round trips, not bytes: a row with
+// its three neighbour planes summed one after another, then the pointwise
+// words, then the stores, behind a load of the solve state, is five round
+// trips.  Here four lanes (a quad) share one row: lane dk + 1 sums neighbour
+// plane dk (9 blocks, 27 stencil and 27 x words, all in flight together; lane
+// 3 repeats plane 0 and drops it), every lane loads the row's pointwise words
+// and the solve state at the same time, the quad adds its three plane sums in
+// a fixed order (DPP quad swaps: every lane gets the same bits) and lane 3
+// stores the result.  The stores are bounds-checked buffer stores whose offset
+// is out of range once the solve has stopped, so no load waits for the state.
+// Plane dk of row (n, c): its 27 stencil words and 9 x triples, loaded all at
+// once (mg_plane_load), then summed (mg_plane_sum).  Branch-free as mg_row:
+// blocks of neighbours outside the grid are stored zeros and their x index is
+// clamped to the node itself.
+struct MgPlane {
+    float s[9][3], x[9][3];
+};
+__device__ __forceinline__ void mg_plane_load(const MgLevel& L, const float* __restrict__ x, long long n, int c,
+                                              int dk, MgPlane& P) {
+    int j, i, k;
+    mg_decode(L, n, j, i, k);
+    const long long sj = L.NJ, si = (long long)L.NI * L.NJ;
+    const bool kin = unsigned(k + dk) < unsigned(L.NK);
+    const float* Sc = L.S + (long long)(3 * c + (dk + 1) * 81) * L.nnodes + n;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+        const int dj = p % 3 - 1, di = p / 3 - 1;
+        const bool in = kin && unsigned(j + dj) < unsigned(L.NJ) && unsigned(i + di) < unsigned(L.NI);
+        const long long m = 3 * (in ? n + dk * si + di * sj + dj : n);
+        const float* sp = Sc + (long long)(p * 9) * L.nnodes;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            P.s[p][d] = sp[d * L.nnodes];
+            P.x[p][d] = x[m + d];
+        }
+    }
+}
+__device__ __forceinline__ float mg_plane_sum(const MgPlane& P) {
+    float y = 0.0f;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) y += P.s[p][0] * P.x[p][0] + P.s[p][1] * P.x[p][1] + P.s[p][2] * P.x[p][2];
+    return y;
+}
+
+// Sum of the plane sums of lanes 0..2 of a quad (lane 3 passes 0), the same
+// bits in all four lanes.
+__device__ __forceinline__ float mg_quad_sum(float v) {
+    const float a = v + __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF,
+                                                                         false));   // quad_perm [1,0,3,2]
+    return a + __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, a), 0x4E, 0xF, 0xF,
+                                                                  false));          // quad_perm [2,3,0,1]
+}
+
+// Vector (per-lane) loads of the solve state and the smoother constants.  The
+// address is offset by a zero the compiler cannot see through, so the value
+// is not taken as wave-uniform: it stays in a vector register, joins the data
+// loads' memory counter and is only waited for where it is used (a store's
+// offset), instead of forcing a wait for a scalar branch before the data loads
+// are issued.
+__device__ __forceinline__ int opaque_zero() {
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+__device__ __forceinline__ bool mg_live_v(const PcgState* st) {
+    if (!st) return true;
+    return __hip_atomic_load(const_cast<int*>(&st->status) + opaque_zero(), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT) == ST_RUNNING;
+}
+__device__ __forceinline__ float ld_vf(const float* p) {
+    return __hip_atomic_load(const_cast<float*>(p) + opaque_zero(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Chebyshev step on a stencil level: x_out = x_in + d, d = ca d + cb dinv (b - A x_in).
+// QUAD (small levels, latency-bound): a quad per row, 4 ndof threads.  Row
+// form (large levels, where a wave's loads should cover 64 consecutive
+// stencil words): a thread per row, the three planes summed in turn as
+// mg_row, with the pointwise words, the constants and the solve state loaded
+// before the first plane.  Both grid-stride.
+template <bool QUAD>
 __global__ void __launch_bounds__(256) k_mg_cheb_st(MgLevel L, const float* __restrict__ xin, float* __restrict__ xout,
                                                     const MgConst* cst, int step, const PcgState* st) {
-    if (!mg_live(st)) return;
-    const float ca = step ? cst->c1a : 0.0f, cb = step ? cst->c1b : cst->c0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
-         t += (long long)gridDim.x * blockDim.x) {
+    const unsigned vbytes = unsigned(L.ndof) * sizeof(float);
+    const int lane = QUAD ? int(threadIdx.x & 3) : 3;
+    const long long nt = QUAD ? 4 * L.ndof : L.ndof;
+    // QUAD: 4 ndof is a multiple of 4 and so is every quad's first thread: a
+    // quad is wholly in or out of range, and its DPP sums see only active lanes
+    for (long long t4 = (long long)blockIdx.x * blockDim.x + threadIdx.x; t4 < nt;
+         t4 += (long long)gridDim.x * blockDim.x) {
+        const long long t = QUAD ? t4 >> 2 : t4;
         const int c = int(unsigned(t) / unsigned(L.nnodes));
         const long long n = t - c * L.nnodes, q = 3 * n + c;
-        const float y = mg_row(L, xin, n, c);
-        const float dn = ca * L.d[q] + cb * L.dinv[q] * (L.b[q] - y);
-        L.d[q] = dn;
-        xout[q] = xin[q] + dn;
+        float y;
+        if constexpr (QUAD) {
+            MgPlane P;
+            mg_plane_load(L, xin, n, c, lane == 3 ? 0 : lane - 1, P);
+            const float dq = L.d[q], dv = L.dinv[q], bv = L.b[q], xv = xin[q];
+            const float ca = ld_vf(&cst->c1a), cb = ld_vf(step ? &cst->c1b : &cst->c0);
+            const bool live = mg_live_v(st);
+            __builtin_amdgcn_sched_barrier(0);     // every load above is issued before any arithmetic
+            y = mg_quad_sum(lane == 3 ? 0.0f : mg_plane_sum(P));
+            const float dn = (step ? ca * dq : 0.0f) + cb * dv * (bv - y);
+            const int off = (live && lane == 3) ? int(sizeof(float) * q) : OOB;
+            Buf<float>::st(dn, buf_rsrc(L.d, vbytes), off);
+            Buf<float>::st(xv + dn, buf_rsrc(xout, vbytes), off);
+        } else {
+            const bool live = mg_live_v(st);
+            const float ca = ld_vf(&cst->c1a), cb = ld_vf(step ? &cst->c1b : &cst->c0);
+            const float dq = L.d[q], dv = L.dinv[q], bv = L.b[q], xv = xin[q];
+            y = mg_row(L, xin, n, c);
+            const float dn = (step ? ca * dq : 0.0f) + cb * dv * (bv - y);
+            const int off = live ? int(sizeof(float) * q) : OOB;
+            Buf<float>::st(dn, buf_rsrc(L.d, vbytes), off);
+            Buf<float>::st(xv + dn, buf_rsrc(xout, vbytes), off);
+        }
     }
 }
 
 // Masked residual r = (dinv != 0) (b - A x) on a stencil level (the input of
-// the restriction).
+// the restriction).  QUAD or row form, as k_mg_cheb_st.
+template <bool QUAD>
 __global__ void __launch_bounds__(256) k_mg_resid_st(MgLevel L, const float* __restrict__ x, float* __restrict__ r,
                                                      const PcgState* st) {
-    if (!mg_live(st)) return;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
-         t += (long long)gridDim.x * blockDim.x) {
+    const unsigned vbytes = unsigned(L.ndof) * sizeof(float);
+    const int lane = QUAD ? int(threadIdx.x & 3) : 3;
+    const long long nt = QUAD ? 4 * L.ndof : L.ndof;
+    for (long long t4 = (long long)blockIdx.x * blockDim.x + threadIdx.x; t4 < nt;
+         t4 += (long long)gridDim.x * blockDim.x) {
+        const long long t = QUAD ? t4 >> 2 : t4;
         const int c = int(unsigned(t) / unsigned(L.nnodes));
         const long long n = t - c * L.nnodes, q = 3 * n + c;
-        const float y = mg_row(L, x, n, c);
-        r[q] = L.dinv[q] != 0.0f ? L.b[q] - y : 0.0f;
+        if constexpr (QUAD) {
+            MgPlane P;
+            mg_plane_load(L, x, n, c, lane == 3 ? 0 : lane - 1, P);
+            const float dv = L.dinv[q], bv = L.b[q];
+            const bool live = mg_live_v(st);
+            __builtin_amdgcn_sched_barrier(0);
+            const float y = mg_quad_sum(lane == 3 ? 0.0f : mg_plane_sum(P));
+            const int off = (live && lane == 3) ? int(sizeof(float) * q) : OOB;
+            Buf<float>::st(dv != 0.0f ? bv - y : 0.0f, buf_rsrc(r, vbytes), off);
+        } else {
+            const bool live = mg_live_v(st);
+            const float dv = L.dinv[q], bv = L.b[q];
+            const float y = mg_row(L, x, n, c);
+            Buf<float>::st(dv != 0.0f ? bv - y : 0.0f, buf_rsrc(r, vbytes), live ? int(sizeof(float) * q) : OOB);
+        }
     }
 }
+
+// Levels up to this many DOFs run the stencil sweeps in the quad form.
+constexpr long long MG_QUAD_DOF = 32768;
 
 // Coarse node In of the restriction bC = P^T rF (rF the fine level's masked
 // residual), all three components: the 27 fine nodes around the node's fine
 // position (weights 1, 1/2, 1/4, 1/8 by the number of offset axes; a kept axis
-// has only the centre), each a 12-byte read.
-__device__ __forceinline__ void mg_restrict_node(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
-                                                 const float* __restrict__ rF, long long In, float (&acc)[3]) {
+// has only the centre), each a 12-byte read.  mg_restrict_load issues all 27
+// reads, mg_restrict_sum adds them.
+struct MgRestrict {
+    float v[27][3], w[27];
+};
+// OPAQUE: the clamped index passes through an empty asm statement, so the
+// compiler cannot tell that absent neighbours read the centre; otherwise it
+// loads the centre once and branches around every other load, which puts a
+// memory wait between them.  All 27 reads are then issued at once.
+template <bool OPAQUE>
+__device__ __forceinline__ void mg_restrict_load(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
+                                                 const float* __restrict__ rF, long long In, MgRestrict& R) {
     int Ij, Ii, Ik;
     mg_decode(C, In, Ij, Ii, Ik);
     const int fj = Ij * ry, fi = Ii * rx, fk = Ik * rz;
     // all 27 offsets, branch-free: weight 0 (and the index clamped to the
     // centre) for an offset along a kept axis or outside the grid
-    acc[0] = acc[1] = acc[2] = 0.0f;
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
         const int oj = o % 3 - 1, oi = (o / 3) % 3 - 1, ok = o / 9 - 1;
-        const bool in = (oj == 0 || (ry == 2 && unsigned(fj + oj) < unsigned(F.NJ))) &&
-                        (oi == 0 || (rx == 2 && unsigned(fi + oi) < unsigned(F.NI))) &&
-                        (ok == 0 || (rz == 2 && unsigned(fk + ok) < unsigned(F.NK)));
-        const float w = in ? (oj ? 0.5f : 1.0f) * (oi ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f) : 0.0f;
-        const long long f = 3 * (in ? ((long long)(fk + ok) * F.NI + (fi + oi)) * F.NJ + (fj + oj)
-                                    : ((long long)fk * F.NI + fi) * F.NJ + fj);
+        // (bitwise operators: no short-circuit branches between the loads)
+        const bool in = ((oj == 0) | ((ry == 2) & (unsigned(fj + oj) < unsigned(F.NJ)))) &
+                        ((oi == 0) | ((rx == 2) & (unsigned(fi + oi) < unsigned(F.NI)))) &
+                        ((ok == 0) | ((rz == 2) & (unsigned(fk + ok) < unsigned(F.NK))));
+        R.w[o] = in ? (oj ? 0.5f : 1.0f) * (oi ? 0.5f : 1.0f) * (ok ? 0.5f : 1.0f) : 0.0f;
+        long long f = 3 * (in ? ((long long)(fk + ok) * F.NI + (fi + oi)) * F.NJ + (fj + oj)
+                              : ((long long)fk * F.NI + fi) * F.NJ + fj);
+        if constexpr (OPAQUE) asm volatile("" : "+v"(f));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] += w * rF[f + c];
+        for (int c = 0; c < 3; ++c) R.v[o][c] = rF[f + c];
     }
 }
+__device__ __forceinline__ void mg_restrict_sum(const MgRestrict& R, float (&acc)[3]) {
+    acc[0] = acc[1] = acc[2] = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 27; ++o)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += R.w[o] * R.v[o][c];
+}
+__device__ __forceinline__ void mg_restrict_node(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
+                                                 const float* __restrict__ rF, long long In, float (&acc)[3]) {
+    MgRestrict R;
+    mg_restrict_load<false>(F, C, rx, ry, rz, rF, In, R);
+    mg_restrict_sum(R, acc);
+}
 
-// Fine node fn of the coarse correction xF += M_F P xC.
+// Fine node fn of the coarse correction xF += M_F P xC.  BUF: xF is global
+// memory, stored through bounds-checked buffer stores (dropped unless live);
+// otherwise plain stores (the bottom kernel's LDS views).
+template <bool BUF>
 __device__ __forceinline__ void mg_prolong_node(const MgLevel& F, const MgLevel& C, int rx, int ry, int rz,
                                                 float* __restrict__ xF, const float* __restrict__ xC,
-                                                const float* __restrict__ maskF, long long fn) {
+                                                const float* __restrict__ maskF, long long fn,
+                                                const PcgState* st = nullptr) {
     int fj, fi, fk;
     mg_decode(F, fn, fj, fi, fk);
     // parents per axis: f = r I (weight 1) or, on a halved axis at an odd f,
@@ -555,9 +705,28 @@ __device__ __forceinline__ void mg_prolong_node(const MgLevel& F, const MgLevel&
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[c] += w * xC[I + c];
     }
+    if constexpr (BUF) {
+        // (live = false: the solve has stopped; the stores are dropped by the
+        // buffer range check, so no load waits for the solve state)
+        const unsigned vbytes = unsigned(F.ndof) * sizeof(float);
+        float m[3], xv[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-        if (maskF[3 * fn + c] != 0.0f) xF[3 * fn + c] += acc[c];
+        for (int c = 0; c < 3; ++c) {
+            m[c] = maskF[3 * fn + c];
+            xv[c] = xF[3 * fn + c];
+        }
+        const bool live = mg_live_v(st);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long long q = 3 * fn + c;
+            const int off = (live && m[c] != 0.0f) ? int(sizeof(float) * q) : OOB;
+            Buf<float>::st(xv[c] + acc[c], buf_rsrc(xF, vbytes), off);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (maskF[3 * fn + c] != 0.0f) xF[3 * fn + c] += acc[c];
+    }
 }
 
 // Restriction bC = P^T rF, one thread per coarse node; with pre0 the coarse
@@ -566,20 +735,29 @@ __device__ __forceinline__ void mg_prolong_node(const MgLevel& F, const MgLevel&
 __global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C, int rx, int ry, int rz,
                                                      const float* __restrict__ rF, const MgConst* cstC, int pre0,
                                                      const PcgState* st) {
-    if (!mg_live(st)) return;
-    const float c0 = pre0 ? cstC->c0 : 0.0f;
+    const unsigned vbytes = unsigned(C.ndof) * sizeof(float);
     for (long long In = (long long)blockIdx.x * blockDim.x + threadIdx.x; In < C.nnodes;
          In += (long long)gridDim.x * blockDim.x) {
-        float b[3];
-        mg_restrict_node(F, C, rx, ry, rz, rF, In, b);
+        MgRestrict R;
+        mg_restrict_load<true>(F, C, rx, ry, rz, rF, In, R);
+        float b[3], dv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dv[c] = C.dinv[3 * In + c];
+        // the solve state and c0 are loaded beside the data and only gate or
+        // scale the stores
+        const float c0 = ld_vf(&cstC->c0);
+        const bool live = mg_live_v(st);
+        __builtin_amdgcn_sched_barrier(0);
+        mg_restrict_sum(R, b);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const long long q = 3 * In + c;
-            C.b[q] = b[c];
+            const int off = live ? int(sizeof(float) * q) : OOB;
+            Buf<float>::st(b[c], buf_rsrc(C.b, vbytes), off);
             if (pre0) {
-                const float d = c0 * C.dinv[q] * b[c];
-                C.d[q] = d;
-                C.x0[q] = d;
+                const float d = c0 * dv[c] * b[c];
+                Buf<float>::st(d, buf_rsrc(C.d, vbytes), off);
+                Buf<float>::st(d, buf_rsrc(C.x0, vbytes), off);
             }
         }
     }
@@ -589,10 +767,9 @@ __global__ void __launch_bounds__(256) k_mg_restrict(MgLevel F, MgLevel C, int r
 __global__ void __launch_bounds__(256) k_mg_prolong(MgLevel F, MgLevel C, int rx, int ry, int rz,
                                                     float* __restrict__ xF, const float* __restrict__ xC,
                                                     const float* __restrict__ maskF, const PcgState* st) {
-    if (!mg_live(st)) return;
     for (long long fn = (long long)blockIdx.x * blockDim.x + threadIdx.x; fn < F.nnodes;
          fn += (long long)gridDim.x * blockDim.x)
-        mg_prolong_node(F, C, rx, ry, rz, xF, xC, maskF, fn);
+        mg_prolong_node<true>(F, C, rx, ry, rz, xF, xC, maskF, fn, st);
 }
 
 // The small levels of a cycle in one workgroup: levels lb .. nl-1 (the last
@@ -769,7 +946,7 @@ __global__ void __launch_bounds__(MG_BOTTOM_THREADS) k_mg_bottom(MgBottom B, con
         const MgLevel C = mg_bottom_view<STAGED>(B, smem, l + 1);
         const MgConst k = B.cst[l];
         for (long long fn = threadIdx.x; fn < F.nnodes; fn += blockDim.x)
-            mg_prolong_node(F, C, F.rx, F.ry, F.rz, F.x1, C.x1, F.dinv, fn);
+            mg_prolong_node<false>(F, C, F.rx, F.ry, F.rz, F.x1, C.x1, F.dinv, fn);
         __syncthreads();
         mg_bottom_cheb(F, k, F.x1, F.x0, 0);
         __syncthreads();
