@@ -1717,12 +1717,16 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
         hipLaunchKernelGGL(k_mg_level1, dim3(unsigned(std::min<long long>(nblk, 16384))), dim3(256), 0, s, d, L1,
                            L0.rx, L0.ry, L0.rz, s_e64, K064, (const double*)P.M, (const unsigned*)P.emask);
     }
-    hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[1].nnodes)), dim3(256), 0, s, P.L[1], P.cst + 1);
+    hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[1].ndof)), dim3(256), 0, s, P.L[1], P.cst + 1);
     for (int l = 1; l + 1 < nl; ++l) {
         const MgLevel& F = P.L[l];
-        hipLaunchKernelGGL(k_mg_rap, dim3(mg_blocks(243 * P.L[l + 1].nnodes)), dim3(256), 0, s, F, P.L[l + 1], F.rx,
-                           F.ry, F.rz);
-        hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[l + 1].nnodes)), dim3(256), 0, s, P.L[l + 1],
+        // lanes per RAP entry: about 2^20 threads per launch, 1..32
+        const long long entries = 243 * P.L[l + 1].nnodes;
+        int lg = 0;
+        while (lg < 5 && (entries << (lg + 1)) <= (1LL << 20)) ++lg;
+        hipLaunchKernelGGL(k_mg_rap, dim3(mg_blocks(entries << lg)), dim3(256), 0, s, F, P.L[l + 1], F.rx, F.ry,
+                           F.rz, lg);
+        hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[l + 1].ndof)), dim3(256), 0, s, P.L[l + 1],
                            P.cst + l + 1);
     }
     const MgLevel& Lc = P.L[nl - 1];
@@ -1982,32 +1986,56 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
         std::memcpy(key.ptr, ptrs, sizeof ptrs);
         key.extra[0] = (unsigned long long)d64.variant;   // the FP64 residual's kernel
         key.extra[1] = (unsigned long long)d64.layers;
-        key.chunk = int(std::min<long long>(MG_CHUNK, maxiter));
-        GraphRef exec;
         NodeArgs<double> Rq = R;
         Rq.only_on_replace = 1;
-        rc = get_graph(key, [&](hipStream_t cs) -> int {
-            for (int it = 0; it < key.chunk; ++it) {
-                int r1 = launch_op<float, MODE_DIRZ>(d32, D, cs);
-                if (r1) return r1;
-                hipLaunchKernelGGL((k_pcg_update<float, true, true, true>), dim3(update_blocks<float>(d.ndof)),
-                                   dim3(STREAM_THREADS), 0, cs, d.ndof, w.x, w.r, (const float*)w.p0,
-                                   (const float*)w.p1, (const float*)w.q, (const float*)w.dinv32, w.st, w.partials);
-                hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, u, w.x, (const float*)w.p0,
-                                   (const float*)w.p1, (const PcgState*)w.st);
-                if ((r1 = launch_op<double, MODE_RESID>(d64, Rq, cs))) return r1;
-                if ((r1 = mg_vcycle(d32, P, prod0, w.st, cs, true))) return r1;
-            }
-            LAUNCHCHK();
-            return H8_OK;
-        }, &exec);
-        if (rc) return rc;
+        // one captured graph per chunk length (8, 4, 2 iterations): the solve
+        // launches full chunks while far from the tolerance and shorter ones at
+        // the end, where the iterations of a full chunk past the stop would be
+        // launches that return at once (about 3 us each, 29 per iteration)
+        auto graph_of = [&](int chunk, GraphRef* exec) -> int {
+            key.chunk = chunk;
+            return get_graph(key, [&](hipStream_t cs) -> int {
+                for (int it = 0; it < chunk; ++it) {
+                    int r1 = launch_op<float, MODE_DIRZ>(d32, D, cs);
+                    if (r1) return r1;
+                    hipLaunchKernelGGL((k_pcg_update<float, true, true, true>), dim3(update_blocks<float>(d.ndof)),
+                                       dim3(STREAM_THREADS), 0, cs, d.ndof, w.x, w.r, (const float*)w.p0,
+                                       (const float*)w.p1, (const float*)w.q, (const float*)w.dinv32, w.st,
+                                       w.partials);
+                    hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, u, w.x,
+                                       (const float*)w.p0, (const float*)w.p1, (const PcgState*)w.st);
+                    if ((r1 = launch_op<double, MODE_RESID>(d64, Rq, cs))) return r1;
+                    if ((r1 = mg_vcycle(d32, P, prod0, w.st, cs, true))) return r1;
+                }
+                LAUNCHCHK();
+                return H8_OK;
+            }, exec);
+        };
+        int chunk = int(std::min<long long>(MG_CHUNK, maxiter));
+        double rel_prev = hst.rel;
+        long long k_prev = hst.k;
         for (;;) {
+            GraphRef exec;
+            rc = graph_of(chunk, &exec);
+            if (rc) return rc;
             HIPCHK(hipGraphLaunch(exec->exec, s));
             ++launches;
             HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (hst.status != ST_RUNNING) break;
+            // iterations still needed, from the decay of the recurrence residual
+            // over the last chunk (a replacement inside it can make the estimate
+            // short: the next chunk is then launched as well)
+            const double its = double(hst.k - k_prev);
+            int next = MG_CHUNK;
+            if (its > 0 && hst.rel > 0.0 && hst.rel < rel_prev && rel_prev > 0.0) {
+                const double rate = std::log(hst.rel / rel_prev) / its;       // < 0
+                const double need = std::log(tol / hst.rel) / rate + 1.0;     // + the replacement's iteration
+                next = need <= 2.0 ? 2 : (need <= 4.0 ? 4 : MG_CHUNK);
+            }
+            chunk = int(std::min<long long>(next, maxiter));
+            rel_prev = hst.rel;
+            k_prev = hst.k;
         }
     }
     HIPCHK(hipEventRecord(e1, s));

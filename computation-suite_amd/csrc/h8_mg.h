@@ -45,9 +45,18 @@ constexpr int MG_STENCIL = 243;            // 27 neighbours x 3 x 3
 constexpr long long MG_COARSE_DOF = 128;   // stop coarsening at or below this many DOFs
 constexpr int MG_LDS_GJ = 128;             // coarsest levels up to this many DOFs are inverted in LDS
 constexpr long long MG_DENSE_MAX = 512;    // the coarsest level must be this small (dense inverse)
-constexpr float MG_CHEB_RATIO = 30.0f;     // smoothing interval [lam / ratio, lam]
-constexpr long long MG_BOTTOM_NODES = 256;   // levels this small run in one workgroup (k_mg_bottom)
-constexpr int MG_BOTTOM_THREADS = 512;
+#ifndef H8_MG_CHEB_RATIO
+#define H8_MG_CHEB_RATIO 30.0f
+#endif
+constexpr float MG_CHEB_RATIO = H8_MG_CHEB_RATIO;   // smoothing interval [lam / ratio, lam]
+#ifndef H8_MG_BOTTOM_NODES
+#define H8_MG_BOTTOM_NODES 256
+#endif
+#ifndef H8_MG_BOTTOM_THREADS
+#define H8_MG_BOTTOM_THREADS 512
+#endif
+constexpr long long MG_BOTTOM_NODES = H8_MG_BOTTOM_NODES;   // levels this small run in one workgroup (k_mg_bottom)
+constexpr int MG_BOTTOM_THREADS = H8_MG_BOTTOM_THREADS;
 
 // One level of the hierarchy (by value in kernel arguments).
 struct MgLevel {
@@ -115,6 +124,23 @@ __global__ void __launch_bounds__(256) k_mg_mc(const double* __restrict__ K0, in
     }
 }
 
+// Block-wide max of a non-negative float, then one atomicMax of its bits per
+// block (per-thread atomics on the one word serialise: an 81-node level's
+// kernel took 25 us).  Every thread of the block must call it.
+__device__ __forceinline__ void block_max_to(unsigned* word, float v) {
+    __shared__ float s_max[16];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) s_max[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < nw; ++w) m = fmaxf(m, s_max[w]);
+        if (m > 0.0f) atomicMax(word, __float_as_uint(m));
+    }
+}
+
 // Gershgorin bound of D^-1 K on level 0 from the elements: row (n, c) has
 // sum_j |K_ij| <= sum_e s_e sum_(b,d) |K0[3a+c][3b+d]| (a = the node's corner
 // in e).  Free rows only; the bound's bits go to cst->lam_bits by atomicMax.
@@ -146,7 +172,7 @@ __global__ void __launch_bounds__(256) k_mg_gersh0(GridDev g, const float* __res
             if (di != 0.0f) best = fmaxf(best, 1.0001f * sum[c] * di);
         }
     }
-    if (best > 0.0f) atomicMax(&cst->lam_bits, __float_as_uint(best));
+    block_max_to(&cst->lam_bits, best);
 }
 
 // Free-DOF bits of every fine element (bit 3a + c: corner a, component c).
@@ -276,12 +302,22 @@ __global__ void __launch_bounds__(256) k_mg_level1(GridDev g, MgLevel C, int rx,
 }
 
 // Stencil of level l+1 from the stencil of level l (A_{l+1} = P^T A_l P; A_l
-// is zero at its masked DOFs, so P needs no mask here): one thread per stencil
+// is zero at its masked DOFs, so P needs no mask here): 32 lanes per stencil
 // entry (coarse node I, neighbour block nb, entry q = 3 c + d of the block).
-__global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, int ry, int rz) {
+__global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, int ry, int rz, int lg) {
+    // 2^lg lanes per entry (G = 1..32, so that a launch has about a million
+    // threads): lane l takes the fine nodes at offsets o = l, l + G, ... < 27,
+    // (o % 3, o / 3 % 3, o / 9) - 1, of I's support (offsets along kept axes
+    // and outside the fine grid take nothing) and sums their stencil
+    // neighbours inside J's support; the lanes' sums are added by a fixed
+    // butterfly (deterministic for a given G).  With one thread walking all 27
+    // fine nodes, the small levels' entries were a serial chain of loads.
     const long long total = 243LL * C.nnodes;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
+    const int G = 1 << lg;
+    const int l = threadIdx.x & (G - 1);
+    // (the grid stride is a multiple of G: the lanes of a group stay together)
+    for (long long t = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> lg; t < total;
+         t += ((long long)gridDim.x * blockDim.x) >> lg) {
         const int e = int(unsigned(t) / unsigned(C.nnodes));   // nb * 9 + q
         const int nb = e / 9, q = e % 9;
         const long long In = t - (long long)e * C.nnodes;
@@ -289,56 +325,58 @@ __global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, in
         mg_decode(C, In, Ij, Ii, Ik);
         const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
         double acc = 0.0;
-        if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
-            // fine nodes of I's support: offsets -1..1 on halved axes, 0 on kept ones
-            for (int ok = -(rz - 1); ok <= rz - 1; ++ok)
-                for (int oi = -(rx - 1); oi <= rx - 1; ++oi)
-                    for (int oj = -(ry - 1); oj <= ry - 1; ++oj) {
-                        const int fj = Ij * ry + oj, fi = Ii * rx + oi, fk = Ik * rz + ok;
-                        if (fj < 0 || fj >= F.NJ || fi < 0 || fi >= F.NI || fk < 0 || fk >= F.NK) continue;
-                        const double wi = (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
-                        const long long fn = ((long long)fk * F.NI + fi) * F.NJ + fj;
-                        // stencil neighbours g = f + s of f inside J's support (|g - r J| <= r - 1
-                        // per axis) and inside the fine grid
-                        const int sj0 = max(-1, max(Jj * ry - (ry - 1) - fj, -fj)),
-                                  sj1 = min(1, min(Jj * ry + (ry - 1) - fj, F.NJ - 1 - fj));
-                        const int si0 = max(-1, max(Ji * rx - (rx - 1) - fi, -fi)),
-                                  si1 = min(1, min(Ji * rx + (rx - 1) - fi, F.NI - 1 - fi));
-                        const int sk0 = max(-1, max(Jk * rz - (rz - 1) - fk, -fk)),
-                                  sk1 = min(1, min(Jk * rz + (rz - 1) - fk, F.NK - 1 - fk));
-                        for (int sk = sk0; sk <= sk1; ++sk)
-                            for (int si = si0; si <= si1; ++si)
-                                for (int sj = sj0; sj <= sj1; ++sj) {
-                                    const int sn = ((sk + 1) * 3 + si + 1) * 3 + sj + 1;
-                                    const int uj = fj + sj - Jj * ry, ui = fi + si - Ji * rx, uk = fk + sk - Jk * rz;
-                                    const double w = wi * (uj ? 0.5 : 1.0) * (ui ? 0.5 : 1.0) * (uk ? 0.5 : 1.0);
-                                    acc += w * double(F.S[(long long)(sn * 9 + q) * F.nnodes + fn]);
-                                }
+        if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK)
+        for (int o = l; o < 27; o += G) {
+            const int oj = o % 3 - 1, oi = (o / 3) % 3 - 1, ok = o / 9 - 1;
+            const int fj = Ij * ry + oj, fi = Ii * rx + oi, fk = Ik * rz + ok;
+            if (!((ry == 2 || oj == 0) && (rx == 2 || oi == 0) && (rz == 2 || ok == 0) && fj >= 0 && fj < F.NJ &&
+                  fi >= 0 && fi < F.NI && fk >= 0 && fk < F.NK))
+                continue;
+            const double wi = (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
+            const long long fn = ((long long)fk * F.NI + fi) * F.NJ + fj;
+            // stencil neighbours g = f + s of f inside J's support (|g - r J| <= r - 1
+            // per axis) and inside the fine grid
+            const int sj0 = max(-1, max(Jj * ry - (ry - 1) - fj, -fj)),
+                      sj1 = min(1, min(Jj * ry + (ry - 1) - fj, F.NJ - 1 - fj));
+            const int si0 = max(-1, max(Ji * rx - (rx - 1) - fi, -fi)),
+                      si1 = min(1, min(Ji * rx + (rx - 1) - fi, F.NI - 1 - fi));
+            const int sk0 = max(-1, max(Jk * rz - (rz - 1) - fk, -fk)),
+                      sk1 = min(1, min(Jk * rz + (rz - 1) - fk, F.NK - 1 - fk));
+            for (int sk = sk0; sk <= sk1; ++sk)
+                for (int si = si0; si <= si1; ++si)
+                    for (int sj = sj0; sj <= sj1; ++sj) {
+                        const int sn = ((sk + 1) * 3 + si + 1) * 3 + sj + 1;
+                        const int uj = fj + sj - Jj * ry, ui = fi + si - Ji * rx, uk = fk + sk - Jk * rz;
+                        const double w = wi * (uj ? 0.5 : 1.0) * (ui ? 0.5 : 1.0) * (uk ? 0.5 : 1.0);
+                        acc += w * double(F.S[(long long)(sn * 9 + q) * F.nnodes + fn]);
                     }
         }
-        C.S[(long long)e * C.nnodes + In] = float(acc);
+        for (int off = G >> 1; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, G);
+        if (l == 0) C.S[(long long)e * C.nnodes + In] = float(acc);
     }
 }
 
 // Inverse diagonal (0 where the row is zero: a masked DOF) and the Gershgorin
 // bound of a stencil level.
 __global__ void __launch_bounds__(256) k_mg_diag(MgLevel L, MgConst* cst) {
+    // a thread per row (n, c), its 81 stencil words loaded at once
     float best = 0.0f;
-    for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < L.nnodes;
-         n += (long long)gridDim.x * blockDim.x) {
-        for (int c = 0; c < 3; ++c) {
-            const float dg = L.S[(long long)(13 * 9 + 4 * c) * L.nnodes + n];
-            const float di = dg > 0.0f ? 1.0f / dg : 0.0f;
-            L.dinv[3 * n + c] = di;
-            if (di != 0.0f) {
-                float sum = 0.0f;
-                for (int nb = 0; nb < 27; ++nb)
-                    for (int d = 0; d < 3; ++d) sum += fabsf(L.S[(long long)(nb * 9 + 3 * c + d) * L.nnodes + n]);
-                best = fmaxf(best, 1.0001f * sum * di);
-            }
-        }
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.ndof;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = int(unsigned(t) / unsigned(L.nnodes));
+        const long long n = t - (long long)c * L.nnodes;
+        float w[81];
+#pragma unroll
+        for (int k = 0; k < 81; ++k) w[k] = L.S[(long long)((k / 3) * 9 + 3 * c + k % 3) * L.nnodes + n];
+        const float dg = w[13 * 3 + c];
+        const float di = dg > 0.0f ? 1.0f / dg : 0.0f;
+        L.dinv[3 * n + c] = di;
+        float sum = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 81; ++k) sum += fabsf(w[k]);
+        if (di != 0.0f) best = fmaxf(best, 1.0001f * sum * di);
     }
-    if (best > 0.0f) atomicMax(&cst->lam_bits, __float_as_uint(best));
+    block_max_to(&cst->lam_bits, best);
 }
 
 // Chebyshev coefficients of every level from its bound.

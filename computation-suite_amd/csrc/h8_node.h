@@ -378,6 +378,94 @@ __device__ __forceinline__ void node_output(const NodeArgs<T>& A, int gn, const 
     }
 }
 
+// Per-node operands of the fused outputs beyond the product (b, d, dinv of a
+// Chebyshev step; b and dinv of the masked residual; f, the mask and dinv32 of
+// the FP64 residual), loaded ahead of their use by the marching sector kernel:
+// a step's node phase then finds them in registers instead of waiting on a
+// memory round trip.  Modes without such operands keep nothing.
+template <int MODE> __host__ __device__ constexpr bool has_node_ex() {
+    return MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_MGRES || MODE == MODE_RESID;
+}
+template <typename T>
+struct NodeEx {
+    T a[3], b[3], c[3];
+};
+
+// Loads of node gn's operands (gn = DOF of component 0; off = false: out of
+// range, the loads return 0 and are not used).  Branch-free buffer loads.
+template <typename T, int MODE>
+__device__ __forceinline__ void node_ex_load(const NodeArgs<T>& A, int gn, bool on, NodeEx<T>& e) {
+    if constexpr (has_node_ex<MODE>()) {
+        const unsigned vbytes = unsigned(A.g.ndof) * sizeof(T);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int o = on ? int(sizeof(T)) * (gn + c) : OOB;
+            if constexpr (MODE == MODE_RESID) {
+                // T = double: f, the mask (FP64 dinv), dinv32 (FP32, widened)
+                e.a[c] = Buf<T>::ld(buf_rsrc(A.f, vbytes), o);
+                e.c[c] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), o);
+                e.b[c] = T(Buf<float>::ld(buf_rsrc(A.dinv32, unsigned(A.g.ndof) * sizeof(float)),
+                                          on ? int(sizeof(float)) * (gn + c) : OOB));
+            } else {
+                e.a[c] = Buf<T>::ld(buf_rsrc(A.cb_b, vbytes), o);
+                e.c[c] = Buf<T>::ld(buf_rsrc(A.dinv, vbytes), o);
+                if constexpr (MODE == MODE_CHEB)
+                    e.b[c] = Buf<T>::ld(buf_rsrc(A.cb_d, vbytes), A.cheb_step ? o : OOB);
+                else
+                    e.b[c] = T(0);
+            }
+        }
+    }
+}
+
+// node_output with the operands of NodeEx (the same arithmetic, in the same
+// order, as node_output's loads; other modes go to node_output).
+template <typename T, int MODE>
+__device__ __forceinline__ void node_output_ex(const NodeArgs<T>& A, int gn, const T (&qv)[3], const T (&ctr)[3],
+                                               const NodeEx<T>& e, double (&part)[2]) {
+    if constexpr (MODE == MODE_MGRES) {
+        const unsigned vbytes = unsigned(A.g.ndof) * sizeof(T);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int q = gn + c;
+            Buf<T>::st(e.c[c] != T(0) ? e.a[c] - qv[c] : T(0), buf_rsrc(A.y, vbytes), int(sizeof(T)) * q);
+        }
+    } else if constexpr (MODE == MODE_CHEB0) {
+        const T ca = T(A.cheb[1]), cb = T(A.cheb[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int q = gn + c;
+            const T dn = ca * ctr[c] + cb * e.c[c] * (e.a[c] - qv[c]);
+            A.cb_d[q] = dn;
+            A.cb_out[q] = ctr[c] + dn;
+        }
+    } else if constexpr (MODE == MODE_CHEB) {
+        const T ca = A.cheb_step ? T(A.cheb[1]) : T(0), cb = A.cheb_step ? T(A.cheb[2]) : T(A.cheb[0]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int q = gn + c;
+            const T b = e.a[c];
+            const T dn = (A.cheb_step ? ca * e.b[c] : T(0)) + cb * e.c[c] * (b - qv[c]);
+            A.cb_d[q] = dn;
+            const T xo = ctr[c] + dn;
+            A.cb_out[q] = xo;
+            part[0] += double(b) * double(xo);
+        }
+    } else if constexpr (MODE == MODE_RESID) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double msk = e.c[c];
+            const double r64 = (msk != 0.0) ? (double(e.a[c]) - double(qv[c])) : 0.0;
+            const float r32 = float(r64);
+            A.r32[gn + c] = r32;
+            part[0] += r64 * r64;
+            part[1] += double(r32) * double(float(e.b[c]) * r32);
+        }
+    } else {
+        node_output<T, MODE>(A, gn, qv, ctr, part);
+    }
+}
+
 // One brick of the node-centric K*u: stage (unless PF staged it already),
 // compute the owned nodes, write the outputs, accumulate the fused partials.
 template <typename T, int TJ, int TI, int TK, int MODE>

@@ -424,6 +424,11 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh,
     __syncthreads();
 
     T carry[3] = {T(0), T(0), T(0)};
+    // the node operands of the fused outputs, one layer ahead: step s loads
+    // layer s + 1's into ex_next, and ex holds layer s's (loaded in step s - 1)
+    NodeEx<T> ex, ex_next;
+    const int gn0 = 3 * (ni * g.NJ + nj);                   // DOF of the thread's node in layer 0
+    const int lstride = 3 * g.NI * g.NJ;
     for (int s = kb - 1; s < ke; ++s) {
         const int q = s - (kb - 1);                          // step index
         const T* lo = sRing + (q % 3) * L::LW;               // layer s
@@ -431,6 +436,8 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh,
         // prefetch layer s + 2 and the next slab's element scale
         const bool more = s + 2 <= ke;
         if (more) lay.template load<MODE>(A, p_old, first, s + 2, pending && s + 2 < ke);
+        if constexpr (has_node_ex<MODE>())
+            node_ex_load<T, MODE>(A, gn0 + (s + 1) * lstride, nown && s + 1 < ke, ex_next);
         const T se = se_next;
         se_next = Buf<T>::ld(buf_rsrc(A.s_e, sbytes),
                              (ecol && s + 1 < g.nz && s + 1 < ke) ? int(sizeof(T)) * (((s + 1) * g.nx + ie) * g.ny + je)
@@ -489,7 +496,12 @@ __device__ __forceinline__ void sector_march(const NodeArgs<T>& A, const TA* kh,
                 ctr[c] = lo[lbase + c];
                 carry[c] = sh;
             }
-            if (nown && s >= kb) node_output<T, MODE>(A, 3 * ((s * g.NI + ni) * g.NJ + nj), qv, ctr, part);
+            if constexpr (has_node_ex<MODE>()) {
+                if (nown && s >= kb) node_output_ex<T, MODE>(A, gn0 + s * lstride, qv, ctr, ex, part);
+                ex = ex_next;
+            } else {
+                if (nown && s >= kb) node_output<T, MODE>(A, 3 * ((s * g.NI + ni) * g.NJ + nj), qv, ctr, part);
+            }
         }
         // ring refill with layer s + 2 (its slot is not read in this step)
         if (more)

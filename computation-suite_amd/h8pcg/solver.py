@@ -297,11 +297,18 @@ class FESystem:
 
     # ---- solve (fe/solver.py:1726-2045) ----------------------------------------------------
     def _dirichlet_rhs(self, op, F_host):
-        """b = mask(F - K u_D) on the device and the full u_D vector (None if all zero)."""
+        """b = mask(F - K u_D) on the device and the full u_D vector (None if all zero).
+        F_host: a host array, or a float64 tensor already on the operator's device
+        (used in place: no host round trip)."""
         import torch
         fixed = self.fixed_dofs_array
         u_fixed = self._fixed_dof_values_array(fixed)
-        F_t = torch.as_tensor(np.asarray(F_host, dtype=float), device=op.device)
+        if isinstance(F_host, torch.Tensor) and F_host.device == torch.device(op.device):
+            if F_host.dtype != torch.float64 or F_host.numel() != self.n_dof:
+                raise ValueError(f"F must be a float64 vector of {self.n_dof} entries")
+            F_t = F_host.reshape(-1)
+        else:
+            F_t = torch.as_tensor(np.asarray(F_host, dtype=float), device=op.device)
         if len(fixed) and np.any(u_fixed):
             uD = np.zeros(self.n_dof)
             uD[fixed] = u_fixed
@@ -476,6 +483,8 @@ class FESystem:
     def solve_static_device(self, method=None, tol=1e-8, maxiter=None, x0=None, preconditioner=None,
                             precision="fp64", delta=None, F=None):
         """solve_static, returning the full displacement as a device tensor (no host copy).
+        F: the load vector (default: the assembled one); a float64 tensor on the
+        solve's device is used where it lies, without a host copy.
         preconditioner: None (the default: 'jacobi' in FP64, the reference's
         choice; 'mg' with precision='mixed'), 'jacobi', or 'mg' (geometric
         multigrid V-cycle, precision='mixed' only)."""
