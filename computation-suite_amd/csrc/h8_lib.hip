@@ -1732,12 +1732,18 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
     const MgLevel& Lc = P.L[nl - 1];
     const long long nc = Lc.ndof;
     hipLaunchKernelGGL(k_fill<double>, dim3(mg_blocks(nc * nc)), dim3(256), 0, s, nc * nc, P.A, 0.0);
-    hipLaunchKernelGGL(k_mg_dense, dim3(mg_blocks(nc)), dim3(256), 0, s, Lc, P.A);
+    hipLaunchKernelGGL(k_mg_dense, dim3(mg_blocks(27 * nc)), dim3(256), 0, s, Lc, P.A);
     if (nc <= MG_LDS_GJ) {
-        const unsigned lds = unsigned((nc * nc + 2 * nc) * sizeof(double));
-        HIPCHK(hipFuncSetAttribute((const void*)k_mg_gauss_jordan<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   int((MG_LDS_GJ * MG_LDS_GJ + 2 * MG_LDS_GJ) * sizeof(double))));
-        hipLaunchKernelGGL(k_mg_gauss_jordan<true>, dim3(1), dim3(1024), lds, s, P.A, int(nc), P.Ainv, P.flag);
+        // the register form: row/column buffers of both parities and the diagonal
+        const unsigned lds = unsigned(5 * nc * sizeof(double));
+        const long long per = (nc * nc + 1023) / 1024;      // register slots per thread
+        if (per <= 4)
+            hipLaunchKernelGGL((k_mg_gauss_jordan<true, 4>), dim3(1), dim3(1024), lds, s, P.A, int(nc), P.Ainv, P.flag);
+        else if (per <= 8)
+            hipLaunchKernelGGL((k_mg_gauss_jordan<true, 8>), dim3(1), dim3(1024), lds, s, P.A, int(nc), P.Ainv, P.flag);
+        else
+            hipLaunchKernelGGL((k_mg_gauss_jordan<true, 16>), dim3(1), dim3(1024), lds, s, P.A, int(nc), P.Ainv,
+                               P.flag);
     } else {
         hipLaunchKernelGGL(k_mg_gauss_jordan<false>, dim3(1), dim3(1024), unsigned(2 * nc * sizeof(double)), s, P.A,
                            int(nc), P.Ainv, P.flag);

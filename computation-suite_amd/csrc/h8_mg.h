@@ -43,7 +43,7 @@ namespace h8 {
 constexpr int MG_MAX_LEVELS = 12;
 constexpr int MG_STENCIL = 243;            // 27 neighbours x 3 x 3
 constexpr long long MG_COARSE_DOF = 128;   // stop coarsening at or below this many DOFs
-constexpr int MG_LDS_GJ = 128;             // coarsest levels up to this many DOFs are inverted in LDS
+constexpr int MG_LDS_GJ = 128;             // coarsest levels up to this many DOFs are inverted in registers
 constexpr long long MG_DENSE_MAX = 512;    // the coarsest level must be this small (dense inverse)
 #ifndef H8_MG_CHEB_RATIO
 #define H8_MG_CHEB_RATIO 30.0f
@@ -110,14 +110,18 @@ __global__ void __launch_bounds__(256) k_mg_mc(const double* __restrict__ K0, in
             continue;
         }
         const int A = row / 3, i = row % 3, B = col / 3, j = col % 3;
+        // all 64 terms, the 8 of a corner a unrolled so that their K0 loads are
+        // in flight together; a zero weight adds an exact zero, so the sum
+        // equals that of the nonzero terms alone
         double acc = 0.0;
+#pragma unroll 1
         for (int a = 0; a < 8; ++a) {
             const double pa = mg_w(cx + OX(a), OX(A), rx) * mg_w(cy + OY(a), OY(A), ry) * mg_w(cz + OZ(a), OZ(A), rz);
-            if (pa == 0.0) continue;
+#pragma unroll
             for (int b = 0; b < 8; ++b) {
                 const double pb =
                     mg_w(cx + OX(b), OX(B), rx) * mg_w(cy + OY(b), OY(B), ry) * mg_w(cz + OZ(b), OZ(B), rz);
-                if (pb != 0.0) acc += pa * K0[(3 * a + i) * 24 + 3 * b + j] * pb;
+                acc += pa * K0[(3 * a + i) * 24 + 3 * b + j] * pb;
             }
         }
         M[t] = acc;
@@ -396,25 +400,27 @@ __global__ void k_mg_consts(MgConst* cst, int nlev) {
 // Dense matrix of the coarsest level (FP64, row-major, n = L.ndof), identity
 // on masked rows; A must be zero on entry.
 __global__ void __launch_bounds__(256) k_mg_dense(MgLevel L, double* __restrict__ A) {
+    // a thread per (row, neighbour block): the 27 blocks of a row are
+    // independent, so their loads need not wait on one another
     const long long n = L.ndof;
-    for (long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x; row < n;
-         row += (long long)gridDim.x * blockDim.x) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < 27 * n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long row = t / 27;
+        const int nb = int(t - row * 27);
         const long long nd = row / 3;
         const int c = int(row % 3);
         if (L.dinv[row] == 0.0f) {
-            A[row * n + row] = 1.0;
+            if (nb == 13) A[row * n + row] = 1.0;
             continue;
         }
         int j, i, k;
         mg_decode(L, nd, j, i, k);
-        for (int nb = 0; nb < 27; ++nb) {
-            const int mj = j + nb % 3 - 1, mi = i + (nb / 3) % 3 - 1, mk = k + nb / 9 - 1;
-            if (mj < 0 || mj >= L.NJ || mi < 0 || mi >= L.NI || mk < 0 || mk >= L.NK) continue;
-            const long long m = ((long long)mk * L.NI + mi) * L.NJ + mj;
-            for (int d = 0; d < 3; ++d)
-                if (L.dinv[3 * m + d] != 0.0f)
-                    A[row * n + 3 * m + d] = double(L.S[(long long)(nb * 9 + 3 * c + d) * L.nnodes + nd]);
-        }
+        const int mj = j + nb % 3 - 1, mi = i + (nb / 3) % 3 - 1, mk = k + nb / 9 - 1;
+        if (mj < 0 || mj >= L.NJ || mi < 0 || mi >= L.NI || mk < 0 || mk >= L.NK) continue;
+        const long long m = ((long long)mk * L.NI + mi) * L.NJ + mj;
+        for (int d = 0; d < 3; ++d)
+            if (L.dinv[3 * m + d] != 0.0f)
+                A[row * n + 3 * m + d] = double(L.S[(long long)(nb * 9 + 3 * c + d) * L.nnodes + nd]);
     }
 }
 
@@ -423,52 +429,104 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLevel L, double* __restrict_
 // *flag = 1 if a pivot is not above 1e-10 of its row's original diagonal (a
 // safety net for an exactly singular coarse operator; the host does not use
 // the V-cycle when the supports leave rigid-body motions free).  INLDS: the
-// matrix is held in LDS (n <= MG_LDS_GJ; dynamic LDS 8 n^2 + 16 n bytes),
-// else in global memory (dynamic LDS 16 n bytes).
-template <bool INLDS>
+// matrix is held in registers (n <= MG_LDS_GJ; dynamic LDS 40 n bytes for the
+// row/column buffers and the diagonal), else in global memory (dynamic LDS
+// 16 n bytes).
+template <bool INLDS, int PER = 1>
 __global__ void __launch_bounds__(1024) k_mg_gauss_jordan(double* __restrict__ Ag, int n, float* __restrict__ Ainv,
                                                          int* flag) {
     extern __shared__ double smem_gj[];
-    double* fcol = smem_gj;         // column k before the step
-    double* diag0 = smem_gj + n;    // the original diagonal
-    double* A = INLDS ? smem_gj + 2 * n : Ag;
-    __shared__ double s_piv;
     __shared__ int s_bad;
-    const long long nn = (long long)n * n;
+    const int nn = n * n;
     if (threadIdx.x == 0) s_bad = 0;
-    if (INLDS)
-        for (long long t = threadIdx.x; t < nn; t += blockDim.x) A[t] = Ag[t];
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) diag0[j] = A[(long long)j * n + j];
-    __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        if (threadIdx.x == 0) {
-            double p = A[(long long)k * n + k];
+    if constexpr (INLDS) {
+        // n <= MG_LDS_GJ: thread t holds entries t + 1024 e, e < PER, of A in
+        // registers (PER = ceil(n^2 / 1024): every slot a thread walks in a step
+        // costs its VALU time, used or not); step k passes row k and column k through
+        // LDS buffers of alternating parity, so a step needs one barrier (the
+        // buffers of step k + 1 are not those any thread may still read in
+        // step k).  Same arithmetic, in the same order, as the LDS form below.
+        double* buf = smem_gj;               // [2][row k (n), column k (n)]
+        double* diag0 = smem_gj + 4 * n;     // the original diagonal
+        double a[PER];
+        int ri[PER], ci[PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int t = int(threadIdx.x) + 1024 * e;
+            ri[e] = t < nn ? t / n : -1;
+            ci[e] = t < nn ? t - (t / n) * n : -1;
+            a[e] = t < nn ? Ag[t] : 0.0;
+            if (t < nn && ri[e] == ci[e]) diag0[ri[e]] = a[e];
+        }
+        int bad = 0;
+        for (int k = 0; k < n; ++k) {
+            double* fr = buf + (k & 1) * 2 * n;
+            double* fc = fr + n;
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                if (ri[e] == k) fr[ci[e]] = a[e];
+                if (ci[e] == k) fc[ri[e]] = a[e];
+            }
+            __syncthreads();
+            double p = fc[k];
             if (!(p > 1e-10 * diag0[k])) {
-                s_bad = 1;
+                bad = 1;
                 p = 1.0;
             }
-            s_piv = 1.0 / p;
+            const double ip = 1.0 / p;
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int i = ri[e], j = ci[e];
+                if (i < 0) continue;
+                if (i == k) a[e] = (j == k) ? ip : a[e] * ip;
+                else if (j == k) a[e] = -fc[i] * ip;
+                else a[e] -= fc[i] * (fr[j] * ip);
+            }
         }
-        __syncthreads();
-        const double ip = s_piv;
-        // scale row k (A[k][k] becomes 1 / pivot), keep the old column k
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
-            fcol[j] = A[(long long)j * n + k];
-            A[(long long)k * n + j] = (j == k) ? ip : A[(long long)k * n + j] * ip;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int t = int(threadIdx.x) + 1024 * e;
+            if (t < nn) Ainv[t] = float(a[e]);
         }
+        if (threadIdx.x == 0) *flag = bad;
+        return;
+    } else {
+        double* fcol = smem_gj;         // column k before the step
+        double* diag0 = smem_gj + n;    // the original diagonal
+        double* A = Ag;
+        __shared__ double s_piv;
         __syncthreads();
-        for (long long t = threadIdx.x; t < nn; t += blockDim.x) {
-            const int i = int(t / n), j = int(t - (long long)i * n);
-            if (i == k) continue;
-            const double f = fcol[i];
-            if (j == k) A[t] = -f * ip;
-            else A[t] -= f * A[(long long)k * n + j];
+        for (int j = threadIdx.x; j < n; j += blockDim.x) diag0[j] = A[(long long)j * n + j];
+        __syncthreads();
+        for (int k = 0; k < n; ++k) {
+            if (threadIdx.x == 0) {
+                double p = A[(long long)k * n + k];
+                if (!(p > 1e-10 * diag0[k])) {
+                    s_bad = 1;
+                    p = 1.0;
+                }
+                s_piv = 1.0 / p;
+            }
+            __syncthreads();
+            const double ip = s_piv;
+            // scale row k (A[k][k] becomes 1 / pivot), keep the old column k
+            for (int j = threadIdx.x; j < n; j += blockDim.x) {
+                fcol[j] = A[(long long)j * n + k];
+                A[(long long)k * n + j] = (j == k) ? ip : A[(long long)k * n + j] * ip;
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+                const int i = t / n, j = t - i * n;
+                if (i == k) continue;
+                const double f = fcol[i];
+                if (j == k) A[t] = -f * ip;
+                else A[t] -= f * A[(long long)k * n + j];
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) Ainv[t] = float(A[t]);
+        if (threadIdx.x == 0) *flag = s_bad;
     }
-    for (long long t = threadIdx.x; t < nn; t += blockDim.x) Ainv[t] = float(A[t]);
-    if (threadIdx.x == 0) *flag = s_bad;
 }
 
 // x = Ainv b on the coarsest level: one wave per row, lanes over the columns
