@@ -531,6 +531,10 @@ static size_t ws_bytes_ref(const GridDev& d) {
 extern "C" size_t h8_pcg_workspace_bytes(const h8_grid* g, int precision) {
     GridDev d;
     if (make_grid(g, d)) return 0;
+    if (precision != 32 && precision != 64) {
+        fail(H8_ERR_ARG, "precision must be 64 or 32, got %d", precision);
+        return 0;
+    }
     return precision == 32 ? ws_bytes_ref(d) : ws_bytes_64(d);
 }
 
@@ -2415,7 +2419,9 @@ struct BLayout {
 
 int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLayout& L) {
     if (!inst || n_inst < 1) return fail(H8_ERR_ARG, "no instances");
+    if (total_vec < 0) return fail(H8_ERR_ARG, "total_vec must be >= 0, got %lld", total_vec);
     int maxt = 1, maxc = 1;
+    long long n_sys = 0, n_tiles = 0, n_chunks = 0;     // (the tables index them with int)
     for (int i = 0; i < n_inst; ++i) {
         const h8_instance& I = inst[i];
         if (I.nx < 1 || I.ny < 1 || I.nz < 1 || I.nrhs < 1)
@@ -2425,13 +2431,18 @@ int batched_layout(const h8_instance* inst, int n_inst, long long total_vec, BLa
         int rc = make_grid(&g, d);
         if (rc) return rc;
         sector_tiles(d, BEJ, BEI, BKD);
-        L.n_sys += I.nrhs;
-        L.n_tiles += d.ntiles * I.nrhs;
-        const int nch = int((d.ndof + BCH - 1) / BCH);
-        L.n_chunks += nch * I.nrhs;
+        const long long nch = (d.ndof + BCH - 1) / BCH;
+        n_sys += I.nrhs;
+        n_tiles += (long long)d.ntiles * I.nrhs;
+        n_chunks += nch * I.nrhs;
+        if (n_tiles > INT_MAX || n_chunks > INT_MAX)
+            return fail(H8_ERR_ARG, "batch too large: more than 2^31 column tiles or vector chunks");
         maxt = std::max(maxt, d.ntiles);
-        maxc = std::max(maxc, nch);
+        maxc = std::max(maxc, int(nch));
     }
+    L.n_sys = int(n_sys);
+    L.n_tiles = int(n_tiles);
+    L.n_chunks = int(n_chunks);
     L.pstride = std::max(maxt, 3 * maxc);
     size_t p = 0;
     auto take = [&](size_t bytes) { size_t q = p; p += align_up(bytes); return q; };

@@ -144,7 +144,8 @@ int h8_spmv_f32(const h8_grid* g, const float* s_e, const float* K0,
                 const float* u, float* y, void* stream);
 
 /* Workspace for h8_pcg_* / h8_refine_solve (bytes), precision 64 or 32 (32 =
- * the refined solve, which needs both FP32 and FP64 vectors). */
+ * the refined solve, which needs both FP32 and FP64 vectors); 0 (and
+ * h8_last_error) for an invalid grid or precision. */
 size_t h8_pcg_workspace_bytes(const h8_grid* g, int precision);
 
 /* Jacobi-PCG, Hestenes-Stiefel recurrence of preconditioned_cg
