@@ -4,6 +4,7 @@
     python -m h8pcg.profsum traffic --fetch <dir> --write <dir> [--calib-bytes 268435456] [--by-grid]
     python -m h8pcg.profsum launches <dir with *_kernel_trace.csv> --kernel NAME [--last 2]
     python -m h8pcg.profsum runs <dir with *_kernel_trace.csv> --kernel NAME [--grid G] [--min-run 20]
+    python -m h8pcg.profsum window <dir with *_kernel_trace.csv> [--marker k_mg_mc] [--top 40]
 
 ``stats`` prints the per-kernel table of a ``--kernel-trace --stats`` run.
 With ``--by-grid`` both ``stats`` and ``traffic`` keep one row per (kernel,
@@ -21,7 +22,10 @@ whole solve phase per launch, the dispatches of one timing run).  ``runs`` finds
 the runs of at least --min-run consecutive dispatches of one kernel (name
 containing NAME, grid size G if given) -- the back-to-back launches of a timing
 graph -- and prints each run's mean duration, for comparison with the HIP-event
-average a timing helper reports for the same launches.
+average a timing helper reports for the same launches.  ``window`` summarises
+the dispatches from the last one whose name contains --marker (k_mg_mc: the
+first kernel of a multigrid set-up, so the last solve of a run) to the end of
+the trace: wall time, the time kernels ran, and one row per (kernel, grid).
 """
 from __future__ import annotations
 
@@ -148,6 +152,29 @@ def runs(path, kernel, grid, min_run):
               f"{len(d)} | {sum(d) / len(d):.2f} | {min(d):.2f} | {max(d):.2f} |")
 
 
+def window(path, marker, top):
+    f = _one(os.path.join(path, "**", "*kernel_trace.csv"))
+    rows = sorted(csv.DictReader(open(f)), key=lambda r: int(r["Start_Timestamp"]))
+    idx = [i for i, r in enumerate(rows) if marker in r["Kernel_Name"]]
+    if not idx:
+        raise SystemExit(f"profsum: no dispatch of {marker}")
+    w = rows[idx[-1]:]
+    t0 = int(w[0]["Start_Timestamp"])
+    t1 = max(int(r["End_Timestamp"]) for r in w)
+    acc = collections.defaultdict(list)
+    for r in w:
+        acc[(r["Kernel_Name"], int(r["Grid_Size_X"]))].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    busy = sum(sum(v) for v in acc.values())
+    print(f"{len(w)} dispatches from the last {marker}: wall {(t1 - t0) / 1e3:.1f} us, kernels running "
+          f"{busy / 1e3:.1f} us ({100.0 * busy / (t1 - t0):.0f}%)")
+    print()
+    print("| kernel | grid | calls | total us | avg us | % of kernel time |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for (k, g), v in sorted(acc.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        print(f"| `{short_name(k)[:70]}` | {g} | {len(v)} | {sum(v) / 1e3:.1f} | {sum(v) / len(v) / 1e3:.2f} | "
+              f"{100.0 * sum(v) / busy:.1f} |")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -169,6 +196,10 @@ def main(argv=None):
     rn.add_argument("--kernel", required=True)
     rn.add_argument("--grid", type=int, default=None)
     rn.add_argument("--min-run", type=int, default=20)
+    wn = sub.add_parser("window")
+    wn.add_argument("dir")
+    wn.add_argument("--marker", default="k_mg_mc")
+    wn.add_argument("--top", type=int, default=40)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         stats(a.dir, a.top, a.by_grid)
@@ -176,6 +207,8 @@ def main(argv=None):
         launches(a.dir, a.kernel, a.last)
     elif a.cmd == "runs":
         runs(a.dir, a.kernel, a.grid, a.min_run)
+    elif a.cmd == "window":
+        window(a.dir, a.marker, a.top)
     else:
         traffic(a.fetch, a.write, a.calib_bytes, a.by_grid)
 

@@ -5,6 +5,7 @@
     python -m h8pcg.workload_runs batched --seed 4 --n 256 --tol 1e-8
     python -m h8pcg.workload_runs simploop --dims 96,32,16 --iters 20 --vf 0.4 --radius 1.5
     python -m h8pcg.workload_runs hex20 --dims 64,16,16 --tol 1e-8
+    python -m h8pcg.workload_runs refined --dims 128,32,32 --seed 2 --reps 12
 
 simp: SIMP cantilever on an nx x ny x nz unit-cube grid, x = 0 face clamped,
 F_z = -1 at the node (nx, ny/2, 0); stiffness scale s = rho_min + (1 - rho_min)
@@ -20,6 +21,13 @@ hex20: a Hex20 cantilever of nx x ny x nz unit bricks (box_mesh_hex20, E = 210e9
 nu = 0.3, x = 0 clamped, -1e6 in z spread over the x = nx nodes), FP64
 Jacobi-PCG to tol; prints DOFs, iterations, device time per iteration and the
 average time of the operator K u alone.
+
+refined: the large single solve -- an nx x ny x nz cantilever (workloads.cantilever,
+-1e6 consistent tip traction, per-element E = 10**U(9, 12) from seed) -- solved
+`reps` times in the chosen form (FP32 iterations preconditioned by the multigrid
+V-cycle, FP64 residual replacement, true residual < tol) on a device-resident
+load vector; prints the per-solve wall times and the last solve's statistics.
+simp --reps n repeats the forward + gradient pass n times and reports the last.
 
 batched: solve_many over batched_cantilevers(seed, n) (nx from {8, 12, .., 48},
 ny = nz = nx/4, 4..16 unit tip-traction load cases each, E = 10**U(9, 11) per
@@ -65,18 +73,21 @@ def simp(a):
         pb.op.autotune()
     rho = torch.as_tensor(rho_np, device=pb.device).requires_grad_(True)
     setup_s = time.time() - t0
-    _sync()
-    t1 = time.perf_counter()
-    C = topopt.compliance(rho, pb)
-    _sync()
-    t2 = time.perf_counter()
-    C.backward()
-    _sync()
-    t3 = time.perf_counter()
+    for _ in range(max(1, a.reps)):
+        rho.grad = None
+        _sync()
+        t1 = time.perf_counter()
+        C = topopt.compliance(rho, pb)
+        _sync()
+        t2 = time.perf_counter()
+        C.backward()
+        _sync()
+        t3 = time.perf_counter()
     fwd = dict(pb.last_stats)
     g = rho.grad.detach().cpu().numpy()
     out = {"workload": "simp", "dims": [nx, ny, nz], "n_dof": n_dof, "n_free": n_dof - len(fixed), "elements": m,
            "rho": [lo, hi], "seed": a.seed, "precision": a.precision, "tol": a.tol, "setup_s": setup_s,
+           "reps": max(1, a.reps),
            "compliance": float(C), "forward_ms": 1e3 * (t2 - t1), "gradient_ms": 1e3 * (t3 - t2),
            "forward_stats": {k_: fwd.get(k_) for k_ in ("iterations", "inner_iterations", "outer",
                                                         "rel_residual", "true_rel_residual")}}
@@ -94,6 +105,28 @@ def simp(a):
         rel = np.abs(fd - g[els]) / np.abs(g[els])
         out.update({"fd_elements": int(a.fd), "fd_h": 1e-6, "fd_s": time.perf_counter() - t4,
                     "fd_max_rel_diff": float(rel.max()), "fd_median_rel_diff": float(np.median(rel))})
+    print(json.dumps(out), flush=True)
+
+
+def refined(a):
+    import torch
+    from .workloads import cantilever
+    nx, ny, nz = (int(v) for v in a.dims.split(","))
+    E = 10 ** np.random.default_rng(a.seed).uniform(9, 12, nx * ny * nz)
+    fs = cantilever(nx, ny, nz, E_elem=E, device="cuda:0")
+    F = torch.as_tensor(np.asarray(fs.F, dtype=float), device="cuda:0")
+    times = []
+    for _ in range(max(1, a.reps)):
+        _sync()
+        t = time.perf_counter()
+        fs.solve_static_device(tol=a.tol, precision="mixed", F=F)
+        _sync()
+        times.append(1e3 * (time.perf_counter() - t))
+    st = fs.last_solve_stats
+    out = {"workload": "refined", "dims": [nx, ny, nz], "seed": a.seed, "tol": a.tol,
+           "n_free": fs.n_dof - len(fs.fixed_dofs), "solve_ms": times,
+           "stats": {k: st.get(k) for k in ("inner_iterations", "outer", "true_rel_residual", "elapsed_ms",
+                                            "launches", "preconditioner")}}
     print(json.dumps(out), flush=True)
 
 
@@ -207,6 +240,12 @@ def main(argv=None):
     s.add_argument("--tol", type=float, default=1e-10)
     s.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
     s.add_argument("--autotune", action="store_true")
+    s.add_argument("--reps", type=int, default=1)
+    rf = sub.add_parser("refined")
+    rf.add_argument("--dims", default="128,32,32")
+    rf.add_argument("--seed", type=int, default=2)
+    rf.add_argument("--tol", type=float, default=1e-8)
+    rf.add_argument("--reps", type=int, default=12)
     b = sub.add_parser("batched")
     b.add_argument("--seed", type=int, default=4)
     b.add_argument("--n", type=int, default=256)
@@ -225,7 +264,7 @@ def main(argv=None):
     h.add_argument("--tol", type=float, default=1e-8)
     h.add_argument("--reps", type=int, default=50)
     a = ap.parse_args(argv)
-    {"simp": simp, "batched": batched, "simploop": simploop, "hex20": hex20}[a.cmd](a)
+    {"simp": simp, "batched": batched, "simploop": simploop, "hex20": hex20, "refined": refined}[a.cmd](a)
 
 
 if __name__ == "__main__":
