@@ -65,6 +65,8 @@ struct PcgState {
     int mg;            // refined solve preconditioned by the multigrid V-cycle: beta and
                        // rz_old come from r.z of the cycle's z (k_mg_rz), not from dinv r
     int replaced;      // multigrid: the residual was replaced since the last r.z
+    int fold;          // multigrid: the FP64 residual was formed at u + x32 and u has not
+                       // taken x32 yet -- the next update kernel folds it (u += x32)
 };
 
 // Refined solve: replacements without a 1% drop of the true residual after

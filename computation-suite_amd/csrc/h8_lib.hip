@@ -523,9 +523,12 @@ static size_t ws_bytes_64(const GridDev& d) {
     return 7 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
 }
 static size_t ws_bytes_ref(const GridDev& d) {
-    // r, x, p0, p1, q, dinv32, partials, state, and the two z-exchange vectors
+    // r, x, p0, p1, q, dinv32, partials, state, the two z-exchange vectors, and
+    // the multigrid form's FP64 copies of f and u (its graphs then hold only
+    // pointers that do not change between solves)
     const size_t nv = align_up(sizeof(float) * d.ndof);
-    return 8 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState));
+    return 8 * nv + align_up(sizeof(double) * partial_slots(d)) + align_up(sizeof(PcgState)) +
+           2 * align_up(sizeof(double) * d.ndof);
 }
 
 extern "C" size_t h8_pcg_workspace_bytes(const h8_grid* g, int precision) {
@@ -735,10 +738,12 @@ static void launch_update(const GridDev& d, T* x, T* r, T* p0, T* p1, T* q, cons
                           double* partials, hipStream_t s) {
     if (defer_x<T>(d))
         hipLaunchKernelGGL((k_pcg_update<T, REFINED, false>), dim3(update_blocks<T>(d.ndof)), dim3(STREAM_THREADS),
-                           0, s, d.ndof, x, r, (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials);
+                           0, s, d.ndof, x, r, (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials,
+                           (double*)nullptr);
     else
         hipLaunchKernelGGL((k_pcg_update<T, REFINED, true>), dim3(update_blocks<T>(d.ndof)), dim3(STREAM_THREADS),
-                           0, s, d.ndof, x, r, (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials);
+                           0, s, d.ndof, x, r, (const T*)p0, (const T*)p1, (const T*)q, dinv, st, partials,
+                           (double*)nullptr);
 }
 
 // One PCG iteration = direction+product kernel (node-centric) + update kernel.
@@ -1353,7 +1358,8 @@ extern "C" int h8_pcg_constrained_f64(const h8_grid* g, const double* s_e, const
                     hipLaunchKernelGGL(k_cfix, dim3(gf), dim3(256), 0, cs, m, pv, q, st);
                 hipLaunchKernelGGL(k_cpq, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, pv, q, st, partials);
                 hipLaunchKernelGGL((k_pcg_update<double, false, true>), dim3(update_blocks<double>(d.ndof)),
-                                   dim3(STREAM_THREADS), 0, cs, d.ndof, x, r, pv, pv, q, dinv, st, partials);
+                                   dim3(STREAM_THREADS), 0, cs, d.ndof, x, r, pv, pv, q, dinv, st, partials,
+                                   (double*)nullptr);
             }
             LAUNCHCHK();
             return H8_OK;
@@ -1728,8 +1734,15 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
         const long long entries = 243 * P.L[l + 1].nnodes;
         int lg = 0;
         while (lg < 5 && (entries << (lg + 1)) <= (1LL << 20)) ++lg;
-        hipLaunchKernelGGL(k_mg_rap, dim3(mg_blocks(entries << lg)), dim3(256), 0, s, F, P.L[l + 1], F.rx, F.ry,
-                           F.rz, lg);
+        if (lg == 0) {
+            // a large level: one entry per block over a chunk of coarse nodes, XCD-local chunks
+            const long long nblk = 243 * ((P.L[l + 1].nnodes + MG_RAP_CHUNK - 1) / MG_RAP_CHUNK);
+            hipLaunchKernelGGL(k_mg_rap_chunked, dim3(unsigned(nblk)), dim3(MG_RAP_CHUNK), 0, s, F, P.L[l + 1], F.rx,
+                               F.ry, F.rz);
+        } else {
+            hipLaunchKernelGGL(k_mg_rap, dim3(mg_blocks(entries << lg)), dim3(256), 0, s, F, P.L[l + 1], F.rx, F.ry,
+                               F.rz, lg);
+        }
         hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[l + 1].ndof)), dim3(256), 0, s, P.L[l + 1],
                            P.cst + l + 1);
     }
@@ -1866,8 +1879,9 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
 // The refined solve of refine_solve with z = V(r) in the FP32 iterations:
 // per iteration p = z + beta p (sector kernel, MODE_DIRZ), q = K p, alpha;
 // x += alpha p, r -= alpha q and the replacement test; on a replacement the
-// fold and the FP64 residual (both no-ops otherwise); z = V(r); r.z -> beta.
-// All of it is captured in graphs of MG_CHUNK iterations.
+// FP64 residual at u + x (a no-op otherwise; the next update kernel folds x
+// into u, and one fold follows the loop); z = V(r); r.z -> beta.  All of it is
+// captured in graphs of MG_CHUNK (or, near the end, 4 or 2) iterations.
 constexpr int MG_CHUNK = 8;
 
 static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
@@ -1903,6 +1917,14 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     w.partials = (double*)carve(p, sizeof(double) * partial_slots(d));
     w.st = (PcgState*)carve(p, sizeof(PcgState));
     float* z = (float*)carve(p, sizeof(float) * d.ndof);      // the first z-exchange vector of the resident solver
+    (void)carve(p, sizeof(float) * d.ndof);                      // the second (unused here)
+    // f and the FP64 solution live in the workspace during the solve, so that
+    // the captured graphs hold no per-call pointers (a caller that allocates
+    // them afresh every solve, as a SIMP loop does, would otherwise capture
+    // every graph again); u is written once at the end
+    double* f_ws = (double*)carve(p, sizeof(double) * d.ndof);
+    double* u_ws = (double*)carve(p, sizeof(double) * d.ndof);
+    double* const u_out = u;
     MgLevel& L0 = P.L[0];
     L0.b = w.r;
     L0.x1 = z;
@@ -1920,11 +1942,14 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     HIPCHK(hipEventRecord(e0, s));
     const int sb = stream_blocks(d.ndof);
     hipLaunchKernelGGL(k_cast_f64_f32, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, dinv64, w.dinv32);
+    HIPCHK(hipMemcpyAsync(f_ws, f, sizeof(double) * d.ndof, hipMemcpyDeviceToDevice, s));
+    f = f_ws;
     if (warm)
-        hipLaunchKernelGGL(k_masked_sub, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, (const double*)u,
-                           (const double*)nullptr, dinv64, u);
+        hipLaunchKernelGGL(k_masked_sub, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, (const double*)u_out,
+                           (const double*)nullptr, dinv64, u_ws);
     else
-        hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u, 0.0);
+        hipLaunchKernelGGL(k_fill<double>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, u_ws, 0.0);
+    u = u_ws;
     hipLaunchKernelGGL(k_fill<float>, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, w.x, 0.0f);
     LAUNCHCHK();
     rc = mg_setup(d, P, s_e64, K064, s_e32, s);
@@ -1944,9 +1969,13 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&flag, P.flag, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (flag)
+    if (flag) {
+        // (u holds the masked initial guess, as documented)
+        HIPCHK(hipMemcpyAsync(u_out, u_ws, sizeof(double) * d.ndof, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
         return fail(H8_MG_DECLINED, "multigrid: the coarsest operator is singular (the supports leave rigid-body "
                     "motions free); use the Jacobi preconditioner");
+    }
     if (warm && hst.status == ST_DONE) {
         hst.status = ST_RUNNING;
         HIPCHK(hipMemcpyAsync(w.st, &hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
@@ -1967,6 +1996,7 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     NodeArgs<double> R{};
     R.g = d64; R.s_e = s_e64; R.K0 = K064; R.u = u; R.dinv = dinv64; R.f = f; R.r32 = w.r;
     R.dinv32 = w.dinv32; R.st = w.st; R.partials = w.partials;
+    R.x32 = w.x;     // the FP64 residual is formed at u + x (x is folded by the next update)
     NodeArgs<float> prod0{};
     prod0.g = d32; prod0.s_e = s_e32; prod0.K0 = K032; prod0.u = z; prod0.y = L0.y; prod0.st = w.st;
     prod0.partials = w.partials;
@@ -1976,15 +2006,6 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
     long long launches = 0;
     hst.true_rel = hst.rel;
     if (hst.status == ST_RUNNING) {
-        // first residual, z = V(r), r.z
-        rc = launch_op<double, MODE_RESID>(d64, R, s);
-        if (rc) return rc;
-        rc = mg_vcycle(d32, P, prod0, w.st, s, true);     // with r.z and beta
-        if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(&hst, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-    }
-    if (hst.status == ST_RUNNING) {
         GraphKey key{};
         key.kind = 4;
         key.variant = d32.variant;
@@ -1992,28 +2013,37 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
         key.arith = d.arith;
         HIPCHK(hipGetDevice(&key.dev));
         key.nx = d.nx; key.ny = d.ny; key.nz = d.nz;
-        const void* ptrs[] = {s_e64, K064, s_e32, K032, dinv64, f, u, ws, mg_ws};
+        const void* ptrs[] = {s_e64, K064, s_e32, K032, dinv64, ws, mg_ws};   // (f and u are in ws)
         std::memcpy(key.ptr, ptrs, sizeof ptrs);
         key.extra[0] = (unsigned long long)d64.variant;   // the FP64 residual's kernel
         key.extra[1] = (unsigned long long)d64.layers;
         NodeArgs<double> Rq = R;
         Rq.only_on_replace = 1;
+        NodeArgs<double> R0 = R;
+        R0.only_on_replace = 2;
         // one captured graph per chunk length (8, 4, 2 iterations): the solve
         // launches full chunks while far from the tolerance and shorter ones at
         // the end, where the iterations of a full chunk past the stop would be
-        // launches that return at once (about 3 us each, 29 per iteration)
-        auto graph_of = [&](int chunk, GraphRef* exec) -> int {
+        // launches that return at once (about 3 us each, 29 per iteration).  The
+        // first chunk also holds the first residual and z = V(r), r.z (otherwise
+        // some 30 launches made one by one from the host)
+        auto graph_of = [&](int chunk, bool first, GraphRef* exec) -> int {
             key.chunk = chunk;
+            key.extra[2] = first ? 1 : 0;
             return get_graph(key, [&](hipStream_t cs) -> int {
+                if (first) {
+                    int r0 = launch_op<double, MODE_RESID>(d64, R0, cs);
+                    if (r0) return r0;
+                    if ((r0 = mg_vcycle(d32, P, prod0, w.st, cs, true))) return r0;     // with r.z and beta
+                }
                 for (int it = 0; it < chunk; ++it) {
                     int r1 = launch_op<float, MODE_DIRZ>(d32, D, cs);
                     if (r1) return r1;
+                    // (after a replacement the update folds x into u first)
                     hipLaunchKernelGGL((k_pcg_update<float, true, true, true>), dim3(update_blocks<float>(d.ndof)),
                                        dim3(STREAM_THREADS), 0, cs, d.ndof, w.x, w.r, (const float*)w.p0,
                                        (const float*)w.p1, (const float*)w.q, (const float*)w.dinv32, w.st,
-                                       w.partials);
-                    hipLaunchKernelGGL(k_fold, dim3(sb), dim3(STREAM_THREADS), 0, cs, d.ndof, u, w.x,
-                                       (const float*)w.p0, (const float*)w.p1, (const PcgState*)w.st);
+                                       w.partials, u);
                     if ((r1 = launch_op<double, MODE_RESID>(d64, Rq, cs))) return r1;
                     if ((r1 = mg_vcycle(d32, P, prod0, w.st, cs, true))) return r1;
                 }
@@ -2024,9 +2054,9 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
         int chunk = int(std::min<long long>(MG_CHUNK, maxiter));
         double rel_prev = hst.rel;
         long long k_prev = hst.k;
-        for (;;) {
+        for (bool first = true;; first = false) {
             GraphRef exec;
-            rc = graph_of(chunk, &exec);
+            rc = graph_of(chunk, first, &exec);
             if (rc) return rc;
             HIPCHK(hipGraphLaunch(exec->exec, s));
             ++launches;
@@ -2048,6 +2078,11 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
             k_prev = hst.k;
         }
     }
+    // the FP32 iterate not yet taken into u (the last replacement's, or the
+    // progress since it when the solve stopped otherwise)
+    hipLaunchKernelGGL(k_fold_final, dim3(sb), dim3(STREAM_THREADS), 0, s, d.ndof, (const double*)u_ws, w.x,
+                       u_out);
+    LAUNCHCHK();
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -2797,7 +2832,8 @@ extern "C" int h20_pcg_f64(int32_t nx, int32_t ny, int32_t nz, const double* s_e
                 hipLaunchKernelGGL(k_h20_node_pcg, dim3(h20_blocks(g)), dim3(H20_THREADS), 0, cs, g, s_e, K0,
                                    (const double*)p0, (const double*)p1, q, st, partials);
                 hipLaunchKernelGGL((k_pcg_update<double, false, true>), dim3(sb), dim3(STREAM_THREADS), 0, cs, n, x,
-                                   r, (const double*)p0, (const double*)p1, (const double*)q, dinv, st, partials);
+                                   r, (const double*)p0, (const double*)p1, (const double*)q, dinv, st, partials,
+                                   (double*)nullptr);
             }
             LAUNCHCHK();
             return H8_OK;

@@ -360,6 +360,53 @@ __global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, in
     }
 }
 
+// The RAP of a large level (k_mg_rap with one lane per entry reads each fine
+// stencil word for up to 27 entries at distant times, from every XCD: about 4 GB
+// of fetch for the 137 MB level-1 stencil of a 256 x 64 x 64 grid).  Here a
+// block computes one entry e (neighbour block and 3 x 3 position) for a chunk of
+// MG_RAP_CHUNK consecutive coarse nodes, and the blocks are numbered so that XCD
+// x walks a contiguous run of (chunk, entry) pairs, chunk-major: the 243 entries
+// of a chunk run back to back on one XCD while the fine rows they read (about
+// 2 MB) stay in its L2.  Same sums, in the same order, as k_mg_rap with G = 1.
+constexpr int MG_RAP_CHUNK = 256;
+__global__ void __launch_bounds__(MG_RAP_CHUNK) k_mg_rap_chunked(MgLevel F, MgLevel C, int rx, int ry, int rz) {
+    const int nblk = int(gridDim.x);
+    const int L = xcd_local_tile(int(blockIdx.x), nblk);     // (chunk, entry), entry fastest
+    const int e = L % 243, chunk = L / 243;
+    const long long In = (long long)chunk * MG_RAP_CHUNK + threadIdx.x;
+    if (In >= C.nnodes) return;
+    const int nb = e / 9, q = e % 9;
+    int Ij, Ii, Ik;
+    mg_decode(C, In, Ij, Ii, Ik);
+    const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
+    double acc = 0.0;
+    if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK)
+        for (int o = 0; o < 27; ++o) {
+            const int oj = o % 3 - 1, oi = (o / 3) % 3 - 1, ok = o / 9 - 1;
+            const int fj = Ij * ry + oj, fi = Ii * rx + oi, fk = Ik * rz + ok;
+            if (!((ry == 2 || oj == 0) && (rx == 2 || oi == 0) && (rz == 2 || ok == 0) && fj >= 0 && fj < F.NJ &&
+                  fi >= 0 && fi < F.NI && fk >= 0 && fk < F.NK))
+                continue;
+            const double wi = (oj ? 0.5 : 1.0) * (oi ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
+            const long long fn = ((long long)fk * F.NI + fi) * F.NJ + fj;
+            const int sj0 = max(-1, max(Jj * ry - (ry - 1) - fj, -fj)),
+                      sj1 = min(1, min(Jj * ry + (ry - 1) - fj, F.NJ - 1 - fj));
+            const int si0 = max(-1, max(Ji * rx - (rx - 1) - fi, -fi)),
+                      si1 = min(1, min(Ji * rx + (rx - 1) - fi, F.NI - 1 - fi));
+            const int sk0 = max(-1, max(Jk * rz - (rz - 1) - fk, -fk)),
+                      sk1 = min(1, min(Jk * rz + (rz - 1) - fk, F.NK - 1 - fk));
+            for (int sk = sk0; sk <= sk1; ++sk)
+                for (int si = si0; si <= si1; ++si)
+                    for (int sj = sj0; sj <= sj1; ++sj) {
+                        const int sn = ((sk + 1) * 3 + si + 1) * 3 + sj + 1;
+                        const int uj = fj + sj - Jj * ry, ui = fi + si - Ji * rx, uk = fk + sk - Jk * rz;
+                        const double w = wi * (uj ? 0.5 : 1.0) * (ui ? 0.5 : 1.0) * (uk ? 0.5 : 1.0);
+                        acc += w * double(F.S[(long long)(sn * 9 + q) * F.nnodes + fn]);
+                    }
+        }
+    C.S[(long long)e * C.nnodes + In] = float(acc);
+}
+
 // Inverse diagonal (0 where the row is zero: a masked DOF) and the Gershgorin
 // bound of a stencil level.
 __global__ void __launch_bounds__(256) k_mg_diag(MgLevel L, MgConst* cst) {

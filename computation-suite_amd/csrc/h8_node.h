@@ -62,7 +62,10 @@ struct NodeArgs {
     const float* dinv32;  // RESID
     PcgState* st;
     double* partials;
-    int only_on_replace;  // RESID: every block returns at once unless st asks for a replacement
+    int only_on_replace;  // RESID: 1 = every block returns at once unless st asks for a replacement,
+                          // 2 = unless the solve is running (the first residual of a captured solve)
+    const float* x32;     // RESID (sector kernel, multigrid solve): the residual is formed at
+                          // u + x32 (the FP32 iterate not yet folded into u); nullptr: at u
     const float* cheb;    // CHEB: the level's coefficients {c0, c1a, c1b} (device)
     const T* cb_b;        // CHEB: right-hand side
     T* cb_d;              // CHEB: Chebyshev direction (updated in place)
@@ -147,8 +150,10 @@ __device__ inline void resid_finish(PcgState* st, double rr, double rz) {
         st->status = ST_RUNNING;
         if (st->mg) {
             // beta and rz_old follow from the cycle's z of the replaced residual
-            // (k_mg_rz); a replacement without progress restarts the direction
+            // (k_mg_rz); a replacement without progress restarts the direction;
+            // the next update kernel folds x32 into u
             st->replaced = 1;
+            st->fold = 1;
             if (!progress) st->first = 1;
             return;
         }
@@ -577,7 +582,8 @@ __device__ __forceinline__ bool node_setup(const NodeArgs<T>& A, T& beta, int& f
     alpha_prev = T(0);
     pending = 0;
     if constexpr (MODE == MODE_RESID) {
-        if (A.only_on_replace && A.st->status != ST_REPLACE) return false;
+        if (A.only_on_replace == 1 && A.st->status != ST_REPLACE) return false;
+        if (A.only_on_replace == 2 && A.st->status != ST_RUNNING) return false;
     }
     if constexpr (MODE == MODE_PLAIN || MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_MGRES) {
         // a product inside a solve's captured iterations (the multigrid cycle)

@@ -306,6 +306,13 @@ struct SLayer {
                     xv[u] = Buf<T>::ld(buf_rsrc(A.x, vbytes), int((unsigned(sofs[u]) + kofs) | kbad | xbad));
             } else {
                 v0[u] = Buf<T>::ld(buf_rsrc(A.u, vbytes), int(off));
+                if constexpr (MODE == MODE_RESID) {
+                    // + the unfolded FP32 iterate (FP32 word = half the FP64 byte offset;
+                    // an empty buffer when there is none, so the load returns 0)
+                    const unsigned xbytes = A.x32 ? unsigned(g.ndof) * sizeof(float) : 0u;
+                    const int o32 = (off & 0x80000000u) ? OOB : int(off >> 1);
+                    v0[u] += T(Buf<float>::ld(buf_rsrc(A.x32, xbytes), o32));
+                }
             }
         }
     }

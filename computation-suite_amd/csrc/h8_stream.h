@@ -100,18 +100,23 @@ __device__ inline void update_finish(PcgState* st, double rz, double rr) {
 // groups per thread per pass (two when x is updated too) with all loads issued
 // first; the workspace and torch allocations are 256-byte aligned, the tail is
 // scalar.
+// MG (with XUPD, T = float): when PcgState::fold is set, the last FP64 residual
+// was formed at u + x (u64 the FP64 solution, x the FP32 iterate): this kernel
+// folds x into u (u += x) and restarts x from alpha p, instead of x += alpha p.
 template <typename T, bool REFINED, bool XUPD, bool MG = false>
 __global__ void __launch_bounds__(STREAM_THREADS)
 k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p0,
              const T* __restrict__ p1, const T* __restrict__ q, const T* __restrict__ dinv, PcgState* st,
-             double* partials) {
+             double* partials, double* __restrict__ u64) {
     constexpr int V = 16 / sizeof(T);
     constexpr int G = XUPD ? 2 : 4;
     typedef T vec __attribute__((ext_vector_type(V)));
+    typedef double dvec2 __attribute__((ext_vector_type(2)));
     __shared__ double sRed[16 * 2];
     if (st->status != ST_RUNNING) return;
     const T alpha = T(st->alpha);
     const T* __restrict__ p = (st->k & 1) ? p1 : p0;
+    const bool fold = MG && XUPD && st->fold;     // (uniform)
     double v[2] = {0.0, 0.0};
     const long long nvec = n / V;
     const long long gstride = (long long)gridDim.x * blockDim.x;
@@ -134,6 +139,19 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             if (!ok[u]) continue;
+            if constexpr (MG && XUPD && sizeof(T) == 4) {
+                if (fold) {
+                    // u64 += x (4 doubles per FP32 group, two 16-byte halves), x = alpha p
+                    dvec2* ud = reinterpret_cast<dvec2*>(u64) + 2 * (iv0 + u * gstride);
+                    dvec2 a = ud[0], b = ud[1];
+                    a[0] += double(xv[u][0]); a[1] += double(xv[u][1]);
+                    b[0] += double(xv[u][2]); b[1] += double(xv[u][3]);
+                    ud[0] = a;
+                    ud[1] = b;
+#pragma unroll
+                    for (int c = 0; c < V; ++c) xv[u][c] = T(0);
+                }
+            }
 #pragma unroll
             for (int c = 0; c < V; ++c) {
                 if constexpr (XUPD) xv[u][c] += alpha * pv[u][c];
@@ -148,6 +166,12 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     }
     for (long long i = nvec * V + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gstride) {
         const T d = dinv[i];
+        if constexpr (MG && XUPD) {
+            if (fold) {
+                u64[i] += double(x[i]);
+                x[i] = T(0);
+            }
+        }
         if constexpr (XUPD) x[i] += alpha * p[i];
         const T ri = (d != T(0)) ? T(r[i] - alpha * q[i]) : T(0);
         r[i] = ri;
@@ -157,6 +181,7 @@ k_pcg_update(long long n, T* __restrict__ x, T* __restrict__ r, const T* __restr
     double tot[2];
     if (reduce_last_block<2>(v, partials, &st->cnt[1], tot, sRed) && threadIdx.x == 0) {
         if (XUPD) st->pending = 0;   // applied above
+        if (MG && XUPD) st->fold = 0;   // every block has read it (they all finished)
         update_finish<REFINED, MG>(st, tot[0], tot[1]);
     }
 }
@@ -174,6 +199,18 @@ k_fold(long long n, double* __restrict__ u, float* __restrict__ x32, const float
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         u[i] += double(x32[i]) + alpha * double(p32[i]);
+        x32[i] = 0.0f;
+    }
+}
+
+// out = u64 + double(x32); x32 = 0 (the end of the multigrid-preconditioned
+// refined solve: the FP32 iterate not yet folded by an update kernel, and the
+// solution copied from the workspace to the caller's vector).
+__global__ void __launch_bounds__(STREAM_THREADS)
+k_fold_final(long long n, const double* __restrict__ u, float* __restrict__ x32, double* __restrict__ out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        out[i] = u[i] + double(x32[i]);
         x32[i] = 0.0f;
     }
 }

@@ -196,7 +196,10 @@ class GridOperator:
             return self.dinv
         fm = torch.as_tensor(fm_host, device=self.device).contiguous()
         self.free_mask = fm
-        dinv = torch.empty(self.n_dof, dtype=torch.float64, device=self.device)
+        # rebuilt in place: solves capture their iterations in graphs keyed on
+        # the buffers they read, so a stable address keeps those graphs valid
+        dinv = self.dinv if self.dinv is not None and self.dinv.numel() == self.n_dof else \
+            torch.empty(self.n_dof, dtype=torch.float64, device=self.device)
         if preconditioner == "jacobi" and zero_diag_flag is not None:
             with torch.cuda.device(self.device):
                 check(_lib.lib().h8_diag_async_f64(C.byref(self.grid), ptr(self.s_e), ptr(self.K0buf), ptr(fm),
@@ -384,13 +387,20 @@ class GridOperator:
             out[p] = (v, l, us)
         return out
 
-    def set_scale(self, s_e):
+    def update_scale_(self, s):
+        """Element scales replaced in place (the FP64 and FP32 buffers keep their
+        addresses, so captured solve graphs stay valid); the Jacobi diagonal is
+        marked stale and rebuilt by the next set_free_mask."""
         import torch
-        s = torch.as_tensor(s_e, dtype=torch.float64, device=self.device).reshape(-1)
+        s = torch.as_tensor(s, dtype=torch.float64, device=self.device).reshape(-1)
         if s.numel() != self.m:
             raise ValueError("s_e length mismatch")
-        self.s_e = s.contiguous()
-        self.s_e32 = self.s_e.float().contiguous()
+        self._s_e.copy_(s)
+        self.s_e32.copy_(s)
+        self._scale_token += 1
+
+    def set_scale(self, s_e):
+        self.update_scale_(s_e)
         if getattr(self, "free_mask", None) is not None and self.dinv is not None:
             self.set_free_mask(self.free_mask.cpu().numpy())
 

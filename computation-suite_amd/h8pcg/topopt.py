@@ -86,8 +86,7 @@ class SIMPProblem:
 
     def _set_rho(self, rho):
         s, ds = simp_scale(rho, self.p, self.rho_min, with_grad=True)
-        self.op.s_e = s.contiguous()
-        self.op.s_e32 = s.float().contiguous()
+        self.op.update_scale_(s)
         self.op.set_free_mask(self.free_mask)
         return s, ds
 

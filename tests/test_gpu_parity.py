@@ -537,10 +537,13 @@ def test_resident_bricks_split_across_j_agree_with_streaming(h8, gpu):
 
 
 def test_diagonal_cache_follows_scales_and_supports(h8, gpu):
-    """Repeated solves of one system reuse the Jacobi diagonal (no rebuild, no
-    host wait); a new element scale (set_scale or a direct s_e assignment, as
-    the SIMP problem does) or a new support set rebuilds it, and each solve
-    matches the oracle's direct solve of its own system."""
+    """Repeated solves of one system reuse the Jacobi diagonal (no diagonal
+    launch, no host wait); a new element scale (set_scale, update_scale_ or a
+    direct s_e assignment) or a new support set rebuilds it -- in place, so
+    captured graphs stay valid -- and each solve matches the oracle's direct
+    solve of its own system."""
+    import torch
+    from h8pcg import _lib
     nx, ny, nz = 10, 3, 3
     rng = np.random.default_rng(11)
     E = 10 ** rng.uniform(9, 11, nx * ny * nz)
@@ -548,37 +551,54 @@ def test_diagonal_cache_follows_scales_and_supports(h8, gpu):
     op = fs.operator
     c = o.cantilever(nx, ny, nz, E_elem=E)
     Ud = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
-    for prec in ("fp64", "mixed"):
-        U1 = fs.solve_static_device(tol=1e-10, precision=prec).cpu().numpy()
-        d1 = op.dinv
-        U2 = fs.solve_static_device(tol=1e-10, precision=prec).cpu().numpy()
-        assert op.dinv is d1, "an unchanged system rebuilt its diagonal"
-        for U in (U1, U2):
-            assert np.linalg.norm(U - Ud) / np.linalg.norm(Ud) < 1e-8
-    # scales doubled through set_scale (which rebuilds the diagonal at once):
-    # half the displacement
-    s0 = op.s_e.cpu().numpy()
-    d1 = op.dinv
-    op.set_scale(2.0 * s0)
-    U = fs.solve_static_device(tol=1e-10, precision="mixed").cpu().numpy()
-    assert op.dinv is not d1
-    assert np.linalg.norm(U - 0.5 * Ud) / np.linalg.norm(0.5 * Ud) < 1e-8
-    # scales assigned directly: the diagonal follows them too
-    import torch
-    d1 = op.dinv
-    op.s_e = torch.as_tensor(4.0 * s0, device=op.device)
-    op.s_e32 = op.s_e.float()
-    U = fs.solve_static_device(tol=1e-10, precision="fp64").cpu().numpy()
-    assert op.dinv is not d1
-    assert np.linalg.norm(U - 0.25 * Ud) / np.linalg.norm(0.25 * Ud) < 1e-8
-    # one more support set (the x = nx face's y components): a new diagonal
-    op.set_scale(s0)
-    tip = fs.mesh.nodes_on_plane(0, float(nx))
-    fs.fix_dofs(tip, [1])
-    d1 = op.dinv
-    U = fs.solve_static_device(tol=1e-10, precision="mixed").cpu().numpy()
-    assert op.dinv is not d1
-    fixed = sorted(set(c["fixed_dofs"]) | {3 * int(n) + 1 for n in tip})
-    Ut = o.solve_direct(c["K"], c["F"], fixed)
-    assert np.linalg.norm(U - Ut) / np.linalg.norm(Ut) < 1e-8
+    L = _lib.lib()
+    orig = L.h8_diag_f64
+    calls = []
+
+    def counted(*args):
+        calls.append(1)
+        return orig(*args)
+
+    L.h8_diag_f64 = counted
+    try:
+        for prec in ("fp64", "mixed"):
+            U1 = fs.solve_static_device(tol=1e-10, precision=prec).cpu().numpy()
+            n0 = len(calls)
+            U2 = fs.solve_static_device(tol=1e-10, precision=prec).cpu().numpy()
+            assert len(calls) == n0, "an unchanged system rebuilt its diagonal"
+            for U in (U1, U2):
+                assert np.linalg.norm(U - Ud) / np.linalg.norm(Ud) < 1e-8
+        ptr = op.dinv.data_ptr()
+        d1 = op.dinv.clone()
+        # scales doubled through set_scale: half the diagonal's inverse, half the displacement
+        s0 = op.s_e.cpu().numpy()
+        op.set_scale(2.0 * s0)
+        U = fs.solve_static_device(tol=1e-10, precision="mixed").cpu().numpy()
+        assert op.dinv.data_ptr() == ptr
+        assert torch.equal(op.dinv, d1 / 2)
+        assert np.linalg.norm(U - 0.5 * Ud) / np.linalg.norm(0.5 * Ud) < 1e-8
+        # in place (the SIMP problem's way), then by direct assignment
+        op.update_scale_(4.0 * s0)
+        U = fs.solve_static_device(tol=1e-10, precision="fp64").cpu().numpy()
+        assert torch.equal(op.dinv, d1 / 4)
+        assert np.linalg.norm(U - 0.25 * Ud) / np.linalg.norm(0.25 * Ud) < 1e-8
+        op.s_e = torch.as_tensor(8.0 * s0, device=op.device)
+        op.s_e32 = op.s_e.float()
+        U = fs.solve_static_device(tol=1e-10, precision="fp64").cpu().numpy()
+        assert torch.equal(op.dinv, d1 / 8)
+        assert np.linalg.norm(U - 0.125 * Ud) / np.linalg.norm(0.125 * Ud) < 1e-8
+        # one more support set (the x = nx face's y components): a new diagonal
+        op.set_scale(s0)
+        tip = fs.mesh.nodes_on_plane(0, float(nx))
+        fs.fix_dofs(tip, [1])
+        n0 = len(calls)
+        U = fs.solve_static_device(tol=1e-10, precision="mixed").cpu().numpy()
+        assert len(calls) == n0 + 1
+        tipy = [3 * int(n) + 1 for n in tip]
+        assert float(op.dinv[tipy].abs().max()) == 0.0
+        fixed = sorted(set(c["fixed_dofs"]) | set(tipy))
+        Ut = o.solve_direct(c["K"], c["F"], fixed)
+        assert np.linalg.norm(U - Ut) / np.linalg.norm(Ut) < 1e-8
+    finally:
+        L.h8_diag_f64 = orig
     say("diagonal cache ok")
