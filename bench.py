@@ -152,13 +152,14 @@ def roofline_block(op, nx, ny, nz, t_iter_s, reps):
                      "achieved_GBs": k["GBps"], "frac": k["GBps"] * 1e9 / HBM_PEAK,
                      "share_of_iteration": k["calls"] * k["avg_us"] / it_us})
     dom = max(kern, key=lambda k: k["share_of_iteration"])
-    it_bytes, _, levels = kbench.mg_iteration_bytes(nx, ny, nz)
+    it_bytes, _, levels = kbench.mg_iteration_bytes(nx, ny, nz, t["mg_degree"])
     return {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": dom["frac"], "traffic": None, "kernel": dom["kernel"], "kernel_avg_us": dom["avg_us"],
             "bytes_per_launch": dom["bytes"], "share_of_iteration": dom["share_of_iteration"],
             "timing": f"HIP events around {reps} graph-launched launches of each kernel",
             "other_kernels": [k for k in kern if k is not dom],
             "vcycle_us": t["vcycle_us"], "levels": t["levels"], "level_dofs": [lv[0] for lv in levels],
+            "mg_degree": t["mg_degree"],
             "iteration_bytes": it_bytes, "iteration_us": it_us,
             "iteration_frac": it_bytes / t_iter_s / HBM_PEAK}
 
@@ -235,7 +236,7 @@ def target_block(dims, seed, fd_n, reps):
            "compliance": C_mixed, "forward_stored_MB": 8 * n_dof / 1e6,
            "roofline": kernels}
     from h8pcg import kbench
-    it_b, rep_b, _ = kbench.mg_iteration_bytes(nx, ny, nz)
+    it_b, rep_b, _ = kbench.mg_iteration_bytes(nx, ny, nz, kernels["mg_degree"])
     kernels["solve_bytes"] = st_mixed["inner_iterations"] * it_b + st_mixed["outer"] * rep_b
     kernels["solve_frac"] = kernels["solve_bytes"] / t_fwd / HBM_PEAK
     if fd_n > 0:
@@ -341,7 +342,8 @@ def main():
     roof = roofline_block(op, nx, ny, nz, t_solve / max(1, iters), 50)
     b64 = k64 * (13 * n_free * 8 + 8 * m)
     roof["fp64_iteration_contract_frac"] = b64 / dt64 / HBM_PEAK
-    roof["solve_bytes"] = iters * roof["iteration_bytes"] + s_["outer"] * kbench.mg_iteration_bytes(nx, ny, nz)[1]
+    roof["solve_bytes"] = (iters * roof["iteration_bytes"]
+                           + s_["outer"] * kbench.mg_iteration_bytes(nx, ny, nz, roof["mg_degree"])[1])
     roof["solve_frac"] = roof["solve_bytes"] / t_solve / HBM_PEAK
 
     cpu = torch_cpu = None

@@ -25,6 +25,7 @@ struct GridDev {
     int variant;           // H8_VARIANT_*
     int layers;            // node layers per block of the sector kernel (0 = choose)
     int arith;             // element-product arithmetic of FP32 sector/resident kernels (0 = 32, 32, 64)
+    int mg_degree;         // Chebyshev degree of the multigrid level-0 smoother (0 = library choice)
 };
 
 inline void set_tiles(GridDev& d, int TJ, int TI, int TK) {

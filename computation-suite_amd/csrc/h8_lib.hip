@@ -141,9 +141,13 @@ static int make_grid(const h8_grid* g, GridDev& d) {
     if (g->layers < 0) return fail(H8_ERR_ARG, "layers must be >= 0 (0 = library choice), got %d", g->layers);
     if (g->arith != 0 && g->arith != 32 && g->arith != 64)
         return fail(H8_ERR_ARG, "arith must be 0 (library choice), 32 or 64, got %d", g->arith);
+    if (g->mg_degree != 0 && (g->mg_degree < 2 || g->mg_degree > MG_CHEB_MAXDEG))
+        return fail(H8_ERR_ARG, "mg_degree must be 0 (library choice) or 2..%d, got %d", MG_CHEB_MAXDEG,
+                    g->mg_degree);
     d.variant = g->variant;
     d.layers = g->layers;
     d.arith = g->arith;
+    d.mg_degree = g->mg_degree;
     return H8_OK;
 }
 
@@ -1608,6 +1612,7 @@ static int refine_solve(const h8_grid* g, const double* s_e64, const double* K06
 // two levels and the coarsest has at most MG_DENSE_MAX DOFs.
 struct MgPlan {
     int nlev = 0;
+    int deg0 = 2;                 // Chebyshev steps per level-0 smoothing pass
     MgLevel L[MG_MAX_LEVELS];
     MgConst* cst = nullptr;       // [MG_MAX_LEVELS]
     double* M = nullptr;          // the 8 child matrices of level 1, [8][576]
@@ -1632,6 +1637,7 @@ static bool mg_plan(const GridDev& d, MgPlan& P) {
         L.rx = rx; L.ry = ry; L.rz = rz;
         nx /= rx; ny /= ry; nz /= rz;
     }
+    P.deg0 = d.mg_degree ? d.mg_degree : MG_DEG0_DEFAULT;
     // the MG kernels index stencil levels in 32 bits
     if (P.nlev >= 2 && 243LL * P.L[1].nnodes >= (1LL << 31)) return false;
     return P.nlev >= 2 && P.L[P.nlev - 1].ndof <= MG_DENSE_MAX;
@@ -1713,7 +1719,9 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
     if (mg_bottom_args(P).lds_words)
         HIPCHK(hipFuncSetAttribute((const void*)k_mg_bottom<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    MG_BOTTOM_LDS_BYTES));
-    hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(64), 0, s, (long long)(4 * MG_MAX_LEVELS),
+    static_assert(sizeof(MgConst) % sizeof(unsigned) == 0, "MgConst is cleared as words");
+    hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(64), 0, s,
+                       (long long)(sizeof(MgConst) / sizeof(unsigned) * MG_MAX_LEVELS),
                        reinterpret_cast<unsigned*>(P.cst), 0u);
     hipLaunchKernelGGL(k_mg_mc, dim3(18), dim3(256), 0, s, K064, L0.rx, L0.ry, L0.rz, P.M);
     const long long m = (long long)d.nx * d.ny * d.nz;
@@ -1844,13 +1852,24 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
         return launch_op<float, MODE_CHEB>(d32, A, s);
     };
     int rc;
+    // Level 0 smooths deg0 steps per pass.  Pre: the first two in one pass
+    // (into x1), steps 2.. alternating between x1 and x0; post: deg0 steps
+    // from the corrected iterate.  2 deg0 - 2 swaps in all, so the result
+    // lands in x1 whatever the degree.
+    const int deg0 = P.deg0;
+    float* cur0 = L0.x1;
+    auto other0 = [&](float* v) { return v == L0.x1 ? L0.x0 : L0.x1; };
     for (int l = 0; l < lb; ++l) {
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
         if (l == 0) {
-            // x0 = d = c0 dinv b and x1 = x0 + d' in one pass; y = M (b - K x1)
+            // x0 = d = c0 dinv b and x1 = x0 + d' in one pass; y = M (b - K x)
             if ((rc = first0(F.x1))) return rc;
-            if ((rc = resid0(F.x1))) return rc;
+            for (int k = 2; k < deg0; ++k) {
+                if ((rc = cheb0(cur0, other0(cur0), k, 0))) return rc;
+                cur0 = other0(cur0);
+            }
+            if ((rc = resid0(cur0))) return rc;
         } else {
             launch_cheb_st(F, F.x0, F.x1, cst + l, 1, st, s);
             launch_resid_st(F, F.x1, F.y, st, s);
@@ -1862,11 +1881,13 @@ static int mg_vcycle(const GridDev& d32, const MgPlan& P, const NodeArgs<float>&
     for (int l = lb - 1; l >= 0; --l) {
         const MgLevel& F = L[l];
         const MgLevel& C = L[l + 1];
-        hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(F.nnodes)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz, F.x1,
-                           (const float*)C.x1, (const float*)F.dinv, st);
+        hipLaunchKernelGGL(k_mg_prolong, dim3(mg_blocks(F.nnodes)), dim3(256), 0, s, F, C, F.rx, F.ry, F.rz,
+                           l == 0 ? cur0 : F.x1, (const float*)C.x1, (const float*)F.dinv, st);
         if (l == 0) {
-            if ((rc = cheb0(F.x1, F.x0, 0, 0))) return rc;
-            if ((rc = cheb0(F.x0, F.x1, 1, fuse_rz ? 1 : 0))) return rc;
+            for (int k = 0; k < deg0; ++k) {
+                if ((rc = cheb0(cur0, other0(cur0), k, (fuse_rz && k == deg0 - 1) ? 1 : 0))) return rc;
+                cur0 = other0(cur0);
+            }
         } else {
             launch_cheb_st(F, F.x1, F.x0, cst + l, 0, st, s);
             launch_cheb_st(F, F.x0, F.x1, cst + l, 1, st, s);
@@ -2017,6 +2038,7 @@ static int refine_solve_mg(const h8_grid* g, const double* s_e64, const double* 
         std::memcpy(key.ptr, ptrs, sizeof ptrs);
         key.extra[0] = (unsigned long long)d64.variant;   // the FP64 residual's kernel
         key.extra[1] = (unsigned long long)d64.layers;
+        key.extra[3] = (unsigned long long)P.deg0;         // the level-0 steps of every captured cycle
         NodeArgs<double> Rq = R;
         Rq.only_on_replace = 1;
         NodeArgs<double> R0 = R;
@@ -2155,8 +2177,8 @@ extern "C" int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const dou
 // avg_ms[0] = the level-0 product y = K x (sector kernel, FP32), avg_ms[1] =
 // one Chebyshev step on level 1 (stencil product fused with the update; level
 // 1 is a stencil level when the hierarchy has more than two levels, else 0),
-// avg_ms[2] = one whole V-cycle.  info[0..2] = levels, DOFs of level 1,
-// nodes of level 1.  Measurement helper with no counterpart in the reference;
+// avg_ms[2] = one whole V-cycle.  info[0..3] = levels, DOFs of level 1,
+// nodes of level 1, Chebyshev steps per level-0 smoothing pass.  Measurement helper with no counterpart in the reference;
 // scribbles on both workspaces.
 extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const double* K064, const float* s_e32,
                                   const float* K032, const double* dinv64, void* ws, size_t ws_bytes, void* mg_ws,
@@ -2267,6 +2289,7 @@ extern "C" int h8_mg_time_kernels(const h8_grid* g, const double* s_e64, const d
         info[0] = P.nlev;
         info[1] = has1 ? L1.ndof : 0;
         info[2] = has1 ? L1.nnodes : 0;
+        info[3] = P.deg0;
     }
     return H8_OK;
 }

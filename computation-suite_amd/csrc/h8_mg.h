@@ -49,6 +49,15 @@ constexpr long long MG_DENSE_MAX = 512;    // the coarsest level must be this sm
 #define H8_MG_CHEB_RATIO 30.0f
 #endif
 constexpr float MG_CHEB_RATIO = H8_MG_CHEB_RATIO;   // smoothing interval [lam / ratio, lam]
+// Degree of the level-0 smoother: every smoothing pass of the finest level is
+// this many Chebyshev steps (stencil levels always take 2).  Each step is one
+// operator application, so a higher degree buys a stronger cycle (fewer
+// iterations) for more level-0 launches per cycle.
+constexpr int MG_CHEB_MAXDEG = 6;
+#ifndef H8_MG_DEG0
+#define H8_MG_DEG0 4
+#endif
+constexpr int MG_DEG0_DEFAULT = H8_MG_DEG0;
 #ifndef H8_MG_BOTTOM_NODES
 #define H8_MG_BOTTOM_NODES 256
 #endif
@@ -76,7 +85,11 @@ struct MgLevel {
 // Per-level smoother constants, in device memory (written by the setup kernels).
 struct MgConst {
     unsigned lam_bits;     // Gershgorin bound (bits of a positive float; atomicMax)
+    // Chebyshev steps d = ca_k d + cb_k dinv (b - A x), x += d: step 0 has
+    // ca = 0, cb = c0; step k >= 1 reads (ca_k, cb_k) = (&c0)[2k - 1], (&c0)[2k]
+    // (c1a, c1b, then ck in pairs), so a kernel indexes one float array
     float c0, c1a, c1b;
+    float ck[2 * (MG_CHEB_MAXDEG - 2)];
 };
 
 __device__ __forceinline__ bool mg_live(const PcgState* st) { return !st || st->status == ST_RUNNING; }
@@ -438,10 +451,19 @@ __global__ void k_mg_consts(MgConst* cst, int nlev) {
     if (!(lam > 0.0f)) lam = 1.0f;     // a level with no free DOF: any value
     const float a = lam / MG_CHEB_RATIO;
     const float theta = 0.5f * (lam + a), delta = 0.5f * (lam - a);
-    const float sigma = theta / delta, rho0 = 1.0f / sigma, rho1 = 1.0f / (2.0f * sigma - rho0);
-    cst[l].c0 = 1.0f / theta;
-    cst[l].c1a = rho1 * rho0;
-    cst[l].c1b = 2.0f * rho1 / delta;
+    const float sigma = theta / delta;
+    // the three-term recurrence of the scaled Chebyshev polynomials (Saad,
+    // Iterative Methods for Sparse Linear Systems, Alg. 12.1): rho_0 = 1/sigma,
+    // rho_k = 1/(2 sigma - rho_{k-1}); step k: ca = rho_k rho_{k-1}, cb = 2 rho_k / delta
+    float* c = &cst[l].c0;
+    c[0] = 1.0f / theta;
+    float rho = 1.0f / sigma;
+    for (int k = 1; k < MG_CHEB_MAXDEG; ++k) {
+        const float rn = 1.0f / (2.0f * sigma - rho);
+        c[2 * k - 1] = rn * rho;
+        c[2 * k] = 2.0f * rn / delta;
+        rho = rn;
+    }
 }
 
 // Dense matrix of the coarsest level (FP64, row-major, n = L.ndof), identity

@@ -66,11 +66,11 @@ struct NodeArgs {
                           // 2 = unless the solve is running (the first residual of a captured solve)
     const float* x32;     // RESID (sector kernel, multigrid solve): the residual is formed at
                           // u + x32 (the FP32 iterate not yet folded into u); nullptr: at u
-    const float* cheb;    // CHEB: the level's coefficients {c0, c1a, c1b} (device)
+    const float* cheb;    // CHEB: the level's coefficients {c0, c1a, c1b, c2a, c2b, ...} (device)
     const T* cb_b;        // CHEB: right-hand side
     T* cb_d;              // CHEB: Chebyshev direction (updated in place)
     T* cb_out;            // CHEB: x_out (not u: other blocks still read u)
-    int cheb_step;        // CHEB: 0 = first step of a pass (ca = 0, cb = c0), 1 = second
+    int cheb_step;        // CHEB: step k of a pass (0: ca = 0, cb = c0; k: cheb[2k-1], cheb[2k])
     int cheb_rz;          // CHEB: reduce b.x_out and finish the iteration's beta
 };
 
@@ -359,7 +359,7 @@ __device__ __forceinline__ void node_output(const NodeArgs<T>& A, int gn, const 
             A.cb_out[q] = ctr[c] + dn;
         }
     } else if constexpr (MODE == MODE_CHEB) {
-        const T ca = A.cheb_step ? T(A.cheb[1]) : T(0), cb = A.cheb_step ? T(A.cheb[2]) : T(A.cheb[0]);
+        const T ca = A.cheb_step ? T(A.cheb[2 * A.cheb_step - 1]) : T(0), cb = T(A.cheb[2 * A.cheb_step]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int q = gn + c;
@@ -445,7 +445,7 @@ __device__ __forceinline__ void node_output_ex(const NodeArgs<T>& A, int gn, con
             A.cb_out[q] = ctr[c] + dn;
         }
     } else if constexpr (MODE == MODE_CHEB) {
-        const T ca = A.cheb_step ? T(A.cheb[1]) : T(0), cb = A.cheb_step ? T(A.cheb[2]) : T(A.cheb[0]);
+        const T ca = A.cheb_step ? T(A.cheb[2 * A.cheb_step - 1]) : T(0), cb = T(A.cheb[2 * A.cheb_step]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int q = gn + c;

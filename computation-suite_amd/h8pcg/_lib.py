@@ -35,7 +35,7 @@ K0_WORDS_F32 = 792   # FP32 K0: cast of the FP64 K0, then the FP64 sector table 
 
 class H8Grid(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("variant", C.c_int32),
-                ("layers", C.c_int32), ("arith", C.c_int32)]
+                ("layers", C.c_int32), ("arith", C.c_int32), ("mg_degree", C.c_int32)]
 
 
 class H8PcgStats(C.Structure):

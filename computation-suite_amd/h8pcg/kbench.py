@@ -178,9 +178,11 @@ def _st(n, N, reads, writes):
     return W * (243 * N + (reads + writes) * n)
 
 
-def mg_cycle_kernels(nx, ny, nz):
+def mg_cycle_kernels(nx, ny, nz, deg0):
     """The kernels of one V-cycle with their algorithmic bytes per launch and
-    launches per cycle, keyed by the slots of h8_mg_time_kernels:
+    launches per cycle, keyed by the slots of h8_mg_time_kernels (deg0 =
+    Chebyshev steps per level-0 smoothing pass: the first pair in slot 4, then
+    deg0 - 2 steps before the residual and deg0 after the prolongation):
       3 level-0 Chebyshev step (u, s_e, b, d, dinv in; d, x out),
       4 level-0 first step pair from zero (b, dinv, s_e in; d, x out),
       5 level-0 masked residual (u, s_e, b, dinv in; r out),
@@ -195,7 +197,7 @@ def mg_cycle_kernels(nx, ny, nz):
     lb = mg_bottom_level(levels)
     out = {
         3: {"kernel": "k_sector<float,16,16,MODE_CHEB> (level-0 Chebyshev step)", "bytes": W * (6 * n0 + m),
-            "calls": 2},
+            "calls": 2 * int(deg0) - 2},
         4: {"kernel": "k_sector<float,16,16,MODE_CHEB0> (level-0 first step pair)", "bytes": W * (4 * n0 + m),
             "calls": 1},
         5: {"kernel": "k_sector<float,16,16,MODE_MGRES> (level-0 masked residual)", "bytes": W * (4 * n0 + m),
@@ -226,13 +228,13 @@ def _bottom_bytes(levels, lb):
     return b
 
 
-def mg_iteration_bytes(nx, ny, nz):
+def mg_iteration_bytes(nx, ny, nz, deg0):
     """Algorithmic HBM bytes of one FP32 iteration of the multigrid-preconditioned
     refined solve, the kernels as the library runs them:
       PCG: direction + product (z, p_old, s_e in; p, q out), update (x, r, p,
            q, dinv in; x, r out);
-      level 0: first step pair, masked residual, restriction, prolongation,
-           two Chebyshev steps (mg_cycle_kernels);
+      level 0: first step pair, deg0 - 2 Chebyshev steps, masked residual,
+           restriction, prolongation, deg0 Chebyshev steps (mg_cycle_kernels);
       stencil level l (1 <= l < the bottom level): first step in the
            restriction into it, one Chebyshev step and the masked residual,
            restriction, prolongation, two Chebyshev steps;
@@ -249,7 +251,7 @@ def mg_iteration_bytes(nx, ny, nz):
     if len(levels) == 1:
         cyc = W * (n0 * n0 + 2 * n0)
     else:
-        cyc = sum(k["bytes"] * k["calls"] for key, k in mg_cycle_kernels(nx, ny, nz).items() if key not in (1, 9))
+        cyc = sum(k["bytes"] * k["calls"] for key, k in mg_cycle_kernels(nx, ny, nz, deg0).items() if key not in (1, 9))
         for li in range(1, lb):
             n, N = levels[li]
             nc = levels[li + 1][0]
@@ -267,19 +269,19 @@ def time_mg(op, reps):
     from . import _lib
     ws, mws = op.workspace(32), op.mg_workspace()
     avg = (C.c_double * _lib.MG_TIMES)()
-    info = (C.c_int64 * 3)()
+    info = (C.c_int64 * 4)()
     _lib.check(_lib.lib().h8_mg_time_kernels(C.byref(op.grid32), _lib.ptr(op.s_e), _lib.ptr(op.K0buf),
                                              _lib.ptr(op.s_e32), _lib.ptr(op.K0_32), _lib.ptr(op.dinv),
                                              _lib.ptr(ws), ws.numel(), _lib.ptr(mws), mws.numel(), int(reps), avg,
                                              info, _lib.stream_handle(op.device)), "h8_mg_time_kernels")
-    kern = mg_cycle_kernels(op.nx, op.ny, op.nz)
+    kern = mg_cycle_kernels(op.nx, op.ny, op.nz, int(info[3]))
     kern[0] = {"kernel": "k_sector<float,16,16,MODE_PLAIN> (level-0 product y = K x)",
                "bytes": W * (2 * op.n_dof + op.m), "calls": 0}
     for slot, k in kern.items():
         k["avg_us"] = 1e3 * avg[slot]
         k["GBps"] = k["bytes"] / k["avg_us"] * 1e-3 if k["avg_us"] > 0 else None
     return {"levels": int(info[0]), "level1_dof": int(info[1]), "level1_nodes": int(info[2]),
-            "vcycle_us": 1e3 * avg[2], "kernels": kern}
+            "mg_degree": int(info[3]), "vcycle_us": 1e3 * avg[2], "kernels": kern}
 
 
 def main(argv=None):

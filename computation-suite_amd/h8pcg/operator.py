@@ -376,6 +376,14 @@ class GridOperator:
             raise ValueError(f"arith must be 0, 32 or 64, got {arith}")
         self.grid32.arith = int(arith)
 
+    def set_mg_degree(self, degree):
+        """Chebyshev steps per smoothing pass on the finest level of the
+        multigrid preconditioner: 0 (library choice) or 2..6.  Coarser levels
+        always take 2 steps."""
+        if int(degree) != 0 and not 2 <= int(degree) <= 6:
+            raise ValueError(f"mg_degree must be 0 (library choice) or 2..6, got {degree}")
+        self.grid32.mg_degree = int(degree)
+
     def autotune(self, precisions=(32, 64), cache=True):
         """Times the candidate kernels for this grid and keeps the fastest per
         precision (see h8pcg.autotune).  Returns {precision: (variant, layers, us)}."""

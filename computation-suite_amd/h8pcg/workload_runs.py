@@ -68,6 +68,9 @@ def simp(a):
     t0 = time.time()
     pb = topopt.SIMPProblem(nx, ny, nz, F, fixed, p=3.0, rho_min=1e-9, device="cuda:0", tol=a.tol,
                             precision=a.precision)
+    pb.op.set_mg_degree(a.mg_degree)
+    if a.delta is not None:
+        pb.op.DELTA = dict(pb.op.DELTA, mg=a.delta, jacobi=a.delta)
     if a.autotune:
         pb.op.set_free_mask(pb.free_mask)
         pb.op.autotune()
@@ -87,6 +90,7 @@ def simp(a):
     g = rho.grad.detach().cpu().numpy()
     out = {"workload": "simp", "dims": [nx, ny, nz], "n_dof": n_dof, "n_free": n_dof - len(fixed), "elements": m,
            "rho": [lo, hi], "seed": a.seed, "precision": a.precision, "tol": a.tol, "setup_s": setup_s,
+           "mg_degree": a.mg_degree, "delta": a.delta,
            "reps": max(1, a.reps),
            "compliance": float(C), "forward_ms": 1e3 * (t2 - t1), "gradient_ms": 1e3 * (t3 - t2),
            "forward_stats": {k_: fwd.get(k_) for k_ in ("iterations", "inner_iterations", "outer",
@@ -114,16 +118,18 @@ def refined(a):
     nx, ny, nz = (int(v) for v in a.dims.split(","))
     E = 10 ** np.random.default_rng(a.seed).uniform(9, 12, nx * ny * nz)
     fs = cantilever(nx, ny, nz, E_elem=E, device="cuda:0")
+    fs.operator.set_mg_degree(a.mg_degree)
     F = torch.as_tensor(np.asarray(fs.F, dtype=float), device="cuda:0")
     times = []
     for _ in range(max(1, a.reps)):
         _sync()
         t = time.perf_counter()
-        fs.solve_static_device(tol=a.tol, precision="mixed", F=F)
+        fs.solve_static_device(tol=a.tol, precision="mixed", F=F, delta=a.delta)
         _sync()
         times.append(1e3 * (time.perf_counter() - t))
     st = fs.last_solve_stats
-    out = {"workload": "refined", "dims": [nx, ny, nz], "seed": a.seed, "tol": a.tol,
+    out = {"workload": "refined", "dims": [nx, ny, nz], "seed": a.seed, "tol": a.tol, "mg_degree": a.mg_degree,
+           "delta": a.delta,
            "n_free": fs.n_dof - len(fs.fixed_dofs), "solve_ms": times,
            "stats": {k: st.get(k) for k in ("inner_iterations", "outer", "true_rel_residual", "elapsed_ms",
                                             "launches", "preconditioner")}}
@@ -241,11 +247,15 @@ def main(argv=None):
     s.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
     s.add_argument("--autotune", action="store_true")
     s.add_argument("--reps", type=int, default=1)
+    s.add_argument("--mg-degree", type=int, default=0, help="level-0 Chebyshev steps of the V-cycle (0 = library)")
+    s.add_argument("--delta", type=float, default=None, help="residual-replacement threshold of the mixed solve")
     rf = sub.add_parser("refined")
     rf.add_argument("--dims", default="128,32,32")
     rf.add_argument("--seed", type=int, default=2)
     rf.add_argument("--tol", type=float, default=1e-8)
     rf.add_argument("--reps", type=int, default=12)
+    rf.add_argument("--mg-degree", type=int, default=0, help="level-0 Chebyshev steps of the V-cycle (0 = library)")
+    rf.add_argument("--delta", type=float, default=None, help="residual-replacement threshold (None: the library's)")
     b = sub.add_parser("batched")
     b.add_argument("--seed", type=int, default=4)
     b.add_argument("--n", type=int, default=256)

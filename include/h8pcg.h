@@ -70,6 +70,9 @@ typedef struct h8_grid {
                                0 = library choice (32), 32 = FP32 products, 64 = FP64 products
                                of the FP32 data (outputs rounded to FP32).  The node-centric
                                and MFMA kernels compute in their storage precision. */
+    int32_t mg_degree;      /* Chebyshev steps per smoothing pass on the finest level of the
+                               multigrid preconditioner (h8_refine_solve_mg, h8_mg_vcycle_f32):
+                               0 = library choice, else 2..6; coarser levels take 2 */
 } h8_grid;
 
 typedef struct h8_pcg_stats {
@@ -243,8 +246,9 @@ int h8_mg_vcycle_f32(const h8_grid* g, const double* s_e64, const double* K064, 
  *   7 the prolongation 1 -> 0,
  *   8 the single-workgroup kernel of the small levels,
  *   9 the level-1 masked residual (stencil product);
- * a slot that does not apply to the grid's hierarchy is 0.  info[0..2] (may
- * be NULL) = levels, DOFs and nodes of level 1.  ws:
+ * a slot that does not apply to the grid's hierarchy is 0.  info[0..3] (may
+ * be NULL) = levels, DOFs and nodes of level 1, and the Chebyshev steps per
+ * level-0 smoothing pass (2 deg - 2 level-0 steps of slot 3 per cycle).  ws:
  * h8_pcg_workspace_bytes(g, 32); mg_ws: h8_mg_workspace_bytes(g).
  * Measurement helper with no counterpart in the reference; scribbles on both
  * workspaces. */

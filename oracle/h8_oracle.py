@@ -64,11 +64,13 @@ PINNING (function -> independent checks):
   V-cycle) :              the level plan follows the halving rule; the
                           prolongation reproduces every linear field exactly;
                           on a free body every Galerkin operator annihilates the
-                          six rigid-body modes of its grid and is symmetric; the
-                          two Chebyshev steps equal the scaled Chebyshev
-                          polynomial T2((theta - lam)/delta) / T2(theta/delta);
-                          the V-cycle is symmetric positive definite and PCG
-                          with it reaches spsolve's solution (tests/test_oracle_mg.py)
+                          six rigid-body modes of its grid and is symmetric; k
+                          Chebyshev steps (k = 2..6, mg_cheb_coefficients) equal
+                          the scaled Chebyshev polynomial
+                          Tk((theta - lam)/delta) / Tk(theta/delta) (numpy's
+                          Chebyshev basis); the V-cycle is symmetric positive
+                          definite (level-0 degree 2 and 4) and PCG with it
+                          reaches spsolve's solution (tests/test_oracle_mg.py)
   hex20_shape_and_derivs / hex20_B / hex20_ke / hex20_box_by_midpoints /
   assemble_csr_npe :      partition of unity, Kronecker property at the 20
                           nodes, the isoparametric identity, derivatives = central
@@ -812,7 +814,7 @@ def mg_levels(nx, ny, nz):
     return out
 
 
-def mg_hierarchy(K, free, dims, K_abs=None, store32=False):
+def mg_hierarchy(K, free, dims, K_abs=None, store32=False, degree0=2):
     """Levels of the V-cycle for the full-length K (n x n) with free mask
     `free` (bool, n) on the box grid dims = (nx, ny, nz) (node order
     k (nx+1)(ny+1) + i (ny+1) + j, DOF 3 node + c).  A_0 = M K M; A_{l+1} =
@@ -821,7 +823,8 @@ def mg_hierarchy(K, free, dims, K_abs=None, store32=False):
     times 1.0001; level 0 from K_abs = the assembly of s_e |K0| when given).
     store32: every coarse operator rounded to FP32 as soon as it is formed
     (the library stores them so; the next product and the dense inverse of
-    the coarsest level then start from the rounded values)."""
+    the coarsest level then start from the rounded values).  degree0:
+    Chebyshev steps per smoothing pass on level 0 (coarser levels: 2)."""
     plan = mg_levels(*dims)
     if plan is None:
         raise ValueError("grid does not coarsen far enough")
@@ -854,20 +857,31 @@ def mg_hierarchy(K, free, dims, K_abs=None, store32=False):
     Ad[:, masked] = 0.0
     Ad[masked, masked] = 1.0
     c["Ainv"] = np.linalg.inv(Ad)
-    for lev in levels:
-        lam = lev["lam"]
-        a = lam / MG_CHEB_RATIO
-        theta, delta = 0.5 * (lam + a), 0.5 * (lam - a)
-        sigma = theta / delta
-        rho0 = 1.0 / sigma
-        rho1 = 1.0 / (2.0 * sigma - rho0)
-        lev["cheb"] = ((0.0, 1.0 / theta), (rho1 * rho0, 2.0 * rho1 / delta))
+    for li, lev in enumerate(levels):
+        lev["cheb"] = mg_cheb_coefficients(lev["lam"], degree0 if li == 0 else 2)
     return levels
 
 
+def mg_cheb_coefficients(lam, degree):
+    """(ca_k, cb_k) of the Chebyshev steps d = ca d + cb dinv (b - A x), x += d,
+    k = 0 .. degree-1, on the interval [lam / MG_CHEB_RATIO, lam] (the
+    three-term recurrence of Saad, Iterative Methods for Sparse Linear Systems,
+    2nd ed., Alg. 12.1: rho_0 = 1/sigma, rho_k = 1/(2 sigma - rho_{k-1}))."""
+    a = lam / MG_CHEB_RATIO
+    theta, delta = 0.5 * (lam + a), 0.5 * (lam - a)
+    sigma = theta / delta
+    out = [(0.0, 1.0 / theta)]
+    rho = 1.0 / sigma
+    for _ in range(1, degree):
+        rn = 1.0 / (2.0 * sigma - rho)
+        out.append((rn * rho, 2.0 * rn / delta))
+        rho = rn
+    return tuple(out)
+
+
 def _mg_smooth(lev, x, b, d, zero_x):
-    """Two Chebyshev steps d = ca d + cb dinv (b - A x), x += d (Saad, Iterative
-    Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1)."""
+    """The level's Chebyshev steps d = ca d + cb dinv (b - A x), x += d (Saad,
+    Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1)."""
     for s, (ca, cb) in enumerate(lev["cheb"]):
         r = b if (zero_x and s == 0) else b - lev["A"] @ x
         d = (0.0 if (zero_x and s == 0) else ca * d) + cb * lev["dinv"] * r
@@ -876,8 +890,9 @@ def _mg_smooth(lev, x, b, d, zero_x):
 
 
 def mg_vcycle(levels, r, level=0):
-    """z = V(r): Chebyshev(2) pre-smoothing from x = 0, coarse correction with
-    the restricted residual, Chebyshev(2) post-smoothing (same polynomial)."""
+    """z = V(r): Chebyshev pre-smoothing from x = 0, coarse correction with
+    the restricted residual, Chebyshev post-smoothing (the same polynomial,
+    so the cycle is symmetric)."""
     lev = levels[level]
     if level == len(levels) - 1:
         return lev["Ainv"] @ r

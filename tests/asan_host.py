@@ -65,7 +65,7 @@ L = C.CDLL(sys.argv[1])
 
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("variant", C.c_int32),
-                ("layers", C.c_int32), ("arith", C.c_int32)]
+                ("layers", C.c_int32), ("arith", C.c_int32), ("mg_degree", C.c_int32)]
 
 class Inst(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nrhs", C.c_int32),
@@ -105,6 +105,8 @@ for dims in [(1, 1, 1), (2, 2, 2), (5, 3, 2), (20, 4, 4), (33, 17, 9), (128, 32,
     print("mg workspace", dims, mg, flush=True)
     ok(L.h20_pcg_workspace_bytes(*dims) > 0, f"hex20 workspace {dims}")
 ok(L.h8_mg_workspace_bytes(C.byref(Grid(7, 3, 3, 0, 0, 0))) == 0, "mg on a grid that does not coarsen")
+for deg, want in ((0, True), (2, True), (6, True), (1, False), (7, False), (-3, False)):
+    ok((L.h8_mg_workspace_bytes(C.byref(Grid(16, 4, 4, 0, 0, 0, deg))) > 0) == want, f"mg_degree {deg}")
 # invalid grids: zero size, negative layers, unknown variant/arith, too many DOFs, NULL
 for bad, frag in [((0, 4, 4, 0, 0, 0), "nx, ny, nz"), ((4, 4, 4, 99, 0, 0), "variant"),
                   ((4, 4, 4, 0, -1, 0), "layers"), ((4, 4, 4, 0, 0, 16), "arith"),

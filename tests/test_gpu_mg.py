@@ -22,19 +22,22 @@ def _cantilever(h8, nx, ny, nz, E):
     return fs
 
 
-@pytest.mark.parametrize("dims", [(32, 8, 8), (40, 3, 4)])
-def test_vcycle_matches_oracle(h8, gpu, dims):
+@pytest.mark.parametrize("dims,degree", [((32, 8, 8), 2), ((32, 8, 8), 3), ((32, 8, 8), 4), ((32, 8, 8), 6),
+                                          ((40, 3, 4), 2), ((40, 3, 4), 5)])
+def test_vcycle_matches_oracle(h8, gpu, dims, degree):
     """z = V(r) on the device equals the oracle's V-cycle (FP64 arithmetic,
     coarse operators P^T A P from scipy, stored in FP32 as the library stores
-    them) to FP32 accuracy; (40, 3, 4) keeps its odd y axis on the coarse
-    level.  (Without the FP32 storage the coarsest inverse alone differs by
-    up to its condition number times the FP32 rounding.)"""
+    them) to FP32 accuracy, for level-0 smoothing degrees 2..6 (odd degrees
+    take the other buffer order); (40, 3, 4) keeps its odd y axis on the
+    coarse level.  (Without the FP32 storage the coarsest inverse alone
+    differs by up to its condition number times the FP32 rounding.)"""
     import torch
     nx, ny, nz = dims
     rng = np.random.default_rng(5)
     E = 10 ** rng.uniform(9, 12, nx * ny * nz)
     fs = _cantilever(h8, nx, ny, nz, E)
     op = fs.operator
+    op.set_mg_degree(degree)
     free = fs._free_mask().astype(bool)
     op.set_free_mask(free.astype(np.uint8))
     r = rng.standard_normal(op.n_dof) * free
@@ -43,10 +46,10 @@ def test_vcycle_matches_oracle(h8, gpu, dims):
     nodes, elems = o.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
     ke = o.hex8_ke(nodes[elems[0]], o.d_solid3d(1.0, 0.3))
     Kabs = o.assemble_csr(nodes, elems, None, scale=E, ke=np.broadcast_to(np.abs(ke), (len(elems), 24, 24)))
-    H = o.mg_hierarchy(c["K"].tocsr(), free, dims, K_abs=Kabs, store32=True)
+    H = o.mg_hierarchy(c["K"].tocsr(), free, dims, K_abs=Kabs, store32=True, degree0=degree)
     zr = o.mg_vcycle(H, r.astype(np.float32).astype(float))
     err = np.linalg.norm(z - zr) / np.linalg.norm(zr)
-    say("vcycle", dims, "levels", [lv["A"].shape[0] for lv in H], "rel err", err)
+    say("vcycle", dims, "degree", degree, "levels", [lv["A"].shape[0] for lv in H], "rel err", err)
     assert np.all(z[~free] == 0.0)
     assert err < 1e-5
 
@@ -82,6 +85,30 @@ def test_mg_refined_solve_matches_direct(h8, gpu, dims, span):
     assert err < 1e-8
     assert st["inner_iterations"] < 120
     assert st["inner_iterations"] < stj["inner_iterations"] / 3
+
+
+def test_mg_degree_solves_agree(h8, gpu):
+    """The refined solve at level-0 smoothing degrees 2..6 and the library's
+    choice (0) reaches the direct solution to 1e-8; a stronger smoother takes
+    fewer FP32 iterations."""
+    nx, ny, nz = 32, 8, 8
+    E = 210e9 * 10 ** np.random.default_rng(6).uniform(-3, 0, nx * ny * nz)
+    fs = _cantilever(h8, nx, ny, nz, E)
+    c = o.cantilever(nx, ny, nz, E_elem=E)
+    Ud = o.solve_direct(c["K"], c["F"], c["fixed_dofs"])
+    its = {}
+    for deg in (2, 3, 4, 6, 0):
+        fs.operator.set_mg_degree(deg)
+        U = np.asarray(fs.solve_static(tol=1e-9, precision="mixed"))
+        st = fs.last_solve_stats
+        err = np.linalg.norm(U - Ud) / np.linalg.norm(Ud)
+        say("degree", deg, "fp32 iters", st["inner_iterations"], "replacements", st["outer"], "err", err)
+        assert st["preconditioner"] == "mg"
+        assert err < 1e-8
+        its[deg] = st["inner_iterations"]
+    assert its[6] < its[2] and its[4] < its[2]
+    with pytest.raises(ValueError):
+        fs.operator.set_mg_degree(7)
 
 
 def test_mg_deterministic_and_warm_start(h8, gpu):

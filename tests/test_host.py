@@ -46,6 +46,13 @@ def test_library_non_device_calls():
     assert rc == _lib.H8_ERR_ARG and "nx, ny, nz" in _lib.last_error()
     with pytest.raises(ValueError):
         _lib.check(rc)
+    # the multigrid smoothing degree: 0 (library choice) or 2..6
+    for deg, ok in ((0, True), (2, True), (6, True), (1, False), (7, False), (-1, False)):
+        gd = _lib.H8Grid(128, 32, 32, 0, 0, 0, deg)
+        assert (L.h8_mg_workspace_bytes(gd) > 0) == ok, deg
+        if not ok:
+            rc = L.h8_spmv_f64(gd, None, None, None, None, None)
+            assert rc == _lib.H8_ERR_ARG and "mg_degree" in _lib.last_error()
 
 
 def test_box_mesh_matches_oracle_and_structure():

@@ -13,6 +13,7 @@ by mathematics, not by the oracle's own formulas:
   - the level plan halves the even axes until at most 128 DOFs.
 """
 import numpy as np
+import pytest
 import scipy.sparse.linalg as spla
 
 from oracle import h8_oracle as o
@@ -82,24 +83,30 @@ def test_galerkin_operators_keep_rigid_modes_and_symmetry():
     assert len(H) >= 3 and dims == H[0]["dims"]
 
 
-def test_chebyshev_steps_are_the_scaled_chebyshev_polynomial():
+@pytest.mark.parametrize("degree", [2, 3, 4, 5, 6])
+def test_chebyshev_steps_are_the_scaled_chebyshev_polynomial(degree):
     nx, ny, nz = 8, 4, 4
     _, _, K = _box(nx, ny, nz)
     free = np.ones(K.shape[0], bool)
-    lev = o.mg_hierarchy(K, free, (nx, ny, nz))[0]
+    lev = o.mg_hierarchy(K, free, (nx, ny, nz), degree0=degree)[0]
+    assert len(lev["cheb"]) == degree
     lam_max = lev["lam"]
     a = lam_max / o.MG_CHEB_RATIO
     theta, delta = 0.5 * (lam_max + a), 0.5 * (lam_max - a)
     lam = np.linspace(0.0, lam_max, 41)
-    # a diagonal "matrix" diag(lam) with dinv = 1: two steps from zero on b = 1
+    # a diagonal "matrix" diag(lam) with dinv = 1: `degree` steps from zero on b = 1
     import scipy.sparse as sp
     scal = {"A": sp.diags(lam).tocsr(), "dinv": np.ones_like(lam), "cheb": lev["cheb"]}
     x, _ = o._mg_smooth(scal, np.zeros_like(lam), np.ones_like(lam), np.zeros_like(lam), True)
-
-    def T2(t):
-        return 2 * t * t - 1
-
-    assert np.allclose(1.0 - lam * x, T2((theta - lam) / delta) / T2(theta / delta), rtol=0, atol=1e-13)
+    # the error factor 1 - lam x(lam) is Tk((theta - lam)/delta) / Tk(theta/delta)
+    Tk = np.polynomial.chebyshev.Chebyshev.basis(degree)
+    assert np.allclose(1.0 - lam * x, Tk((theta - lam) / delta) / Tk(theta / delta), rtol=0, atol=1e-13)
+    # and its largest magnitude on [a, lam_max] is 1 / Tk(theta/delta) (equioscillation)
+    on = lam >= a
+    assert abs(np.abs(1.0 - lam[on] * x[on]).max() - 1.0 / Tk(theta / delta)) < 1e-2 / Tk(theta / delta)
+    # a coarse level keeps two steps whatever the level-0 degree
+    coarse = o.mg_hierarchy(K, free, (nx, ny, nz), degree0=degree)[1]
+    assert len(coarse["cheb"]) == 2
 
 
 def test_vcycle_is_spd_and_preconditions_pcg():
@@ -113,10 +120,11 @@ def test_vcycle_is_spd_and_preconditions_pcg():
     H = o.mg_hierarchy(K, fm, (nx, ny, nz))
     rng = np.random.default_rng(3)
     r1, r2 = rng.standard_normal((2, K.shape[0])) * fm
-    z1, z2 = o.mg_vcycle(H, r1), o.mg_vcycle(H, r2)
-    assert abs(r2 @ z1 - r1 @ z2) <= 1e-11 * np.sqrt((r1 @ z1) * (r2 @ z2))
-    assert r1 @ z1 > 0 and r2 @ z2 > 0
-    assert np.all(z1[~fm] == 0.0)
+    for HH in (H, o.mg_hierarchy(K, fm, (nx, ny, nz), degree0=4)):
+        z1, z2 = o.mg_vcycle(HH, r1), o.mg_vcycle(HH, r2)
+        assert abs(r2 @ z1 - r1 @ z2) <= 1e-11 * np.sqrt((r1 @ z1) * (r2 @ z2))
+        assert r1 @ z1 > 0 and r2 @ z2 > 0
+        assert np.all(z1[~fm] == 0.0)
     # PCG on the reduced system with M^-1 = V restricted to the free DOFs
     Kff, Ff = o.reduced_system(K, F, free, fixed)
     n = K.shape[0]
