@@ -174,8 +174,9 @@ def topology_optimize_compliance(fesystem, D, F_ext, volume_fraction, p=3.0, rho
     through the general adjoint (a second solve) instead of the closed form.
     Returns {"rho", "compliance_history", "u_final"} as the reference does.
     timings: a dict that receives the device-synchronised seconds spent in each
-    stage ("solve", "sensitivity", "filter", "oc"), summed over the iterations
-    (None: no synchronisation between the stages)."""
+    stage ("solve", "sensitivity", "filter", "oc"), summed over the iterations,
+    and each iteration's seconds under timings["per_iteration"][stage] (None:
+    no synchronisation between the stages)."""
     import time
     import torch
     clock = [time.perf_counter()]
@@ -185,6 +186,7 @@ def topology_optimize_compliance(fesystem, D, F_ext, volume_fraction, p=3.0, rho
             torch.cuda.synchronize()
             t = time.perf_counter()
             timings[stage] = timings.get(stage, 0.0) + t - clock[0]
+            timings.setdefault("per_iteration", {}).setdefault(stage, []).append(t - clock[0])
             clock[0] = t
     from .mesh import structured_grid
     if sensitivity not in ("closed_form", "adjoint"):

@@ -166,7 +166,9 @@ def simploop(a):
     print(json.dumps({"workload": "simploop", "dims": [nx, ny, nz], "elements": nx * ny * nz, "iterations": it,
                       "vf": a.vf, "radius": a.radius, "solve_tol": a.tol, "precision": a.precision,
                       "wall_ms": 1e3 * wall, "stage_ms_per_iteration": {k: 1e3 * v / it for k, v in tm.items()
-                                                                        if k != "setup"},
+                                                                        if k not in ("setup", "per_iteration")},
+                      "stage_ms_each_iteration": {k: [round(1e3 * x, 3) for x in v]
+                                                  for k, v in tm.get("per_iteration", {}).items() if k != "setup"},
                       "compliance_first_last": [res["compliance_history"][0], res["compliance_history"][-1]],
                       "volume": float(res["rho"].mean())}), flush=True)
 
