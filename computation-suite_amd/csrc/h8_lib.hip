@@ -1729,11 +1729,12 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
     hipLaunchKernelGGL(k_mg_gersh0, dim3(mg_blocks(d.nnodes)), dim3(256), 0, s, d, s_e32, K064,
                        (const float*)L0.dinv, P.cst);
     {
+        // a thread per (coarse node, neighbour block) pair of level 1
         const MgLevel& L1 = P.L[1];
-        const long long runs = (L1.NJ + MG_GT - 1) / MG_GT;
-        const long long nblk = runs * L1.NI * L1.NK;
-        hipLaunchKernelGGL(k_mg_level1, dim3(unsigned(std::min<long long>(nblk, 16384))), dim3(256), 0, s, d, L1,
-                           L0.rx, L0.ry, L0.rz, s_e64, K064, (const double*)P.M, (const unsigned*)P.emask);
+        const long long pairs = 27LL * L1.nnodes;
+        const long long nblk = std::min<long long>((pairs + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_mg_level1, dim3(unsigned(nblk)), dim3(256), 0, s, d, L1, L0.rx, L0.ry, L0.rz, s_e64,
+                           K064, (const double*)P.M, (const unsigned*)P.emask);
     }
     hipLaunchKernelGGL(k_mg_diag, dim3(mg_blocks(P.L[1].ndof)), dim3(256), 0, s, P.L[1], P.cst + 1);
     for (int l = 1; l + 1 < nl; ++l) {

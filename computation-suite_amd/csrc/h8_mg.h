@@ -110,6 +110,11 @@ __device__ __forceinline__ void mg_decode(const MgLevel& L, long long n, int& j,
 __host__ __device__ constexpr double mg_w(int f, int C, int r) {
     return r == 1 ? (f == C ? 1.0 : 0.0) : (f == 2 * C ? 1.0 : (f == 1 ? 0.5 : 0.0));
 }
+// The same weight as a power of two: mg_w = 2^-mg_wexp, or 0 when mg_wexp is 16
+// (a sum of three is then >= 16).
+__host__ __device__ constexpr unsigned mg_wexp(int f, int C, int r) {
+    return r == 1 ? (f == C ? 0u : 16u) : (f == 2 * C ? 0u : (f == 1 ? 1u : 16u));
+}
 
 // M_c = P_c^T K0 P_c for the (up to 8) children c of a coarse element with
 // ratios (rx, ry, rz); child c = cx + 2 cy + 4 cz, entries [c][24][24].
@@ -216,110 +221,129 @@ __global__ void __launch_bounds__(256) k_mg_elem_mask(GridDev g, const float* __
 // elements E holding both I (corner A) and J = I + nb (corner B) and over E's
 // fine children e (child position ch), s_e (P_ch^T K0 P_ch)[3A + c][3B + d]
 // -- the fixed child blocks M (k_mg_mc) -- when every DOF of e is free; a
-// child with fixed DOFs is evaluated with its free mask (K0 read from global
-// memory: only boundary elements take that path).  A block takes a run of up
-// to MG_GT coarse nodes along j at one (i, k) and stages M and the scales and
-// masks of the fine elements around the run in LDS; a thread owns one (node,
-// neighbour) pair and accumulates its nine entries in FP64 (coarse elements,
-// then children, in a fixed order), so the element loops and the scale and
-// mask loads are shared by nine products.  Adjacent threads hold adjacent
-// nodes of one neighbour block, so the stores of an entry are contiguous.
-constexpr int MG_GT = 32;
-constexpr int MG_L1_ELEMS = (MG_GT + 1) * 2 * 4 * 4;   // fine elements around a run, at most
+// child with fixed DOFs is evaluated with its free mask from K0.  A thread
+// owns one (node, neighbour) pair and accumulates its nine entries in FP64
+// (coarse elements, then children, in a fixed order), so the element loops
+// and the scale and mask loads are shared by nine products.  Pairs are
+// numbered neighbour-major (p = nb * nodes + I): the 64 lanes of a wave hold
+// consecutive nodes of one neighbour block, so the element loops, the corner
+// pair (A, B) and with them every read of M and K0 (staged in LDS) are the
+// same for the whole wave -- one broadcast LDS read per value, no divergence
+// except where a boundary clips the loops -- and the stores of an entry are
+// contiguous.  The masked children (boundary elements only) then take the
+// same uniform loops.
 __global__ void __launch_bounds__(256) k_mg_level1(GridDev g, MgLevel C, int rx, int ry, int rz,
                                                    const double* __restrict__ s_e, const double* __restrict__ K0,
                                                    const double* __restrict__ Mg, const unsigned* __restrict__ emask) {
     __shared__ double M[8 * 576];
-    __shared__ double sS[MG_L1_ELEMS];
-    __shared__ unsigned sMask[MG_L1_ELEMS];
+    __shared__ double sK0[576];
     for (int t = threadIdx.x; t < 8 * 576; t += blockDim.x) M[t] = Mg[t];
-    const int runs = (C.NJ + MG_GT - 1) / MG_GT;              // runs per j-row, balanced lengths
-    const int rlen = (C.NJ + runs - 1) / runs;
-    const long long nblk = (long long)runs * C.NI * C.NK;
-    for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
-        const int r = int(b % runs);
-        const long long q = b / runs;
-        const int Ii = int(q % C.NI), Ik = int(q / C.NI);
-        const int Ij0 = r * rlen, nt = min(rlen, C.NJ - Ij0);
-        if (nt <= 0) continue;                                 // (uniform in the block)
-        // fine elements around the run: j in [(Ij0-1) ry, (Ij0+nt) ry), i in
-        // [(Ii-1) rx, (Ii+1) rx), k in [(Ik-1) rz, (Ik+1) rz), clipped to the grid
-        const int ej0 = max(0, (Ij0 - 1) * ry), ej1 = min(g.ny, (Ij0 + nt) * ry);
-        const int ei0 = max(0, (Ii - 1) * rx), ei1 = min(g.nx, (Ii + 1) * rx);
-        const int ek0 = max(0, (Ik - 1) * rz), ek1 = min(g.nz, (Ik + 1) * rz);
-        const int wj = ej1 - ej0, wi = ei1 - ei0, wk = ek1 - ek0;
-        __syncthreads();                                       // the previous run is done with sS
-        for (int t = threadIdx.x; t < wj * wi * wk; t += blockDim.x) {
-            const int lj = t % wj, li = (t / wj) % wi, lk = t / (wj * wi);
-            const long long fe = ((long long)(ek0 + lk) * g.nx + (ei0 + li)) * g.ny + (ej0 + lj);
-            sS[t] = s_e[fe];
-            sMask[t] = emask[fe];
-        }
-        __syncthreads();
-        const long long I0 = ((long long)Ik * C.NI + Ii) * C.NJ + Ij0;
-        for (int k = threadIdx.x; k < 27 * nt; k += blockDim.x) {
-            const int nb = k / nt, in = k - nb * nt;
-            const int Ij = Ij0 + in;
-            const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
-            double acc[9];
+    for (int t = threadIdx.x; t < 576; t += blockDim.x) sK0[t] = K0[t];
+    __syncthreads();
+    // 27 x nodes < 2^31 (the plan keeps 243 x nodes of a stencil level below it)
+    const unsigned nn = unsigned(C.nnodes), total = 27u * nn;
+    for (unsigned pr = blockIdx.x * blockDim.x + threadIdx.x; pr < total; pr += gridDim.x * blockDim.x) {
+        const int nb = int(pr / nn);
+        const long long In = pr - unsigned(nb) * nn;
+        int Ij, Ii, Ik;
+        mg_decode(C, In, Ij, Ii, Ik);
+        const int Jj = Ij + nb % 3 - 1, Ji = Ii + (nb / 3) % 3 - 1, Jk = Ik + nb / 9 - 1;
+        double acc[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) acc[e] = 0.0;
-            if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
-                const int Ej0 = max(max(Ij, Jj) - 1, 0), Ej1 = min(min(Ij, Jj), C.ny - 1);
-                const int Ei0 = max(max(Ii, Ji) - 1, 0), Ei1 = min(min(Ii, Ji), C.nx - 1);
-                const int Ek0 = max(max(Ik, Jk) - 1, 0), Ek1 = min(min(Ik, Jk), C.nz - 1);
-                for (int Ek = Ek0; Ek <= Ek1; ++Ek)
-                    for (int Ei = Ei0; Ei <= Ei1; ++Ei)
-                        for (int Ej = Ej0; Ej <= Ej1; ++Ej) {
-                            const int A = corner_of(Ii - Ei, Ij - Ej, Ik - Ek);
-                            const int B = corner_of(Ji - Ei, Jj - Ej, Jk - Ek);
+        for (int e = 0; e < 9; ++e) acc[e] = 0.0;
+        if (Jj >= 0 && Jj < C.NJ && Ji >= 0 && Ji < C.NI && Jk >= 0 && Jk < C.NK) {
+            const int Ej0 = max(max(Ij, Jj) - 1, 0), Ej1 = min(min(Ij, Jj), C.ny - 1);
+            const int Ei0 = max(max(Ii, Ji) - 1, 0), Ei1 = min(min(Ii, Ji), C.nx - 1);
+            const int Ek0 = max(max(Ik, Jk) - 1, 0), Ek1 = min(min(Ik, Jk), C.nz - 1);
+            for (int Ek = Ek0; Ek <= Ek1; ++Ek)
+                for (int Ei = Ei0; Ei <= Ei1; ++Ei)
+                    for (int Ej = Ej0; Ej <= Ej1; ++Ej) {
+                        const int A = corner_of(Ii - Ei, Ij - Ej, Ik - Ek);
+                        const int B = corner_of(Ji - Ei, Jj - Ej, Jk - Ek);
+                        // the children's scales and masks, loaded together
+                        double sv[8];
+                        unsigned mk[8];
+                        bool partial = false;
 #pragma unroll
+                        for (int ch = 0; ch < 8; ++ch) {
+                            const int cx = ch & 1, cy = (ch >> 1) & 1, cz = ch >> 2;
+                            const bool on = cx < rx && cy < ry && cz < rz;
+                            const long long fe = on ? ((long long)(Ek * rz + cz) * g.nx + (Ei * rx + cx)) * g.ny +
+                                                          (Ej * ry + cy)
+                                                    : 0;
+                            sv[ch] = on ? s_e[fe] : 0.0;
+                            mk[ch] = on ? emask[fe] : 0u;
+                            partial = partial || (mk[ch] != 0xFFFFFFu && mk[ch] != 0u);
+                        }
+                        if (!partial) {
+                            // every child fully free (or absent): the fixed blocks
+#pragma unroll
+                            for (int ch = 0; ch < 8; ++ch) {
+                                if (mk[ch] != 0xFFFFFFu) continue;
+                                const double* m = M + ch * 576 + (3 * A) * 24 + 3 * B;
+#pragma unroll
+                                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                                    for (int d = 0; d < 3; ++d) acc[3 * c + d] += sv[ch] * m[c * 24 + d];
+                            }
+                        } else {
+                            // a child with fixed DOFs (boundary elements): its block from K0
+                            // under its free mask, the interpolation weights of its corners
+                            // formed once; the children in the same order as above
                             for (int ch = 0; ch < 8; ++ch) {
                                 const int cx = ch & 1, cy = (ch >> 1) & 1, cz = ch >> 2;
                                 if (cx >= rx || cy >= ry || cz >= rz) continue;
-                                const int le = ((Ek * rz + cz - ek0) * wi + (Ei * rx + cx - ei0)) * wj +
-                                               (Ej * ry + cy - ej0);
-                                const double sv = sS[le];
-                                const unsigned mask = sMask[le];
+                                const long long fe = ((long long)(Ek * rz + cz) * g.nx + (Ei * rx + cx)) * g.ny +
+                                                     (Ej * ry + cy);
+                                const double svc = s_e[fe];
+                                const unsigned mask = emask[fe];
                                 if (mask == 0xFFFFFFu) {
                                     const double* m = M + ch * 576 + (3 * A) * 24 + 3 * B;
 #pragma unroll
                                     for (int c = 0; c < 3; ++c)
 #pragma unroll
-                                        for (int d = 0; d < 3; ++d) acc[3 * c + d] += sv * m[c * 24 + d];
+                                        for (int d = 0; d < 3; ++d) acc[3 * c + d] += svc * m[c * 24 + d];
                                 } else if (mask) {
+                                    // B's corner weights as exponents (w = 2^-k; k >= 16: 0),
+                                    // five bits each, so the loops below stay rolled
+                                    unsigned long long pbc = 0;
+#pragma unroll
+                                    for (int bb = 0; bb < 8; ++bb) {
+                                        const unsigned k = mg_wexp(cx + OX(bb), OX(B), rx) +
+                                                           mg_wexp(cy + OY(bb), OY(B), ry) +
+                                                           mg_wexp(cz + OZ(bb), OZ(B), rz);
+                                        pbc |= (unsigned long long)min(k, 31u) << (5 * bb);
+                                    }
                                     for (int a = 0; a < 8; ++a) {
                                         const double pa = mg_w(cx + OX(a), OX(A), rx) * mg_w(cy + OY(a), OY(A), ry) *
                                                           mg_w(cz + OZ(a), OZ(A), rz);
                                         if (pa == 0.0) continue;
                                         for (int bb = 0; bb < 8; ++bb) {
-                                            const double pb = mg_w(cx + OX(bb), OX(B), rx) *
-                                                              mg_w(cy + OY(bb), OY(B), ry) *
-                                                              mg_w(cz + OZ(bb), OZ(B), rz);
-                                            if (pb == 0.0) continue;
-                                            const double w = sv * pa * pb;
+                                            const unsigned k = unsigned(pbc >> (5 * bb)) & 31u;
+                                            if (k >= 16u) continue;
+                                            const double w = svc * pa * ldexp(1.0, -int(k));
 #pragma unroll
                                             for (int c = 0; c < 3; ++c) {
                                                 if (!((mask >> (3 * a + c)) & 1u)) continue;
 #pragma unroll
                                                 for (int d = 0; d < 3; ++d)
                                                     if ((mask >> (3 * bb + d)) & 1u)
-                                                        acc[3 * c + d] += w * K0[(3 * a + c) * 24 + 3 * bb + d];
+                                                        acc[3 * c + d] += w * sK0[(3 * a + c) * 24 + 3 * bb + d];
                                             }
                                         }
                                     }
                                 }
                             }
                         }
-            }
-#pragma unroll
-            for (int e = 0; e < 9; ++e) C.S[(long long)(nb * 9 + e) * C.nnodes + I0 + in] = float(acc[e]);
+                    }
         }
+#pragma unroll
+        for (int e = 0; e < 9; ++e) C.S[(long long)(nb * 9 + e) * C.nnodes + In] = float(acc[e]);
     }
 }
 
 // Stencil of level l+1 from the stencil of level l (A_{l+1} = P^T A_l P; A_l
-// is zero at its masked DOFs, so P needs no mask here): 32 lanes per stencil
+// is zero at its masked DOFs, so P needs no mask here): 1-32 lanes per stencil
 // entry (coarse node I, neighbour block nb, entry q = 3 c + d of the block).
 __global__ void __launch_bounds__(256) k_mg_rap(MgLevel F, MgLevel C, int rx, int ry, int rz, int lg) {
     // 2^lg lanes per entry (G = 1..32, so that a launch has about a million
