@@ -9,7 +9,9 @@ benchmark and the target are quoted on), each against the oracle on the CPU:
   256 uneven cantilevers x 4..16 load cases (seed 4): every system's true
       residual, and a sample against spsolve;
   256x64x64 SIMP, rho ~ U(0.2,1) (seed 5): the FP32-refined forward solve's
-      true residual, and the gradient against central differences on 32 elements.
+      true residual, and the gradient against central differences on 32 elements;
+  the multigrid V-cycle of the 128x32x32 system (the levels large enough for
+      the set-up's XCD-chunked Galerkin product) vs the oracle's V-cycle.
 
 True residuals are recomputed on the CPU from the element matrices
 (oracle.element_matvec, pinned to the assembled CSR product).  The PCG stops
@@ -195,3 +197,32 @@ def test_config_large_adjoint_256x64x64(h8, gpu):
     rel = np.abs(fd - g[els]) / np.abs(g[els])
     say("simp 256x64x64 fd max rel", rel.max(), "median", np.median(rel))
     assert rel.max() < 1e-6
+
+
+def test_config_vcycle_128x32x32_matches_oracle(h8, gpu):
+    """One V-cycle of the bench system (128x32x32, E = 10**U(9,12), seed 2; six
+    levels, the level-2 stencil from the XCD-chunked Galerkin product) equals
+    the oracle's V-cycle (scipy triple products in FP64, coarse operators
+    stored in FP32 as the library stores them, level-0 smoothing degree 4)."""
+    import torch
+    from h8pcg.workloads import cantilever
+    nx, ny, nz = 128, 32, 32
+    E = 10 ** np.random.default_rng(2).uniform(9, 12, nx * ny * nz)
+    fs = cantilever(nx, ny, nz, E_elem=E, device="cuda:0")
+    op = fs.operator
+    op.set_mg_degree(4)
+    free = fs._free_mask().astype(bool)
+    op.set_free_mask(free.astype(np.uint8))
+    r = np.random.default_rng(21).standard_normal(op.n_dof) * free
+    z = op.mg_vcycle(torch.as_tensor(r, dtype=torch.float32, device=op.device)).cpu().numpy().astype(float)
+    nodes, elems = o.box_mesh(float(nx), float(ny), float(nz), nx, ny, nz)
+    D = o.d_solid3d(1.0, 0.3)
+    ke = o.hex8_ke(nodes[elems[0]], D)
+    K = o.assemble_csr(nodes, elems, D, scale=E)
+    Kabs = o.assemble_csr(nodes, elems, None, scale=E, ke=np.broadcast_to(np.abs(ke), (len(elems), 24, 24)))
+    H = o.mg_hierarchy(K.tocsr(), free, (nx, ny, nz), K_abs=Kabs, store32=True, degree0=4)
+    zr = o.mg_vcycle(H, r.astype(np.float32).astype(float))
+    err = np.linalg.norm(z - zr) / np.linalg.norm(zr)
+    say("vcycle 128x32x32 levels", [lv["A"].shape[0] for lv in H], "rel err", err)
+    assert np.all(z[~free] == 0.0)
+    assert err < 1e-5
