@@ -1637,7 +1637,7 @@ static bool mg_plan(const GridDev& d, MgPlan& P) {
         L.rx = rx; L.ry = ry; L.rz = rz;
         nx /= rx; ny /= ry; nz /= rz;
     }
-    P.deg0 = d.mg_degree ? d.mg_degree : MG_DEG0_DEFAULT;
+    P.deg0 = d.mg_degree ? d.mg_degree : mg_deg0_default(d.ndof);
     // the MG kernels index stencil levels in 32 bits
     if (P.nlev >= 2 && 243LL * P.L[1].nnodes >= (1LL << 31)) return false;
     return P.nlev >= 2 && P.L[P.nlev - 1].ndof <= MG_DENSE_MAX;

@@ -378,8 +378,8 @@ class GridOperator:
 
     def set_mg_degree(self, degree):
         """Chebyshev steps per smoothing pass on the finest level of the
-        multigrid preconditioner: 0 (library choice) or 2..6.  Coarser levels
-        always take 2 steps."""
+        multigrid preconditioner: 0 (library choice: 4 up to 2^20 DOFs, 3
+        above) or 2..6.  Coarser levels always take 2 steps."""
         if int(degree) != 0 and not 2 <= int(degree) <= 6:
             raise ValueError(f"mg_degree must be 0 (library choice) or 2..6, got {degree}")
         self.grid32.mg_degree = int(degree)

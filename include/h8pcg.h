@@ -72,7 +72,8 @@ typedef struct h8_grid {
                                and MFMA kernels compute in their storage precision. */
     int32_t mg_degree;      /* Chebyshev steps per smoothing pass on the finest level of the
                                multigrid preconditioner (h8_refine_solve_mg, h8_mg_vcycle_f32):
-                               0 = library choice, else 2..6; coarser levels take 2 */
+                               0 = library choice (4 up to 2^20 DOFs, 3 above), else 2..6;
+                               coarser levels take 2 */
 } h8_grid;
 
 typedef struct h8_pcg_stats {
