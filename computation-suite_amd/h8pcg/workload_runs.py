@@ -124,15 +124,18 @@ def refined(a):
     for _ in range(max(1, a.reps)):
         _sync()
         t = time.perf_counter()
-        fs.solve_static_device(tol=a.tol, precision="mixed", F=F, delta=a.delta)
+        if a.precision == "mixed":
+            fs.solve_static_device(tol=a.tol, precision="mixed", F=F, delta=a.delta)
+        else:
+            fs.solve_static_device(tol=a.tol, precision="fp64", F=F)
         _sync()
         times.append(1e3 * (time.perf_counter() - t))
     st = fs.last_solve_stats
     out = {"workload": "refined", "dims": [nx, ny, nz], "seed": a.seed, "tol": a.tol, "mg_degree": a.mg_degree,
-           "delta": a.delta,
+           "delta": a.delta, "precision": a.precision,
            "n_free": fs.n_dof - len(fs.fixed_dofs), "solve_ms": times,
-           "stats": {k: st.get(k) for k in ("inner_iterations", "outer", "true_rel_residual", "elapsed_ms",
-                                            "launches", "preconditioner")}}
+           "stats": {k: st.get(k) for k in ("iterations", "inner_iterations", "outer", "true_rel_residual",
+                                            "elapsed_ms", "launches", "preconditioner")}}
     print(json.dumps(out), flush=True)
 
 
@@ -258,6 +261,8 @@ def main(argv=None):
     rf.add_argument("--reps", type=int, default=12)
     rf.add_argument("--mg-degree", type=int, default=0, help="level-0 Chebyshev steps of the V-cycle (0 = library)")
     rf.add_argument("--delta", type=float, default=None, help="residual-replacement threshold (None: the library's)")
+    rf.add_argument("--precision", default="mixed", choices=("mixed", "fp64"),
+                    help="mixed: the multigrid-preconditioned FP32-refined solve; fp64: Jacobi-PCG in FP64")
     b = sub.add_parser("batched")
     b.add_argument("--seed", type=int, default=4)
     b.add_argument("--n", type=int, default=256)
