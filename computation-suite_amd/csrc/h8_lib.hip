@@ -1770,6 +1770,11 @@ static int mg_setup(const GridDev& d, MgPlan& P, const double* s_e64, const doub
         else
             hipLaunchKernelGGL((k_mg_gauss_jordan<true, 16>), dim3(1), dim3(1024), lds, s, P.A, int(nc), P.Ainv,
                                P.flag);
+    } else if (nc <= MG_SWEEP_MAX) {
+        // the symmetric sweep on the upper triangle in registers (a 3 x 3 x 3 coarsest grid)
+        static_assert(MG_SWEEP_MAX * (MG_SWEEP_MAX + 1) / 2 <= 1024 * 19, "sweep slots");
+        hipLaunchKernelGGL(k_mg_sweep_sym<19>, dim3(1), dim3(1024), unsigned(3 * nc * sizeof(double)), s,
+                           (const double*)P.A, int(nc), P.Ainv, P.flag);
     } else {
         hipLaunchKernelGGL(k_mg_gauss_jordan<false>, dim3(1), dim3(1024), unsigned(2 * nc * sizeof(double)), s, P.A,
                            int(nc), P.Ainv, P.flag);

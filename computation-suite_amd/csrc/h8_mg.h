@@ -624,6 +624,83 @@ __global__ void __launch_bounds__(1024) k_mg_gauss_jordan(double* __restrict__ A
     }
 }
 
+// Inverse of the SPD n x n matrix A (128 < n <= MG_SWEEP_MAX: a 3 x 3 x 3
+// coarsest grid) by the symmetric sweep operator (Goodnight, "A tutorial on
+// the SWEEP operator", The American Statistician 33, 1979): sweeping pivot k
+// of a symmetric matrix, p = a_kk,
+//     a_ij -= a_ik a_kj / p (i, j != k),  a_ik = a_kj = a_.k / p,  a_kk = -1 / p,
+// keeps it symmetric, and sweeping every pivot leaves -A^-1.  Only the upper
+// triangle is held: thread t owns entries t + 1024 e, e < PER, of the packed
+// upper triangle in registers (PER = ceil(n (n+1) / 2048), half the slots of
+// the square form, so it fits where the square form spills); step k passes
+// row k through an LDS buffer of alternating parity, one barrier per step.
+// The pivots are the Schur complements' diagonals, positive for an SPD matrix;
+// *flag = 1 if one is not above 1e-10 of its original diagonal.  Dynamic LDS
+// 24 n bytes (row buffers of both parities and the diagonal).
+constexpr int MG_SWEEP_MAX = 192;
+template <int PER>
+__global__ void __launch_bounds__(1024) k_mg_sweep_sym(const double* __restrict__ Ag, int n,
+                                                       float* __restrict__ Ainv, int* flag) {
+    extern __shared__ double smem_sw[];
+    double* buf = smem_sw;                   // [2][n]: row k of step k, by parity
+    double* diag0 = smem_sw + 2 * n;
+    const int nt = n * (n + 1) / 2;
+    double a[PER];
+    int rc[PER];                             // (row << 16) | column, row <= column; -1 past the triangle
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int t = int(threadIdx.x) + 1024 * e;
+        int i = -1, j = -1;
+        if (t < nt) {
+            // row i holds the n - i entries (i, i .. n-1); rows before it hold
+            // i n - i (i - 1) / 2 entries
+            i = int((2.0 * n + 1.0 - sqrt((2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * t)) * 0.5);
+            while (i > 0 && i * n - i * (i - 1) / 2 > t) --i;
+            while ((i + 1) * n - (i + 1) * i / 2 <= t) ++i;
+            j = i + (t - (i * n - i * (i - 1) / 2));
+        }
+        rc[e] = t < nt ? (i << 16) | j : -1;
+        a[e] = t < nt ? Ag[i * n + j] : 0.0;
+        if (t < nt && i == j) diag0[i] = a[e];
+    }
+    int bad = 0;
+    for (int k = 0; k < n; ++k) {
+        double* r = buf + (k & 1) * n;       // row k: r[m] = a_km
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            if (rc[e] < 0) continue;
+            const int i = rc[e] >> 16, j = rc[e] & 0xFFFF;
+            if (i == k) r[j] = a[e];
+            else if (j == k) r[i] = a[e];
+        }
+        __syncthreads();
+        double p = r[k];
+        if (!(p > 1e-10 * diag0[k])) {
+            bad = 1;
+            p = 1.0;
+        }
+        const double ip = 1.0 / p;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            if (rc[e] < 0) continue;
+            const int i = rc[e] >> 16, j = rc[e] & 0xFFFF;
+            if (i == k && j == k) a[e] = -ip;
+            else if (i == k || j == k) a[e] *= ip;
+            else a[e] -= r[i] * (r[j] * ip);
+        }
+    }
+    // -A^-1 is held: both triangles of Ainv
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        if (rc[e] < 0) continue;
+        const int i = rc[e] >> 16, j = rc[e] & 0xFFFF;
+        const float v = float(-a[e]);
+        Ainv[i * n + j] = v;
+        Ainv[j * n + i] = v;
+    }
+    if (threadIdx.x == 0) *flag = bad;
+}
+
 // x = Ainv b on the coarsest level: one wave per row, lanes over the columns
 // (Ainv is symmetric, so row r is read as column r), a fixed-order wave sum.
 __global__ void __launch_bounds__(256) k_mg_dense_solve(const float* __restrict__ Ainv, int n,

@@ -23,13 +23,14 @@ def _cantilever(h8, nx, ny, nz, E):
 
 
 @pytest.mark.parametrize("dims,degree", [((32, 8, 8), 2), ((32, 8, 8), 3), ((32, 8, 8), 4), ((32, 8, 8), 6),
-                                          ((40, 3, 4), 2), ((40, 3, 4), 5)])
+                                          ((40, 3, 4), 2), ((40, 3, 4), 5), ((24, 6, 6), 4)])
 def test_vcycle_matches_oracle(h8, gpu, dims, degree):
     """z = V(r) on the device equals the oracle's V-cycle (FP64 arithmetic,
     coarse operators P^T A P from scipy, stored in FP32 as the library stores
     them) to FP32 accuracy, for level-0 smoothing degrees 2..6 (odd degrees
     take the other buffer order); (40, 3, 4) keeps its odd y axis on the
-    coarse level.  (Without the FP32 storage the coarsest inverse alone
+    coarse level; (24, 6, 6) ends on a 3 x 3 x 3 coarsest grid of 192 DOFs,
+    inverted by the symmetric sweep.  (Without the FP32 storage the coarsest inverse alone
     differs by up to its condition number times the FP32 rounding.)"""
     import torch
     nx, ny, nz = dims
